@@ -1,0 +1,248 @@
+/*
+ * cvo_c.h -- C ABI of the MI355X (gfx950) CVO / RKHS-BA registration library.
+ *
+ * Every entry point is extern "C", takes plain pointers and sizes, returns a
+ * cvo_status and never exits the process.  Each declaration names the
+ * reference interface it replaces (paths relative to the UnifiedCvo tree).
+ *
+ * Conventions shared with the reference:
+ *   - A pose argument "T" of cvo_align / cvo_inner_product / cvo_function_angle /
+ *     cvo_compute_association is the reference's init_guess_transform: the
+ *     target cloud is mapped into the source frame by T^-1
+ *     (src/cvo/CvoGPU.cu:98-130 update_tf, :1306-1307).
+ *   - cvo_align's T_out is the reference's `transform` output, i.e. the map that
+ *     takes target coordinates into the source frame ([R^T, -R^T t]).
+ *   - 4x4 matrices are row-major float[16]; BA poses are row-major 3x4 double[12]
+ *     exactly like CvoFrame::pose_vec (include/UnifiedCvo/cvo/CvoFrame.hpp:26).
+ */
+#ifndef CVO_C_H
+#define CVO_C_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  CVO_OK = 0,
+  CVO_ERR_INVALID_ARG = 1,   /* null pointer, negative size, bad dimension    */
+  CVO_ERR_EMPTY = 2,         /* an input cloud has zero points                 */
+  CVO_ERR_IO = 3,            /* file could not be opened                       */
+  CVO_ERR_PARSE = 4,         /* YAML value could not be converted              */
+  CVO_ERR_HIP = 5,           /* a HIP runtime call failed                      */
+  CVO_ERR_NO_DEVICE = 6,     /* no HIP device visible                          */
+  CVO_ERR_OOM = 7,           /* device allocation failed                       */
+  CVO_ERR_UNSUPPORTED = 8,   /* option not implemented (e.g. is_using_kdtree)  */
+  CVO_ERR_LIMIT = 9          /* size beyond a documented limit                 */
+} cvo_status;
+
+/*
+ * cvo_params -- field-for-field the reference's CvoParams
+ * (include/UnifiedCvo/cvo/CvoParams.hpp:12-79), same names and types.  Every
+ * field has a default (cvo_params_default); the reference leaves max_step and
+ * step uninitialised (:80-136), here they default to 0.01 and 0.
+ */
+typedef struct {
+  float ell_init_first_frame;
+  float ell_init;
+  float ell_min;
+  int32_t min_ell_iter_limit;
+  float ell_max;
+  double dl;
+  double dl_step;
+  float sigma;
+  float sp_thres;
+  float c;
+  float d;
+  float c_ell;
+  float c_sigma;
+  float s_ell;
+  float s_sigma;
+  int32_t MAX_ITER;
+  float eps;
+  float eps_2;
+  float min_step;
+  float max_step;
+  float step;
+  int32_t nearest_neighbors_max;
+  float ell_decay_rate;
+  float ell_decay_rate_first_frame;
+  int32_t ell_decay_start;
+  int32_t ell_decay_start_first_frame;
+  int32_t indicator_window_size;
+  float indicator_stable_threshold;
+  int32_t is_pcl_visualization_on;
+  int32_t is_using_least_square;
+  int32_t is_ell_adaptive;
+  int32_t is_full_ip_matrix;
+  int32_t is_using_geometry;
+  int32_t is_using_intensity;
+  int32_t is_using_semantics;
+  int32_t is_using_range_ell;
+  int32_t is_using_kdtree;
+  int32_t is_exporting_association;
+  int32_t is_using_geometric_type;
+  int32_t is_global_angle_registration;
+  int32_t multiframe_using_cpu;
+  int32_t multiframe_max_iters;
+  float multiframe_ell_init;
+  float multiframe_ell_min;
+  float multiframe_ell_max;
+  int32_t multiframe_iter_per_ell;
+  float multiframe_ell_decay_rate;
+  int32_t multiframe_iterations_per_ell;
+  int32_t multiframe_iterations_per_solve;
+  int32_t multiframe_expected_points;
+  float multiframe_downsample_voxel_size;
+  int32_t multiframe_num_neighbors;
+  int32_t multiframe_least_squares_num_threads;
+  int32_t multiframe_min_nonzeros;
+  int32_t multiframe_is_optimizing_ell;
+  int32_t multiframe_is_sorting_inner_product;
+  int32_t multiframe_is_free_memory_each_iter;
+} cvo_params;
+
+/* A borrowed host view of one point cloud (CvoPointCloud, include/UnifiedCvo/
+ * utils/CvoPointCloud.hpp:38-221, in structure-of-arrays form).
+ * xyz: n*3; feat: n*feat_dim or NULL; sem: n*num_classes or NULL;
+ * geo_type: n*2 or NULL.  Read only during the call. */
+typedef struct {
+  int32_t n;
+  int32_t feat_dim;
+  int32_t num_classes;
+  const float* xyz;
+  const float* feat;
+  const float* sem;
+  const float* geo_type;
+} cvo_cloud_view;
+
+/* Association in CSR form (replaces cvo::Association, include/UnifiedCvo/cvo/
+ * Association.hpp:7-14).  Rows are source points in input order; within a row
+ * the target ids ascend.  Arrays are owned by the library; free with
+ * cvo_assoc_free. */
+typedef struct {
+  int32_t rows;
+  int32_t cols;            /* number of target points                         */
+  int64_t nnz;
+  int64_t* row_ptr;        /* rows+1                                          */
+  int32_t* col;            /* nnz target ids                                  */
+  float* val;              /* nnz kernel values a_ij                          */
+} cvo_assoc;
+
+typedef struct cvo_ctx cvo_ctx;       /* params + HIP stream + workspace     */
+typedef struct cvo_dcloud cvo_dcloud; /* a cloud resident in device memory   */
+
+/* Per-iteration record of an align call (optional trace). */
+typedef struct {
+  float ell;
+  float step;
+  float ip;               /* indicator nnz / sqrt(N*M)                       */
+  int32_t nnz;
+  int32_t max_row;
+  int32_t num_neighbors;  /* row cap K used in this iteration                */
+  float dist;             /* ||log(dT)|| of the step                         */
+  float flow_norm;        /* ||[omega; v]|| before normalisation             */
+} cvo_iter_record;
+
+/* Device time and algorithmic traffic of the library's kernels, accumulated
+ * while cvo_set_profiling(ctx, 1) is on.  Index: CVO_KSTAT_*. */
+enum {
+  CVO_KSTAT_ASSOC = 0,     /* association + flow                            */
+  CVO_KSTAT_OVERFLOW = 1,  /* capped-row exact path                         */
+  CVO_KSTAT_STEP = 2,      /* step terms + controller                       */
+  CVO_KSTAT_GRID = 3,      /* grid build                                    */
+  CVO_KSTAT_BA_PAIR = 4,   /* BA pair weights / normal equations            */
+  CVO_KSTAT_BA_SOLVE = 5,  /* BA FP64 solve                                 */
+  CVO_KSTAT_MLP = 6,       /* learned proposer GEMMs                        */
+  CVO_KSTAT_COUNT = 8
+};
+typedef struct {
+  double ms[CVO_KSTAT_COUNT];        /* summed device milliseconds          */
+  int64_t launches[CVO_KSTAT_COUNT];
+  double bytes[CVO_KSTAT_COUNT];     /* summed algorithmic bytes            */
+  double pairs_stored;               /* sum over iterations of nnz          */
+  double candidates;                 /* grid candidates visited (overhead)  */
+} cvo_kernel_stats;
+
+/* ---- parameters ---------------------------------------------------------- */
+
+/* Defaults of CvoParams::CvoParams() (CvoParams.hpp:80-136). */
+cvo_status cvo_params_default(cvo_params* out);
+/* read_CvoParams_yaml (CvoParams.hpp:203-323): a key present in the file
+ * overrides the value already in *p; absent keys keep it; unknown keys are
+ * ignored.  Accepts the %YAML:1.0 header and '#' comments. */
+cvo_status cvo_params_read_yaml(const char* path, cvo_params* p);
+
+/* ---- context ------------------------------------------------------------- */
+
+/* CvoGPU::CvoGPU(const std::string& yaml) (src/cvo/CvoGPU.cu:65-74). */
+cvo_status cvo_create(const char* yaml_path, cvo_ctx** out);
+cvo_status cvo_create_from_params(const cvo_params* p, cvo_ctx** out);
+/* CvoGPU::get_params / write_params (CvoGPU.hpp, CvoGPU.cu:76-80). */
+cvo_status cvo_get_params(const cvo_ctx* ctx, cvo_params* out);
+cvo_status cvo_set_params(cvo_ctx* ctx, const cvo_params* p);
+void cvo_destroy(cvo_ctx* ctx);
+const char* cvo_status_string(cvo_status s);
+/* Last error text of this context (HIP error string or argument check). */
+const char* cvo_last_error(const cvo_ctx* ctx);
+
+/* ---- device-resident clouds --------------------------------------------- */
+
+/* CvoPointCloud_to_gpu (src/cvo/CvoGPU_impl.cu:220-289): copies the view
+ * into device memory once, so repeated calls skip the upload. */
+cvo_status cvo_cloud_upload(cvo_ctx* ctx, const cvo_cloud_view* v, cvo_dcloud** out);
+void cvo_cloud_free(cvo_ctx* ctx, cvo_dcloud* c);
+int32_t cvo_cloud_size(const cvo_dcloud* c);
+
+/* ---- two-view registration ---------------------------------------------- */
+
+/* CvoGPU::align(source, target, init_guess_T, transform, association,
+ * registration_seconds, num_iters) (src/cvo/CvoGPU.cu:1516-1563,
+ * align_impl :1239-1477).  ret_code: 0, or -1 when the gradient vanished.
+ * Empty input: returns CVO_ERR_EMPTY with ret_code 0 and T_out = T_init^-1
+ * semantics of the reference's early return (it leaves transform untouched;
+ * here T_out is set to the identity).  seconds = device time of the
+ * iteration loop, like the reference's cudaEvent timer. */
+cvo_status cvo_align(cvo_ctx* ctx, const cvo_cloud_view* src, const cvo_cloud_view* tgt,
+                     const float T_init[16], float T_out[16], cvo_assoc* assoc_out,
+                     double* seconds, int32_t* iters, int32_t* ret_code);
+cvo_status cvo_align_d(cvo_ctx* ctx, const cvo_dcloud* src, const cvo_dcloud* tgt,
+                       const float T_init[16], float T_out[16], cvo_assoc* assoc_out,
+                       double* seconds, int32_t* iters, int32_t* ret_code);
+
+/* Optional per-iteration trace of the next align call(s): up to cap records. */
+cvo_status cvo_set_trace(cvo_ctx* ctx, int32_t cap);
+cvo_status cvo_get_trace(cvo_ctx* ctx, cvo_iter_record* out, int32_t cap, int32_t* n_out);
+
+/* ---- inner product / cos angle / association ---------------------------- */
+
+/* CvoGPU::inner_product_gpu (src/cvo/CvoGPU.cu:1673-1748): sum of all stored
+ * a_ij at pose T and lengthscale ell, row cap nearest_neighbors_max. */
+cvo_status cvo_inner_product(cvo_ctx* ctx, const cvo_cloud_view* src, const cvo_cloud_view* tgt,
+                             const float T[16], float ell, double* ip);
+cvo_status cvo_inner_product_d(cvo_ctx* ctx, const cvo_dcloud* src, const cvo_dcloud* tgt,
+                               const float T[16], float ell, double* ip);
+/* CvoGPU::function_angle (src/cvo/CvoGPU.cu:1768-1806): approx != 0 gives
+ * ip / (sqrt(N) sqrt(M)), otherwise ip / sqrt(ip_xx ip_yy). */
+cvo_status cvo_function_angle(cvo_ctx* ctx, const cvo_cloud_view* src, const cvo_cloud_view* tgt,
+                              const float T[16], float ell, int32_t approx, double* cos_out);
+/* CvoGPU::compute_association_gpu (src/cvo/CvoGPU.cu:1836-1871). */
+cvo_status cvo_compute_association(cvo_ctx* ctx, const cvo_cloud_view* src, const cvo_cloud_view* tgt,
+                                   const float T[16], float ell, cvo_assoc* out);
+void cvo_assoc_free(cvo_assoc* a);
+
+/* ---- profiling ----------------------------------------------------------- */
+
+cvo_status cvo_set_profiling(cvo_ctx* ctx, int32_t on);
+cvo_status cvo_get_kernel_stats(cvo_ctx* ctx, cvo_kernel_stats* out);
+cvo_status cvo_reset_kernel_stats(cvo_ctx* ctx);
+
+/* Library version string. */
+const char* cvo_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CVO_C_H */

@@ -1,0 +1,1030 @@
+// CPU restatement (oracle) of the reference CVO two-view path and RKHS-BA window.
+//
+// TEST INFRASTRUCTURE ONLY: the checker of the HIP library.  Loaded by tests/,
+// __graft_entry__.smoke() and bench.py's cpu_baseline leg; never by the product.
+//
+// What pins each function (independent of this file's own formulas):
+//   orc_assoc            brute force (brute=1) vs grid (brute=0) set equality;
+//                        tests/golden vectors of a few hand-sized pairs computed
+//                        with numpy from the kernel definition; gate/threshold
+//                        closed forms.
+//   orc_flow             numpy FP64 closed form on tiny inputs (golden).
+//   orc_step_coeffs      numpy polynomial composition of the 4th-order path
+//                        expansion (golden) -- see tests/golden/make_golden.py.
+//   orc_cubic_roots      numpy.roots (companion-matrix eigenvalues) golden.
+//   orc_exp_*, orc_log_* scipy.linalg.expm / logm golden.
+//   orc_indicator_sequence  scripted sequences with hand-derived decisions.
+//   orc_align            convergence to the generating pose of seeded scenes.
+//   BA functions         finite differences, scipy expm, numpy dense assembly.
+//
+// Build: oracle/Makefile (g++ -O3 -fopenmp -ffp-contract=off).
+
+#include "cvo_oracle.h"
+
+#include <omp.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <complex>
+#include <cstdint>
+#include <cstring>
+#include <deque>
+#include <limits>
+#include <vector>
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Canonical float helpers shared (by definition, not by code) with the HIP path.
+// ---------------------------------------------------------------------------
+
+struct Pose64 {
+  double R[9];  // row-major
+  double T[3];
+};
+
+Pose64 pose_from16(const float T[16]) {
+  Pose64 P;
+  for (int r = 0; r < 3; ++r) {
+    for (int c = 0; c < 3; ++c) P.R[r * 3 + c] = T[r * 4 + c];
+    P.T[r] = T[r * 4 + 3];
+  }
+  return P;
+}
+
+// update_tf (CvoGPU.cu:98-130): the target is mapped by [R^T, -R^T T].
+// Canonical: A = float(R^T), b = float(-(R^T T)) evaluated in double.
+void applied_transform(const Pose64& P, float A[9], float b[3]) {
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) A[i * 3 + j] = static_cast<float>(P.R[j * 3 + i]);
+  for (int i = 0; i < 3; ++i)
+    b[i] = static_cast<float>(-(P.R[0 * 3 + i] * P.T[0] + P.R[1 * 3 + i] * P.T[1] + P.R[2 * 3 + i] * P.T[2]));
+}
+
+// transform_point_R_T (CvoGPU_impl.cu:32-83) with the canonical FMA order.
+inline void xform_f(const float A[9], const float b[3], const float* y, float* o) {
+  for (int i = 0; i < 3; ++i)
+    o[i] = std::fmaf(A[i * 3 + 2], y[2], std::fmaf(A[i * 3 + 1], y[1], std::fmaf(A[i * 3 + 0], y[0], b[i])));
+}
+
+// squared_dist(p_b, p_a) (gpu_utils.cuh:72-78): diff = b - a.
+inline float sqdist3_f(const float* b, const float* a) {
+  float dx = b[0] - a[0], dy = b[1] - a[1], dz = b[2] - a[2];
+  return std::fmaf(dz, dz, std::fmaf(dy, dy, dx * dx));
+}
+
+// squared_dist<float>(a, b, dim) (gpu_utils.cuh:33-41): sequential accumulation.
+inline float sqdist_n_f(const float* a, const float* b, int n) {
+  float r = 0.0f;
+  for (int i = 0; i < n; ++i) {
+    float t = a[i] - b[i];
+    r = std::fmaf(t, t, r);
+  }
+  return r;
+}
+
+inline float dot_n_f(const float* a, const float* b, int n) {
+  float r = 0.0f;
+  for (int i = 0; i < n; ++i) r = std::fmaf(a[i], b[i], r);
+  return r;
+}
+
+struct Consts {
+  float sigma2, c2, c_sigma2, s_ell, s_sigma2;
+  float lg, lc, ls;  // logf(sp/..)
+  float d2_c_thres, d2_s_thres;
+  int geo, col, sem, gtype;
+  float sp;
+};
+
+Consts make_consts(const cvo_params* p) {
+  Consts k;
+  k.sigma2 = p->sigma * p->sigma;
+  k.c2 = p->c_ell * p->c_ell;
+  k.c_sigma2 = p->c_sigma * p->c_sigma;
+  k.s_ell = p->s_ell;
+  k.s_sigma2 = p->s_sigma * p->s_sigma;
+  k.lg = logf(p->sp_thres / k.sigma2);
+  k.lc = logf(p->sp_thres / k.c_sigma2);
+  k.ls = logf(p->sp_thres / k.s_sigma2);
+  k.geo = p->is_using_geometry;
+  k.col = p->is_using_intensity;
+  k.sem = p->is_using_semantics;
+  k.gtype = p->is_using_geometric_type;
+  k.sp = p->sp_thres;
+  // fill_in_A_mat_gpu thresholds (CvoGPU.cu:981-987)
+  k.d2_c_thres = k.col ? static_cast<float>(-2.0 * (double)k.c2 * (double)k.lc) : 1.0f;
+  k.d2_s_thres = k.sem ? static_cast<float>(-2.0 * (double)k.s_ell * (double)k.s_ell * (double)k.ls) : 1.0f;
+  return k;
+}
+
+// compute_range_ell (CvoGPU.cu:90-94), applied unconditionally at :978-979.
+inline float range_ell(float ell, const float* x) {
+  float dist = sqrtf(std::fmaf(x[2], x[2], std::fmaf(x[1], x[1], x[0] * x[0])));
+  return static_cast<float>(((double)dist / 500.0 + 1.0) * (double)ell);
+}
+
+inline float d2_geo_thres(const Consts& k, float l) {
+  return k.geo ? static_cast<float>(-2.0 * (double)l * (double)l * (double)k.lg) : 1.0f;
+}
+
+struct CloudRef {
+  int n, F, K;
+  const float *xyz, *feat, *sem, *geo;
+};
+
+CloudRef ref(const cvo_cloud_view* v) {
+  CloudRef c{v->n, v->feat_dim, v->num_classes, v->xyz, v->feat, v->sem, v->geo_type};
+  return c;
+}
+
+// Mirror evaluation of one pair (CvoGPU.cu:997-1076).  Returns false when a gate
+// rejects; a = product kernel before the sp_thres test.  Gate margins (relative
+// distance to each threshold) are reported for the near-cutoff exemption.
+struct PairEval {
+  bool ok;
+  int gate_stage;  // 0 geo-type, 1 geo, 2 color, 3 sem, 4 product
+  float a;
+  double margin;
+};
+
+inline PairEval eval_pair_mirror(const Consts& k, const CloudRef& S, const CloudRef& Tg, int i, int j,
+                                 const float* xi, const float* yp, float l, float d2_thres) {
+  PairEval r{false, 0, 0.0f, 1e30};
+  float geo_sim = 1.0f, kk = 1.0f, ck = 1.0f, sk = 1.0f;
+  if (k.gtype) {
+    const float* ga = S.geo + 2 * (size_t)i;
+    const float* gb = Tg.geo + 2 * (size_t)j;
+    float n2a = dot_n_f(ga, ga, 2), n2b = dot_n_f(gb, gb, 2), dab = dot_n_f(ga, gb, 2);
+    geo_sim = dab * dab / (n2a * n2b);
+    r.margin = std::min(r.margin, std::fabs((double)geo_sim - 0.01) / 0.01);
+    if (geo_sim < 0.01f) { r.gate_stage = 0; return r; }
+  }
+  if (k.geo) {
+    float d2 = sqdist3_f(yp, xi);
+    r.margin = std::min(r.margin, std::fabs((double)d2 - (double)d2_thres) / std::max(1e-30, (double)d2_thres));
+    if (!(d2 < d2_thres)) { r.gate_stage = 1; return r; }
+    kk = static_cast<float>((double)k.sigma2 * std::exp((double)(-d2) / (2.0 * (double)l * (double)l)));
+  }
+  if (k.col) {
+    float d2c = sqdist_n_f(S.feat + (size_t)i * S.F, Tg.feat + (size_t)j * Tg.F, S.F);
+    r.margin = std::min(r.margin, std::fabs((double)d2c - (double)k.d2_c_thres) / std::max(1e-30, (double)k.d2_c_thres));
+    if (!(d2c < k.d2_c_thres)) { r.gate_stage = 2; return r; }
+    ck = static_cast<float>((double)k.c_sigma2 * std::exp((double)(-d2c) / (2.0 * (double)k.c2)));
+  }
+  if (k.sem) {
+    float d2s = sqdist_n_f(S.sem + (size_t)i * S.K, Tg.sem + (size_t)j * Tg.K, S.K);
+    r.margin = std::min(r.margin, std::fabs((double)d2s - (double)k.d2_s_thres) / std::max(1e-30, (double)k.d2_s_thres));
+    if (!(d2s < k.d2_s_thres)) { r.gate_stage = 3; return r; }
+    sk = static_cast<float>((double)k.s_sigma2 * std::exp((double)(-d2s) / (2.0 * (double)k.s_ell * (double)k.s_ell)));
+  }
+  r.a = ck * kk * sk * geo_sim;
+  r.gate_stage = 4;
+  r.ok = true;
+  return r;
+}
+
+// FP64 evaluation of one pair (same algorithm in double).
+inline PairEval eval_pair_fp64(const Consts& k, const cvo_params* p, const CloudRef& S, const CloudRef& Tg,
+                               int i, int j, const double* xi, const double* yp, double l) {
+  PairEval r{false, 0, 0.0f, 1e30};
+  double geo_sim = 1, kk = 1, ck = 1, sk = 1;
+  if (k.gtype) {
+    const float* ga = S.geo + 2 * (size_t)i;
+    const float* gb = Tg.geo + 2 * (size_t)j;
+    double n2a = (double)ga[0] * ga[0] + (double)ga[1] * ga[1];
+    double n2b = (double)gb[0] * gb[0] + (double)gb[1] * gb[1];
+    double dab = (double)ga[0] * gb[0] + (double)ga[1] * gb[1];
+    geo_sim = dab * dab / (n2a * n2b);
+    if (geo_sim < 0.01) return r;
+  }
+  double sigma2 = (double)p->sigma * p->sigma;
+  if (k.geo) {
+    double dx = yp[0] - xi[0], dy = yp[1] - xi[1], dz = yp[2] - xi[2];
+    double d2 = dx * dx + dy * dy + dz * dz;
+    double thr = -2.0 * l * l * std::log((double)p->sp_thres / sigma2);
+    if (!(d2 < thr)) return r;
+    kk = sigma2 * std::exp(-d2 / (2.0 * l * l));
+  }
+  if (k.col) {
+    double c2 = (double)p->c_ell * p->c_ell, cs2 = (double)p->c_sigma * p->c_sigma;
+    double d2c = 0;
+    for (int f = 0; f < S.F; ++f) {
+      double t = (double)S.feat[(size_t)i * S.F + f] - Tg.feat[(size_t)j * Tg.F + f];
+      d2c += t * t;
+    }
+    if (!(d2c < -2.0 * c2 * std::log((double)p->sp_thres / cs2))) return r;
+    ck = cs2 * std::exp(-d2c / (2.0 * c2));
+  }
+  if (k.sem) {
+    double s2 = (double)p->s_ell * p->s_ell, ss2 = (double)p->s_sigma * p->s_sigma;
+    double d2s = 0;
+    for (int f = 0; f < S.K; ++f) {
+      double t = (double)S.sem[(size_t)i * S.K + f] - Tg.sem[(size_t)j * Tg.K + f];
+      d2s += t * t;
+    }
+    if (!(d2s < -2.0 * s2 * std::log((double)p->sp_thres / ss2))) return r;
+    sk = ss2 * std::exp(-d2s / (2.0 * s2));
+  }
+  double a = ck * kk * sk * geo_sim;
+  r.a = static_cast<float>(a);
+  r.margin = a;  // exact value carried in margin for the fp64 path
+  r.ok = true;
+  r.gate_stage = 4;
+  return r;
+}
+
+// Uniform grid over the transformed target (the oracle's own radius search; it
+// returns candidate supersets, the exact gates decide).
+struct Grid {
+  double h = 1.0;
+  std::vector<std::pair<uint64_t, int>> keyed;  // (cell key, index) sorted
+  static uint64_t key(int64_t cx, int64_t cy, int64_t cz) {
+    const int64_t off = 1 << 20;
+    return ((uint64_t)(cx + off) << 42) | ((uint64_t)(cy + off) << 21) | (uint64_t)(cz + off);
+  }
+  void build(const float* pts, int m, double cell) {
+    h = cell;
+    keyed.resize(m);
+    for (int j = 0; j < m; ++j) {
+      int64_t cx = (int64_t)std::floor(pts[3 * j] / h), cy = (int64_t)std::floor(pts[3 * j + 1] / h),
+              cz = (int64_t)std::floor(pts[3 * j + 2] / h);
+      keyed[j] = {key(cx, cy, cz), j};
+    }
+    std::sort(keyed.begin(), keyed.end());
+  }
+  template <class F>
+  void query(const float* x, double r, F&& f) const {
+    int64_t lo[3], hi[3];
+    for (int d = 0; d < 3; ++d) {
+      lo[d] = (int64_t)std::floor((x[d] - r) / h);
+      hi[d] = (int64_t)std::floor((x[d] + r) / h);
+    }
+    for (int64_t cx = lo[0]; cx <= hi[0]; ++cx)
+      for (int64_t cy = lo[1]; cy <= hi[1]; ++cy)
+        for (int64_t cz = lo[2]; cz <= hi[2]; ++cz) {
+          uint64_t kk = key(cx, cy, cz);
+          auto it = std::lower_bound(keyed.begin(), keyed.end(), std::make_pair(kk, -1));
+          for (; it != keyed.end() && it->first == kk; ++it) f(it->second);
+        }
+  }
+};
+
+struct AssocOut {
+  std::vector<int32_t> counts, cols;
+  std::vector<float> vals;
+  orc_counters cnt{};
+};
+
+// fill_in_A_mat_gpu semantics (CvoGPU.cu:951-1099).  yprime: the transformed
+// target in float (mirror) -- for precision FP64 the double transform is used.
+void assoc_core(const cvo_params* p, const CloudRef& S, const CloudRef& Tg, const Pose64& P, float ell,
+                int K, int precision, int brute, AssocOut& out) {
+  const Consts k = make_consts(p);
+  const int N = S.n, M = Tg.n;
+  out.counts.assign(N, 0);
+  out.cols.assign((size_t)N * K, -1);
+  out.vals.assign((size_t)N * K, 0.0f);
+  std::vector<float> yp((size_t)M * 3);
+  std::vector<double> yp64;
+  float A[9], b[3];
+  applied_transform(P, A, b);
+  for (int j = 0; j < M; ++j) xform_f(A, b, Tg.xyz + 3 * (size_t)j, &yp[3 * (size_t)j]);
+  if (precision == ORC_FP64) {
+    yp64.resize((size_t)M * 3);
+    for (int j = 0; j < M; ++j) {
+      const float* y = Tg.xyz + 3 * (size_t)j;
+      for (int i = 0; i < 3; ++i) {
+        double s = -(P.R[0 * 3 + i] * P.T[0] + P.R[1 * 3 + i] * P.T[1] + P.R[2 * 3 + i] * P.T[2]);
+        for (int c = 0; c < 3; ++c) s += P.R[c * 3 + i] * (double)y[c];
+        yp64[3 * (size_t)j + i] = s;
+      }
+    }
+  }
+  bool use_grid = !brute && k.geo;
+  Grid grid;
+  double rmax = 0;
+  if (use_grid) {
+    for (int i = 0; i < N; ++i) {
+      float l = range_ell(ell, S.xyz + 3 * (size_t)i);
+      rmax = std::max(rmax, std::sqrt(std::max(0.0, (double)d2_geo_thres(k, l))));
+    }
+    rmax = rmax * 1.001 + 1e-6;
+    // the grid holds the float transformed points for both precisions (it only
+    // proposes candidates; the exact gates decide)
+    grid.build(yp.data(), M, std::max(rmax, 1e-6));
+  }
+  int64_t c_geo = 0, c_col = 0, c_sem = 0, c_pass = 0, c_store = 0;
+  int32_t mrow = 0;
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : c_geo, c_col, c_sem, c_pass, c_store) reduction(max : mrow)
+  for (int i = 0; i < N; ++i) {
+    const float* xi = S.xyz + 3 * (size_t)i;
+    float l = range_ell(ell, xi);
+    float d2t = d2_geo_thres(k, l);
+    double l64 = ((double)std::sqrt((double)xi[0] * xi[0] + (double)xi[1] * xi[1] + (double)xi[2] * xi[2]) / 500.0 + 1.0) * (double)ell;
+    double xi64[3] = {xi[0], xi[1], xi[2]};
+    std::vector<int> cand;
+    if (use_grid) {
+      double r = std::sqrt(std::max(0.0, (double)d2t)) * 1.001 + 1e-5 * (1.0 + std::fabs(xi[0]) + std::fabs(xi[1]) + std::fabs(xi[2]));
+      grid.query(xi, r, [&](int j) { cand.push_back(j); });
+      std::sort(cand.begin(), cand.end());
+    }
+    int stored = 0;
+    auto visit = [&](int j) {
+      PairEval e;
+      if (precision == ORC_FP64) e = eval_pair_fp64(k, p, S, Tg, i, j, xi64, &yp64[3 * (size_t)j], l64);
+      else e = eval_pair_mirror(k, S, Tg, i, j, xi, &yp[3 * (size_t)j], l, d2t);
+      if (e.gate_stage >= 2) c_geo++;
+      if (e.gate_stage >= 3) c_col++;
+      if (e.gate_stage >= 4) c_sem++;
+      bool pass = e.ok && (precision == ORC_FP64 ? (double)e.margin > (double)p->sp_thres : e.a > p->sp_thres);
+      if (pass) {
+        c_pass++;
+        if (stored < K) {
+          out.cols[(size_t)i * K + stored] = j;
+          out.vals[(size_t)i * K + stored] = e.a;
+          stored++;
+        }
+      }
+    };
+    if (use_grid) {
+      for (int j : cand) visit(j);
+    } else {
+      for (int j = 0; j < M; ++j) visit(j);
+    }
+    out.counts[i] = stored;
+    c_store += stored;
+    mrow = std::max(mrow, stored);
+  }
+  // gate stages are counted per candidate; stage>=2 means the geometric gate passed
+  // (or geometry is off).  Without geometry every target is a geo "pass".
+  out.cnt.geo = c_geo;
+  out.cnt.color = c_col;
+  out.cnt.sem = c_sem;
+  out.cnt.passed = c_pass;
+  out.cnt.stored = c_store;
+  out.cnt.max_row = mrow;
+}
+
+// ---------------------------------------------------------------------------
+// Flow (CvoGPU.cu:2001-2120) and step coefficients (:333-462).
+// ---------------------------------------------------------------------------
+
+void flow_core(const cvo_params* p, int N, const float* X, const float* Y, int K, const int32_t* counts,
+               const int32_t* cols, const float* vals, int precision, double out6[6], double unit6[6]) {
+  double sum[6] = {0, 0, 0, 0, 0, 0};
+  if (precision == ORC_MIRROR) {
+    std::vector<double> rows((size_t)N * 6);
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < N; ++i) {
+      const float* x = X + 3 * (size_t)i;
+      float om[3] = {0, 0, 0}, v[3] = {0, 0, 0};
+      for (int s = 0; s < counts[i]; ++s) {
+        int j = cols[(size_t)i * K + s];
+        float a = vals[(size_t)i * K + s];
+        const float* y = Y + 3 * (size_t)j;
+        float cr[3] = {x[1] * y[2] - x[2] * y[1], x[2] * y[0] - x[0] * y[2], x[0] * y[1] - x[1] * y[0]};
+        for (int d = 0; d < 3; ++d) {
+          om[d] = om[d] + cr[d] * a;
+          v[d] = v[d] + (y[d] - x[d]) * a;
+        }
+      }
+      for (int d = 0; d < 3; ++d) {
+        rows[(size_t)i * 6 + d] = (double)(om[d] / p->c);
+        rows[(size_t)i * 6 + 3 + d] = (double)(v[d] / p->d);
+      }
+    }
+    for (int i = 0; i < N; ++i)
+      for (int d = 0; d < 6; ++d) sum[d] += rows[(size_t)i * 6 + d];
+    float f[6];
+    for (int d = 0; d < 6; ++d) f[d] = static_cast<float>(sum[d]);
+    float z = 0.0f;
+    for (int d = 0; d < 6; ++d) z += f[d] * f[d];
+    if (z > 0.0f) {
+      float s = sqrtf(z);
+      for (int d = 0; d < 6; ++d) f[d] = f[d] / s;
+    }
+    for (int d = 0; d < 6; ++d) {
+      out6[d] = sum[d];
+      unit6[d] = f[d];
+    }
+  } else {
+    std::vector<double> rows((size_t)N * 6);
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < N; ++i) {
+      const float* xf = X + 3 * (size_t)i;
+      double x[3] = {xf[0], xf[1], xf[2]};
+      double om[3] = {0, 0, 0}, v[3] = {0, 0, 0};
+      for (int s = 0; s < counts[i]; ++s) {
+        int j = cols[(size_t)i * K + s];
+        double a = vals[(size_t)i * K + s];
+        const float* yf = Y + 3 * (size_t)j;
+        double y[3] = {yf[0], yf[1], yf[2]};
+        double cr[3] = {x[1] * y[2] - x[2] * y[1], x[2] * y[0] - x[0] * y[2], x[0] * y[1] - x[1] * y[0]};
+        for (int d = 0; d < 3; ++d) {
+          om[d] += cr[d] * a;
+          v[d] += (y[d] - x[d]) * a;
+        }
+      }
+      for (int d = 0; d < 3; ++d) {
+        rows[(size_t)i * 6 + d] = om[d] / (double)p->c;
+        rows[(size_t)i * 6 + 3 + d] = v[d] / (double)p->d;
+      }
+    }
+    for (int i = 0; i < N; ++i)
+      for (int d = 0; d < 6; ++d) sum[d] += rows[(size_t)i * 6 + d];
+    double z = 0;
+    for (int d = 0; d < 6; ++d) z += sum[d] * sum[d];
+    double s = z > 0 ? std::sqrt(z) : 1.0;
+    for (int d = 0; d < 6; ++d) {
+      out6[d] = sum[d];
+      unit6[d] = z > 0 ? sum[d] / s : sum[d];
+    }
+  }
+}
+
+template <class T>
+inline void cross3(const T* a, const T* b, T* o) {
+  o[0] = a[1] * b[2] - a[2] * b[1];
+  o[1] = a[2] * b[0] - a[0] * b[2];
+  o[2] = a[0] * b[1] - a[1] * b[0];
+}
+template <class T>
+inline T dot3(const T* a, const T* b) {
+  return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+
+void step_core(const cvo_params* p, float ell, int N, const float* X, const float* Y, int K, const int32_t* counts,
+               const int32_t* cols, const float* vals, const double flow6[6], int precision, double bcde[4]) {
+  double tot[4] = {0, 0, 0, 0};
+  std::vector<double> rows((size_t)N * 4);
+  if (precision == ORC_MIRROR) {
+    float w[3] = {(float)flow6[0], (float)flow6[1], (float)flow6[2]};
+    float v[3] = {(float)flow6[3], (float)flow6[4], (float)flow6[5]};
+    // Omega = skew(w) and its powers (float), compute_step_size_xi (:333-378)
+    float W[9] = {0, -w[2], w[1], w[2], 0, -w[0], -w[1], w[0], 0};
+    auto mm = [](const float* A, const float* B, float* C) {
+      for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) C[r * 3 + c] = A[r * 3 + 0] * B[0 * 3 + c] + A[r * 3 + 1] * B[1 * 3 + c] + A[r * 3 + 2] * B[2 * 3 + c];
+    };
+    auto mv = [](const float* A, const float* x, float* o) {
+      for (int r = 0; r < 3; ++r) o[r] = A[r * 3 + 0] * x[0] + A[r * 3 + 1] * x[1] + A[r * 3 + 2] * x[2];
+    };
+    float W2[9], W3[9], W4[9];
+    mm(W, W, W2);
+    mm(W2, W, W3);
+    mm(W3, W, W4);
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < N; ++i) {
+      const float* x = X + 3 * (size_t)i;
+      float px_norm = sqrtf(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+      float temp_ell = ell;
+      if (p->is_using_range_ell) temp_ell = static_cast<float>(((double)px_norm / 500.0 + 1.0) * (double)ell);
+      float temp_coef = static_cast<float>(1.0 / (2.0 * (double)temp_ell * (double)temp_ell));
+      double B = 0, C = 0, D = 0, E = 0;
+      for (int s = 0; s < counts[i]; ++s) {
+        int j = cols[(size_t)i * K + s];
+        float Aij = vals[(size_t)i * K + s];
+        const float* z = Y + 3 * (size_t)j;
+        float xiz[3], t1[3], t2[3], xi2z[3], xi3z[3], xi4z[3];
+        cross3(w, z, xiz);
+        for (int d = 0; d < 3; ++d) xiz[d] += v[d];
+        mv(W2, z, t1); mv(W, v, t2);
+        for (int d = 0; d < 3; ++d) xi2z[d] = t1[d] + t2[d];
+        mv(W3, z, t1); mv(W2, v, t2);
+        for (int d = 0; d < 3; ++d) xi3z[d] = t1[d] + t2[d];
+        mv(W4, z, t1); mv(W3, v, t2);
+        for (int d = 0; d < 3; ++d) xi4z[d] = t1[d] + t2[d];
+        float normxiz2 = dot3(xiz, xiz);
+        float xiz_dot_xi2z = -dot3(xiz, xi2z);
+        float epsil_const = dot3(xi2z, xi2z) + 2.0f * dot3(xiz, xi3z);
+        float diff[3] = {x[0] - z[0], x[1] - z[1], x[2] - z[2]};
+        float two_xi2z[3] = {2.0f * xi2z[0], 2.0f * xi2z[1], 2.0f * xi2z[2]};
+        float m_xi3z[3] = {-xi3z[0], -xi3z[1], -xi3z[2]};
+        float two_xi4z[3] = {2.0f * xi4z[0], 2.0f * xi4z[1], 2.0f * xi4z[2]};
+        float beta = static_cast<float>(-2.0 * (double)temp_coef * (double)dot3(xiz, diff));
+        float gamma = -temp_coef * (normxiz2 + dot3(two_xi2z, diff));
+        float delta = static_cast<float>(2.0 * (double)temp_coef * (double)(xiz_dot_xi2z + dot3(m_xi3z, diff)));
+        float epsil = -temp_coef * (epsil_const + dot3(two_xi4z, diff));
+        double bi = (double)(Aij * beta);
+        double ci = (double)Aij * ((double)gamma + (double)(beta * beta) / 2.0);
+        double di = (double)Aij * ((double)(delta + beta * gamma) + (double)(beta * beta * beta) / 6.0);
+        double ei = (double)Aij * ((double)(epsil + beta * delta) + 0.5 * (double)beta * (double)beta * (double)gamma +
+                                   0.5 * (double)gamma * (double)gamma +
+                                   (1 / 24.0) * (double)beta * (double)beta * (double)beta * (double)beta);
+        B += bi; C += ci; D += di; E += ei;
+      }
+      rows[(size_t)i * 4 + 0] = B; rows[(size_t)i * 4 + 1] = C;
+      rows[(size_t)i * 4 + 2] = D; rows[(size_t)i * 4 + 3] = E;
+    }
+  } else {
+    double w[3] = {flow6[0], flow6[1], flow6[2]};
+    double v[3] = {flow6[3], flow6[4], flow6[5]};
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < N; ++i) {
+      const float* xf = X + 3 * (size_t)i;
+      double x[3] = {xf[0], xf[1], xf[2]};
+      double temp_ell = ell;
+      if (p->is_using_range_ell) temp_ell = (std::sqrt(dot3(x, x)) / 500.0 + 1.0) * (double)ell;
+      double c = 1.0 / (2.0 * temp_ell * temp_ell);
+      double B = 0, C = 0, D = 0, E = 0;
+      for (int s = 0; s < counts[i]; ++s) {
+        int j = cols[(size_t)i * K + s];
+        double a = vals[(size_t)i * K + s];
+        const float* zf = Y + 3 * (size_t)j;
+        double z[3] = {zf[0], zf[1], zf[2]};
+        // xi^k z = w x xi^{k-1} z (homogeneous twist powers), xi z = w x z + v
+        double x1[3], x2[3], x3[3], x4[3];
+        cross3(w, z, x1);
+        for (int d = 0; d < 3; ++d) x1[d] += v[d];
+        cross3(w, x1, x2);
+        cross3(w, x2, x3);
+        cross3(w, x3, x4);
+        double diff[3] = {x[0] - z[0], x[1] - z[1], x[2] - z[2]};
+        double beta = -2.0 * c * dot3(x1, diff);
+        double gamma = -c * (dot3(x1, x1) + 2.0 * dot3(x2, diff));
+        double delta = 2.0 * c * (-dot3(x1, x2) - dot3(x3, diff));
+        double epsil = -c * (dot3(x2, x2) + 2.0 * dot3(x1, x3) + 2.0 * dot3(x4, diff));
+        B += a * beta;
+        C += a * (gamma + beta * beta / 2.0);
+        D += a * (delta + beta * gamma + beta * beta * beta / 6.0);
+        E += a * (epsil + beta * delta + 0.5 * beta * beta * gamma + 0.5 * gamma * gamma + beta * beta * beta * beta / 24.0);
+      }
+      rows[(size_t)i * 4 + 0] = B; rows[(size_t)i * 4 + 1] = C;
+      rows[(size_t)i * 4 + 2] = D; rows[(size_t)i * 4 + 3] = E;
+    }
+  }
+  for (int i = 0; i < N; ++i)
+    for (int d = 0; d < 4; ++d) tot[d] += rows[(size_t)i * 4 + d];
+  for (int d = 0; d < 4; ++d) bcde[d] = tot[d];
+}
+
+// Roots of a cubic by the Aberth-Ehrlich simultaneous iteration in long double
+// (a different method from the device's closed form, so the two cross-check).
+void cubic_roots(const double c[4], double re[3], double im[3]) {
+  typedef std::complex<long double> C;
+  long double a0 = c[0];
+  if (a0 == 0.0L || !std::isfinite((double)a0)) {
+    for (int i = 0; i < 3; ++i) { re[i] = std::numeric_limits<double>::quiet_NaN(); im[i] = 0; }
+    return;
+  }
+  C q[4] = {C(1.0L), C((long double)c[1] / a0), C((long double)c[2] / a0), C((long double)c[3] / a0)};
+  for (int i = 1; i < 4; ++i) {
+    if (!std::isfinite((double)q[i].real())) {
+      for (int k = 0; k < 3; ++k) { re[k] = std::numeric_limits<double>::quiet_NaN(); im[k] = 0; }
+      return;
+    }
+  }
+  auto P = [&](C z) { return ((q[0] * z + q[1]) * z + q[2]) * z + q[3]; };
+  auto dP = [&](C z) { return (3.0L * q[0] * z + 2.0L * q[1]) * z + q[2]; };
+  long double bound = 1.0L;
+  for (int i = 1; i < 4; ++i) bound = std::max(bound, 1.0L + std::abs(q[i]));
+  C z[3];
+  for (int k = 0; k < 3; ++k) z[k] = std::polar(bound * 0.5L, (long double)(0.4 + 2.0943951023931954923 * k));
+  for (int it = 0; it < 500; ++it) {
+    long double change = 0;
+    for (int k = 0; k < 3; ++k) {
+      C pk = P(z[k]), dk = dP(z[k]);
+      if (std::abs(pk) == 0.0L) continue;
+      C ratio = pk / dk;
+      C s = 0;
+      for (int m = 0; m < 3; ++m)
+        if (m != k) s += 1.0L / (z[k] - z[m]);
+      C w = ratio / (1.0L - ratio * s);
+      if (!std::isfinite((double)std::abs(w))) w = ratio;
+      z[k] -= w;
+      change = std::max(change, std::abs(w) / std::max(1.0L, std::abs(z[k])));
+    }
+    if (change < 1e-17L) break;
+  }
+  // Polish with Newton on each root; conjugate pairs of a real polynomial
+  for (int k = 0; k < 3; ++k) {
+    for (int it = 0; it < 3; ++it) {
+      C d = dP(z[k]);
+      if (std::abs(d) == 0.0L) break;
+      z[k] -= P(z[k]) / d;
+    }
+    re[k] = (double)z[k].real();
+    im[k] = (double)z[k].imag();
+  }
+}
+
+// compute_step_size root selection (CvoGPU.cu:504-538).  Note the reference's
+// clamp runs on temp_step itself, so "no usable root" (temp_step = DBL_MAX) ends
+// at max_step.
+double select_step(const cvo_params* p, const double bcde[4]) {
+  double coef[4] = {4.0 * bcde[3], 3.0 * bcde[2], 2.0 * bcde[1], bcde[0]};
+  double re[3], im[3];
+  cubic_roots(coef, re, im);
+  double temp = std::numeric_limits<double>::max();
+  for (int i = 0; i < 3; ++i)
+    if (re[i] > 0 && re[i] < temp && std::fabs(im[i]) < 1e-5) temp = re[i];
+  double step;
+  if (temp > (double)p->max_step) step = p->max_step;
+  else if (temp < (double)p->min_step) step = p->min_step;
+  else step = temp;
+  return step;
+}
+
+// Exp_SEK3 float form (LieGroup.cpp:244-275) including its theta<1e-6 branch.
+void exp_sek3_f32(const float xi[6], float dt, float out[12]) {
+  float w[3] = {xi[0], xi[1], xi[2]};
+  float theta = sqrtf(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+  float R[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, J[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+  if (!(theta < 1e-6f)) {
+    float A[9] = {0, -w[2], w[1], w[2], 0, -w[0], -w[1], w[0], 0};
+    float A2[9];
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) A2[r * 3 + c] = A[r * 3 + 0] * A[0 * 3 + c] + A[r * 3 + 1] * A[1 * 3 + c] + A[r * 3 + 2] * A[2 * 3 + c];
+    float theta2 = theta * theta;
+    float st = sinf(dt * theta), ct = cosf(dt * theta);
+    float omc = (1 - ct) / theta2;
+    float k3 = (dt * theta - st) / (theta2 * theta);
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) {
+        float I = r == c ? 1.0f : 0.0f;
+        R[r * 3 + c] = I + (st / theta) * A[r * 3 + c] + omc * A2[r * 3 + c];
+        J[r * 3 + c] = dt * I + omc * A[r * 3 + c] + k3 * A2[r * 3 + c];
+      }
+  }
+  for (int r = 0; r < 3; ++r) {
+    for (int c = 0; c < 3; ++c) out[r * 4 + c] = R[r * 3 + c];
+    out[r * 4 + 3] = J[r * 3 + 0] * xi[3] + J[r * 3 + 1] * xi[4] + J[r * 3 + 2] * xi[5];
+  }
+}
+
+// exp(dt * [w; v]) in FP64 with exact small-angle series.
+void exp_se3_f64(const double xi[6], double dt, double out[12]) {
+  double ph[3] = {xi[0] * dt, xi[1] * dt, xi[2] * dt};
+  double u[3] = {xi[3] * dt, xi[4] * dt, xi[5] * dt};
+  double th2 = ph[0] * ph[0] + ph[1] * ph[1] + ph[2] * ph[2];
+  double th = std::sqrt(th2);
+  double A, B, Cc;  // sin/th, (1-cos)/th^2, (th - sin)/th^3
+  if (th < 1e-3) {
+    A = 1 - th2 / 6 + th2 * th2 / 120;
+    B = 0.5 - th2 / 24 + th2 * th2 / 720;
+    Cc = 1.0 / 6 - th2 / 120 + th2 * th2 / 5040;
+  } else {
+    A = std::sin(th) / th;
+    B = (1 - std::cos(th)) / th2;
+    Cc = (th - std::sin(th)) / (th2 * th);
+  }
+  double W[9] = {0, -ph[2], ph[1], ph[2], 0, -ph[0], -ph[1], ph[0], 0};
+  double W2[9];
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) W2[r * 3 + c] = W[r * 3 + 0] * W[0 * 3 + c] + W[r * 3 + 1] * W[1 * 3 + c] + W[r * 3 + 2] * W[2 * 3 + c];
+  double V[9];
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) {
+      double I = r == c ? 1.0 : 0.0;
+      out[r * 4 + c] = I + A * W[r * 3 + c] + B * W2[r * 3 + c];
+      V[r * 3 + c] = I + B * W[r * 3 + c] + Cc * W2[r * 3 + c];
+    }
+  for (int r = 0; r < 3; ++r) out[r * 4 + 3] = V[r * 3 + 0] * u[0] + V[r * 3 + 1] * u[1] + V[r * 3 + 2] * u[2];
+}
+
+// SE(3) log (tangent [upsilon; omega]) via the rotation's unit quaternion, the
+// same construction as Sophus::SE3d::log used by dist_se3_cpu.
+void log_se3(const double T[12], double out[6]) {
+  double R[9];
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) R[r * 3 + c] = T[r * 4 + c];
+  // Shepperd's method
+  double qw, qx, qy, qz;
+  double tr = R[0] + R[4] + R[8];
+  if (tr > 0) {
+    double s = std::sqrt(tr + 1.0) * 2;
+    qw = 0.25 * s; qx = (R[7] - R[5]) / s; qy = (R[2] - R[6]) / s; qz = (R[3] - R[1]) / s;
+  } else if (R[0] > R[4] && R[0] > R[8]) {
+    double s = std::sqrt(1.0 + R[0] - R[4] - R[8]) * 2;
+    qw = (R[7] - R[5]) / s; qx = 0.25 * s; qy = (R[1] + R[3]) / s; qz = (R[2] + R[6]) / s;
+  } else if (R[4] > R[8]) {
+    double s = std::sqrt(1.0 + R[4] - R[0] - R[8]) * 2;
+    qw = (R[2] - R[6]) / s; qx = (R[1] + R[3]) / s; qy = 0.25 * s; qz = (R[5] + R[7]) / s;
+  } else {
+    double s = std::sqrt(1.0 + R[8] - R[0] - R[4]) * 2;
+    qw = (R[3] - R[1]) / s; qx = (R[2] + R[6]) / s; qy = (R[5] + R[7]) / s; qz = 0.25 * s;
+  }
+  double qn = std::sqrt(qw * qw + qx * qx + qy * qy + qz * qz);
+  qw /= qn; qx /= qn; qy /= qn; qz /= qn;
+  double n2 = qx * qx + qy * qy + qz * qz, n = std::sqrt(n2);
+  double f;  // omega = f * q_vec
+  if (n2 < 1e-20) {
+    f = 2.0 / qw - 2.0 * n2 / (qw * qw * qw);
+  } else if (std::fabs(qw) < 1e-10) {
+    f = (qw > 0 ? M_PI : -M_PI) / n;
+  } else {
+    f = 2.0 * std::atan(n / qw) / n;
+  }
+  double w[3] = {f * qx, f * qy, f * qz};
+  double th = std::sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+  double W[9] = {0, -w[2], w[1], w[2], 0, -w[0], -w[1], w[0], 0};
+  double W2[9];
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) W2[r * 3 + c] = W[r * 3 + 0] * W[0 * 3 + c] + W[r * 3 + 1] * W[1 * 3 + c] + W[r * 3 + 2] * W[2 * 3 + c];
+  double k2;
+  if (th < 1e-6) k2 = 1.0 / 12.0;
+  else k2 = (1.0 - th * std::cos(0.5 * th) / (2.0 * std::sin(0.5 * th))) / (th * th);
+  double t[3] = {T[3], T[7], T[11]};
+  for (int r = 0; r < 3; ++r) {
+    double s = 0;
+    for (int c = 0; c < 3; ++c) {
+      double Vi = (r == c ? 1.0 : 0.0) - 0.5 * W[r * 3 + c] + k2 * W2[r * 3 + c];
+      s += Vi * t[c];
+    }
+    out[r] = s;
+  }
+  out[3] = w[0]; out[4] = w[1]; out[5] = w[2];
+}
+
+double log_se3_norm(const double T[12]) {
+  double v[6];
+  log_se3(T, v);
+  double s = 0;
+  for (int i = 0; i < 6; ++i) s += v[i] * v[i];
+  return std::sqrt(s);
+}
+
+// A_sparsity_indicator_ell_update (CvoGPU.cu:547-665) with float sums.
+template <class T>
+struct Indicator {
+  std::deque<T> qs, qe;
+  T ss = 0, se = 0;
+  bool update(T ind, int W, T thr) {
+    bool dec = false;
+    if ((int)qs.size() < W) { qs.push_back(ind); ss += ind; }
+    if ((int)qs.size() >= W && (int)qe.size() < W) { qe.push_back(ind); se += ind; }
+    if ((int)qs.size() >= W && (int)qe.size() >= W) {
+      T ratio = se / ss;
+      if (ratio > (T)1 - thr && ratio < (T)1 + thr) {
+        dec = true;
+        qs.clear(); qe.clear();
+        ss = 0; se = 0;
+      } else {
+        T f = qe.front();
+        se -= f;
+        ss += f;
+        qs.push_back(f);
+        qe.pop_front();
+        ss -= qs.front();
+        qs.pop_front();
+        qe.push_back(ind);
+        se += ind;
+      }
+    }
+    return dec;
+  }
+};
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+
+extern "C" int orc_assoc(const cvo_params* p, const cvo_cloud_view* src, const cvo_cloud_view* tgt,
+                         const float T[16], float ell, int32_t K, int32_t precision, int32_t brute,
+                         int32_t* counts, int32_t* cols, float* vals, orc_counters* cnt) {
+  if (!p || !src || !tgt || !T || K <= 0) return 1;
+  AssocOut o;
+  assoc_core(p, ref(src), ref(tgt), pose_from16(T), ell, K, precision, brute, o);
+  if (counts) std::memcpy(counts, o.counts.data(), sizeof(int32_t) * o.counts.size());
+  if (cols) std::memcpy(cols, o.cols.data(), sizeof(int32_t) * o.cols.size());
+  if (vals) std::memcpy(vals, o.vals.data(), sizeof(float) * o.vals.size());
+  if (cnt) *cnt = o.cnt;
+  return 0;
+}
+
+extern "C" int orc_pair_value(const cvo_params* p, const cvo_cloud_view* src, const cvo_cloud_view* tgt,
+                              const float T[16], float ell, int32_t i, int32_t j, double* a_out,
+                              double* gate_margin) {
+  const Consts k = make_consts(p);
+  CloudRef S = ref(src), Tg = ref(tgt);
+  float A[9], b[3], yp[3];
+  applied_transform(pose_from16(T), A, b);
+  xform_f(A, b, Tg.xyz + 3 * (size_t)j, yp);
+  const float* xi = S.xyz + 3 * (size_t)i;
+  float l = range_ell(ell, xi);
+  PairEval e = eval_pair_mirror(k, S, Tg, i, j, xi, yp, l, d2_geo_thres(k, l));
+  // also the exact (double) product value, for the sp_thres margin
+  double a_exact = -1.0;
+  if (e.ok) a_exact = (double)e.a;
+  if (a_out) *a_out = a_exact;
+  if (gate_margin) *gate_margin = e.margin;
+  return e.ok ? 0 : 2;
+}
+
+extern "C" int orc_transform_target(const float T[16], int32_t m, const float* xyz, float* out) {
+  float A[9], b[3];
+  applied_transform(pose_from16(T), A, b);
+  for (int j = 0; j < m; ++j) xform_f(A, b, xyz + 3 * (size_t)j, out + 3 * (size_t)j);
+  return 0;
+}
+
+extern "C" int orc_flow(const cvo_params* p, int32_t n, const float* src_xyz, const float* yprime, int32_t K,
+                        const int32_t* counts, const int32_t* cols, const float* vals, int32_t precision,
+                        double out6[6], double unit6[6]) {
+  flow_core(p, n, src_xyz, yprime, K, counts, cols, vals, precision, out6, unit6);
+  return 0;
+}
+
+extern "C" int orc_step_coeffs(const cvo_params* p, float ell, int32_t n, const float* src_xyz, const float* yprime,
+                               int32_t K, const int32_t* counts, const int32_t* cols, const float* vals,
+                               const double flow6[6], int32_t precision, double bcde[4]) {
+  step_core(p, ell, n, src_xyz, yprime, K, counts, cols, vals, flow6, precision, bcde);
+  return 0;
+}
+
+extern "C" int orc_cubic_roots(const double c[4], double re[3], double im[3]) {
+  cubic_roots(c, re, im);
+  return 0;
+}
+
+extern "C" double orc_select_step(const cvo_params* p, const double bcde[4]) { return select_step(p, bcde); }
+
+extern "C" int orc_exp_sek3_f32(const float xi[6], float dt, float out12[12]) {
+  exp_sek3_f32(xi, dt, out12);
+  return 0;
+}
+
+extern "C" int orc_exp_se3_f64(const double xi[6], double dt, double out12[12]) {
+  exp_se3_f64(xi, dt, out12);
+  return 0;
+}
+
+extern "C" double orc_log_se3_norm(const double T12[12]) { return log_se3_norm(T12); }
+
+extern "C" int orc_log_se3(const double T12[12], double out6[6]) {
+  log_se3(T12, out6);
+  return 0;
+}
+
+extern "C" int orc_indicator_sequence(int32_t window, float threshold, int32_t n, const float* ind, int32_t* dec) {
+  Indicator<float> st;
+  for (int i = 0; i < n; ++i) dec[i] = st.update(ind[i], window, threshold) ? 1 : 0;
+  return 0;
+}
+
+extern "C" int orc_align(const cvo_params* p, const cvo_cloud_view* src, const cvo_cloud_view* tgt,
+                         const float T_init[16], int32_t precision, int32_t brute, float T_out[16],
+                         int32_t* iters, int32_t* ret_code, cvo_iter_record* trace, int32_t trace_cap) {
+  if (!p || !src || !tgt || !T_init || !T_out) return 1;
+  CloudRef S = ref(src), Tg = ref(tgt);
+  const int N = S.n, M = Tg.n;
+  // pose state: float (mirror) or double
+  float Rf[9], Tf[3];
+  double Rd[9], Td[3];
+  for (int r = 0; r < 3; ++r) {
+    for (int c = 0; c < 3; ++c) { Rf[r * 3 + c] = T_init[r * 4 + c]; Rd[r * 3 + c] = T_init[r * 4 + c]; }
+    Tf[r] = T_init[r * 4 + 3];
+    Td[r] = T_init[r * 4 + 3];
+  }
+  float ell = p->ell_init;
+  double ell64 = p->ell_init;
+  int K = p->nearest_neighbors_max;
+  Indicator<float> ind32;
+  Indicator<double> ind64;
+  int ret = 0, k = 0;
+  std::vector<float> yp((size_t)M * 3);
+  for (; k < p->MAX_ITER; ++k) {
+    Pose64 P;
+    for (int i = 0; i < 9; ++i) P.R[i] = precision == ORC_MIRROR ? (double)Rf[i] : Rd[i];
+    for (int i = 0; i < 3; ++i) P.T[i] = precision == ORC_MIRROR ? (double)Tf[i] : Td[i];
+    float A[9], b[3];
+    applied_transform(P, A, b);
+    for (int j = 0; j < M; ++j) xform_f(A, b, Tg.xyz + 3 * (size_t)j, &yp[3 * (size_t)j]);
+    AssocOut o;
+    float ell_used = precision == ORC_MIRROR ? ell : (float)ell64;
+    assoc_core(p, S, Tg, P, ell_used, K, precision, brute, o);
+    double sum6[6], unit6[6];
+    flow_core(p, N, S.xyz, yp.data(), K, o.counts.data(), o.cols.data(), o.vals.data(), precision, sum6, unit6);
+    double bcde[4];
+    step_core(p, ell_used, N, S.xyz, yp.data(), K, o.counts.data(), o.cols.data(), o.vals.data(), unit6, precision, bcde);
+    double step = select_step(p, bcde);
+    if (precision == ORC_MIRROR) step = (double)(float)step;  // cvo_state->step is float
+    // stop test on the normalised flow (CvoGPU.cu:1354-1360)
+    double nw = std::sqrt(unit6[0] * unit6[0] + unit6[1] * unit6[1] + unit6[2] * unit6[2]);
+    double nv = std::sqrt(unit6[3] * unit6[3] + unit6[4] * unit6[4] + unit6[5] * unit6[5]);
+    cvo_iter_record rec{};
+    rec.ell = ell_used;
+    rec.step = (float)step;
+    rec.nnz = (int32_t)o.cnt.stored;
+    rec.max_row = o.cnt.max_row;
+    rec.num_neighbors = K;
+    rec.flow_norm = (float)std::sqrt(sum6[0] * sum6[0] + sum6[1] * sum6[1] + sum6[2] * sum6[2] + sum6[3] * sum6[3] +
+                                     sum6[4] * sum6[4] + sum6[5] * sum6[5]);
+    if (nw < p->eps && nv < p->eps) {
+      if (nw < 1e-8 && nv < 1e-8) ret = -1;
+      if (trace && k < trace_cap) trace[k] = rec;
+      break;
+    }
+    double dRT[12];
+    if (precision == ORC_MIRROR) {
+      float xi[6];
+      for (int d = 0; d < 6; ++d) xi[d] = (float)unit6[d];
+      float dt[12];
+      exp_sek3_f32(xi, (float)step, dt);
+      double dR[9], dT[3];
+      for (int r = 0; r < 3; ++r) {
+        for (int c = 0; c < 3; ++c) dR[r * 3 + c] = dt[r * 4 + c];
+        dT[r] = dt[r * 4 + 3];
+      }
+      float Tn[3], Rn[9];
+      for (int r = 0; r < 3; ++r)
+        Tn[r] = (float)((double)Rf[r * 3 + 0] * dT[0] + (double)Rf[r * 3 + 1] * dT[1] + (double)Rf[r * 3 + 2] * dT[2] + (double)Tf[r]);
+      for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c)
+          Rn[r * 3 + c] = (float)((double)Rf[r * 3 + 0] * dR[0 * 3 + c] + (double)Rf[r * 3 + 1] * dR[1 * 3 + c] + (double)Rf[r * 3 + 2] * dR[2 * 3 + c]);
+      std::memcpy(Rf, Rn, sizeof(Rf));
+      std::memcpy(Tf, Tn, sizeof(Tf));
+      for (int r = 0; r < 3; ++r) {
+        for (int c = 0; c < 3; ++c) dRT[r * 4 + c] = dR[r * 3 + c];
+        dRT[r * 4 + 3] = dT[r];
+      }
+    } else {
+      exp_se3_f64(unit6, step, dRT);
+      double Rn[9], Tn[3];
+      for (int r = 0; r < 3; ++r) {
+        Tn[r] = Td[r];
+        for (int c = 0; c < 3; ++c) Tn[r] += Rd[r * 3 + c] * dRT[c * 4 + 3];
+        for (int c = 0; c < 3; ++c) {
+          double s = 0;
+          for (int m = 0; m < 3; ++m) s += Rd[r * 3 + m] * dRT[m * 4 + c];
+          Rn[r * 3 + c] = s;
+        }
+      }
+      std::memcpy(Rd, Rn, sizeof(Rd));
+      std::memcpy(Td, Tn, sizeof(Td));
+    }
+    double dist = log_se3_norm(dRT);
+    rec.dist = (float)dist;
+    float ip = (float)((double)o.cnt.stored / std::sqrt((double)N * (double)M));
+    rec.ip = ip;
+    bool dec = precision == ORC_MIRROR ? ind32.update(ip, p->indicator_window_size, p->indicator_stable_threshold)
+                                       : ind64.update((double)ip, p->indicator_window_size, (double)p->indicator_stable_threshold);
+    if (trace && k < trace_cap) trace[k] = rec;
+    if (dist < p->eps_2) break;
+    if (k > p->ell_decay_start && dec) {
+      ell = ell * p->ell_decay_rate;
+      if (ell < p->ell_min) ell = p->ell_min;
+      ell64 = ell64 * (double)p->ell_decay_rate;
+      if (ell64 < (double)p->ell_min) ell64 = p->ell_min;
+    }
+    K = std::min(p->nearest_neighbors_max, (int)(o.cnt.max_row * 1.2));
+  }
+  if (iters) *iters = k;
+  if (ret_code) *ret_code = ret;
+  // final update_tf: transform = [R^T, -R^T T]
+  std::memset(T_out, 0, sizeof(float) * 16);
+  T_out[15] = 1.0f;
+  if (precision == ORC_MIRROR) {
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) T_out[r * 4 + c] = Rf[c * 3 + r];
+    for (int r = 0; r < 3; ++r)
+      T_out[r * 4 + 3] = -(T_out[r * 4 + 0] * Tf[0] + T_out[r * 4 + 1] * Tf[1] + T_out[r * 4 + 2] * Tf[2]);
+  } else {
+    for (int r = 0; r < 3; ++r) {
+      for (int c = 0; c < 3; ++c) T_out[r * 4 + c] = (float)Rd[c * 3 + r];
+      T_out[r * 4 + 3] = (float)(-(Rd[0 * 3 + r] * Td[0] + Rd[1 * 3 + r] * Td[1] + Rd[2 * 3 + r] * Td[2]));
+    }
+  }
+  return 0;
+}
+
+extern "C" double orc_inner_product(const cvo_params* p, const cvo_cloud_view* src, const cvo_cloud_view* tgt,
+                                    const float T[16], float ell, int32_t precision) {
+  AssocOut o;
+  int K = p->nearest_neighbors_max;
+  assoc_core(p, ref(src), ref(tgt), pose_from16(T), ell, K, precision, 0, o);
+  double s = 0;
+  for (int i = 0; i < src->n; ++i)
+    for (int q = 0; q < o.counts[i]; ++q) s += (double)o.vals[(size_t)i * K + q];
+  return s;
+}
+
+extern "C" double orc_time_iterations(const cvo_params* p, const cvo_cloud_view* src, const cvo_cloud_view* tgt,
+                                      const float T[16], float ell, int32_t K, int32_t n_iter, int32_t threads) {
+  int old = omp_get_max_threads();
+  if (threads > 0) omp_set_num_threads(threads);
+  CloudRef S = ref(src), Tg = ref(tgt);
+  Pose64 P = pose_from16(T);
+  std::vector<float> yp((size_t)Tg.n * 3);
+  auto t0 = std::chrono::steady_clock::now();
+  for (int it = 0; it < n_iter; ++it) {
+    float A[9], b[3];
+    applied_transform(P, A, b);
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j < Tg.n; ++j) xform_f(A, b, Tg.xyz + 3 * (size_t)j, &yp[3 * (size_t)j]);
+    AssocOut o;
+    assoc_core(p, S, Tg, P, ell, K, ORC_MIRROR, 0, o);
+    double sum6[6], unit6[6], bcde[4];
+    flow_core(p, S.n, S.xyz, yp.data(), K, o.counts.data(), o.cols.data(), o.vals.data(), ORC_MIRROR, sum6, unit6);
+    step_core(p, ell, S.n, S.xyz, yp.data(), K, o.counts.data(), o.cols.data(), o.vals.data(), unit6, ORC_MIRROR, bcde);
+    volatile double sink = select_step(p, bcde);
+    (void)sink;
+  }
+  auto t1 = std::chrono::steady_clock::now();
+  omp_set_num_threads(old);
+  return std::chrono::duration<double>(t1 - t0).count() / std::max(1, n_iter);
+}

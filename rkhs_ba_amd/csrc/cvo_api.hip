@@ -1,0 +1,922 @@
+// C ABI of the gfx950 CVO library: contexts, device clouds, two-view align,
+// inner product / cos angle / association export.
+//
+// Per align call the device work is: source spatial sort, target grid levels,
+// then the iteration loop {assoc_kernel, overflow_kernel, step_kernel}, which is
+// captured in a hipGraph of kChunk iterations and replayed until the device
+// controller sets `done`.  The host never reads device state inside the loop
+// except one 4-byte flag per chunk, double-buffered so the GPU stays busy.
+
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "cvo_internal.h"
+
+namespace cvo {
+
+void launch_assoc(const AssocProblem* probs, int n_probs, int total_blocks, const KConsts& kc, hipStream_t s);
+void launch_overflow(const AssocProblem* probs, int n_probs, const KConsts& kc, hipStream_t s);
+void launch_step(const AssocProblem* probs, int n_probs, int total_blocks, const KConsts& kc, hipStream_t s);
+void launch_init_state(const AssocProblem* probs, int n_probs, const float* T16, int stride16, const KConsts& kc,
+                       float ell0, int K0, hipStream_t s);
+void launch_grid_count(const float* xyz, int n, const GridLevel& L, uint32_t* key, uint32_t* count, hipStream_t s);
+void launch_iota(uint32_t* v, int n, hipStream_t s);
+void launch_gather(const uint32_t* order, int n, const float* xyz, const float* feat, int F, const float* sem, int K,
+                   const float* geo, float4* pos, float* feat_s, float* sem_s, float* geo_s, uint32_t* inv_perm,
+                   hipStream_t s);
+void launch_morton(const float* xyz, int n, const float* bmin, const float* bmax, uint32_t* key, hipStream_t s);
+void launch_export(const AssocProblem* P_dev, int n_src, int64_t* counts_orig, const int64_t* row_ptr, int32_t* col,
+                   float* val, int phase, hipStream_t s);
+void launch_all_done(const AssocProblem* probs, int n_probs, int32_t* out, hipStream_t s);
+
+cvo_status set_err(cvo_ctx* ctx, cvo_status s, const std::string& msg) {
+  if (ctx) ctx->last_error = msg;
+  return s;
+}
+
+cvo_status hip_check(cvo_ctx* ctx, hipError_t e, const char* what) {
+  std::string m = std::string(what) + ": " + hipGetErrorString(e);
+  if (e == hipErrorOutOfMemory) return set_err(ctx, CVO_ERR_OOM, m);
+  return set_err(ctx, CVO_ERR_HIP, m);
+}
+
+cvo_status ensure(cvo_ctx* ctx, DevBuf& b, size_t bytes) {
+  if (bytes == 0) bytes = 16;
+  if (b.cap >= bytes) return CVO_OK;
+  if (b.p) {
+    CVO_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    CVO_HIP(ctx, hipFree(b.p));
+    b.p = nullptr;
+    b.cap = 0;
+  }
+  size_t cap = bytes + bytes / 4;
+  CVO_HIP(ctx, hipMalloc(&b.p, cap));
+  b.cap = cap;
+  return CVO_OK;
+}
+
+template <class T>
+static T* as(DevBuf& b) {
+  return reinterpret_cast<T*>(b.p);
+}
+
+static size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+// ---------------------------------------------------------------------------
+// Kernel constants
+// ---------------------------------------------------------------------------
+
+static KConsts make_kconsts(const cvo_params& p, int F, int Ks, int mode) {
+  KConsts k;
+  std::memset(&k, 0, sizeof(k));
+  k.sigma2 = p.sigma * p.sigma;
+  k.c_sigma2 = p.c_sigma * p.c_sigma;
+  k.s_sigma2 = p.s_sigma * p.s_sigma;
+  k.c2 = p.c_ell * p.c_ell;
+  k.s_ell = p.s_ell;
+  k.lg = logf(p.sp_thres / k.sigma2);
+  float lc = logf(p.sp_thres / k.c_sigma2);
+  float ls = logf(p.sp_thres / k.s_sigma2);
+  k.use_geo = p.is_using_geometry;
+  k.use_col = p.is_using_intensity;
+  k.use_sem = p.is_using_semantics;
+  k.use_gtype = p.is_using_geometric_type;
+  k.d2_c_thres = k.use_col ? (float)(-2.0 * (double)k.c2 * (double)lc) : 1.0f;
+  k.d2_s_thres = k.use_sem ? (float)(-2.0 * (double)k.s_ell * (double)k.s_ell * (double)ls) : 1.0f;
+  k.q_c = (float)(1.0 / (2.0 * (double)k.c2));
+  k.q_s = (float)(1.0 / (2.0 * (double)k.s_ell * (double)k.s_ell));
+  k.sp = p.sp_thres;
+  float amp = 1.0f;
+  if (k.use_geo) amp *= k.sigma2;
+  if (k.use_col) amp *= k.c_sigma2;
+  if (k.use_sem) amp *= k.s_sigma2;
+  k.amp = amp;
+  k.F = F;
+  k.Ks = Ks;
+  k.c = p.c;
+  k.d = p.d;
+  k.eps = p.eps;
+  k.eps_2 = p.eps_2;
+  k.min_step = p.min_step;
+  k.max_step = p.max_step;
+  k.ell_min = p.ell_min;
+  k.ell_decay_rate = p.ell_decay_rate;
+  k.ell_decay_start = p.ell_decay_start;
+  k.max_iter = p.MAX_ITER;
+  k.nn_max = p.nearest_neighbors_max;
+  k.window = p.indicator_window_size;
+  k.ind_thr = p.indicator_stable_threshold;
+  k.range_ell_in_step = p.is_using_range_ell;
+  k.rq_coef = k.use_geo ? sqrtf(-2.0f * k.lg) * 1.001f : 0.0f;
+  k.mode = mode;
+  return k;
+}
+
+// ---------------------------------------------------------------------------
+// Workspace of one or more association problems over one source/target pair
+// ---------------------------------------------------------------------------
+
+struct GridHost {
+  GridDev dev;  // host copy of the device header
+  GridDev* dev_ptr = nullptr;
+};
+
+static cvo_status validate_pair(cvo_ctx* ctx, const cvo_dcloud* s, const cvo_dcloud* t) {
+  const cvo_params& p = ctx->params;
+  if (!s || !t) return set_err(ctx, CVO_ERR_INVALID_ARG, "null cloud");
+  if (s->n == 0 || t->n == 0) return set_err(ctx, CVO_ERR_EMPTY, "empty cloud");
+  if (p.is_using_kdtree) return set_err(ctx, CVO_ERR_UNSUPPORTED, "is_using_kdtree is not supported");
+  if (p.is_using_intensity && (!s->feat || !t->feat || s->F != t->F || s->F <= 0))
+    return set_err(ctx, CVO_ERR_INVALID_ARG, "is_using_intensity needs features of equal dimension on both clouds");
+  if (p.is_using_semantics && (!s->sem || !t->sem || s->K != t->K || s->K <= 0))
+    return set_err(ctx, CVO_ERR_INVALID_ARG, "is_using_semantics needs class vectors of equal dimension on both clouds");
+  if (p.is_using_geometric_type && (!s->geo || !t->geo))
+    return set_err(ctx, CVO_ERR_INVALID_ARG, "is_using_geometric_type needs geo_type on both clouds");
+  if (p.nearest_neighbors_max < 1 || p.nearest_neighbors_max > 1024)
+    return set_err(ctx, CVO_ERR_LIMIT, "nearest_neighbors_max must be in [1, 1024]");
+  if (p.indicator_window_size < 1 || p.indicator_window_size > kMaxWindow)
+    return set_err(ctx, CVO_ERR_LIMIT, "indicator_window_size must be in [1, 64]");
+  return CVO_OK;
+}
+
+// Radius needed at lengthscale ell (host mirror of needed_radius).
+static float radius_for(const KConsts& kc, float ell, float max_norm, float tq) {
+  return kc.rq_coef * ell * (1.0f + max_norm / 500.0f) * 1.0001f + 1e-4f * (1.0f + max_norm + tq) + 1e-6f;
+}
+
+// Spatially sorted source rows (Morton order over the source bounding box).
+static cvo_status prepare_source(cvo_ctx* ctx, const cvo_dcloud* s) {
+  hipStream_t st = ctx->stream;
+  const int n = s->n;
+  CVO_TRY(ensure(ctx, ctx->b_src_key, sizeof(uint32_t) * n * 2));
+  CVO_TRY(ensure(ctx, ctx->b_src_perm, sizeof(uint32_t) * n * 2));
+  uint32_t* keys = as<uint32_t>(ctx->b_src_key);
+  uint32_t* keys2 = keys + n;
+  uint32_t* idx = as<uint32_t>(ctx->b_src_perm);
+  uint32_t* idx2 = idx + n;
+  launch_morton(s->xyz, n, s->bmin, s->bmax, keys, st);
+  launch_iota(idx, n, st);
+  size_t tmp = 0;
+  CVO_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, keys, keys2, idx, idx2, n, 0, 30, st));
+  CVO_TRY(ensure(ctx, ctx->b_cub, tmp));
+  CVO_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(ctx->b_cub.p, tmp, keys, keys2, idx, idx2, n, 0, 30, st));
+  CVO_TRY(ensure(ctx, ctx->b_src_pos, sizeof(float4) * n));
+  float* fs = nullptr;
+  float* ss = nullptr;
+  float* gs = nullptr;
+  if (s->feat) {
+    CVO_TRY(ensure(ctx, ctx->b_src_feat, sizeof(float) * (size_t)n * s->F));
+    fs = as<float>(ctx->b_src_feat);
+  }
+  if (s->sem) {
+    CVO_TRY(ensure(ctx, ctx->b_src_sem, sizeof(float) * (size_t)n * s->K));
+    ss = as<float>(ctx->b_src_sem);
+  }
+  if (s->geo) {
+    CVO_TRY(ensure(ctx, ctx->b_src_geo, sizeof(float) * (size_t)n * 2));
+    gs = as<float>(ctx->b_src_geo);
+  }
+  launch_gather(idx2, n, s->xyz, s->feat, s->F, s->sem, s->K, s->geo, as<float4>(ctx->b_src_pos), fs, ss, gs, nullptr,
+                st);
+  CVO_HIP(ctx, hipGetLastError());
+  return CVO_OK;
+}
+
+// Grid levels over the target for cell sizes hs (descending).
+static cvo_status build_grid(cvo_ctx* ctx, const cvo_dcloud* t, const std::vector<float>& hs, GridHost& gh) {
+  hipStream_t st = ctx->stream;
+  const int n = t->n;
+  const int L = (int)hs.size();
+  GridDev& G = gh.dev;
+  std::memset(&G, 0, sizeof(G));
+  G.n_levels = L;
+  G.n = n;
+  // layout
+  std::vector<size_t> off_cells(L), off_pos(L), off_feat(L), off_sem(L), off_geo(L), off_inv(L);
+  size_t total = 0;
+  uint32_t max_cells = 0;
+  for (int l = 0; l < L; ++l) {
+    GridLevel& lv = G.lv[l];
+    float h = hs[l];
+    float pad = 1e-3f * h;
+    double ext[3];
+    for (int d = 0; d < 3; ++d) ext[d] = (double)t->bmax[d] - (double)t->bmin[d] + 2.0 * pad;
+    double cells = 1.0;
+    for (int d = 0; d < 3; ++d) cells *= std::floor(ext[d] / h) + 1.0;
+    if (cells > (double)kMaxCells) h = (float)(h * std::cbrt(cells / (double)kMaxCells) * 1.01);
+    lv.h = h;
+    lv.inv_h = 1.0f / h;
+    lv.ox = t->bmin[0] - pad;
+    lv.oy = t->bmin[1] - pad;
+    lv.oz = t->bmin[2] - pad;
+    lv.nx = (int)std::floor(ext[0] / h) + 1;
+    lv.ny = (int)std::floor(ext[1] / h) + 1;
+    lv.nz = (int)std::floor(ext[2] / h) + 1;
+    uint32_t nc = (uint32_t)lv.nx * (uint32_t)lv.ny * (uint32_t)lv.nz;
+    max_cells = std::max(max_cells, nc);
+    off_cells[l] = total; total = align_up(total + sizeof(uint32_t) * (nc + 1), 256);
+    off_pos[l] = total; total = align_up(total + sizeof(float4) * n, 256);
+    off_feat[l] = total; total = align_up(total + (t->feat ? sizeof(float) * (size_t)n * t->F : 0), 256);
+    off_sem[l] = total; total = align_up(total + (t->sem ? sizeof(float) * (size_t)n * t->K : 0), 256);
+    off_geo[l] = total; total = align_up(total + (t->geo ? sizeof(float) * (size_t)n * 2 : 0), 256);
+    off_inv[l] = total; total = align_up(total + sizeof(uint32_t) * n, 256);
+  }
+  CVO_TRY(ensure(ctx, ctx->b_grid_store, total));
+  char* base = as<char>(ctx->b_grid_store);
+  // temporaries: keys, keys2, idx, idx2, counts
+  CVO_TRY(ensure(ctx, ctx->b_tmp0, sizeof(uint32_t) * (size_t)n * 4));
+  CVO_TRY(ensure(ctx, ctx->b_tmp1, sizeof(uint32_t) * ((size_t)max_cells + 1)));
+  uint32_t* keys = as<uint32_t>(ctx->b_tmp0);
+  uint32_t* keys2 = keys + n;
+  uint32_t* idx = keys2 + n;
+  uint32_t* idx2 = idx + n;
+  uint32_t* counts = as<uint32_t>(ctx->b_tmp1);
+  for (int l = 0; l < L; ++l) {
+    GridLevel& lv = G.lv[l];
+    uint32_t nc = (uint32_t)lv.nx * (uint32_t)lv.ny * (uint32_t)lv.nz;
+    uint32_t* cell_start = reinterpret_cast<uint32_t*>(base + off_cells[l]);
+    CVO_HIP(ctx, hipMemsetAsync(counts, 0, sizeof(uint32_t) * (nc + 1), st));
+    launch_grid_count(t->xyz, n, lv, keys, counts, st);
+    size_t tmp = 0;
+    CVO_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, counts, cell_start, (int)nc + 1, st));
+    size_t tmp2 = 0;
+    int bits = 1;
+    while ((1u << bits) < nc && bits < 32) ++bits;
+    CVO_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(nullptr, tmp2, keys, keys2, idx, idx2, n, 0, bits, st));
+    CVO_TRY(ensure(ctx, ctx->b_cub, std::max(tmp, tmp2)));
+    CVO_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(ctx->b_cub.p, tmp, counts, cell_start, (int)nc + 1, st));
+    launch_iota(idx, n, st);
+    CVO_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(ctx->b_cub.p, tmp2, keys, keys2, idx, idx2, n, 0, bits, st));
+    lv.cell_start = cell_start;
+    lv.pos = reinterpret_cast<const float4*>(base + off_pos[l]);
+    lv.feat = t->feat ? reinterpret_cast<const float*>(base + off_feat[l]) : nullptr;
+    lv.sem = t->sem ? reinterpret_cast<const float*>(base + off_sem[l]) : nullptr;
+    lv.geo = t->geo ? reinterpret_cast<const float*>(base + off_geo[l]) : nullptr;
+    lv.inv_perm = reinterpret_cast<const uint32_t*>(base + off_inv[l]);
+    launch_gather(idx2, n, t->xyz, t->feat, t->F, t->sem, t->K, t->geo, const_cast<float4*>(lv.pos),
+                  const_cast<float*>(lv.feat), const_cast<float*>(lv.sem), const_cast<float*>(lv.geo),
+                  const_cast<uint32_t*>(lv.inv_perm), st);
+  }
+  CVO_HIP(ctx, hipGetLastError());
+  CVO_TRY(ensure(ctx, ctx->b_gridhdr, sizeof(GridDev)));
+  gh.dev_ptr = as<GridDev>(ctx->b_gridhdr);
+  CVO_HIP(ctx, hipMemcpyAsync(gh.dev_ptr, &G, sizeof(GridDev), hipMemcpyHostToDevice, st));
+  return CVO_OK;
+}
+
+// Cell sizes for lengthscales from ell_hi down to ell_lo, ratio 0.7.
+static std::vector<float> level_sizes(const KConsts& kc, float ell_hi, float ell_lo, float max_norm, float tq) {
+  std::vector<float> hs;
+  // without the geometric gate every row takes the full scan; one coarse level
+  if (!kc.use_geo) return {1.0e4f};
+  float r_hi = radius_for(kc, ell_hi, max_norm, tq);
+  float r_lo = radius_for(kc, ell_lo, max_norm, tq);
+  if (!(r_hi > 0)) r_hi = 1.0f;
+  float h = r_hi * 1.02f;
+  for (int l = 0; l < kMaxLevels; ++l) {
+    hs.push_back(h);
+    if (h * 0.7f < r_lo) break;
+    h *= 0.7f;
+  }
+  return hs;
+}
+
+struct Problems {
+  int n_probs = 0;
+  int total_blocks = 0;
+  AssocProblem* dev = nullptr;
+  std::vector<AssocProblem> host;
+  ProblemState* states = nullptr;
+  cvo_iter_record* trace = nullptr;
+  int trace_cap = 0;
+};
+
+// Buffers and descriptors of n_probs problems sharing one source and target.
+static cvo_status setup_problems(cvo_ctx* ctx, const cvo_dcloud* s, const cvo_dcloud* t, const GridHost& gh,
+                                 int n_probs, int cap_alloc, int trace_cap, Problems& pr) {
+  const int n = s->n;
+  const int nb = (n + kRowsPerBlock - 1) / kRowsPerBlock;
+  pr.n_probs = n_probs;
+  pr.total_blocks = nb * n_probs;
+  CVO_TRY(ensure(ctx, ctx->b_pairs, sizeof(uint2) * (size_t)n * cap_alloc * n_probs));
+  CVO_TRY(ensure(ctx, ctx->b_counts, sizeof(uint32_t) * (size_t)n * n_probs));
+  CVO_TRY(ensure(ctx, ctx->b_ovf, sizeof(uint32_t) * (size_t)n * n_probs));
+  CVO_TRY(ensure(ctx, ctx->b_state, sizeof(ProblemState) * n_probs));
+  CVO_TRY(ensure(ctx, ctx->b_probs, sizeof(AssocProblem) * n_probs));
+  CVO_TRY(ensure(ctx, ctx->b_part_a, sizeof(double) * 8 * (size_t)nb * n_probs));
+  CVO_TRY(ensure(ctx, ctx->b_part_ovf, sizeof(double) * 8 * (size_t)kOvfBlocks * n_probs));
+  CVO_TRY(ensure(ctx, ctx->b_part_s, sizeof(double) * 4 * (size_t)nb * n_probs));
+  CVO_TRY(ensure(ctx, ctx->b_part_cnt, sizeof(uint32_t) * 2 * (size_t)nb * n_probs));
+  pr.trace_cap = trace_cap;
+  if (trace_cap > 0) {
+    CVO_TRY(ensure(ctx, ctx->b_trace, sizeof(cvo_iter_record) * (size_t)trace_cap * n_probs));
+    pr.trace = as<cvo_iter_record>(ctx->b_trace);
+  } else {
+    pr.trace = nullptr;
+  }
+  // ovf stamps start at 0 (iteration stamps are iter+1 >= 1)
+  CVO_HIP(ctx, hipMemsetAsync(ctx->b_ovf.p, 0, sizeof(uint32_t) * (size_t)n * n_probs, ctx->stream));
+  CVO_HIP(ctx, hipMemsetAsync(ctx->b_state.p, 0, sizeof(ProblemState) * n_probs, ctx->stream));
+  pr.states = as<ProblemState>(ctx->b_state);
+  pr.host.resize(n_probs);
+  for (int q = 0; q < n_probs; ++q) {
+    AssocProblem& P = pr.host[q];
+    std::memset(&P, 0, sizeof(P));
+    P.src_pos = as<float4>(ctx->b_src_pos);
+    P.src_feat = s->feat ? as<float>(ctx->b_src_feat) : nullptr;
+    P.src_sem = s->sem ? as<float>(ctx->b_src_sem) : nullptr;
+    P.src_geo = s->geo ? as<float>(ctx->b_src_geo) : nullptr;
+    P.n_src = n;
+    P.n_tgt = t->n;
+    P.grid = gh.dev_ptr;
+    P.tgt_xyz = t->xyz;
+    P.tgt_feat = t->feat;
+    P.tgt_sem = t->sem;
+    P.tgt_geo = t->geo;
+    P.st = pr.states + q;
+    P.pairs = as<uint2>(ctx->b_pairs) + (size_t)q * n * cap_alloc;
+    P.counts = as<uint32_t>(ctx->b_counts) + (size_t)q * n;
+    P.ovf_stamp = as<uint32_t>(ctx->b_ovf) + (size_t)q * n;
+    P.cap_alloc = cap_alloc;
+    P.block_begin = q * nb;
+    P.n_blocks = nb;
+    P.trace_cap = trace_cap;
+    P.part_a = as<double>(ctx->b_part_a) + (size_t)q * nb * 8;
+    P.part_ovf = as<double>(ctx->b_part_ovf) + (size_t)q * kOvfBlocks * 8;
+    P.part_s = as<double>(ctx->b_part_s) + (size_t)q * nb * 4;
+    P.part_cnt = as<uint32_t>(ctx->b_part_cnt) + (size_t)q * nb * 2;
+    P.trace = pr.trace ? pr.trace + (size_t)q * trace_cap : nullptr;
+  }
+  pr.dev = as<AssocProblem>(ctx->b_probs);
+  CVO_HIP(ctx, hipMemcpyAsync(pr.dev, pr.host.data(), sizeof(AssocProblem) * n_probs, hipMemcpyHostToDevice,
+                              ctx->stream));
+  return CVO_OK;
+}
+
+static cvo_status pinned(cvo_ctx* ctx, size_t bytes) {
+  if (ctx->pinned_cap >= bytes) return CVO_OK;
+  if (ctx->pinned) CVO_HIP(ctx, hipHostFree(ctx->pinned));
+  ctx->pinned = nullptr;
+  CVO_HIP(ctx, hipHostMalloc(&ctx->pinned, bytes));
+  ctx->pinned_cap = bytes;
+  return CVO_OK;
+}
+
+static cvo_status ensure_events(cvo_ctx* ctx, size_t n) {
+  while (ctx->ev_pool.size() < n) {
+    hipEvent_t e;
+    CVO_HIP(ctx, hipEventCreate(&e));
+    ctx->ev_pool.push_back(e);
+  }
+  return CVO_OK;
+}
+
+// Iteration loop: replays chunks of kChunk iterations until every problem is
+// done.  Profiling mode launches kernel by kernel with events around each.
+static constexpr int kChunk = 24;
+
+static cvo_status run_loop(cvo_ctx* ctx, const Problems& pr, const KConsts& kc, int max_iter, bool with_step,
+                           double* device_seconds) {
+  hipStream_t st = ctx->stream;
+  CVO_TRY(pinned(ctx, 64));
+  volatile int32_t* flags = reinterpret_cast<volatile int32_t*>(ctx->pinned);
+  // The done flags of all problems are gathered with one strided copy of the
+  // states' `done` fields.
+  const size_t pitch = sizeof(ProblemState);
+  CVO_TRY(ensure(ctx, ctx->b_tmp3, sizeof(int32_t) * pr.n_probs + 64));
+  auto read_all_done = [&](bool* all) -> cvo_status {
+    std::vector<int32_t> fl(pr.n_probs);
+    CVO_HIP(ctx, hipMemcpy2DAsync((void*)(ctx->b_tmp3.p), sizeof(int32_t), &pr.states[0].done, pitch, sizeof(int32_t),
+                                  pr.n_probs, hipMemcpyDeviceToDevice, st));
+    CVO_HIP(ctx, hipMemcpyAsync(fl.data(), ctx->b_tmp3.p, sizeof(int32_t) * pr.n_probs, hipMemcpyDeviceToHost, st));
+    CVO_HIP(ctx, hipStreamSynchronize(st));
+    *all = true;
+    for (int q = 0; q < pr.n_probs; ++q) *all = *all && fl[q];
+    return CVO_OK;
+  };
+  CVO_TRY(ensure_events(ctx, 8));
+  hipEvent_t t0 = ctx->ev_pool[0], t1 = ctx->ev_pool[1];
+  hipEvent_t done_ev[2] = {ctx->ev_pool[2], ctx->ev_pool[3]};
+  CVO_HIP(ctx, hipEventRecord(t0, st));
+
+  if (ctx->profiling) {
+    // kernel by kernel, events around each launch
+    CVO_TRY(ensure_events(ctx, 8 + 6));
+    hipEvent_t e[6];
+    for (int i = 0; i < 6; ++i) e[i] = ctx->ev_pool[8 + i];
+    for (int it = 0; it < max_iter; ++it) {
+      CVO_HIP(ctx, hipEventRecord(e[0], st));
+      launch_assoc(pr.dev, pr.n_probs, pr.total_blocks, kc, st);
+      CVO_HIP(ctx, hipEventRecord(e[1], st));
+      launch_overflow(pr.dev, pr.n_probs, kc, st);
+      CVO_HIP(ctx, hipEventRecord(e[2], st));
+      if (with_step) launch_step(pr.dev, pr.n_probs, pr.total_blocks, kc, st);
+      CVO_HIP(ctx, hipEventRecord(e[3], st));
+      bool all = true;
+      CVO_TRY(read_all_done(&all));
+      float ms[3];
+      CVO_HIP(ctx, hipEventElapsedTime(&ms[0], e[0], e[1]));
+      CVO_HIP(ctx, hipEventElapsedTime(&ms[1], e[1], e[2]));
+      CVO_HIP(ctx, hipEventElapsedTime(&ms[2], e[2], e[3]));
+      ctx->stats.ms[CVO_KSTAT_ASSOC] += ms[0];
+      ctx->stats.ms[CVO_KSTAT_OVERFLOW] += ms[1];
+      ctx->stats.ms[CVO_KSTAT_STEP] += with_step ? ms[2] : 0.0;
+      ctx->stats.launches[CVO_KSTAT_ASSOC] += 1;
+      ctx->stats.launches[CVO_KSTAT_OVERFLOW] += 1;
+      if (with_step) ctx->stats.launches[CVO_KSTAT_STEP] += 1;
+      if (!with_step || all) break;
+    }
+  } else if (!with_step) {
+    launch_assoc(pr.dev, pr.n_probs, pr.total_blocks, kc, st);
+    launch_overflow(pr.dev, pr.n_probs, kc, st);
+    CVO_HIP(ctx, hipGetLastError());
+  } else {
+    // capture one chunk of iterations into a graph
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    CVO_HIP(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    for (int it = 0; it < kChunk; ++it) {
+      launch_assoc(pr.dev, pr.n_probs, pr.total_blocks, kc, st);
+      launch_overflow(pr.dev, pr.n_probs, kc, st);
+      launch_step(pr.dev, pr.n_probs, pr.total_blocks, kc, st);
+    }
+    hipError_t ce = hipStreamEndCapture(st, &graph);
+    if (ce != hipSuccess) return hip_check(ctx, ce, "hipStreamEndCapture");
+    hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (ie != hipSuccess) {
+      (void)hipGraphDestroy(graph);
+      return hip_check(ctx, ie, "hipGraphInstantiate");
+    }
+    const int n_chunks = (max_iter + kChunk - 1) / kChunk;
+    bool finished = false;
+    cvo_status status = CVO_OK;
+    int32_t* dflag = as<int32_t>(ctx->b_tmp3);
+    // Chunk c writes its all-done flag into pinned slot c % 2; the host waits
+    // for chunk c-1 while chunk c runs.
+    for (int c = 0; c < n_chunks && !finished; ++c) {
+      hipError_t le = hipGraphLaunch(exec, st);
+      if (le != hipSuccess) { status = hip_check(ctx, le, "hipGraphLaunch"); break; }
+      launch_all_done(pr.dev, pr.n_probs, dflag + (c & 1), st);
+      hipError_t me = hipMemcpyAsync((void*)(flags + (c & 1)), dflag + (c & 1), sizeof(int32_t),
+                                     hipMemcpyDeviceToHost, st);
+      if (me != hipSuccess) { status = hip_check(ctx, me, "flag copy"); break; }
+      me = hipEventRecord(done_ev[c & 1], st);
+      if (me != hipSuccess) { status = hip_check(ctx, me, "hipEventRecord"); break; }
+      if (c >= 1) {
+        me = hipEventSynchronize(done_ev[(c - 1) & 1]);
+        if (me != hipSuccess) { status = hip_check(ctx, me, "hipEventSynchronize"); break; }
+        if (flags[(c - 1) & 1]) finished = true;
+      }
+    }
+    hipError_t se = hipStreamSynchronize(st);
+    (void)hipGraphExecDestroy(exec);
+    (void)hipGraphDestroy(graph);
+    if (status != CVO_OK) return status;
+    if (se != hipSuccess) return hip_check(ctx, se, "hipStreamSynchronize");
+  }
+  CVO_HIP(ctx, hipEventRecord(t1, st));
+  CVO_HIP(ctx, hipEventSynchronize(t1));
+  float ms = 0;
+  CVO_HIP(ctx, hipEventElapsedTime(&ms, t0, t1));
+  if (device_seconds) *device_seconds = ms / 1000.0;
+  CVO_HIP(ctx, hipGetLastError());
+  return CVO_OK;
+}
+
+static void mat_inverse_rigid(const float T[16], float out[16]) {
+  // out = [R^T, -R^T t]
+  for (int r = 0; r < 3; ++r) {
+    for (int c = 0; c < 3; ++c) out[r * 4 + c] = T[c * 4 + r];
+    out[r * 4 + 3] = -(T[0 * 4 + r] * T[3] + T[1 * 4 + r] * T[7] + T[2 * 4 + r] * T[11]);
+  }
+  out[12] = out[13] = out[14] = 0.0f;
+  out[15] = 1.0f;
+}
+
+static cvo_status export_assoc(cvo_ctx* ctx, const cvo_dcloud* s, const cvo_dcloud* t, const Problems& pr,
+                               cvo_assoc* out) {
+  hipStream_t st = ctx->stream;
+  const int n = s->n;
+  CVO_TRY(ensure(ctx, ctx->b_export, sizeof(int64_t) * (size_t)(n + 1) * 2));
+  int64_t* cnt = as<int64_t>(ctx->b_export);
+  int64_t* rp = cnt + (n + 1);
+  CVO_HIP(ctx, hipMemsetAsync(cnt, 0, sizeof(int64_t) * (n + 1), st));
+  launch_export(pr.dev, n, cnt, nullptr, nullptr, nullptr, 0, st);
+  size_t tmp = 0;
+  CVO_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, cnt, rp, n + 1, st));
+  CVO_TRY(ensure(ctx, ctx->b_cub, tmp));
+  CVO_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(ctx->b_cub.p, tmp, cnt, rp, n + 1, st));
+  int64_t nnz = 0;
+  CVO_HIP(ctx, hipMemcpyAsync(&nnz, rp + n, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  CVO_HIP(ctx, hipStreamSynchronize(st));
+  CVO_TRY(ensure(ctx, ctx->b_tmp2, (sizeof(int32_t) + sizeof(float)) * (size_t)std::max<int64_t>(nnz, 1)));
+  int32_t* dcol = as<int32_t>(ctx->b_tmp2);
+  float* dval = reinterpret_cast<float*>(dcol + std::max<int64_t>(nnz, 1));
+  launch_export(pr.dev, n, nullptr, rp, dcol, dval, 1, st);
+  CVO_HIP(ctx, hipGetLastError());
+  out->rows = n;
+  out->cols = t->n;
+  out->nnz = nnz;
+  out->row_ptr = static_cast<int64_t*>(std::malloc(sizeof(int64_t) * (n + 1)));
+  out->col = static_cast<int32_t*>(std::malloc(sizeof(int32_t) * std::max<int64_t>(nnz, 1)));
+  out->val = static_cast<float*>(std::malloc(sizeof(float) * std::max<int64_t>(nnz, 1)));
+  if (!out->row_ptr || !out->col || !out->val) return set_err(ctx, CVO_ERR_OOM, "host allocation of association");
+  CVO_HIP(ctx, hipMemcpyAsync(out->row_ptr, rp, sizeof(int64_t) * (n + 1), hipMemcpyDeviceToHost, st));
+  if (nnz > 0) {
+    CVO_HIP(ctx, hipMemcpyAsync(out->col, dcol, sizeof(int32_t) * nnz, hipMemcpyDeviceToHost, st));
+    CVO_HIP(ctx, hipMemcpyAsync(out->val, dval, sizeof(float) * nnz, hipMemcpyDeviceToHost, st));
+  }
+  CVO_HIP(ctx, hipStreamSynchronize(st));
+  return CVO_OK;
+}
+
+}  // namespace cvo
+
+using namespace cvo;
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+
+extern "C" const char* cvo_version(void) { return "rkhs_ba_amd 0.1 (gfx950)"; }
+
+extern "C" const char* cvo_status_string(cvo_status s) {
+  switch (s) {
+    case CVO_OK: return "ok";
+    case CVO_ERR_INVALID_ARG: return "invalid argument";
+    case CVO_ERR_EMPTY: return "empty point cloud";
+    case CVO_ERR_IO: return "file could not be opened";
+    case CVO_ERR_PARSE: return "YAML value could not be parsed";
+    case CVO_ERR_HIP: return "HIP runtime error";
+    case CVO_ERR_NO_DEVICE: return "no HIP device";
+    case CVO_ERR_OOM: return "out of device memory";
+    case CVO_ERR_UNSUPPORTED: return "unsupported option";
+    case CVO_ERR_LIMIT: return "size beyond a documented limit";
+  }
+  return "unknown status";
+}
+
+extern "C" const char* cvo_last_error(const cvo_ctx* ctx) { return ctx ? ctx->last_error.c_str() : ""; }
+
+extern "C" cvo_status cvo_create_from_params(const cvo_params* p, cvo_ctx** out) {
+  if (!p || !out) return CVO_ERR_INVALID_ARG;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return CVO_ERR_NO_DEVICE;
+  cvo_ctx* ctx = new cvo_ctx();
+  ctx->params = *p;
+  if (hipGetDevice(&ctx->device) != hipSuccess) {
+    delete ctx;
+    return CVO_ERR_HIP;
+  }
+  if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete ctx;
+    return CVO_ERR_HIP;
+  }
+  *out = ctx;
+  return CVO_OK;
+}
+
+extern "C" cvo_status cvo_create(const char* yaml_path, cvo_ctx** out) {
+  if (!out) return CVO_ERR_INVALID_ARG;
+  cvo_params p;
+  cvo_params_default(&p);
+  if (yaml_path) {
+    cvo_status s = cvo_params_read_yaml(yaml_path, &p);
+    if (s != CVO_OK) return s;
+  }
+  return cvo_create_from_params(&p, out);
+}
+
+extern "C" cvo_status cvo_get_params(const cvo_ctx* ctx, cvo_params* out) {
+  if (!ctx || !out) return CVO_ERR_INVALID_ARG;
+  *out = ctx->params;
+  return CVO_OK;
+}
+
+extern "C" cvo_status cvo_set_params(cvo_ctx* ctx, const cvo_params* p) {
+  if (!ctx || !p) return CVO_ERR_INVALID_ARG;
+  ctx->params = *p;
+  return CVO_OK;
+}
+
+static void free_buf(DevBuf& b) {
+  if (b.p) (void)hipFree(b.p);
+  b.p = nullptr;
+  b.cap = 0;
+}
+
+extern "C" void cvo_destroy(cvo_ctx* ctx) {
+  if (!ctx) return;
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  DevBuf* bufs[] = {&ctx->b_src_pos, &ctx->b_src_feat, &ctx->b_src_sem, &ctx->b_src_geo, &ctx->b_src_key,
+                    &ctx->b_src_perm, &ctx->b_grid_store, &ctx->b_pairs, &ctx->b_counts, &ctx->b_ovf,
+                    &ctx->b_state, &ctx->b_probs, &ctx->b_gridhdr, &ctx->b_part_a, &ctx->b_part_ovf,
+                    &ctx->b_part_s, &ctx->b_part_cnt, &ctx->b_trace, &ctx->b_tmp0, &ctx->b_tmp1,
+                    &ctx->b_tmp2, &ctx->b_tmp3, &ctx->b_cub, &ctx->b_export};
+  for (DevBuf* b : bufs) free_buf(*b);
+  for (auto& b : ctx->b_ba) free_buf(b);
+  for (auto& b : ctx->b_mlp) free_buf(b);
+  for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
+  if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+extern "C" cvo_status cvo_cloud_upload(cvo_ctx* ctx, const cvo_cloud_view* v, cvo_dcloud** out) {
+  if (!ctx || !v || !out) return CVO_ERR_INVALID_ARG;
+  *out = nullptr;
+  if (v->n < 0 || (v->n > 0 && !v->xyz)) return set_err(ctx, CVO_ERR_INVALID_ARG, "bad cloud view");
+  if ((v->feat && v->feat_dim <= 0) || (v->sem && v->num_classes <= 0) || v->feat_dim > 64 || v->num_classes > 64)
+    return set_err(ctx, CVO_ERR_INVALID_ARG, "feature/class dimension must be in [1, 64]");
+  cvo_dcloud* c = new cvo_dcloud();
+  c->n = v->n;
+  c->F = v->feat ? v->feat_dim : 0;
+  c->K = v->sem ? v->num_classes : 0;
+  c->has_geo = v->geo_type != nullptr;
+  for (int d = 0; d < 3; ++d) {
+    c->bmin[d] = 0.0f;
+    c->bmax[d] = 0.0f;
+  }
+  if (c->n > 0) {
+    for (int d = 0; d < 3; ++d) c->bmin[d] = c->bmax[d] = v->xyz[d];
+    double mx = 0;
+    for (int j = 0; j < c->n; ++j) {
+      for (int d = 0; d < 3; ++d) {
+        float x = v->xyz[3 * (size_t)j + d];
+        if (!std::isfinite(x)) {
+          delete c;
+          return set_err(ctx, CVO_ERR_INVALID_ARG, "non-finite coordinate in cloud");
+        }
+        c->bmin[d] = std::min(c->bmin[d], x);
+        c->bmax[d] = std::max(c->bmax[d], x);
+      }
+      const float* x = v->xyz + 3 * (size_t)j;
+      mx = std::max(mx, std::sqrt((double)x[0] * x[0] + (double)x[1] * x[1] + (double)x[2] * x[2]));
+    }
+    c->max_norm = (float)mx;
+    auto up = [&](float** dst, const float* src, size_t count) -> cvo_status {
+      if (!src) return CVO_OK;
+      CVO_HIP(ctx, hipMalloc(dst, sizeof(float) * count));
+      CVO_HIP(ctx, hipMemcpyAsync(*dst, src, sizeof(float) * count, hipMemcpyHostToDevice, ctx->stream));
+      return CVO_OK;
+    };
+    cvo_status s = up(&c->xyz, v->xyz, 3 * (size_t)c->n);
+    if (s == CVO_OK) s = up(&c->feat, v->feat, (size_t)c->n * c->F);
+    if (s == CVO_OK) s = up(&c->sem, v->sem, (size_t)c->n * c->K);
+    if (s == CVO_OK) s = up(&c->geo, v->geo_type, (size_t)c->n * 2);
+    if (s == CVO_OK && hipStreamSynchronize(ctx->stream) != hipSuccess) s = set_err(ctx, CVO_ERR_HIP, "upload sync");
+    if (s != CVO_OK) {
+      cvo_cloud_free(ctx, c);
+      return s;
+    }
+  }
+  *out = c;
+  return CVO_OK;
+}
+
+extern "C" void cvo_cloud_free(cvo_ctx* ctx, cvo_dcloud* c) {
+  if (!c) return;
+  if (ctx && ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  if (c->xyz) (void)hipFree(c->xyz);
+  if (c->feat) (void)hipFree(c->feat);
+  if (c->sem) (void)hipFree(c->sem);
+  if (c->geo) (void)hipFree(c->geo);
+  delete c;
+}
+
+extern "C" int32_t cvo_cloud_size(const cvo_dcloud* c) { return c ? c->n : 0; }
+
+extern "C" cvo_status cvo_set_trace(cvo_ctx* ctx, int32_t cap) {
+  if (!ctx || cap < 0) return CVO_ERR_INVALID_ARG;
+  ctx->trace_cap = cap;
+  ctx->trace_n = 0;
+  return CVO_OK;
+}
+
+extern "C" cvo_status cvo_get_trace(cvo_ctx* ctx, cvo_iter_record* out, int32_t cap, int32_t* n_out) {
+  if (!ctx || (!out && cap > 0)) return CVO_ERR_INVALID_ARG;
+  int n = std::min<int>(cap, ctx->trace_n);
+  for (int i = 0; i < n; ++i) out[i] = ctx->trace[i];
+  if (n_out) *n_out = ctx->trace_n;
+  return CVO_OK;
+}
+
+extern "C" cvo_status cvo_set_profiling(cvo_ctx* ctx, int32_t on) {
+  if (!ctx) return CVO_ERR_INVALID_ARG;
+  ctx->profiling = on != 0;
+  return CVO_OK;
+}
+
+extern "C" cvo_status cvo_get_kernel_stats(cvo_ctx* ctx, cvo_kernel_stats* out) {
+  if (!ctx || !out) return CVO_ERR_INVALID_ARG;
+  *out = ctx->stats;
+  return CVO_OK;
+}
+
+extern "C" cvo_status cvo_reset_kernel_stats(cvo_ctx* ctx) {
+  if (!ctx) return CVO_ERR_INVALID_ARG;
+  std::memset(&ctx->stats, 0, sizeof(ctx->stats));
+  return CVO_OK;
+}
+
+// Algorithmic bytes of one association launch and one step launch.
+static void account_bytes(cvo_ctx* ctx, const cvo_dcloud* s, const cvo_dcloud* t, const KConsts& kc,
+                          const std::vector<cvo_iter_record>& tr, int n_it, bool with_step) {
+  const double per_pt = 12.0 + (kc.use_col ? 4.0 * kc.F : 0.0) + (kc.use_sem ? 4.0 * kc.Ks : 0.0);
+  for (int i = 0; i < n_it && i < (int)tr.size(); ++i) {
+    const double S = tr[i].nnz;
+    ctx->stats.bytes[CVO_KSTAT_ASSOC] += per_pt * (s->n + t->n) + 8.0 * S + 4.0 * s->n;
+    if (with_step) ctx->stats.bytes[CVO_KSTAT_STEP] += 8.0 * S + 4.0 * s->n + 12.0 * (s->n + t->n);
+    ctx->stats.pairs_stored += S;
+  }
+}
+
+extern "C" cvo_status cvo_align_d(cvo_ctx* ctx, const cvo_dcloud* src, const cvo_dcloud* tgt, const float T_init[16],
+                                  float T_out[16], cvo_assoc* assoc_out, double* seconds, int32_t* iters,
+                                  int32_t* ret_code) {
+  if (!ctx || !T_init || !T_out) return CVO_ERR_INVALID_ARG;
+  if (ret_code) *ret_code = 0;
+  if (iters) *iters = 0;
+  if (seconds) *seconds = 0.0;
+  if (!src || !tgt || src->n == 0 || tgt->n == 0) {
+    // the int overload returns 0 on empty input and leaves transform untouched
+    for (int i = 0; i < 16; ++i) T_out[i] = (i % 5 == 0) ? 1.0f : 0.0f;
+    return set_err(ctx, CVO_ERR_EMPTY, "empty point cloud");
+  }
+  CVO_TRY(validate_pair(ctx, src, tgt));
+  const cvo_params& p = ctx->params;
+  KConsts kc = make_kconsts(p, src->F, src->K, MODE_ALIGN);
+  kc.n_src_f = (float)src->n;
+  kc.n_tgt_f = (float)tgt->n;
+  kc.max_src_norm = src->max_norm;
+  CVO_TRY(prepare_source(ctx, src));
+  float tq = std::max(std::fabs(T_init[3]), std::max(std::fabs(T_init[7]), std::fabs(T_init[11])));
+  GridHost gh;
+  float ell_lo = std::min(p.ell_min, p.ell_init);
+  CVO_TRY(build_grid(ctx, tgt, level_sizes(kc, std::max(p.ell_init, p.ell_min), ell_lo, src->max_norm, tq), gh));
+  const bool want_trace = ctx->trace_cap > 0 || ctx->profiling;
+  const int trace_cap = want_trace ? std::max(ctx->trace_cap, ctx->profiling ? p.MAX_ITER : 0) : 0;
+  Problems pr;
+  CVO_TRY(setup_problems(ctx, src, tgt, gh, 1, p.nearest_neighbors_max, trace_cap, pr));
+  CVO_TRY(ensure(ctx, ctx->b_tmp2, 64 * sizeof(float)));
+  CVO_HIP(ctx, hipMemcpyAsync(ctx->b_tmp2.p, T_init, sizeof(float) * 16, hipMemcpyHostToDevice, ctx->stream));
+  launch_init_state(pr.dev, 1, as<float>(ctx->b_tmp2), 16, kc, p.ell_init, p.nearest_neighbors_max, ctx->stream);
+  CVO_HIP(ctx, hipGetLastError());
+  double dev_s = 0.0;
+  if (p.MAX_ITER > 0) CVO_TRY(run_loop(ctx, pr, kc, p.MAX_ITER, true, &dev_s));
+  ProblemState hs;
+  CVO_HIP(ctx, hipMemcpyAsync(&hs, pr.states, sizeof(ProblemState), hipMemcpyDeviceToHost, ctx->stream));
+  CVO_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  // T_out = [R^T, -R^T T] from the FP64 pose
+  for (int r = 0; r < 3; ++r) {
+    for (int c = 0; c < 3; ++c) T_out[r * 4 + c] = (float)hs.R[c * 3 + r];
+    T_out[r * 4 + 3] = (float)(-(hs.R[0 * 3 + r] * hs.T[0] + hs.R[1 * 3 + r] * hs.T[1] + hs.R[2 * 3 + r] * hs.T[2]));
+  }
+  T_out[12] = T_out[13] = T_out[14] = 0.0f;
+  T_out[15] = 1.0f;
+  int n_it = p.MAX_ITER <= 0 ? 0 : (hs.done ? hs.iters_out : p.MAX_ITER);
+  if (iters) *iters = n_it;
+  if (ret_code) *ret_code = hs.ret;
+  if (seconds) *seconds = dev_s;
+  if (want_trace) {
+    int n_rec = std::min(trace_cap, n_it + 1);
+    ctx->trace.resize(n_rec);
+    if (n_rec > 0)
+      CVO_HIP(ctx, hipMemcpy(ctx->trace.data(), pr.trace, sizeof(cvo_iter_record) * n_rec, hipMemcpyDeviceToHost));
+    ctx->trace_n = n_rec;
+    if (ctx->profiling) account_bytes(ctx, src, tgt, kc, ctx->trace, n_rec, true);
+  }
+  if (assoc_out && p.is_exporting_association) CVO_TRY(export_assoc(ctx, src, tgt, pr, assoc_out));
+  return CVO_OK;
+}
+
+// RAII holder for uploaded clouds of a host-view call.
+struct Uploaded {
+  cvo_ctx* ctx;
+  cvo_dcloud* c = nullptr;
+  ~Uploaded() {
+    if (c) cvo_cloud_free(ctx, c);
+  }
+};
+
+extern "C" cvo_status cvo_align(cvo_ctx* ctx, const cvo_cloud_view* src, const cvo_cloud_view* tgt,
+                                const float T_init[16], float T_out[16], cvo_assoc* assoc_out, double* seconds,
+                                int32_t* iters, int32_t* ret_code) {
+  if (!ctx || !src || !tgt || !T_init || !T_out) return CVO_ERR_INVALID_ARG;
+  if (src->n == 0 || tgt->n == 0) return cvo_align_d(ctx, nullptr, nullptr, T_init, T_out, assoc_out, seconds, iters, ret_code);
+  Uploaded a{ctx}, b{ctx};
+  CVO_TRY(cvo_cloud_upload(ctx, src, &a.c));
+  CVO_TRY(cvo_cloud_upload(ctx, tgt, &b.c));
+  return cvo_align_d(ctx, a.c, b.c, T_init, T_out, assoc_out, seconds, iters, ret_code);
+}
+
+// Single association pass at pose T and lengthscale ell; the problem stays in
+// the context's buffers for an export.
+static cvo_status single_pass(cvo_ctx* ctx, const cvo_dcloud* src, const cvo_dcloud* tgt, const float T[16], float ell,
+                              Problems& pr, KConsts& kc, double* ip) {
+  CVO_TRY(validate_pair(ctx, src, tgt));
+  const cvo_params& p = ctx->params;
+  kc = make_kconsts(p, src->F, src->K, MODE_IP);
+  kc.n_src_f = (float)src->n;
+  kc.n_tgt_f = (float)tgt->n;
+  kc.max_src_norm = src->max_norm;
+  kc.max_iter = 1;
+  CVO_TRY(prepare_source(ctx, src));
+  float tq = std::max(std::fabs(T[3]), std::max(std::fabs(T[7]), std::fabs(T[11])));
+  GridHost gh;
+  std::vector<float> hs = level_sizes(kc, ell, ell, src->max_norm, tq);
+  CVO_TRY(build_grid(ctx, tgt, hs, gh));
+  CVO_TRY(setup_problems(ctx, src, tgt, gh, 1, p.nearest_neighbors_max, 0, pr));
+  CVO_TRY(ensure(ctx, ctx->b_tmp2, 64 * sizeof(float)));
+  CVO_HIP(ctx, hipMemcpyAsync(ctx->b_tmp2.p, T, sizeof(float) * 16, hipMemcpyHostToDevice, ctx->stream));
+  launch_init_state(pr.dev, 1, as<float>(ctx->b_tmp2), 16, kc, ell, p.nearest_neighbors_max, ctx->stream);
+  CVO_TRY(run_loop(ctx, pr, kc, 1, false, nullptr));
+  ProblemState hs2;
+  CVO_HIP(ctx, hipMemcpyAsync(&hs2, pr.states, sizeof(ProblemState), hipMemcpyDeviceToHost, ctx->stream));
+  CVO_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (ip) *ip = hs2.ip;
+  return CVO_OK;
+}
+
+extern "C" cvo_status cvo_inner_product_d(cvo_ctx* ctx, const cvo_dcloud* src, const cvo_dcloud* tgt, const float T[16],
+                                          float ell, double* ip) {
+  if (!ctx || !T || !ip) return CVO_ERR_INVALID_ARG;
+  *ip = 0.0;
+  if (!src || !tgt || src->n == 0 || tgt->n == 0) return set_err(ctx, CVO_ERR_EMPTY, "empty point cloud");
+  Problems pr;
+  KConsts kc;
+  return single_pass(ctx, src, tgt, T, ell, pr, kc, ip);
+}
+
+extern "C" cvo_status cvo_inner_product(cvo_ctx* ctx, const cvo_cloud_view* src, const cvo_cloud_view* tgt,
+                                        const float T[16], float ell, double* ip) {
+  if (!ctx || !src || !tgt || !T || !ip) return CVO_ERR_INVALID_ARG;
+  *ip = 0.0;
+  if (src->n == 0 || tgt->n == 0) return set_err(ctx, CVO_ERR_EMPTY, "empty point cloud");
+  Uploaded a{ctx}, b{ctx};
+  CVO_TRY(cvo_cloud_upload(ctx, src, &a.c));
+  CVO_TRY(cvo_cloud_upload(ctx, tgt, &b.c));
+  return cvo_inner_product_d(ctx, a.c, b.c, T, ell, ip);
+}
+
+extern "C" cvo_status cvo_function_angle(cvo_ctx* ctx, const cvo_cloud_view* src, const cvo_cloud_view* tgt,
+                                         const float T[16], float ell, int32_t approx, double* cos_out) {
+  if (!ctx || !src || !tgt || !T || !cos_out) return CVO_ERR_INVALID_ARG;
+  *cos_out = 0.0;
+  // function_angle returns 0 on empty input (CvoGPU.cu:1775-1777)
+  if (src->n == 0 || tgt->n == 0) return CVO_OK;
+  Uploaded a{ctx}, b{ctx};
+  CVO_TRY(cvo_cloud_upload(ctx, src, &a.c));
+  CVO_TRY(cvo_cloud_upload(ctx, tgt, &b.c));
+  double fxfz = 0.0;
+  CVO_TRY(cvo_inner_product_d(ctx, a.c, b.c, T, ell, &fxfz));
+  double fx, fz;
+  if (approx) {
+    fx = std::sqrt((double)src->n);
+    fz = std::sqrt((double)tgt->n);
+  } else {
+    const float I[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+    double xx = 0, zz = 0;
+    CVO_TRY(cvo_inner_product_d(ctx, a.c, a.c, I, ell, &xx));
+    CVO_TRY(cvo_inner_product_d(ctx, b.c, b.c, I, ell, &zz));
+    fx = std::sqrt(xx);
+    fz = std::sqrt(zz);
+  }
+  *cos_out = fxfz / (fx * fz);
+  return CVO_OK;
+}
+
+extern "C" cvo_status cvo_compute_association(cvo_ctx* ctx, const cvo_cloud_view* src, const cvo_cloud_view* tgt,
+                                              const float T[16], float ell, cvo_assoc* out) {
+  if (!ctx || !src || !tgt || !T || !out) return CVO_ERR_INVALID_ARG;
+  std::memset(out, 0, sizeof(*out));
+  if (src->n == 0 || tgt->n == 0) return CVO_OK;  // the reference returns without output
+  Uploaded a{ctx}, b{ctx};
+  CVO_TRY(cvo_cloud_upload(ctx, src, &a.c));
+  CVO_TRY(cvo_cloud_upload(ctx, tgt, &b.c));
+  Problems pr;
+  KConsts kc;
+  double ip = 0.0;
+  CVO_TRY(single_pass(ctx, a.c, b.c, T, ell, pr, kc, &ip));
+  return export_assoc(ctx, a.c, b.c, pr, out);
+}
+
+extern "C" void cvo_assoc_free(cvo_assoc* a) {
+  if (!a) return;
+  std::free(a->row_ptr);
+  std::free(a->col);
+  std::free(a->val);
+  a->row_ptr = nullptr;
+  a->col = nullptr;
+  a->val = nullptr;
+  a->nnz = 0;
+}

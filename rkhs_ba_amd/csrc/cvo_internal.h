@@ -1,0 +1,203 @@
+// Internal device/host data structures of the gfx950 CVO library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "cvo_c.h"
+
+namespace cvo {
+
+constexpr int kMaxLevels = 12;         // grid levels per target
+constexpr int kMaxWindow = 64;         // indicator window capacity
+constexpr int kRowsPerBlock = 16;      // assoc / step: rows per 256-thread block
+constexpr int kGroup = 16;             // lanes per row
+constexpr int kBlock = 256;
+constexpr int kOvfBlocks = 96;         // capped-row kernel blocks per problem
+constexpr uint32_t kMaxCells = 1u << 25;
+
+// One grid level over a target cloud (uniform dense grid, x-fastest cells).
+struct GridLevel {
+  float h, inv_h;
+  float ox, oy, oz;
+  int32_t nx, ny, nz;
+  const uint32_t* cell_start;  // nx*ny*nz + 1
+  const float4* pos;           // sorted: x, y, z, original id (bits)
+  const float* feat;           // sorted, n*F
+  const float* sem;            // sorted, n*K
+  const float* geo;            // sorted, n*2
+  const uint32_t* inv_perm;    // original id -> sorted position
+};
+
+struct GridDev {
+  int32_t n_levels;
+  int32_t n;
+  GridLevel lv[kMaxLevels];
+};
+
+// Per-problem iteration state, resident in device memory.  One per
+// registration (two-view), per start (multi-start) or per edge (BA).
+struct alignas(16) ProblemState {
+  // pose (FP64), R row-major: the reference's (R, T) of align_impl
+  double R[9];
+  double T[3];
+  // per-iteration accumulators (written by the last block of each pass)
+  double flow[6];
+  double ip;
+  double bcde[4];
+  double step;
+  double dist;
+  uint64_t nnz;
+  // float transforms used by the kernels
+  float A[9], b[3];     // target map y' = A y + b
+  float Rq[9], Tq[3];   // query map x~ = Rq x + Tq into the target frame
+  float As[9], bs[3];   // source map (identity for two-view)
+  float ell;
+  float r_need;         // query radius bound for level selection
+  int32_t has_src_xf;
+  int32_t K;            // row cap
+  int32_t level;
+  int32_t iter;
+  int32_t done;
+  int32_t ret;
+  int32_t iters_out;
+  uint32_t max_row;
+  uint32_t n_overflow;
+  uint32_t blocks_done_assoc;
+  uint32_t blocks_done_ovf;
+  uint32_t blocks_done_step;
+  // indicator windows (A_sparsity_indicator_ell_update, float sums)
+  float qs[kMaxWindow], qe[kMaxWindow];
+  int32_t qs_head, qs_size, qe_head, qe_size;
+  float ss, se;
+  float flow_norm;
+  int32_t level_assoc;  // grid level of the last association pass
+};
+
+// Read-only description of one association problem.
+struct AssocProblem {
+  const float4* src_pos;   // sorted source: x, y, z, original row id (bits)
+  const float* src_feat;
+  const float* src_sem;
+  const float* src_geo;
+  int32_t n_src;
+  int32_t n_tgt;
+  const GridDev* grid;
+  const float* tgt_xyz;    // original order (capped-row path)
+  const float* tgt_feat;
+  const float* tgt_sem;
+  const float* tgt_geo;
+  ProblemState* st;
+  uint2* pairs;            // n_src * cap_alloc: (sorted target position, a bits)
+  uint32_t* counts;        // n_src
+  uint32_t* ovf_stamp;     // n_src
+  int32_t cap_alloc;
+  int32_t block_begin;     // first assoc/step block of this problem
+  int32_t n_blocks;        // assoc/step blocks of this problem
+  int32_t trace_cap;
+  double* part_a;          // per block: flow6, ip, -> 8 doubles
+  double* part_ovf;        // per ovf block: 8 doubles
+  double* part_s;          // per block: 4 doubles
+  uint32_t* part_cnt;      // per block: nnz (lo,hi) and max row
+  cvo_iter_record* trace;  // per iteration (may be null)
+};
+
+// Scalar constants of the kernel (fill_in_A_mat_gpu, CvoGPU.cu:966-987).
+struct KConsts {
+  float sigma2, c_sigma2, s_sigma2;
+  float c2, s_ell;
+  float lg;                   // logf(sp / sigma2)
+  float d2_c_thres, d2_s_thres;
+  float q_c, q_s;             // 1 / (2 c2), 1 / (2 s_ell^2)
+  float sp;
+  float amp;                  // sigma2 * c_sigma2 * s_sigma2 (enabled factors)
+  int32_t use_geo, use_col, use_sem, use_gtype;
+  int32_t F, Ks;              // feature and class dimensions
+  float c, d;                 // flow scales
+  // controller constants
+  float eps, eps_2, min_step, max_step;
+  float ell_min, ell_decay_rate;
+  int32_t ell_decay_start, max_iter, nn_max, window;
+  float ind_thr;
+  int32_t range_ell_in_step;
+  float n_src_f, n_tgt_f;     // for the indicator (two-view: N, M)
+  float rq_coef;              // sqrt(-2 lg) * 1.001
+  float max_src_norm;
+  int32_t mode;               // 0 align, 1 inner product, 2 BA
+};
+
+enum Mode { MODE_ALIGN = 0, MODE_IP = 1, MODE_BA = 2 };
+
+}  // namespace cvo
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+};
+
+struct cvo_dcloud {
+  int32_t n = 0, F = 0, K = 0;
+  bool has_geo = false;
+  float* xyz = nullptr;   // device, n*3
+  float* feat = nullptr;  // device, n*F
+  float* sem = nullptr;
+  float* geo = nullptr;
+  float bmin[3], bmax[3];  // bounding box
+  float max_norm = 0;
+  std::vector<float> host_xyz;  // kept for the bounding box / sort keys
+};
+
+struct cvo_ctx {
+  cvo_params params;
+  hipStream_t stream = nullptr;
+  int device = 0;
+  std::string last_error;
+  int32_t trace_cap = 0;
+  std::vector<cvo_iter_record> trace;
+  int32_t trace_n = 0;
+  bool profiling = false;
+  cvo_kernel_stats stats{};
+  // workspace arena, grown on demand
+  DevBuf b_src_pos, b_src_feat, b_src_sem, b_src_geo, b_src_key, b_src_perm;
+  DevBuf b_grid_store;    // all levels' arrays
+  DevBuf b_pairs, b_counts, b_ovf;
+  DevBuf b_state, b_probs, b_gridhdr;
+  DevBuf b_part_a, b_part_ovf, b_part_s, b_part_cnt;
+  DevBuf b_trace;
+  DevBuf b_tmp0, b_tmp1, b_tmp2, b_tmp3, b_cub;
+  DevBuf b_export;
+  void* pinned = nullptr;  // small pinned host buffer for flags
+  size_t pinned_cap = 0;
+  hipEvent_t ev_a = nullptr, ev_b = nullptr;
+  std::vector<hipEvent_t> ev_pool;
+  // BA scratch
+  DevBuf b_ba[16];
+  // model scratch
+  DevBuf b_mlp[16];
+};
+
+// Internal helpers shared by the translation units.
+namespace cvo {
+cvo_status set_err(cvo_ctx* ctx, cvo_status s, const std::string& msg);
+cvo_status hip_check(cvo_ctx* ctx, hipError_t e, const char* what);
+cvo_status ensure(cvo_ctx* ctx, DevBuf& b, size_t bytes);
+}  // namespace cvo
+
+#define CVO_HIP(ctx, call)                                              \
+  do {                                                                  \
+    hipError_t _e = (call);                                             \
+    if (_e != hipSuccess) return cvo::hip_check((ctx), _e, #call);      \
+  } while (0)
+
+#define CVO_TRY(expr)                      \
+  do {                                     \
+    cvo_status _s = (expr);                \
+    if (_s != CVO_OK) return _s;           \
+  } while (0)

@@ -1,0 +1,1211 @@
+// gfx950 kernels of the CVO two-view loop.
+//
+//   grid_cell_count / grid_gather : uniform-grid levels over a target cloud
+//   src_sort_keys / src_gather    : spatially sorted copy of the source rows
+//   assoc_kernel                  : ell-ball search + product kernel + row store
+//                                   (first K by target index) + fused flow
+//                                   (fill_in_A_mat_gpu CvoGPU.cu:951-1099 and
+//                                   compute_flow_gpu_no_eigen :2001-2062)
+//   overflow_kernel               : exact index-ordered scan of rows whose
+//                                   passing pairs exceed the cap K
+//   step_kernel                   : Taylor step terms (:333-462) + device
+//                                   controller (:464-544, :1289-1433)
+//
+// Rows are processed by 16-lane groups (four rows per wave64, sixteen per
+// 256-thread block).  Reductions are block partials summed in fixed order by
+// the last block, so results are bitwise reproducible run to run.
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#include "cvo_internal.h"
+
+namespace cvo {
+
+// ---------------------------------------------------------------------------
+// small helpers
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ void affine3(const float* A, const float* b, float y0, float y1, float y2, float& o0,
+                                        float& o1, float& o2) {
+  o0 = fmaf(A[2], y2, fmaf(A[1], y1, fmaf(A[0], y0, b[0])));
+  o1 = fmaf(A[5], y2, fmaf(A[4], y1, fmaf(A[3], y0, b[1])));
+  o2 = fmaf(A[8], y2, fmaf(A[7], y1, fmaf(A[6], y0, b[2])));
+}
+
+// compute_range_ell (CvoGPU.cu:90-94); distance with the canonical FMA order.
+__device__ __forceinline__ float range_ell(float ell, float x0, float x1, float x2) {
+  float dist = sqrtf(fmaf(x2, x2, fmaf(x1, x1, x0 * x0)));
+  return (float)(((double)dist / 500.0 + 1.0) * (double)ell);
+}
+
+template <typename T>
+__device__ __forceinline__ T shfl16(T v, int src) {
+  return __shfl(v, src, 16);
+}
+
+__device__ __forceinline__ double shfl_xor_d(double v, int m) { return __shfl_xor(v, m, 16); }
+
+// Exact evaluation of a pair's kernel value with FP64 exponentials rounded to
+// float, the reference's rounding points (CvoGPU.cu:1020-1054).  Used only for
+// pairs whose fast FP32 value lies within 2e-5 relative of sp_thres, so the
+// stored set equals the reference's.
+__device__ __noinline__ float exact_value(const KConsts& kc, float d2, float l, float d2c, float d2s, float geo_sim) {
+  float k = 1.0f, ck = 1.0f, sk = 1.0f;
+  if (kc.use_geo) k = (float)((double)kc.sigma2 * exp((double)(-d2) / (2.0 * (double)l * (double)l)));
+  if (kc.use_col) ck = (float)((double)kc.c_sigma2 * exp((double)(-d2c) / (2.0 * (double)kc.c2)));
+  if (kc.use_sem)
+    sk = (float)((double)kc.s_sigma2 * exp((double)(-d2s) / (2.0 * (double)kc.s_ell * (double)kc.s_ell)));
+  return ck * k * sk * geo_sim;
+}
+
+__device__ __forceinline__ float geo_similarity(const float* ga, const float* gb) {
+  float n2a = fmaf(ga[1], ga[1], ga[0] * ga[0]);
+  float n2b = fmaf(gb[1], gb[1], gb[0] * gb[0]);
+  float dab = fmaf(ga[1], gb[1], ga[0] * gb[0]);
+  return dab * dab / (n2a * n2b);
+}
+
+// floor(v) clamped to [-1, n] in float before the conversion, so extreme or
+// non-finite query coordinates cannot overflow the int cell index (NaN -> -1).
+__device__ __forceinline__ int cell_floor(float v, int n) {
+  v = fminf(fmaxf(v, -1.0f), (float)n);
+  return (int)floorf(v);
+}
+
+// Locate the problem of a flat block index.
+__device__ __forceinline__ int find_problem(const AssocProblem* probs, int n_probs, int blk) {
+  int lo = 0, hi = n_probs - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (probs[mid].block_begin <= blk) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// ---------------------------------------------------------------------------
+// Grid construction
+// ---------------------------------------------------------------------------
+
+__global__ void grid_cell_count(const float* __restrict__ xyz, int n, float ox, float oy, float oz, float inv_h,
+                                int nx, int ny, int nz, uint32_t* __restrict__ key, uint32_t* __restrict__ count) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  int cx = (int)floorf((xyz[3 * j + 0] - ox) * inv_h);
+  int cy = (int)floorf((xyz[3 * j + 1] - oy) * inv_h);
+  int cz = (int)floorf((xyz[3 * j + 2] - oz) * inv_h);
+  cx = min(max(cx, 0), nx - 1);
+  cy = min(max(cy, 0), ny - 1);
+  cz = min(max(cz, 0), nz - 1);
+  uint32_t c = ((uint32_t)cz * (uint32_t)ny + (uint32_t)cy) * (uint32_t)nx + (uint32_t)cx;
+  key[j] = c;
+  atomicAdd(&count[c], 1u);
+}
+
+__global__ void iota_u32(uint32_t* __restrict__ v, int n) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < n) v[j] = (uint32_t)j;
+}
+
+// Gathers the cloud into sorted order: pos = (x, y, z, original id bits).
+__global__ void gather_sorted(const uint32_t* __restrict__ order, int n, const float* __restrict__ xyz,
+                              const float* __restrict__ feat, int F, const float* __restrict__ sem, int K,
+                              const float* __restrict__ geo, float4* __restrict__ pos, float* __restrict__ feat_s,
+                              float* __restrict__ sem_s, float* __restrict__ geo_s, uint32_t* __restrict__ inv_perm) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  uint32_t j = order[s];
+  pos[s] = make_float4(xyz[3 * j], xyz[3 * j + 1], xyz[3 * j + 2], __uint_as_float(j));
+  if (feat_s)
+    for (int f = 0; f < F; ++f) feat_s[(size_t)s * F + f] = feat[(size_t)j * F + f];
+  if (sem_s)
+    for (int k = 0; k < K; ++k) sem_s[(size_t)s * K + k] = sem[(size_t)j * K + k];
+  if (geo_s) {
+    geo_s[2 * (size_t)s] = geo[2 * (size_t)j];
+    geo_s[2 * (size_t)s + 1] = geo[2 * (size_t)j + 1];
+  }
+  if (inv_perm) inv_perm[j] = (uint32_t)s;
+}
+
+// 30-bit Morton key of a point in its cloud's bounding box (source row order).
+__device__ __forceinline__ uint32_t spread10(uint32_t v) {
+  v &= 0x3ffu;
+  v = (v | (v << 16)) & 0x030000FFu;
+  v = (v | (v << 8)) & 0x0300F00Fu;
+  v = (v | (v << 4)) & 0x030C30C3u;
+  v = (v | (v << 2)) & 0x09249249u;
+  return v;
+}
+
+__global__ void morton_keys(const float* __restrict__ xyz, int n, float ox, float oy, float oz, float sx, float sy,
+                            float sz, uint32_t* __restrict__ key) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  uint32_t a = (uint32_t)fminf(fmaxf((xyz[3 * j] - ox) * sx, 0.0f), 1023.0f);
+  uint32_t b = (uint32_t)fminf(fmaxf((xyz[3 * j + 1] - oy) * sy, 0.0f), 1023.0f);
+  uint32_t c = (uint32_t)fminf(fmaxf((xyz[3 * j + 2] - oz) * sz, 0.0f), 1023.0f);
+  key[j] = spread10(a) | (spread10(b) << 1) | (spread10(c) << 2);
+}
+
+// ---------------------------------------------------------------------------
+// Association + fused flow
+// ---------------------------------------------------------------------------
+
+// Loads a scalar feature (compile-time or runtime dimension).
+template <int N>
+struct Dim {
+  __device__ static constexpr int get(int) { return N; }
+};
+template <>
+struct Dim<-1> {
+  __device__ static int get(int rt) { return rt; }
+};
+
+struct RowResult {
+  double acc[7];  // omega(3), v(3), ip
+  uint32_t stored;
+  bool overflow;
+};
+
+// One row handled by a 16-lane group.  FC/KC: color/semantic dimension when the
+// gate is on (0 off, -1 runtime).  MODE_ALIGN accumulates the flow, MODE_IP and
+// MODE_BA only the sum of a.
+template <int FC, int KC>
+__device__ void assoc_row(const AssocProblem& P, const ProblemState* st, const KConsts& kc, const GridLevel& G,
+                          int r, int g, int mode, float ell, int K, const float* A, const float* bt, const float* Rq,
+                          const float* Tq, bool src_xf, const float* As, const float* bs, RowResult& out) {
+  for (int c = 0; c < 7; ++c) out.acc[c] = 0.0;
+  out.stored = 0;
+  out.overflow = false;
+  const int F = Dim<FC>::get(kc.F);
+  const int KS = Dim<KC>::get(kc.Ks);
+  float4 x4 = P.src_pos[r];
+  float x0 = x4.x, x1 = x4.y, x2 = x4.z;
+  if (src_xf) {
+    float t0, t1, t2;
+    affine3(As, bs, x0, x1, x2, t0, t1, t2);
+    x0 = t0; x1 = t1; x2 = t2;
+  }
+  const float l = range_ell(ell, x0, x1, x2);
+  const float d2t = kc.use_geo ? (float)(-2.0 * (double)l * (double)l * (double)kc.lg) : 1.0f;
+  const float qg = (float)(1.0 / (2.0 * (double)l * (double)l));
+  // source features in registers when the dimension is static
+  constexpr int FR = FC > 0 ? FC : 1;
+  constexpr int KR = KC > 0 ? KC : 1;
+  float xf[FR], xs[KR];
+  const float* xf_g = P.src_feat ? P.src_feat + (size_t)r * kc.F : nullptr;
+  const float* xs_g = P.src_sem ? P.src_sem + (size_t)r * kc.Ks : nullptr;
+  if (FC > 0)
+    for (int f = 0; f < FR; ++f) xf[f] = xf_g[f];
+  if (KC > 0)
+    for (int k = 0; k < KR; ++k) xs[k] = xs_g[k];
+  float xg[2] = {0.0f, 0.0f};
+  if (kc.use_gtype) {
+    xg[0] = P.src_geo[2 * (size_t)r];
+    xg[1] = P.src_geo[2 * (size_t)r + 1];
+  }
+  // query box in the target frame
+  float q0, q1, q2;
+  affine3(Rq, Tq, x0, x1, x2, q0, q1, q2);
+  const float rq = sqrtf(d2t) * 1.001f + 1e-4f * (1.0f + fmaxf(fabsf(q0), fmaxf(fabsf(q1), fabsf(q2))));
+  int lo0 = cell_floor((q0 - rq - G.ox) * G.inv_h, G.nx), hi0 = cell_floor((q0 + rq - G.ox) * G.inv_h, G.nx);
+  int lo1 = cell_floor((q1 - rq - G.oy) * G.inv_h, G.ny), hi1 = cell_floor((q1 + rq - G.oy) * G.inv_h, G.ny);
+  int lo2 = cell_floor((q2 - rq - G.oz) * G.inv_h, G.nz), hi2 = cell_floor((q2 + rq - G.oz) * G.inv_h, G.nz);
+  int nruns = 0, span1 = 0;
+  if (!kc.use_geo) {
+    // no distance gate: every target is a candidate, so the row takes the
+    // exact full scan of the capped-row kernel
+    out.stored = (uint32_t)K + 1u;
+    out.overflow = true;
+    return;
+  }
+  if (!(hi0 < 0 || lo0 >= G.nx || hi1 < 0 || lo1 >= G.ny || hi2 < 0 || lo2 >= G.nz)) {
+    lo0 = max(lo0, 0); hi0 = min(hi0, G.nx - 1);
+    lo1 = max(lo1, 0); hi1 = min(hi1, G.ny - 1);
+    lo2 = max(lo2, 0); hi2 = min(hi2, G.nz - 1);
+    span1 = hi1 - lo1 + 1;
+    nruns = span1 * (hi2 - lo2 + 1);
+  }
+  const uint32_t Ku = (uint32_t)K;
+  uint32_t stored = 0;
+  bool stop = false;
+  for (int rbase = 0; rbase < nruns && !stop; rbase += kGroup) {
+    int myrun = rbase + g;
+    uint32_t rs = 0, rl = 0;
+    if (myrun < nruns) {
+      int iz = myrun / span1, iy = myrun - iz * span1;
+      uint32_t c0 = ((uint32_t)(lo2 + iz) * (uint32_t)G.ny + (uint32_t)(lo1 + iy)) * (uint32_t)G.nx + (uint32_t)lo0;
+      rs = G.cell_start[c0];
+      rl = G.cell_start[c0 + (uint32_t)(hi0 - lo0) + 1u] - rs;
+    }
+    uint32_t incl = rl;
+#pragma unroll
+    for (int o = 1; o < kGroup; o <<= 1) {
+      uint32_t t = __shfl_up(incl, o, 16);
+      if (g >= o) incl += t;
+    }
+    const uint32_t total = shfl16(incl, 15);
+    const uint32_t excl = incl - rl;
+    for (uint32_t base = 0; base < total; base += kGroup) {
+      const uint32_t idx = base + (uint32_t)g;
+      int kk = 0;
+#pragma unroll
+      for (int s = 8; s > 0; s >>= 1) {
+        uint32_t e = shfl16(excl, kk + s);
+        if (e <= idx) kk += s;
+      }
+      const uint32_t run_start = shfl16(rs, kk);
+      const uint32_t run_excl = shfl16(excl, kk);
+      bool pass = false;
+      float a = 0.0f;
+      float y0 = 0.0f, y1 = 0.0f, y2 = 0.0f;
+      uint32_t pidx = 0;
+      if (idx < total) {
+        pidx = run_start + (idx - run_excl);
+        float4 y4 = G.pos[pidx];
+        affine3(A, bt, y4.x, y4.y, y4.z, y0, y1, y2);
+        float dx = y0 - x0, dy = y1 - x1, dz = y2 - x2;
+        float d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+        pass = !kc.use_geo || d2 < d2t;
+        float geo_sim = 1.0f;
+        if (pass && kc.use_gtype) {
+          float gb[2] = {G.geo[2 * (size_t)pidx], G.geo[2 * (size_t)pidx + 1]};
+          geo_sim = geo_similarity(xg, gb);
+          pass = !(geo_sim < 0.01f);
+        }
+        float d2c = 0.0f, d2s = 0.0f;
+        if (FC != 0 && pass) {
+          const float* yf = G.feat + (size_t)pidx * F;
+          for (int f = 0; f < F; ++f) {
+            float t = (FC > 0 ? xf[f < FR ? f : 0] : xf_g[f]) - yf[f];
+            d2c = fmaf(t, t, d2c);
+          }
+          pass = d2c < kc.d2_c_thres;
+        }
+        if (KC != 0 && pass) {
+          const float* ys = G.sem + (size_t)pidx * KS;
+          for (int k = 0; k < KS; ++k) {
+            float t = (KC > 0 ? xs[k < KR ? k : 0] : xs_g[k]) - ys[k];
+            d2s = fmaf(t, t, d2s);
+          }
+          pass = d2s < kc.d2_s_thres;
+        }
+        if (pass) {
+          float ex = kc.use_geo ? d2 * qg : 0.0f;
+          if (FC != 0) ex = fmaf(d2c, kc.q_c, ex);
+          if (KC != 0) ex = fmaf(d2s, kc.q_s, ex);
+          a = kc.amp * expf(-ex) * geo_sim;
+          if (fabsf(a - kc.sp) <= 2e-5f * kc.sp) a = exact_value(kc, d2, l, d2c, d2s, geo_sim);
+          pass = a > kc.sp;
+        }
+      }
+      const uint64_t m = __ballot(pass);
+      const uint32_t gm = (uint32_t)(m >> (16 * ((threadIdx.x & 63) >> 4))) & 0xFFFFu;
+      const uint32_t slot = stored + (uint32_t)__popc(gm & ((1u << g) - 1u));
+      if (pass && slot < Ku) {
+        P.pairs[(size_t)r * P.cap_alloc + slot] = make_uint2(pidx, __float_as_uint(a));
+        if (mode == MODE_ALIGN) {
+          float c0 = x1 * y2 - x2 * y1, c1 = x2 * y0 - x0 * y2, c2 = x0 * y1 - x1 * y0;
+          out.acc[0] += (double)(c0 * a);
+          out.acc[1] += (double)(c1 * a);
+          out.acc[2] += (double)(c2 * a);
+          out.acc[3] += (double)((y0 - x0) * a);
+          out.acc[4] += (double)((y1 - x1) * a);
+          out.acc[5] += (double)((y2 - x2) * a);
+        }
+        out.acc[6] += (double)a;
+      }
+      stored += (uint32_t)__popc(gm);
+      if (stored > Ku) {
+        stop = true;
+        break;
+      }
+    }
+  }
+  out.stored = stored;
+  out.overflow = stored > Ku;
+}
+
+template <int FC, int KC>
+__global__ __launch_bounds__(kBlock) void assoc_kernel(const AssocProblem* __restrict__ probs, int n_probs,
+                                                       KConsts kc) {
+  const int p = find_problem(probs, n_probs, (int)blockIdx.x);
+  const AssocProblem& P = probs[p];
+  ProblemState* st = P.st;
+  if (st->done) return;
+  const int lb = (int)blockIdx.x - P.block_begin;
+  const int tid = threadIdx.x;
+  const int g = tid & (kGroup - 1);
+  const int grp = tid >> 4;
+  const int r = lb * kRowsPerBlock + grp;
+  const int mode = kc.mode;
+  const float ell = st->ell;
+  const int K = st->K;
+  const int iter = st->iter;
+  float A[9], bt[3], Rq[9], Tq[3], As[9], bs[3];
+  for (int i = 0; i < 9; ++i) { A[i] = st->A[i]; Rq[i] = st->Rq[i]; As[i] = st->As[i]; }
+  for (int i = 0; i < 3; ++i) { bt[i] = st->b[i]; Tq[i] = st->Tq[i]; bs[i] = st->bs[i]; }
+  const bool src_xf = st->has_src_xf != 0;
+  const GridLevel& G = P.grid->lv[st->level];
+
+  __shared__ double s_row[kRowsPerBlock][8];
+  __shared__ uint32_t s_cnt[kRowsPerBlock];
+  RowResult res;
+  for (int c = 0; c < 7; ++c) res.acc[c] = 0.0;
+  res.stored = 0;
+  res.overflow = false;
+  if (r < P.n_src) assoc_row<FC, KC>(P, st, kc, G, r, g, mode, ell, K, A, bt, Rq, Tq, src_xf, As, bs, res);
+  // reduce the row's lanes
+#pragma unroll
+  for (int c = 0; c < 7; ++c) {
+    double v = res.acc[c];
+    v += shfl_xor_d(v, 8);
+    v += shfl_xor_d(v, 4);
+    v += shfl_xor_d(v, 2);
+    v += shfl_xor_d(v, 1);
+    res.acc[c] = v;
+  }
+  if (g == 0) {
+    uint32_t cnt = 0;
+    if (r < P.n_src) {
+      // a capped row is counted by the capped-row kernel, which rewrites it
+      cnt = res.overflow ? 0u : res.stored;
+      P.counts[r] = cnt;
+      if (res.overflow) {
+        P.ovf_stamp[r] = (uint32_t)iter + 1u;
+        atomicAdd(&st->n_overflow, 1u);
+      }
+    }
+    for (int c = 0; c < 7; ++c) s_row[grp][c] = res.overflow ? 0.0 : res.acc[c];
+    s_cnt[grp] = cnt;
+  }
+  __syncthreads();
+  if (tid < 7) {
+    double s = 0.0;
+    for (int q = 0; q < kRowsPerBlock; ++q) s += s_row[q][tid];
+    P.part_a[(size_t)lb * 8 + tid] = s;
+  } else if (tid == 7) {
+    uint32_t nz = 0, mx = 0;
+    for (int q = 0; q < kRowsPerBlock; ++q) {
+      nz += s_cnt[q];
+      mx = max(mx, s_cnt[q]);
+    }
+    P.part_cnt[(size_t)lb * 2] = nz;
+    P.part_cnt[(size_t)lb * 2 + 1] = mx;
+  }
+  // last block of this problem reduces the partials in fixed order
+  __shared__ int s_last;
+  __syncthreads();
+  if (tid == 0) {
+    __threadfence();
+    uint32_t prev = __hip_atomic_fetch_add(&st->blocks_done_assoc, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    s_last = (prev == (uint32_t)P.n_blocks - 1u);
+  }
+  __syncthreads();
+  if (!s_last) return;
+  __threadfence();
+  __shared__ double s_red[kBlock];
+  __shared__ uint64_t s_nz[kBlock];
+  __shared__ uint32_t s_mx[kBlock];
+  double v[7] = {0, 0, 0, 0, 0, 0, 0};
+  uint64_t nz = 0;
+  uint32_t mx = 0;
+  for (int b = tid; b < P.n_blocks; b += kBlock) {
+    for (int c = 0; c < 7; ++c) v[c] += __builtin_nontemporal_load(&P.part_a[(size_t)b * 8 + c]);
+    nz += __builtin_nontemporal_load(&P.part_cnt[(size_t)b * 2]);
+    mx = max(mx, __builtin_nontemporal_load(&P.part_cnt[(size_t)b * 2 + 1]));
+  }
+  s_nz[tid] = nz;
+  s_mx[tid] = mx;
+  double out[7];
+  for (int c = 0; c < 7; ++c) {
+    s_red[tid] = v[c];
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+      if (tid < w) s_red[tid] += s_red[tid + w];
+      __syncthreads();
+    }
+    out[c] = s_red[0];
+    __syncthreads();
+  }
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if (tid < w) {
+      s_nz[tid] += s_nz[tid + w];
+      s_mx[tid] = max(s_mx[tid], s_mx[tid + w]);
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    for (int c = 0; c < 6; ++c) st->flow[c] = out[c];
+    st->ip = out[6];
+    st->nnz = s_nz[0];
+    st->max_row = s_mx[0];
+    st->blocks_done_assoc = 0;
+    st->level_assoc = st->level;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Rows whose passing pairs exceed the cap: the reference's exact scan over all
+// targets in ascending index, keeping the first K (CvoGPU.cu:994-996).
+// ---------------------------------------------------------------------------
+
+template <int FC, int KC>
+__global__ __launch_bounds__(kBlock) void overflow_kernel(const AssocProblem* __restrict__ probs, int n_probs,
+                                                          KConsts kc) {
+  const AssocProblem& P = probs[blockIdx.y];
+  ProblemState* st = P.st;
+  if (st->done || st->n_overflow == 0) return;
+  const int tid = threadIdx.x;
+  const int mode = kc.mode;
+  const float ell = st->ell;
+  const uint32_t K = (uint32_t)st->K;
+  const uint32_t stamp = (uint32_t)st->iter + 1u;
+  float A[9], bt[3], As[9], bs[3];
+  for (int i = 0; i < 9; ++i) { A[i] = st->A[i]; As[i] = st->As[i]; }
+  for (int i = 0; i < 3; ++i) { bt[i] = st->b[i]; bs[i] = st->bs[i]; }
+  const bool src_xf = st->has_src_xf != 0;
+  const GridLevel& G = P.grid->lv[st->level];
+  const int F = Dim<FC>::get(kc.F);
+  const int KS = Dim<KC>::get(kc.Ks);
+  const int chunk = (P.n_src + kOvfBlocks - 1) / kOvfBlocks;
+  const int r0 = blockIdx.x * chunk, r1 = min(P.n_src, r0 + chunk);
+  __shared__ uint32_t s_wave[kBlock / 64];
+  __shared__ double s_red[kBlock];
+  double blk_acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // flow6, ip, stored pairs
+  for (int r = r0; r < r1; ++r) {
+    if (P.ovf_stamp[r] != stamp) continue;  // uniform
+    float4 x4 = P.src_pos[r];
+    float x0 = x4.x, x1 = x4.y, x2 = x4.z;
+    if (src_xf) {
+      float t0, t1, t2;
+      affine3(As, bs, x0, x1, x2, t0, t1, t2);
+      x0 = t0; x1 = t1; x2 = t2;
+    }
+    const float l = range_ell(ell, x0, x1, x2);
+    const float d2t = kc.use_geo ? (float)(-2.0 * (double)l * (double)l * (double)kc.lg) : 1.0f;
+    const float qg = (float)(1.0 / (2.0 * (double)l * (double)l));
+    const float* xf = P.src_feat ? P.src_feat + (size_t)r * kc.F : nullptr;
+    const float* xs = P.src_sem ? P.src_sem + (size_t)r * kc.Ks : nullptr;
+    float xg[2] = {0.0f, 0.0f};
+    if (kc.use_gtype) {
+      xg[0] = P.src_geo[2 * (size_t)r];
+      xg[1] = P.src_geo[2 * (size_t)r + 1];
+    }
+    double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+    uint32_t stored = 0;
+    for (int base = 0; base < P.n_tgt && stored < K; base += kBlock) {
+      const int j = base + tid;
+      bool pass = false;
+      float a = 0.0f, y0 = 0.0f, y1 = 0.0f, y2 = 0.0f;
+      if (j < P.n_tgt) {
+        affine3(A, bt, P.tgt_xyz[3 * (size_t)j], P.tgt_xyz[3 * (size_t)j + 1], P.tgt_xyz[3 * (size_t)j + 2], y0, y1, y2);
+        float dx = y0 - x0, dy = y1 - x1, dz = y2 - x2;
+        float d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+        pass = !kc.use_geo || d2 < d2t;
+        float geo_sim = 1.0f;
+        if (pass && kc.use_gtype) {
+          geo_sim = geo_similarity(xg, P.tgt_geo + 2 * (size_t)j);
+          pass = !(geo_sim < 0.01f);
+        }
+        float d2c = 0.0f, d2s = 0.0f;
+        if (FC != 0 && pass) {
+          const float* yf = P.tgt_feat + (size_t)j * F;
+          for (int f = 0; f < F; ++f) {
+            float t = xf[f] - yf[f];
+            d2c = fmaf(t, t, d2c);
+          }
+          pass = d2c < kc.d2_c_thres;
+        }
+        if (KC != 0 && pass) {
+          const float* ys = P.tgt_sem + (size_t)j * KS;
+          for (int k = 0; k < KS; ++k) {
+            float t = xs[k] - ys[k];
+            d2s = fmaf(t, t, d2s);
+          }
+          pass = d2s < kc.d2_s_thres;
+        }
+        if (pass) {
+          float ex = kc.use_geo ? d2 * qg : 0.0f;
+          if (FC != 0) ex = fmaf(d2c, kc.q_c, ex);
+          if (KC != 0) ex = fmaf(d2s, kc.q_s, ex);
+          a = kc.amp * expf(-ex) * geo_sim;
+          if (fabsf(a - kc.sp) <= 2e-5f * kc.sp) a = exact_value(kc, d2, l, d2c, d2s, geo_sim);
+          pass = a > kc.sp;
+        }
+      }
+      // block-wide exclusive prefix of pass flags, in index order
+      const uint64_t m = __ballot(pass);
+      const int lane = tid & 63, wave = tid >> 6;
+      if (lane == 0) s_wave[wave] = (uint32_t)__popcll(m);
+      __syncthreads();
+      uint32_t before = 0, tot = 0;
+      for (int w = 0; w < kBlock / 64; ++w) {
+        if (w < wave) before += s_wave[w];
+        tot += s_wave[w];
+      }
+      before += (uint32_t)__popcll(m & ((lane == 0) ? 0ull : (~0ull >> (64 - lane))));
+      const uint32_t slot = stored + before;
+      if (pass && slot < K) {
+        P.pairs[(size_t)r * P.cap_alloc + slot] = make_uint2(G.inv_perm[j], __float_as_uint(a));
+        if (mode == MODE_ALIGN) {
+          float c0 = x1 * y2 - x2 * y1, c1 = x2 * y0 - x0 * y2, c2 = x0 * y1 - x1 * y0;
+          acc[0] += (double)(c0 * a);
+          acc[1] += (double)(c1 * a);
+          acc[2] += (double)(c2 * a);
+          acc[3] += (double)((y0 - x0) * a);
+          acc[4] += (double)((y1 - x1) * a);
+          acc[5] += (double)((y2 - x2) * a);
+        }
+        acc[6] += (double)a;
+      }
+      stored += tot;
+      __syncthreads();
+    }
+    if (tid == 0) {
+      const uint32_t kept = min(stored, K);
+      P.counts[r] = kept;
+      atomicMax(&st->max_row, kept);
+      blk_acc[7] += (double)kept;
+    }
+    for (int c = 0; c < 7; ++c) {
+      s_red[tid] = acc[c];
+      __syncthreads();
+      for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if (tid < w) s_red[tid] += s_red[tid + w];
+        __syncthreads();
+      }
+      blk_acc[c] += s_red[0];
+      __syncthreads();
+    }
+  }
+  if (tid < 7) P.part_ovf[(size_t)blockIdx.x * 8 + tid] = blk_acc[tid];
+  if (tid == 0) P.part_ovf[(size_t)blockIdx.x * 8 + 7] = blk_acc[7];
+  __shared__ int s_last;
+  __syncthreads();
+  if (tid == 0) {
+    __threadfence();
+    uint32_t prev = __hip_atomic_fetch_add(&st->blocks_done_ovf, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    s_last = (prev == (uint32_t)kOvfBlocks - 1u);
+  }
+  __syncthreads();
+  if (!s_last) return;
+  __threadfence();
+  if (tid == 0) {
+    double add[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int b = 0; b < kOvfBlocks; ++b)
+      for (int c = 0; c < 8; ++c) add[c] += __builtin_nontemporal_load(&P.part_ovf[(size_t)b * 8 + c]);
+    for (int c = 0; c < 6; ++c) st->flow[c] += add[c];
+    st->ip += add[6];
+    st->nnz += (uint64_t)add[7];
+    st->blocks_done_ovf = 0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Device controller pieces (FP64)
+// ---------------------------------------------------------------------------
+
+// Real roots of a s^3 + b s^2 + c s + d by the trigonometric / Cardano closed
+// form with two Newton polish steps; complex pairs report their imaginary part
+// for the |imag| < 1e-5 rule of compute_step_size (CvoGPU.cu:513-519).
+__device__ void cubic_solve(double a, double b, double c, double d, double re[3], double im[3]) {
+  const double nan = __longlong_as_double(0x7ff8000000000000ll);
+  re[0] = re[1] = re[2] = nan;
+  im[0] = im[1] = im[2] = 0.0;
+  if (!(a != 0.0)) return;
+  const double p2 = b / a, p1 = c / a, p0 = d / a;
+  if (!isfinite(p2) || !isfinite(p1) || !isfinite(p0)) return;
+  const double sh = p2 / 3.0;
+  const double p = p1 - p2 * sh;
+  const double q = 2.0 * sh * sh * sh - sh * p1 + p0;
+  const double disc = 0.25 * q * q + p * p * p / 27.0;
+  auto polish = [&](double s) {
+    for (int it = 0; it < 2; ++it) {
+      double f = ((s + p2) * s + p1) * s + p0;
+      double df = (3.0 * s + 2.0 * p2) * s + p1;
+      if (df == 0.0) break;
+      double ns = s - f / df;
+      if (!isfinite(ns)) break;
+      s = ns;
+    }
+    return s;
+  };
+  if (disc > 0.0) {
+    const double sq = sqrt(disc);
+    const double u = cbrt(-0.5 * q + (q > 0 ? -sq : sq));
+    const double t = (u != 0.0) ? u - p / (3.0 * u) : 0.0;
+    re[0] = polish(t - sh);
+    const double ci = sqrt(fmax(0.75 * t * t + p, 0.0));
+    re[1] = re[2] = -0.5 * t - sh;
+    im[1] = ci;
+    im[2] = -ci;
+  } else if (p == 0.0) {
+    re[0] = re[1] = re[2] = -sh;
+  } else {
+    const double m = 2.0 * sqrt(-p / 3.0);
+    double arg = 3.0 * q / (p * m);
+    arg = fmin(1.0, fmax(-1.0, arg));
+    const double th = acos(arg) / 3.0;
+    for (int k = 0; k < 3; ++k) re[k] = polish(m * cos(th - 2.0943951023931954923 * k) - sh);
+  }
+}
+
+__device__ double select_step_dev(const KConsts& kc, const double bcde[4]) {
+  double re[3], im[3];
+  cubic_solve(4.0 * bcde[3], 3.0 * bcde[2], 2.0 * bcde[1], bcde[0], re, im);
+  double temp = 1.7976931348623157e308;
+  for (int i = 0; i < 3; ++i)
+    if (re[i] > 0 && re[i] < temp && fabs(im[i]) < 1e-5) temp = re[i];
+  if (temp > (double)kc.max_step) return (double)kc.max_step;
+  if (temp < (double)kc.min_step) return (double)kc.min_step;
+  return temp;
+}
+
+// exp(s [w; v]) for a unit twist, FP64, exact small-angle series.
+__device__ void se3_exp(const double xi[6], double s, double R[9], double t[3]) {
+  const double ph[3] = {xi[0] * s, xi[1] * s, xi[2] * s};
+  const double u[3] = {xi[3] * s, xi[4] * s, xi[5] * s};
+  const double th2 = ph[0] * ph[0] + ph[1] * ph[1] + ph[2] * ph[2];
+  const double th = sqrt(th2);
+  double ca, cb, cc;
+  if (th < 1e-3) {
+    ca = 1.0 - th2 / 6.0 + th2 * th2 / 120.0;
+    cb = 0.5 - th2 / 24.0 + th2 * th2 / 720.0;
+    cc = 1.0 / 6.0 - th2 / 120.0 + th2 * th2 / 5040.0;
+  } else {
+    double sn, cs;
+    sincos(th, &sn, &cs);
+    ca = sn / th;
+    cb = (1.0 - cs) / th2;
+    cc = (th - sn) / (th2 * th);
+  }
+  const double W[9] = {0, -ph[2], ph[1], ph[2], 0, -ph[0], -ph[1], ph[0], 0};
+  double V[9];
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) {
+      double w2 = W[r * 3 + 0] * W[0 * 3 + c] + W[r * 3 + 1] * W[1 * 3 + c] + W[r * 3 + 2] * W[2 * 3 + c];
+      double I = (r == c) ? 1.0 : 0.0;
+      R[r * 3 + c] = I + ca * W[r * 3 + c] + cb * w2;
+      V[r * 3 + c] = I + cb * W[r * 3 + c] + cc * w2;
+    }
+  for (int r = 0; r < 3; ++r) t[r] = V[r * 3 + 0] * u[0] + V[r * 3 + 1] * u[1] + V[r * 3 + 2] * u[2];
+}
+
+// ||log(T)|| of a rigid transform, from the rotation's unit quaternion
+// (the Sophus SE3d log norm of dist_se3_cpu, src/cvo/CvoGPU.cpp:28-33).
+__device__ double se3_log_norm(const double R[9], const double t[3]) {
+  double qw, qx, qy, qz;
+  const double tr = R[0] + R[4] + R[8];
+  if (tr > 0) {
+    double s = sqrt(tr + 1.0) * 2.0;
+    qw = 0.25 * s; qx = (R[7] - R[5]) / s; qy = (R[2] - R[6]) / s; qz = (R[3] - R[1]) / s;
+  } else if (R[0] > R[4] && R[0] > R[8]) {
+    double s = sqrt(1.0 + R[0] - R[4] - R[8]) * 2.0;
+    qw = (R[7] - R[5]) / s; qx = 0.25 * s; qy = (R[1] + R[3]) / s; qz = (R[2] + R[6]) / s;
+  } else if (R[4] > R[8]) {
+    double s = sqrt(1.0 + R[4] - R[0] - R[8]) * 2.0;
+    qw = (R[2] - R[6]) / s; qx = (R[1] + R[3]) / s; qy = 0.25 * s; qz = (R[5] + R[7]) / s;
+  } else {
+    double s = sqrt(1.0 + R[8] - R[0] - R[4]) * 2.0;
+    qw = (R[3] - R[1]) / s; qx = (R[2] + R[6]) / s; qy = (R[5] + R[7]) / s; qz = 0.25 * s;
+  }
+  const double qn = sqrt(qw * qw + qx * qx + qy * qy + qz * qz);
+  qw /= qn; qx /= qn; qy /= qn; qz /= qn;
+  const double n2 = qx * qx + qy * qy + qz * qz, n = sqrt(n2);
+  double f;
+  if (n2 < 1e-20) f = 2.0 / qw - 2.0 * n2 / (qw * qw * qw);
+  else if (fabs(qw) < 1e-10) f = (qw > 0 ? M_PI : -M_PI) / n;
+  else f = 2.0 * atan(n / qw) / n;
+  const double w[3] = {f * qx, f * qy, f * qz};
+  const double th = sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+  const double W[9] = {0, -w[2], w[1], w[2], 0, -w[0], -w[1], w[0], 0};
+  const double k2 = th < 1e-6 ? 1.0 / 12.0 : (1.0 - th * cos(0.5 * th) / (2.0 * sin(0.5 * th))) / (th * th);
+  double s2 = w[0] * w[0] + w[1] * w[1] + w[2] * w[2];
+  for (int r = 0; r < 3; ++r) {
+    double acc = 0.0;
+    for (int c = 0; c < 3; ++c) {
+      double w2 = W[r * 3 + 0] * W[0 * 3 + c] + W[r * 3 + 1] * W[1 * 3 + c] + W[r * 3 + 2] * W[2 * 3 + c];
+      double vi = ((r == c) ? 1.0 : 0.0) - 0.5 * W[r * 3 + c] + k2 * w2;
+      acc += vi * t[c];
+    }
+    s2 += acc * acc;
+  }
+  return sqrt(s2);
+}
+
+// A_sparsity_indicator_ell_update (CvoGPU.cu:547-665): two FIFO windows with
+// float sums; ring buffers of modulus W+1.
+__device__ bool indicator_update(ProblemState* st, float ind, int W, float thr) {
+  const int cap = W + 1;
+  bool dec = false;
+  if (st->qs_size < W) {
+    st->qs[(st->qs_head + st->qs_size) % cap] = ind;
+    st->qs_size++;
+    st->ss += ind;
+  }
+  if (st->qs_size >= W && st->qe_size < W) {
+    st->qe[(st->qe_head + st->qe_size) % cap] = ind;
+    st->qe_size++;
+    st->se += ind;
+  }
+  if (st->qs_size >= W && st->qe_size >= W) {
+    const float ratio = st->se / st->ss;
+    if (ratio > 1.0f - thr && ratio < 1.0f + thr) {
+      dec = true;
+      st->qs_head = st->qs_size = st->qe_head = st->qe_size = 0;
+      st->ss = 0.0f;
+      st->se = 0.0f;
+    } else {
+      const float f = st->qe[st->qe_head];
+      st->se -= f;
+      st->ss += f;
+      st->qs[(st->qs_head + st->qs_size) % cap] = f;
+      st->qs_size++;
+      st->qe_head = (st->qe_head + 1) % cap;
+      st->qe_size--;
+      st->ss -= st->qs[st->qs_head];
+      st->qs_head = (st->qs_head + 1) % cap;
+      st->qs_size--;
+      st->qe[(st->qe_head + st->qe_size) % cap] = ind;
+      st->qe_size++;
+      st->se += ind;
+    }
+  }
+  return dec;
+}
+
+// Float maps of the next iteration from the FP64 pose: target map [R^T, -R^T T]
+// (update_tf, CvoGPU.cu:98-130) and the query map (R, T).
+__device__ void set_maps(ProblemState* st) {
+  for (int i = 0; i < 3; ++i) {
+    for (int j = 0; j < 3; ++j) {
+      st->A[i * 3 + j] = (float)st->R[j * 3 + i];
+      st->Rq[i * 3 + j] = (float)st->R[i * 3 + j];
+    }
+    st->b[i] = (float)(-(st->R[0 * 3 + i] * st->T[0] + st->R[1 * 3 + i] * st->T[1] + st->R[2 * 3 + i] * st->T[2]));
+    st->Tq[i] = (float)st->T[i];
+  }
+}
+
+__device__ int choose_level(const GridDev* grid, float r_need) {
+  int L = 0;
+  for (int l = 0; l < grid->n_levels; ++l)
+    if (grid->lv[l].h >= r_need) L = l;
+  return L;
+}
+
+__device__ float needed_radius(const KConsts& kc, const ProblemState* st, float ell) {
+  float tq = fmaxf(fabsf(st->Tq[0]), fmaxf(fabsf(st->Tq[1]), fabsf(st->Tq[2])));
+  return kc.rq_coef * ell * (1.0f + kc.max_src_norm / 500.0f) * 1.0001f + 1e-4f * (1.0f + kc.max_src_norm + tq) + 1e-6f;
+}
+
+// One controller step of align_impl (CvoGPU.cu:1342-1433), FP64 pose state.
+__device__ void controller(ProblemState* st, const KConsts& kc, const AssocProblem& P) {
+  cvo_iter_record rec;
+  const int k = st->iter;
+  double fl[6];
+  double n2 = 0.0;
+  for (int c = 0; c < 6; ++c) {
+    fl[c] = st->flow[c];
+    n2 += fl[c] * fl[c];
+  }
+  const double nrm = sqrt(n2);
+  double xi[6];
+  for (int c = 0; c < 6; ++c) xi[c] = n2 > 0.0 ? fl[c] / nrm : fl[c];
+  float xf[6];
+  for (int c = 0; c < 6; ++c) xf[c] = (float)xi[c];
+  const double nw = sqrt((double)xf[0] * xf[0] + (double)xf[1] * xf[1] + (double)xf[2] * xf[2]);
+  const double nv = sqrt((double)xf[3] * xf[3] + (double)xf[4] * xf[4] + (double)xf[5] * xf[5]);
+  rec.ell = st->ell;
+  rec.nnz = (int32_t)st->nnz;
+  rec.max_row = (int32_t)st->max_row;
+  rec.num_neighbors = st->K;
+  rec.flow_norm = (float)nrm;
+  rec.step = 0.0f;
+  rec.dist = 0.0f;
+  rec.ip = (float)((double)st->nnz / sqrt((double)kc.n_src_f * (double)kc.n_tgt_f));
+  if (nw < (double)kc.eps && nv < (double)kc.eps) {
+    st->ret = (nw < 1e-8 && nv < 1e-8) ? -1 : 0;
+    st->done = 1;
+    st->iters_out = k;
+    if (P.trace && k < P.trace_cap) P.trace[k] = rec;
+    return;
+  }
+  const double step = select_step_dev(kc, st->bcde);
+  st->step = step;
+  double dR[9], dT[3];
+  se3_exp(xi, step, dR, dT);
+  double Rn[9], Tn[3];
+  for (int r = 0; r < 3; ++r) {
+    Tn[r] = st->T[r] + st->R[r * 3 + 0] * dT[0] + st->R[r * 3 + 1] * dT[1] + st->R[r * 3 + 2] * dT[2];
+    for (int c = 0; c < 3; ++c)
+      Rn[r * 3 + c] = st->R[r * 3 + 0] * dR[0 * 3 + c] + st->R[r * 3 + 1] * dR[1 * 3 + c] + st->R[r * 3 + 2] * dR[2 * 3 + c];
+  }
+  for (int i = 0; i < 9; ++i) st->R[i] = Rn[i];
+  for (int i = 0; i < 3; ++i) st->T[i] = Tn[i];
+  const double dist = se3_log_norm(dR, dT);
+  st->dist = dist;
+  rec.step = (float)step;
+  rec.dist = (float)dist;
+  const bool dec = indicator_update(st, rec.ip, kc.window, kc.ind_thr);
+  if (P.trace && k < P.trace_cap) P.trace[k] = rec;
+  set_maps(st);
+  if (dist < (double)kc.eps_2) {
+    st->done = 1;
+    st->iters_out = k;
+    return;
+  }
+  if (k > kc.ell_decay_start && dec) {
+    float e = st->ell * kc.ell_decay_rate;
+    if (e < kc.ell_min) e = kc.ell_min;
+    st->ell = e;
+  }
+  st->K = min(kc.nn_max, (int)((double)st->max_row * 1.2));
+  if (st->K < 1) st->K = 1;
+  st->iter = k + 1;
+  if (st->iter >= kc.max_iter) {
+    st->done = 1;
+    st->iters_out = kc.max_iter;
+  }
+  st->level = choose_level(P.grid, needed_radius(kc, st, st->ell));
+  st->n_overflow = 0;
+}
+
+// ---------------------------------------------------------------------------
+// Step terms (compute_step_size_xi + compute_step_size_poly_coeff) + controller
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ void cross_f(float a0, float a1, float a2, float b0, float b1, float b2, float& o0,
+                                        float& o1, float& o2) {
+  o0 = a1 * b2 - a2 * b1;
+  o1 = a2 * b0 - a0 * b2;
+  o2 = a0 * b1 - a1 * b0;
+}
+
+__global__ __launch_bounds__(kBlock) void step_kernel(const AssocProblem* __restrict__ probs, int n_probs,
+                                                      KConsts kc) {
+  const int p = find_problem(probs, n_probs, (int)blockIdx.x);
+  const AssocProblem& P = probs[p];
+  ProblemState* st = P.st;
+  if (st->done) return;
+  const int lb = (int)blockIdx.x - P.block_begin;
+  const int tid = threadIdx.x;
+  const int g = tid & (kGroup - 1);
+  const int grp = tid >> 4;
+  const int r = lb * kRowsPerBlock + grp;
+  const float ell = st->ell;
+  const GridLevel& G = P.grid->lv[st->level];
+  float A[9], bt[3], As[9], bs[3];
+  for (int i = 0; i < 9; ++i) { A[i] = st->A[i]; As[i] = st->As[i]; }
+  for (int i = 0; i < 3; ++i) { bt[i] = st->b[i]; bs[i] = st->bs[i]; }
+  // normalised flow (compute_flow, CvoGPU.cu:2099-2104), used in float
+  double n2 = 0.0;
+  double fl[6];
+  for (int c = 0; c < 6; ++c) {
+    fl[c] = st->flow[c];
+    n2 += fl[c] * fl[c];
+  }
+  const double nrm = n2 > 0.0 ? sqrt(n2) : 1.0;
+  const float w0 = (float)(fl[0] / nrm), w1 = (float)(fl[1] / nrm), w2 = (float)(fl[2] / nrm);
+  const float v0 = (float)(fl[3] / nrm), v1 = (float)(fl[4] / nrm), v2 = (float)(fl[5] / nrm);
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  if (r < P.n_src) {
+    float4 x4 = P.src_pos[r];
+    float x0 = x4.x, x1 = x4.y, x2 = x4.z;
+    if (st->has_src_xf) {
+      float t0, t1, t2;
+      affine3(As, bs, x0, x1, x2, t0, t1, t2);
+      x0 = t0; x1 = t1; x2 = t2;
+    }
+    float temp_ell = ell;
+    if (kc.range_ell_in_step) {
+      float px = sqrtf(x0 * x0 + x1 * x1 + x2 * x2);
+      temp_ell = (float)(((double)px / 500.0 + 1.0) * (double)ell);
+    }
+    const float tc = (float)(1.0 / (2.0 * (double)temp_ell * (double)temp_ell));
+    const uint32_t cnt = P.counts[r];
+    const uint2* row = P.pairs + (size_t)r * P.cap_alloc;
+    for (uint32_t s = (uint32_t)g; s < cnt; s += kGroup) {
+      const uint2 e = row[s];
+      const float a = __uint_as_float(e.y);
+      const float4 y4 = G.pos[e.x];
+      float z0, z1, z2;
+      affine3(A, bt, y4.x, y4.y, y4.z, z0, z1, z2);
+      // xi^k z: xiz = w x z + v, xi^{k+1} z = w x xi^k z
+      float a0, a1, a2, b0, b1, b2, c0, c1, c2, d0, d1, d2;
+      cross_f(w0, w1, w2, z0, z1, z2, a0, a1, a2);
+      a0 += v0; a1 += v1; a2 += v2;
+      cross_f(w0, w1, w2, a0, a1, a2, b0, b1, b2);
+      cross_f(w0, w1, w2, b0, b1, b2, c0, c1, c2);
+      cross_f(w0, w1, w2, c0, c1, c2, d0, d1, d2);
+      const float df0 = x0 - z0, df1 = x1 - z1, df2 = x2 - z2;
+      const float normxiz2 = a0 * a0 + a1 * a1 + a2 * a2;
+      const float xiz_dot_xi2z = -(a0 * b0 + a1 * b1 + a2 * b2);
+      const float epsil_const = (b0 * b0 + b1 * b1 + b2 * b2) + 2.0f * (a0 * c0 + a1 * c1 + a2 * c2);
+      const float beta = (float)(-2.0 * (double)tc * (double)(a0 * df0 + a1 * df1 + a2 * df2));
+      const float gamma = -tc * (normxiz2 + 2.0f * (b0 * df0 + b1 * df1 + b2 * df2));
+      const float delta = (float)(2.0 * (double)tc * (double)(xiz_dot_xi2z - (c0 * df0 + c1 * df1 + c2 * df2)));
+      const float epsil = -tc * (epsil_const + 2.0f * (d0 * df0 + d1 * df1 + d2 * df2));
+      const double ad = (double)a, bd = (double)beta, gd = (double)gamma;
+      acc[0] += (double)(a * beta);
+      acc[1] += ad * (gd + (double)(beta * beta) / 2.0);
+      acc[2] += ad * ((double)(delta + beta * gamma) + (double)(beta * beta * beta) / 6.0);
+      acc[3] += ad * ((double)(epsil + beta * delta) + 0.5 * bd * bd * gd + 0.5 * gd * gd +
+                      (1.0 / 24.0) * bd * bd * bd * bd);
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    double v = acc[c];
+    v += shfl_xor_d(v, 8);
+    v += shfl_xor_d(v, 4);
+    v += shfl_xor_d(v, 2);
+    v += shfl_xor_d(v, 1);
+    acc[c] = v;
+  }
+  __shared__ double s_row[kRowsPerBlock][4];
+  if (g == 0)
+    for (int c = 0; c < 4; ++c) s_row[grp][c] = acc[c];
+  __syncthreads();
+  if (tid < 4) {
+    double s = 0.0;
+    for (int q = 0; q < kRowsPerBlock; ++q) s += s_row[q][tid];
+    P.part_s[(size_t)lb * 4 + tid] = s;
+  }
+  __shared__ int s_last;
+  __syncthreads();
+  if (tid == 0) {
+    __threadfence();
+    uint32_t prev = __hip_atomic_fetch_add(&st->blocks_done_step, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    s_last = (prev == (uint32_t)P.n_blocks - 1u);
+  }
+  __syncthreads();
+  if (!s_last) return;
+  __threadfence();
+  __shared__ double s_red[kBlock];
+  double v[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int b = tid; b < P.n_blocks; b += kBlock)
+    for (int c = 0; c < 4; ++c) v[c] += __builtin_nontemporal_load(&P.part_s[(size_t)b * 4 + c]);
+  double out[4];
+  for (int c = 0; c < 4; ++c) {
+    s_red[tid] = v[c];
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+      if (tid < w) s_red[tid] += s_red[tid + w];
+      __syncthreads();
+    }
+    out[c] = s_red[0];
+    __syncthreads();
+  }
+  if (tid == 0) {
+    for (int c = 0; c < 4; ++c) st->bcde[c] = out[c];
+    st->blocks_done_step = 0;
+    controller(st, kc, P);
+  }
+}
+
+// Initial state of a problem: pose from a row-major float 4x4, maps, level.
+__global__ void init_state_kernel(const AssocProblem* __restrict__ probs, int n_probs, const float* __restrict__ T16,
+                                  int stride16, KConsts kc, float ell0, int K0) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n_probs) return;
+  const AssocProblem& P = probs[p];
+  ProblemState* st = P.st;
+  const float* T = T16 + (size_t)p * stride16;
+  for (int r = 0; r < 3; ++r) {
+    for (int c = 0; c < 3; ++c) st->R[r * 3 + c] = (double)T[r * 4 + c];
+    st->T[r] = (double)T[r * 4 + 3];
+  }
+  for (int i = 0; i < 9; ++i) st->As[i] = (i % 4 == 0) ? 1.0f : 0.0f;
+  for (int i = 0; i < 3; ++i) st->bs[i] = 0.0f;
+  st->has_src_xf = 0;
+  for (int c = 0; c < 6; ++c) st->flow[c] = 0.0;
+  for (int c = 0; c < 4; ++c) st->bcde[c] = 0.0;
+  st->ip = 0.0;
+  st->step = 0.0;
+  st->dist = 0.0;
+  st->nnz = 0;
+  st->ell = ell0;
+  st->K = K0;
+  st->iter = 0;
+  st->done = 0;
+  st->ret = 0;
+  st->iters_out = 0;
+  st->max_row = 0;
+  st->n_overflow = 0;
+  st->blocks_done_assoc = st->blocks_done_ovf = st->blocks_done_step = 0;
+  st->qs_head = st->qs_size = st->qe_head = st->qe_size = 0;
+  st->ss = st->se = 0.0f;
+  set_maps(st);
+  st->level = choose_level(P.grid, needed_radius(kc, st, ell0));
+}
+
+// ---------------------------------------------------------------------------
+// Association export: CSR in input row order, target ids ascending per row.
+// ---------------------------------------------------------------------------
+
+__global__ void export_counts(const AssocProblem* __restrict__ P, int64_t* __restrict__ counts_orig) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= P->n_src) return;
+  uint32_t orig = __float_as_uint(P->src_pos[r].w);
+  counts_orig[orig] = (int64_t)P->counts[r];
+}
+
+// One 64-lane wave per row: gathers (target id, a), bitonic sort by id in LDS.
+__global__ __launch_bounds__(64) void export_rows(const AssocProblem* __restrict__ P, const int64_t* __restrict__ row_ptr,
+                                                  int32_t* __restrict__ col, float* __restrict__ val) {
+  __shared__ uint32_t s_id[1024];
+  __shared__ float s_v[1024];
+  const int r = blockIdx.x;
+  if (r >= P->n_src) return;
+  const GridLevel& G = P->grid->lv[P->st->level_assoc];
+  const uint32_t cnt = P->counts[r];
+  const uint32_t orig = __float_as_uint(P->src_pos[r].w);
+  uint32_t n2 = 1;
+  while (n2 < cnt) n2 <<= 1;
+  for (uint32_t s = threadIdx.x; s < n2; s += 64) {
+    if (s < cnt) {
+      uint2 e = P->pairs[(size_t)r * P->cap_alloc + s];
+      s_id[s] = __float_as_uint(G.pos[e.x].w);
+      s_v[s] = __uint_as_float(e.y);
+    } else {
+      s_id[s] = 0xffffffffu;
+      s_v[s] = 0.0f;
+    }
+  }
+  __syncthreads();
+  for (uint32_t k = 2; k <= n2; k <<= 1) {
+    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+      for (uint32_t i = threadIdx.x; i < n2; i += 64) {
+        uint32_t ixj = i ^ j;
+        if (ixj > i) {
+          bool up = (i & k) == 0;
+          uint32_t a = s_id[i], b = s_id[ixj];
+          if ((a > b) == up) {
+            s_id[i] = b;
+            s_id[ixj] = a;
+            float t = s_v[i];
+            s_v[i] = s_v[ixj];
+            s_v[ixj] = t;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  const int64_t base = row_ptr[orig];
+  for (uint32_t s = threadIdx.x; s < cnt; s += 64) {
+    col[base + s] = (int32_t)s_id[s];
+    val[base + s] = s_v[s];
+  }
+}
+
+// All-done flag of a set of problems (one block), for the host's chunk loop.
+__global__ void all_done_kernel(const AssocProblem* __restrict__ probs, int n_probs, int32_t* __restrict__ out) {
+  __shared__ int s_any;
+  if (threadIdx.x == 0) s_any = 0;
+  __syncthreads();
+  for (int q = threadIdx.x; q < n_probs; q += blockDim.x)
+    if (!probs[q].st->done) s_any = 1;
+  __syncthreads();
+  if (threadIdx.x == 0) *out = s_any ? 0 : 1;
+}
+
+// ---------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------
+
+void launch_all_done(const AssocProblem* probs, int n_probs, int32_t* out, hipStream_t s) {
+  hipLaunchKernelGGL(all_done_kernel, dim3(1), dim3(256), 0, s, probs, n_probs, out);
+}
+
+template <int FC, int KC>
+static void launch_assoc_t(const AssocProblem* probs, int n_probs, int total_blocks, const KConsts& kc,
+                           hipStream_t s) {
+  hipLaunchKernelGGL((assoc_kernel<FC, KC>), dim3(total_blocks), dim3(kBlock), 0, s, probs, n_probs, kc);
+}
+
+template <int FC, int KC>
+static void launch_ovf_t(const AssocProblem* probs, int n_probs, const KConsts& kc, hipStream_t s) {
+  hipLaunchKernelGGL((overflow_kernel<FC, KC>), dim3(kOvfBlocks, n_probs), dim3(kBlock), 0, s, probs, n_probs, kc);
+}
+
+static int fsel(const KConsts& kc) {
+  if (!kc.use_col) return 0;
+  return kc.F == 5 ? 5 : -1;
+}
+static int ksel(const KConsts& kc) {
+  if (!kc.use_sem) return 0;
+  return kc.Ks == 19 ? 19 : -1;
+}
+
+#define CVO_DISPATCH_FK(FN, ...)                              \
+  do {                                                        \
+    int f_ = fsel(kc), k_ = ksel(kc);                         \
+    if (f_ == 0 && k_ == 0) FN<0, 0>(__VA_ARGS__);            \
+    else if (f_ == 0 && k_ == 19) FN<0, 19>(__VA_ARGS__);     \
+    else if (f_ == 0) FN<0, -1>(__VA_ARGS__);                 \
+    else if (f_ == 5 && k_ == 0) FN<5, 0>(__VA_ARGS__);       \
+    else if (f_ == 5 && k_ == 19) FN<5, 19>(__VA_ARGS__);     \
+    else if (f_ == 5) FN<5, -1>(__VA_ARGS__);                 \
+    else if (k_ == 0) FN<-1, 0>(__VA_ARGS__);                 \
+    else if (k_ == 19) FN<-1, 19>(__VA_ARGS__);               \
+    else FN<-1, -1>(__VA_ARGS__);                             \
+  } while (0)
+
+void launch_assoc(const AssocProblem* probs, int n_probs, int total_blocks, const KConsts& kc, hipStream_t s) {
+  CVO_DISPATCH_FK(launch_assoc_t, probs, n_probs, total_blocks, kc, s);
+}
+
+void launch_overflow(const AssocProblem* probs, int n_probs, const KConsts& kc, hipStream_t s) {
+  CVO_DISPATCH_FK(launch_ovf_t, probs, n_probs, kc, s);
+}
+
+void launch_step(const AssocProblem* probs, int n_probs, int total_blocks, const KConsts& kc, hipStream_t s) {
+  hipLaunchKernelGGL(step_kernel, dim3(total_blocks), dim3(kBlock), 0, s, probs, n_probs, kc);
+}
+
+void launch_init_state(const AssocProblem* probs, int n_probs, const float* T16, int stride16, const KConsts& kc,
+                       float ell0, int K0, hipStream_t s) {
+  hipLaunchKernelGGL(init_state_kernel, dim3((n_probs + 63) / 64), dim3(64), 0, s, probs, n_probs, T16, stride16, kc,
+                     ell0, K0);
+}
+
+void launch_grid_count(const float* xyz, int n, const GridLevel& L, uint32_t* key, uint32_t* count, hipStream_t s) {
+  hipLaunchKernelGGL(grid_cell_count, dim3((n + 255) / 256), dim3(256), 0, s, xyz, n, L.ox, L.oy, L.oz, L.inv_h, L.nx,
+                     L.ny, L.nz, key, count);
+}
+
+void launch_iota(uint32_t* v, int n, hipStream_t s) {
+  hipLaunchKernelGGL(iota_u32, dim3((n + 255) / 256), dim3(256), 0, s, v, n);
+}
+
+void launch_gather(const uint32_t* order, int n, const float* xyz, const float* feat, int F, const float* sem, int K,
+                   const float* geo, float4* pos, float* feat_s, float* sem_s, float* geo_s, uint32_t* inv_perm,
+                   hipStream_t s) {
+  hipLaunchKernelGGL(gather_sorted, dim3((n + 255) / 256), dim3(256), 0, s, order, n, xyz, feat, F, sem, K, geo, pos,
+                     feat_s, sem_s, geo_s, inv_perm);
+}
+
+void launch_morton(const float* xyz, int n, const float* bmin, const float* bmax, uint32_t* key, hipStream_t s) {
+  float sc[3];
+  for (int d = 0; d < 3; ++d) {
+    float ext = bmax[d] - bmin[d];
+    sc[d] = ext > 0 ? 1023.0f / ext : 0.0f;
+  }
+  hipLaunchKernelGGL(morton_keys, dim3((n + 255) / 256), dim3(256), 0, s, xyz, n, bmin[0], bmin[1], bmin[2], sc[0],
+                     sc[1], sc[2], key);
+}
+
+void launch_export(const AssocProblem* P_dev, int n_src, int64_t* counts_orig, const int64_t* row_ptr, int32_t* col,
+                   float* val, int phase, hipStream_t s) {
+  if (phase == 0)
+    hipLaunchKernelGGL(export_counts, dim3((n_src + 255) / 256), dim3(256), 0, s, P_dev, counts_orig);
+  else
+    hipLaunchKernelGGL(export_rows, dim3(n_src), dim3(64), 0, s, P_dev, row_ptr, col, val);
+}
+
+}  // namespace cvo

@@ -1,0 +1,298 @@
+"""Python host mirror of the reference ``cvo::CvoGPU`` interface over the C ABI.
+
+The methods keep the reference's names, argument order and meaning
+(include/UnifiedCvo/cvo/CvoGPU.hpp:43-274).  All compute runs in
+``librkhs_ba_amd.so`` (HIP, gfx950); there is no CPU fallback: when the library
+or a GPU is missing the calls raise ``CvoError``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+from ._abi import (Assoc, CvoParams, HeldView, IterRecord, KernelStats, KSTAT_NAMES, mat16)
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "librkhs_ba_amd.so")
+_lib = None
+
+STATUS = {0: "ok", 1: "invalid argument", 2: "empty point cloud", 3: "file could not be opened",
+          4: "YAML value could not be parsed", 5: "HIP runtime error", 6: "no HIP device",
+          7: "out of device memory", 8: "unsupported option", 9: "size beyond a documented limit"}
+
+
+class CvoError(RuntimeError):
+    def __init__(self, status: int, msg: str = ""):
+        self.status = status
+        super().__init__(f"{STATUS.get(status, status)}{': ' + msg if msg else ''}")
+
+
+def lib() -> C.CDLL:
+    """Loads librkhs_ba_amd.so (raises when it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise CvoError(5, f"{LIB_PATH} is missing; run __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        P, i32, f, d = C.c_void_p, C.c_int32, C.c_float, C.c_double
+        L.cvo_status_string.restype = C.c_char_p
+        L.cvo_last_error.restype = C.c_char_p
+        L.cvo_last_error.argtypes = [P]
+        L.cvo_version.restype = C.c_char_p
+        L.cvo_cloud_size.restype = i32
+        L.cvo_cloud_size.argtypes = [P]
+        L.cvo_destroy.argtypes = [P]
+        L.cvo_destroy.restype = None
+        L.cvo_cloud_free.argtypes = [P, P]
+        L.cvo_cloud_free.restype = None
+        L.cvo_assoc_free.argtypes = [P]
+        L.cvo_assoc_free.restype = None
+        L.cvo_inner_product.argtypes = [P, P, P, P, f, P]
+        L.cvo_inner_product_d.argtypes = [P, P, P, P, f, P]
+        L.cvo_function_angle.argtypes = [P, P, P, P, f, i32, P]
+        L.cvo_compute_association.argtypes = [P, P, P, P, f, P]
+        L.cvo_align.argtypes = [P, P, P, P, P, P, P, P, P]
+        L.cvo_align_d.argtypes = [P, P, P, P, P, P, P, P, P]
+        _lib = L
+    return _lib
+
+
+def _check(st: int, ctx=None):
+    if st != 0:
+        msg = ""
+        if ctx:
+            msg = lib().cvo_last_error(ctx).decode(errors="replace")
+        raise CvoError(st, msg)
+
+
+def default_params() -> CvoParams:
+    """CvoParams::CvoParams() defaults (CvoParams.hpp:80-136)."""
+    p = CvoParams()
+    _check(lib().cvo_params_default(C.byref(p)))
+    return p
+
+
+def read_params_yaml(path: str, base: Optional[CvoParams] = None) -> CvoParams:
+    """read_CvoParams_yaml (CvoParams.hpp:203-323) on top of ``base`` (defaults)."""
+    p = default_params() if base is None else base.copy()
+    _check(lib().cvo_params_read_yaml(os.fsencode(path), C.byref(p)))
+    return p
+
+
+@dataclass
+class Association:
+    """cvo::Association (Association.hpp:7-14) as CSR: row_ptr (N+1), col, val.
+    source_inliers / target_inliers follow gpu_association_to_cpu
+    (CvoGPU_impl.cu:370-431) without its duplicate-order nondeterminism."""
+    rows: int
+    cols: int
+    row_ptr: np.ndarray
+    col: np.ndarray
+    val: np.ndarray
+
+    @property
+    def nnz(self) -> int:
+        return int(self.row_ptr[-1]) if self.row_ptr.size else 0
+
+    @property
+    def source_inliers(self) -> np.ndarray:
+        return np.nonzero(np.diff(self.row_ptr) > 0)[0]
+
+    @property
+    def target_inliers(self) -> np.ndarray:
+        return self.col.copy()
+
+    def pairs(self):
+        """(row, col, val) triplets."""
+        rows = np.repeat(np.arange(self.rows), np.diff(self.row_ptr))
+        return rows, self.col, self.val
+
+
+@dataclass
+class CvoResultInfo:
+    """CvoResultInfo (CvoGPU.hpp:34-40)."""
+    T_s2t: np.ndarray = field(default_factory=lambda: np.eye(4, dtype=np.float32))
+    num_iters: int = 0
+    registration_seconds: float = 0.0
+    return_code: int = 0
+    association: Optional[Association] = None
+
+
+def _take_assoc(a: Assoc) -> Association:
+    n = a.rows
+    nnz = int(a.nnz)
+    if a.row_ptr:
+        row_ptr = np.ctypeslib.as_array(a.row_ptr, shape=(n + 1,)).copy()
+        col = np.ctypeslib.as_array(a.col, shape=(max(nnz, 1),))[:nnz].copy()
+        val = np.ctypeslib.as_array(a.val, shape=(max(nnz, 1),))[:nnz].copy()
+    else:
+        row_ptr = np.zeros(n + 1, np.int64)
+        col = np.zeros(0, np.int32)
+        val = np.zeros(0, np.float32)
+    lib().cvo_assoc_free(C.byref(a))
+    return Association(n, int(a.cols), row_ptr, col, val)
+
+
+class DeviceCloud:
+    """A cloud resident in device memory (cvo_cloud_upload)."""
+
+    def __init__(self, owner: "CvoGPU", cloud):
+        self._owner = owner
+        self._held = HeldView(cloud)
+        self.handle = C.c_void_p()
+        _check(lib().cvo_cloud_upload(owner._ctx, self._held.ref, C.byref(self.handle)), owner._ctx)
+        self.n = self._held.view.n
+
+    def free(self):
+        if self.handle and self._owner._ctx:
+            lib().cvo_cloud_free(self._owner._ctx, self.handle)
+        self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class CvoGPU:
+    """cvo::CvoGPU (CvoGPU.hpp:43-274) on one MI355X."""
+
+    def __init__(self, param_file: Optional[str] = None, params: Optional[CvoParams] = None):
+        self._ctx = C.c_void_p()
+        if params is not None:
+            _check(lib().cvo_create_from_params(C.byref(params), C.byref(self._ctx)))
+        else:
+            _check(lib().cvo_create(os.fsencode(param_file) if param_file else None, C.byref(self._ctx)))
+
+    def close(self):
+        if self._ctx:
+            lib().cvo_destroy(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- params -------------------------------------------------------------
+    def get_params(self) -> CvoParams:
+        p = CvoParams()
+        _check(lib().cvo_get_params(self._ctx, C.byref(p)))
+        return p
+
+    def write_params(self, p: CvoParams) -> None:
+        _check(lib().cvo_set_params(self._ctx, C.byref(p)), self._ctx)
+
+    # -- device clouds ------------------------------------------------------
+    def upload(self, cloud) -> DeviceCloud:
+        return DeviceCloud(self, cloud)
+
+    # -- two-view -----------------------------------------------------------
+    def align(self, source, target, init_guess_transform, association: bool = False):
+        """int align(source, target, init_guess_T, transform, association,
+        registration_seconds, num_iters) (CvoGPU.cu:1516-1563).  Returns
+        (ret, transform 4x4, seconds, iters, association-or-None)."""
+        T0 = mat16(init_guess_transform)
+        Tout = np.zeros(16, np.float32)
+        secs = C.c_double()
+        iters = C.c_int32()
+        ret = C.c_int32()
+        a = Assoc()
+        if isinstance(source, DeviceCloud):
+            st = lib().cvo_align_d(self._ctx, source.handle, target.handle, T0.ctypes.data, Tout.ctypes.data,
+                                   C.byref(a) if association else None, C.byref(secs), C.byref(iters), C.byref(ret))
+        else:
+            hs, ht = HeldView(source), HeldView(target)
+            st = lib().cvo_align(self._ctx, hs.ref, ht.ref, T0.ctypes.data, Tout.ctypes.data,
+                                 C.byref(a) if association else None, C.byref(secs), C.byref(iters), C.byref(ret))
+        if st == 2:  # empty input: the reference returns 0 and leaves transform
+            return 0, np.eye(4, dtype=np.float32), 0.0, 0, None
+        _check(st, self._ctx)
+        assoc = _take_assoc(a) if (association and a.row_ptr) else None
+        return ret.value, Tout.reshape(4, 4), secs.value, iters.value, assoc
+
+    def align_result(self, source, target, init_guess_T_s2t, is_returning_association: bool = False) -> CvoResultInfo:
+        """CvoResultInfo align(source, target, T_init_s2t, is_returning_association)
+        (CvoGPU.cu:1566-1611): inverts its init; -1 on empty input."""
+        res = CvoResultInfo()
+        if getattr(source, "n", 0) == 0 or getattr(target, "n", 0) == 0:
+            res.return_code = -1
+            return res
+        T0 = np.linalg.inv(np.asarray(init_guess_T_s2t, np.float64)).astype(np.float32)
+        ret, T, secs, iters, assoc = self.align(source, target, T0, association=is_returning_association)
+        res.T_s2t, res.num_iters, res.registration_seconds, res.return_code = T, iters, secs, ret
+        res.association = assoc
+        return res
+
+    def inner_product_gpu(self, source, target, init_guess_transform, ell: float) -> float:
+        """inner_product_gpu (CvoGPU.cu:1734-1748)."""
+        T0 = mat16(init_guess_transform)
+        ip = C.c_double()
+        if isinstance(source, DeviceCloud):
+            st = lib().cvo_inner_product_d(self._ctx, source.handle, target.handle, T0.ctypes.data, C.c_float(ell),
+                                           C.byref(ip))
+        else:
+            hs, ht = HeldView(source), HeldView(target)
+            st = lib().cvo_inner_product(self._ctx, hs.ref, ht.ref, T0.ctypes.data, C.c_float(ell), C.byref(ip))
+        if st == 2:
+            return 0.0
+        _check(st, self._ctx)
+        return ip.value
+
+    def function_angle(self, source, target, t2s_frame_transform, ell: float, is_approximate: bool = True,
+                       is_gpu: bool = True) -> float:
+        """function_angle (CvoGPU.cu:1768-1806).  is_gpu is accepted for
+        signature parity; the computation always runs on the GPU."""
+        T0 = mat16(t2s_frame_transform)
+        hs, ht = HeldView(source), HeldView(target)
+        out = C.c_double()
+        _check(lib().cvo_function_angle(self._ctx, hs.ref, ht.ref, T0.ctypes.data, C.c_float(ell),
+                                        int(bool(is_approximate)), C.byref(out)), self._ctx)
+        return out.value
+
+    def compute_association_gpu(self, source, target, T_target_frame_to_source_frame, lengthscale: float) -> Association:
+        """compute_association_gpu (CvoGPU.cu:1836-1871)."""
+        T0 = mat16(T_target_frame_to_source_frame)
+        hs, ht = HeldView(source), HeldView(target)
+        a = Assoc()
+        _check(lib().cvo_compute_association(self._ctx, hs.ref, ht.ref, T0.ctypes.data, C.c_float(lengthscale),
+                                             C.byref(a)), self._ctx)
+        if not a.row_ptr:
+            n = hs.view.n
+            return Association(n, ht.view.n, np.zeros(n + 1, np.int64), np.zeros(0, np.int32), np.zeros(0, np.float32))
+        return _take_assoc(a)
+
+    # -- instrumentation ----------------------------------------------------
+    def set_trace(self, cap: int) -> None:
+        _check(lib().cvo_set_trace(self._ctx, int(cap)), self._ctx)
+
+    def get_trace(self) -> List[IterRecord]:
+        n = C.c_int32()
+        _check(lib().cvo_get_trace(self._ctx, None, 0, C.byref(n)), self._ctx)
+        buf = (IterRecord * max(1, n.value))()
+        _check(lib().cvo_get_trace(self._ctx, buf, n.value, C.byref(n)), self._ctx)
+        return [buf[i] for i in range(n.value)]
+
+    def set_profiling(self, on: bool) -> None:
+        _check(lib().cvo_set_profiling(self._ctx, int(bool(on))), self._ctx)
+
+    def kernel_stats(self) -> dict:
+        s = KernelStats()
+        _check(lib().cvo_get_kernel_stats(self._ctx, C.byref(s)), self._ctx)
+        return {"ms": {k: s.ms[i] for i, k in enumerate(KSTAT_NAMES)},
+                "launches": {k: s.launches[i] for i, k in enumerate(KSTAT_NAMES)},
+                "bytes": {k: s.bytes[i] for i, k in enumerate(KSTAT_NAMES)},
+                "pairs_stored": s.pairs_stored, "candidates": s.candidates}
+
+    def reset_kernel_stats(self) -> None:
+        _check(lib().cvo_reset_kernel_stats(self._ctx), self._ctx)
+
+
+def version() -> str:
+    return lib().cvo_version().decode()

@@ -1,0 +1,205 @@
+"""GPU parity of the two-view path against the CPU oracle (same seeded inputs).
+
+Bars (written in each test):
+  * association: identical (row, target) sets except pairs whose oracle value is
+    within 1e-6 relative of sp_thres or whose distance is within 1e-6 relative
+    of a gate; stored values within 4e-6 relative (one FP32 exp of the summed
+    exponent vs the reference's three FP64 exps rounded to float);
+  * inner product: 1e-5 relative of the FP64 oracle;
+  * align: final pose against the FP32-mirror oracle and the generating pose.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import CONFIGS, params_from_yaml, rot_err_deg, trans_err
+from rkhs_ba_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def compare_assoc(oracle, params, src, tgt, T, ell, gpu_assoc, K=None, val_rtol=4e-6):
+    K = K or params.nearest_neighbors_max
+    counts, cols, vals, _ = oracle.assoc(params, src, tgt, T, ell, K)
+    want = {}
+    for i in range(src.n):
+        for s in range(counts[i]):
+            want[(i, int(cols[i, s]))] = float(vals[i, s])
+    rows, gcol, gval = gpu_assoc.pairs()
+    got = {(int(r), int(c)): float(v) for r, c, v in zip(rows, gcol, gval)}
+    diff = set(got) ^ set(want)
+    bad = []
+    for (i, j) in diff:
+        a, margin = oracle.pair_value(params, src, tgt, T, ell, i, j)
+        near_sp = a > 0 and abs(a - params.sp_thres) <= 1e-6 * params.sp_thres
+        if not (near_sp or margin <= 1e-6):
+            bad.append((i, j, a, margin, (i, j) in got))
+    # rows that hit the cap keep the first K by target index: a near-cutoff
+    # pair may shift which K are kept; only exempt rows at the cap
+    bad = [b for b in bad if counts[b[0]] < K]
+    assert not bad, bad[:10]
+    common = set(got) & set(want)
+    rel = max((abs(got[k] - want[k]) / want[k] for k in common), default=0.0)
+    assert rel <= val_rtol, rel
+    # the GPU CSR is sorted by target id within each row
+    for i in range(src.n):
+        seg = gcol[gpu_assoc.row_ptr[i]:gpu_assoc.row_ptr[i + 1]]
+        assert (np.diff(seg) > 0).all()
+    return len(common), len(diff), rel
+
+
+def test_association_planes_spheres_fixed_poses(cvo_gpu, oracle):
+    p = params_from_yaml("planes_spheres_5k.yaml")
+    cvo_gpu.write_params(p)
+    src, tgt, Tgt = synth.planes_spheres_pair(0)
+    rng = np.random.default_rng(1)
+    for ell in (0.5, 0.2, 0.05):
+        for trial in range(2):
+            T = Tgt @ synth.rigid(synth.rot_axis_angle(rng.normal(size=3), rng.uniform(0, 0.05)),
+                                  rng.normal(size=3) * 0.05)
+            say(f"assoc 5k ell={ell} trial={trial}")
+            a = cvo_gpu.compute_association_gpu(src, tgt, T, ell)
+            n_common, n_diff, rel = compare_assoc(oracle, p, src, tgt, T, ell, a)
+            assert n_common > 100
+
+
+def test_association_street_semantics(cvo_gpu, oracle):
+    """19-class semantic gate on a 20k street scene (the 60k layout, subsampled
+    to keep the CPU oracle quick)."""
+    p = params_from_yaml("street_60k.yaml")
+    cvo_gpu.write_params(p)
+    src, tgt, Tgt = synth.street_pair(4, n=20000)
+    for ell in (0.5, 0.1):
+        say(f"assoc street20k ell={ell}")
+        a = cvo_gpu.compute_association_gpu(src, tgt, Tgt, ell)
+        n_common, n_diff, rel = compare_assoc(oracle, p, src, tgt, Tgt, ell, a)
+        assert n_common > 1000
+
+
+def test_association_row_cap_exact_path(cvo_gpu, oracle):
+    """A cap far below the rows' pass counts sends rows through the exact
+    index-ordered path (first K targets by index, CvoGPU.cu:994-996)."""
+    p = params_from_yaml("planes_spheres_5k.yaml")
+    p.nearest_neighbors_max = 6
+    cvo_gpu.write_params(p)
+    src, tgt, Tgt = synth.planes_spheres_pair(6, 3000, 3000)
+    say("assoc cap=6")
+    a = cvo_gpu.compute_association_gpu(src, tgt, Tgt, 0.9)
+    counts = np.diff(a.row_ptr)
+    assert counts.max() == 6 and (counts == 6).sum() > 100
+    compare_assoc(oracle, p, src, tgt, Tgt, 0.9, a, K=6)
+
+
+def test_association_geometric_type_and_no_color(cvo_gpu, oracle):
+    p = params_from_yaml("planes_spheres_5k.yaml")
+    p.is_using_intensity = 0
+    p.is_using_geometric_type = 1
+    cvo_gpu.write_params(p)
+    src, tgt, Tgt = synth.planes_spheres_pair(7, 2000, 2000)
+    rng = np.random.default_rng(3)
+    src.geo_type = rng.uniform(0.05, 1, size=(src.n, 2)).astype(np.float32)
+    tgt.geo_type = rng.uniform(0.05, 1, size=(tgt.n, 2)).astype(np.float32)
+    say("assoc geo-type")
+    a = cvo_gpu.compute_association_gpu(src, tgt, Tgt, 0.4)
+    compare_assoc(oracle, p, src, tgt, Tgt, 0.4, a)
+
+
+def test_inner_product_and_cos_angle(cvo_gpu, oracle):
+    p = params_from_yaml("planes_spheres_5k.yaml")
+    cvo_gpu.write_params(p)
+    src, tgt, Tgt = synth.planes_spheres_pair(0)
+    for ell in (0.5, 0.1):
+        say(f"inner product ell={ell}")
+        ip = cvo_gpu.inner_product_gpu(src, tgt, Tgt, ell)
+        ref = oracle.inner_product(p, src, tgt, Tgt, ell, oracle.FP64)
+        assert ip == pytest.approx(ref, rel=1e-5)
+        c = cvo_gpu.function_angle(src, tgt, Tgt, ell, True)
+        assert c == pytest.approx(ref / np.sqrt(src.n * tgt.n), rel=1e-5)
+    # exact cos of a cloud with itself at identity is 1
+    assert cvo_gpu.function_angle(src, src, np.eye(4), 0.3, False) == pytest.approx(1.0, abs=1e-12)
+    # empty input gives 0 like the reference
+    empty = synth.Cloud(np.zeros((0, 3), np.float32), np.zeros((0, 5), np.float32))
+    assert cvo_gpu.function_angle(src, empty, np.eye(4), 0.3) == 0.0
+
+
+def test_align_planes_spheres_matches_oracle(cvo_gpu, oracle):
+    p = params_from_yaml("planes_spheres_5k.yaml")
+    cvo_gpu.write_params(p)
+    src, tgt, Tgt = synth.planes_spheres_pair(0)
+    say("align 5k gpu")
+    ret, T, secs, iters, _ = cvo_gpu.align(src, tgt, np.eye(4))
+    say(f"  iters={iters} secs={secs:.3f}")
+    say("align 5k oracle (mirror)")
+    To, it_o, ret_o, _ = oracle.align(p, src, tgt, np.eye(4))
+    Tinv = np.linalg.inv(Tgt)
+    assert ret == 0 and iters == it_o == p.MAX_ITER
+    # against the mirror oracle (same algorithm, reference rounding)
+    assert rot_err_deg(T, To) < np.degrees(1e-3) and trans_err(T, To) < 1e-3, (rot_err_deg(T, To), trans_err(T, To))
+    # against the generating motion
+    assert rot_err_deg(T, Tinv) < 0.2 and trans_err(T, Tinv) < 0.01
+
+
+def test_align_is_deterministic_and_graph_equals_eager(cvo_gpu):
+    p = params_from_yaml("planes_spheres_5k.yaml")
+    p.MAX_ITER = 300
+    cvo_gpu.write_params(p)
+    src, tgt, _ = synth.planes_spheres_pair(1, 3000, 3000)
+    say("align graph x2 + eager")
+    r1 = cvo_gpu.align(src, tgt, np.eye(4))
+    r2 = cvo_gpu.align(src, tgt, np.eye(4))
+    cvo_gpu.set_profiling(True)
+    try:
+        r3 = cvo_gpu.align(src, tgt, np.eye(4))
+    finally:
+        cvo_gpu.set_profiling(False)
+    assert np.array_equal(r1[1], r2[1]) and np.array_equal(r1[1], r3[1])
+    assert r1[3] == r2[3] == r3[3]
+
+
+def test_align_trace_and_annealing(cvo_gpu):
+    p = params_from_yaml("planes_spheres_5k.yaml")
+    p.MAX_ITER = 800
+    cvo_gpu.write_params(p)
+    cvo_gpu.set_trace(800)
+    src, tgt, _ = synth.planes_spheres_pair(0)
+    say("align with trace")
+    cvo_gpu.align(src, tgt, np.eye(4))
+    tr = cvo_gpu.get_trace()
+    cvo_gpu.set_trace(0)
+    assert len(tr) >= 700
+    ells = [r.ell for r in tr]
+    assert ells[0] == pytest.approx(0.5) and min(ells) < 0.2
+    assert all(b <= a for a, b in zip(ells, ells[1:]))
+    ks = [r.num_neighbors for r in tr]
+    assert ks[0] == 512 and all(k <= 512 for k in ks)
+    # K_{k+1} = min(nn_max, int(1.2 max_row_k))  (CvoGPU.cu:1431)
+    for a, b in zip(tr, tr[1:]):
+        assert b.num_neighbors == min(512, int(a.max_row * 1.2))
+    assert all(0 < r.step <= p.max_step + 1e-7 for r in tr)
+
+
+def test_align_edge_cases(cvo_gpu):
+    p = params_from_yaml("planes_spheres_5k.yaml")
+    cvo_gpu.write_params(p)
+    src, tgt, _ = synth.planes_spheres_pair(2, 500, 500)
+    empty = synth.Cloud(np.zeros((0, 3), np.float32), np.zeros((0, 5), np.float32))
+    say("align empty")
+    ret, T, secs, iters, _ = cvo_gpu.align(src, empty, np.eye(4))
+    assert ret == 0 and iters == 0
+    res = cvo_gpu.align_result(src, empty, np.eye(4))
+    assert res.return_code == -1
+    say("align disjoint")
+    far = tgt.transformed(synth.rigid(np.eye(3), np.array([500.0, 0, 0])))
+    ret, T, secs, iters, _ = cvo_gpu.align(src, far, np.eye(4))
+    assert ret == -1 and iters == 0
+    # missing features with is_using_intensity raises
+    from rkhs_ba_amd import CvoError
+    bare = synth.Cloud(src.xyz)
+    with pytest.raises(CvoError):
+        cvo_gpu.align(bare, tgt, np.eye(4))
