@@ -36,7 +36,7 @@ def pose_error(T, T_gt):
     return ang, float(np.linalg.norm(E[:3, 3]))
 
 
-def cpu_baseline(params, src, tgt, T_gt, trace, threads, budget_s=20.0):
+def cpu_baseline(params, src, tgt, T_gt, trace, threads, budget_s=12.0):
     """Times the oracle's iteration (FP32-mirror algorithm, OpenMP over rows,
     grid radius search) at each distinct lengthscale the GPU run visited, two
     iterations per lengthscale, and weights by the GPU's iteration count at that
@@ -52,8 +52,11 @@ def cpu_baseline(params, src, tgt, T_gt, trace, threads, budget_s=20.0):
     sampled = 0
     t_start = time.time()
     per_iter = {}
-    for ell, (count, K) in sorted(ells.items(), reverse=True):
-        n_it = 1 if (time.time() - t_start) > budget_s else 2
+    order = sorted(ells.items(), reverse=True)
+    # size the sample to about budget_s seconds from one probe iteration
+    probe = pyoracle.time_iterations(params, src, tgt, T_gt, order[0][0], max(order[0][1][1], 1), 1, threads)
+    n_it = int(min(20, max(2, budget_s / max(1e-3, probe * len(order)))))
+    for ell, (count, K) in order:
         t = pyoracle.time_iterations(params, src, tgt, T_gt, ell, max(K, 1), n_it, threads)
         per_iter[ell] = t
         sampled += n_it

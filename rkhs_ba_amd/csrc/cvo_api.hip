@@ -14,6 +14,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -24,7 +25,7 @@ namespace cvo {
 
 void launch_assoc(const AssocProblem* probs, int n_probs, int total_blocks, const KConsts& kc, hipStream_t s);
 void launch_overflow(const AssocProblem* probs, int n_probs, const KConsts& kc, hipStream_t s);
-void launch_step(const AssocProblem* probs, int n_probs, int total_blocks, const KConsts& kc, hipStream_t s);
+void launch_step(const AssocProblem* probs, int n_probs, int max_src_rows, const KConsts& kc, hipStream_t s);
 void launch_init_state(const AssocProblem* probs, int n_probs, const float* T16, int stride16, const KConsts& kc,
                        float ell0, int K0, hipStream_t s);
 void launch_grid_count(const float* xyz, int n, const GridLevel& L, uint32_t* key, uint32_t* count, hipStream_t s);
@@ -36,6 +37,8 @@ void launch_morton(const float* xyz, int n, const float* bmin, const float* bmax
 void launch_export(const AssocProblem* P_dev, int n_src, int64_t* counts_orig, const int64_t* row_ptr, int32_t* col,
                    float* val, int phase, hipStream_t s);
 void launch_all_done(const AssocProblem* probs, int n_probs, int32_t* out, hipStream_t s);
+void launch_pass_totals(const AssocProblem* probs, int n_probs, hipStream_t s);
+void launch_controller(const AssocProblem* probs, int n_probs, const KConsts& kc, hipStream_t s);
 
 cvo_status set_err(cvo_ctx* ctx, cvo_status s, const std::string& msg) {
   if (ctx) ctx->last_error = msg;
@@ -292,6 +295,7 @@ static std::vector<float> level_sizes(const KConsts& kc, float ell_hi, float ell
 struct Problems {
   int n_probs = 0;
   int total_blocks = 0;
+  int max_rows = 0;  // largest source row count over the problems
   AssocProblem* dev = nullptr;
   std::vector<AssocProblem> host;
   ProblemState* states = nullptr;
@@ -306,13 +310,14 @@ static cvo_status setup_problems(cvo_ctx* ctx, const cvo_dcloud* s, const cvo_dc
   const int nb = (n + kRowsPerBlock - 1) / kRowsPerBlock;
   pr.n_probs = n_probs;
   pr.total_blocks = nb * n_probs;
+  pr.max_rows = n;
   CVO_TRY(ensure(ctx, ctx->b_pairs, sizeof(uint2) * (size_t)n * cap_alloc * n_probs));
   CVO_TRY(ensure(ctx, ctx->b_counts, sizeof(uint32_t) * (size_t)n * n_probs));
   CVO_TRY(ensure(ctx, ctx->b_ovf, sizeof(uint32_t) * (size_t)n * n_probs));
   CVO_TRY(ensure(ctx, ctx->b_state, sizeof(ProblemState) * n_probs));
   CVO_TRY(ensure(ctx, ctx->b_probs, sizeof(AssocProblem) * n_probs));
   CVO_TRY(ensure(ctx, ctx->b_part_a, sizeof(double) * 8 * (size_t)nb * n_probs));
-  CVO_TRY(ensure(ctx, ctx->b_part_ovf, sizeof(double) * 8 * (size_t)kOvfBlocks * n_probs));
+  CVO_TRY(ensure(ctx, ctx->b_part_ovf, sizeof(double) * 10 * (size_t)kOvfBlocks * n_probs));
   CVO_TRY(ensure(ctx, ctx->b_part_s, sizeof(double) * 4 * (size_t)nb * n_probs));
   CVO_TRY(ensure(ctx, ctx->b_part_cnt, sizeof(uint32_t) * 2 * (size_t)nb * n_probs));
   pr.trace_cap = trace_cap;
@@ -350,7 +355,7 @@ static cvo_status setup_problems(cvo_ctx* ctx, const cvo_dcloud* s, const cvo_dc
     P.n_blocks = nb;
     P.trace_cap = trace_cap;
     P.part_a = as<double>(ctx->b_part_a) + (size_t)q * nb * 8;
-    P.part_ovf = as<double>(ctx->b_part_ovf) + (size_t)q * kOvfBlocks * 8;
+    P.part_ovf = as<double>(ctx->b_part_ovf) + (size_t)q * kOvfBlocks * 10;
     P.part_s = as<double>(ctx->b_part_s) + (size_t)q * nb * 4;
     P.part_cnt = as<uint32_t>(ctx->b_part_cnt) + (size_t)q * nb * 2;
     P.trace = pr.trace ? pr.trace + (size_t)q * trace_cap : nullptr;
@@ -417,8 +422,12 @@ static cvo_status run_loop(cvo_ctx* ctx, const Problems& pr, const KConsts& kc, 
       launch_assoc(pr.dev, pr.n_probs, pr.total_blocks, kc, st);
       CVO_HIP(ctx, hipEventRecord(e[1], st));
       launch_overflow(pr.dev, pr.n_probs, kc, st);
+      if (!with_step) launch_pass_totals(pr.dev, pr.n_probs, st);
       CVO_HIP(ctx, hipEventRecord(e[2], st));
-      if (with_step) launch_step(pr.dev, pr.n_probs, pr.total_blocks, kc, st);
+      if (with_step) {
+        launch_step(pr.dev, pr.n_probs, pr.max_rows, kc, st);
+        launch_controller(pr.dev, pr.n_probs, kc, st);
+      }
       CVO_HIP(ctx, hipEventRecord(e[3], st));
       bool all = true;
       CVO_TRY(read_all_done(&all));
@@ -437,23 +446,33 @@ static cvo_status run_loop(cvo_ctx* ctx, const Problems& pr, const KConsts& kc, 
   } else if (!with_step) {
     launch_assoc(pr.dev, pr.n_probs, pr.total_blocks, kc, st);
     launch_overflow(pr.dev, pr.n_probs, kc, st);
+    launch_pass_totals(pr.dev, pr.n_probs, st);
     CVO_HIP(ctx, hipGetLastError());
   } else {
-    // capture one chunk of iterations into a graph
+    // one chunk of iterations, captured in a graph unless CVO_NO_GRAPH is set
+    // (direct launches, e.g. under a kernel tracer)
+    auto launch_chunk = [&]() {
+      for (int it = 0; it < kChunk; ++it) {
+        launch_assoc(pr.dev, pr.n_probs, pr.total_blocks, kc, st);
+        launch_overflow(pr.dev, pr.n_probs, kc, st);
+        launch_step(pr.dev, pr.n_probs, pr.max_rows, kc, st);
+        launch_controller(pr.dev, pr.n_probs, kc, st);
+      }
+    };
+    const char* ng = std::getenv("CVO_NO_GRAPH");
+    const bool use_graph = !(ng && ng[0] && ng[0] != '0');
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    CVO_HIP(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    for (int it = 0; it < kChunk; ++it) {
-      launch_assoc(pr.dev, pr.n_probs, pr.total_blocks, kc, st);
-      launch_overflow(pr.dev, pr.n_probs, kc, st);
-      launch_step(pr.dev, pr.n_probs, pr.total_blocks, kc, st);
-    }
-    hipError_t ce = hipStreamEndCapture(st, &graph);
-    if (ce != hipSuccess) return hip_check(ctx, ce, "hipStreamEndCapture");
-    hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-    if (ie != hipSuccess) {
-      (void)hipGraphDestroy(graph);
-      return hip_check(ctx, ie, "hipGraphInstantiate");
+    if (use_graph) {
+      CVO_HIP(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      launch_chunk();
+      hipError_t ce = hipStreamEndCapture(st, &graph);
+      if (ce != hipSuccess) return hip_check(ctx, ce, "hipStreamEndCapture");
+      hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      if (ie != hipSuccess) {
+        (void)hipGraphDestroy(graph);
+        return hip_check(ctx, ie, "hipGraphInstantiate");
+      }
     }
     const int n_chunks = (max_iter + kChunk - 1) / kChunk;
     bool finished = false;
@@ -462,8 +481,13 @@ static cvo_status run_loop(cvo_ctx* ctx, const Problems& pr, const KConsts& kc, 
     // Chunk c writes its all-done flag into pinned slot c % 2; the host waits
     // for chunk c-1 while chunk c runs.
     for (int c = 0; c < n_chunks && !finished; ++c) {
-      hipError_t le = hipGraphLaunch(exec, st);
-      if (le != hipSuccess) { status = hip_check(ctx, le, "hipGraphLaunch"); break; }
+      hipError_t le = hipSuccess;
+      if (use_graph) le = hipGraphLaunch(exec, st);
+      else {
+        launch_chunk();
+        le = hipGetLastError();
+      }
+      if (le != hipSuccess) { status = hip_check(ctx, le, "chunk launch"); break; }
       launch_all_done(pr.dev, pr.n_probs, dflag + (c & 1), st);
       hipError_t me = hipMemcpyAsync((void*)(flags + (c & 1)), dflag + (c & 1), sizeof(int32_t),
                                      hipMemcpyDeviceToHost, st);
@@ -477,8 +501,10 @@ static cvo_status run_loop(cvo_ctx* ctx, const Problems& pr, const KConsts& kc, 
       }
     }
     hipError_t se = hipStreamSynchronize(st);
-    (void)hipGraphExecDestroy(exec);
-    (void)hipGraphDestroy(graph);
+    if (use_graph) {
+      (void)hipGraphExecDestroy(exec);
+      (void)hipGraphDestroy(graph);
+    }
     if (status != CVO_OK) return status;
     if (se != hipSuccess) return hip_check(ctx, se, "hipStreamSynchronize");
   }

@@ -51,12 +51,13 @@ __device__ __forceinline__ double shfl_xor_d(double v, int m) { return __shfl_xo
 // float, the reference's rounding points (CvoGPU.cu:1020-1054).  Used only for
 // pairs whose fast FP32 value lies within 2e-5 relative of sp_thres, so the
 // stored set equals the reference's.
-__device__ __noinline__ float exact_value(const KConsts& kc, float d2, float l, float d2c, float d2s, float geo_sim) {
+__device__ __noinline__ float exact_value(float sigma2, float c_sigma2, float s_sigma2, float c2, float s_ell,
+                                         int use_geo, int use_col, int use_sem, float d2, float l, float d2c,
+                                         float d2s, float geo_sim) {
   float k = 1.0f, ck = 1.0f, sk = 1.0f;
-  if (kc.use_geo) k = (float)((double)kc.sigma2 * exp((double)(-d2) / (2.0 * (double)l * (double)l)));
-  if (kc.use_col) ck = (float)((double)kc.c_sigma2 * exp((double)(-d2c) / (2.0 * (double)kc.c2)));
-  if (kc.use_sem)
-    sk = (float)((double)kc.s_sigma2 * exp((double)(-d2s) / (2.0 * (double)kc.s_ell * (double)kc.s_ell)));
+  if (use_geo) k = (float)((double)sigma2 * exp((double)(-d2) / (2.0 * (double)l * (double)l)));
+  if (use_col) ck = (float)((double)c_sigma2 * exp((double)(-d2c) / (2.0 * (double)c2)));
+  if (use_sem) sk = (float)((double)s_sigma2 * exp((double)(-d2s) / (2.0 * (double)s_ell * (double)s_ell)));
   return ck * k * sk * geo_sim;
 }
 
@@ -163,6 +164,93 @@ struct Dim<-1> {
   __device__ static int get(int rt) { return rt; }
 };
 
+// Float maps of one iteration, loaded once per block from the problem state.
+struct Maps {
+  float A[9], b[3];    // target: y' = A y + b
+  float Rq[9], Tq[3];  // query into the target frame
+  float As[9], bs[3];  // source map
+  bool src_xf;
+};
+
+__device__ __forceinline__ void load_maps(const ProblemState* st, Maps& m) {
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    m.A[i] = st->A[i];
+    m.Rq[i] = st->Rq[i];
+    m.As[i] = st->As[i];
+  }
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    m.b[i] = st->b[i];
+    m.Tq[i] = st->Tq[i];
+    m.bs[i] = st->bs[i];
+  }
+  m.src_xf = st->has_src_xf != 0;
+}
+
+// Scalars of the exact path (passed by value: no private copy of KConsts).
+struct ExactK {
+  float sigma2, c_sigma2, s_sigma2, c2, s_ell;
+  int use_geo, use_col, use_sem;
+};
+
+__device__ __forceinline__ ExactK exact_consts(const KConsts& kc) {
+  ExactK e;
+  e.sigma2 = kc.sigma2;
+  e.c_sigma2 = kc.c_sigma2;
+  e.s_sigma2 = kc.s_sigma2;
+  e.c2 = kc.c2;
+  e.s_ell = kc.s_ell;
+  e.use_geo = kc.use_geo;
+  e.use_col = kc.use_col;
+  e.use_sem = kc.use_sem;
+  return e;
+}
+
+// The kernel value of a candidate pair: gates, then the product kernel.
+// Returns false when a gate rejects or a <= sp_thres.
+template <int FC, int KC>
+__device__ __forceinline__ bool pair_kernel(const KConsts& kc, const ExactK& ek, float x0, float x1, float x2,
+                                            float y0, float y1, float y2, float d2t, float qg, float l,
+                                            const float* xf, const float* yf, const float* xs, const float* ys,
+                                            const float* xg, const float* yg, float& a) {
+  const float dx = y0 - x0, dy = y1 - x1, dz = y2 - x2;
+  const float d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+  if (kc.use_geo && !(d2 < d2t)) return false;
+  float geo_sim = 1.0f;
+  if (kc.use_gtype) {
+    geo_sim = geo_similarity(xg, yg);
+    if (geo_sim < 0.01f) return false;
+  }
+  float d2c = 0.0f, d2s = 0.0f;
+  if (FC != 0) {
+    const int F = Dim<FC>::get(kc.F);
+#pragma unroll
+    for (int f = 0; f < F; ++f) {
+      float t = xf[f] - yf[f];
+      d2c = fmaf(t, t, d2c);
+    }
+    if (!(d2c < kc.d2_c_thres)) return false;
+  }
+  if (KC != 0) {
+    const int KS = Dim<KC>::get(kc.Ks);
+#pragma unroll
+    for (int k = 0; k < KS; ++k) {
+      float t = xs[k] - ys[k];
+      d2s = fmaf(t, t, d2s);
+    }
+    if (!(d2s < kc.d2_s_thres)) return false;
+  }
+  float ex = kc.use_geo ? d2 * qg : 0.0f;
+  if (FC != 0) ex = fmaf(d2c, kc.q_c, ex);
+  if (KC != 0) ex = fmaf(d2s, kc.q_s, ex);
+  a = kc.amp * expf(-ex) * geo_sim;
+  if (fabsf(a - kc.sp) <= 2e-5f * kc.sp)
+    a = exact_value(ek.sigma2, ek.c_sigma2, ek.s_sigma2, ek.c2, ek.s_ell, ek.use_geo, ek.use_col, ek.use_sem, d2, l,
+                    d2c, d2s, geo_sim);
+  return a > kc.sp;
+}
+
 struct RowResult {
   double acc[7];  // omega(3), v(3), ip
   uint32_t stored;
@@ -173,34 +261,33 @@ struct RowResult {
 // gate is on (0 off, -1 runtime).  MODE_ALIGN accumulates the flow, MODE_IP and
 // MODE_BA only the sum of a.
 template <int FC, int KC>
-__device__ void assoc_row(const AssocProblem& P, const ProblemState* st, const KConsts& kc, const GridLevel& G,
-                          int r, int g, int mode, float ell, int K, const float* A, const float* bt, const float* Rq,
-                          const float* Tq, bool src_xf, const float* As, const float* bs, RowResult& out) {
+__device__ __forceinline__ void assoc_row(const AssocProblem& P, const KConsts& kc, const ExactK& ek,
+                                          const GridLevel& G, const Maps& mp, int r, int g, int mode, float ell,
+                                          int K, const float* xf, const float* xs, RowResult& out) {
+#pragma unroll
   for (int c = 0; c < 7; ++c) out.acc[c] = 0.0;
   out.stored = 0;
   out.overflow = false;
+  const uint32_t Ku = (uint32_t)K;
+  if (!kc.use_geo) {
+    // no distance gate: every target is a candidate, so the row takes the
+    // exact full scan of the capped-row kernel
+    out.stored = Ku + 1u;
+    out.overflow = true;
+    return;
+  }
   const int F = Dim<FC>::get(kc.F);
   const int KS = Dim<KC>::get(kc.Ks);
   float4 x4 = P.src_pos[r];
   float x0 = x4.x, x1 = x4.y, x2 = x4.z;
-  if (src_xf) {
+  if (mp.src_xf) {
     float t0, t1, t2;
-    affine3(As, bs, x0, x1, x2, t0, t1, t2);
+    affine3(mp.As, mp.bs, x0, x1, x2, t0, t1, t2);
     x0 = t0; x1 = t1; x2 = t2;
   }
   const float l = range_ell(ell, x0, x1, x2);
-  const float d2t = kc.use_geo ? (float)(-2.0 * (double)l * (double)l * (double)kc.lg) : 1.0f;
+  const float d2t = (float)(-2.0 * (double)l * (double)l * (double)kc.lg);
   const float qg = (float)(1.0 / (2.0 * (double)l * (double)l));
-  // source features in registers when the dimension is static
-  constexpr int FR = FC > 0 ? FC : 1;
-  constexpr int KR = KC > 0 ? KC : 1;
-  float xf[FR], xs[KR];
-  const float* xf_g = P.src_feat ? P.src_feat + (size_t)r * kc.F : nullptr;
-  const float* xs_g = P.src_sem ? P.src_sem + (size_t)r * kc.Ks : nullptr;
-  if (FC > 0)
-    for (int f = 0; f < FR; ++f) xf[f] = xf_g[f];
-  if (KC > 0)
-    for (int k = 0; k < KR; ++k) xs[k] = xs_g[k];
   float xg[2] = {0.0f, 0.0f};
   if (kc.use_gtype) {
     xg[0] = P.src_geo[2 * (size_t)r];
@@ -208,19 +295,12 @@ __device__ void assoc_row(const AssocProblem& P, const ProblemState* st, const K
   }
   // query box in the target frame
   float q0, q1, q2;
-  affine3(Rq, Tq, x0, x1, x2, q0, q1, q2);
+  affine3(mp.Rq, mp.Tq, x0, x1, x2, q0, q1, q2);
   const float rq = sqrtf(d2t) * 1.001f + 1e-4f * (1.0f + fmaxf(fabsf(q0), fmaxf(fabsf(q1), fabsf(q2))));
   int lo0 = cell_floor((q0 - rq - G.ox) * G.inv_h, G.nx), hi0 = cell_floor((q0 + rq - G.ox) * G.inv_h, G.nx);
   int lo1 = cell_floor((q1 - rq - G.oy) * G.inv_h, G.ny), hi1 = cell_floor((q1 + rq - G.oy) * G.inv_h, G.ny);
   int lo2 = cell_floor((q2 - rq - G.oz) * G.inv_h, G.nz), hi2 = cell_floor((q2 + rq - G.oz) * G.inv_h, G.nz);
   int nruns = 0, span1 = 0;
-  if (!kc.use_geo) {
-    // no distance gate: every target is a candidate, so the row takes the
-    // exact full scan of the capped-row kernel
-    out.stored = (uint32_t)K + 1u;
-    out.overflow = true;
-    return;
-  }
   if (!(hi0 < 0 || lo0 >= G.nx || hi1 < 0 || lo1 >= G.ny || hi2 < 0 || lo2 >= G.nz)) {
     lo0 = max(lo0, 0); hi0 = min(hi0, G.nx - 1);
     lo1 = max(lo1, 0); hi1 = min(hi1, G.ny - 1);
@@ -228,15 +308,14 @@ __device__ void assoc_row(const AssocProblem& P, const ProblemState* st, const K
     span1 = hi1 - lo1 + 1;
     nruns = span1 * (hi2 - lo2 + 1);
   }
-  const uint32_t Ku = (uint32_t)K;
   uint32_t stored = 0;
   bool stop = false;
   for (int rbase = 0; rbase < nruns && !stop; rbase += kGroup) {
-    int myrun = rbase + g;
+    const int myrun = rbase + g;
     uint32_t rs = 0, rl = 0;
     if (myrun < nruns) {
-      int iz = myrun / span1, iy = myrun - iz * span1;
-      uint32_t c0 = ((uint32_t)(lo2 + iz) * (uint32_t)G.ny + (uint32_t)(lo1 + iy)) * (uint32_t)G.nx + (uint32_t)lo0;
+      const int iz = myrun / span1, iy = myrun - iz * span1;
+      const uint32_t c0 = ((uint32_t)(lo2 + iz) * (uint32_t)G.ny + (uint32_t)(lo1 + iy)) * (uint32_t)G.nx + (uint32_t)lo0;
       rs = G.cell_start[c0];
       rl = G.cell_start[c0 + (uint32_t)(hi0 - lo0) + 1u] - rs;
     }
@@ -259,47 +338,16 @@ __device__ void assoc_row(const AssocProblem& P, const ProblemState* st, const K
       const uint32_t run_start = shfl16(rs, kk);
       const uint32_t run_excl = shfl16(excl, kk);
       bool pass = false;
-      float a = 0.0f;
-      float y0 = 0.0f, y1 = 0.0f, y2 = 0.0f;
+      float a = 0.0f, y0 = 0.0f, y1 = 0.0f, y2 = 0.0f;
       uint32_t pidx = 0;
       if (idx < total) {
         pidx = run_start + (idx - run_excl);
-        float4 y4 = G.pos[pidx];
-        affine3(A, bt, y4.x, y4.y, y4.z, y0, y1, y2);
-        float dx = y0 - x0, dy = y1 - x1, dz = y2 - x2;
-        float d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-        pass = !kc.use_geo || d2 < d2t;
-        float geo_sim = 1.0f;
-        if (pass && kc.use_gtype) {
-          float gb[2] = {G.geo[2 * (size_t)pidx], G.geo[2 * (size_t)pidx + 1]};
-          geo_sim = geo_similarity(xg, gb);
-          pass = !(geo_sim < 0.01f);
-        }
-        float d2c = 0.0f, d2s = 0.0f;
-        if (FC != 0 && pass) {
-          const float* yf = G.feat + (size_t)pidx * F;
-          for (int f = 0; f < F; ++f) {
-            float t = (FC > 0 ? xf[f < FR ? f : 0] : xf_g[f]) - yf[f];
-            d2c = fmaf(t, t, d2c);
-          }
-          pass = d2c < kc.d2_c_thres;
-        }
-        if (KC != 0 && pass) {
-          const float* ys = G.sem + (size_t)pidx * KS;
-          for (int k = 0; k < KS; ++k) {
-            float t = (KC > 0 ? xs[k < KR ? k : 0] : xs_g[k]) - ys[k];
-            d2s = fmaf(t, t, d2s);
-          }
-          pass = d2s < kc.d2_s_thres;
-        }
-        if (pass) {
-          float ex = kc.use_geo ? d2 * qg : 0.0f;
-          if (FC != 0) ex = fmaf(d2c, kc.q_c, ex);
-          if (KC != 0) ex = fmaf(d2s, kc.q_s, ex);
-          a = kc.amp * expf(-ex) * geo_sim;
-          if (fabsf(a - kc.sp) <= 2e-5f * kc.sp) a = exact_value(kc, d2, l, d2c, d2s, geo_sim);
-          pass = a > kc.sp;
-        }
+        const float4 y4 = G.pos[pidx];
+        affine3(mp.A, mp.b, y4.x, y4.y, y4.z, y0, y1, y2);
+        pass = pair_kernel<FC, KC>(kc, ek, x0, x1, x2, y0, y1, y2, d2t, qg, l, xf,
+                                   FC != 0 ? G.feat + (size_t)pidx * F : nullptr, xs,
+                                   KC != 0 ? G.sem + (size_t)pidx * KS : nullptr, xg,
+                                   kc.use_gtype ? G.geo + 2 * (size_t)pidx : nullptr, a);
       }
       const uint64_t m = __ballot(pass);
       const uint32_t gm = (uint32_t)(m >> (16 * ((threadIdx.x & 63) >> 4))) & 0xFFFFu;
@@ -307,7 +355,7 @@ __device__ void assoc_row(const AssocProblem& P, const ProblemState* st, const K
       if (pass && slot < Ku) {
         P.pairs[(size_t)r * P.cap_alloc + slot] = make_uint2(pidx, __float_as_uint(a));
         if (mode == MODE_ALIGN) {
-          float c0 = x1 * y2 - x2 * y1, c1 = x2 * y0 - x0 * y2, c2 = x0 * y1 - x1 * y0;
+          const float c0 = x1 * y2 - x2 * y1, c1 = x2 * y0 - x0 * y2, c2 = x0 * y1 - x1 * y0;
           out.acc[0] += (double)(c0 * a);
           out.acc[1] += (double)(c1 * a);
           out.acc[2] += (double)(c2 * a);
@@ -328,12 +376,15 @@ __device__ void assoc_row(const AssocProblem& P, const ProblemState* st, const K
   out.overflow = stored > Ku;
 }
 
+// Block partials of the association pass: part_a[block] = flow6, ip, -;
+// part_cnt[block] = stored pairs, max row.  Summed by overflow_kernel (first
+// stage) and the step / controller kernels (second stage).
 template <int FC, int KC>
 __global__ __launch_bounds__(kBlock) void assoc_kernel(const AssocProblem* __restrict__ probs, int n_probs,
                                                        KConsts kc) {
   const int p = find_problem(probs, n_probs, (int)blockIdx.x);
   const AssocProblem& P = probs[p];
-  ProblemState* st = P.st;
+  const ProblemState* st = P.st;
   if (st->done) return;
   const int lb = (int)blockIdx.x - P.block_begin;
   const int tid = threadIdx.x;
@@ -344,20 +395,37 @@ __global__ __launch_bounds__(kBlock) void assoc_kernel(const AssocProblem* __res
   const float ell = st->ell;
   const int K = st->K;
   const int iter = st->iter;
-  float A[9], bt[3], Rq[9], Tq[3], As[9], bs[3];
-  for (int i = 0; i < 9; ++i) { A[i] = st->A[i]; Rq[i] = st->Rq[i]; As[i] = st->As[i]; }
-  for (int i = 0; i < 3; ++i) { bt[i] = st->b[i]; Tq[i] = st->Tq[i]; bs[i] = st->bs[i]; }
-  const bool src_xf = st->has_src_xf != 0;
+  Maps mp;
+  load_maps(st, mp);
+  const ExactK ek = exact_consts(kc);
   const GridLevel& G = P.grid->lv[st->level];
 
   __shared__ double s_row[kRowsPerBlock][8];
   __shared__ uint32_t s_cnt[kRowsPerBlock];
+  // the block's 16 source rows' features and class vectors, staged in LDS
+  constexpr int FL = FC > 0 ? FC : (FC == 0 ? 1 : 64);
+  constexpr int KL = KC > 0 ? KC : (KC == 0 ? 1 : 64);
+  __shared__ float s_feat[kRowsPerBlock * FL];
+  __shared__ float s_sem[kRowsPerBlock * KL];
+  const int row0 = lb * kRowsPerBlock;
+  const int nrows = min(kRowsPerBlock, P.n_src - row0);
+  if (FC != 0) {
+    const int F = Dim<FC>::get(kc.F);
+    for (int t = tid; t < nrows * F; t += kBlock) s_feat[t] = P.src_feat[(size_t)row0 * F + t];
+  }
+  if (KC != 0) {
+    const int KS = Dim<KC>::get(kc.Ks);
+    for (int t = tid; t < nrows * KS; t += kBlock) s_sem[t] = P.src_sem[(size_t)row0 * KS + t];
+  }
+  __syncthreads();
   RowResult res;
+#pragma unroll
   for (int c = 0; c < 7; ++c) res.acc[c] = 0.0;
   res.stored = 0;
   res.overflow = false;
-  if (r < P.n_src) assoc_row<FC, KC>(P, st, kc, G, r, g, mode, ell, K, A, bt, Rq, Tq, src_xf, As, bs, res);
-  // reduce the row's lanes
+  if (r < P.n_src)
+    assoc_row<FC, KC>(P, kc, ek, G, mp, r, g, mode, ell, K, s_feat + grp * Dim<FC>::get(kc.F),
+                      s_sem + grp * Dim<KC>::get(kc.Ks), res);
 #pragma unroll
   for (int c = 0; c < 7; ++c) {
     double v = res.acc[c];
@@ -375,9 +443,10 @@ __global__ __launch_bounds__(kBlock) void assoc_kernel(const AssocProblem* __res
       P.counts[r] = cnt;
       if (res.overflow) {
         P.ovf_stamp[r] = (uint32_t)iter + 1u;
-        atomicAdd(&st->n_overflow, 1u);
+        atomicAdd(&P.st->n_overflow, 1u);
       }
     }
+#pragma unroll
     for (int c = 0; c < 7; ++c) s_row[grp][c] = res.overflow ? 0.0 : res.acc[c];
     s_cnt[grp] = cnt;
   }
@@ -395,78 +464,60 @@ __global__ __launch_bounds__(kBlock) void assoc_kernel(const AssocProblem* __res
     P.part_cnt[(size_t)lb * 2] = nz;
     P.part_cnt[(size_t)lb * 2 + 1] = mx;
   }
-  // last block of this problem reduces the partials in fixed order
-  __shared__ int s_last;
-  __syncthreads();
-  if (tid == 0) {
-    __threadfence();
-    uint32_t prev = __hip_atomic_fetch_add(&st->blocks_done_assoc, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    s_last = (prev == (uint32_t)P.n_blocks - 1u);
-  }
-  __syncthreads();
-  if (!s_last) return;
-  __threadfence();
-  __shared__ double s_red[kBlock];
-  __shared__ uint64_t s_nz[kBlock];
-  __shared__ uint32_t s_mx[kBlock];
-  double v[7] = {0, 0, 0, 0, 0, 0, 0};
-  uint64_t nz = 0;
-  uint32_t mx = 0;
-  for (int b = tid; b < P.n_blocks; b += kBlock) {
-    for (int c = 0; c < 7; ++c) v[c] += __builtin_nontemporal_load(&P.part_a[(size_t)b * 8 + c]);
-    nz += __builtin_nontemporal_load(&P.part_cnt[(size_t)b * 2]);
-    mx = max(mx, __builtin_nontemporal_load(&P.part_cnt[(size_t)b * 2 + 1]));
-  }
-  s_nz[tid] = nz;
-  s_mx[tid] = mx;
-  double out[7];
-  for (int c = 0; c < 7; ++c) {
-    s_red[tid] = v[c];
-    __syncthreads();
-    for (int w = kBlock / 2; w > 0; w >>= 1) {
-      if (tid < w) s_red[tid] += s_red[tid + w];
-      __syncthreads();
-    }
-    out[c] = s_red[0];
-    __syncthreads();
-  }
-  for (int w = kBlock / 2; w > 0; w >>= 1) {
-    if (tid < w) {
-      s_nz[tid] += s_nz[tid + w];
-      s_mx[tid] = max(s_mx[tid], s_mx[tid + w]);
-    }
-    __syncthreads();
-  }
-  if (tid == 0) {
-    for (int c = 0; c < 6; ++c) st->flow[c] = out[c];
-    st->ip = out[6];
-    st->nnz = s_nz[0];
-    st->max_row = s_mx[0];
-    st->blocks_done_assoc = 0;
-    st->level_assoc = st->level;
-  }
 }
 
 // ---------------------------------------------------------------------------
 // Rows whose passing pairs exceed the cap: the reference's exact scan over all
 // targets in ascending index, keeping the first K (CvoGPU.cu:994-996).
+// Block partials part_ovf[block] = flow6, ip, stored, max row.
 // ---------------------------------------------------------------------------
 
 template <int FC, int KC>
 __global__ __launch_bounds__(kBlock) void overflow_kernel(const AssocProblem* __restrict__ probs, int n_probs,
                                                           KConsts kc) {
   const AssocProblem& P = probs[blockIdx.y];
-  ProblemState* st = P.st;
-  if (st->done || st->n_overflow == 0) return;
+  const ProblemState* st = P.st;
+  if (st->done) return;
   const int tid = threadIdx.x;
   const int mode = kc.mode;
+  // (a) this block's slice of the association partials: parallel loads, one
+  // fixed-order LDS tree over all ten columns
+  __shared__ double s_part[10];
+  __shared__ double s_tree[10][64];
+  {
+    const int per = (P.n_blocks + kOvfBlocks - 1) / kOvfBlocks;
+    const int b0 = blockIdx.x * per, b1 = min(P.n_blocks, b0 + per);
+    for (int t = tid; t < 10 * 64; t += kBlock) {
+      const int c = t / 64, i = t - c * 64;
+      double v = 0.0;
+      for (int b = b0 + i; b < b1; b += 64) {
+        if (c < 7) v += P.part_a[(size_t)b * 8 + c];
+        else if (c == 7) v += (double)P.part_cnt[(size_t)b * 2];
+        else if (c == 8) v = fmax(v, (double)P.part_cnt[(size_t)b * 2 + 1]);
+      }
+      s_tree[c][i] = v;
+    }
+    __syncthreads();
+    for (int w = 32; w > 0; w >>= 1) {
+      for (int t = tid; t < 10 * w; t += kBlock) {
+        const int c = t / w, i = t - c * w;
+        s_tree[c][i] = (c == 8) ? fmax(s_tree[c][i], s_tree[c][i + w]) : s_tree[c][i] + s_tree[c][i + w];
+      }
+      __syncthreads();
+    }
+    if (tid < 10) s_part[tid] = s_tree[tid][0];
+  }
+  __syncthreads();
+  if (st->n_overflow == 0) {  // uniform
+    if (tid < 10) P.part_ovf[(size_t)blockIdx.x * 10 + tid] = s_part[tid];
+    return;
+  }
   const float ell = st->ell;
   const uint32_t K = (uint32_t)st->K;
   const uint32_t stamp = (uint32_t)st->iter + 1u;
-  float A[9], bt[3], As[9], bs[3];
-  for (int i = 0; i < 9; ++i) { A[i] = st->A[i]; As[i] = st->As[i]; }
-  for (int i = 0; i < 3; ++i) { bt[i] = st->b[i]; bs[i] = st->bs[i]; }
-  const bool src_xf = st->has_src_xf != 0;
+  Maps mp;
+  load_maps(st, mp);
+  const ExactK ek = exact_consts(kc);
   const GridLevel& G = P.grid->lv[st->level];
   const int F = Dim<FC>::get(kc.F);
   const int KS = Dim<KC>::get(kc.Ks);
@@ -474,14 +525,15 @@ __global__ __launch_bounds__(kBlock) void overflow_kernel(const AssocProblem* __
   const int r0 = blockIdx.x * chunk, r1 = min(P.n_src, r0 + chunk);
   __shared__ uint32_t s_wave[kBlock / 64];
   __shared__ double s_red[kBlock];
-  double blk_acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // flow6, ip, stored pairs
+  double blk_acc[7] = {0, 0, 0, 0, 0, 0, 0};
+  uint32_t blk_nnz = 0, blk_max = 0;
   for (int r = r0; r < r1; ++r) {
     if (P.ovf_stamp[r] != stamp) continue;  // uniform
     float4 x4 = P.src_pos[r];
     float x0 = x4.x, x1 = x4.y, x2 = x4.z;
-    if (src_xf) {
+    if (mp.src_xf) {
       float t0, t1, t2;
-      affine3(As, bs, x0, x1, x2, t0, t1, t2);
+      affine3(mp.As, mp.bs, x0, x1, x2, t0, t1, t2);
       x0 = t0; x1 = t1; x2 = t2;
     }
     const float l = range_ell(ell, x0, x1, x2);
@@ -501,40 +553,12 @@ __global__ __launch_bounds__(kBlock) void overflow_kernel(const AssocProblem* __
       bool pass = false;
       float a = 0.0f, y0 = 0.0f, y1 = 0.0f, y2 = 0.0f;
       if (j < P.n_tgt) {
-        affine3(A, bt, P.tgt_xyz[3 * (size_t)j], P.tgt_xyz[3 * (size_t)j + 1], P.tgt_xyz[3 * (size_t)j + 2], y0, y1, y2);
-        float dx = y0 - x0, dy = y1 - x1, dz = y2 - x2;
-        float d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-        pass = !kc.use_geo || d2 < d2t;
-        float geo_sim = 1.0f;
-        if (pass && kc.use_gtype) {
-          geo_sim = geo_similarity(xg, P.tgt_geo + 2 * (size_t)j);
-          pass = !(geo_sim < 0.01f);
-        }
-        float d2c = 0.0f, d2s = 0.0f;
-        if (FC != 0 && pass) {
-          const float* yf = P.tgt_feat + (size_t)j * F;
-          for (int f = 0; f < F; ++f) {
-            float t = xf[f] - yf[f];
-            d2c = fmaf(t, t, d2c);
-          }
-          pass = d2c < kc.d2_c_thres;
-        }
-        if (KC != 0 && pass) {
-          const float* ys = P.tgt_sem + (size_t)j * KS;
-          for (int k = 0; k < KS; ++k) {
-            float t = xs[k] - ys[k];
-            d2s = fmaf(t, t, d2s);
-          }
-          pass = d2s < kc.d2_s_thres;
-        }
-        if (pass) {
-          float ex = kc.use_geo ? d2 * qg : 0.0f;
-          if (FC != 0) ex = fmaf(d2c, kc.q_c, ex);
-          if (KC != 0) ex = fmaf(d2s, kc.q_s, ex);
-          a = kc.amp * expf(-ex) * geo_sim;
-          if (fabsf(a - kc.sp) <= 2e-5f * kc.sp) a = exact_value(kc, d2, l, d2c, d2s, geo_sim);
-          pass = a > kc.sp;
-        }
+        affine3(mp.A, mp.b, P.tgt_xyz[3 * (size_t)j], P.tgt_xyz[3 * (size_t)j + 1], P.tgt_xyz[3 * (size_t)j + 2],
+                y0, y1, y2);
+        pass = pair_kernel<FC, KC>(kc, ek, x0, x1, x2, y0, y1, y2, d2t, qg, l, xf,
+                                   FC != 0 ? P.tgt_feat + (size_t)j * F : nullptr, xs,
+                                   KC != 0 ? P.tgt_sem + (size_t)j * KS : nullptr, xg,
+                                   kc.use_gtype ? P.tgt_geo + 2 * (size_t)j : nullptr, a);
       }
       // block-wide exclusive prefix of pass flags, in index order
       const uint64_t m = __ballot(pass);
@@ -551,7 +575,7 @@ __global__ __launch_bounds__(kBlock) void overflow_kernel(const AssocProblem* __
       if (pass && slot < K) {
         P.pairs[(size_t)r * P.cap_alloc + slot] = make_uint2(G.inv_perm[j], __float_as_uint(a));
         if (mode == MODE_ALIGN) {
-          float c0 = x1 * y2 - x2 * y1, c1 = x2 * y0 - x0 * y2, c2 = x0 * y1 - x1 * y0;
+          const float c0 = x1 * y2 - x2 * y1, c1 = x2 * y0 - x0 * y2, c2 = x0 * y1 - x1 * y0;
           acc[0] += (double)(c0 * a);
           acc[1] += (double)(c1 * a);
           acc[2] += (double)(c2 * a);
@@ -564,12 +588,10 @@ __global__ __launch_bounds__(kBlock) void overflow_kernel(const AssocProblem* __
       stored += tot;
       __syncthreads();
     }
-    if (tid == 0) {
-      const uint32_t kept = min(stored, K);
-      P.counts[r] = kept;
-      atomicMax(&st->max_row, kept);
-      blk_acc[7] += (double)kept;
-    }
+    const uint32_t kept = min(stored, K);
+    if (tid == 0) P.counts[r] = kept;
+    blk_nnz += kept;
+    blk_max = max(blk_max, kept);
     for (int c = 0; c < 7; ++c) {
       s_red[tid] = acc[c];
       __syncthreads();
@@ -581,26 +603,12 @@ __global__ __launch_bounds__(kBlock) void overflow_kernel(const AssocProblem* __
       __syncthreads();
     }
   }
-  if (tid < 7) P.part_ovf[(size_t)blockIdx.x * 8 + tid] = blk_acc[tid];
-  if (tid == 0) P.part_ovf[(size_t)blockIdx.x * 8 + 7] = blk_acc[7];
-  __shared__ int s_last;
-  __syncthreads();
+  double* out = P.part_ovf + (size_t)blockIdx.x * 10;
+  if (tid < 7) out[tid] = s_part[tid] + blk_acc[tid];
   if (tid == 0) {
-    __threadfence();
-    uint32_t prev = __hip_atomic_fetch_add(&st->blocks_done_ovf, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    s_last = (prev == (uint32_t)kOvfBlocks - 1u);
-  }
-  __syncthreads();
-  if (!s_last) return;
-  __threadfence();
-  if (tid == 0) {
-    double add[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (int b = 0; b < kOvfBlocks; ++b)
-      for (int c = 0; c < 8; ++c) add[c] += __builtin_nontemporal_load(&P.part_ovf[(size_t)b * 8 + c]);
-    for (int c = 0; c < 6; ++c) st->flow[c] += add[c];
-    st->ip += add[6];
-    st->nnz += (uint64_t)add[7];
-    st->blocks_done_ovf = 0;
+    out[7] = s_part[7] + (double)blk_nnz;
+    out[8] = fmax(s_part[8], (double)blk_max);
+    out[9] = 0.0;
   }
 }
 
@@ -797,13 +805,21 @@ __device__ int choose_level(const GridDev* grid, float r_need) {
   return L;
 }
 
+__device__ int choose_level_tab(const float* h, int n_levels, float r_need) {
+  int L = 0;
+  for (int l = 0; l < n_levels; ++l)
+    if (h[l] >= r_need) L = l;
+  return L;
+}
+
 __device__ float needed_radius(const KConsts& kc, const ProblemState* st, float ell) {
   float tq = fmaxf(fabsf(st->Tq[0]), fmaxf(fabsf(st->Tq[1]), fabsf(st->Tq[2])));
   return kc.rq_coef * ell * (1.0f + kc.max_src_norm / 500.0f) * 1.0001f + 1e-4f * (1.0f + kc.max_src_norm + tq) + 1e-6f;
 }
 
 // One controller step of align_impl (CvoGPU.cu:1342-1433), FP64 pose state.
-__device__ void controller(ProblemState* st, const KConsts& kc, const AssocProblem& P) {
+__device__ void controller(ProblemState* st, const KConsts& kc, const AssocProblem& P, const float* level_h,
+                           int n_levels) {
   cvo_iter_record rec;
   const int k = st->iter;
   double fl[6];
@@ -846,7 +862,11 @@ __device__ void controller(ProblemState* st, const KConsts& kc, const AssocProbl
   }
   for (int i = 0; i < 9; ++i) st->R[i] = Rn[i];
   for (int i = 0; i < 3; ++i) st->T[i] = Tn[i];
-  const double dist = se3_log_norm(dR, dT);
+  // ||log(exp(step xi))|| = step ||xi|| exactly while the rotation angle
+  // step*|w| stays below pi; the general log otherwise
+  const double wn = sqrt(xi[0] * xi[0] + xi[1] * xi[1] + xi[2] * xi[2]);
+  const double xn = sqrt(xi[0] * xi[0] + xi[1] * xi[1] + xi[2] * xi[2] + xi[3] * xi[3] + xi[4] * xi[4] + xi[5] * xi[5]);
+  const double dist = (step * wn < 3.0) ? step * xn : se3_log_norm(dR, dT);
   st->dist = dist;
   rec.step = (float)step;
   rec.dist = (float)dist;
@@ -870,12 +890,14 @@ __device__ void controller(ProblemState* st, const KConsts& kc, const AssocProbl
     st->done = 1;
     st->iters_out = kc.max_iter;
   }
-  st->level = choose_level(P.grid, needed_radius(kc, st, st->ell));
+  st->level = choose_level_tab(level_h, n_levels, needed_radius(kc, st, st->ell));
   st->n_overflow = 0;
 }
 
+
 // ---------------------------------------------------------------------------
-// Step terms (compute_step_size_xi + compute_step_size_poly_coeff) + controller
+// Step terms (compute_step_size_xi + compute_step_size_poly_coeff, CvoGPU.cu:
+// 333-462): block partials part_s[block] = B, C, D, E (FP64).
 // ---------------------------------------------------------------------------
 
 __device__ __forceinline__ void cross_f(float a0, float a1, float a2, float b0, float b1, float b2, float& o0,
@@ -885,55 +907,74 @@ __device__ __forceinline__ void cross_f(float a0, float a1, float a2, float b0, 
   o2 = a0 * b1 - a1 * b0;
 }
 
+// Flow of the iteration from the first-stage partials (kOvfBlocks of them),
+// summed in fixed order by every block that needs it.
+__device__ __forceinline__ void reduce_stage1(const AssocProblem& P, double (*s_tree)[128], double out[10]) {
+  const int tid = threadIdx.x;
+  for (int t = tid; t < 10 * 128; t += kBlock) {
+    const int c = t / 128, i = t - c * 128;
+    s_tree[c][i] = (i < kOvfBlocks) ? P.part_ovf[(size_t)i * 10 + c] : 0.0;
+  }
+  __syncthreads();
+  for (int w = 64; w > 0; w >>= 1) {
+    for (int t = tid; t < 10 * w; t += kBlock) {
+      const int c = t / w, i = t - c * w;
+      s_tree[c][i] = (c == 8) ? fmax(s_tree[c][i], s_tree[c][i + w]) : s_tree[c][i] + s_tree[c][i + w];
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int c = 0; c < 10; ++c) out[c] = s_tree[c][0];
+  __syncthreads();
+}
+
+// One thread per source row; grid (row blocks, problems).
 __global__ __launch_bounds__(kBlock) void step_kernel(const AssocProblem* __restrict__ probs, int n_probs,
                                                       KConsts kc) {
-  const int p = find_problem(probs, n_probs, (int)blockIdx.x);
-  const AssocProblem& P = probs[p];
-  ProblemState* st = P.st;
+  const AssocProblem& P = probs[blockIdx.y];
+  const ProblemState* st = P.st;
   if (st->done) return;
-  const int lb = (int)blockIdx.x - P.block_begin;
   const int tid = threadIdx.x;
-  const int g = tid & (kGroup - 1);
-  const int grp = tid >> 4;
-  const int r = lb * kRowsPerBlock + grp;
+  const int r = blockIdx.x * kBlock + tid;
+  const int nsb = (P.n_src + kBlock - 1) / kBlock;
+  if ((int)blockIdx.x >= nsb) return;  // uniform: blocks beyond this problem's rows
+  __shared__ double s_tree[10][128];
+  double tot[10];
+  reduce_stage1(P, s_tree, tot);
   const float ell = st->ell;
   const GridLevel& G = P.grid->lv[st->level];
-  float A[9], bt[3], As[9], bs[3];
-  for (int i = 0; i < 9; ++i) { A[i] = st->A[i]; As[i] = st->As[i]; }
-  for (int i = 0; i < 3; ++i) { bt[i] = st->b[i]; bs[i] = st->bs[i]; }
+  Maps mp;
+  load_maps(st, mp);
   // normalised flow (compute_flow, CvoGPU.cu:2099-2104), used in float
   double n2 = 0.0;
-  double fl[6];
-  for (int c = 0; c < 6; ++c) {
-    fl[c] = st->flow[c];
-    n2 += fl[c] * fl[c];
-  }
+#pragma unroll
+  for (int c = 0; c < 6; ++c) n2 += tot[c] * tot[c];
   const double nrm = n2 > 0.0 ? sqrt(n2) : 1.0;
-  const float w0 = (float)(fl[0] / nrm), w1 = (float)(fl[1] / nrm), w2 = (float)(fl[2] / nrm);
-  const float v0 = (float)(fl[3] / nrm), v1 = (float)(fl[4] / nrm), v2 = (float)(fl[5] / nrm);
+  const float w0 = (float)(tot[0] / nrm), w1 = (float)(tot[1] / nrm), w2 = (float)(tot[2] / nrm);
+  const float v0 = (float)(tot[3] / nrm), v1 = (float)(tot[4] / nrm), v2 = (float)(tot[5] / nrm);
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
   if (r < P.n_src) {
     float4 x4 = P.src_pos[r];
     float x0 = x4.x, x1 = x4.y, x2 = x4.z;
-    if (st->has_src_xf) {
+    if (mp.src_xf) {
       float t0, t1, t2;
-      affine3(As, bs, x0, x1, x2, t0, t1, t2);
+      affine3(mp.As, mp.bs, x0, x1, x2, t0, t1, t2);
       x0 = t0; x1 = t1; x2 = t2;
     }
     float temp_ell = ell;
     if (kc.range_ell_in_step) {
-      float px = sqrtf(x0 * x0 + x1 * x1 + x2 * x2);
+      const float px = sqrtf(x0 * x0 + x1 * x1 + x2 * x2);
       temp_ell = (float)(((double)px / 500.0 + 1.0) * (double)ell);
     }
     const float tc = (float)(1.0 / (2.0 * (double)temp_ell * (double)temp_ell));
     const uint32_t cnt = P.counts[r];
     const uint2* row = P.pairs + (size_t)r * P.cap_alloc;
-    for (uint32_t s = (uint32_t)g; s < cnt; s += kGroup) {
+    for (uint32_t s = 0; s < cnt; ++s) {
       const uint2 e = row[s];
       const float a = __uint_as_float(e.y);
       const float4 y4 = G.pos[e.x];
       float z0, z1, z2;
-      affine3(A, bt, y4.x, y4.y, y4.z, z0, z1, z2);
+      affine3(mp.A, mp.b, y4.x, y4.y, y4.z, z0, z1, z2);
       // xi^k z: xiz = w x z + v, xi^{k+1} z = w x xi^k z
       float a0, a1, a2, b0, b1, b2, c0, c1, c2, d0, d1, d2;
       cross_f(w0, w1, w2, z0, z1, z2, a0, a1, a2);
@@ -957,53 +998,99 @@ __global__ __launch_bounds__(kBlock) void step_kernel(const AssocProblem* __rest
                       (1.0 / 24.0) * bd * bd * bd * bd);
     }
   }
+  // block partial in fixed order: wave shuffles then LDS over the 4 waves
+  __shared__ double s_w[4][kBlock / 64];
+  const int lane = tid & 63, wave = tid >> 6;
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
     double v = acc[c];
-    v += shfl_xor_d(v, 8);
-    v += shfl_xor_d(v, 4);
-    v += shfl_xor_d(v, 2);
-    v += shfl_xor_d(v, 1);
-    acc[c] = v;
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if (lane == 0) s_w[c][wave] = v;
   }
-  __shared__ double s_row[kRowsPerBlock][4];
-  if (g == 0)
-    for (int c = 0; c < 4; ++c) s_row[grp][c] = acc[c];
   __syncthreads();
   if (tid < 4) {
-    double s = 0.0;
-    for (int q = 0; q < kRowsPerBlock; ++q) s += s_row[q][tid];
-    P.part_s[(size_t)lb * 4 + tid] = s;
+    double v = 0.0;
+    for (int w = 0; w < kBlock / 64; ++w) v += s_w[tid][w];
+    P.part_s[(size_t)blockIdx.x * 4 + tid] = v;
   }
-  __shared__ int s_last;
-  __syncthreads();
-  if (tid == 0) {
-    __threadfence();
-    uint32_t prev = __hip_atomic_fetch_add(&st->blocks_done_step, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    s_last = (prev == (uint32_t)P.n_blocks - 1u);
+}
+
+// One block per problem: totals of the iteration (flow, counters, step
+// coefficients) in fixed order, then the scalar controller on thread 0 with
+// the state staged in LDS.
+__global__ __launch_bounds__(kBlock) void controller_kernel(const AssocProblem* __restrict__ probs, KConsts kc) {
+  const AssocProblem& P = probs[blockIdx.x];
+  ProblemState* st = P.st;
+  if (st->done) return;
+  const int tid = threadIdx.x;
+  __shared__ ProblemState s_st;
+  __shared__ float s_h[kMaxLevels];
+  __shared__ int s_nl;
+  // stage the state
+  {
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(st);
+    uint32_t* dst = reinterpret_cast<uint32_t*>(&s_st);
+    for (int i = tid; i < (int)(sizeof(ProblemState) / 4); i += kBlock) dst[i] = src[i];
+    if (tid == 0) s_nl = P.grid->n_levels;
+    if (tid < kMaxLevels) s_h[tid] = P.grid->lv[tid].h;
+  }
+  __shared__ double s_tree[10][128];
+  double tot[10];
+  reduce_stage1(P, s_tree, tot);
+  // step partials: each thread sums a strided subset of blocks for all four
+  // coefficients, then one fixed-order tree
+  const int nsb = (P.n_src + kBlock - 1) / kBlock;
+  __shared__ double s_b[4][kBlock];
+  {
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int b = tid; b < nsb; b += kBlock)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) v[c] += P.part_s[(size_t)b * 4 + c];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) s_b[c][tid] = v[c];
   }
   __syncthreads();
-  if (!s_last) return;
-  __threadfence();
-  __shared__ double s_red[kBlock];
-  double v[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int b = tid; b < P.n_blocks; b += kBlock)
-    for (int c = 0; c < 4; ++c) v[c] += __builtin_nontemporal_load(&P.part_s[(size_t)b * 4 + c]);
-  double out[4];
-  for (int c = 0; c < 4; ++c) {
-    s_red[tid] = v[c];
-    __syncthreads();
-    for (int w = kBlock / 2; w > 0; w >>= 1) {
-      if (tid < w) s_red[tid] += s_red[tid + w];
-      __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    for (int t = tid; t < 4 * w; t += kBlock) {
+      const int c = t / w, i = t - c * w;
+      s_b[c][i] += s_b[c][i + w];
     }
-    out[c] = s_red[0];
     __syncthreads();
   }
+  double bcde[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) bcde[c] = s_b[c][0];
   if (tid == 0) {
-    for (int c = 0; c < 4; ++c) st->bcde[c] = out[c];
-    st->blocks_done_step = 0;
-    controller(st, kc, P);
+    for (int c = 0; c < 6; ++c) s_st.flow[c] = tot[c];
+    s_st.ip = tot[6];
+    s_st.nnz = (uint64_t)tot[7];
+    s_st.max_row = (uint32_t)tot[8];
+    s_st.level_assoc = s_st.level;
+    for (int c = 0; c < 4; ++c) s_st.bcde[c] = bcde[c];
+    controller(&s_st, kc, P, s_h, s_nl);
+  }
+  __syncthreads();
+  {
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(&s_st);
+    uint32_t* dst = reinterpret_cast<uint32_t*>(st);
+    for (int i = tid; i < (int)(sizeof(ProblemState) / 4); i += kBlock) dst[i] = src[i];
+  }
+}
+
+// Totals of a single association pass (inner product / association export):
+// one block per problem.
+__global__ __launch_bounds__(kBlock) void pass_totals_kernel(const AssocProblem* __restrict__ probs) {
+  const AssocProblem& P = probs[blockIdx.x];
+  ProblemState* st = P.st;
+  __shared__ double s_tree[10][128];
+  double tot[10];
+  reduce_stage1(P, s_tree, tot);
+  if (threadIdx.x == 0) {
+    for (int c = 0; c < 6; ++c) st->flow[c] = tot[c];
+    st->ip = tot[6];
+    st->nnz = (uint64_t)tot[7];
+    st->max_row = (uint32_t)tot[8];
+    st->level_assoc = st->level;
   }
 }
 
@@ -1164,8 +1251,17 @@ void launch_overflow(const AssocProblem* probs, int n_probs, const KConsts& kc, 
   CVO_DISPATCH_FK(launch_ovf_t, probs, n_probs, kc, s);
 }
 
-void launch_step(const AssocProblem* probs, int n_probs, int total_blocks, const KConsts& kc, hipStream_t s) {
-  hipLaunchKernelGGL(step_kernel, dim3(total_blocks), dim3(kBlock), 0, s, probs, n_probs, kc);
+void launch_step(const AssocProblem* probs, int n_probs, int max_src_rows, const KConsts& kc, hipStream_t s) {
+  hipLaunchKernelGGL(step_kernel, dim3((max_src_rows + kBlock - 1) / kBlock, n_probs), dim3(kBlock), 0, s, probs,
+                     n_probs, kc);
+}
+
+void launch_pass_totals(const AssocProblem* probs, int n_probs, hipStream_t s) {
+  hipLaunchKernelGGL(pass_totals_kernel, dim3(n_probs), dim3(kBlock), 0, s, probs);
+}
+
+void launch_controller(const AssocProblem* probs, int n_probs, const KConsts& kc, hipStream_t s) {
+  hipLaunchKernelGGL(controller_kernel, dim3(n_probs), dim3(kBlock), 0, s, probs, kc);
 }
 
 void launch_init_state(const AssocProblem* probs, int n_probs, const float* T16, int stride16, const KConsts& kc,
