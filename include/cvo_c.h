@@ -233,6 +233,23 @@ cvo_status cvo_compute_association(cvo_ctx* ctx, const cvo_cloud_view* src, cons
                                    const float T[16], float ell, cvo_assoc* out);
 void cvo_assoc_free(cvo_assoc* a);
 
+/* ---- batched multi-start global registration ----------------------------- */
+
+/* Replaces CvoGPU::get_nearest_init_pose + align (src/cvo/CvoGPU.cpp:367-398,
+ * src/cvo/CvoGPU.cu:1533-1543).  T_starts: n_starts initial poses (row-major
+ * 4x4, the align init convention).  mode 0: every start runs the full align
+ * loop, all starts batched into one launch per kernel per iteration with
+ * per-start lengthscale annealing; each final pose is scored by the cos angle
+ * (function_angle, approximate form) at score_ell and the largest wins.
+ * mode 1: the reference's global initialisation -- score the initial poses
+ * only, keep the first maximum, then one align from it.  T_best: the winning
+ * registration's transform (target -> source, like cvo_align's T_out).
+ * per_start_T (n*16), per_start_cos (n), per_start_iters (n) are optional. */
+cvo_status cvo_align_multistart(cvo_ctx* ctx, const cvo_dcloud* src, const cvo_dcloud* tgt, int32_t n_starts,
+                                const float* T_starts, float score_ell, int32_t mode, float T_best[16],
+                                int32_t* best_index, float* per_start_T, double* per_start_cos,
+                                int32_t* per_start_iters, double* seconds);
+
 /* ---- profiling ----------------------------------------------------------- */
 
 cvo_status cvo_set_profiling(cvo_ctx* ctx, int32_t on);

@@ -1028,3 +1028,383 @@ extern "C" double orc_time_iterations(const cvo_params* p, const cvo_cloud_view*
   omp_set_num_threads(old);
   return std::chrono::duration<double>(t1 - t0).count() / std::max(1, n_iter);
 }
+
+// ===========================================================================
+// IRLS bundle adjustment (src/cvo/IRLS.cpp:148-367, IRLS_State_GPU.cu:144-336,
+// IRLS_State_GPU.cpp:10-146, IRLS_Cost_CPU.hpp:266-371,
+// local_parameterization_se3.hpp:21-99).  The Ceres solve is replaced by the
+// documented Levenberg-Marquardt schedule shared with the device solver:
+//   D = clamp(diag(H), 1e-6, 1e32); solve (H + D/mu) d = -g;
+//   rho = (F - F_new) / (-(g.d + d.H.d/2)); accept if rho > 1e-3:
+//     mu /= max(1/3, 1 - (2 rho - 1)^3), v = 2;  else mu /= v, v *= 2;
+//   mu starts at 1e4; stop on max iterations, |dF|/F < 1e-5, ||g||inf < 1e-5,
+//   ||d|| < 1e-5 (||x|| + 1e-5).
+// ===========================================================================
+
+namespace {
+
+struct Mat34 {
+  double m[12];
+  double& operator[](int i) { return m[i]; }
+  const double& operator[](int i) const { return m[i]; }
+  operator double*() { return m; }
+  operator const double*() const { return m; }
+};
+
+// T * Exp([rho; phi]) with the translation part first (Exp_SE3(.., false),
+// LieGroup.cpp:168-192), exact small-angle series.
+void se3_plus(const double T[12], const double d[6], double out[12]) {
+  double xi[6] = {d[3], d[4], d[5], d[0], d[1], d[2]};  // [w; v]
+  double E[12];
+  exp_se3_f64(xi, 1.0, E);
+  for (int r = 0; r < 3; ++r) {
+    for (int c = 0; c < 3; ++c)
+      out[r * 4 + c] = T[r * 4 + 0] * E[0 * 4 + c] + T[r * 4 + 1] * E[1 * 4 + c] + T[r * 4 + 2] * E[2 * 4 + c];
+    out[r * 4 + 3] = T[r * 4 + 0] * E[3] + T[r * 4 + 1] * E[7] + T[r * 4 + 2] * E[11] + T[r * 4 + 3];
+  }
+}
+
+// LocalParameterizationSE3::ComputeJacobian (local_parameterization_se3.hpp:49-83)
+void plus_jacobian(const double T[12], double J[72]) {
+  for (int i = 0; i < 72; ++i) J[i] = 0.0;
+  const double d1[3] = {T[0], T[4], T[8]}, d2[3] = {T[1], T[5], T[9]}, d3[3] = {T[2], T[6], T[10]};
+  auto S = [&](int r, int c, double v) { J[r * 6 + c] = v; };
+  S(0, 4, -d3[0]); S(0, 5, d2[0]); S(4, 4, -d3[1]); S(4, 5, d2[1]); S(8, 4, -d3[2]); S(8, 5, d2[2]);
+  S(1, 3, d3[0]); S(1, 5, -d1[0]); S(5, 3, d3[1]); S(5, 5, -d1[1]); S(9, 3, d3[2]); S(9, 5, -d1[2]);
+  S(2, 3, -d2[0]); S(2, 4, d1[0]); S(6, 3, -d2[1]); S(6, 4, d1[1]); S(10, 3, -d2[2]); S(10, 4, d1[2]);
+  for (int c = 0; c < 3; ++c) {
+    S(3, c, T[0 * 4 + c]);
+    S(7, c, T[1 * 4 + c]);
+    S(11, c, T[2 * 4 + c]);
+  }
+}
+
+// PairwiseAnalyticalDiffFunctor::Evaluate (IRLS_Cost_CPU.hpp:305-357)
+void pair_residual(const double T1[12], const double T2[12], const double p1[3], const double p2[3], double a,
+                   double* r, double J1[12], double J2[12]) {
+  double P1[3], P2[3], diff[3];
+  for (int i = 0; i < 3; ++i) {
+    P1[i] = T1[i * 4 + 0] * p1[0] + T1[i * 4 + 1] * p1[1] + T1[i * 4 + 2] * p1[2] + T1[i * 4 + 3];
+    P2[i] = T2[i * 4 + 0] * p2[0] + T2[i * 4 + 1] * p2[1] + T2[i * 4 + 2] * p2[2] + T2[i * 4 + 3];
+    diff[i] = P1[i] - P2[i];
+  }
+  *r = a * (diff[0] * diff[0] + diff[1] * diff[1] + diff[2] * diff[2]);
+  const double h1[4] = {p1[0], p1[1], p1[2], 1.0}, h2[4] = {p2[0], p2[1], p2[2], 1.0};
+  for (int i = 0; i < 3; ++i)
+    for (int k = 0; k < 4; ++k) {
+      J1[i * 4 + k] = diff[i] * h1[k];   // diff^T DT1
+      J2[i * 4 + k] = -diff[i] * h2[k];  // -diff^T DT2
+    }
+}
+
+struct Pair {
+  int e;
+  float p1[3], p2[3];
+  float a;
+};
+
+// Local (6-dof) Jacobians of one pair: J_k(12) * ComputeJacobian(T_k)(12x6).
+void local_jacobians(const double T1[12], const double T2[12], const Pair& q, double* r, double j1[6], double j2[6]) {
+  double p1[3] = {q.p1[0], q.p1[1], q.p1[2]}, p2[3] = {q.p2[0], q.p2[1], q.p2[2]};
+  double J1[12], J2[12], P1j[72], P2j[72];
+  pair_residual(T1, T2, p1, p2, (double)q.a, r, J1, J2);
+  plus_jacobian(T1, P1j);
+  plus_jacobian(T2, P2j);
+  for (int c = 0; c < 6; ++c) {
+    double s1 = 0, s2 = 0;
+    for (int k = 0; k < 12; ++k) {
+      s1 += J1[k] * P1j[k * 6 + c];
+      s2 += J2[k] * P2j[k * 6 + c];
+    }
+    j1[c] = s1;
+    j2[c] = s2;
+  }
+}
+
+// Dense normal equations over free frames: H (6F x 6F), g (6F), cost = 1/2 sum r^2.
+void normal_equations(int n_frames, const std::vector<Mat34>& poses, const std::vector<int>& free_idx, int nfree,
+                      const std::vector<std::pair<int, int>>& edge_frames, const std::vector<char>& edge_on,
+                      const std::vector<Pair>& pairs, std::vector<double>& H, std::vector<double>& g, double* cost,
+                      int threads) {
+  const int n = 6 * nfree;
+  H.assign((size_t)n * n, 0.0);
+  g.assign(n, 0.0);
+  double c = 0.0;
+  int nt = std::max(1, threads);
+  std::vector<std::vector<double>> Hs(nt, std::vector<double>((size_t)n * n, 0.0)), gs(nt, std::vector<double>(n, 0.0));
+  std::vector<double> cs(nt, 0.0);
+#pragma omp parallel for num_threads(nt) schedule(static)
+  for (size_t k = 0; k < pairs.size(); ++k) {
+    const Pair& q = pairs[k];
+    if (!edge_on[q.e]) continue;
+    int t = omp_get_thread_num();
+    int f1 = edge_frames[q.e].first, f2 = edge_frames[q.e].second;
+    double r, j1[6], j2[6];
+    local_jacobians(poses[f1], poses[f2], q, &r, j1, j2);
+    cs[t] += 0.5 * r * r;
+    int o[2] = {free_idx[f1], free_idx[f2]};
+    const double* js[2] = {j1, j2};
+    for (int u = 0; u < 2; ++u) {
+      if (o[u] < 0) continue;
+      for (int a = 0; a < 6; ++a) {
+        gs[t][6 * o[u] + a] += js[u][a] * r;
+        for (int w = 0; w < 2; ++w) {
+          if (o[w] < 0) continue;
+          for (int b = 0; b < 6; ++b) Hs[t][(size_t)(6 * o[u] + a) * n + 6 * o[w] + b] += js[u][a] * js[w][b];
+        }
+      }
+    }
+  }
+  for (int t = 0; t < nt; ++t) {
+    for (size_t i = 0; i < H.size(); ++i) H[i] += Hs[t][i];
+    for (int i = 0; i < n; ++i) g[i] += gs[t][i];
+    c += cs[t];
+  }
+  *cost = c;
+}
+
+bool cholesky_solve(std::vector<double> A, int n, const std::vector<double>& b, std::vector<double>& x) {
+  for (int j = 0; j < n; ++j) {
+    double s = A[(size_t)j * n + j];
+    for (int k = 0; k < j; ++k) s -= A[(size_t)j * n + k] * A[(size_t)j * n + k];
+    if (!(s > 0)) return false;
+    double d = std::sqrt(s);
+    A[(size_t)j * n + j] = d;
+    for (int i = j + 1; i < n; ++i) {
+      double t = A[(size_t)i * n + j];
+      for (int k = 0; k < j; ++k) t -= A[(size_t)i * n + k] * A[(size_t)j * n + k];
+      A[(size_t)i * n + j] = t / d;
+    }
+  }
+  x = b;
+  for (int i = 0; i < n; ++i) {
+    double t = x[i];
+    for (int k = 0; k < i; ++k) t -= A[(size_t)i * n + k] * x[k];
+    x[i] = t / A[(size_t)i * n + i];
+  }
+  for (int i = n - 1; i >= 0; --i) {
+    double t = x[i];
+    for (int k = i + 1; k < n; ++k) t -= A[(size_t)k * n + i] * x[k];
+    x[i] = t / A[(size_t)i * n + i];
+  }
+  return true;
+}
+
+// change_of_all_poses (IRLS.cpp:99-108): sum ||log(T_old^-1 T_new)||
+double pose_change(const std::vector<Mat34>& a, const std::vector<Mat34>& b) {
+  double s = 0;
+  for (size_t f = 0; f < a.size(); ++f) {
+    double D[12];
+    for (int r = 0; r < 3; ++r) {
+      for (int c = 0; c < 3; ++c)
+        D[r * 4 + c] = a[f][0 * 4 + r] * b[f][0 * 4 + c] + a[f][1 * 4 + r] * b[f][1 * 4 + c] + a[f][2 * 4 + r] * b[f][2 * 4 + c];
+      D[r * 4 + 3] = a[f][0 * 4 + r] * (b[f][3] - a[f][3]) + a[f][1 * 4 + r] * (b[f][7] - a[f][7]) +
+                     a[f][2 * 4 + r] * (b[f][11] - a[f][11]);
+    }
+    s += log_se3_norm(D);
+  }
+  return s;
+}
+
+// World points of a frame in float (CvoFrameGPU::transform_pointcloud, float
+// pose, canonical FMA order).
+std::vector<float> world_points(const cvo_cloud_view& v, const double T[12]) {
+  float A[9], b[3];
+  for (int r = 0; r < 3; ++r) {
+    for (int c = 0; c < 3; ++c) A[r * 3 + c] = (float)T[r * 4 + c];
+    b[r] = (float)T[r * 4 + 3];
+  }
+  std::vector<float> out((size_t)v.n * 3);
+  for (int j = 0; j < v.n; ++j) xform_f(A, b, v.xyz + 3 * (size_t)j, &out[3 * (size_t)j]);
+  return out;
+}
+
+}  // namespace
+
+extern "C" int orc_ba_pair_residual(const double T1[12], const double T2[12], const double p1[3], const double p2[3],
+                                    double a, double* r, double J1[12], double J2[12]) {
+  pair_residual(T1, T2, p1, p2, a, r, J1, J2);
+  return 0;
+}
+
+extern "C" int orc_se3_plus(const double T[12], const double delta[6], double out[12]) {
+  se3_plus(T, delta, out);
+  return 0;
+}
+
+extern "C" int orc_se3_plus_jacobian(const double T[12], double J[72]) {
+  plus_jacobian(T, J);
+  return 0;
+}
+
+extern "C" int orc_ba_normal_equations(int32_t n_frames, const double* poses, int32_t n_pairs,
+                                       const int32_t* pair_frames, const float* p1, const float* p2, const float* a,
+                                       double* H, double* g, double* cost) {
+  std::vector<Mat34> P(n_frames);
+  for (int f = 0; f < n_frames; ++f)
+    for (int i = 0; i < 12; ++i) P[f][i] = poses[12 * f + i];
+  std::vector<int> free_idx(n_frames);
+  for (int f = 0; f < n_frames; ++f) free_idx[f] = f;
+  // one "edge" per distinct (f1, f2) is not needed here: every pair carries its frames
+  std::vector<std::pair<int, int>> ef(n_pairs);
+  std::vector<char> on(n_pairs, 1);
+  std::vector<Pair> pairs(n_pairs);
+  for (int k = 0; k < n_pairs; ++k) {
+    ef[k] = {pair_frames[2 * k], pair_frames[2 * k + 1]};
+    pairs[k].e = k;
+    for (int i = 0; i < 3; ++i) {
+      pairs[k].p1[i] = p1[3 * k + i];
+      pairs[k].p2[i] = p2[3 * k + i];
+    }
+    pairs[k].a = a[k];
+  }
+  std::vector<double> Hv, gv;
+  normal_equations(n_frames, P, free_idx, n_frames, ef, on, pairs, Hv, gv, cost, 1);
+  std::memcpy(H, Hv.data(), sizeof(double) * Hv.size());
+  std::memcpy(g, gv.data(), sizeof(double) * gv.size());
+  return 0;
+}
+
+extern "C" int orc_align_multiframe(const cvo_params* p, int32_t n_frames, const cvo_cloud_view* frames,
+                                    double* poses, const uint8_t* hold_const, int32_t n_edges, const int32_t* edges,
+                                    int32_t threads, int32_t* iters_out, double* per_iter_seconds, int32_t max_records) {
+  if (!p || n_frames <= 0 || !frames || !poses || n_edges < 0) return 1;
+  int old = omp_get_max_threads();
+  if (threads > 0) omp_set_num_threads(threads);
+  std::vector<Mat34> X(n_frames);
+  for (int f = 0; f < n_frames; ++f)
+    for (int i = 0; i < 12; ++i) X[f][i] = poses[12 * f + i];
+  std::vector<int> free_idx(n_frames, -1);
+  int nfree = 0;
+  for (int f = 0; f < n_frames; ++f)
+    if (!hold_const || !hold_const[f]) free_idx[f] = nfree++;
+  std::vector<std::pair<int, int>> ef(n_edges);
+  for (int e = 0; e < n_edges; ++e) ef[e] = {edges[2 * e], edges[2 * e + 1]};
+  // BinaryStateGPU state (IRLS_State_GPU.cu:18-69)
+  std::vector<double> ell(n_edges, p->multiframe_ell_init);
+  const double ell_max = (double)p->multiframe_ell_init * 1.2, ell_min = p->multiframe_ell_min;
+  std::vector<int> K(n_edges, p->multiframe_num_neighbors), last_max(n_edges, 0), iters_per_ell(n_edges, 0);
+  std::vector<int64_t> nnz_last(n_edges, 0), nnz(n_edges, 0);
+  Pose64 I64;
+  for (int i = 0; i < 9; ++i) I64.R[i] = (i % 4 == 0) ? 1.0 : 0.0;
+  for (int i = 0; i < 3; ++i) I64.T[i] = 0.0;
+  int it = 0;
+  const int n = 6 * nfree;
+  while (true) {
+    auto t0 = std::chrono::steady_clock::now();
+    std::vector<Mat34> X_old = X;
+    // world points (float) and per-edge association
+    std::vector<std::vector<float>> W(n_frames);
+    for (int f = 0; f < n_frames; ++f) W[f] = world_points(frames[f], X[f]);
+    std::vector<Pair> pairs;
+    int counter_nonzero = 0;
+    std::vector<char> on(n_edges, 0);
+    for (int e = 0; e < n_edges; ++e) {
+      const int f1 = ef[e].first, f2 = ef[e].second;
+      nnz_last[e] = nnz[e];
+      if (last_max[e] > 0) K[e] = std::min(p->multiframe_num_neighbors, (int)(last_max[e] * 1.1));
+      CloudRef S = ref(&frames[f1]), Tg = ref(&frames[f2]);
+      S.xyz = W[f1].data();
+      Tg.xyz = W[f2].data();
+      AssocOut o;
+      assoc_core(p, S, Tg, I64, (float)ell[e], K[e], ORC_MIRROR, 0, o);
+      nnz[e] = o.cnt.stored;
+      last_max[e] = o.cnt.max_row;
+      if (nnz[e] > p->multiframe_min_nonzeros) counter_nonzero++;
+      on[e] = (nnz[e] > 0 && nnz[e] >= p->multiframe_min_nonzeros);
+      if (!on[e]) continue;
+      for (int i = 0; i < S.n; ++i)
+        for (int s = 0; s < o.counts[i]; ++s) {
+          Pair q;
+          q.e = e;
+          int j = o.cols[(size_t)i * K[e] + s];
+          for (int d = 0; d < 3; ++d) {
+            q.p1[d] = frames[f1].xyz[3 * (size_t)i + d];
+            q.p2[d] = frames[f2].xyz[3 * (size_t)j + d];
+          }
+          q.a = o.vals[(size_t)i * K[e] + s];
+          pairs.push_back(q);
+        }
+    }
+    if (counter_nonzero == 0) break;  // IRLS.cpp:249-250
+    // Levenberg-Marquardt over the fixed pairs
+    if (n > 0) {
+      std::vector<double> H, g, Hc, gc;
+      double F = 0, Fc = 0;
+      normal_equations(n_frames, X, free_idx, nfree, ef, on, pairs, H, g, &F, threads);
+      double mu = 1e4, v = 2.0;
+      for (int lm = 0; lm < p->multiframe_iterations_per_solve; ++lm) {
+        double gmax = 0;
+        for (int i = 0; i < n; ++i) gmax = std::max(gmax, std::fabs(g[i]));
+        if (gmax < 1e-5) break;
+        std::vector<double> A = H, d;
+        for (int i = 0; i < n; ++i) {
+          double D = std::min(std::max(H[(size_t)i * n + i], 1e-6), 1e32);
+          A[(size_t)i * n + i] += D / mu;
+        }
+        std::vector<double> mg(n);
+        for (int i = 0; i < n; ++i) mg[i] = -g[i];
+        if (!cholesky_solve(A, n, mg, d)) {
+          mu /= v;
+          v *= 2;
+          continue;
+        }
+        double dn = 0, xn = 0;
+        for (int i = 0; i < n; ++i) dn += d[i] * d[i];
+        for (int f = 0; f < n_frames; ++f)
+          for (int i = 0; i < 12; ++i) xn += X[f][i] * X[f][i];
+        if (std::sqrt(dn) < 1e-5 * (std::sqrt(xn) + 1e-5)) break;
+        std::vector<Mat34> Xc = X;
+        for (int f = 0; f < n_frames; ++f)
+          if (free_idx[f] >= 0) se3_plus(X[f], &d[6 * free_idx[f]], Xc[f]);
+        normal_equations(n_frames, Xc, free_idx, nfree, ef, on, pairs, Hc, gc, &Fc, threads);
+        double gd = 0, dHd = 0;
+        for (int i = 0; i < n; ++i) {
+          gd += g[i] * d[i];
+          double s = 0;
+          for (int j = 0; j < n; ++j) s += H[(size_t)i * n + j] * d[j];
+          dHd += d[i] * s;
+        }
+        const double pred = -(gd + 0.5 * dHd);
+        const double rho = pred > 0 ? (F - Fc) / pred : -1.0;
+        if (rho > 1e-3) {
+          const double rel = std::fabs(F - Fc) / std::max(F, 1e-300);
+          X = Xc;
+          H = Hc;
+          g = gc;
+          F = Fc;
+          mu = mu / std::max(1.0 / 3.0, 1.0 - std::pow(2.0 * rho - 1.0, 3));
+          mu = std::min(mu, 1e16);
+          v = 2.0;
+          if (rel < 1e-5) break;
+        } else {
+          mu /= v;
+          v *= 2.0;
+        }
+      }
+    }
+    // per-edge lengthscale (BinaryStateGPU::update_ell, IRLS_State_GPU.cpp:117-146)
+    bool all_min = true;
+    for (int e = 0; e < n_edges; ++e) {
+      iters_per_ell[e]++;
+      if (nnz_last[e] > 0 && nnz[e] <= nnz_last[e] && iters_per_ell[e] / p->multiframe_iter_per_ell > 0) {
+        ell[e] = ell[e] * (double)p->multiframe_ell_decay_rate;
+        iters_per_ell[e] = 0;
+      }
+      if (ell[e] < ell_min) ell[e] = ell_min;
+      else if (ell[e] > ell_max) ell[e] = ell_max;
+      all_min = all_min && std::fabs((float)ell[e] - p->multiframe_ell_min) < 1e-4;
+    }
+    const double change = pose_change(X_old, X);
+    auto t1 = std::chrono::steady_clock::now();
+    if (per_iter_seconds && it < max_records) per_iter_seconds[it] = std::chrono::duration<double>(t1 - t0).count();
+    it++;
+    if (all_min && change < 1e-5) break;
+    if (it >= p->multiframe_max_iters) break;
+  }
+  for (int f = 0; f < n_frames; ++f)
+    for (int i = 0; i < 12; ++i) poses[12 * f + i] = X[f][i];
+  if (iters_out) *iters_out = it;
+  omp_set_num_threads(old);
+  return 0;
+}

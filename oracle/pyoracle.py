@@ -184,3 +184,61 @@ def time_iterations(params, src, tgt, T, ell, K, n_iter, threads):
     T16 = mat16(T)
     return lib().orc_time_iterations(C.byref(params), hs.ref, ht.ref, _p(T16), C.c_float(ell), int(K), int(n_iter),
                                      int(threads))
+
+
+# ---- IRLS bundle adjustment -------------------------------------------------
+
+def ba_pair_residual(T1, T2, p1, p2, a):
+    T1 = np.ascontiguousarray(np.asarray(T1, np.float64)[:3, :4].reshape(12))
+    T2 = np.ascontiguousarray(np.asarray(T2, np.float64)[:3, :4].reshape(12))
+    q1 = np.ascontiguousarray(p1, np.float64)
+    q2 = np.ascontiguousarray(p2, np.float64)
+    r = C.c_double()
+    J1 = np.zeros(12)
+    J2 = np.zeros(12)
+    lib().orc_ba_pair_residual(_p(T1), _p(T2), _p(q1), _p(q2), C.c_double(a), C.byref(r), _p(J1), _p(J2))
+    return r.value, J1, J2
+
+
+def se3_plus(T, delta):
+    T = np.ascontiguousarray(np.asarray(T, np.float64)[:3, :4].reshape(12))
+    d = np.ascontiguousarray(delta, np.float64)
+    out = np.zeros(12)
+    lib().orc_se3_plus(_p(T), _p(d), _p(out))
+    return out.reshape(3, 4)
+
+
+def se3_plus_jacobian(T):
+    T = np.ascontiguousarray(np.asarray(T, np.float64)[:3, :4].reshape(12))
+    J = np.zeros(72)
+    lib().orc_se3_plus_jacobian(_p(T), _p(J))
+    return J.reshape(12, 6)
+
+
+def ba_normal_equations(poses, pair_frames, p1, p2, a):
+    P = np.ascontiguousarray(np.asarray(poses, np.float64)[:, :3, :4].reshape(-1))
+    nf = P.size // 12
+    pf = np.ascontiguousarray(pair_frames, np.int32)
+    q1 = np.ascontiguousarray(p1, np.float32)
+    q2 = np.ascontiguousarray(p2, np.float32)
+    aa = np.ascontiguousarray(a, np.float32)
+    H = np.zeros((6 * nf, 6 * nf))
+    g = np.zeros(6 * nf)
+    cost = C.c_double()
+    lib().orc_ba_normal_equations(nf, _p(P), len(aa), _p(pf), _p(q1), _p(q2), _p(aa), _p(H), _p(g), C.byref(cost))
+    return H, g, cost.value
+
+
+def align_multiframe(params, frames, poses, hold_const, edges, threads=8, max_records=1000):
+    held = [HeldView(f) for f in frames]
+    views = (held[0].view.__class__ * len(held))(*[h.view for h in held])
+    P = np.ascontiguousarray(np.asarray(poses, np.float64)[:, :3, :4].reshape(-1))
+    hc = np.ascontiguousarray(hold_const, np.uint8)
+    E = np.ascontiguousarray(edges, np.int32).reshape(-1)
+    it = C.c_int32()
+    secs = np.zeros(max_records)
+    lib().orc_align_multiframe(C.byref(params), len(frames), views, _p(P), _p(hc), len(E) // 2, _p(E), int(threads),
+                               C.byref(it), _p(secs), max_records)
+    out = np.tile(np.eye(4), (len(frames), 1, 1))
+    out[:, :3, :4] = P.reshape(-1, 3, 4)
+    return out, it.value, secs[:it.value]

@@ -38,6 +38,7 @@ void launch_export(const AssocProblem* P_dev, int n_src, int64_t* counts_orig, c
                    float* val, int phase, hipStream_t s);
 void launch_all_done(const AssocProblem* probs, int n_probs, int32_t* out, hipStream_t s);
 void launch_pass_totals(const AssocProblem* probs, int n_probs, hipStream_t s);
+void launch_rescore(const AssocProblem* probs, int n_probs, const KConsts& kc, float ell, int K, hipStream_t s);
 void launch_controller(const AssocProblem* probs, int n_probs, const KConsts& kc, hipStream_t s);
 
 cvo_status set_err(cvo_ctx* ctx, cvo_status s, const std::string& msg) {
@@ -77,7 +78,7 @@ static size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 // Kernel constants
 // ---------------------------------------------------------------------------
 
-static KConsts make_kconsts(const cvo_params& p, int F, int Ks, int mode) {
+KConsts make_kconsts(const cvo_params& p, int F, int Ks, int mode) {
   KConsts k;
   std::memset(&k, 0, sizeof(k));
   k.sigma2 = p.sigma * p.sigma;
@@ -127,12 +128,8 @@ static KConsts make_kconsts(const cvo_params& p, int F, int Ks, int mode) {
 // Workspace of one or more association problems over one source/target pair
 // ---------------------------------------------------------------------------
 
-struct GridHost {
-  GridDev dev;  // host copy of the device header
-  GridDev* dev_ptr = nullptr;
-};
 
-static cvo_status validate_pair(cvo_ctx* ctx, const cvo_dcloud* s, const cvo_dcloud* t) {
+cvo_status validate_pair(cvo_ctx* ctx, const cvo_dcloud* s, const cvo_dcloud* t) {
   const cvo_params& p = ctx->params;
   if (!s || !t) return set_err(ctx, CVO_ERR_INVALID_ARG, "null cloud");
   if (s->n == 0 || t->n == 0) return set_err(ctx, CVO_ERR_EMPTY, "empty cloud");
@@ -151,12 +148,13 @@ static cvo_status validate_pair(cvo_ctx* ctx, const cvo_dcloud* s, const cvo_dcl
 }
 
 // Radius needed at lengthscale ell (host mirror of needed_radius).
-static float radius_for(const KConsts& kc, float ell, float max_norm, float tq) {
+float radius_for(const KConsts& kc, float ell, float max_norm, float tq) {
   return kc.rq_coef * ell * (1.0f + max_norm / 500.0f) * 1.0001f + 1e-4f * (1.0f + max_norm + tq) + 1e-6f;
 }
 
 // Spatially sorted source rows (Morton order over the source bounding box).
-static cvo_status prepare_source(cvo_ctx* ctx, const cvo_dcloud* s) {
+cvo_status prepare_source_into(cvo_ctx* ctx, const cvo_dcloud* s, DevBuf& b_pos, DevBuf& b_feat, DevBuf& b_sem,
+                               DevBuf& b_geo) {
   hipStream_t st = ctx->stream;
   const int n = s->n;
   CVO_TRY(ensure(ctx, ctx->b_src_key, sizeof(uint32_t) * n * 2));
@@ -171,30 +169,35 @@ static cvo_status prepare_source(cvo_ctx* ctx, const cvo_dcloud* s) {
   CVO_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, keys, keys2, idx, idx2, n, 0, 30, st));
   CVO_TRY(ensure(ctx, ctx->b_cub, tmp));
   CVO_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(ctx->b_cub.p, tmp, keys, keys2, idx, idx2, n, 0, 30, st));
-  CVO_TRY(ensure(ctx, ctx->b_src_pos, sizeof(float4) * n));
+  CVO_TRY(ensure(ctx, b_pos, sizeof(float4) * n));
   float* fs = nullptr;
   float* ss = nullptr;
   float* gs = nullptr;
   if (s->feat) {
-    CVO_TRY(ensure(ctx, ctx->b_src_feat, sizeof(float) * (size_t)n * s->F));
-    fs = as<float>(ctx->b_src_feat);
+    CVO_TRY(ensure(ctx, b_feat, sizeof(float) * (size_t)n * s->F));
+    fs = as<float>(b_feat);
   }
   if (s->sem) {
-    CVO_TRY(ensure(ctx, ctx->b_src_sem, sizeof(float) * (size_t)n * s->K));
-    ss = as<float>(ctx->b_src_sem);
+    CVO_TRY(ensure(ctx, b_sem, sizeof(float) * (size_t)n * s->K));
+    ss = as<float>(b_sem);
   }
   if (s->geo) {
-    CVO_TRY(ensure(ctx, ctx->b_src_geo, sizeof(float) * (size_t)n * 2));
-    gs = as<float>(ctx->b_src_geo);
+    CVO_TRY(ensure(ctx, b_geo, sizeof(float) * (size_t)n * 2));
+    gs = as<float>(b_geo);
   }
-  launch_gather(idx2, n, s->xyz, s->feat, s->F, s->sem, s->K, s->geo, as<float4>(ctx->b_src_pos), fs, ss, gs, nullptr,
+  launch_gather(idx2, n, s->xyz, s->feat, s->F, s->sem, s->K, s->geo, as<float4>(b_pos), fs, ss, gs, nullptr,
                 st);
   CVO_HIP(ctx, hipGetLastError());
   return CVO_OK;
 }
 
+static cvo_status prepare_source(cvo_ctx* ctx, const cvo_dcloud* s) {
+  return prepare_source_into(ctx, s, ctx->b_src_pos, ctx->b_src_feat, ctx->b_src_sem, ctx->b_src_geo);
+}
+
 // Grid levels over the target for cell sizes hs (descending).
-static cvo_status build_grid(cvo_ctx* ctx, const cvo_dcloud* t, const std::vector<float>& hs, GridHost& gh) {
+cvo_status build_grid_into(cvo_ctx* ctx, const cvo_dcloud* t, const std::vector<float>& hs, GridHost& gh,
+                           DevBuf& b_store, DevBuf& b_hdr) {
   hipStream_t st = ctx->stream;
   const int n = t->n;
   const int L = (int)hs.size();
@@ -232,8 +235,8 @@ static cvo_status build_grid(cvo_ctx* ctx, const cvo_dcloud* t, const std::vecto
     off_geo[l] = total; total = align_up(total + (t->geo ? sizeof(float) * (size_t)n * 2 : 0), 256);
     off_inv[l] = total; total = align_up(total + sizeof(uint32_t) * n, 256);
   }
-  CVO_TRY(ensure(ctx, ctx->b_grid_store, total));
-  char* base = as<char>(ctx->b_grid_store);
+  CVO_TRY(ensure(ctx, b_store, total));
+  char* base = as<char>(b_store);
   // temporaries: keys, keys2, idx, idx2, counts
   CVO_TRY(ensure(ctx, ctx->b_tmp0, sizeof(uint32_t) * (size_t)n * 4));
   CVO_TRY(ensure(ctx, ctx->b_tmp1, sizeof(uint32_t) * ((size_t)max_cells + 1)));
@@ -269,14 +272,18 @@ static cvo_status build_grid(cvo_ctx* ctx, const cvo_dcloud* t, const std::vecto
                   const_cast<uint32_t*>(lv.inv_perm), st);
   }
   CVO_HIP(ctx, hipGetLastError());
-  CVO_TRY(ensure(ctx, ctx->b_gridhdr, sizeof(GridDev)));
-  gh.dev_ptr = as<GridDev>(ctx->b_gridhdr);
+  CVO_TRY(ensure(ctx, b_hdr, sizeof(GridDev)));
+  gh.dev_ptr = as<GridDev>(b_hdr);
   CVO_HIP(ctx, hipMemcpyAsync(gh.dev_ptr, &G, sizeof(GridDev), hipMemcpyHostToDevice, st));
   return CVO_OK;
 }
 
+static cvo_status build_grid(cvo_ctx* ctx, const cvo_dcloud* t, const std::vector<float>& hs, GridHost& gh) {
+  return build_grid_into(ctx, t, hs, gh, ctx->b_grid_store, ctx->b_gridhdr);
+}
+
 // Cell sizes for lengthscales from ell_hi down to ell_lo, ratio 0.7.
-static std::vector<float> level_sizes(const KConsts& kc, float ell_hi, float ell_lo, float max_norm, float tq) {
+std::vector<float> level_sizes(const KConsts& kc, float ell_hi, float ell_lo, float max_norm, float tq) {
   std::vector<float> hs;
   // without the geometric gate every row takes the full scan; one coarse level
   if (!kc.use_geo) return {1.0e4f};
@@ -366,7 +373,7 @@ static cvo_status setup_problems(cvo_ctx* ctx, const cvo_dcloud* s, const cvo_dc
   return CVO_OK;
 }
 
-static cvo_status pinned(cvo_ctx* ctx, size_t bytes) {
+cvo_status pinned(cvo_ctx* ctx, size_t bytes) {
   if (ctx->pinned_cap >= bytes) return CVO_OK;
   if (ctx->pinned) CVO_HIP(ctx, hipHostFree(ctx->pinned));
   ctx->pinned = nullptr;
@@ -375,7 +382,7 @@ static cvo_status pinned(cvo_ctx* ctx, size_t bytes) {
   return CVO_OK;
 }
 
-static cvo_status ensure_events(cvo_ctx* ctx, size_t n) {
+cvo_status ensure_events(cvo_ctx* ctx, size_t n) {
   while (ctx->ev_pool.size() < n) {
     hipEvent_t e;
     CVO_HIP(ctx, hipEventCreate(&e));
@@ -945,4 +952,111 @@ extern "C" void cvo_assoc_free(cvo_assoc* a) {
   a->col = nullptr;
   a->val = nullptr;
   a->nnz = 0;
+}
+
+// ---------------------------------------------------------------------------
+// Batched multi-start global registration
+// ---------------------------------------------------------------------------
+
+static void pose_to_output(const ProblemState& hs, float* T_out) {
+  for (int r = 0; r < 3; ++r) {
+    for (int c = 0; c < 3; ++c) T_out[r * 4 + c] = (float)hs.R[c * 3 + r];
+    T_out[r * 4 + 3] = (float)(-(hs.R[0 * 3 + r] * hs.T[0] + hs.R[1 * 3 + r] * hs.T[1] + hs.R[2 * 3 + r] * hs.T[2]));
+  }
+  T_out[12] = T_out[13] = T_out[14] = 0.0f;
+  T_out[15] = 1.0f;
+}
+
+extern "C" cvo_status cvo_align_multistart(cvo_ctx* ctx, const cvo_dcloud* src, const cvo_dcloud* tgt, int32_t n_starts,
+                                           const float* T_starts, float score_ell, int32_t mode, float T_best[16],
+                                           int32_t* best_index, float* per_start_T, double* per_start_cos,
+                                           int32_t* per_start_iters, double* seconds) {
+  if (!ctx || !T_starts || !T_best || n_starts <= 0 || !(score_ell > 0)) return CVO_ERR_INVALID_ARG;
+  if (n_starts > 4096) return set_err(ctx, CVO_ERR_LIMIT, "at most 4096 starts per call");
+  if (!src || !tgt || src->n == 0 || tgt->n == 0) return set_err(ctx, CVO_ERR_EMPTY, "empty point cloud");
+  CVO_TRY(validate_pair(ctx, src, tgt));
+  const cvo_params& p = ctx->params;
+  CVO_TRY(ensure_events(ctx, 8));
+  hipEvent_t e0 = ctx->ev_pool[4], e1 = ctx->ev_pool[5];
+  CVO_HIP(ctx, hipEventRecord(e0, ctx->stream));
+  KConsts kc = make_kconsts(p, src->F, src->K, MODE_ALIGN);
+  kc.n_src_f = (float)src->n;
+  kc.n_tgt_f = (float)tgt->n;
+  kc.max_src_norm = src->max_norm;
+  KConsts kci = kc;
+  kci.mode = MODE_IP;
+  CVO_TRY(prepare_source(ctx, src));
+  float tq = 0.0f;
+  for (int s = 0; s < n_starts; ++s)
+    tq = std::max(tq, std::max(std::fabs(T_starts[16 * s + 3]),
+                               std::max(std::fabs(T_starts[16 * s + 7]), std::fabs(T_starts[16 * s + 11]))));
+  const float ell_hi = std::max(std::max(p.ell_init, p.ell_min), score_ell);
+  const float ell_lo = std::min(std::min(p.ell_min, p.ell_init), score_ell);
+  GridHost gh;
+  CVO_TRY(build_grid(ctx, tgt, level_sizes(kc, ell_hi, ell_lo, src->max_norm, tq), gh));
+  const int n_run = (mode == 1) ? n_starts : n_starts;
+  Problems pr;
+  CVO_TRY(setup_problems(ctx, src, tgt, gh, n_run, p.nearest_neighbors_max, 0, pr));
+  CVO_TRY(ensure(ctx, ctx->b_tmp2, sizeof(float) * 16 * (size_t)n_starts + 64));
+  CVO_HIP(ctx, hipMemcpyAsync(ctx->b_tmp2.p, T_starts, sizeof(float) * 16 * n_starts, hipMemcpyHostToDevice,
+                              ctx->stream));
+  launch_init_state(pr.dev, n_starts, as<float>(ctx->b_tmp2), 16, kc, p.ell_init, p.nearest_neighbors_max,
+                    ctx->stream);
+  CVO_HIP(ctx, hipGetLastError());
+  std::vector<ProblemState> hs(n_starts);
+  std::vector<double> cosv(n_starts, 0.0);
+  std::vector<int32_t> its(n_starts, 0);
+  const double norm = std::sqrt((double)src->n) * std::sqrt((double)tgt->n);
+  int best = 0;
+  if (mode == 0) {
+    if (p.MAX_ITER > 0) CVO_TRY(run_loop(ctx, pr, kc, p.MAX_ITER, true, nullptr));
+    CVO_HIP(ctx, hipMemcpyAsync(hs.data(), pr.states, sizeof(ProblemState) * n_starts, hipMemcpyDeviceToHost,
+                                ctx->stream));
+    CVO_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    for (int s = 0; s < n_starts; ++s) its[s] = p.MAX_ITER <= 0 ? 0 : (hs[s].done ? hs[s].iters_out : p.MAX_ITER);
+    launch_rescore(pr.dev, n_starts, kci, score_ell, p.nearest_neighbors_max, ctx->stream);
+    CVO_TRY(run_loop(ctx, pr, kci, 1, false, nullptr));
+    CVO_HIP(ctx, hipMemcpyAsync(hs.data(), pr.states, sizeof(ProblemState) * n_starts, hipMemcpyDeviceToHost,
+                                ctx->stream));
+    CVO_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    for (int s = 0; s < n_starts; ++s) {
+      cosv[s] = hs[s].ip / norm;
+      if (cosv[s] > cosv[best]) best = s;
+      if (per_start_T) pose_to_output(hs[s], per_start_T + 16 * (size_t)s);
+    }
+    pose_to_output(hs[best], T_best);
+  } else {
+    // the reference's get_nearest_init_pose (CvoGPU.cpp:367-398): score the
+    // initial poses only, keep the first maximum, then one align from it
+    launch_rescore(pr.dev, n_starts, kci, score_ell, p.nearest_neighbors_max, ctx->stream);
+    CVO_TRY(run_loop(ctx, pr, kci, 1, false, nullptr));
+    CVO_HIP(ctx, hipMemcpyAsync(hs.data(), pr.states, sizeof(ProblemState) * n_starts, hipMemcpyDeviceToHost,
+                                ctx->stream));
+    CVO_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    double best_ip = 0.0;
+    best = 0;
+    for (int s = 0; s < n_starts; ++s) {
+      cosv[s] = hs[s].ip / norm;
+      if (hs[s].ip > best_ip) {
+        best_ip = hs[s].ip;
+        best = s;
+      }
+      if (per_start_T) pose_to_output(hs[s], per_start_T + 16 * (size_t)s);
+    }
+    int32_t it = 0, ret = 0;
+    double secs = 0;
+    CVO_TRY(cvo_align_d(ctx, src, tgt, T_starts + 16 * (size_t)best, T_best, nullptr, &secs, &it, &ret));
+    its[best] = it;
+  }
+  CVO_HIP(ctx, hipEventRecord(e1, ctx->stream));
+  CVO_HIP(ctx, hipEventSynchronize(e1));
+  float ms = 0;
+  CVO_HIP(ctx, hipEventElapsedTime(&ms, e0, e1));
+  if (seconds) *seconds = ms / 1000.0;
+  if (best_index) *best_index = best;
+  if (per_start_cos)
+    for (int s = 0; s < n_starts; ++s) per_start_cos[s] = cosv[s];
+  if (per_start_iters)
+    for (int s = 0; s < n_starts; ++s) per_start_iters[s] = its[s];
+  return CVO_OK;
 }

@@ -185,9 +185,25 @@ struct cvo_ctx {
 
 // Internal helpers shared by the translation units.
 namespace cvo {
+struct GridHost {
+  GridDev dev;  // host copy of the device header
+  GridDev* dev_ptr = nullptr;
+};
 cvo_status set_err(cvo_ctx* ctx, cvo_status s, const std::string& msg);
 cvo_status hip_check(cvo_ctx* ctx, hipError_t e, const char* what);
 cvo_status ensure(cvo_ctx* ctx, DevBuf& b, size_t bytes);
+KConsts make_kconsts(const cvo_params& p, int F, int Ks, int mode);
+cvo_status validate_pair(cvo_ctx* ctx, const cvo_dcloud* s, const cvo_dcloud* t);
+float radius_for(const KConsts& kc, float ell, float max_norm, float tq);
+std::vector<float> level_sizes(const KConsts& kc, float ell_hi, float ell_lo, float max_norm, float tq);
+cvo_status ensure_events(cvo_ctx* ctx, size_t n);
+cvo_status prepare_source_into(cvo_ctx* ctx, const cvo_dcloud* s, DevBuf& b_pos, DevBuf& b_feat, DevBuf& b_sem,
+                               DevBuf& b_geo);
+cvo_status build_grid_into(cvo_ctx* ctx, const cvo_dcloud* t, const std::vector<float>& hs, GridHost& gh,
+                           DevBuf& b_store, DevBuf& b_hdr);
+void launch_assoc(const AssocProblem* probs, int n_probs, int total_blocks, const KConsts& kc, hipStream_t s);
+void launch_overflow(const AssocProblem* probs, int n_probs, const KConsts& kc, hipStream_t s);
+void launch_pass_totals(const AssocProblem* probs, int n_probs, hipStream_t s);
 }  // namespace cvo
 
 #define CVO_HIP(ctx, call)                                              \

@@ -1130,6 +1130,22 @@ __global__ void init_state_kernel(const AssocProblem* __restrict__ probs, int n_
   st->level = choose_level(P.grid, needed_radius(kc, st, ell0));
 }
 
+// Switches every problem to a single scoring pass at lengthscale ell with the
+// full row cap, keeping its pose (multi-start scoring by cos angle).
+__global__ void rescore_state_kernel(const AssocProblem* __restrict__ probs, int n_probs, KConsts kc, float ell,
+                                     int K) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n_probs) return;
+  const AssocProblem& P = probs[p];
+  ProblemState* st = P.st;
+  st->ell = ell;
+  st->K = K;
+  st->done = 0;
+  st->n_overflow = 0;
+  set_maps(st);
+  st->level = choose_level(P.grid, needed_radius(kc, st, ell));
+}
+
 // ---------------------------------------------------------------------------
 // Association export: CSR in input row order, target ids ascending per row.
 // ---------------------------------------------------------------------------
@@ -1254,6 +1270,10 @@ void launch_overflow(const AssocProblem* probs, int n_probs, const KConsts& kc, 
 void launch_step(const AssocProblem* probs, int n_probs, int max_src_rows, const KConsts& kc, hipStream_t s) {
   hipLaunchKernelGGL(step_kernel, dim3((max_src_rows + kBlock - 1) / kBlock, n_probs), dim3(kBlock), 0, s, probs,
                      n_probs, kc);
+}
+
+void launch_rescore(const AssocProblem* probs, int n_probs, const KConsts& kc, float ell, int K, hipStream_t s) {
+  hipLaunchKernelGGL(rescore_state_kernel, dim3((n_probs + 63) / 64), dim3(64), 0, s, probs, n_probs, kc, ell, K);
 }
 
 void launch_pass_totals(const AssocProblem* probs, int n_probs, hipStream_t s) {

@@ -57,6 +57,7 @@ def lib() -> C.CDLL:
         L.cvo_compute_association.argtypes = [P, P, P, P, f, P]
         L.cvo_align.argtypes = [P, P, P, P, P, P, P, P, P]
         L.cvo_align_d.argtypes = [P, P, P, P, P, P, P, P, P]
+        L.cvo_align_multistart.argtypes = [P, P, P, i32, P, f, i32, P, P, P, P, P, P]
         _lib = L
     return _lib
 
@@ -229,6 +230,36 @@ class CvoGPU:
         res.T_s2t, res.num_iters, res.registration_seconds, res.return_code = T, iters, secs, ret
         res.association = assoc
         return res
+
+    def align_multistart(self, source, target, T_starts, score_ell: float, mode: int = 0) -> dict:
+        """Batched global registration: every start in T_starts (n,4,4, align
+        init convention) is registered in one batched launch sequence (mode 0)
+        and scored by the cos angle at score_ell; mode 1 scores the initial
+        poses only and aligns from the best (get_nearest_init_pose,
+        CvoGPU.cpp:367-398)."""
+        own = []
+        if not isinstance(source, DeviceCloud):
+            source = self.upload(source)
+            own.append(source)
+        if not isinstance(target, DeviceCloud):
+            target = self.upload(target)
+            own.append(target)
+        Ts = np.ascontiguousarray(np.asarray(T_starts, np.float32).reshape(-1, 16))
+        n = Ts.shape[0]
+        T_best = np.zeros(16, np.float32)
+        per_T = np.zeros((n, 16), np.float32)
+        per_cos = np.zeros(n, np.float64)
+        per_it = np.zeros(n, np.int32)
+        best = C.c_int32()
+        secs = C.c_double()
+        _check(lib().cvo_align_multistart(self._ctx, source.handle, target.handle, n, Ts.ctypes.data,
+                                          C.c_float(score_ell), int(mode), T_best.ctypes.data, C.byref(best),
+                                          per_T.ctypes.data, per_cos.ctypes.data, per_it.ctypes.data,
+                                          C.byref(secs)), self._ctx)
+        for c in own:
+            c.free()
+        return {"T_best": T_best.reshape(4, 4), "best_index": best.value, "per_start_T": per_T.reshape(n, 4, 4),
+                "per_start_cos": per_cos, "per_start_iters": per_it, "seconds": secs.value}
 
     def inner_product_gpu(self, source, target, init_guess_transform, ell: float) -> float:
         """inner_product_gpu (CvoGPU.cu:1734-1748)."""
