@@ -250,6 +250,23 @@ cvo_status cvo_align_multistart(cvo_ctx* ctx, const cvo_dcloud* src, const cvo_d
                                 int32_t* best_index, float* per_start_T, double* per_start_cos,
                                 int32_t* per_start_iters, double* seconds);
 
+/* ---- multi-frame IRLS bundle adjustment ---------------------------------- */
+
+/* Replaces CvoGPU::align(frames, hold_const, edges, seconds)
+ * (src/cvo/CvoGPU.cu:1614-1671) together with CvoBatchIRLS::solve
+ * (src/cvo/IRLS.cpp:148-367) and BinaryStateGPU (src/cvo/IRLS_State_GPU.cu:
+ * 144-336, src/cvo/IRLS_State_GPU.cpp:10-146).  frames: n_frames device
+ * clouds (sensor frame).  poses: n_frames * 12 doubles, row-major 3x4
+ * world <- sensor (CvoFrame::pose_vec), updated in place.  hold_const
+ * (nullable): nonzero keeps a frame fixed.  edges: 2 * n_edges frame indices
+ * (f1, f2); each edge associates f1's points (rows) with f2's.  Limits: 64
+ * frames, 20 free frames, 4096 edges.  per_iter_seconds (optional, capacity
+ * max_iter_records) receives the wall time of each IRLS outer iteration;
+ * seconds the whole call; iters the outer iterations run. */
+cvo_status cvo_align_multiframe(cvo_ctx* ctx, int32_t n_frames, const cvo_dcloud* const* frames, double* poses,
+                                const uint8_t* hold_const, int32_t n_edges, const int32_t* edges, double* seconds,
+                                int32_t* iters, double* per_iter_seconds, int32_t max_iter_records);
+
 /* ---- profiling ----------------------------------------------------------- */
 
 cvo_status cvo_set_profiling(cvo_ctx* ctx, int32_t on);

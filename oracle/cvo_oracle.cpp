@@ -26,6 +26,8 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <complex>
 #include <cstdint>
 #include <cstring>
@@ -1310,6 +1312,9 @@ extern "C" int orc_align_multiframe(const cvo_params* p, int32_t n_frames, const
       assoc_core(p, S, Tg, I64, (float)ell[e], K[e], ORC_MIRROR, 0, o);
       nnz[e] = o.cnt.stored;
       last_max[e] = o.cnt.max_row;
+      if (std::getenv("ORC_BA_DEBUG"))
+        std::fprintf(stderr, "it %d edge %d (%d,%d) ell %.6f K %d nnz %lld max_row %d\n", it, e, f1, f2, ell[e], K[e],
+                     (long long)nnz[e], last_max[e]);
       if (nnz[e] > p->multiframe_min_nonzeros) counter_nonzero++;
       on[e] = (nnz[e] > 0 && nnz[e] >= p->multiframe_min_nonzeros);
       if (!on[e]) continue;
@@ -1367,6 +1372,8 @@ extern "C" int orc_align_multiframe(const cvo_params* p, int32_t n_frames, const
         }
         const double pred = -(gd + 0.5 * dHd);
         const double rho = pred > 0 ? (F - Fc) / pred : -1.0;
+        if (std::getenv("ORC_BA_DEBUG"))
+          std::fprintf(stderr, "it %d lm %d F %.9e Fc %.9e pred %.6e rho %.6e mu %.3e\n", it, lm, F, Fc, pred, rho, mu);
         if (rho > 1e-3) {
           const double rel = std::fabs(F - Fc) / std::max(F, 1e-300);
           X = Xc;

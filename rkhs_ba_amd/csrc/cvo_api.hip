@@ -657,6 +657,7 @@ extern "C" void cvo_destroy(cvo_ctx* ctx) {
                     &ctx->b_tmp2, &ctx->b_tmp3, &ctx->b_cub, &ctx->b_export};
   for (DevBuf* b : bufs) free_buf(*b);
   for (auto& b : ctx->b_ba) free_buf(b);
+  for (auto& b : ctx->b_frames) free_buf(b);
   for (auto& b : ctx->b_mlp) free_buf(b);
   for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
   if (ctx->pinned) (void)hipHostFree(ctx->pinned);

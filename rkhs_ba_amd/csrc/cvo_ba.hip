@@ -1,0 +1,956 @@
+// Multi-frame IRLS bundle adjustment on the device: CvoGPU::align over frames
+// and edges (src/cvo/CvoGPU.cu:1614-1671) with CvoBatchIRLS::solve
+// (src/cvo/IRLS.cpp:148-367) and BinaryStateGPU (src/cvo/IRLS_State_GPU.cu:
+// 144-336, src/cvo/IRLS_State_GPU.cpp:10-146).
+//
+// One IRLS outer iteration is one replay of a captured graph:
+//   ba_prepare        per-edge maps (world transforms of both frames), cap K,
+//                     grid level
+//   assoc / overflow  per-edge association between the frames' world points
+//   pass_totals       per-edge stored pairs and largest row
+//   ba_edges          edge on/off; stop when no edge has enough pairs
+//   (ba_pair -> ba_edge_reduce -> ba_solve) x (iterations_per_solve + 1)
+//                     Levenberg-Marquardt over the fixed pairs
+//   ba_finish         per-edge lengthscale, pose change, convergence
+//
+// Residuals and Jacobians are the reference's PairwiseAnalyticalDiffFunctor
+// (src/cvo/IRLS_Cost_CPU.hpp:305-357): r = a |T1 p1 - T2 p2|^2 with
+// J1 = diff^T dT1, J2 = -diff^T dT2 (no factor 2a), in the local
+// parameterisation T <- T Exp([rho; phi]) (local_parameterization_se3.hpp:
+// 21-99).  With w = P1 - P2 (world), u = [w; P1 x w] the local Jacobians are
+// j1 = M1 u and j2 = -M2 u, M = [[R^T, 0], [-R^T [t]x, R^T]], so each edge only
+// accumulates S = sum u u^T, h = sum r u and c = sum r^2 over its pairs.
+//
+// The Ceres solve is replaced by this Levenberg-Marquardt schedule:
+//   D = clamp(diag(H), 1e-6, 1e32); solve (H + D/mu) d = -g;
+//   rho = (F - F_new) / (-(g.d + d.H.d/2)); accept if rho > 1e-3:
+//     mu /= max(1/3, 1 - (2 rho - 1)^3) (capped at 1e16), v = 2;
+//   else mu /= v, v *= 2.  mu = 1e4 at the start of every solve; stop on
+//   iterations_per_solve candidates, |dF|/F < 1e-5 after an accepted step,
+//   ||g||inf < 1e-5 or ||d|| < 1e-5 (||x|| + 1e-5).
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "cvo_device.h"
+#include "cvo_internal.h"
+
+namespace cvo {
+
+constexpr int kBaMaxFrames = 64;
+constexpr int kBaMaxFree = 20;                 // free frames: 120 dof, Cholesky in 115 KB of LDS
+constexpr int kBaMaxDof = 6 * kBaMaxFree;
+constexpr int kBaMaxEdges = 4096;
+constexpr int kBaTerms = 28;                   // S (21, upper triangle), h (6), c (1)
+constexpr int kBaContrib = 121;                // B11, B12, B22 (36 each), g1, g2 (6 each), c
+constexpr int kBaRowsPerBlock = 256;           // pair pass: one row per thread
+
+struct BaEdge {
+  double ell;
+  int64_t nnz, nnz_last;
+  int32_t f1, f2;
+  int32_t K, last_max, iters_per_ell, on;
+  int32_t n_rows, n_pair_blocks;
+};
+
+struct BaDev {
+  int32_t n_frames, n_edges, nfree, n;
+  int32_t iter, done, max_iters, iters_per_solve;
+  int32_t lm_iter, lm_stop, eval_pending, eval_is_candidate;
+  int32_t K_init, min_nonzeros, iter_per_ell, counter_nonzero;
+  int32_t stopped_no_edges, accepted_total, evals_total, pad0;
+  float ell_min_f, decay_f, rq_coef, pad1;
+  double ell_min, ell_max;
+  double F, Fn, mu, v, change;
+  int32_t free_idx[kBaMaxFrames];
+  int32_t frame_of_free[kBaMaxFree];
+  float frame_norm[kBaMaxFrames];
+  double X[kBaMaxFrames * 12];     // current poses, row-major 3x4 (world <- sensor)
+  double Xe[kBaMaxFrames * 12];    // poses of the pending pair pass
+  double Xold[kBaMaxFrames * 12];  // poses at the start of the outer iteration
+  double g[kBaMaxDof], gn[kBaMaxDof], d[kBaMaxDof];
+};
+
+// ---------------------------------------------------------------------------
+// Helpers
+// ---------------------------------------------------------------------------
+
+// M = [[R^T, 0], [-R^T [t]x, R^T]] of a 3x4 pose, row-major 6x6.
+__device__ inline void pose_M(const double* T, double M[36]) {
+  double Rt[9];
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) Rt[r * 3 + c] = T[c * 4 + r];
+  const double t0 = T[3], t1 = T[7], t2 = T[11];
+  const double tx[9] = {0.0, -t2, t1, t2, 0.0, -t0, -t1, t0, 0.0};
+  for (int i = 0; i < 36; ++i) M[i] = 0.0;
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) {
+      M[r * 6 + c] = Rt[r * 3 + c];
+      M[(r + 3) * 6 + c + 3] = Rt[r * 3 + c];
+      double s = 0.0;
+      for (int k = 0; k < 3; ++k) s += Rt[r * 3 + k] * tx[k * 3 + c];
+      M[(r + 3) * 6 + c] = -s;
+    }
+}
+
+// T <- T Exp([rho; phi]), translation part first.
+__device__ inline void pose_plus(const double* T, const double* dlt, double* out) {
+  const double xi[6] = {dlt[3], dlt[4], dlt[5], dlt[0], dlt[1], dlt[2]};
+  double dR[9], dT[3];
+  se3_exp(xi, 1.0, dR, dT);
+  for (int r = 0; r < 3; ++r) {
+    for (int c = 0; c < 3; ++c)
+      out[r * 4 + c] = T[r * 4 + 0] * dR[0 * 3 + c] + T[r * 4 + 1] * dR[1 * 3 + c] + T[r * 4 + 2] * dR[2 * 3 + c];
+    out[r * 4 + 3] = T[r * 4 + 0] * dT[0] + T[r * 4 + 1] * dT[1] + T[r * 4 + 2] * dT[2] + T[r * 4 + 3];
+  }
+}
+
+// Float maps of an edge: source and target world transforms, and the query
+// map from the world into the target's sensor frame.
+__device__ inline void edge_maps(const double* T1, const double* T2, ProblemState* st) {
+  for (int r = 0; r < 3; ++r) {
+    for (int c = 0; c < 3; ++c) {
+      st->As[r * 3 + c] = (float)T1[r * 4 + c];
+      st->A[r * 3 + c] = (float)T2[r * 4 + c];
+      st->Rq[r * 3 + c] = (float)T2[c * 4 + r];
+    }
+    st->bs[r] = (float)T1[r * 4 + 3];
+    st->b[r] = (float)T2[r * 4 + 3];
+    st->Tq[r] = (float)(-(T2[0 * 4 + r] * T2[3] + T2[1 * 4 + r] * T2[7] + T2[2 * 4 + r] * T2[11]));
+  }
+  st->has_src_xf = 1;
+}
+
+// ---------------------------------------------------------------------------
+// Kernels
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void ba_prepare_kernel(BaDev* __restrict__ ba, BaEdge* __restrict__ edges,
+                                                         const AssocProblem* __restrict__ probs) {
+  const int tid = threadIdx.x;
+  const bool done = ba->done != 0;
+  const int E = ba->n_edges;
+  if (done) {
+    for (int e = tid; e < E; e += blockDim.x) probs[e].st->done = 1;
+    return;
+  }
+  for (int i = tid; i < ba->n_frames * 12; i += blockDim.x) {
+    ba->Xold[i] = ba->X[i];
+    ba->Xe[i] = ba->X[i];
+  }
+  for (int e = tid; e < E; e += blockDim.x) {
+    BaEdge& ed = edges[e];
+    ProblemState* st = probs[e].st;
+    ed.nnz_last = ed.nnz;
+    if (ed.last_max > 0) ed.K = min(ba->K_init, (int)((double)ed.last_max * 1.1));
+    if (ed.K < 1) ed.K = 1;
+    const double* T1 = ba->X + 12 * ed.f1;
+    const double* T2 = ba->X + 12 * ed.f2;
+    edge_maps(T1, T2, st);
+    st->ell = (float)ed.ell;
+    st->K = ed.K;
+    st->iter = ba->iter;
+    st->done = 0;
+    st->n_overflow = 0;
+    // query radius bound at this lengthscale over the world-frame source
+    const double tn = sqrt(T1[3] * T1[3] + T1[7] * T1[7] + T1[11] * T1[11]);
+    const float sn = ba->frame_norm[ed.f1] + (float)tn;
+    const float tq = fmaxf(fabsf(st->Tq[0]), fmaxf(fabsf(st->Tq[1]), fabsf(st->Tq[2])));
+    const float r_need =
+        ba->rq_coef * st->ell * (1.0f + sn / 500.0f) * 1.0001f + 1e-4f * (1.0f + sn + tq) + 1e-6f;
+    st->level = choose_level(probs[e].grid, r_need);
+  }
+}
+
+// Edge switches after the association (IRLS.cpp:235-250,
+// IRLS_State_GPU.cpp:17-21) and the start of the Levenberg-Marquardt solve.
+__global__ __launch_bounds__(256) void ba_edges_kernel(BaDev* __restrict__ ba, BaEdge* __restrict__ edges,
+                                                       const AssocProblem* __restrict__ probs) {
+  if (ba->done) return;
+  const int tid = threadIdx.x;
+  __shared__ int s_cnt[256];
+  int cnt = 0;
+  for (int e = tid; e < ba->n_edges; e += blockDim.x) {
+    BaEdge& ed = edges[e];
+    const ProblemState* st = probs[e].st;
+    ed.nnz = (int64_t)st->nnz;
+    ed.last_max = (int32_t)st->max_row;
+    ed.on = (ed.nnz > 0 && ed.nnz >= (int64_t)ba->min_nonzeros) ? 1 : 0;
+    if (ed.nnz > (int64_t)ba->min_nonzeros) cnt++;
+  }
+  s_cnt[tid] = cnt;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (tid < w) s_cnt[tid] += s_cnt[tid + w];
+    __syncthreads();
+  }
+  if (tid == 0) {
+    ba->counter_nonzero = s_cnt[0];
+    if (s_cnt[0] == 0) {
+      ba->done = 1;
+      ba->stopped_no_edges = 1;
+      ba->eval_pending = 0;
+      return;
+    }
+    ba->mu = 1e4;
+    ba->v = 2.0;
+    ba->lm_iter = 0;
+    ba->lm_stop = 0;
+    ba->eval_is_candidate = 0;
+    ba->eval_pending = ba->n > 0 ? 1 : 0;
+  }
+}
+
+// FP64 pair pass at the poses Xe: one thread per source row of an edge, block
+// partials of S (21), h (6), c (1).
+__global__ __launch_bounds__(kBaRowsPerBlock) void ba_pair_kernel(const BaDev* __restrict__ ba,
+                                                                  const BaEdge* __restrict__ edges,
+                                                                  const AssocProblem* __restrict__ probs,
+                                                                  double* __restrict__ part, int max_blocks) {
+  if (ba->done || !ba->eval_pending) return;
+  const int e = blockIdx.y;
+  const BaEdge& ed = edges[e];
+  if (!ed.on || (int)blockIdx.x >= ed.n_pair_blocks) return;
+  const AssocProblem& P = probs[e];
+  const int tid = threadIdx.x;
+  const int r = blockIdx.x * kBaRowsPerBlock + tid;
+  double T1[12], T2[12];
+#pragma unroll
+  for (int i = 0; i < 12; ++i) {
+    T1[i] = ba->Xe[12 * ed.f1 + i];
+    T2[i] = ba->Xe[12 * ed.f2 + i];
+  }
+  double acc[kBaTerms];
+#pragma unroll
+  for (int k = 0; k < kBaTerms; ++k) acc[k] = 0.0;
+  if (r < P.n_src) {
+    const GridLevel& G = P.grid->lv[P.st->level_assoc];
+    const float4 x4 = P.src_pos[r];
+    const double p0 = x4.x, p1 = x4.y, p2 = x4.z;
+    const double X0 = T1[0] * p0 + T1[1] * p1 + T1[2] * p2 + T1[3];
+    const double X1 = T1[4] * p0 + T1[5] * p1 + T1[6] * p2 + T1[7];
+    const double X2 = T1[8] * p0 + T1[9] * p1 + T1[10] * p2 + T1[11];
+    const uint32_t cnt = P.counts[r];
+    const uint2* row = P.pairs + (size_t)r * P.cap_alloc;
+    for (uint32_t s = 0; s < cnt; ++s) {
+      const uint2 pe = row[s];
+      const double a = (double)__uint_as_float(pe.y);
+      const float4 y4 = G.pos[pe.x];
+      const double q0 = y4.x, q1 = y4.y, q2 = y4.z;
+      const double w0 = X0 - (T2[0] * q0 + T2[1] * q1 + T2[2] * q2 + T2[3]);
+      const double w1 = X1 - (T2[4] * q0 + T2[5] * q1 + T2[6] * q2 + T2[7]);
+      const double w2 = X2 - (T2[8] * q0 + T2[9] * q1 + T2[10] * q2 + T2[11]);
+      const double res = a * (w0 * w0 + w1 * w1 + w2 * w2);
+      const double u[6] = {w0, w1, w2, X1 * w2 - X2 * w1, X2 * w0 - X0 * w2, X0 * w1 - X1 * w0};
+      int k = 0;
+#pragma unroll
+      for (int i = 0; i < 6; ++i)
+#pragma unroll
+        for (int j = i; j < 6; ++j) acc[k++] += u[i] * u[j];
+#pragma unroll
+      for (int i = 0; i < 6; ++i) acc[21 + i] += res * u[i];
+      acc[27] += res * res;
+    }
+  }
+  // fixed-order block partial: wave shuffles, then the four waves in LDS
+  __shared__ double s_w[kBaTerms][kBaRowsPerBlock / 64];
+  const int lane = tid & 63, wave = tid >> 6;
+#pragma unroll
+  for (int k = 0; k < kBaTerms; ++k) {
+    double v = acc[k];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if (lane == 0) s_w[k][wave] = v;
+  }
+  __syncthreads();
+  if (tid < kBaTerms) {
+    double v = 0.0;
+    for (int w = 0; w < kBaRowsPerBlock / 64; ++w) v += s_w[tid][w];
+    part[((size_t)e * max_blocks + blockIdx.x) * kBaTerms + tid] = v;
+  }
+}
+
+// Per-edge totals of the pair pass and the edge's blocks of the normal
+// equations at the poses Xe: B11 = M1 S M1^T, B12 = -M1 S M2^T,
+// B22 = M2 S M2^T, g1 = M1 h, g2 = -M2 h.
+__global__ __launch_bounds__(256) void ba_edge_reduce_kernel(const BaDev* __restrict__ ba,
+                                                             const BaEdge* __restrict__ edges,
+                                                             const double* __restrict__ part, int max_blocks,
+                                                             double* __restrict__ contrib) {
+  if (ba->done || !ba->eval_pending) return;
+  const int e = blockIdx.x;
+  const BaEdge& ed = edges[e];
+  if (!ed.on) return;
+  const int tid = threadIdx.x;
+  __shared__ double s_p[kBaTerms][8];
+  __shared__ double s_t[kBaTerms];
+  __shared__ double s_S[36], s_M1[36], s_M2[36], s_T1[36], s_T2[36];
+  if (tid < kBaTerms * 8) {
+    const int k = tid >> 3, l = tid & 7;
+    double v = 0.0;
+    for (int b = l; b < ed.n_pair_blocks; b += 8) v += part[((size_t)e * max_blocks + b) * kBaTerms + k];
+    s_p[k][l] = v;
+  }
+  if (tid == 0) pose_M(ba->Xe + 12 * ed.f1, s_M1);
+  if (tid == 1) pose_M(ba->Xe + 12 * ed.f2, s_M2);
+  __syncthreads();
+  if (tid < kBaTerms) {
+    double v = 0.0;
+    for (int l = 0; l < 8; ++l) v += s_p[tid][l];
+    s_t[tid] = v;
+  }
+  __syncthreads();
+  if (tid < 36) {
+    // full symmetric S from the packed upper triangle
+    const int i = tid / 6, j = tid % 6;
+    const int a = min(i, j), b = max(i, j);
+    const int k = a * 6 - a * (a - 1) / 2 + (b - a);
+    s_S[tid] = s_t[k];
+  }
+  __syncthreads();
+  if (tid < 72) {
+    const int which = tid / 36, idx = tid % 36, i = idx / 6, j = idx % 6;
+    const double* M = which ? s_M2 : s_M1;
+    double v = 0.0;
+    for (int k = 0; k < 6; ++k) v += M[i * 6 + k] * s_S[k * 6 + j];
+    (which ? s_T2 : s_T1)[idx] = v;
+  }
+  __syncthreads();
+  double* out = contrib + (size_t)e * kBaContrib;
+  if (tid < 108) {
+    const int blk = tid / 36, idx = tid % 36, i = idx / 6, j = idx % 6;
+    const double* Tm = (blk == 2) ? s_T2 : s_T1;
+    const double* M = (blk == 0) ? s_M1 : s_M2;
+    double v = 0.0;
+    for (int k = 0; k < 6; ++k) v += Tm[i * 6 + k] * M[j * 6 + k];
+    out[tid] = (blk == 1) ? -v : v;
+  } else if (tid < 120) {
+    const int which = (tid - 108) / 6, i = (tid - 108) % 6;
+    const double* M = which ? s_M2 : s_M1;
+    double v = 0.0;
+    for (int k = 0; k < 6; ++k) v += M[i * 6 + k] * s_t[21 + k];
+    out[tid] = which ? -v : v;
+  } else if (tid == 120) {
+    out[120] = s_t[27];
+  }
+}
+
+// Assembles H, g and F at the poses Xe from the edge blocks, applies the
+// Levenberg-Marquardt decision for a candidate, and prepares the next
+// candidate (or stops the solve).  One workgroup; the damped system is
+// factored in LDS.
+__global__ __launch_bounds__(256) void ba_solve_kernel(BaDev* __restrict__ ba, const BaEdge* __restrict__ edges,
+                                                       const double* __restrict__ contrib, double* __restrict__ H,
+                                                       double* __restrict__ Hn) {
+  if (ba->done || !ba->eval_pending) return;
+  const int tid = threadIdx.x;
+  const int n = ba->n, E = ba->n_edges;
+  __shared__ double s_A[kBaMaxDof * kBaMaxDof];
+  __shared__ double s_b[kBaMaxDof];
+  __shared__ double s_red[256];
+  __shared__ int s_e1[256], s_e2[256];
+  __shared__ int s_flag, s_fail;
+  __shared__ double s_scalar[4];
+
+  // (1) assemble Hn (full), gn and Fn in a fixed edge order
+  for (int e0 = 0; e0 < E; e0 += 256) {
+    const int ne = min(256, E - e0);
+    __syncthreads();
+    if (tid < ne) {
+      const BaEdge& ed = edges[e0 + tid];
+      s_e1[tid] = ed.on ? ba->free_idx[ed.f1] : -2;
+      s_e2[tid] = ed.on ? ba->free_idx[ed.f2] : -2;
+    }
+    __syncthreads();
+    for (int idx = tid; idx < n * n; idx += 256) {
+      const int i = idx / n, j = idx - (idx / n) * n;
+      const int fi = i / 6, fj = j / 6, a = i % 6, b = j % 6;
+      double s = (e0 == 0) ? 0.0 : Hn[idx];
+      for (int q = 0; q < ne; ++q) {
+        const int o1 = s_e1[q], o2 = s_e2[q];
+        if (o1 == -2) continue;
+        const double* C = contrib + (size_t)(e0 + q) * kBaContrib;
+        if (o1 == fi && o1 == fj) s += C[a * 6 + b];
+        if (o2 == fi && o2 == fj) s += C[72 + a * 6 + b];
+        if (o1 == fi && o2 == fj) s += C[36 + a * 6 + b];
+        if (o2 == fi && o1 == fj) s += C[36 + b * 6 + a];
+      }
+      Hn[idx] = s;
+    }
+    for (int i = tid; i < n; i += 256) {
+      const int fi = i / 6, a = i % 6;
+      double s = (e0 == 0) ? 0.0 : ba->gn[i];
+      for (int q = 0; q < ne; ++q) {
+        const int o1 = s_e1[q], o2 = s_e2[q];
+        if (o1 == -2) continue;
+        const double* C = contrib + (size_t)(e0 + q) * kBaContrib;
+        if (o1 == fi) s += C[108 + a];
+        if (o2 == fi) s += C[114 + a];
+      }
+      ba->gn[i] = s;
+    }
+    if (tid == 0) {
+      double s = (e0 == 0) ? 0.0 : ba->Fn;
+      for (int q = 0; q < ne; ++q)
+        if (s_e1[q] != -2) s += 0.5 * contrib[(size_t)(e0 + q) * kBaContrib + 120];
+      ba->Fn = s;
+    }
+  }
+  __syncthreads();
+
+  // (2) decision on the evaluated poses
+  if (!ba->eval_is_candidate) {
+    for (int i = tid; i < n * n; i += 256) H[i] = Hn[i];
+    for (int i = tid; i < n; i += 256) ba->g[i] = ba->gn[i];
+    if (tid == 0) ba->F = ba->Fn;
+  } else {
+    // predicted decrease -(g.d + d.H.d / 2) with H, g at the current poses
+    double part = 0.0;
+    for (int i = tid; i < n; i += 256) {
+      double hd = 0.0;
+      for (int j = 0; j < n; ++j) hd += H[(size_t)i * n + j] * ba->d[j];
+      part += ba->d[i] * (ba->g[i] + 0.5 * hd);
+    }
+    s_red[tid] = part;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+      if (tid < w) s_red[tid] += s_red[tid + w];
+      __syncthreads();
+    }
+    if (tid == 0) {
+      const double pred = -s_red[0];
+      const double F = ba->F, Fn = ba->Fn;
+      const double rho = pred > 0.0 ? (F - Fn) / pred : -1.0;
+      ba->evals_total++;
+      if (rho > 1e-3) {
+        const double rel = fabs(F - Fn) / fmax(F, 1e-300);
+        double mu = ba->mu / fmax(1.0 / 3.0, 1.0 - pow(2.0 * rho - 1.0, 3.0));
+        ba->mu = fmin(mu, 1e16);
+        ba->v = 2.0;
+        ba->F = Fn;
+        ba->accepted_total++;
+        s_flag = 1;
+        if (rel < 1e-5) ba->lm_stop = 1;
+      } else {
+        ba->mu /= ba->v;
+        ba->v *= 2.0;
+        s_flag = 0;
+      }
+      ba->lm_iter++;
+    }
+    __syncthreads();
+    if (s_flag) {
+      for (int i = tid; i < n * n; i += 256) H[i] = Hn[i];
+      for (int i = tid; i < n; i += 256) ba->g[i] = ba->gn[i];
+      for (int i = tid; i < ba->n_frames * 12; i += 256) ba->X[i] = ba->Xe[i];
+    }
+  }
+  __syncthreads();
+
+  // (3) next candidate
+  while (true) {
+    if (tid == 0) {
+      s_flag = 0;
+      if (ba->lm_stop || ba->lm_iter >= ba->iters_per_solve) {
+        s_flag = 1;
+      } else {
+        double gmax = 0.0;
+        for (int i = 0; i < n; ++i) gmax = fmax(gmax, fabs(ba->g[i]));
+        if (gmax < 1e-5) {
+          ba->lm_stop = 1;
+          s_flag = 1;
+        }
+      }
+      s_scalar[0] = ba->mu;
+      s_fail = 0;
+    }
+    __syncthreads();
+    if (s_flag) {
+      if (tid == 0) ba->eval_pending = 0;
+      return;
+    }
+    const double mu = s_scalar[0];
+    for (int idx = tid; idx < n * n; idx += 256) {
+      const int i = idx / n, j = idx - (idx / n) * n;
+      double v = H[idx];
+      if (i == j) v += fmin(fmax(v, 1e-6), 1e32) / mu;
+      s_A[idx] = v;
+    }
+    for (int i = tid; i < n; i += 256) s_b[i] = -ba->g[i];
+    __syncthreads();
+    // right-looking Cholesky, lower triangle
+    for (int j = 0; j < n; ++j) {
+      if (tid == 0) {
+        const double s = s_A[j * n + j];
+        if (!(s > 0.0)) s_fail = 1;
+        else s_A[j * n + j] = sqrt(s);
+      }
+      __syncthreads();
+      if (s_fail) break;
+      const double djj = s_A[j * n + j];
+      for (int i = j + 1 + tid; i < n; i += 256) s_A[i * n + j] /= djj;
+      __syncthreads();
+      const int m = n - j - 1;
+      for (int idx = tid; idx < m * m; idx += 256) {
+        const int i = j + 1 + idx / m, k = j + 1 + idx % m;
+        if (k <= i) s_A[i * n + k] -= s_A[i * n + j] * s_A[k * n + j];
+      }
+      __syncthreads();
+    }
+    if (s_fail) {
+      if (tid == 0) {
+        ba->mu /= ba->v;
+        ba->v *= 2.0;
+        ba->lm_iter++;
+      }
+      __syncthreads();
+      continue;
+    }
+    // L y = b, L^T d = y
+    for (int i = 0; i < n; ++i) {
+      if (tid == 0) s_b[i] /= s_A[i * n + i];
+      __syncthreads();
+      const double yi = s_b[i];
+      for (int k = i + 1 + tid; k < n; k += 256) s_b[k] -= s_A[k * n + i] * yi;
+      __syncthreads();
+    }
+    for (int i = n - 1; i >= 0; --i) {
+      if (tid == 0) s_b[i] /= s_A[i * n + i];
+      __syncthreads();
+      const double xi = s_b[i];
+      for (int k = tid; k < i; k += 256) s_b[k] -= s_A[i * n + k] * xi;
+      __syncthreads();
+    }
+    if (tid == 0) {
+      double dn = 0.0, xn = 0.0;
+      for (int i = 0; i < n; ++i) dn += s_b[i] * s_b[i];
+      for (int i = 0; i < ba->n_frames * 12; ++i) xn += ba->X[i] * ba->X[i];
+      s_flag = (sqrt(dn) < 1e-5 * (sqrt(xn) + 1e-5)) ? 1 : 0;
+      if (s_flag) {
+        ba->lm_stop = 1;
+        ba->eval_pending = 0;
+      }
+    }
+    __syncthreads();
+    if (s_flag) return;
+    for (int i = tid; i < n; i += 256) ba->d[i] = s_b[i];
+    for (int f = tid; f < ba->n_frames; f += 256) {
+      const int o = ba->free_idx[f];
+      if (o >= 0) pose_plus(ba->X + 12 * f, s_b + 6 * o, ba->Xe + 12 * f);
+      else
+        for (int i = 0; i < 12; ++i) ba->Xe[12 * f + i] = ba->X[12 * f + i];
+    }
+    if (tid == 0) {
+      ba->eval_pending = 1;
+      ba->eval_is_candidate = 1;
+    }
+    return;
+  }
+}
+
+// Per-edge lengthscale (BinaryStateGPU::update_ell, IRLS_State_GPU.cpp:
+// 117-146, mode 0), pose change (IRLS.cpp:99-108) and convergence
+// (IRLS.cpp:325-339).
+__global__ void ba_finish_kernel(BaDev* __restrict__ ba, BaEdge* __restrict__ edges) {
+  if (ba->done || threadIdx.x != 0) return;
+  bool all_min = true;
+  for (int e = 0; e < ba->n_edges; ++e) {
+    BaEdge& ed = edges[e];
+    ed.iters_per_ell++;
+    if (ed.nnz_last > 0 && ed.nnz <= ed.nnz_last && ed.iters_per_ell / ba->iter_per_ell > 0) {
+      ed.ell = ed.ell * (double)ba->decay_f;
+      ed.iters_per_ell = 0;
+    }
+    if (ed.ell < ba->ell_min) ed.ell = ba->ell_min;
+    else if (ed.ell > ba->ell_max) ed.ell = ba->ell_max;
+    all_min = all_min && fabsf((float)ed.ell - ba->ell_min_f) < 1e-4f;
+  }
+  double change = 0.0;
+  for (int f = 0; f < ba->n_frames; ++f) {
+    const double* A = ba->Xold + 12 * f;
+    const double* B = ba->X + 12 * f;
+    double R[9], t[3];
+    for (int r = 0; r < 3; ++r) {
+      for (int c = 0; c < 3; ++c)
+        R[r * 3 + c] = A[0 * 4 + r] * B[0 * 4 + c] + A[1 * 4 + r] * B[1 * 4 + c] + A[2 * 4 + r] * B[2 * 4 + c];
+      t[r] = A[0 * 4 + r] * (B[3] - A[3]) + A[1 * 4 + r] * (B[7] - A[7]) + A[2 * 4 + r] * (B[11] - A[11]);
+    }
+    change += se3_log_norm(R, t);
+  }
+  ba->change = change;
+  ba->iter++;
+  if ((all_min && change < 1e-5) || ba->iter >= ba->max_iters) ba->done = 1;
+}
+
+// ---------------------------------------------------------------------------
+// Host
+// ---------------------------------------------------------------------------
+
+template <class T>
+static T* as(DevBuf& b) {
+  return reinterpret_cast<T*>(b.p);
+}
+
+enum {
+  BA_PAIRS = 0, BA_COUNTS, BA_OVF, BA_STATE, BA_PROBS, BA_PART_A, BA_PART_OVF, BA_PART_CNT,
+  BA_DEV, BA_EDGES, BA_H, BA_HN, BA_PAIR_PART, BA_CONTRIB, BA_FLAG
+};
+
+// Six buffers per frame: sorted source (pos, feat, sem, geo), grid store and
+// grid header.
+constexpr int kFrameBufs = 6;
+
+static cvo_status validate_frames(cvo_ctx* ctx, int n_frames, const cvo_dcloud* const* frames, const double* poses,
+                                  const uint8_t* hold_const, int n_edges, const int32_t* edges, int* nfree_out) {
+  const cvo_params& p = ctx->params;
+  if (n_frames <= 0 || !frames || !poses) return set_err(ctx, CVO_ERR_INVALID_ARG, "no frames");
+  if (n_frames > kBaMaxFrames) return set_err(ctx, CVO_ERR_LIMIT, "at most 64 frames");
+  if (n_edges < 0 || n_edges > kBaMaxEdges) return set_err(ctx, CVO_ERR_LIMIT, "edge count out of [0, 4096]");
+  if (n_edges > 0 && !edges) return set_err(ctx, CVO_ERR_INVALID_ARG, "null edge list");
+  for (int f = 0; f < n_frames; ++f) {
+    if (!frames[f]) return set_err(ctx, CVO_ERR_INVALID_ARG, "null frame");
+    if (frames[f]->n == 0) return set_err(ctx, CVO_ERR_EMPTY, "empty frame");
+    if (frames[f]->F != frames[0]->F || frames[f]->K != frames[0]->K)
+      return set_err(ctx, CVO_ERR_INVALID_ARG, "frames differ in feature or class dimension");
+    for (int i = 0; i < 12; ++i)
+      if (!std::isfinite(poses[12 * f + i])) return set_err(ctx, CVO_ERR_INVALID_ARG, "non-finite pose");
+  }
+  int nfree = 0;
+  for (int f = 0; f < n_frames; ++f)
+    if (!hold_const || !hold_const[f]) nfree++;
+  if (nfree > kBaMaxFree) return set_err(ctx, CVO_ERR_LIMIT, "at most 20 frames may be free");
+  for (int e = 0; e < n_edges; ++e) {
+    const int a = edges[2 * e], b = edges[2 * e + 1];
+    if (a < 0 || a >= n_frames || b < 0 || b >= n_frames || a == b)
+      return set_err(ctx, CVO_ERR_INVALID_ARG, "edge frame index out of range or self edge");
+    CVO_TRY(validate_pair(ctx, frames[a], frames[b]));
+  }
+  if (p.multiframe_num_neighbors < 1 || p.multiframe_num_neighbors > 1024)
+    return set_err(ctx, CVO_ERR_LIMIT, "multiframe_num_neighbors must be in [1, 1024]");
+  if (p.multiframe_iter_per_ell <= 0) return set_err(ctx, CVO_ERR_INVALID_ARG, "multiframe_iter_per_ell must be > 0");
+  if (!(p.multiframe_ell_init > 0.0f) || !(p.multiframe_ell_min > 0.0f))
+    return set_err(ctx, CVO_ERR_INVALID_ARG, "multiframe lengthscales must be positive");
+  *nfree_out = nfree;
+  return CVO_OK;
+}
+
+void launch_assoc(const AssocProblem* probs, int n_probs, int total_blocks, const KConsts& kc, hipStream_t s);
+void launch_overflow(const AssocProblem* probs, int n_probs, const KConsts& kc, hipStream_t s);
+void launch_pass_totals(const AssocProblem* probs, int n_probs, hipStream_t s);
+
+struct BaLaunch {
+  BaDev* ba;
+  BaEdge* edges;
+  AssocProblem* probs;
+  int n_edges, total_blocks, max_pair_blocks, lm_rounds;
+  double* part;
+  double* contrib;
+  double* H;
+  double* Hn;
+  KConsts kc;
+};
+
+static void ba_assoc_phase(const BaLaunch& L, hipStream_t s) {
+  hipLaunchKernelGGL(ba_prepare_kernel, dim3(1), dim3(256), 0, s, L.ba, L.edges, L.probs);
+  if (L.n_edges > 0) {
+    launch_assoc(L.probs, L.n_edges, L.total_blocks, L.kc, s);
+    launch_overflow(L.probs, L.n_edges, L.kc, s);
+    launch_pass_totals(L.probs, L.n_edges, s);
+  }
+  hipLaunchKernelGGL(ba_edges_kernel, dim3(1), dim3(256), 0, s, L.ba, L.edges, L.probs);
+}
+
+static void ba_lm_round(const BaLaunch& L, hipStream_t s) {
+  hipLaunchKernelGGL(ba_pair_kernel, dim3(L.max_pair_blocks, L.n_edges), dim3(kBaRowsPerBlock), 0, s, L.ba,
+                     L.edges, L.probs, L.part, L.max_pair_blocks);
+  hipLaunchKernelGGL(ba_edge_reduce_kernel, dim3(L.n_edges), dim3(256), 0, s, L.ba, L.edges, L.part,
+                     L.max_pair_blocks, L.contrib);
+  hipLaunchKernelGGL(ba_solve_kernel, dim3(1), dim3(256), 0, s, L.ba, L.edges, L.contrib, L.H, L.Hn);
+}
+
+static void ba_iteration(const BaLaunch& L, hipStream_t s) {
+  ba_assoc_phase(L, s);
+  if (L.n_edges > 0)
+    for (int k = 0; k < L.lm_rounds; ++k) ba_lm_round(L, s);
+  hipLaunchKernelGGL(ba_finish_kernel, dim3(1), dim3(64), 0, s, L.ba, L.edges);
+}
+
+}  // namespace cvo
+
+using namespace cvo;
+
+extern "C" cvo_status cvo_align_multiframe(cvo_ctx* ctx, int32_t n_frames, const cvo_dcloud* const* frames,
+                                           double* poses, const uint8_t* hold_const, int32_t n_edges,
+                                           const int32_t* edges, double* seconds, int32_t* iters,
+                                           double* per_iter_seconds, int32_t max_iter_records) {
+  if (!ctx) return CVO_ERR_INVALID_ARG;
+  if (seconds) *seconds = 0.0;
+  if (iters) *iters = 0;
+  int nfree = 0;
+  CVO_TRY(validate_frames(ctx, n_frames, frames, poses, hold_const, n_edges, edges, &nfree));
+  const cvo_params& p = ctx->params;
+  hipStream_t st = ctx->stream;
+  const auto t_start = std::chrono::steady_clock::now();
+  KConsts kc = make_kconsts(p, frames[0]->F, frames[0]->K, MODE_BA);
+
+  // frames that are sources / targets of some edge
+  std::vector<char> is_src(n_frames, 0), is_tgt(n_frames, 0);
+  for (int e = 0; e < n_edges; ++e) {
+    is_src[edges[2 * e]] = 1;
+    is_tgt[edges[2 * e + 1]] = 1;
+  }
+  float max_world = 0.0f, max_t = 0.0f;
+  for (int f = 0; f < n_frames; ++f) {
+    const double* T = poses + 12 * f;
+    const float tn = (float)std::sqrt(T[3] * T[3] + T[7] * T[7] + T[11] * T[11]);
+    max_world = std::max(max_world, frames[f]->max_norm + tn);
+    max_t = std::max(max_t, tn);
+  }
+  if (ctx->b_frames.size() < (size_t)kFrameBufs * n_frames) ctx->b_frames.resize((size_t)kFrameBufs * n_frames);
+  std::vector<GridHost> grids(n_frames);
+  const float ell_hi = std::max(p.multiframe_ell_init * 1.2f, p.multiframe_ell_min);
+  const float ell_lo = std::min(p.multiframe_ell_min, p.multiframe_ell_init);
+  const std::vector<float> hs = level_sizes(kc, ell_hi, ell_lo, max_world * 1.05f, max_t * 1.05f + 1.0f);
+  for (int f = 0; f < n_frames; ++f) {
+    DevBuf* fb = &ctx->b_frames[(size_t)kFrameBufs * f];
+    if (is_src[f]) CVO_TRY(prepare_source_into(ctx, frames[f], fb[0], fb[1], fb[2], fb[3]));
+    if (is_tgt[f]) CVO_TRY(build_grid_into(ctx, frames[f], hs, grids[f], fb[4], fb[5]));
+  }
+
+  // per-edge association problems
+  const int cap = p.multiframe_num_neighbors;
+  size_t tot_rows = 0;
+  int total_blocks = 0, max_pair_blocks = 1;
+  std::vector<size_t> row_off(n_edges + 1, 0);
+  for (int e = 0; e < n_edges; ++e) {
+    const int n = frames[edges[2 * e]]->n;
+    row_off[e + 1] = row_off[e] + (size_t)n;
+    total_blocks += (n + kRowsPerBlock - 1) / kRowsPerBlock;
+    max_pair_blocks = std::max(max_pair_blocks, (n + kBaRowsPerBlock - 1) / kBaRowsPerBlock);
+  }
+  tot_rows = row_off[n_edges];
+  const size_t E1 = (size_t)std::max(n_edges, 1);
+  DevBuf* B = ctx->b_ba;
+  CVO_TRY(ensure(ctx, B[BA_PAIRS], sizeof(uint2) * std::max<size_t>(tot_rows, 1) * cap));
+  CVO_TRY(ensure(ctx, B[BA_COUNTS], sizeof(uint32_t) * std::max<size_t>(tot_rows, 1)));
+  CVO_TRY(ensure(ctx, B[BA_OVF], sizeof(uint32_t) * std::max<size_t>(tot_rows, 1)));
+  CVO_TRY(ensure(ctx, B[BA_STATE], sizeof(ProblemState) * E1));
+  CVO_TRY(ensure(ctx, B[BA_PROBS], sizeof(AssocProblem) * E1));
+  CVO_TRY(ensure(ctx, B[BA_PART_A], sizeof(double) * 8 * (size_t)std::max(total_blocks, 1)));
+  CVO_TRY(ensure(ctx, B[BA_PART_OVF], sizeof(double) * 10 * kOvfBlocks * E1));
+  CVO_TRY(ensure(ctx, B[BA_PART_CNT], sizeof(uint32_t) * 2 * (size_t)std::max(total_blocks, 1)));
+  CVO_TRY(ensure(ctx, B[BA_DEV], sizeof(BaDev)));
+  CVO_TRY(ensure(ctx, B[BA_EDGES], sizeof(BaEdge) * E1));
+  CVO_TRY(ensure(ctx, B[BA_H], sizeof(double) * kBaMaxDof * kBaMaxDof));
+  CVO_TRY(ensure(ctx, B[BA_HN], sizeof(double) * kBaMaxDof * kBaMaxDof));
+  CVO_TRY(ensure(ctx, B[BA_PAIR_PART], sizeof(double) * kBaTerms * (size_t)max_pair_blocks * E1));
+  CVO_TRY(ensure(ctx, B[BA_CONTRIB], sizeof(double) * kBaContrib * E1));
+  CVO_HIP(ctx, hipMemsetAsync(B[BA_OVF].p, 0, sizeof(uint32_t) * std::max<size_t>(tot_rows, 1), st));
+  CVO_HIP(ctx, hipMemsetAsync(B[BA_STATE].p, 0, sizeof(ProblemState) * E1, st));
+
+  std::vector<AssocProblem> hp(E1);
+  std::vector<BaEdge> he(E1);
+  int blk = 0;
+  for (int e = 0; e < n_edges; ++e) {
+    const int f1 = edges[2 * e], f2 = edges[2 * e + 1];
+    const cvo_dcloud* s = frames[f1];
+    const cvo_dcloud* t = frames[f2];
+    DevBuf* sb = &ctx->b_frames[(size_t)kFrameBufs * f1];
+    const int n = s->n;
+    const int nb = (n + kRowsPerBlock - 1) / kRowsPerBlock;
+    AssocProblem& P = hp[e];
+    std::memset(&P, 0, sizeof(P));
+    P.src_pos = as<float4>(sb[0]);
+    P.src_feat = s->feat ? as<float>(sb[1]) : nullptr;
+    P.src_sem = s->sem ? as<float>(sb[2]) : nullptr;
+    P.src_geo = s->geo ? as<float>(sb[3]) : nullptr;
+    P.n_src = n;
+    P.n_tgt = t->n;
+    P.grid = grids[f2].dev_ptr;
+    P.tgt_xyz = t->xyz;
+    P.tgt_feat = t->feat;
+    P.tgt_sem = t->sem;
+    P.tgt_geo = t->geo;
+    P.st = as<ProblemState>(B[BA_STATE]) + e;
+    P.pairs = as<uint2>(B[BA_PAIRS]) + row_off[e] * cap;
+    P.counts = as<uint32_t>(B[BA_COUNTS]) + row_off[e];
+    P.ovf_stamp = as<uint32_t>(B[BA_OVF]) + row_off[e];
+    P.cap_alloc = cap;
+    P.block_begin = blk;
+    P.n_blocks = nb;
+    P.part_a = as<double>(B[BA_PART_A]) + (size_t)blk * 8;
+    P.part_ovf = as<double>(B[BA_PART_OVF]) + (size_t)e * kOvfBlocks * 10;
+    P.part_s = nullptr;
+    P.part_cnt = as<uint32_t>(B[BA_PART_CNT]) + (size_t)blk * 2;
+    blk += nb;
+    BaEdge& ed = he[e];
+    std::memset(&ed, 0, sizeof(ed));
+    ed.ell = (double)p.multiframe_ell_init;
+    ed.f1 = f1;
+    ed.f2 = f2;
+    ed.K = cap;
+    ed.n_rows = n;
+    ed.n_pair_blocks = (n + kBaRowsPerBlock - 1) / kBaRowsPerBlock;
+  }
+  if (n_edges > 0) {
+    CVO_HIP(ctx, hipMemcpyAsync(B[BA_PROBS].p, hp.data(), sizeof(AssocProblem) * n_edges, hipMemcpyHostToDevice, st));
+    CVO_HIP(ctx, hipMemcpyAsync(B[BA_EDGES].p, he.data(), sizeof(BaEdge) * n_edges, hipMemcpyHostToDevice, st));
+  }
+  BaDev* hb = new BaDev;
+  std::memset(hb, 0, sizeof(BaDev));
+  hb->n_frames = n_frames;
+  hb->n_edges = n_edges;
+  hb->nfree = nfree;
+  hb->n = 6 * nfree;
+  hb->max_iters = p.multiframe_max_iters;
+  hb->iters_per_solve = p.multiframe_iterations_per_solve;
+  hb->K_init = cap;
+  hb->min_nonzeros = p.multiframe_min_nonzeros;
+  hb->iter_per_ell = p.multiframe_iter_per_ell;
+  hb->ell_min_f = p.multiframe_ell_min;
+  hb->decay_f = p.multiframe_ell_decay_rate;
+  hb->rq_coef = kc.rq_coef;
+  hb->ell_min = (double)p.multiframe_ell_min;
+  hb->ell_max = (double)p.multiframe_ell_init * 1.2;
+  int o = 0;
+  for (int f = 0; f < n_frames; ++f) {
+    hb->free_idx[f] = (!hold_const || !hold_const[f]) ? o++ : -1;
+    if (hb->free_idx[f] >= 0) hb->frame_of_free[hb->free_idx[f]] = f;
+    hb->frame_norm[f] = frames[f]->max_norm;
+    for (int i = 0; i < 12; ++i) hb->X[12 * f + i] = poses[12 * f + i];
+  }
+  hb->done = p.multiframe_max_iters <= 0 ? 1 : 0;
+  hipError_t ce = hipMemcpyAsync(B[BA_DEV].p, hb, sizeof(BaDev), hipMemcpyHostToDevice, st);
+  hipError_t se = hipStreamSynchronize(st);  // hb is pageable host memory
+  delete hb;
+  if (ce != hipSuccess) return hip_check(ctx, ce, "BA state upload");
+  if (se != hipSuccess) return hip_check(ctx, se, "BA state upload");
+
+  BaLaunch L;
+  L.ba = as<BaDev>(B[BA_DEV]);
+  L.edges = as<BaEdge>(B[BA_EDGES]);
+  L.probs = as<AssocProblem>(B[BA_PROBS]);
+  L.n_edges = n_edges;
+  L.total_blocks = total_blocks;
+  L.max_pair_blocks = max_pair_blocks;
+  L.lm_rounds = std::max(0, p.multiframe_iterations_per_solve) + 1;
+  L.part = as<double>(B[BA_PAIR_PART]);
+  L.contrib = as<double>(B[BA_CONTRIB]);
+  L.H = as<double>(B[BA_H]);
+  L.Hn = as<double>(B[BA_HN]);
+  L.kc = kc;
+
+  CVO_TRY(pinned(ctx, 64));
+  volatile int32_t* flag = reinterpret_cast<volatile int32_t*>(ctx->pinned);
+  const char* ng = std::getenv("CVO_NO_GRAPH");
+  const bool use_graph = !(ng && ng[0] && ng[0] != '0') && !ctx->profiling;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  if (use_graph) {
+    CVO_HIP(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    ba_iteration(L, st);
+    hipError_t e1 = hipStreamEndCapture(st, &graph);
+    if (e1 != hipSuccess) return hip_check(ctx, e1, "hipStreamEndCapture");
+    hipError_t e2 = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (e2 != hipSuccess) {
+      (void)hipGraphDestroy(graph);
+      return hip_check(ctx, e2, "hipGraphInstantiate");
+    }
+  }
+  CVO_TRY(ensure_events(ctx, 24));
+  hipEvent_t ev[6];
+  for (int i = 0; i < 6; ++i) ev[i] = ctx->ev_pool[16 + i];
+  cvo_status status = CVO_OK;
+  const int max_outer = std::max(0, p.multiframe_max_iters) + 1;
+  for (int it = 0; it < max_outer; ++it) {
+    const auto ti0 = std::chrono::steady_clock::now();
+    hipError_t le = hipSuccess;
+    if (use_graph) {
+      le = hipGraphLaunch(exec, st);
+    } else if (ctx->profiling) {
+      (void)hipEventRecord(ev[0], st);
+      hipLaunchKernelGGL(ba_prepare_kernel, dim3(1), dim3(256), 0, st, L.ba, L.edges, L.probs);
+      (void)hipEventRecord(ev[1], st);
+      if (n_edges > 0) launch_assoc(L.probs, n_edges, total_blocks, kc, st);
+      (void)hipEventRecord(ev[2], st);
+      if (n_edges > 0) {
+        launch_overflow(L.probs, n_edges, kc, st);
+        launch_pass_totals(L.probs, n_edges, st);
+      }
+      hipLaunchKernelGGL(ba_edges_kernel, dim3(1), dim3(256), 0, st, L.ba, L.edges, L.probs);
+      (void)hipEventRecord(ev[3], st);
+      float ms_pair = 0.0f, ms_solve = 0.0f;
+      for (int k = 0; k < (n_edges > 0 ? L.lm_rounds : 0); ++k) {
+        (void)hipEventRecord(ev[4], st);
+        hipLaunchKernelGGL(ba_pair_kernel, dim3(L.max_pair_blocks, n_edges), dim3(kBaRowsPerBlock), 0, st, L.ba,
+                           L.edges, L.probs, L.part, L.max_pair_blocks);
+        (void)hipEventRecord(ev[5], st);
+        hipLaunchKernelGGL(ba_edge_reduce_kernel, dim3(n_edges), dim3(256), 0, st, L.ba, L.edges, L.part,
+                           L.max_pair_blocks, L.contrib);
+        hipLaunchKernelGGL(ba_solve_kernel, dim3(1), dim3(256), 0, st, L.ba, L.edges, L.contrib, L.H, L.Hn);
+        (void)hipEventRecord(ev[3], st);
+        (void)hipEventSynchronize(ev[3]);
+        float a = 0, b = 0;
+        (void)hipEventElapsedTime(&a, ev[4], ev[5]);
+        (void)hipEventElapsedTime(&b, ev[5], ev[3]);
+        ms_pair += a;
+        ms_solve += b;
+      }
+      hipLaunchKernelGGL(ba_finish_kernel, dim3(1), dim3(64), 0, st, L.ba, L.edges);
+      (void)hipEventSynchronize(ev[3]);
+      float m_assoc = 0, m_ovf = 0;
+      (void)hipEventElapsedTime(&m_assoc, ev[1], ev[2]);
+      (void)hipEventElapsedTime(&m_ovf, ev[2], ev[3]);
+      ctx->stats.ms[CVO_KSTAT_ASSOC] += m_assoc;
+      ctx->stats.launches[CVO_KSTAT_ASSOC] += 1;
+      ctx->stats.ms[CVO_KSTAT_BA_PAIR] += ms_pair;
+      ctx->stats.launches[CVO_KSTAT_BA_PAIR] += n_edges > 0 ? L.lm_rounds : 0;
+      ctx->stats.ms[CVO_KSTAT_BA_SOLVE] += ms_solve;
+      ctx->stats.launches[CVO_KSTAT_BA_SOLVE] += n_edges > 0 ? L.lm_rounds : 0;
+      le = hipGetLastError();
+    } else {
+      ba_iteration(L, st);
+      le = hipGetLastError();
+    }
+    if (le != hipSuccess) {
+      status = hip_check(ctx, le, "BA iteration launch");
+      break;
+    }
+    // BaDev::iter and BaDev::done are adjacent
+    hipError_t me = hipMemcpyAsync((void*)flag, &L.ba->iter, sizeof(int32_t) * 2, hipMemcpyDeviceToHost, st);
+    if (me == hipSuccess) me = hipStreamSynchronize(st);
+    if (me != hipSuccess) {
+      status = hip_check(ctx, me, "BA flag readback");
+      break;
+    }
+    const auto ti1 = std::chrono::steady_clock::now();
+    if (per_iter_seconds && it < max_iter_records) per_iter_seconds[it] = std::chrono::duration<double>(ti1 - ti0).count();
+    if (flag[1]) break;
+  }
+  if (use_graph) {
+    (void)hipGraphExecDestroy(exec);
+    (void)hipGraphDestroy(graph);
+  }
+  if (status != CVO_OK) return status;
+  BaDev* out = new BaDev;
+  hipError_t re = hipMemcpy(out, L.ba, sizeof(BaDev), hipMemcpyDeviceToHost);
+  if (re != hipSuccess) {
+    delete out;
+    return hip_check(ctx, re, "BA state readback");
+  }
+  for (int f = 0; f < n_frames; ++f)
+    for (int i = 0; i < 12; ++i) poses[12 * f + i] = out->X[12 * f + i];
+  if (iters) *iters = out->iter;
+  ctx->ba_last_accepted = out->accepted_total;
+  ctx->ba_last_evals = out->evals_total;
+  delete out;
+  if (seconds) *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+  return CVO_OK;
+}

@@ -179,6 +179,8 @@ struct cvo_ctx {
   std::vector<hipEvent_t> ev_pool;
   // BA scratch
   DevBuf b_ba[16];
+  std::vector<DevBuf> b_frames;  // per frame: sorted source (4) and grid (2)
+  int32_t ba_last_accepted = 0, ba_last_evals = 0;
   // model scratch
   DevBuf b_mlp[16];
 };
@@ -197,6 +199,7 @@ cvo_status validate_pair(cvo_ctx* ctx, const cvo_dcloud* s, const cvo_dcloud* t)
 float radius_for(const KConsts& kc, float ell, float max_norm, float tq);
 std::vector<float> level_sizes(const KConsts& kc, float ell_hi, float ell_lo, float max_norm, float tq);
 cvo_status ensure_events(cvo_ctx* ctx, size_t n);
+cvo_status pinned(cvo_ctx* ctx, size_t bytes);
 cvo_status prepare_source_into(cvo_ctx* ctx, const cvo_dcloud* s, DevBuf& b_pos, DevBuf& b_feat, DevBuf& b_sem,
                                DevBuf& b_geo);
 cvo_status build_grid_into(cvo_ctx* ctx, const cvo_dcloud* t, const std::vector<float>& hs, GridHost& gh,

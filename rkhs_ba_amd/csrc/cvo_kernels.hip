@@ -20,25 +20,13 @@
 #include <stdint.h>
 
 #include "cvo_internal.h"
+#include "cvo_device.h"
 
 namespace cvo {
 
 // ---------------------------------------------------------------------------
 // small helpers
 // ---------------------------------------------------------------------------
-
-__device__ __forceinline__ void affine3(const float* A, const float* b, float y0, float y1, float y2, float& o0,
-                                        float& o1, float& o2) {
-  o0 = fmaf(A[2], y2, fmaf(A[1], y1, fmaf(A[0], y0, b[0])));
-  o1 = fmaf(A[5], y2, fmaf(A[4], y1, fmaf(A[3], y0, b[1])));
-  o2 = fmaf(A[8], y2, fmaf(A[7], y1, fmaf(A[6], y0, b[2])));
-}
-
-// compute_range_ell (CvoGPU.cu:90-94); distance with the canonical FMA order.
-__device__ __forceinline__ float range_ell(float ell, float x0, float x1, float x2) {
-  float dist = sqrtf(fmaf(x2, x2, fmaf(x1, x1, x0 * x0)));
-  return (float)(((double)dist / 500.0 + 1.0) * (double)ell);
-}
 
 template <typename T>
 __device__ __forceinline__ T shfl16(T v, int src) {
@@ -672,78 +660,6 @@ __device__ double select_step_dev(const KConsts& kc, const double bcde[4]) {
   return temp;
 }
 
-// exp(s [w; v]) for a unit twist, FP64, exact small-angle series.
-__device__ void se3_exp(const double xi[6], double s, double R[9], double t[3]) {
-  const double ph[3] = {xi[0] * s, xi[1] * s, xi[2] * s};
-  const double u[3] = {xi[3] * s, xi[4] * s, xi[5] * s};
-  const double th2 = ph[0] * ph[0] + ph[1] * ph[1] + ph[2] * ph[2];
-  const double th = sqrt(th2);
-  double ca, cb, cc;
-  if (th < 1e-3) {
-    ca = 1.0 - th2 / 6.0 + th2 * th2 / 120.0;
-    cb = 0.5 - th2 / 24.0 + th2 * th2 / 720.0;
-    cc = 1.0 / 6.0 - th2 / 120.0 + th2 * th2 / 5040.0;
-  } else {
-    double sn, cs;
-    sincos(th, &sn, &cs);
-    ca = sn / th;
-    cb = (1.0 - cs) / th2;
-    cc = (th - sn) / (th2 * th);
-  }
-  const double W[9] = {0, -ph[2], ph[1], ph[2], 0, -ph[0], -ph[1], ph[0], 0};
-  double V[9];
-  for (int r = 0; r < 3; ++r)
-    for (int c = 0; c < 3; ++c) {
-      double w2 = W[r * 3 + 0] * W[0 * 3 + c] + W[r * 3 + 1] * W[1 * 3 + c] + W[r * 3 + 2] * W[2 * 3 + c];
-      double I = (r == c) ? 1.0 : 0.0;
-      R[r * 3 + c] = I + ca * W[r * 3 + c] + cb * w2;
-      V[r * 3 + c] = I + cb * W[r * 3 + c] + cc * w2;
-    }
-  for (int r = 0; r < 3; ++r) t[r] = V[r * 3 + 0] * u[0] + V[r * 3 + 1] * u[1] + V[r * 3 + 2] * u[2];
-}
-
-// ||log(T)|| of a rigid transform, from the rotation's unit quaternion
-// (the Sophus SE3d log norm of dist_se3_cpu, src/cvo/CvoGPU.cpp:28-33).
-__device__ double se3_log_norm(const double R[9], const double t[3]) {
-  double qw, qx, qy, qz;
-  const double tr = R[0] + R[4] + R[8];
-  if (tr > 0) {
-    double s = sqrt(tr + 1.0) * 2.0;
-    qw = 0.25 * s; qx = (R[7] - R[5]) / s; qy = (R[2] - R[6]) / s; qz = (R[3] - R[1]) / s;
-  } else if (R[0] > R[4] && R[0] > R[8]) {
-    double s = sqrt(1.0 + R[0] - R[4] - R[8]) * 2.0;
-    qw = (R[7] - R[5]) / s; qx = 0.25 * s; qy = (R[1] + R[3]) / s; qz = (R[2] + R[6]) / s;
-  } else if (R[4] > R[8]) {
-    double s = sqrt(1.0 + R[4] - R[0] - R[8]) * 2.0;
-    qw = (R[2] - R[6]) / s; qx = (R[1] + R[3]) / s; qy = 0.25 * s; qz = (R[5] + R[7]) / s;
-  } else {
-    double s = sqrt(1.0 + R[8] - R[0] - R[4]) * 2.0;
-    qw = (R[3] - R[1]) / s; qx = (R[2] + R[6]) / s; qy = (R[5] + R[7]) / s; qz = 0.25 * s;
-  }
-  const double qn = sqrt(qw * qw + qx * qx + qy * qy + qz * qz);
-  qw /= qn; qx /= qn; qy /= qn; qz /= qn;
-  const double n2 = qx * qx + qy * qy + qz * qz, n = sqrt(n2);
-  double f;
-  if (n2 < 1e-20) f = 2.0 / qw - 2.0 * n2 / (qw * qw * qw);
-  else if (fabs(qw) < 1e-10) f = (qw > 0 ? M_PI : -M_PI) / n;
-  else f = 2.0 * atan(n / qw) / n;
-  const double w[3] = {f * qx, f * qy, f * qz};
-  const double th = sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
-  const double W[9] = {0, -w[2], w[1], w[2], 0, -w[0], -w[1], w[0], 0};
-  const double k2 = th < 1e-6 ? 1.0 / 12.0 : (1.0 - th * cos(0.5 * th) / (2.0 * sin(0.5 * th))) / (th * th);
-  double s2 = w[0] * w[0] + w[1] * w[1] + w[2] * w[2];
-  for (int r = 0; r < 3; ++r) {
-    double acc = 0.0;
-    for (int c = 0; c < 3; ++c) {
-      double w2 = W[r * 3 + 0] * W[0 * 3 + c] + W[r * 3 + 1] * W[1 * 3 + c] + W[r * 3 + 2] * W[2 * 3 + c];
-      double vi = ((r == c) ? 1.0 : 0.0) - 0.5 * W[r * 3 + c] + k2 * w2;
-      acc += vi * t[c];
-    }
-    s2 += acc * acc;
-  }
-  return sqrt(s2);
-}
-
 // A_sparsity_indicator_ell_update (CvoGPU.cu:547-665): two FIFO windows with
 // float sums; ring buffers of modulus W+1.
 __device__ bool indicator_update(ProblemState* st, float ind, int W, float thr) {
@@ -796,13 +712,6 @@ __device__ void set_maps(ProblemState* st) {
     st->b[i] = (float)(-(st->R[0 * 3 + i] * st->T[0] + st->R[1 * 3 + i] * st->T[1] + st->R[2 * 3 + i] * st->T[2]));
     st->Tq[i] = (float)st->T[i];
   }
-}
-
-__device__ int choose_level(const GridDev* grid, float r_need) {
-  int L = 0;
-  for (int l = 0; l < grid->n_levels; ++l)
-    if (grid->lv[l].h >= r_need) L = l;
-  return L;
 }
 
 __device__ int choose_level_tab(const float* h, int n_levels, float r_need) {

@@ -58,6 +58,7 @@ def lib() -> C.CDLL:
         L.cvo_align.argtypes = [P, P, P, P, P, P, P, P, P]
         L.cvo_align_d.argtypes = [P, P, P, P, P, P, P, P, P]
         L.cvo_align_multistart.argtypes = [P, P, P, i32, P, f, i32, P, P, P, P, P, P]
+        L.cvo_align_multiframe.argtypes = [P, i32, P, P, P, i32, P, P, P, P, i32]
         _lib = L
     return _lib
 
@@ -160,6 +161,24 @@ class DeviceCloud:
             pass
 
 
+class CvoFrame:
+    """cvo::CvoFrame (CvoFrame.hpp:12-39): a sensor-frame cloud and its pose
+    (12 doubles, row-major 3x4, world <- sensor) updated in place by
+    CvoGPU.align_frames."""
+
+    def __init__(self, points, pose_vec=None):
+        self.points = points
+        pv = np.array(np.eye(4) if pose_vec is None else pose_vec, np.float64)  # own copy, like double[12]
+        if pv.ndim == 2:
+            pv = pv[:3, :4]
+        self.pose_vec = np.ascontiguousarray(pv.reshape(12)).copy()
+
+    def pose(self) -> np.ndarray:
+        T = np.eye(4)
+        T[:3, :4] = self.pose_vec.reshape(3, 4)
+        return T
+
+
 class CvoGPU:
     """cvo::CvoGPU (CvoGPU.hpp:43-274) on one MI355X."""
 
@@ -260,6 +279,57 @@ class CvoGPU:
             c.free()
         return {"T_best": T_best.reshape(4, 4), "best_index": best.value, "per_start_T": per_T.reshape(n, 4, 4),
                 "per_start_cos": per_cos, "per_start_iters": per_it, "seconds": secs.value}
+
+    def align_multiframe(self, frames, poses, hold_const=None, edges=(), max_records: int = 1024) -> dict:
+        """IRLS bundle adjustment of a window of frames (CvoGPU::align over
+        frames, CvoGPU.cu:1614-1671; CvoBatchIRLS::solve, IRLS.cpp:148-367).
+        frames: clouds (sensor frame) or DeviceClouds; poses: (n,3,4) or (n,4,4)
+        world <- sensor; edges: (E,2) frame index pairs.  Returns the updated
+        poses (n,4,4), the outer iteration count and the timings."""
+        own = []
+        dev = []
+        for fr in frames:
+            if isinstance(fr, DeviceCloud):
+                dev.append(fr)
+            else:
+                d = self.upload(fr)
+                own.append(d)
+                dev.append(d)
+        n = len(dev)
+        P = np.ascontiguousarray(np.asarray(poses, np.float64)[:, :3, :4].reshape(n, 12))
+        hc = None
+        if hold_const is not None:
+            hc = np.ascontiguousarray(np.asarray(hold_const, np.uint8).reshape(n))
+        E = np.ascontiguousarray(np.asarray(edges, np.int32).reshape(-1, 2))
+        handles = (C.c_void_p * max(n, 1))(*[d.handle.value for d in dev])
+        secs = C.c_double()
+        it = C.c_int32()
+        per = np.zeros(max(1, max_records), np.float64)
+        try:
+            _check(lib().cvo_align_multiframe(self._ctx, n, handles, P.ctypes.data,
+                                              hc.ctypes.data if hc is not None else None, E.shape[0],
+                                              E.ctypes.data if E.size else None, C.byref(secs), C.byref(it),
+                                              per.ctypes.data, max_records), self._ctx)
+        finally:
+            for c in own:
+                c.free()
+        out = np.tile(np.eye(4), (n, 1, 1))
+        out[:, :3, :4] = P.reshape(n, 3, 4)
+        return {"poses": out, "iters": it.value, "seconds": secs.value,
+                "per_iter_seconds": per[:min(it.value + 1, max_records)]}
+
+    def align_frames(self, frames, hold_const, edges) -> float:
+        """CvoGPU::align(std::vector<CvoFrame::Ptr>&, hold_const, edges, seconds)
+        (CvoGPU.hpp:156-164): frames carry `points` and a 12-double `pose_vec`
+        that is updated in place; edges are pairs of frames.  Returns the
+        seconds of the call."""
+        idx = {id(f): i for i, f in enumerate(frames)}
+        E = [(idx[id(a)], idx[id(b)]) for a, b in edges]
+        poses = np.stack([np.asarray(f.pose_vec, np.float64).reshape(3, 4) for f in frames])
+        r = self.align_multiframe([f.points for f in frames], poses, hold_const, E)
+        for f, T in zip(frames, r["poses"]):
+            np.copyto(f.pose_vec, T[:3, :4].reshape(12))
+        return r["seconds"]
 
     def inner_product_gpu(self, source, target, init_guess_transform, ell: float) -> float:
         """inner_product_gpu (CvoGPU.cu:1734-1748)."""

@@ -230,6 +230,104 @@ def icosahedral_table():
     return [v for _, v in mats]
 
 
+# ---------------------------------------------------------------------------
+def _rand_pose(rng, tscale=3.0):
+    w = rng.normal(size=3)
+    w *= rng.uniform(0.1, 2.5) / np.linalg.norm(w)
+    T = np.eye(4)
+    T[:3, :3] = scipy.linalg.expm(skew(w))
+    T[:3, 3] = rng.normal(size=3) * tscale
+    return T
+
+
+def _plus(T, d):
+    """T Exp([rho; phi]) by the matrix exponential of the twist."""
+    return T @ scipy.linalg.expm(twist_hat(np.r_[d[3:], d[:3]]))
+
+
+def _residual(T1, T2, p1, p2, a):
+    diff = (T1[:3, :3] @ p1 + T1[:3, 3]) - (T2[:3, :3] @ p2 + T2[:3, 3])
+    return a * float(diff @ diff)
+
+
+def ba_golden(rng):
+    """Bundle-adjustment pieces by finite differences and dense assembly:
+    the retraction T Exp([rho; phi]) (expm), its 12x6 Jacobian at 0, the pair
+    residual r = a |T1 p1 - T2 p2|^2 with the Jacobians the reference supplies
+    (d r / d vec(T) divided by 2a: IRLS_Cost_CPU.hpp:339-351), and the normal
+    equations H = sum j j^T, g = sum r j, F = sum r^2 / 2 of a tiny window."""
+    out = {"plus": [], "plus_jac": [], "pair": [], "normal": None}
+    for scale in (1e-9, 1e-4, 0.05, 0.7, 2.0):
+        T = _rand_pose(rng)
+        d = rng.normal(size=6)
+        d *= scale / np.linalg.norm(d)
+        out["plus"].append({"T": T[:3, :4].ravel().tolist(), "d": d.tolist(), "out": _plus(T, d)[:3, :4].ravel().tolist()})
+    for _ in range(3):
+        T = _rand_pose(rng)
+        h = 1e-6
+        J = np.zeros((12, 6))
+        for c in range(6):
+            e = np.zeros(6)
+            e[c] = h
+            J[:, c] = ((_plus(T, e) - _plus(T, -e))[:3, :4].ravel()) / (2 * h)
+        out["plus_jac"].append({"T": T[:3, :4].ravel().tolist(), "J": J.ravel().tolist()})
+    for _ in range(4):
+        T1, T2 = _rand_pose(rng), _rand_pose(rng)
+        p1, p2 = rng.normal(size=3) * 4, rng.normal(size=3) * 4
+        a = float(rng.uniform(0.01, 1.0))
+        r = _residual(T1, T2, p1, p2, a)
+        J1, J2 = np.zeros(12), np.zeros(12)
+        h = 1e-5
+        for k in range(12):
+            E = np.zeros((4, 4))
+            E[k // 4, k % 4] = h
+            J1[k] = (_residual(T1 + E, T2, p1, p2, a) - _residual(T1 - E, T2, p1, p2, a)) / (2 * h) / (2 * a)
+            J2[k] = (_residual(T1, T2 + E, p1, p2, a) - _residual(T1, T2 - E, p1, p2, a)) / (2 * h) / (2 * a)
+        out["pair"].append({"T1": T1[:3, :4].ravel().tolist(), "T2": T2[:3, :4].ravel().tolist(),
+                            "p1": p1.tolist(), "p2": p2.tolist(), "a": a, "r": r,
+                            "J1": J1.tolist(), "J2": J2.tolist()})
+    # normal equations of a 3-frame window, 12 pairs on frame pairs (0,1),(1,2),(0,2)
+    poses = [_rand_pose(rng, 1.0) for _ in range(3)]
+    fr = [(0, 1), (1, 2), (0, 2)]
+    pairs = []
+    for k in range(12):
+        f1, f2 = fr[k % 3]
+        p1 = rng.normal(size=3).astype(np.float32)
+        p2 = rng.normal(size=3).astype(np.float32)
+        a = np.float32(rng.uniform(0.05, 1.0))
+        pairs.append((f1, f2, p1, p2, a))
+    n = 18
+    H = np.zeros((n, n))
+    g = np.zeros(n)
+    F = 0.0
+    h = 1e-6
+    for f1, f2, p1, p2, a in pairs:
+        p1d, p2d, ad = p1.astype(np.float64), p2.astype(np.float64), float(a)
+        r = _residual(poses[f1], poses[f2], p1d, p2d, ad)
+        js = {}
+        for f in (f1, f2):
+            j = np.zeros(6)
+            for c in range(6):
+                e = np.zeros(6)
+                e[c] = h
+                Pp = [P.copy() for P in poses]
+                Pm = [P.copy() for P in poses]
+                Pp[f] = _plus(poses[f], e)
+                Pm[f] = _plus(poses[f], -e)
+                j[c] = (_residual(Pp[f1], Pp[f2], p1d, p2d, ad) - _residual(Pm[f1], Pm[f2], p1d, p2d, ad)) / (2 * h) / (2 * ad)
+            js[f] = j
+        F += 0.5 * r * r
+        for u in (f1, f2):
+            g[6 * u:6 * u + 6] += r * js[u]
+            for w in (f1, f2):
+                H[6 * u:6 * u + 6, 6 * w:6 * w + 6] += np.outer(js[u], js[w])
+    out["normal"] = {"poses": [P[:3, :4].ravel().tolist() for P in poses],
+                     "pair_frames": [[f1, f2] for f1, f2, _, _, _ in pairs],
+                     "p1": [p1.tolist() for _, _, p1, _, _ in pairs], "p2": [p2.tolist() for _, _, _, p2, _ in pairs],
+                     "a": [float(a) for *_, a in pairs], "H": H.ravel().tolist(), "g": g.tolist(), "F": F}
+    return out
+
+
 def main():
     rng = np.random.default_rng(20261202)
     json.dump(params_golden(), open(os.path.join(HERE, "params_yaml.json"), "w"), indent=1, sort_keys=True)
@@ -237,6 +335,7 @@ def main():
     json.dump(se3_golden(rng), open(os.path.join(HERE, "se3_exp_log.json"), "w"), indent=1)
     json.dump(assoc_golden(rng), open(os.path.join(HERE, "assoc_tiny.json"), "w"))
     json.dump(flow_step_golden(rng), open(os.path.join(HERE, "flow_step.json"), "w"), indent=1)
+    json.dump(ba_golden(np.random.default_rng(20261203)), open(os.path.join(HERE, "ba_pieces.json"), "w"))
     ico = icosahedral_table()
     if ico is not None:
         json.dump(ico, open(os.path.join(HERE, "icosahedral_rotations.json"), "w"))
