@@ -1,13 +1,22 @@
 """Benchmark of the CVO registration hot path on one MI355X.
 
-Default workload: plain two-view CVO on the 60k-point street scene (5 color
-features, 19-class semantics, 5 deg / 1 m motion) -- registrations/s with both
-clouds resident in HBM before the timed region (source sort, grid build, the
-whole iteration loop and the pose readback are inside it).
+Headline workload (the JSON line's `value`): plain two-view CVO on the
+60k-point street scene (5 color features, 19-class semantics, 5 deg / 1 m
+motion), registrations/s with both clouds resident in HBM before the timed
+region (source sort, grid build, the whole iteration loop and the pose
+readback are inside it).
 
-Prints one JSON line (rank 0) with the metric, the roofline of the dominant
-kernel (association) measured with HIP events, and the CPU oracle timed on the
-host cores on a bounded sample of the same workload.
+Further legs, reported as nested objects of the same line:
+  ba_window   IRLS multi-frame bundle adjustment, 12 frames of 5k-40k points,
+              every frame pair whose exact cos angle exceeds 0.05 is an edge;
+              s/iter = median over outer iterations 2..end
+  multistart  global registration of two 20k-point clouds (uniform random
+              rotation, 2 m) from 96 starts on a super-Fibonacci SO(3) grid in
+              one batched launch sequence; starts/s and success
+
+Each leg carries the roofline of its dominant kernel (HIP events on the
+library's stream) and the CPU oracle timed on the host cores on a bounded
+sample of the same workload.
 """
 from __future__ import annotations
 
@@ -24,6 +33,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "registrations/s @60k pts + BA-window s/iter; batched/learned vs plain, 1 GPU"
 HBM_PEAK_GBS = 8000.0
+CONFIGS = os.path.join(ROOT, "rkhs_ba_amd", "configs")
 
 
 def log(msg):
@@ -36,11 +46,19 @@ def pose_error(T, T_gt):
     return ang, float(np.linalg.norm(E[:3, 3]))
 
 
-def cpu_baseline(params, src, tgt, T_gt, trace, threads, budget_s=12.0):
+def cpu_threads():
+    return min(16, os.cpu_count() or 1)
+
+
+# ---------------------------------------------------------------------------
+# two-view (headline)
+# ---------------------------------------------------------------------------
+
+def cpu_twoview(params, src, tgt, T_gt, trace, threads, budget_s=12.0):
     """Times the oracle's iteration (FP32-mirror algorithm, OpenMP over rows,
-    grid radius search) at each distinct lengthscale the GPU run visited, two
-    iterations per lengthscale, and weights by the GPU's iteration count at that
-    lengthscale: an estimate of one full registration on the host cores."""
+    grid radius search) at each distinct lengthscale the GPU run visited and
+    weights by the GPU's iteration count at that lengthscale: an estimate of
+    one full registration on the host cores."""
     from oracle import pyoracle
     ells = {}
     for r in trace:
@@ -53,7 +71,6 @@ def cpu_baseline(params, src, tgt, T_gt, trace, threads, budget_s=12.0):
     t_start = time.time()
     per_iter = {}
     order = sorted(ells.items(), reverse=True)
-    # size the sample to about budget_s seconds from one probe iteration
     probe = pyoracle.time_iterations(params, src, tgt, T_gt, order[0][0], max(order[0][1][1], 1), 1, threads)
     n_it = int(min(20, max(2, budget_s / max(1e-3, probe * len(order)))))
     for ell, (count, K) in order:
@@ -64,48 +81,30 @@ def cpu_baseline(params, src, tgt, T_gt, trace, threads, budget_s=12.0):
     return total_est, sampled, per_iter, time.time() - t_start
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--points", type=int, default=60000)
-    ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--profile-only", action="store_true", help="timed loop only (for rocprofv3)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU; multi-GPU sharding is design-only",
-                          "n_gpus": args.gpus}))
-        sys.exit(2)
-
+def leg_twoview(args):
     from rkhs_ba_amd import CvoGPU, read_params_yaml, synth
-    params = read_params_yaml(os.path.join(ROOT, "rkhs_ba_amd", "configs", "street_60k.yaml"))
-    log(f"[bench] generating street scene n={args.points} seed={args.seed}")
+    params = read_params_yaml(os.path.join(CONFIGS, "street_60k.yaml"))
+    log(f"[bench] two-view: street scene n={args.points} seed={args.seed}")
     src, tgt, T_gt = synth.street_pair(args.seed, n=args.points)
     cvo = CvoGPU(params=params)
     dsrc, dtgt = cvo.upload(src), cvo.upload(tgt)
     I = np.eye(4, dtype=np.float32)
-
     for w in range(args.warmup):
-        log(f"[bench] warmup {w}")
+        log(f"[bench] two-view warmup {w}")
         cvo.align(dsrc, dtgt, I)
-    log(f"[bench] timed: {args.steps} registrations")
+    log(f"[bench] two-view timed: {args.steps} registrations")
     t0 = time.perf_counter()
-    results = []
-    for s in range(args.steps):
-        results.append(cvo.align(dsrc, dtgt, I))
-    t1 = time.perf_counter()
-    wall = t1 - t0
+    results = [cvo.align(dsrc, dtgt, I) for _ in range(args.steps)]
+    wall = time.perf_counter() - t0
     ret, T, dev_s, iters, _ = results[-1]
     rot_e, tr_e = pose_error(T, T_gt)
-    log(f"[bench] {wall / args.steps * 1e3:.1f} ms/registration, {iters} iters, err {rot_e:.4f} deg {tr_e * 100:.2f} cm")
+    log(f"[bench] two-view {wall / args.steps * 1e3:.1f} ms/registration, {iters} iters, "
+        f"err {rot_e:.4f} deg {tr_e * 100:.2f} cm")
+    out = {"value": args.steps / wall, "ms_per_step": wall / args.steps * 1e3, "iterations": iters,
+           "device_loop_ms": dev_s * 1e3, "pose_error": {"deg": rot_e, "m": tr_e}, "params": params}
     if args.profile_only:
-        return
-
-    # Kernel-level roofline: one more registration with events around every
-    # launch (same workload, same stream).
-    log("[bench] profiled registration")
+        return out
+    log("[bench] two-view profiled registration")
     cvo.reset_kernel_stats()
     cvo.set_profiling(True)
     cvo.set_trace(params.MAX_ITER)
@@ -117,49 +116,210 @@ def main():
     assoc_ms = st["ms"]["assoc"] / n_assoc
     assoc_bytes = st["bytes"]["assoc"] / n_assoc
     achieved = assoc_bytes / (assoc_ms * 1e-3) / 1e9
-    step_ms = st["ms"]["step"] / max(1, st["launches"]["step"])
-    ovf_ms = st["ms"]["overflow"] / max(1, st["launches"]["overflow"])
-
-    cpu = None
+    out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                       "frac": achieved / HBM_PEAK_GBS, "traffic": args.traffic,
+                       "kernel": "assoc_kernel", "avg_launch_ms": assoc_ms,
+                       "algorithmic_bytes_per_launch": assoc_bytes, "launches": n_assoc}
+    out["kernels_ms_per_launch"] = {
+        "assoc": assoc_ms,
+        "overflow": st["ms"]["overflow"] / max(1, st["launches"]["overflow"]),
+        "step": st["ms"]["step"] / max(1, st["launches"]["step"])}
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
-        log(f"[bench] cpu baseline on {threads} threads")
-        est_s, sampled, per_iter, spent = cpu_baseline(params, src, tgt, T_gt, trace, threads)
-        cpu = {"value": 1.0 / est_s if est_s > 0 else None, "unit": "registrations/s", "cores": threads,
-               "kind": "port",
-               "sample": (f"oracle FP32-mirror iteration (grid radius search, OpenMP) timed {sampled} times over the "
-                          f"{len(per_iter)} lengthscales of the GPU run, weighted by the GPU's iterations per "
-                          f"lengthscale ({len(trace)} iterations); {spent:.1f} s of CPU work"),
-               "seconds_per_registration": est_s}
+        threads = cpu_threads()
+        log(f"[bench] two-view cpu baseline on {threads} threads")
+        est_s, sampled, per_iter, spent = cpu_twoview(params, src, tgt, T_gt, trace, threads)
+        out["cpu_baseline"] = {
+            "value": 1.0 / est_s if est_s > 0 else None, "unit": "registrations/s", "cores": threads, "kind": "port",
+            "sample": (f"oracle FP32-mirror iteration (grid radius search, OpenMP) timed {sampled} times over the "
+                       f"{len(per_iter)} lengthscales of the GPU run, weighted by the GPU's iterations per "
+                       f"lengthscale ({len(trace)} iterations); {spent:.1f} s of CPU work"),
+            "seconds_per_registration": est_s}
+    cvo.close()
+    return out
 
-    ms = wall / args.steps * 1e3
+
+# ---------------------------------------------------------------------------
+# IRLS multi-frame BA window
+# ---------------------------------------------------------------------------
+
+def window_edges(cvo, clouds, poses, ell, thr=0.05):
+    """Frame pairs whose exact cos angle (function_angle, approx=False) at the
+    initial poses exceeds thr."""
+    edges = []
+    for i in range(len(clouds)):
+        for j in range(i + 1, len(clouds)):
+            T = (np.linalg.inv(poses[j]) @ poses[i]).astype(np.float32)
+            if cvo.function_angle(clouds[i], clouds[j], T, ell, is_approximate=False) > thr:
+                edges.append((i, j))
+    return edges
+
+
+def rel_pose_errors(P, gts):
+    out = []
+    for f in range(1, len(P)):
+        E = np.linalg.inv(np.linalg.inv(gts[0]) @ gts[f]) @ (np.linalg.inv(P[0]) @ P[f])
+        out.append((float(np.degrees(np.arccos(np.clip((np.trace(E[:3, :3]) - 1) / 2, -1, 1)))),
+                    float(np.linalg.norm(E[:3, 3]))))
+    return np.array(out)
+
+
+def leg_ba(args):
+    from rkhs_ba_amd import CvoGPU, read_params_yaml, synth
+    params = read_params_yaml(os.path.join(CONFIGS, "ba_window.yaml"))
+    log("[bench] BA window: 12 frames, 5k-40k points")
+    clouds, gts, inits = synth.window_frames(args.seed, n_frames=12, n_min=5000, n_max=40000)
+    cvo = CvoGPU(params=params)
+    edges = window_edges(cvo, clouds, inits, params.multiframe_ell_init)
+    hold = [1] + [0] * 11
+    dev = [cvo.upload(c) for c in clouds]
+    log(f"[bench] BA window: {len(edges)} edges, {sum(c.n for c in clouds)} points; warmup")
+    cvo.align_multiframe(dev, inits, hold, edges)
+    log("[bench] BA window timed run")
+    t0 = time.perf_counter()
+    r = cvo.align_multiframe(dev, inits, hold, edges)
+    wall = time.perf_counter() - t0
+    per = r["per_iter_seconds"][:r["iters"]]
+    s_iter = float(np.median(per[1:])) if len(per) > 1 else float(per[0])
+    e0, e1 = rel_pose_errors(inits, gts), rel_pose_errors(r["poses"], gts)
+    log(f"[bench] BA window {r['iters']} iterations, {s_iter * 1e3:.2f} ms/iter (median 2..end), "
+        f"first {per[0] * 1e3:.2f} ms; max error {e0[:, 0].max():.3f}->{e1[:, 0].max():.3f} deg")
+    out = {"s_per_iter": s_iter, "first_iter_s": float(per[0]), "iterations": r["iters"], "wall_s": wall,
+           "frames": len(clouds), "edges": len(edges), "points": int(sum(c.n for c in clouds)),
+           "max_error_before": {"deg": float(e0[:, 0].max()), "m": float(e0[:, 1].max())},
+           "max_error_after": {"deg": float(e1[:, 0].max()), "m": float(e1[:, 1].max())}}
+    if args.profile_only:
+        return out
+    log("[bench] BA window profiled run")
+    cvo.reset_kernel_stats()
+    cvo.set_profiling(True)
+    cvo.align_multiframe(dev, inits, hold, edges)
+    cvo.set_profiling(False)
+    st = cvo.kernel_stats()
+    n_pair = max(1, st["launches"]["ba_pair"])
+    pair_ms = st["ms"]["ba_pair"] / n_pair
+    pair_bytes = st["bytes"]["ba_pair"] / n_pair
+    assoc_ms = st["ms"]["assoc"] / max(1, st["launches"]["assoc"])
+    assoc_bytes = st["bytes"]["assoc"] / max(1, st["launches"]["assoc"])
+    out["roofline"] = {"bound": "hbm", "achieved": pair_bytes / (pair_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                       "unit": "GB/s", "frac": pair_bytes / (pair_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                       "kernel": "ba_pair_kernel", "avg_launch_ms": pair_ms,
+                       "algorithmic_bytes_per_launch": pair_bytes, "launches": n_pair, "traffic": None}
+    out["assoc_roofline"] = {"achieved": assoc_bytes / (assoc_ms * 1e-3) / 1e9, "avg_launch_ms": assoc_ms,
+                             "algorithmic_bytes_per_launch": assoc_bytes}
+    out["kernels_ms_per_iteration"] = {
+        k: st["ms"][k] / max(1, st["launches"]["assoc"]) for k in ("assoc", "overflow", "ba_pair", "ba_solve")}
+    if not args.no_cpu:
+        from oracle import pyoracle
+        threads = cpu_threads()
+        p1 = read_params_yaml(os.path.join(CONFIGS, "ba_window.yaml"))
+        p1.multiframe_max_iters = 1
+        log(f"[bench] BA window cpu baseline: one outer iteration on {threads} threads")
+        t0 = time.perf_counter()
+        _, it, secs = pyoracle.align_multiframe(p1, clouds, inits, hold, edges, threads=threads)
+        spent = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": float(secs[0]), "unit": "s/iter", "cores": threads, "kind": "port",
+                               "sample": f"oracle IRLS outer iteration 1 (association of all {len(edges)} edges "
+                                         f"+ up to {p1.multiframe_iterations_per_solve} LM candidates), "
+                                         f"{spent:.1f} s of CPU work",
+                               "gpu_first_iter_s": float(per[0]),
+                               "speedup_first_iter": float(secs[0]) / float(per[0])}
+    cvo.close()
+    return out
+
+
+# ---------------------------------------------------------------------------
+# batched multi-start global registration
+# ---------------------------------------------------------------------------
+
+def leg_multistart(args):
+    from rkhs_ba_amd import CvoGPU, read_params_yaml, synth
+    params = read_params_yaml(os.path.join(CONFIGS, "multistart_20k.yaml"))
+    params.MAX_ITER = 1000
+    log("[bench] multi-start: 20k/20k, uniform rotation + 2 m, 96 starts")
+    src, tgt, T_gt = synth.planes_spheres_pair(100 + args.seed, 20000, 20000, uniform_rot=True, trans=2.0)
+    cx = src.xyz.mean(0).astype(np.float64)
+    cy = tgt.xyz.mean(0).astype(np.float64)
+    Ts = np.array([synth.rigid(R, cy - R @ cx) for R in synth.so3_grid(96)], np.float32)
+    cvo = CvoGPU(params=params)
+    ds, dt = cvo.upload(src), cvo.upload(tgt)
+    cvo.align_multistart(ds, dt, Ts, 0.3, mode=0)
+    t0 = time.perf_counter()
+    r = cvo.align_multistart(ds, dt, Ts, 0.3, mode=0)
+    wall = time.perf_counter() - t0
+    e = pose_error(r["T_best"], T_gt)
+    per_ok = sum(1 for T in r["per_start_T"] if pose_error(T, T_gt)[0] < np.degrees(0.01) and
+                 pose_error(T, T_gt)[1] < 0.01)
+    success = bool(e[0] < np.degrees(0.01) and e[1] < 0.01)
+    log(f"[bench] multi-start {wall:.2f} s, best err {e[0]:.4f} deg {e[1] * 100:.2f} cm, "
+        f"{per_ok}/96 starts converge")
+    out = {"seconds": wall, "starts_per_s": 96 / wall, "success": success,
+           "best_error": {"deg": e[0], "m": e[1]}, "starts_converged": per_ok,
+           "iterations_total": int(r["per_start_iters"].sum())}
+    if not args.no_cpu and not args.profile_only:
+        from oracle import pyoracle
+        threads = cpu_threads()
+        t1 = pyoracle.time_iterations(params, src, tgt, np.linalg.inv(T_gt).astype(np.float32), params.ell_init,
+                                      params.nearest_neighbors_max, 3, threads)
+        est = t1 * out["iterations_total"]
+        out["cpu_baseline"] = {"value": 96 / est, "unit": "starts/s", "cores": threads, "kind": "port",
+                               "sample": "oracle iteration at ell_init timed 3 times x the GPU run's total "
+                                         "iterations over all 96 starts",
+                               "seconds_estimate": est}
+    cvo.close()
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--points", type=int, default=60000)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--legs", default="twoview,ba,multistart")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
+    ap.add_argument("--profile-only", action="store_true", help="timed loops only (for rocprofv3)")
+    ap.add_argument("--traffic", type=float, default=None,
+                    help="HBM bytes per association launch from rocprofv3 counters, if measured")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; multi-GPU sharding is design-only",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+    legs = set(args.legs.split(","))
+    tv = leg_twoview(args)
+    params = tv.pop("params")
+    extra = {}
+    if "ba" in legs:
+        extra["ba_window"] = leg_ba(args)
+    if "multistart" in legs:
+        extra["multistart"] = leg_multistart(args)
+    cpu = tv.get("cpu_baseline")
     out = {
         "metric": METRIC,
-        "value": args.steps / wall,
+        "value": tv["value"],
         "unit": "registrations/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms,
+        "ms_per_step": tv["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32 (f64 accumulation and pose)",
         "data": "synthetic (seeded street scene, clouds resident in HBM)",
         "config": {"workload": "two-view CVO, 60k/60k street scene, 5 features + 19 classes, 5deg/1m",
-                   "points": args.points, "iterations": iters, "max_iter": params.MAX_ITER,
+                   "points": args.points, "iterations": tv["iterations"], "max_iter": params.MAX_ITER,
                    "parallelism": "single GPU"},
-        "iterations_per_registration": iters,
-        "device_loop_ms": dev_s * 1e3,
-        "pose_error": {"deg": rot_e, "m": tr_e},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "assoc_kernel", "avg_launch_ms": assoc_ms, "algorithmic_bytes_per_launch": assoc_bytes,
-                     "launches": n_assoc},
-        "kernels_ms_per_launch": {"assoc": assoc_ms, "overflow": ovf_ms, "step": step_ms},
+        "iterations_per_registration": tv["iterations"],
+        "device_loop_ms": tv["device_loop_ms"],
+        "pose_error": tv["pose_error"],
+        "roofline": tv.get("roofline"),
+        "kernels_ms_per_launch": tv.get("kernels_ms_per_launch"),
         "cpu_baseline": cpu,
-        "speedup_vs_cpu": (args.steps / wall) / cpu["value"] if cpu and cpu["value"] else None,
+        "speedup_vs_cpu": tv["value"] / cpu["value"] if cpu and cpu["value"] else None,
     }
+    out.update(extra)
     print(json.dumps(out))
 
 

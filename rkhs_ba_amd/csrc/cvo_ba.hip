@@ -52,6 +52,7 @@ constexpr int kBaMaxEdges = 4096;
 constexpr int kBaTerms = 28;                   // S (21, upper triangle), h (6), c (1)
 constexpr int kBaContrib = 121;                // B11, B12, B22 (36 each), g1, g2 (6 each), c
 constexpr int kBaRowsPerBlock = 256;           // pair pass: one row per thread
+constexpr int kSolveThreads = 64;              // one wave: the factorisation's barriers stay cheap
 
 struct BaEdge {
   double ell;
@@ -59,6 +60,7 @@ struct BaEdge {
   int32_t f1, f2;
   int32_t K, last_max, iters_per_ell, on;
   int32_t n_rows, n_pair_blocks;
+  int32_t pad0, pad1;
 };
 
 struct BaDev {
@@ -66,7 +68,7 @@ struct BaDev {
   int32_t iter, done, max_iters, iters_per_solve;
   int32_t lm_iter, lm_stop, eval_pending, eval_is_candidate;
   int32_t K_init, min_nonzeros, iter_per_ell, counter_nonzero;
-  int32_t stopped_no_edges, accepted_total, evals_total, pad0;
+  int32_t stopped_no_edges, accepted_total, evals_total, h_cur;
   float ell_min_f, decay_f, rq_coef, pad1;
   double ell_min, ell_max;
   double F, Fn, mu, v, change;
@@ -101,8 +103,10 @@ __device__ inline void pose_M(const double* T, double M[36]) {
     }
 }
 
-// T <- T Exp([rho; phi]), translation part first.
-__device__ inline void pose_plus(const double* T, const double* dlt, double* out) {
+// T <- T Exp([rho; phi]), translation part first.  out may alias Tin.
+__device__ inline void pose_plus(const double* Tin, const double* dlt, double* out) {
+  double T[12];
+  for (int i = 0; i < 12; ++i) T[i] = Tin[i];
   const double xi[6] = {dlt[3], dlt[4], dlt[5], dlt[0], dlt[1], dlt[2]};
   double dR[9], dT[3];
   se3_exp(xi, 1.0, dR, dT);
@@ -209,8 +213,11 @@ __global__ __launch_bounds__(256) void ba_edges_kernel(BaDev* __restrict__ ba, B
   }
 }
 
-// FP64 pair pass at the poses Xe: one thread per source row of an edge, block
-// partials of S (21), h (6), c (1).
+// FP64 pair pass at the poses Xe, block partials of S (21), h (6), c (1).
+// Each wave takes 64 source rows of one edge (lane i owns row i and its world
+// point), prefix-sums their pair counts and spreads the flattened pairs over
+// its lanes, so every lane's loads are independent and the lanes stay busy
+// whatever the row lengths.
 __global__ __launch_bounds__(kBaRowsPerBlock) void ba_pair_kernel(const BaDev* __restrict__ ba,
                                                                   const BaEdge* __restrict__ edges,
                                                                   const AssocProblem* __restrict__ probs,
@@ -220,49 +227,77 @@ __global__ __launch_bounds__(kBaRowsPerBlock) void ba_pair_kernel(const BaDev* _
   const BaEdge& ed = edges[e];
   if (!ed.on || (int)blockIdx.x >= ed.n_pair_blocks) return;
   const AssocProblem& P = probs[e];
-  const int tid = threadIdx.x;
-  const int r = blockIdx.x * kBaRowsPerBlock + tid;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wr0 = blockIdx.x * kBaRowsPerBlock + wave * 64;
+  // pairs are stored at the row stride of this iteration's cap (the
+  // reference re-strides its pair matrix the same way)
+  const size_t row_stride = (size_t)P.st->K;
   double T1[12], T2[12];
 #pragma unroll
   for (int i = 0; i < 12; ++i) {
     T1[i] = ba->Xe[12 * ed.f1 + i];
     T2[i] = ba->Xe[12 * ed.f2 + i];
   }
+  // the lane's own row: world point and pair count
+  const int r_own = wr0 + lane;
+  uint32_t cnt = 0;
+  double X0 = 0.0, X1 = 0.0, X2 = 0.0;
+  if (r_own < P.n_src) {
+    cnt = P.counts[r_own];
+    const float4 x4 = P.src_pos[r_own];
+    const double p0 = x4.x, p1 = x4.y, p2 = x4.z;
+    X0 = fma(T1[2], p2, fma(T1[1], p1, fma(T1[0], p0, T1[3])));
+    X1 = fma(T1[6], p2, fma(T1[5], p1, fma(T1[4], p0, T1[7])));
+    X2 = fma(T1[10], p2, fma(T1[9], p1, fma(T1[8], p0, T1[11])));
+  }
+  uint32_t incl = cnt;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t t = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += t;
+  }
+  const uint32_t total = __shfl(incl, 63, 64);
+  const uint32_t excl = incl - cnt;
   double acc[kBaTerms];
 #pragma unroll
   for (int k = 0; k < kBaTerms; ++k) acc[k] = 0.0;
-  if (r < P.n_src) {
-    const GridLevel& G = P.grid->lv[P.st->level_assoc];
-    const float4 x4 = P.src_pos[r];
-    const double p0 = x4.x, p1 = x4.y, p2 = x4.z;
-    const double X0 = T1[0] * p0 + T1[1] * p1 + T1[2] * p2 + T1[3];
-    const double X1 = T1[4] * p0 + T1[5] * p1 + T1[6] * p2 + T1[7];
-    const double X2 = T1[8] * p0 + T1[9] * p1 + T1[10] * p2 + T1[11];
-    const uint32_t cnt = P.counts[r];
-    const uint2* row = P.pairs + (size_t)r * P.cap_alloc;
-    for (uint32_t s = 0; s < cnt; ++s) {
-      const uint2 pe = row[s];
-      const double a = (double)__uint_as_float(pe.y);
-      const float4 y4 = G.pos[pe.x];
+  for (uint32_t base = 0; base < total; base += 64) {
+    const uint32_t idx = base + (uint32_t)lane;
+    // row of flattened pair idx: the last lane whose exclusive offset <= idx
+    int k = 0;
+#pragma unroll
+    for (int s = 32; s > 0; s >>= 1) {
+      const uint32_t ex = __shfl(excl, k + s, 64);
+      if (ex <= idx) k += s;
+    }
+    const uint32_t slot = idx - __shfl(excl, k, 64);
+    const double Y0 = __shfl(X0, k, 64), Y1 = __shfl(X1, k, 64), Y2 = __shfl(X2, k, 64);
+    if (idx < total) {
+      // FP64 with explicit fused multiply-adds (the library is built with
+      // contraction off for the FP32 gates)
+      const float4 y4 = P.pairs4[(size_t)(wr0 + k) * row_stride + slot];
+      const double a = (double)y4.w;
       const double q0 = y4.x, q1 = y4.y, q2 = y4.z;
-      const double w0 = X0 - (T2[0] * q0 + T2[1] * q1 + T2[2] * q2 + T2[3]);
-      const double w1 = X1 - (T2[4] * q0 + T2[5] * q1 + T2[6] * q2 + T2[7]);
-      const double w2 = X2 - (T2[8] * q0 + T2[9] * q1 + T2[10] * q2 + T2[11]);
-      const double res = a * (w0 * w0 + w1 * w1 + w2 * w2);
-      const double u[6] = {w0, w1, w2, X1 * w2 - X2 * w1, X2 * w0 - X0 * w2, X0 * w1 - X1 * w0};
-      int k = 0;
+      const double w0 = Y0 - fma(T2[2], q2, fma(T2[1], q1, fma(T2[0], q0, T2[3])));
+      const double w1 = Y1 - fma(T2[6], q2, fma(T2[5], q1, fma(T2[4], q0, T2[7])));
+      const double w2 = Y2 - fma(T2[10], q2, fma(T2[9], q1, fma(T2[8], q0, T2[11])));
+      const double res = a * fma(w2, w2, fma(w1, w1, w0 * w0));
+      const double u[6] = {w0, w1, w2, fma(Y1, w2, -Y2 * w1), fma(Y2, w0, -Y0 * w2), fma(Y0, w1, -Y1 * w0)};
+      int q = 0;
 #pragma unroll
       for (int i = 0; i < 6; ++i)
 #pragma unroll
-        for (int j = i; j < 6; ++j) acc[k++] += u[i] * u[j];
+        for (int j = i; j < 6; ++j) {
+          acc[q] = fma(u[i], u[j], acc[q]);
+          ++q;
+        }
 #pragma unroll
-      for (int i = 0; i < 6; ++i) acc[21 + i] += res * u[i];
-      acc[27] += res * res;
+      for (int i = 0; i < 6; ++i) acc[21 + i] = fma(res, u[i], acc[21 + i]);
+      acc[27] = fma(res, res, acc[27]);
     }
   }
   // fixed-order block partial: wave shuffles, then the four waves in LDS
   __shared__ double s_w[kBaTerms][kBaRowsPerBlock / 64];
-  const int lane = tid & 63, wave = tid >> 6;
 #pragma unroll
   for (int k = 0; k < kBaTerms; ++k) {
     double v = acc[k];
@@ -342,226 +377,244 @@ __global__ __launch_bounds__(256) void ba_edge_reduce_kernel(const BaDev* __rest
   }
 }
 
-// Assembles H, g and F at the poses Xe from the edge blocks, applies the
-// Levenberg-Marquardt decision for a candidate, and prepares the next
-// candidate (or stops the solve).  One workgroup; the damped system is
-// factored in LDS.
-__global__ __launch_bounds__(256) void ba_solve_kernel(BaDev* __restrict__ ba, const BaEdge* __restrict__ edges,
-                                                       const double* __restrict__ contrib, double* __restrict__ H,
-                                                       double* __restrict__ Hn) {
-  if (ba->done || !ba->eval_pending) return;
-  const int tid = threadIdx.x;
-  const int n = ba->n, E = ba->n_edges;
-  __shared__ double s_A[kBaMaxDof * kBaMaxDof];
-  __shared__ double s_b[kBaMaxDof];
-  __shared__ double s_red[256];
-  __shared__ int s_e1[256], s_e2[256];
-  __shared__ int s_flag, s_fail;
-  __shared__ double s_scalar[4];
+// Sum and maximum over the 64 lanes of a wave (butterfly; every lane gets the
+// result, in a fixed order).
+__device__ inline double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ inline double wave_max(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
 
-  // (1) assemble Hn (full), gn and Fn in a fixed edge order
-  for (int e0 = 0; e0 < E; e0 += 256) {
-    const int ne = min(256, E - e0);
+// H, g and F at the poses Xe from the edge blocks: one workgroup per 6x6 block
+// (fi, fj) of H, then one per 6-block of g (the first also sums F); every sum
+// runs over the edges in index order.  H goes to the buffer that is not the
+// current linearisation.
+__global__ __launch_bounds__(64) void ba_assemble_kernel(BaDev* __restrict__ ba, const BaEdge* __restrict__ edges,
+                                                         const double* __restrict__ contrib,
+                                                         double* __restrict__ Hbuf) {
+  if (ba->done || !ba->eval_pending) return;
+  const int nf = ba->nfree, n = ba->n, E = ba->n_edges;
+  const int b = blockIdx.x, t = threadIdx.x;
+  double* Hn = Hbuf + (size_t)(1 - ba->h_cur) * kBaMaxDof * kBaMaxDof;
+  __shared__ int s_o1[1024], s_o2[1024];
+  const bool hblock = b < nf * nf;
+  const int fi = hblock ? b / nf : b - nf * nf;
+  const int fj = hblock ? b - (b / nf) * nf : 0;
+  const int a = t / 6, c = t % 6;
+  double s = 0.0;
+  for (int e0 = 0; e0 < E; e0 += 1024) {
+    const int ne = min(1024, E - e0);
     __syncthreads();
-    if (tid < ne) {
-      const BaEdge& ed = edges[e0 + tid];
-      s_e1[tid] = ed.on ? ba->free_idx[ed.f1] : -2;
-      s_e2[tid] = ed.on ? ba->free_idx[ed.f2] : -2;
+    for (int q = t; q < ne; q += 64) {
+      const BaEdge& ed = edges[e0 + q];
+      s_o1[q] = ed.on ? ba->free_idx[ed.f1] : -2;
+      s_o2[q] = ed.on ? ba->free_idx[ed.f2] : -2;
     }
     __syncthreads();
-    for (int idx = tid; idx < n * n; idx += 256) {
-      const int i = idx / n, j = idx - (idx / n) * n;
-      const int fi = i / 6, fj = j / 6, a = i % 6, b = j % 6;
-      double s = (e0 == 0) ? 0.0 : Hn[idx];
-      for (int q = 0; q < ne; ++q) {
-        const int o1 = s_e1[q], o2 = s_e2[q];
-        if (o1 == -2) continue;
-        const double* C = contrib + (size_t)(e0 + q) * kBaContrib;
-        if (o1 == fi && o1 == fj) s += C[a * 6 + b];
-        if (o2 == fi && o2 == fj) s += C[72 + a * 6 + b];
-        if (o1 == fi && o2 == fj) s += C[36 + a * 6 + b];
-        if (o2 == fi && o1 == fj) s += C[36 + b * 6 + a];
+    for (int q = 0; q < ne; ++q) {
+      const int o1 = s_o1[q], o2 = s_o2[q];
+      if (o1 == -2) continue;
+      const double* C = contrib + (size_t)(e0 + q) * kBaContrib;
+      if (hblock) {
+        if (t < 36) {
+          if (o1 == fi && o1 == fj) s += C[a * 6 + c];
+          if (o2 == fi && o2 == fj) s += C[72 + a * 6 + c];
+          if (o1 == fi && o2 == fj) s += C[36 + a * 6 + c];
+          if (o2 == fi && o1 == fj) s += C[36 + c * 6 + a];
+        }
+      } else if (t < 6) {
+        if (o1 == fi) s += C[108 + t];
+        if (o2 == fi) s += C[114 + t];
+      } else if (t == 6 && fi == 0) {
+        s += 0.5 * C[120];
       }
-      Hn[idx] = s;
-    }
-    for (int i = tid; i < n; i += 256) {
-      const int fi = i / 6, a = i % 6;
-      double s = (e0 == 0) ? 0.0 : ba->gn[i];
-      for (int q = 0; q < ne; ++q) {
-        const int o1 = s_e1[q], o2 = s_e2[q];
-        if (o1 == -2) continue;
-        const double* C = contrib + (size_t)(e0 + q) * kBaContrib;
-        if (o1 == fi) s += C[108 + a];
-        if (o2 == fi) s += C[114 + a];
-      }
-      ba->gn[i] = s;
-    }
-    if (tid == 0) {
-      double s = (e0 == 0) ? 0.0 : ba->Fn;
-      for (int q = 0; q < ne; ++q)
-        if (s_e1[q] != -2) s += 0.5 * contrib[(size_t)(e0 + q) * kBaContrib + 120];
-      ba->Fn = s;
     }
   }
-  __syncthreads();
+  if (hblock) {
+    if (t < 36) Hn[(size_t)(6 * fi + a) * n + 6 * fj + c] = s;
+  } else if (t < 6) {
+    ba->gn[6 * fi + t] = s;
+  } else if (t == 6 && fi == 0) {
+    ba->Fn = s;
+  }
+}
 
-  // (2) decision on the evaluated poses
-  if (!ba->eval_is_candidate) {
-    for (int i = tid; i < n * n; i += 256) H[i] = Hn[i];
-    for (int i = tid; i < n; i += 256) ba->g[i] = ba->gn[i];
-    if (tid == 0) ba->F = ba->Fn;
-  } else {
-    // predicted decrease -(g.d + d.H.d / 2) with H, g at the current poses
+// One wave: the Levenberg-Marquardt decision on an evaluated candidate, then
+// the next candidate (or the end of the solve).  Scalars of the schedule are
+// kept uniform in registers and written back by lane 0; the damped system is
+// factored in LDS (right-looking Cholesky), the triangular solves keep the
+// right-hand side in registers (rows lane and lane + 64).
+__global__ __launch_bounds__(kSolveThreads) void ba_solve_kernel(BaDev* __restrict__ ba, double* __restrict__ Hbuf) {
+  if (ba->done || !ba->eval_pending) return;
+  const int lane = threadIdx.x;
+  const int n = ba->n;
+  const int ld = n | 1;  // odd row stride in LDS
+  __shared__ double s_A[kBaMaxDof * (kBaMaxDof + 1)];
+  __shared__ double s_d[kBaMaxDof];
+  const bool candidate = ba->eval_is_candidate != 0;
+  int cur = ba->h_cur;
+  double mu = ba->mu, v = ba->v, F = ba->F;
+  const double Fn = ba->Fn;
+  int lm_iter = ba->lm_iter, lm_stop = ba->lm_stop;
+  int accepted_total = ba->accepted_total, evals_total = ba->evals_total;
+  const int max_lm = ba->iters_per_solve;
+  const int r0 = lane, r1 = lane + 64;
+  bool take = !candidate;
+  if (candidate) {
+    // predicted decrease -(g.d + d.H.d / 2) at the current linearisation
+    const double* H = Hbuf + (size_t)cur * kBaMaxDof * kBaMaxDof;
     double part = 0.0;
-    for (int i = tid; i < n; i += 256) {
+    for (int i = lane; i < n; i += 64) {
       double hd = 0.0;
       for (int j = 0; j < n; ++j) hd += H[(size_t)i * n + j] * ba->d[j];
       part += ba->d[i] * (ba->g[i] + 0.5 * hd);
     }
-    s_red[tid] = part;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-      if (tid < w) s_red[tid] += s_red[tid + w];
-      __syncthreads();
+    const double pred = -wave_sum(part);
+    const double rho = pred > 0.0 ? (F - Fn) / pred : -1.0;
+    evals_total++;
+    if (rho > 1e-3) {
+      const double rel = fabs(F - Fn) / fmax(F, 1e-300);
+      mu = fmin(mu / fmax(1.0 / 3.0, 1.0 - pow(2.0 * rho - 1.0, 3.0)), 1e16);
+      v = 2.0;
+      accepted_total++;
+      if (rel < 1e-5) lm_stop = 1;
+      take = true;
+    } else {
+      mu /= v;
+      v *= 2.0;
     }
-    if (tid == 0) {
-      const double pred = -s_red[0];
-      const double F = ba->F, Fn = ba->Fn;
-      const double rho = pred > 0.0 ? (F - Fn) / pred : -1.0;
-      ba->evals_total++;
-      if (rho > 1e-3) {
-        const double rel = fabs(F - Fn) / fmax(F, 1e-300);
-        double mu = ba->mu / fmax(1.0 / 3.0, 1.0 - pow(2.0 * rho - 1.0, 3.0));
-        ba->mu = fmin(mu, 1e16);
-        ba->v = 2.0;
-        ba->F = Fn;
-        ba->accepted_total++;
-        s_flag = 1;
-        if (rel < 1e-5) ba->lm_stop = 1;
-      } else {
-        ba->mu /= ba->v;
-        ba->v *= 2.0;
-        s_flag = 0;
-      }
-      ba->lm_iter++;
-    }
-    __syncthreads();
-    if (s_flag) {
-      for (int i = tid; i < n * n; i += 256) H[i] = Hn[i];
-      for (int i = tid; i < n; i += 256) ba->g[i] = ba->gn[i];
-      for (int i = tid; i < ba->n_frames * 12; i += 256) ba->X[i] = ba->Xe[i];
-    }
+    lm_iter++;
   }
-  __syncthreads();
-
-  // (3) next candidate
+  // current linearisation: g in registers, poses X
+  const double* Xs = (take && candidate) ? ba->Xe : ba->X;
+  double g0 = 0.0, g1 = 0.0;
+  if (take) {
+    if (r0 < n) g0 = ba->gn[r0];
+    if (r1 < n) g1 = ba->gn[r1];
+    if (r0 < n) ba->g[r0] = g0;
+    if (r1 < n) ba->g[r1] = g1;
+    F = Fn;
+    cur = 1 - cur;
+  } else {
+    if (r0 < n) g0 = ba->g[r0];
+    if (r1 < n) g1 = ba->g[r1];
+  }
+  double xs2 = 0.0;
+  for (int i = lane; i < ba->n_frames * 12; i += 64) {
+    const double x = Xs[i];
+    xs2 += x * x;
+    if (take && candidate) ba->X[i] = x;
+  }
+  const double xn = sqrt(wave_sum(xs2));
+  const double* H = Hbuf + (size_t)cur * kBaMaxDof * kBaMaxDof;
+  int pending = 0;
   while (true) {
-    if (tid == 0) {
-      s_flag = 0;
-      if (ba->lm_stop || ba->lm_iter >= ba->iters_per_solve) {
-        s_flag = 1;
-      } else {
-        double gmax = 0.0;
-        for (int i = 0; i < n; ++i) gmax = fmax(gmax, fabs(ba->g[i]));
-        if (gmax < 1e-5) {
-          ba->lm_stop = 1;
-          s_flag = 1;
-        }
-      }
-      s_scalar[0] = ba->mu;
-      s_fail = 0;
+    if (lm_stop || lm_iter >= max_lm) break;
+    const double gmax = wave_max(fmax(fabs(g0), fabs(g1)));
+    if (gmax < 1e-5) {
+      lm_stop = 1;
+      break;
+    }
+    for (int idx = lane; idx < n * n; idx += 64) {
+      const int i = idx / n, j = idx - i * n;
+      double x = H[idx];
+      if (i == j) x += fmin(fmax(x, 1e-6), 1e32) / mu;
+      s_A[i * ld + j] = x;
     }
     __syncthreads();
-    if (s_flag) {
-      if (tid == 0) ba->eval_pending = 0;
-      return;
-    }
-    const double mu = s_scalar[0];
-    for (int idx = tid; idx < n * n; idx += 256) {
-      const int i = idx / n, j = idx - (idx / n) * n;
-      double v = H[idx];
-      if (i == j) v += fmin(fmax(v, 1e-6), 1e32) / mu;
-      s_A[idx] = v;
-    }
-    for (int i = tid; i < n; i += 256) s_b[i] = -ba->g[i];
-    __syncthreads();
-    // right-looking Cholesky, lower triangle
+    bool fail = false;
     for (int j = 0; j < n; ++j) {
-      if (tid == 0) {
-        const double s = s_A[j * n + j];
-        if (!(s > 0.0)) s_fail = 1;
-        else s_A[j * n + j] = sqrt(s);
+      const double ajj = s_A[j * ld + j];
+      if (!(ajj > 0.0)) {
+        fail = true;
+        break;
       }
+      const double djj = sqrt(ajj);
+      for (int i = j + 1 + lane; i < n; i += 64) s_A[i * ld + j] /= djj;
+      if (lane == 0) s_A[j * ld + j] = djj;
       __syncthreads();
-      if (s_fail) break;
-      const double djj = s_A[j * n + j];
-      for (int i = j + 1 + tid; i < n; i += 256) s_A[i * n + j] /= djj;
-      __syncthreads();
-      const int m = n - j - 1;
-      for (int idx = tid; idx < m * m; idx += 256) {
-        const int i = j + 1 + idx / m, k = j + 1 + idx % m;
-        if (k <= i) s_A[i * n + k] -= s_A[i * n + j] * s_A[k * n + j];
+      for (int i = j + 1 + lane; i < n; i += 64) {
+        const double lij = s_A[i * ld + j];
+        for (int k = j + 1; k <= i; ++k) s_A[i * ld + k] -= lij * s_A[k * ld + j];
       }
       __syncthreads();
     }
-    if (s_fail) {
-      if (tid == 0) {
-        ba->mu /= ba->v;
-        ba->v *= 2.0;
-        ba->lm_iter++;
-      }
+    if (fail) {
+      mu /= v;
+      v *= 2.0;
+      lm_iter++;
       __syncthreads();
       continue;
     }
-    // L y = b, L^T d = y
+    // L y = -g, then L^T d = y
+    double b0 = -g0, b1 = -g1;
     for (int i = 0; i < n; ++i) {
-      if (tid == 0) s_b[i] /= s_A[i * n + i];
-      __syncthreads();
-      const double yi = s_b[i];
-      for (int k = i + 1 + tid; k < n; k += 256) s_b[k] -= s_A[k * n + i] * yi;
-      __syncthreads();
+      const double bi = (i < 64) ? __shfl(b0, i, 64) : __shfl(b1, i - 64, 64);
+      const double yi = bi / s_A[i * ld + i];
+      if (r0 == i) b0 = yi;
+      if (r1 == i) b1 = yi;
+      if (r0 > i && r0 < n) b0 -= s_A[r0 * ld + i] * yi;
+      if (r1 > i && r1 < n) b1 -= s_A[r1 * ld + i] * yi;
     }
     for (int i = n - 1; i >= 0; --i) {
-      if (tid == 0) s_b[i] /= s_A[i * n + i];
-      __syncthreads();
-      const double xi = s_b[i];
-      for (int k = tid; k < i; k += 256) s_b[k] -= s_A[i * n + k] * xi;
-      __syncthreads();
+      const double bi = (i < 64) ? __shfl(b0, i, 64) : __shfl(b1, i - 64, 64);
+      const double xi = bi / s_A[i * ld + i];
+      if (r0 == i) b0 = xi;
+      if (r1 == i) b1 = xi;
+      if (r0 < i) b0 -= s_A[i * ld + r0] * xi;
+      if (r1 < i) b1 -= s_A[i * ld + r1] * xi;
     }
-    if (tid == 0) {
-      double dn = 0.0, xn = 0.0;
-      for (int i = 0; i < n; ++i) dn += s_b[i] * s_b[i];
-      for (int i = 0; i < ba->n_frames * 12; ++i) xn += ba->X[i] * ba->X[i];
-      s_flag = (sqrt(dn) < 1e-5 * (sqrt(xn) + 1e-5)) ? 1 : 0;
-      if (s_flag) {
-        ba->lm_stop = 1;
-        ba->eval_pending = 0;
-      }
+    double dd = 0.0;
+    if (r0 < n) dd += b0 * b0;
+    if (r1 < n) dd += b1 * b1;
+    const double dn = sqrt(wave_sum(dd));
+    if (dn < 1e-5 * (xn + 1e-5)) {
+      lm_stop = 1;
+      break;
+    }
+    if (r0 < n) {
+      s_d[r0] = b0;
+      ba->d[r0] = b0;
+    }
+    if (r1 < n) {
+      s_d[r1] = b1;
+      ba->d[r1] = b1;
     }
     __syncthreads();
-    if (s_flag) return;
-    for (int i = tid; i < n; i += 256) ba->d[i] = s_b[i];
-    for (int f = tid; f < ba->n_frames; f += 256) {
+    for (int f = lane; f < ba->n_frames; f += 64) {
       const int o = ba->free_idx[f];
-      if (o >= 0) pose_plus(ba->X + 12 * f, s_b + 6 * o, ba->Xe + 12 * f);
+      if (o >= 0) pose_plus(Xs + 12 * f, s_d + 6 * o, ba->Xe + 12 * f);
       else
-        for (int i = 0; i < 12; ++i) ba->Xe[12 * f + i] = ba->X[12 * f + i];
+        for (int i = 0; i < 12; ++i) ba->Xe[12 * f + i] = Xs[12 * f + i];
     }
-    if (tid == 0) {
-      ba->eval_pending = 1;
-      ba->eval_is_candidate = 1;
-    }
-    return;
+    pending = 1;
+    break;
+  }
+  if (lane == 0) {
+    ba->h_cur = cur;
+    ba->mu = mu;
+    ba->v = v;
+    ba->F = F;
+    ba->lm_iter = lm_iter;
+    ba->lm_stop = lm_stop;
+    ba->accepted_total = accepted_total;
+    ba->evals_total = evals_total;
+    ba->eval_pending = pending;
+    ba->eval_is_candidate = pending;
   }
 }
 
 // Per-edge lengthscale (BinaryStateGPU::update_ell, IRLS_State_GPU.cpp:
 // 117-146, mode 0), pose change (IRLS.cpp:99-108) and convergence
-// (IRLS.cpp:325-339).
-__global__ void ba_finish_kernel(BaDev* __restrict__ ba, BaEdge* __restrict__ edges) {
-  if (ba->done || threadIdx.x != 0) return;
-  bool all_min = true;
-  for (int e = 0; e < ba->n_edges; ++e) {
+// (IRLS.cpp:325-339).  One wave: lanes over edges and frames.
+__global__ __launch_bounds__(64) void ba_finish_kernel(BaDev* __restrict__ ba, BaEdge* __restrict__ edges) {
+  if (ba->done) return;
+  const int lane = threadIdx.x;
+  int not_min = 0;
+  for (int e = lane; e < ba->n_edges; e += 64) {
     BaEdge& ed = edges[e];
     ed.iters_per_ell++;
     if (ed.nnz_last > 0 && ed.nnz <= ed.nnz_last && ed.iters_per_ell / ba->iter_per_ell > 0) {
@@ -570,10 +623,10 @@ __global__ void ba_finish_kernel(BaDev* __restrict__ ba, BaEdge* __restrict__ ed
     }
     if (ed.ell < ba->ell_min) ed.ell = ba->ell_min;
     else if (ed.ell > ba->ell_max) ed.ell = ba->ell_max;
-    all_min = all_min && fabsf((float)ed.ell - ba->ell_min_f) < 1e-4f;
+    if (!(fabsf((float)ed.ell - ba->ell_min_f) < 1e-4f)) not_min = 1;
   }
   double change = 0.0;
-  for (int f = 0; f < ba->n_frames; ++f) {
+  for (int f = lane; f < ba->n_frames; f += 64) {
     const double* A = ba->Xold + 12 * f;
     const double* B = ba->X + 12 * f;
     double R[9], t[3];
@@ -584,9 +637,13 @@ __global__ void ba_finish_kernel(BaDev* __restrict__ ba, BaEdge* __restrict__ ed
     }
     change += se3_log_norm(R, t);
   }
-  ba->change = change;
-  ba->iter++;
-  if ((all_min && change < 1e-5) || ba->iter >= ba->max_iters) ba->done = 1;
+  change = wave_sum(change);
+  const bool all_min = wave_max((double)not_min) == 0.0;
+  if (lane == 0) {
+    ba->change = change;
+    ba->iter++;
+    if ((all_min && change < 1e-5) || ba->iter >= ba->max_iters) ba->done = 1;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -649,11 +706,10 @@ struct BaLaunch {
   BaDev* ba;
   BaEdge* edges;
   AssocProblem* probs;
-  int n_edges, total_blocks, max_pair_blocks, lm_rounds;
+  int n_edges, total_blocks, max_pair_blocks, lm_rounds, n_asm_blocks;
   double* part;
   double* contrib;
-  double* H;
-  double* Hn;
+  double* H;  // two n x n halves: current linearisation and evaluation
   KConsts kc;
 };
 
@@ -672,7 +728,8 @@ static void ba_lm_round(const BaLaunch& L, hipStream_t s) {
                      L.edges, L.probs, L.part, L.max_pair_blocks);
   hipLaunchKernelGGL(ba_edge_reduce_kernel, dim3(L.n_edges), dim3(256), 0, s, L.ba, L.edges, L.part,
                      L.max_pair_blocks, L.contrib);
-  hipLaunchKernelGGL(ba_solve_kernel, dim3(1), dim3(256), 0, s, L.ba, L.edges, L.contrib, L.H, L.Hn);
+  hipLaunchKernelGGL(ba_assemble_kernel, dim3(L.n_asm_blocks), dim3(64), 0, s, L.ba, L.edges, L.contrib, L.H);
+  hipLaunchKernelGGL(ba_solve_kernel, dim3(1), dim3(kSolveThreads), 0, s, L.ba, L.H);
 }
 
 static void ba_iteration(const BaLaunch& L, hipStream_t s) {
@@ -738,7 +795,7 @@ extern "C" cvo_status cvo_align_multiframe(cvo_ctx* ctx, int32_t n_frames, const
   tot_rows = row_off[n_edges];
   const size_t E1 = (size_t)std::max(n_edges, 1);
   DevBuf* B = ctx->b_ba;
-  CVO_TRY(ensure(ctx, B[BA_PAIRS], sizeof(uint2) * std::max<size_t>(tot_rows, 1) * cap));
+  CVO_TRY(ensure(ctx, B[BA_PAIRS], sizeof(float4) * std::max<size_t>(tot_rows, 1) * cap));
   CVO_TRY(ensure(ctx, B[BA_COUNTS], sizeof(uint32_t) * std::max<size_t>(tot_rows, 1)));
   CVO_TRY(ensure(ctx, B[BA_OVF], sizeof(uint32_t) * std::max<size_t>(tot_rows, 1)));
   CVO_TRY(ensure(ctx, B[BA_STATE], sizeof(ProblemState) * E1));
@@ -748,8 +805,7 @@ extern "C" cvo_status cvo_align_multiframe(cvo_ctx* ctx, int32_t n_frames, const
   CVO_TRY(ensure(ctx, B[BA_PART_CNT], sizeof(uint32_t) * 2 * (size_t)std::max(total_blocks, 1)));
   CVO_TRY(ensure(ctx, B[BA_DEV], sizeof(BaDev)));
   CVO_TRY(ensure(ctx, B[BA_EDGES], sizeof(BaEdge) * E1));
-  CVO_TRY(ensure(ctx, B[BA_H], sizeof(double) * kBaMaxDof * kBaMaxDof));
-  CVO_TRY(ensure(ctx, B[BA_HN], sizeof(double) * kBaMaxDof * kBaMaxDof));
+  CVO_TRY(ensure(ctx, B[BA_H], sizeof(double) * 2 * kBaMaxDof * kBaMaxDof));
   CVO_TRY(ensure(ctx, B[BA_PAIR_PART], sizeof(double) * kBaTerms * (size_t)max_pair_blocks * E1));
   CVO_TRY(ensure(ctx, B[BA_CONTRIB], sizeof(double) * kBaContrib * E1));
   CVO_HIP(ctx, hipMemsetAsync(B[BA_OVF].p, 0, sizeof(uint32_t) * std::max<size_t>(tot_rows, 1), st));
@@ -779,7 +835,8 @@ extern "C" cvo_status cvo_align_multiframe(cvo_ctx* ctx, int32_t n_frames, const
     P.tgt_sem = t->sem;
     P.tgt_geo = t->geo;
     P.st = as<ProblemState>(B[BA_STATE]) + e;
-    P.pairs = as<uint2>(B[BA_PAIRS]) + row_off[e] * cap;
+    P.pairs = nullptr;
+    P.pairs4 = as<float4>(B[BA_PAIRS]) + row_off[e] * cap;
     P.counts = as<uint32_t>(B[BA_COUNTS]) + row_off[e];
     P.ovf_stamp = as<uint32_t>(B[BA_OVF]) + row_off[e];
     P.cap_alloc = cap;
@@ -841,10 +898,10 @@ extern "C" cvo_status cvo_align_multiframe(cvo_ctx* ctx, int32_t n_frames, const
   L.total_blocks = total_blocks;
   L.max_pair_blocks = max_pair_blocks;
   L.lm_rounds = std::max(0, p.multiframe_iterations_per_solve) + 1;
+  L.n_asm_blocks = std::max(1, nfree * nfree + nfree);
   L.part = as<double>(B[BA_PAIR_PART]);
   L.contrib = as<double>(B[BA_CONTRIB]);
   L.H = as<double>(B[BA_H]);
-  L.Hn = as<double>(B[BA_HN]);
   L.kc = kc;
 
   CVO_TRY(pinned(ctx, 64));
@@ -886,34 +943,62 @@ extern "C" cvo_status cvo_align_multiframe(cvo_ctx* ctx, int32_t n_frames, const
       }
       hipLaunchKernelGGL(ba_edges_kernel, dim3(1), dim3(256), 0, st, L.ba, L.edges, L.probs);
       (void)hipEventRecord(ev[3], st);
+      // algorithmic bytes: association reads both clouds' points and writes
+      // the pairs; a pair pass reads the stored pairs, the target positions
+      // they name and the source rows
+      std::vector<BaEdge> eh(n_edges);
+      if (n_edges > 0)
+        (void)hipMemcpyAsync(eh.data(), L.edges, sizeof(BaEdge) * n_edges, hipMemcpyDeviceToHost, st);
+      (void)hipStreamSynchronize(st);
+      float m_assoc = 0, m_ovf = 0;
+      (void)hipEventElapsedTime(&m_assoc, ev[1], ev[2]);
+      (void)hipEventElapsedTime(&m_ovf, ev[2], ev[3]);
+      double pair_bytes = 0.0;
+      const double per_pt = 12.0 + (kc.use_col ? 4.0 * kc.F : 0.0) + (kc.use_sem ? 4.0 * kc.Ks : 0.0);
+      for (int e = 0; e < n_edges; ++e) {
+        const double S = (double)eh[e].nnz;
+        ctx->stats.bytes[CVO_KSTAT_ASSOC] += per_pt * (frames[eh[e].f1]->n + frames[eh[e].f2]->n) + 16.0 * S +
+                                             4.0 * frames[eh[e].f1]->n;
+        ctx->stats.pairs_stored += S;
+        if (eh[e].on) pair_bytes += 16.0 * S + 20.0 * eh[e].n_rows;
+      }
       float ms_pair = 0.0f, ms_solve = 0.0f;
+      int pair_launches = 0;
       for (int k = 0; k < (n_edges > 0 ? L.lm_rounds : 0); ++k) {
+        int32_t pend[2] = {0, 0};
+        (void)hipMemcpyAsync(pend, &L.ba->eval_pending, sizeof(int32_t) * 2, hipMemcpyDeviceToHost, st);
+        (void)hipStreamSynchronize(st);
+        const bool work = pend[0] != 0;
         (void)hipEventRecord(ev[4], st);
         hipLaunchKernelGGL(ba_pair_kernel, dim3(L.max_pair_blocks, n_edges), dim3(kBaRowsPerBlock), 0, st, L.ba,
                            L.edges, L.probs, L.part, L.max_pair_blocks);
         (void)hipEventRecord(ev[5], st);
         hipLaunchKernelGGL(ba_edge_reduce_kernel, dim3(n_edges), dim3(256), 0, st, L.ba, L.edges, L.part,
                            L.max_pair_blocks, L.contrib);
-        hipLaunchKernelGGL(ba_solve_kernel, dim3(1), dim3(256), 0, st, L.ba, L.edges, L.contrib, L.H, L.Hn);
+        hipLaunchKernelGGL(ba_assemble_kernel, dim3(L.n_asm_blocks), dim3(64), 0, st, L.ba, L.edges, L.contrib,
+                           L.H);
+        hipLaunchKernelGGL(ba_solve_kernel, dim3(1), dim3(kSolveThreads), 0, st, L.ba, L.H);
         (void)hipEventRecord(ev[3], st);
         (void)hipEventSynchronize(ev[3]);
         float a = 0, b = 0;
         (void)hipEventElapsedTime(&a, ev[4], ev[5]);
         (void)hipEventElapsedTime(&b, ev[5], ev[3]);
-        ms_pair += a;
-        ms_solve += b;
+        if (work) {
+          ms_pair += a;
+          ms_solve += b;
+          pair_launches++;
+          ctx->stats.bytes[CVO_KSTAT_BA_PAIR] += pair_bytes;
+        }
       }
       hipLaunchKernelGGL(ba_finish_kernel, dim3(1), dim3(64), 0, st, L.ba, L.edges);
-      (void)hipEventSynchronize(ev[3]);
-      float m_assoc = 0, m_ovf = 0;
-      (void)hipEventElapsedTime(&m_assoc, ev[1], ev[2]);
-      (void)hipEventElapsedTime(&m_ovf, ev[2], ev[3]);
       ctx->stats.ms[CVO_KSTAT_ASSOC] += m_assoc;
       ctx->stats.launches[CVO_KSTAT_ASSOC] += 1;
+      ctx->stats.ms[CVO_KSTAT_OVERFLOW] += m_ovf;
+      ctx->stats.launches[CVO_KSTAT_OVERFLOW] += 1;
       ctx->stats.ms[CVO_KSTAT_BA_PAIR] += ms_pair;
-      ctx->stats.launches[CVO_KSTAT_BA_PAIR] += n_edges > 0 ? L.lm_rounds : 0;
+      ctx->stats.launches[CVO_KSTAT_BA_PAIR] += pair_launches;
       ctx->stats.ms[CVO_KSTAT_BA_SOLVE] += ms_solve;
-      ctx->stats.launches[CVO_KSTAT_BA_SOLVE] += n_edges > 0 ? L.lm_rounds : 0;
+      ctx->stats.launches[CVO_KSTAT_BA_SOLVE] += pair_launches;
       le = hipGetLastError();
     } else {
       ba_iteration(L, st);

@@ -92,6 +92,7 @@ struct AssocProblem {
   const float* tgt_geo;
   ProblemState* st;
   uint2* pairs;            // n_src * cap_alloc: (sorted target position, a bits)
+  float4* pairs4;          // bundle adjustment: n_src * cap_alloc (target xyz in its frame, a)
   uint32_t* counts;        // n_src
   uint32_t* ovf_stamp;     // n_src
   int32_t cap_alloc;

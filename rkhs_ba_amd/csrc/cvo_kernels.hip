@@ -328,9 +328,10 @@ __device__ __forceinline__ void assoc_row(const AssocProblem& P, const KConsts& 
       bool pass = false;
       float a = 0.0f, y0 = 0.0f, y1 = 0.0f, y2 = 0.0f;
       uint32_t pidx = 0;
+      float4 y4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
       if (idx < total) {
         pidx = run_start + (idx - run_excl);
-        const float4 y4 = G.pos[pidx];
+        y4 = G.pos[pidx];
         affine3(mp.A, mp.b, y4.x, y4.y, y4.z, y0, y1, y2);
         pass = pair_kernel<FC, KC>(kc, ek, x0, x1, x2, y0, y1, y2, d2t, qg, l, xf,
                                    FC != 0 ? G.feat + (size_t)pidx * F : nullptr, xs,
@@ -341,7 +342,8 @@ __device__ __forceinline__ void assoc_row(const AssocProblem& P, const KConsts& 
       const uint32_t gm = (uint32_t)(m >> (16 * ((threadIdx.x & 63) >> 4))) & 0xFFFFu;
       const uint32_t slot = stored + (uint32_t)__popc(gm & ((1u << g) - 1u));
       if (pass && slot < Ku) {
-        P.pairs[(size_t)r * P.cap_alloc + slot] = make_uint2(pidx, __float_as_uint(a));
+        if (mode == MODE_BA) P.pairs4[(size_t)r * Ku + slot] = make_float4(y4.x, y4.y, y4.z, a);
+        else P.pairs[(size_t)r * P.cap_alloc + slot] = make_uint2(pidx, __float_as_uint(a));
         if (mode == MODE_ALIGN) {
           const float c0 = x1 * y2 - x2 * y1, c1 = x2 * y0 - x0 * y2, c2 = x0 * y1 - x1 * y0;
           out.acc[0] += (double)(c0 * a);
@@ -561,7 +563,10 @@ __global__ __launch_bounds__(kBlock) void overflow_kernel(const AssocProblem* __
       before += (uint32_t)__popcll(m & ((lane == 0) ? 0ull : (~0ull >> (64 - lane))));
       const uint32_t slot = stored + before;
       if (pass && slot < K) {
-        P.pairs[(size_t)r * P.cap_alloc + slot] = make_uint2(G.inv_perm[j], __float_as_uint(a));
+        if (mode == MODE_BA)
+          P.pairs4[(size_t)r * K + slot] =
+              make_float4(P.tgt_xyz[3 * (size_t)j], P.tgt_xyz[3 * (size_t)j + 1], P.tgt_xyz[3 * (size_t)j + 2], a);
+        else P.pairs[(size_t)r * P.cap_alloc + slot] = make_uint2(G.inv_perm[j], __float_as_uint(a));
         if (mode == MODE_ALIGN) {
           const float c0 = x1 * y2 - x2 * y1, c1 = x2 * y0 - x0 * y2, c2 = x0 * y1 - x1 * y0;
           acc[0] += (double)(c0 * a);
