@@ -267,6 +267,40 @@ cvo_status cvo_align_multiframe(cvo_ctx* ctx, int32_t n_frames, const cvo_dcloud
                                 const uint8_t* hold_const, int32_t n_edges, const int32_t* edges, double* seconds,
                                 int32_t* iters, double* per_iter_seconds, int32_t max_iter_records);
 
+/* ---- learned pose proposer (no counterpart in the reference) -------------- */
+
+/* For global registration callers (the multi-start path): a point-set
+ * encoder shared by both clouds (8 -> 64 -> 128 -> 256, max-pool) and a head
+ * (512 -> 256 -> 8 x 7 + 8) proposing 8 rigid transforms with confidences.
+ * BF16 weights and activations, FP32 accumulation on MFMA, FP32 master weights
+ * trained with Adam by the library's own forward and backward kernels.
+ * Clouds are passed as n_pairs x npts points of 8 floats (xyz in metres, 5
+ * features); npts is a multiple of 64.  Parameters are one flat FP32 vector
+ * (W1 8x64, b1, W2 64x128, b2, W3 128x256, b3, W4 512x256, b4, W5 256x64, b5;
+ * row-major, input index first). */
+typedef struct cvo_model cvo_model;
+cvo_status cvo_model_create(cvo_ctx* ctx, int32_t k, uint64_t seed, cvo_model** out); /* k must be 8 */
+void cvo_model_destroy(cvo_model* m);
+int64_t cvo_model_num_params(void);
+cvo_status cvo_model_get_weights(const cvo_model* m, float* out, int64_t cap, int64_t* n_out);
+cvo_status cvo_model_set_weights(cvo_model* m, const float* w, int64_t n); /* also resets Adam state */
+/* targets: n_pairs x 7 (unit quaternion w,x,y,z and translation of the
+ * source-to-target transform, the align-init convention).  Mean loss over
+ * the batch; grad_out (optional, num_params floats). */
+cvo_status cvo_model_gradient(cvo_ctx* ctx, cvo_model* m, int32_t n_pairs, int32_t npts, const float* a,
+                              const float* b, const float* targets, float* grad_out, float* loss_out);
+/* One Adam step (beta 0.9 / 0.999) with learning rate lr; loss_out optional. */
+cvo_status cvo_model_train_step(cvo_ctx* ctx, cvo_model* m, int32_t n_pairs, int32_t npts, const float* a,
+                                const float* b, const float* targets, float lr, float* loss_out);
+/* Batched inference: quat_t n_pairs x 8 x 7 (unit quaternion with w >= 0,
+ * translation in metres; source-to-target), conf n_pairs x 8 (softmax);
+ * raw_out (optional) n_pairs x 64 head outputs. */
+cvo_status cvo_propose_poses(cvo_ctx* ctx, const cvo_model* m, int32_t n_pairs, int32_t npts, const float* a,
+                             const float* b, float* quat_t, float* conf, float* raw_out);
+/* The BF16 MFMA GEMM on host arrays, C (M x N) = A (M x K) B (K x N), inputs
+ * rounded to BF16, FP32 result: the tile kernel the proposer runs on. */
+cvo_status cvo_gemm_bf16(cvo_ctx* ctx, int32_t M, int32_t N, int32_t K, const float* A, const float* B, float* C);
+
 /* ---- profiling ----------------------------------------------------------- */
 
 cvo_status cvo_set_profiling(cvo_ctx* ctx, int32_t on);
