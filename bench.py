@@ -13,6 +13,9 @@ Further legs, reported as nested objects of the same line:
   multistart  global registration of two 20k-point clouds (uniform random
               rotation, 2 m) from 96 starts on a super-Fibonacci SO(3) grid in
               one batched launch sequence; starts/s and success
+  learned     the learned proposer: inference of 256 pairs x 2048 points
+              (MFMA roofline of its forward), and global registration from its
+              8 proposals (grid fallback) against the plain 96-start grid
 
 Each leg carries the roofline of its dominant kernel (HIP events on the
 library's stream) and the CPU oracle timed on the host cores on a bounded
@@ -269,6 +272,71 @@ def leg_multistart(args):
     return out
 
 
+# ---------------------------------------------------------------------------
+# learned proposals + CVO vs the plain grid
+# ---------------------------------------------------------------------------
+
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+
+
+def leg_learned(args):
+    from rkhs_ba_amd import CvoGPU, read_params_yaml, synth
+    from rkhs_ba_amd import proposer as P
+    params = read_params_yaml(os.path.join(CONFIGS, "multistart_20k.yaml"))
+    params.MAX_ITER = 1000
+    cvo = CvoGPU(params=params)
+    prop = P.PoseProposer.load(cvo)
+    log("[bench] learned: inference batch of 256 pairs x 2048 points")
+    a, b, t = P.make_batch(range(8_000_000, 8_000_256))
+    prop.propose(a, b)
+    t0 = time.perf_counter()
+    reps = 3
+    for _ in range(reps):
+        prop.propose(a, b)
+    wall = (time.perf_counter() - t0) / reps
+    cvo.reset_kernel_stats()
+    cvo.set_profiling(True)
+    prop.propose(a, b)
+    cvo.set_profiling(False)
+    st = cvo.kernel_stats()
+    fwd_ms = st["ms"]["mlp"] / max(1, st["launches"]["mlp"])
+    flops = st["bytes"]["mlp"] / max(1, st["launches"]["mlp"])
+    achieved = flops / (fwd_ms * 1e-3) / 1e12
+    out = {"inference_pairs_per_s": 256 / wall, "inference_batch_ms": wall * 1e3, "forward_ms": fwd_ms,
+           "roofline": {"bound": "mfma", "achieved": achieved, "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                        "frac": achieved / MFMA_BF16_PEAK_TFLOPS, "kernel": "gemm_bf16_kernel (encoder + head)",
+                        "flops_per_launch": flops, "traffic": None}}
+    n = args.learned_pairs
+    ok_g = ok_l = 0
+    t_g = t_l = 0.0
+    fallbacks = 0
+    for s in range(n):
+        src, tgt, Tgt = synth.planes_spheres_pair(7100 + s, 20000, 20000, uniform_rot=True, trans=2.0)
+        g = P.register_global(cvo, src, tgt, None)
+        l = P.register_global(cvo, src, tgt, prop, min_cos=P.DEFAULT_MIN_COS)
+        eg, el = pose_error(g["T_best"], Tgt), pose_error(l["T_best"], Tgt)
+        ok_g += eg[0] <= np.degrees(0.01) and eg[1] <= 0.01
+        ok_l += el[0] <= np.degrees(0.01) and el[1] <= 0.01
+        t_g += g["seconds"]
+        t_l += l["seconds"]
+        fallbacks += int(l["fallback"])
+    log(f"[bench] learned: grid {ok_g}/{n} in {t_g / n:.3f} s/pair, learned {ok_l}/{n} in {t_l / n:.3f} s/pair")
+    out.update({"pairs": n, "grid_success": ok_g / n, "learned_success": ok_l / n, "grid_s_per_pair": t_g / n,
+                "learned_s_per_pair": t_l / n, "throughput_vs_grid": t_g / max(t_l, 1e-9), "fallbacks": fallbacks})
+    if not args.no_cpu and not args.profile_only:
+        from oracle import mlp_oracle
+        w = prop.get_weights().astype(np.float64)
+        t0 = time.perf_counter()
+        mlp_oracle.forward(w, a[:32].astype(np.float64), b[:32].astype(np.float64), quantize=False)
+        cpu_s = (time.perf_counter() - t0) * 256 / 32
+        out["cpu_baseline"] = {"value": 256 / cpu_s, "unit": "pairs/s (inference)", "cores": cpu_threads(),
+                               "kind": "port", "sample": "numpy FP64 forward of the same network on 32 of the "
+                                                         "256 pairs, scaled to the batch"}
+    prop.close()
+    cvo.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -276,7 +344,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--points", type=int, default=60000)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--legs", default="twoview,ba,multistart")
+    ap.add_argument("--legs", default="twoview,ba,multistart,learned")
+    ap.add_argument("--learned-pairs", type=int, default=4, help="pairs for learned vs grid registration")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
     ap.add_argument("--profile-only", action="store_true", help="timed loops only (for rocprofv3)")
     ap.add_argument("--traffic", type=float, default=None,
@@ -294,6 +363,8 @@ def main():
         extra["ba_window"] = leg_ba(args)
     if "multistart" in legs:
         extra["multistart"] = leg_multistart(args)
+    if "learned" in legs:
+        extra["learned"] = leg_learned(args)
     cpu = tv.get("cpu_baseline")
     out = {
         "metric": METRIC,

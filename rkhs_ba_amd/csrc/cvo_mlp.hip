@@ -968,7 +968,23 @@ extern "C" cvo_status cvo_propose_poses(cvo_ctx* ctx, const cvo_model* mc, int32
   cvo_model* m = const_cast<cvo_model*>(mc);
   hipStream_t s = ctx->stream;
   CVO_TRY(upload_points(m, n_pairs, npts, a, b));
+  if (ctx->profiling) {
+    CVO_TRY(ensure_events(ctx, 2));
+    CVO_HIP(ctx, hipEventRecord(ctx->ev_pool[0], s));
+  }
   CVO_TRY(forward(m, n_pairs, npts));
+  if (ctx->profiling) {
+    CVO_HIP(ctx, hipEventRecord(ctx->ev_pool[1], s));
+    CVO_HIP(ctx, hipEventSynchronize(ctx->ev_pool[1]));
+    float ms = 0.0f;
+    CVO_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev_pool[0], ctx->ev_pool[1]));
+    // algorithmic flops of the forward: 2 per multiply-add (input padded to 16)
+    const double pts = 2.0 * n_pairs * npts;
+    const double macs = pts * (kMlpInPad * kH1 + kH1 * kH2 + kH2 * kH3) + (double)n_pairs * (kZ * kZ1 + kZ1 * kMlpOut);
+    ctx->stats.ms[CVO_KSTAT_MLP] += ms;
+    ctx->stats.launches[CVO_KSTAT_MLP] += 1;
+    ctx->stats.bytes[CVO_KSTAT_MLP] += 2.0 * macs;
+  }
   CVO_TRY(ensure(ctx, m->ws[WS_QT], sizeof(float) * n_pairs * kMlpK * 7));
   CVO_TRY(ensure(ctx, m->ws[WS_CONF], sizeof(float) * n_pairs * kMlpK));
   hipLaunchKernelGGL(mlp_decode_kernel, dim3((n_pairs + 63) / 64), dim3(64), 0, s, wsp<float>(m, WS_O),
