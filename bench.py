@@ -36,6 +36,10 @@ sys.path.insert(0, ROOT)
 
 METRIC = "registrations/s @60k pts + BA-window s/iter; batched/learned vs plain, 1 GPU"
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per association launch on the 60k street workload, from rocprofv3
+# FETCH_SIZE / WRITE_SIZE passes scaled by a 256 MiB copy probe
+# (profiles/assoc_street60k_pmc_traffic.md)
+ASSOC_TRAFFIC_BYTES = 42.94e6
 CONFIGS = os.path.join(ROOT, "rkhs_ba_amd", "configs")
 
 
@@ -348,9 +352,21 @@ def main():
     ap.add_argument("--learned-pairs", type=int, default=4, help="pairs for learned vs grid registration")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
     ap.add_argument("--profile-only", action="store_true", help="timed loops only (for rocprofv3)")
-    ap.add_argument("--traffic", type=float, default=None,
-                    help="HBM bytes per association launch from rocprofv3 counters, if measured")
+    ap.add_argument("--traffic", type=float, default=ASSOC_TRAFFIC_BYTES,
+                    help="HBM bytes per association launch from rocprofv3 counters")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only copy 256 MiB device-to-device 5 times (calibrates HBM byte counters)")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        src = torch.empty(64 << 20, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        src.fill_(1.0)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": 256 << 20, "copies": 5}))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU sharding is design-only",
                           "n_gpus": args.gpus}))

@@ -301,6 +301,15 @@ cvo_status cvo_propose_poses(cvo_ctx* ctx, const cvo_model* m, int32_t n_pairs, 
  * rounded to BF16, FP32 result: the tile kernel the proposer runs on. */
 cvo_status cvo_gemm_bf16(cvo_ctx* ctx, int32_t M, int32_t N, int32_t K, const float* A, const float* B, float* C);
 
+/* ---- pose arithmetic ------------------------------------------------------ */
+
+/* The align loop's pose update.  CVO_POSE_FP64 (default): FP64 pose, SE(3)
+ * exponential and step.  CVO_POSE_REFERENCE: the reference's float pose with
+ * its FP32 Exp_SEK3 (LieGroup.cpp:244-275, small-angle branch included) and a
+ * float step (CvoGPU.cu:1289-1341), for bit-level comparisons with it. */
+enum { CVO_POSE_FP64 = 0, CVO_POSE_REFERENCE = 1 };
+cvo_status cvo_set_pose_arithmetic(cvo_ctx* ctx, int32_t mode);
+
 /* ---- profiling ----------------------------------------------------------- */
 
 cvo_status cvo_set_profiling(cvo_ctx* ctx, int32_t on);

@@ -376,7 +376,42 @@ void assoc_core(const cvo_params* p, const CloudRef& S, const CloudRef& Tg, cons
 void flow_core(const cvo_params* p, int N, const float* X, const float* Y, int K, const int32_t* counts,
                const int32_t* cols, const float* vals, int precision, double out6[6], double unit6[6]) {
   double sum[6] = {0, 0, 0, 0, 0, 0};
-  if (precision == ORC_MIRROR) {
+  if (precision == ORC_POLICY) {
+    // the library's flow: float products a * (x x y) and a * (y - x), summed
+    // in FP64; omega / c and v / d on the totals; FP64 normalisation
+    std::vector<double> rows((size_t)N * 6);
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < N; ++i) {
+      const float* x = X + 3 * (size_t)i;
+      double acc[6] = {0, 0, 0, 0, 0, 0};
+      for (int s = 0; s < counts[i]; ++s) {
+        const int j = cols[(size_t)i * K + s];
+        const float a = vals[(size_t)i * K + s];
+        const float* y = Y + 3 * (size_t)j;
+        const float c0 = x[1] * y[2] - x[2] * y[1], c1 = x[2] * y[0] - x[0] * y[2], c2 = x[0] * y[1] - x[1] * y[0];
+        acc[0] += (double)(c0 * a);
+        acc[1] += (double)(c1 * a);
+        acc[2] += (double)(c2 * a);
+        acc[3] += (double)((y[0] - x[0]) * a);
+        acc[4] += (double)((y[1] - x[1]) * a);
+        acc[5] += (double)((y[2] - x[2]) * a);
+      }
+      for (int d = 0; d < 6; ++d) rows[(size_t)i * 6 + d] = acc[d];
+    }
+    for (int i = 0; i < N; ++i)
+      for (int d = 0; d < 6; ++d) sum[d] += rows[(size_t)i * 6 + d];
+    for (int d = 0; d < 3; ++d) {
+      sum[d] /= (double)p->c;
+      sum[3 + d] /= (double)p->d;
+    }
+    double z = 0.0;
+    for (int d = 0; d < 6; ++d) z += sum[d] * sum[d];
+    const double s = z > 0.0 ? std::sqrt(z) : 1.0;
+    for (int d = 0; d < 6; ++d) {
+      out6[d] = sum[d];
+      unit6[d] = z > 0.0 ? sum[d] / s : sum[d];
+    }
+  } else if (precision == ORC_MIRROR) {
     std::vector<double> rows((size_t)N * 6);
 #pragma omp parallel for schedule(static)
     for (int i = 0; i < N; ++i) {
@@ -461,7 +496,53 @@ void step_core(const cvo_params* p, float ell, int N, const float* X, const floa
                const int32_t* cols, const float* vals, const double flow6[6], int precision, double bcde[4]) {
   double tot[4] = {0, 0, 0, 0};
   std::vector<double> rows((size_t)N * 4);
-  if (precision == ORC_MIRROR) {
+  if (precision == ORC_POLICY) {
+    // the library's step terms: xi^k z by repeated cross products in float
+    const float w0 = (float)flow6[0], w1 = (float)flow6[1], w2 = (float)flow6[2];
+    const float v0 = (float)flow6[3], v1 = (float)flow6[4], v2 = (float)flow6[5];
+    auto cr = [&](float a0, float a1, float a2, float& o0, float& o1, float& o2) {
+      o0 = w1 * a2 - w2 * a1;
+      o1 = w2 * a0 - w0 * a2;
+      o2 = w0 * a1 - w1 * a0;
+    };
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < N; ++i) {
+      const float* x = X + 3 * (size_t)i;
+      float temp_ell = ell;
+      if (p->is_using_range_ell) {
+        const float px = sqrtf(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+        temp_ell = (float)(((double)px / 500.0 + 1.0) * (double)ell);
+      }
+      const float tc = (float)(1.0 / (2.0 * (double)temp_ell * (double)temp_ell));
+      double acc[4] = {0, 0, 0, 0};
+      for (int s = 0; s < counts[i]; ++s) {
+        const int j = cols[(size_t)i * K + s];
+        const float a = vals[(size_t)i * K + s];
+        const float* z = Y + 3 * (size_t)j;
+        float a0, a1, a2, b0, b1, b2, c0, c1, c2, d0, d1, d2;
+        cr(z[0], z[1], z[2], a0, a1, a2);
+        a0 += v0; a1 += v1; a2 += v2;
+        cr(a0, a1, a2, b0, b1, b2);
+        cr(b0, b1, b2, c0, c1, c2);
+        cr(c0, c1, c2, d0, d1, d2);
+        const float df0 = x[0] - z[0], df1 = x[1] - z[1], df2 = x[2] - z[2];
+        const float normxiz2 = a0 * a0 + a1 * a1 + a2 * a2;
+        const float xiz_dot_xi2z = -(a0 * b0 + a1 * b1 + a2 * b2);
+        const float epsil_const = (b0 * b0 + b1 * b1 + b2 * b2) + 2.0f * (a0 * c0 + a1 * c1 + a2 * c2);
+        const float beta = (float)(-2.0 * (double)tc * (double)(a0 * df0 + a1 * df1 + a2 * df2));
+        const float gamma = -tc * (normxiz2 + 2.0f * (b0 * df0 + b1 * df1 + b2 * df2));
+        const float delta = (float)(2.0 * (double)tc * (double)(xiz_dot_xi2z - (c0 * df0 + c1 * df1 + c2 * df2)));
+        const float epsil = -tc * (epsil_const + 2.0f * (d0 * df0 + d1 * df1 + d2 * df2));
+        const double ad = (double)a, bd = (double)beta, gd = (double)gamma;
+        acc[0] += (double)(a * beta);
+        acc[1] += ad * (gd + (double)(beta * beta) / 2.0);
+        acc[2] += ad * ((double)(delta + beta * gamma) + (double)(beta * beta * beta) / 6.0);
+        acc[3] += ad * ((double)(epsil + beta * delta) + 0.5 * bd * bd * gd + 0.5 * gd * gd +
+                        (1.0 / 24.0) * bd * bd * bd * bd);
+      }
+      for (int c = 0; c < 4; ++c) rows[(size_t)i * 4 + c] = acc[c];
+    }
+  } else if (precision == ORC_MIRROR) {
     float w[3] = {(float)flow6[0], (float)flow6[1], (float)flow6[2]};
     float v[3] = {(float)flow6[3], (float)flow6[4], (float)flow6[5]};
     // Omega = skew(w) and its powers (float), compute_step_size_xi (:333-378)
@@ -884,6 +965,11 @@ extern "C" int orc_align(const cvo_params* p, const cvo_cloud_view* src, const c
   }
   float ell = p->ell_init;
   double ell64 = p->ell_init;
+  // ORC_POLICY: mirror kernels, FP64 pose; ORC_MIRROR: float pose as well
+  const int kern = precision == ORC_FP64 ? ORC_FP64 : ORC_MIRROR;  // association
+  const int fkern = precision;                                         // flow and step terms
+  const bool float_pose = precision == ORC_MIRROR;
+  const bool float_ell = precision != ORC_FP64;
   int K = p->nearest_neighbors_max;
   Indicator<float> ind32;
   Indicator<double> ind64;
@@ -891,23 +977,25 @@ extern "C" int orc_align(const cvo_params* p, const cvo_cloud_view* src, const c
   std::vector<float> yp((size_t)M * 3);
   for (; k < p->MAX_ITER; ++k) {
     Pose64 P;
-    for (int i = 0; i < 9; ++i) P.R[i] = precision == ORC_MIRROR ? (double)Rf[i] : Rd[i];
-    for (int i = 0; i < 3; ++i) P.T[i] = precision == ORC_MIRROR ? (double)Tf[i] : Td[i];
+    for (int i = 0; i < 9; ++i) P.R[i] = float_pose ? (double)Rf[i] : Rd[i];
+    for (int i = 0; i < 3; ++i) P.T[i] = float_pose ? (double)Tf[i] : Td[i];
     float A[9], b[3];
     applied_transform(P, A, b);
     for (int j = 0; j < M; ++j) xform_f(A, b, Tg.xyz + 3 * (size_t)j, &yp[3 * (size_t)j]);
     AssocOut o;
-    float ell_used = precision == ORC_MIRROR ? ell : (float)ell64;
-    assoc_core(p, S, Tg, P, ell_used, K, precision, brute, o);
+    float ell_used = float_ell ? ell : (float)ell64;
+    assoc_core(p, S, Tg, P, ell_used, K, kern, brute, o);
     double sum6[6], unit6[6];
-    flow_core(p, N, S.xyz, yp.data(), K, o.counts.data(), o.cols.data(), o.vals.data(), precision, sum6, unit6);
+    flow_core(p, N, S.xyz, yp.data(), K, o.counts.data(), o.cols.data(), o.vals.data(), fkern, sum6, unit6);
     double bcde[4];
-    step_core(p, ell_used, N, S.xyz, yp.data(), K, o.counts.data(), o.cols.data(), o.vals.data(), unit6, precision, bcde);
+    step_core(p, ell_used, N, S.xyz, yp.data(), K, o.counts.data(), o.cols.data(), o.vals.data(), unit6, fkern, bcde);
     double step = select_step(p, bcde);
     if (precision == ORC_MIRROR) step = (double)(float)step;  // cvo_state->step is float
     // stop test on the normalised flow (CvoGPU.cu:1354-1360)
-    double nw = std::sqrt(unit6[0] * unit6[0] + unit6[1] * unit6[1] + unit6[2] * unit6[2]);
-    double nv = std::sqrt(unit6[3] * unit6[3] + unit6[4] * unit6[4] + unit6[5] * unit6[5]);
+    double u6[6];
+    for (int d = 0; d < 6; ++d) u6[d] = precision == ORC_POLICY ? (double)(float)unit6[d] : unit6[d];
+    double nw = std::sqrt(u6[0] * u6[0] + u6[1] * u6[1] + u6[2] * u6[2]);
+    double nv = std::sqrt(u6[3] * u6[3] + u6[4] * u6[4] + u6[5] * u6[5]);
     cvo_iter_record rec{};
     rec.ell = ell_used;
     rec.step = (float)step;
@@ -922,7 +1010,7 @@ extern "C" int orc_align(const cvo_params* p, const cvo_cloud_view* src, const c
       break;
     }
     double dRT[12];
-    if (precision == ORC_MIRROR) {
+    if (float_pose) {
       float xi[6];
       for (int d = 0; d < 6; ++d) xi[d] = (float)unit6[d];
       float dt[12];
@@ -963,7 +1051,7 @@ extern "C" int orc_align(const cvo_params* p, const cvo_cloud_view* src, const c
     rec.dist = (float)dist;
     float ip = (float)((double)o.cnt.stored / std::sqrt((double)N * (double)M));
     rec.ip = ip;
-    bool dec = precision == ORC_MIRROR ? ind32.update(ip, p->indicator_window_size, p->indicator_stable_threshold)
+    bool dec = float_ell ? ind32.update(ip, p->indicator_window_size, p->indicator_stable_threshold)
                                        : ind64.update((double)ip, p->indicator_window_size, (double)p->indicator_stable_threshold);
     if (trace && k < trace_cap) trace[k] = rec;
     if (dist < p->eps_2) break;
@@ -980,7 +1068,7 @@ extern "C" int orc_align(const cvo_params* p, const cvo_cloud_view* src, const c
   // final update_tf: transform = [R^T, -R^T T]
   std::memset(T_out, 0, sizeof(float) * 16);
   T_out[15] = 1.0f;
-  if (precision == ORC_MIRROR) {
+  if (float_pose) {
     for (int r = 0; r < 3; ++r)
       for (int c = 0; c < 3; ++c) T_out[r * 4 + c] = Rf[c * 3 + r];
     for (int r = 0; r < 3; ++r)

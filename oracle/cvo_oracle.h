@@ -15,6 +15,10 @@
  * Every function cites the reference lines it follows (paths relative to the
  * UnifiedCvo tree).  Pins (what checks each function independently) are listed
  * in the header comment of cvo_oracle.cpp.
+  *   ORC_POLICY (2): association, flow and step terms at the reference's
+ *              rounding points (as ORC_MIRROR) with the pose, the SE(3)
+ *              exponential and the step in FP64 -- the GPU library's default
+ *              precision policy (align only).
  */
 #ifndef CVO_ORACLE_H
 #define CVO_ORACLE_H
@@ -26,7 +30,7 @@
 extern "C" {
 #endif
 
-enum { ORC_MIRROR = 0, ORC_FP64 = 1 };
+enum { ORC_MIRROR = 0, ORC_FP64 = 1, ORC_POLICY = 2 };
 
 /* Pair counters of one association pass. */
 typedef struct {

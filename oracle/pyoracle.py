@@ -13,7 +13,7 @@ import numpy as np
 
 from rkhs_ba_amd._abi import (CvoParams, HeldView, IterRecord, mat16)
 
-MIRROR, FP64 = 0, 1
+MIRROR, FP64, POLICY = 0, 1, 2
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "_build", "liboracle_cvo.so")
 _lib = None

@@ -17,6 +17,18 @@ def build_library(jobs: int = 8) -> str:
     return os.path.join(HERE, "librkhs_ba_amd.so")
 
 
+FACADE_CHECK = os.path.join(HERE, "facade_check.bin")
+
+
+def build_facade_check() -> str:
+    """Compiles tests/cpp/facade_check.cpp against the C++ facade
+    (include/cvo/CvoGPU.hpp) and the library."""
+    src = os.path.join(ROOT, "tests", "cpp", "facade_check.cpp")
+    subprocess.run(["g++", "-std=c++17", "-O1", "-I" + os.path.join(ROOT, "include"), src, "-L" + HERE,
+                    "-lrkhs_ba_amd", "-Wl,-rpath,$ORIGIN", "-o", FACADE_CHECK], check=True)
+    return FACADE_CHECK
+
+
 def build_oracle() -> str:
     """Compiles the CPU oracle (test infrastructure) into oracle/_build/."""
     od = os.path.join(ROOT, "oracle")
@@ -27,4 +39,5 @@ def build_oracle() -> str:
 
 if __name__ == "__main__":
     print(build_library())
+    print(build_facade_check())
     print(build_oracle())

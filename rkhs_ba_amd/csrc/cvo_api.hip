@@ -789,6 +789,7 @@ extern "C" cvo_status cvo_align_d(cvo_ctx* ctx, const cvo_dcloud* src, const cvo
   CVO_TRY(validate_pair(ctx, src, tgt));
   const cvo_params& p = ctx->params;
   KConsts kc = make_kconsts(p, src->F, src->K, MODE_ALIGN);
+  kc.mirror_pose = ctx->pose_arith == CVO_POSE_REFERENCE ? 1 : 0;
   kc.n_src_f = (float)src->n;
   kc.n_tgt_f = (float)tgt->n;
   kc.max_src_norm = src->max_norm;
@@ -981,6 +982,7 @@ extern "C" cvo_status cvo_align_multistart(cvo_ctx* ctx, const cvo_dcloud* src, 
   hipEvent_t e0 = ctx->ev_pool[4], e1 = ctx->ev_pool[5];
   CVO_HIP(ctx, hipEventRecord(e0, ctx->stream));
   KConsts kc = make_kconsts(p, src->F, src->K, MODE_ALIGN);
+  kc.mirror_pose = ctx->pose_arith == CVO_POSE_REFERENCE ? 1 : 0;
   kc.n_src_f = (float)src->n;
   kc.n_tgt_f = (float)tgt->n;
   kc.max_src_norm = src->max_norm;
@@ -1059,5 +1061,13 @@ extern "C" cvo_status cvo_align_multistart(cvo_ctx* ctx, const cvo_dcloud* src, 
     for (int s = 0; s < n_starts; ++s) per_start_cos[s] = cosv[s];
   if (per_start_iters)
     for (int s = 0; s < n_starts; ++s) per_start_iters[s] = its[s];
+  return CVO_OK;
+}
+
+extern "C" cvo_status cvo_set_pose_arithmetic(cvo_ctx* ctx, int32_t mode) {
+  if (!ctx) return CVO_ERR_INVALID_ARG;
+  if (mode != CVO_POSE_FP64 && mode != CVO_POSE_REFERENCE)
+    return set_err(ctx, CVO_ERR_INVALID_ARG, "pose arithmetic must be CVO_POSE_FP64 or CVO_POSE_REFERENCE");
+  ctx->pose_arith = mode;
   return CVO_OK;
 }

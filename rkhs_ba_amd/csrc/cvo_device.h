@@ -94,6 +94,36 @@ __device__ inline double se3_log_norm(const double R[9], const double t[3]) {
   return sqrt(s2);
 }
 
+// Exp_SEK3 of the reference in FP32 (LieGroup.cpp:244-275), including its
+// small-angle branch (J = I below 1e-6, so a pure translation step moves v
+// unscaled): out = [R | J v] row-major 3x4.  Used only when the library is
+// asked to reproduce the reference's float pose arithmetic.
+__device__ inline void exp_sek3_f32(const float xi[6], float dt, float out[12]) {
+  const float w0 = xi[0], w1 = xi[1], w2 = xi[2];
+  const float theta = sqrtf(w0 * w0 + w1 * w1 + w2 * w2);
+  float R[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, J[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+  if (!(theta < 1e-6f)) {
+    const float A[9] = {0, -w2, w1, w2, 0, -w0, -w1, w0, 0};
+    float A2[9];
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) A2[r * 3 + c] = A[r * 3 + 0] * A[0 * 3 + c] + A[r * 3 + 1] * A[1 * 3 + c] + A[r * 3 + 2] * A[2 * 3 + c];
+    const float theta2 = theta * theta;
+    const float st = sinf(dt * theta), ct = cosf(dt * theta);
+    const float omc = (1 - ct) / theta2;
+    const float k3 = (dt * theta - st) / (theta2 * theta);
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) {
+        const float I = r == c ? 1.0f : 0.0f;
+        R[r * 3 + c] = I + (st / theta) * A[r * 3 + c] + omc * A2[r * 3 + c];
+        J[r * 3 + c] = dt * I + omc * A[r * 3 + c] + k3 * A2[r * 3 + c];
+      }
+  }
+  for (int r = 0; r < 3; ++r) {
+    for (int c = 0; c < 3; ++c) out[r * 4 + c] = R[r * 3 + c];
+    out[r * 4 + 3] = J[r * 3 + 0] * xi[3] + J[r * 3 + 1] * xi[4] + J[r * 3 + 2] * xi[5];
+  }
+}
+
 __device__ inline int choose_level(const GridDev* grid, float r_need) {
   int L = 0;
   for (int l = 0; l < grid->n_levels; ++l)

@@ -128,6 +128,7 @@ struct KConsts {
   float rq_coef;              // sqrt(-2 lg) * 1.001
   float max_src_norm;
   int32_t mode;               // 0 align, 1 inner product, 2 BA
+  int32_t mirror_pose;        // 1: the reference's FP32 Exp_SEK3 and float pose
 };
 
 enum Mode { MODE_ALIGN = 0, MODE_IP = 1, MODE_BA = 2 };
@@ -182,6 +183,7 @@ struct cvo_ctx {
   DevBuf b_ba[16];
   std::vector<DevBuf> b_frames;  // per frame: sorted source (4) and grid (2)
   int32_t ba_last_accepted = 0, ba_last_evals = 0;
+  int32_t pose_arith = 0;  // CVO_POSE_FP64 or CVO_POSE_REFERENCE
   // model scratch
   DevBuf b_mlp[16];
 };

@@ -764,23 +764,46 @@ __device__ void controller(ProblemState* st, const KConsts& kc, const AssocProbl
     if (P.trace && k < P.trace_cap) P.trace[k] = rec;
     return;
   }
-  const double step = select_step_dev(kc, st->bcde);
-  st->step = step;
+  double step = select_step_dev(kc, st->bcde);
   double dR[9], dT[3];
-  se3_exp(xi, step, dR, dT);
-  double Rn[9], Tn[3];
-  for (int r = 0; r < 3; ++r) {
-    Tn[r] = st->T[r] + st->R[r * 3 + 0] * dT[0] + st->R[r * 3 + 1] * dT[1] + st->R[r * 3 + 2] * dT[2];
-    for (int c = 0; c < 3; ++c)
-      Rn[r * 3 + c] = st->R[r * 3 + 0] * dR[0 * 3 + c] + st->R[r * 3 + 1] * dR[1 * 3 + c] + st->R[r * 3 + 2] * dR[2 * 3 + c];
+  double dist;
+  if (kc.mirror_pose) {
+    // the reference's float pose arithmetic: float step, FP32 Exp_SEK3, the
+    // pose products in double rounded to float (CvoGPU.cu:1289-1341)
+    step = (double)(float)step;
+    float e[12];
+    exp_sek3_f32(xf, (float)step, e);
+    for (int r = 0; r < 3; ++r) {
+      for (int c = 0; c < 3; ++c) dR[r * 3 + c] = (double)e[r * 4 + c];
+      dT[r] = (double)e[r * 4 + 3];
+    }
+    double Rn[9], Tn[3];
+    for (int r = 0; r < 3; ++r) {
+      Tn[r] = (double)(float)(st->R[r * 3 + 0] * dT[0] + st->R[r * 3 + 1] * dT[1] + st->R[r * 3 + 2] * dT[2] + st->T[r]);
+      for (int c = 0; c < 3; ++c)
+        Rn[r * 3 + c] = (double)(float)(st->R[r * 3 + 0] * dR[0 * 3 + c] + st->R[r * 3 + 1] * dR[1 * 3 + c] +
+                                        st->R[r * 3 + 2] * dR[2 * 3 + c]);
+    }
+    for (int i = 0; i < 9; ++i) st->R[i] = Rn[i];
+    for (int i = 0; i < 3; ++i) st->T[i] = Tn[i];
+    dist = se3_log_norm(dR, dT);
+  } else {
+    se3_exp(xi, step, dR, dT);
+    double Rn[9], Tn[3];
+    for (int r = 0; r < 3; ++r) {
+      Tn[r] = st->T[r] + st->R[r * 3 + 0] * dT[0] + st->R[r * 3 + 1] * dT[1] + st->R[r * 3 + 2] * dT[2];
+      for (int c = 0; c < 3; ++c)
+        Rn[r * 3 + c] = st->R[r * 3 + 0] * dR[0 * 3 + c] + st->R[r * 3 + 1] * dR[1 * 3 + c] + st->R[r * 3 + 2] * dR[2 * 3 + c];
+    }
+    for (int i = 0; i < 9; ++i) st->R[i] = Rn[i];
+    for (int i = 0; i < 3; ++i) st->T[i] = Tn[i];
+    // ||log(exp(step xi))|| = step ||xi|| exactly while the rotation angle
+    // step*|w| stays below pi; the general log otherwise
+    const double wn = sqrt(xi[0] * xi[0] + xi[1] * xi[1] + xi[2] * xi[2]);
+    const double xn = sqrt(xi[0] * xi[0] + xi[1] * xi[1] + xi[2] * xi[2] + xi[3] * xi[3] + xi[4] * xi[4] + xi[5] * xi[5]);
+    dist = (step * wn < 3.0) ? step * xn : se3_log_norm(dR, dT);
   }
-  for (int i = 0; i < 9; ++i) st->R[i] = Rn[i];
-  for (int i = 0; i < 3; ++i) st->T[i] = Tn[i];
-  // ||log(exp(step xi))|| = step ||xi|| exactly while the rotation angle
-  // step*|w| stays below pi; the general log otherwise
-  const double wn = sqrt(xi[0] * xi[0] + xi[1] * xi[1] + xi[2] * xi[2]);
-  const double xn = sqrt(xi[0] * xi[0] + xi[1] * xi[1] + xi[2] * xi[2] + xi[3] * xi[3] + xi[4] * xi[4] + xi[5] * xi[5]);
-  const double dist = (step * wn < 3.0) ? step * xn : se3_log_norm(dR, dT);
+  st->step = step;
   st->dist = dist;
   rec.step = (float)step;
   rec.dist = (float)dist;
@@ -842,6 +865,15 @@ __device__ __forceinline__ void reduce_stage1(const AssocProblem& P, double (*s_
   __syncthreads();
 }
 
+// omega / c and v / d (compute_flow_gpu_no_eigen, CvoGPU.cu:2057-2060).
+__device__ __forceinline__ void scale_flow(double tot[10], const KConsts& kc) {
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    tot[c] /= (double)kc.c;
+    tot[3 + c] /= (double)kc.d;
+  }
+}
+
 // One thread per source row; grid (row blocks, problems).
 __global__ __launch_bounds__(kBlock) void step_kernel(const AssocProblem* __restrict__ probs, int n_probs,
                                                       KConsts kc) {
@@ -855,6 +887,7 @@ __global__ __launch_bounds__(kBlock) void step_kernel(const AssocProblem* __rest
   __shared__ double s_tree[10][128];
   double tot[10];
   reduce_stage1(P, s_tree, tot);
+  scale_flow(tot, kc);
   const float ell = st->ell;
   const GridLevel& G = P.grid->lv[st->level];
   Maps mp;
@@ -951,6 +984,7 @@ __global__ __launch_bounds__(kBlock) void controller_kernel(const AssocProblem* 
   __shared__ double s_tree[10][128];
   double tot[10];
   reduce_stage1(P, s_tree, tot);
+  scale_flow(tot, kc);
   // step partials: each thread sums a strided subset of blocks for all four
   // coefficients, then one fixed-order tree
   const int nsb = (P.n_src + kBlock - 1) / kBlock;

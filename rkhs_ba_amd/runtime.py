@@ -59,6 +59,7 @@ def lib() -> C.CDLL:
         L.cvo_align_d.argtypes = [P, P, P, P, P, P, P, P, P]
         L.cvo_align_multistart.argtypes = [P, P, P, i32, P, f, i32, P, P, P, P, P, P]
         L.cvo_align_multiframe.argtypes = [P, i32, P, P, P, i32, P, P, P, P, i32]
+        L.cvo_set_pose_arithmetic.argtypes = [P, i32]
         _lib = L
     return _lib
 
@@ -379,6 +380,11 @@ class CvoGPU:
         buf = (IterRecord * max(1, n.value))()
         _check(lib().cvo_get_trace(self._ctx, buf, n.value, C.byref(n)), self._ctx)
         return [buf[i] for i in range(n.value)]
+
+    def set_pose_arithmetic(self, mode: str) -> None:
+        """"fp64" (default): FP64 pose and SE(3) exponential; "reference": the
+        reference's float pose with its FP32 Exp_SEK3."""
+        _check(lib().cvo_set_pose_arithmetic(self._ctx, {"fp64": 0, "reference": 1}[mode]), self._ctx)
 
     def set_profiling(self, on: bool) -> None:
         _check(lib().cvo_set_profiling(self._ctx, int(bool(on))), self._ctx)

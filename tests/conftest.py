@@ -44,3 +44,11 @@ def rot_err_deg(A, B):
 
 def trans_err(A, B):
     return float(np.linalg.norm(np.asarray(A, np.float64)[:3, 3] - np.asarray(B, np.float64)[:3, 3]))
+
+
+def rot_dist_rad(A, B):
+    """Angle between two rotations from the Frobenius distance of the matrices
+    (2 asin(|RA - RB|_F / (2 sqrt 2))): linear near zero, so it resolves small
+    angles between float32 matrices, unlike the arccos of the trace."""
+    d = np.linalg.norm(np.asarray(A, np.float64)[:3, :3] - np.asarray(B, np.float64)[:3, :3])
+    return float(2.0 * np.arcsin(min(1.0, d / (2.0 * np.sqrt(2.0)))))
