@@ -88,6 +88,26 @@ def cpu_twoview(params, src, tgt, T_gt, trace, threads, budget_s=12.0):
     return total_est, sampled, per_iter, time.time() - t_start
 
 
+def tune(cvo, ds, dt, args, params=None):
+    """Association launch shape for this workload: the autotuner's choice
+    (cached per workload key), or the library default with --tune default."""
+    if args.tune == "default":
+        return cvo.get_tuning()
+    if params is None:
+        return cvo.autotune(ds, dt, np.eye(4), cache="default")
+    # a workload with its own lengthscale range (BA): tune on a context that
+    # carries that range, then apply the choice
+    from rkhs_ba_amd import CvoGPU
+    tc = CvoGPU(params=params)
+    try:
+        d1, d2 = tc.upload(ds), tc.upload(dt)
+        t = tc.autotune(d1, d2, np.eye(4), cache="default")
+    finally:
+        tc.close()
+    cvo.set_tuning(t["lanes_per_row"], t["xcd_remap"], t["cell_scale"])
+    return t
+
+
 def leg_twoview(args):
     from rkhs_ba_amd import CvoGPU, read_params_yaml, synth
     params = read_params_yaml(os.path.join(CONFIGS, "street_60k.yaml"))
@@ -96,6 +116,7 @@ def leg_twoview(args):
     cvo = CvoGPU(params=params)
     dsrc, dtgt = cvo.upload(src), cvo.upload(tgt)
     I = np.eye(4, dtype=np.float32)
+    tuning = tune(cvo, dsrc, dtgt, args)
     for w in range(args.warmup):
         log(f"[bench] two-view warmup {w}")
         cvo.align(dsrc, dtgt, I)
@@ -108,7 +129,8 @@ def leg_twoview(args):
     log(f"[bench] two-view {wall / args.steps * 1e3:.1f} ms/registration, {iters} iters, "
         f"err {rot_e:.4f} deg {tr_e * 100:.2f} cm")
     out = {"value": args.steps / wall, "ms_per_step": wall / args.steps * 1e3, "iterations": iters,
-           "device_loop_ms": dev_s * 1e3, "pose_error": {"deg": rot_e, "m": tr_e}, "params": params}
+           "device_loop_ms": dev_s * 1e3, "pose_error": {"deg": rot_e, "m": tr_e}, "params": params,
+           "tuning": tuning}
     if args.profile_only:
         return out
     log("[bench] two-view profiled registration")
@@ -178,6 +200,12 @@ def leg_ba(args):
     cvo = CvoGPU(params=params)
     edges = window_edges(cvo, clouds, inits, params.multiframe_ell_init)
     hold = [1] + [0] * 11
+    # tuned on the window's largest edge at the BA lengthscale range
+    bp = params.copy()
+    bp.ell_init, bp.ell_min = params.multiframe_ell_init, params.multiframe_ell_min
+    bp.nearest_neighbors_max = params.multiframe_num_neighbors
+    i0, j0 = max(edges, key=lambda e: clouds[e[0]].n * clouds[e[1]].n)
+    tuning = tune(cvo, clouds[i0], clouds[j0], args, params=bp)
     dev = [cvo.upload(c) for c in clouds]
     log(f"[bench] BA window: {len(edges)} edges, {sum(c.n for c in clouds)} points; warmup")
     cvo.align_multiframe(dev, inits, hold, edges)
@@ -193,7 +221,7 @@ def leg_ba(args):
     out = {"s_per_iter": s_iter, "first_iter_s": float(per[0]), "iterations": r["iters"], "wall_s": wall,
            "frames": len(clouds), "edges": len(edges), "points": int(sum(c.n for c in clouds)),
            "max_error_before": {"deg": float(e0[:, 0].max()), "m": float(e0[:, 1].max())},
-           "max_error_after": {"deg": float(e1[:, 0].max()), "m": float(e1[:, 1].max())}}
+           "max_error_after": {"deg": float(e1[:, 0].max()), "m": float(e1[:, 1].max())}, "tuning": tuning}
     if args.profile_only:
         return out
     log("[bench] BA window profiled run")
@@ -249,6 +277,7 @@ def leg_multistart(args):
     Ts = np.array([synth.rigid(R, cy - R @ cx) for R in synth.so3_grid(96)], np.float32)
     cvo = CvoGPU(params=params)
     ds, dt = cvo.upload(src), cvo.upload(tgt)
+    tuning = tune(cvo, ds, dt, args)
     cvo.align_multistart(ds, dt, Ts, 0.3, mode=0)
     t0 = time.perf_counter()
     r = cvo.align_multistart(ds, dt, Ts, 0.3, mode=0)
@@ -260,7 +289,7 @@ def leg_multistart(args):
     log(f"[bench] multi-start {wall:.2f} s, best err {e[0]:.4f} deg {e[1] * 100:.2f} cm, "
         f"{per_ok}/96 starts converge")
     out = {"seconds": wall, "starts_per_s": 96 / wall, "success": success,
-           "best_error": {"deg": e[0], "m": e[1]}, "starts_converged": per_ok,
+           "best_error": {"deg": e[0], "m": e[1]}, "starts_converged": per_ok, "tuning": tuning,
            "iterations_total": int(r["per_start_iters"].sum())}
     if not args.no_cpu and not args.profile_only:
         from oracle import pyoracle
@@ -351,6 +380,8 @@ def main():
     ap.add_argument("--legs", default="twoview,ba,multistart,learned")
     ap.add_argument("--learned-pairs", type=int, default=4, help="pairs for learned vs grid registration")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
+    ap.add_argument("--tune", choices=("auto", "default"), default="auto",
+                    help="association launch shape: autotuned (cached per workload) or the library default")
     ap.add_argument("--profile-only", action="store_true", help="timed loops only (for rocprofv3)")
     ap.add_argument("--traffic", type=float, default=ASSOC_TRAFFIC_BYTES,
                     help="HBM bytes per association launch from rocprofv3 counters")
@@ -402,6 +433,7 @@ def main():
         "device_loop_ms": tv["device_loop_ms"],
         "pose_error": tv["pose_error"],
         "roofline": tv.get("roofline"),
+        "tuning": tv.get("tuning"),
         "kernels_ms_per_launch": tv.get("kernels_ms_per_launch"),
         "cpu_baseline": cpu,
         "speedup_vs_cpu": tv["value"] / cpu["value"] if cpu and cpu["value"] else None,

@@ -310,6 +310,55 @@ cvo_status cvo_gemm_bf16(cvo_ctx* ctx, int32_t M, int32_t N, int32_t K, const fl
 enum { CVO_POSE_FP64 = 0, CVO_POSE_REFERENCE = 1 };
 cvo_status cvo_set_pose_arithmetic(cvo_ctx* ctx, int32_t mode);
 
+/* ---- association-kernel tuning (no counterpart in the reference) ----------
+ *
+ * The reference's association kernel has one fixed shape (one thread per
+ * source row over every target, CvoGPU.cu:951-1099); here the grid search has
+ * three knobs, none of which changes the pair set, the pair values or the row
+ * cap semantics (only the order pairs are summed in, at FP64 rounding level):
+ *   lanes_per_row  8 or 16 lanes of a wave share one source row;
+ *   xcd_remap      1: each of the eight XCDs takes one contiguous range of
+ *                  the spatially sorted row blocks (its own L2 serves the
+ *                  overlapping target neighbourhoods), 0: dispatch order;
+ *   cell_scale     grid cells are chosen with edge >= cell_scale x the ell-ball
+ *                  radius, in [0.4, 1] (smaller: more cells per query box,
+ *                  fewer candidates outside the ball).
+ * The tuning of a context applies to every later align / multi-start / BA /
+ * inner-product call on it. */
+typedef struct {
+  int32_t lanes_per_row;
+  int32_t xcd_remap;
+  float cell_scale;
+  int32_t source;   /* out: CVO_TUNE_DEFAULT / _SET / _MEASURED / _CACHED */
+  float score_us;   /* out: weighted association time of the chosen setting */
+  int32_t n_tried;  /* out: settings measured (0 when read from the cache) */
+} cvo_tuning;
+enum { CVO_TUNE_DEFAULT = 0, CVO_TUNE_SET = 1, CVO_TUNE_MEASURED = 2, CVO_TUNE_CACHED = 3 };
+
+cvo_status cvo_get_tuning(const cvo_ctx* ctx, cvo_tuning* out);
+/* CVO_ERR_INVALID_ARG for lanes_per_row not in {8, 16}, xcd_remap not in
+ * {0, 1} or cell_scale outside [0.4, 1]. */
+cvo_status cvo_set_tuning(cvo_ctx* ctx, const cvo_tuning* t);
+/* Measures the association pass of this source / target pair at T_init for
+ * every setting (2 x 2 x 3) at three lengthscales (ell_init, their geometric
+ * mean, ell_min; weights 1, 1, 2), sets the fastest on the context and returns
+ * it.  With cache_path non-null the result is looked up first under the
+ * workload key (cvo_tuning_key) and stored after a measurement (a JSON file,
+ * replaced atomically).  The measurement takes well under a second at 60k
+ * points. */
+cvo_status cvo_autotune(cvo_ctx* ctx, const cvo_cloud_view* src, const cvo_cloud_view* tgt, const float T_init[16],
+                        const char* cache_path, cvo_tuning* out);
+cvo_status cvo_autotune_d(cvo_ctx* ctx, const cvo_dcloud* src, const cvo_dcloud* tgt, const float T_init[16],
+                          const char* cache_path, cvo_tuning* out);
+/* The cache key of a workload on this context's device: architecture, CU
+ * count, half-octave size classes of both clouds, feature / class dimensions,
+ * enabled gates, lengthscale range, sparsity threshold and row cap. */
+cvo_status cvo_tuning_key(const cvo_ctx* ctx, const cvo_dcloud* src, const cvo_dcloud* tgt, char* buf, int32_t len);
+/* Cache file access without a device: lookup returns CVO_ERR_IO when the
+ * file or the key is absent; store creates the file (and its directory). */
+cvo_status cvo_tuning_cache_lookup(const char* path, const char* key, cvo_tuning* out);
+cvo_status cvo_tuning_cache_store(const char* path, const char* key, const cvo_tuning* t);
+
 /* ---- profiling ----------------------------------------------------------- */
 
 cvo_status cvo_set_profiling(cvo_ctx* ctx, int32_t on);

@@ -134,6 +134,25 @@ class KernelStats(C.Structure):
 KSTAT_NAMES = ["assoc", "overflow", "step", "grid", "ba_pair", "ba_solve", "mlp", "spare"]
 
 
+class Tuning(C.Structure):
+    """cvo_tuning: launch shape of the association kernel."""
+    _fields_ = [
+        ("lanes_per_row", C.c_int32),
+        ("xcd_remap", C.c_int32),
+        ("cell_scale", C.c_float),
+        ("source", C.c_int32),
+        ("score_us", C.c_float),
+        ("n_tried", C.c_int32),
+    ]
+
+    SOURCES = {0: "default", 1: "set", 2: "measured", 3: "cached"}
+
+    def as_dict(self) -> dict:
+        return {"lanes_per_row": self.lanes_per_row, "xcd_remap": self.xcd_remap,
+                "cell_scale": round(float(self.cell_scale), 4), "source": self.SOURCES.get(self.source, self.source),
+                "score_us": round(float(self.score_us), 3), "n_tried": self.n_tried}
+
+
 def fptr(a):
     if a is None:
         return C.POINTER(C.c_float)()

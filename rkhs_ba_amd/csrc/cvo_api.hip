@@ -191,7 +191,7 @@ cvo_status prepare_source_into(cvo_ctx* ctx, const cvo_dcloud* s, DevBuf& b_pos,
   return CVO_OK;
 }
 
-static cvo_status prepare_source(cvo_ctx* ctx, const cvo_dcloud* s) {
+cvo_status prepare_source(cvo_ctx* ctx, const cvo_dcloud* s) {
   return prepare_source_into(ctx, s, ctx->b_src_pos, ctx->b_src_feat, ctx->b_src_sem, ctx->b_src_geo);
 }
 
@@ -278,17 +278,19 @@ cvo_status build_grid_into(cvo_ctx* ctx, const cvo_dcloud* t, const std::vector<
   return CVO_OK;
 }
 
-static cvo_status build_grid(cvo_ctx* ctx, const cvo_dcloud* t, const std::vector<float>& hs, GridHost& gh) {
+cvo_status build_grid(cvo_ctx* ctx, const cvo_dcloud* t, const std::vector<float>& hs, GridHost& gh) {
   return build_grid_into(ctx, t, hs, gh, ctx->b_grid_store, ctx->b_gridhdr);
 }
 
-// Cell sizes for lengthscales from ell_hi down to ell_lo, ratio 0.7.
-std::vector<float> level_sizes(const KConsts& kc, float ell_hi, float ell_lo, float max_norm, float tq) {
+// Cell sizes for lengthscales from ell_hi down to ell_lo, ratio 0.7, the finest
+// reaching cell_scale x the radius at ell_lo.
+std::vector<float> level_sizes(const KConsts& kc, float ell_hi, float ell_lo, float max_norm, float tq,
+                               float cell_scale) {
   std::vector<float> hs;
   // without the geometric gate every row takes the full scan; one coarse level
   if (!kc.use_geo) return {1.0e4f};
   float r_hi = radius_for(kc, ell_hi, max_norm, tq);
-  float r_lo = radius_for(kc, ell_lo, max_norm, tq);
+  float r_lo = radius_for(kc, ell_lo, max_norm, tq) * cell_scale;
   if (!(r_hi > 0)) r_hi = 1.0f;
   float h = r_hi * 1.02f;
   for (int l = 0; l < kMaxLevels; ++l) {
@@ -299,22 +301,20 @@ std::vector<float> level_sizes(const KConsts& kc, float ell_hi, float ell_lo, fl
   return hs;
 }
 
-struct Problems {
-  int n_probs = 0;
-  int total_blocks = 0;
-  int max_rows = 0;  // largest source row count over the problems
-  AssocProblem* dev = nullptr;
-  std::vector<AssocProblem> host;
-  ProblemState* states = nullptr;
-  cvo_iter_record* trace = nullptr;
-  int trace_cap = 0;
-};
+// The context's association launch shape.
+void apply_tuning(const cvo_ctx* ctx, KConsts& kc) {
+  kc.group = ctx->tuning.lanes_per_row;
+  kc.xcd_remap = ctx->tuning.xcd_remap;
+  kc.cell_scale = ctx->tuning.cell_scale;
+}
 
-// Buffers and descriptors of n_probs problems sharing one source and target.
-static cvo_status setup_problems(cvo_ctx* ctx, const cvo_dcloud* s, const cvo_dcloud* t, const GridHost& gh,
-                                 int n_probs, int cap_alloc, int trace_cap, Problems& pr) {
+// Buffers and descriptors of n_probs problems sharing one source and target;
+// group = lanes per row of the association launch.
+cvo_status setup_problems(cvo_ctx* ctx, const cvo_dcloud* s, const cvo_dcloud* t, const GridHost& gh, int n_probs,
+                          int cap_alloc, int trace_cap, int group, Problems& pr) {
   const int n = s->n;
-  const int nb = (n + kRowsPerBlock - 1) / kRowsPerBlock;
+  const int rb = rows_per_block(group);
+  const int nb = (n + rb - 1) / rb;
   pr.n_probs = n_probs;
   pr.total_blocks = nb * n_probs;
   pr.max_rows = n;
@@ -789,6 +789,7 @@ extern "C" cvo_status cvo_align_d(cvo_ctx* ctx, const cvo_dcloud* src, const cvo
   CVO_TRY(validate_pair(ctx, src, tgt));
   const cvo_params& p = ctx->params;
   KConsts kc = make_kconsts(p, src->F, src->K, MODE_ALIGN);
+  apply_tuning(ctx, kc);
   kc.mirror_pose = ctx->pose_arith == CVO_POSE_REFERENCE ? 1 : 0;
   kc.n_src_f = (float)src->n;
   kc.n_tgt_f = (float)tgt->n;
@@ -797,11 +798,11 @@ extern "C" cvo_status cvo_align_d(cvo_ctx* ctx, const cvo_dcloud* src, const cvo
   float tq = std::max(std::fabs(T_init[3]), std::max(std::fabs(T_init[7]), std::fabs(T_init[11])));
   GridHost gh;
   float ell_lo = std::min(p.ell_min, p.ell_init);
-  CVO_TRY(build_grid(ctx, tgt, level_sizes(kc, std::max(p.ell_init, p.ell_min), ell_lo, src->max_norm, tq), gh));
+  CVO_TRY(build_grid(ctx, tgt, level_sizes(kc, std::max(p.ell_init, p.ell_min), ell_lo, src->max_norm, tq, kc.cell_scale), gh));
   const bool want_trace = ctx->trace_cap > 0 || ctx->profiling;
   const int trace_cap = want_trace ? std::max(ctx->trace_cap, ctx->profiling ? p.MAX_ITER : 0) : 0;
   Problems pr;
-  CVO_TRY(setup_problems(ctx, src, tgt, gh, 1, p.nearest_neighbors_max, trace_cap, pr));
+  CVO_TRY(setup_problems(ctx, src, tgt, gh, 1, p.nearest_neighbors_max, trace_cap, kc.group, pr));
   CVO_TRY(ensure(ctx, ctx->b_tmp2, 64 * sizeof(float)));
   CVO_HIP(ctx, hipMemcpyAsync(ctx->b_tmp2.p, T_init, sizeof(float) * 16, hipMemcpyHostToDevice, ctx->stream));
   launch_init_state(pr.dev, 1, as<float>(ctx->b_tmp2), 16, kc, p.ell_init, p.nearest_neighbors_max, ctx->stream);
@@ -861,6 +862,7 @@ static cvo_status single_pass(cvo_ctx* ctx, const cvo_dcloud* src, const cvo_dcl
   CVO_TRY(validate_pair(ctx, src, tgt));
   const cvo_params& p = ctx->params;
   kc = make_kconsts(p, src->F, src->K, MODE_IP);
+  apply_tuning(ctx, kc);
   kc.n_src_f = (float)src->n;
   kc.n_tgt_f = (float)tgt->n;
   kc.max_src_norm = src->max_norm;
@@ -868,9 +870,9 @@ static cvo_status single_pass(cvo_ctx* ctx, const cvo_dcloud* src, const cvo_dcl
   CVO_TRY(prepare_source(ctx, src));
   float tq = std::max(std::fabs(T[3]), std::max(std::fabs(T[7]), std::fabs(T[11])));
   GridHost gh;
-  std::vector<float> hs = level_sizes(kc, ell, ell, src->max_norm, tq);
+  std::vector<float> hs = level_sizes(kc, ell, ell, src->max_norm, tq, kc.cell_scale);
   CVO_TRY(build_grid(ctx, tgt, hs, gh));
-  CVO_TRY(setup_problems(ctx, src, tgt, gh, 1, p.nearest_neighbors_max, 0, pr));
+  CVO_TRY(setup_problems(ctx, src, tgt, gh, 1, p.nearest_neighbors_max, 0, kc.group, pr));
   CVO_TRY(ensure(ctx, ctx->b_tmp2, 64 * sizeof(float)));
   CVO_HIP(ctx, hipMemcpyAsync(ctx->b_tmp2.p, T, sizeof(float) * 16, hipMemcpyHostToDevice, ctx->stream));
   launch_init_state(pr.dev, 1, as<float>(ctx->b_tmp2), 16, kc, ell, p.nearest_neighbors_max, ctx->stream);
@@ -982,6 +984,7 @@ extern "C" cvo_status cvo_align_multistart(cvo_ctx* ctx, const cvo_dcloud* src, 
   hipEvent_t e0 = ctx->ev_pool[4], e1 = ctx->ev_pool[5];
   CVO_HIP(ctx, hipEventRecord(e0, ctx->stream));
   KConsts kc = make_kconsts(p, src->F, src->K, MODE_ALIGN);
+  apply_tuning(ctx, kc);
   kc.mirror_pose = ctx->pose_arith == CVO_POSE_REFERENCE ? 1 : 0;
   kc.n_src_f = (float)src->n;
   kc.n_tgt_f = (float)tgt->n;
@@ -996,10 +999,10 @@ extern "C" cvo_status cvo_align_multistart(cvo_ctx* ctx, const cvo_dcloud* src, 
   const float ell_hi = std::max(std::max(p.ell_init, p.ell_min), score_ell);
   const float ell_lo = std::min(std::min(p.ell_min, p.ell_init), score_ell);
   GridHost gh;
-  CVO_TRY(build_grid(ctx, tgt, level_sizes(kc, ell_hi, ell_lo, src->max_norm, tq), gh));
+  CVO_TRY(build_grid(ctx, tgt, level_sizes(kc, ell_hi, ell_lo, src->max_norm, tq, kc.cell_scale), gh));
   const int n_run = (mode == 1) ? n_starts : n_starts;
   Problems pr;
-  CVO_TRY(setup_problems(ctx, src, tgt, gh, n_run, p.nearest_neighbors_max, 0, pr));
+  CVO_TRY(setup_problems(ctx, src, tgt, gh, n_run, p.nearest_neighbors_max, 0, kc.group, pr));
   CVO_TRY(ensure(ctx, ctx->b_tmp2, sizeof(float) * 16 * (size_t)n_starts + 64));
   CVO_HIP(ctx, hipMemcpyAsync(ctx->b_tmp2.p, T_starts, sizeof(float) * 16 * n_starts, hipMemcpyHostToDevice,
                               ctx->stream));

@@ -69,7 +69,7 @@ struct BaDev {
   int32_t lm_iter, lm_stop, eval_pending, eval_is_candidate;
   int32_t K_init, min_nonzeros, iter_per_ell, counter_nonzero;
   int32_t stopped_no_edges, accepted_total, evals_total, h_cur;
-  float ell_min_f, decay_f, rq_coef, pad1;
+  float ell_min_f, decay_f, rq_coef, cell_scale;
   double ell_min, ell_max;
   double F, Fn, mu, v, change;
   int32_t free_idx[kBaMaxFrames];
@@ -170,7 +170,7 @@ __global__ __launch_bounds__(256) void ba_prepare_kernel(BaDev* __restrict__ ba,
     const float tq = fmaxf(fabsf(st->Tq[0]), fmaxf(fabsf(st->Tq[1]), fabsf(st->Tq[2])));
     const float r_need =
         ba->rq_coef * st->ell * (1.0f + sn / 500.0f) * 1.0001f + 1e-4f * (1.0f + sn + tq) + 1e-6f;
-    st->level = choose_level(probs[e].grid, r_need);
+    st->level = choose_level(probs[e].grid, r_need * ba->cell_scale);
   }
 }
 
@@ -756,6 +756,8 @@ extern "C" cvo_status cvo_align_multiframe(cvo_ctx* ctx, int32_t n_frames, const
   hipStream_t st = ctx->stream;
   const auto t_start = std::chrono::steady_clock::now();
   KConsts kc = make_kconsts(p, frames[0]->F, frames[0]->K, MODE_BA);
+  apply_tuning(ctx, kc);
+  const int rb = rows_per_block(kc.group);
 
   // frames that are sources / targets of some edge
   std::vector<char> is_src(n_frames, 0), is_tgt(n_frames, 0);
@@ -774,7 +776,7 @@ extern "C" cvo_status cvo_align_multiframe(cvo_ctx* ctx, int32_t n_frames, const
   std::vector<GridHost> grids(n_frames);
   const float ell_hi = std::max(p.multiframe_ell_init * 1.2f, p.multiframe_ell_min);
   const float ell_lo = std::min(p.multiframe_ell_min, p.multiframe_ell_init);
-  const std::vector<float> hs = level_sizes(kc, ell_hi, ell_lo, max_world * 1.05f, max_t * 1.05f + 1.0f);
+  const std::vector<float> hs = level_sizes(kc, ell_hi, ell_lo, max_world * 1.05f, max_t * 1.05f + 1.0f, kc.cell_scale);
   for (int f = 0; f < n_frames; ++f) {
     DevBuf* fb = &ctx->b_frames[(size_t)kFrameBufs * f];
     if (is_src[f]) CVO_TRY(prepare_source_into(ctx, frames[f], fb[0], fb[1], fb[2], fb[3]));
@@ -789,7 +791,7 @@ extern "C" cvo_status cvo_align_multiframe(cvo_ctx* ctx, int32_t n_frames, const
   for (int e = 0; e < n_edges; ++e) {
     const int n = frames[edges[2 * e]]->n;
     row_off[e + 1] = row_off[e] + (size_t)n;
-    total_blocks += (n + kRowsPerBlock - 1) / kRowsPerBlock;
+    total_blocks += (n + rb - 1) / rb;
     max_pair_blocks = std::max(max_pair_blocks, (n + kBaRowsPerBlock - 1) / kBaRowsPerBlock);
   }
   tot_rows = row_off[n_edges];
@@ -820,7 +822,7 @@ extern "C" cvo_status cvo_align_multiframe(cvo_ctx* ctx, int32_t n_frames, const
     const cvo_dcloud* t = frames[f2];
     DevBuf* sb = &ctx->b_frames[(size_t)kFrameBufs * f1];
     const int n = s->n;
-    const int nb = (n + kRowsPerBlock - 1) / kRowsPerBlock;
+    const int nb = (n + rb - 1) / rb;
     AssocProblem& P = hp[e];
     std::memset(&P, 0, sizeof(P));
     P.src_pos = as<float4>(sb[0]);
@@ -874,6 +876,7 @@ extern "C" cvo_status cvo_align_multiframe(cvo_ctx* ctx, int32_t n_frames, const
   hb->ell_min_f = p.multiframe_ell_min;
   hb->decay_f = p.multiframe_ell_decay_rate;
   hb->rq_coef = kc.rq_coef;
+  hb->cell_scale = kc.cell_scale;
   hb->ell_min = (double)p.multiframe_ell_min;
   hb->ell_max = (double)p.multiframe_ell_init * 1.2;
   int o = 0;

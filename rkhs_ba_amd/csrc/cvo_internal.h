@@ -13,9 +13,7 @@ namespace cvo {
 
 constexpr int kMaxLevels = 12;         // grid levels per target
 constexpr int kMaxWindow = 64;         // indicator window capacity
-constexpr int kRowsPerBlock = 16;      // assoc / step: rows per 256-thread block
-constexpr int kGroup = 16;             // lanes per row
-constexpr int kBlock = 256;
+constexpr int kBlock = 256;            // threads per block of the row kernels
 constexpr int kOvfBlocks = 96;         // capped-row kernel blocks per problem
 constexpr uint32_t kMaxCells = 1u << 25;
 
@@ -129,7 +127,14 @@ struct KConsts {
   float max_src_norm;
   int32_t mode;               // 0 align, 1 inner product, 2 BA
   int32_t mirror_pose;        // 1: the reference's FP32 Exp_SEK3 and float pose
+  // association launch shape (none of these changes the pair set)
+  int32_t group;              // lanes per source row: 8 or 16
+  int32_t xcd_remap;          // 1: contiguous row-block ranges per XCD
+  float cell_scale;           // level choice: cell edge >= cell_scale * radius
 };
+
+// Rows per association block for a lane count per row.
+__host__ __device__ constexpr int rows_per_block(int group) { return kBlock / group; }
 
 enum Mode { MODE_ALIGN = 0, MODE_IP = 1, MODE_BA = 2 };
 
@@ -184,6 +189,7 @@ struct cvo_ctx {
   std::vector<DevBuf> b_frames;  // per frame: sorted source (4) and grid (2)
   int32_t ba_last_accepted = 0, ba_last_evals = 0;
   int32_t pose_arith = 0;  // CVO_POSE_FP64 or CVO_POSE_REFERENCE
+  cvo_tuning tuning{16, 0, 1.0f, CVO_TUNE_DEFAULT, 0.0f, 0};
   // model scratch
   DevBuf b_mlp[16];
 };
@@ -200,7 +206,9 @@ cvo_status ensure(cvo_ctx* ctx, DevBuf& b, size_t bytes);
 KConsts make_kconsts(const cvo_params& p, int F, int Ks, int mode);
 cvo_status validate_pair(cvo_ctx* ctx, const cvo_dcloud* s, const cvo_dcloud* t);
 float radius_for(const KConsts& kc, float ell, float max_norm, float tq);
-std::vector<float> level_sizes(const KConsts& kc, float ell_hi, float ell_lo, float max_norm, float tq);
+std::vector<float> level_sizes(const KConsts& kc, float ell_hi, float ell_lo, float max_norm, float tq,
+                               float cell_scale);
+void apply_tuning(const cvo_ctx* ctx, KConsts& kc);
 cvo_status ensure_events(cvo_ctx* ctx, size_t n);
 cvo_status pinned(cvo_ctx* ctx, size_t bytes);
 cvo_status prepare_source_into(cvo_ctx* ctx, const cvo_dcloud* s, DevBuf& b_pos, DevBuf& b_feat, DevBuf& b_sem,
@@ -210,6 +218,24 @@ cvo_status build_grid_into(cvo_ctx* ctx, const cvo_dcloud* t, const std::vector<
 void launch_assoc(const AssocProblem* probs, int n_probs, int total_blocks, const KConsts& kc, hipStream_t s);
 void launch_overflow(const AssocProblem* probs, int n_probs, const KConsts& kc, hipStream_t s);
 void launch_pass_totals(const AssocProblem* probs, int n_probs, hipStream_t s);
+void launch_init_state(const AssocProblem* probs, int n_probs, const float* T16, int stride16, const KConsts& kc,
+                       float ell0, int K0, hipStream_t s);
+cvo_status prepare_source(cvo_ctx* ctx, const cvo_dcloud* s);
+cvo_status build_grid(cvo_ctx* ctx, const cvo_dcloud* t, const std::vector<float>& hs, GridHost& gh);
+
+// Descriptors of n_probs association problems sharing one source and target.
+struct Problems {
+  int n_probs = 0;
+  int total_blocks = 0;
+  int max_rows = 0;  // largest source row count over the problems
+  AssocProblem* dev = nullptr;
+  std::vector<AssocProblem> host;
+  ProblemState* states = nullptr;
+  cvo_iter_record* trace = nullptr;
+  int trace_cap = 0;
+};
+cvo_status setup_problems(cvo_ctx* ctx, const cvo_dcloud* s, const cvo_dcloud* t, const GridHost& gh, int n_probs,
+                          int cap_alloc, int trace_cap, int group, Problems& pr);
 }  // namespace cvo
 
 #define CVO_HIP(ctx, call)                                              \

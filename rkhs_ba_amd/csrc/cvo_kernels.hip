@@ -11,9 +11,9 @@
 //   step_kernel                   : Taylor step terms (:333-462) + device
 //                                   controller (:464-544, :1289-1433)
 //
-// Rows are processed by 16-lane groups (four rows per wave64, sixteen per
-// 256-thread block).  Reductions are block partials summed in fixed order by
-// the last block, so results are bitwise reproducible run to run.
+// Rows are processed by groups of G = 8 or 16 lanes (256 / G rows per
+// 256-thread block).  Reductions are block partials summed in fixed order, so
+// results are bitwise reproducible run to run.
 
 #include <hip/hip_runtime.h>
 #include <math.h>
@@ -28,12 +28,15 @@ namespace cvo {
 // small helpers
 // ---------------------------------------------------------------------------
 
-template <typename T>
-__device__ __forceinline__ T shfl16(T v, int src) {
-  return __shfl(v, src, 16);
+// Workgroups are dispatched to the eight XCDs round robin by id (XCD = id % 8).
+// This maps a dispatch id to a logical block so that every XCD takes one
+// contiguous range of logical blocks: spatially sorted rows whose overlapping
+// target neighbourhoods its own L2 then serves.  A bijection of [0, nblk).
+__device__ __forceinline__ int xcd_block(int bid, int nblk) {
+  const int q = nblk >> 3, r = nblk & 7;
+  const int x = bid & 7, k = bid >> 3;
+  return (x < r) ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
 }
-
-__device__ __forceinline__ double shfl_xor_d(double v, int m) { return __shfl_xor(v, m, 16); }
 
 // Exact evaluation of a pair's kernel value with FP64 exponentials rounded to
 // float, the reference's rounding points (CvoGPU.cu:1020-1054).  Used only for
@@ -245,12 +248,12 @@ struct RowResult {
   bool overflow;
 };
 
-// One row handled by a 16-lane group.  FC/KC: color/semantic dimension when the
+// One row handled by a group of G lanes.  FC/KC: color/semantic dimension when the
 // gate is on (0 off, -1 runtime).  MODE_ALIGN accumulates the flow, MODE_IP and
 // MODE_BA only the sum of a.
-template <int FC, int KC>
+template <int FC, int KC, int G>
 __device__ __forceinline__ void assoc_row(const AssocProblem& P, const KConsts& kc, const ExactK& ek,
-                                          const GridLevel& G, const Maps& mp, int r, int g, int mode, float ell,
+                                          const GridLevel& GL, const Maps& mp, int r, int g, int mode, float ell,
                                           int K, const float* xf, const float* xs, RowResult& out) {
 #pragma unroll
   for (int c = 0; c < 7; ++c) out.acc[c] = 0.0;
@@ -285,61 +288,62 @@ __device__ __forceinline__ void assoc_row(const AssocProblem& P, const KConsts& 
   float q0, q1, q2;
   affine3(mp.Rq, mp.Tq, x0, x1, x2, q0, q1, q2);
   const float rq = sqrtf(d2t) * 1.001f + 1e-4f * (1.0f + fmaxf(fabsf(q0), fmaxf(fabsf(q1), fabsf(q2))));
-  int lo0 = cell_floor((q0 - rq - G.ox) * G.inv_h, G.nx), hi0 = cell_floor((q0 + rq - G.ox) * G.inv_h, G.nx);
-  int lo1 = cell_floor((q1 - rq - G.oy) * G.inv_h, G.ny), hi1 = cell_floor((q1 + rq - G.oy) * G.inv_h, G.ny);
-  int lo2 = cell_floor((q2 - rq - G.oz) * G.inv_h, G.nz), hi2 = cell_floor((q2 + rq - G.oz) * G.inv_h, G.nz);
+  int lo0 = cell_floor((q0 - rq - GL.ox) * GL.inv_h, GL.nx), hi0 = cell_floor((q0 + rq - GL.ox) * GL.inv_h, GL.nx);
+  int lo1 = cell_floor((q1 - rq - GL.oy) * GL.inv_h, GL.ny), hi1 = cell_floor((q1 + rq - GL.oy) * GL.inv_h, GL.ny);
+  int lo2 = cell_floor((q2 - rq - GL.oz) * GL.inv_h, GL.nz), hi2 = cell_floor((q2 + rq - GL.oz) * GL.inv_h, GL.nz);
   int nruns = 0, span1 = 0;
-  if (!(hi0 < 0 || lo0 >= G.nx || hi1 < 0 || lo1 >= G.ny || hi2 < 0 || lo2 >= G.nz)) {
-    lo0 = max(lo0, 0); hi0 = min(hi0, G.nx - 1);
-    lo1 = max(lo1, 0); hi1 = min(hi1, G.ny - 1);
-    lo2 = max(lo2, 0); hi2 = min(hi2, G.nz - 1);
+  if (!(hi0 < 0 || lo0 >= GL.nx || hi1 < 0 || lo1 >= GL.ny || hi2 < 0 || lo2 >= GL.nz)) {
+    lo0 = max(lo0, 0); hi0 = min(hi0, GL.nx - 1);
+    lo1 = max(lo1, 0); hi1 = min(hi1, GL.ny - 1);
+    lo2 = max(lo2, 0); hi2 = min(hi2, GL.nz - 1);
     span1 = hi1 - lo1 + 1;
     nruns = span1 * (hi2 - lo2 + 1);
   }
   uint32_t stored = 0;
   bool stop = false;
-  for (int rbase = 0; rbase < nruns && !stop; rbase += kGroup) {
+  for (int rbase = 0; rbase < nruns && !stop; rbase += G) {
     const int myrun = rbase + g;
     uint32_t rs = 0, rl = 0;
     if (myrun < nruns) {
       const int iz = myrun / span1, iy = myrun - iz * span1;
-      const uint32_t c0 = ((uint32_t)(lo2 + iz) * (uint32_t)G.ny + (uint32_t)(lo1 + iy)) * (uint32_t)G.nx + (uint32_t)lo0;
-      rs = G.cell_start[c0];
-      rl = G.cell_start[c0 + (uint32_t)(hi0 - lo0) + 1u] - rs;
+      const uint32_t c0 = ((uint32_t)(lo2 + iz) * (uint32_t)GL.ny + (uint32_t)(lo1 + iy)) * (uint32_t)GL.nx + (uint32_t)lo0;
+      rs = GL.cell_start[c0];
+      rl = GL.cell_start[c0 + (uint32_t)(hi0 - lo0) + 1u] - rs;
     }
     uint32_t incl = rl;
 #pragma unroll
-    for (int o = 1; o < kGroup; o <<= 1) {
-      uint32_t t = __shfl_up(incl, o, 16);
+    for (int o = 1; o < G; o <<= 1) {
+      uint32_t t = __shfl_up(incl, o, G);
       if (g >= o) incl += t;
     }
-    const uint32_t total = shfl16(incl, 15);
+    const uint32_t total = __shfl(incl, G - 1, G);
     const uint32_t excl = incl - rl;
-    for (uint32_t base = 0; base < total; base += kGroup) {
+    for (uint32_t base = 0; base < total; base += G) {
       const uint32_t idx = base + (uint32_t)g;
+      // the run holding candidate idx: the last run whose prefix is <= idx
       int kk = 0;
 #pragma unroll
-      for (int s = 8; s > 0; s >>= 1) {
-        uint32_t e = shfl16(excl, kk + s);
+      for (int s = G / 2; s > 0; s >>= 1) {
+        uint32_t e = __shfl(excl, kk + s, G);
         if (e <= idx) kk += s;
       }
-      const uint32_t run_start = shfl16(rs, kk);
-      const uint32_t run_excl = shfl16(excl, kk);
+      const uint32_t run_start = __shfl(rs, kk, G);
+      const uint32_t run_excl = __shfl(excl, kk, G);
       bool pass = false;
       float a = 0.0f, y0 = 0.0f, y1 = 0.0f, y2 = 0.0f;
       uint32_t pidx = 0;
       float4 y4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
       if (idx < total) {
         pidx = run_start + (idx - run_excl);
-        y4 = G.pos[pidx];
+        y4 = GL.pos[pidx];
         affine3(mp.A, mp.b, y4.x, y4.y, y4.z, y0, y1, y2);
         pass = pair_kernel<FC, KC>(kc, ek, x0, x1, x2, y0, y1, y2, d2t, qg, l, xf,
-                                   FC != 0 ? G.feat + (size_t)pidx * F : nullptr, xs,
-                                   KC != 0 ? G.sem + (size_t)pidx * KS : nullptr, xg,
-                                   kc.use_gtype ? G.geo + 2 * (size_t)pidx : nullptr, a);
+                                   FC != 0 ? GL.feat + (size_t)pidx * F : nullptr, xs,
+                                   KC != 0 ? GL.sem + (size_t)pidx * KS : nullptr, xg,
+                                   kc.use_gtype ? GL.geo + 2 * (size_t)pidx : nullptr, a);
       }
       const uint64_t m = __ballot(pass);
-      const uint32_t gm = (uint32_t)(m >> (16 * ((threadIdx.x & 63) >> 4))) & 0xFFFFu;
+      const uint32_t gm = (uint32_t)(m >> (G * ((threadIdx.x & 63) / G))) & ((1u << G) - 1u);
       const uint32_t slot = stored + (uint32_t)__popc(gm & ((1u << g) - 1u));
       if (pass && slot < Ku) {
         if (mode == MODE_BA) P.pairs4[(size_t)r * Ku + slot] = make_float4(y4.x, y4.y, y4.z, a);
@@ -369,18 +373,20 @@ __device__ __forceinline__ void assoc_row(const AssocProblem& P, const KConsts& 
 // Block partials of the association pass: part_a[block] = flow6, ip, -;
 // part_cnt[block] = stored pairs, max row.  Summed by overflow_kernel (first
 // stage) and the step / controller kernels (second stage).
-template <int FC, int KC>
+template <int FC, int KC, int G>
 __global__ __launch_bounds__(kBlock) void assoc_kernel(const AssocProblem* __restrict__ probs, int n_probs,
                                                        KConsts kc) {
-  const int p = find_problem(probs, n_probs, (int)blockIdx.x);
+  constexpr int RB = rows_per_block(G);
+  const int bid = kc.xcd_remap ? xcd_block((int)blockIdx.x, (int)gridDim.x) : (int)blockIdx.x;
+  const int p = find_problem(probs, n_probs, bid);
   const AssocProblem& P = probs[p];
   const ProblemState* st = P.st;
   if (st->done) return;
-  const int lb = (int)blockIdx.x - P.block_begin;
+  const int lb = bid - P.block_begin;
   const int tid = threadIdx.x;
-  const int g = tid & (kGroup - 1);
-  const int grp = tid >> 4;
-  const int r = lb * kRowsPerBlock + grp;
+  const int g = tid & (G - 1);
+  const int grp = tid / G;
+  const int r = lb * RB + grp;
   const int mode = kc.mode;
   const float ell = st->ell;
   const int K = st->K;
@@ -388,17 +394,17 @@ __global__ __launch_bounds__(kBlock) void assoc_kernel(const AssocProblem* __res
   Maps mp;
   load_maps(st, mp);
   const ExactK ek = exact_consts(kc);
-  const GridLevel& G = P.grid->lv[st->level];
+  const GridLevel& GL = P.grid->lv[st->level];
 
-  __shared__ double s_row[kRowsPerBlock][8];
-  __shared__ uint32_t s_cnt[kRowsPerBlock];
-  // the block's 16 source rows' features and class vectors, staged in LDS
+  __shared__ double s_row[RB][8];
+  __shared__ uint32_t s_cnt[RB];
+  // the block's source rows' features and class vectors, staged in LDS
   constexpr int FL = FC > 0 ? FC : (FC == 0 ? 1 : 64);
   constexpr int KL = KC > 0 ? KC : (KC == 0 ? 1 : 64);
-  __shared__ float s_feat[kRowsPerBlock * FL];
-  __shared__ float s_sem[kRowsPerBlock * KL];
-  const int row0 = lb * kRowsPerBlock;
-  const int nrows = min(kRowsPerBlock, P.n_src - row0);
+  __shared__ float s_feat[RB * FL];
+  __shared__ float s_sem[RB * KL];
+  const int row0 = lb * RB;
+  const int nrows = min(RB, P.n_src - row0);
   if (FC != 0) {
     const int F = Dim<FC>::get(kc.F);
     for (int t = tid; t < nrows * F; t += kBlock) s_feat[t] = P.src_feat[(size_t)row0 * F + t];
@@ -414,15 +420,13 @@ __global__ __launch_bounds__(kBlock) void assoc_kernel(const AssocProblem* __res
   res.stored = 0;
   res.overflow = false;
   if (r < P.n_src)
-    assoc_row<FC, KC>(P, kc, ek, G, mp, r, g, mode, ell, K, s_feat + grp * Dim<FC>::get(kc.F),
-                      s_sem + grp * Dim<KC>::get(kc.Ks), res);
+    assoc_row<FC, KC, G>(P, kc, ek, GL, mp, r, g, mode, ell, K, s_feat + grp * Dim<FC>::get(kc.F),
+                         s_sem + grp * Dim<KC>::get(kc.Ks), res);
 #pragma unroll
   for (int c = 0; c < 7; ++c) {
     double v = res.acc[c];
-    v += shfl_xor_d(v, 8);
-    v += shfl_xor_d(v, 4);
-    v += shfl_xor_d(v, 2);
-    v += shfl_xor_d(v, 1);
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, G);
     res.acc[c] = v;
   }
   if (g == 0) {
@@ -443,11 +447,11 @@ __global__ __launch_bounds__(kBlock) void assoc_kernel(const AssocProblem* __res
   __syncthreads();
   if (tid < 7) {
     double s = 0.0;
-    for (int q = 0; q < kRowsPerBlock; ++q) s += s_row[q][tid];
+    for (int q = 0; q < RB; ++q) s += s_row[q][tid];
     P.part_a[(size_t)lb * 8 + tid] = s;
   } else if (tid == 7) {
     uint32_t nz = 0, mx = 0;
-    for (int q = 0; q < kRowsPerBlock; ++q) {
+    for (int q = 0; q < RB; ++q) {
       nz += s_cnt[q];
       mx = max(mx, s_cnt[q]);
     }
@@ -827,7 +831,7 @@ __device__ void controller(ProblemState* st, const KConsts& kc, const AssocProbl
     st->done = 1;
     st->iters_out = kc.max_iter;
   }
-  st->level = choose_level_tab(level_h, n_levels, needed_radius(kc, st, st->ell));
+  st->level = choose_level_tab(level_h, n_levels, needed_radius(kc, st, st->ell) * kc.cell_scale);
   st->n_overflow = 0;
 }
 
@@ -1075,7 +1079,7 @@ __global__ void init_state_kernel(const AssocProblem* __restrict__ probs, int n_
   st->qs_head = st->qs_size = st->qe_head = st->qe_size = 0;
   st->ss = st->se = 0.0f;
   set_maps(st);
-  st->level = choose_level(P.grid, needed_radius(kc, st, ell0));
+  st->level = choose_level(P.grid, needed_radius(kc, st, ell0) * kc.cell_scale);
 }
 
 // Switches every problem to a single scoring pass at lengthscale ell with the
@@ -1091,7 +1095,7 @@ __global__ void rescore_state_kernel(const AssocProblem* __restrict__ probs, int
   st->done = 0;
   st->n_overflow = 0;
   set_maps(st);
-  st->level = choose_level(P.grid, needed_radius(kc, st, ell));
+  st->level = choose_level(P.grid, needed_radius(kc, st, ell) * kc.cell_scale);
 }
 
 // ---------------------------------------------------------------------------
@@ -1176,7 +1180,10 @@ void launch_all_done(const AssocProblem* probs, int n_probs, int32_t* out, hipSt
 template <int FC, int KC>
 static void launch_assoc_t(const AssocProblem* probs, int n_probs, int total_blocks, const KConsts& kc,
                            hipStream_t s) {
-  hipLaunchKernelGGL((assoc_kernel<FC, KC>), dim3(total_blocks), dim3(kBlock), 0, s, probs, n_probs, kc);
+  if (kc.group == 8)
+    hipLaunchKernelGGL((assoc_kernel<FC, KC, 8>), dim3(total_blocks), dim3(kBlock), 0, s, probs, n_probs, kc);
+  else
+    hipLaunchKernelGGL((assoc_kernel<FC, KC, 16>), dim3(total_blocks), dim3(kBlock), 0, s, probs, n_probs, kc);
 }
 
 template <int FC, int KC>

@@ -14,7 +14,7 @@ from typing import List, Optional, Sequence
 
 import numpy as np
 
-from ._abi import (Assoc, CvoParams, HeldView, IterRecord, KernelStats, KSTAT_NAMES, mat16)
+from ._abi import (Assoc, CvoParams, HeldView, IterRecord, KernelStats, KSTAT_NAMES, Tuning, mat16)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "librkhs_ba_amd.so")
@@ -60,6 +60,13 @@ def lib() -> C.CDLL:
         L.cvo_align_multistart.argtypes = [P, P, P, i32, P, f, i32, P, P, P, P, P, P]
         L.cvo_align_multiframe.argtypes = [P, i32, P, P, P, i32, P, P, P, P, i32]
         L.cvo_set_pose_arithmetic.argtypes = [P, i32]
+        L.cvo_get_tuning.argtypes = [P, P]
+        L.cvo_set_tuning.argtypes = [P, P]
+        L.cvo_autotune.argtypes = [P, P, P, P, C.c_char_p, P]
+        L.cvo_autotune_d.argtypes = [P, P, P, P, C.c_char_p, P]
+        L.cvo_tuning_key.argtypes = [P, P, P, C.c_char_p, i32]
+        L.cvo_tuning_cache_lookup.argtypes = [C.c_char_p, C.c_char_p, P]
+        L.cvo_tuning_cache_store.argtypes = [C.c_char_p, C.c_char_p, P]
         _lib = L
     return _lib
 
@@ -70,6 +77,31 @@ def _check(st: int, ctx=None):
         if ctx:
             msg = lib().cvo_last_error(ctx).decode(errors="replace")
         raise CvoError(st, msg)
+
+
+def default_tuning_cache() -> str:
+    """Tuning cache file: $RKHS_BA_AMD_TUNE_CACHE, else
+    ~/.cache/rkhs_ba_amd/autotune.json."""
+    env = os.environ.get("RKHS_BA_AMD_TUNE_CACHE")
+    if env:
+        return env
+    return os.path.join(os.path.expanduser("~"), ".cache", "rkhs_ba_amd", "autotune.json")
+
+
+def tuning_cache_lookup(path: str, key: str) -> Optional[dict]:
+    """Entry of a tuning cache file (no device needed); None when absent."""
+    t = Tuning()
+    st = lib().cvo_tuning_cache_lookup(path.encode(), key.encode(), C.byref(t))
+    if st == 3:
+        return None
+    _check(st)
+    return t.as_dict()
+
+
+def tuning_cache_store(path: str, key: str, lanes_per_row: int, xcd_remap: int, cell_scale: float,
+                       score_us: float = 0.0, n_tried: int = 0) -> None:
+    t = Tuning(int(lanes_per_row), int(xcd_remap), float(cell_scale), 1, float(score_us), int(n_tried))
+    _check(lib().cvo_tuning_cache_store(path.encode(), key.encode(), C.byref(t)))
 
 
 def default_params() -> CvoParams:
@@ -388,6 +420,36 @@ class CvoGPU:
 
     def set_profiling(self, on: bool) -> None:
         _check(lib().cvo_set_profiling(self._ctx, int(bool(on))), self._ctx)
+
+    # -- association-kernel tuning ------------------------------------------
+    def get_tuning(self) -> dict:
+        t = Tuning()
+        _check(lib().cvo_get_tuning(self._ctx, C.byref(t)), self._ctx)
+        return t.as_dict()
+
+    def set_tuning(self, lanes_per_row: int = 16, xcd_remap: int = 0, cell_scale: float = 1.0) -> None:
+        t = Tuning(int(lanes_per_row), int(xcd_remap), float(cell_scale), 1, 0.0, 0)
+        _check(lib().cvo_set_tuning(self._ctx, C.byref(t)), self._ctx)
+
+    def autotune(self, source, target, init_guess_transform=None, cache: Optional[str] = "default") -> dict:
+        """Times every association launch shape on this pair and keeps the
+        fastest on the context.  cache: a JSON path, "default"
+        (default_tuning_cache()) or None (measure, do not cache)."""
+        T = mat16(np.eye(4) if init_guess_transform is None else init_guess_transform)
+        path = default_tuning_cache() if cache == "default" else cache
+        t = Tuning()
+        cp = path.encode() if path else None
+        if isinstance(source, DeviceCloud) and isinstance(target, DeviceCloud):
+            _check(lib().cvo_autotune_d(self._ctx, source.handle, target.handle, T.ctypes.data, cp, C.byref(t)), self._ctx)
+        else:
+            hs, ht = HeldView(source), HeldView(target)
+            _check(lib().cvo_autotune(self._ctx, hs.ref, ht.ref, T.ctypes.data, cp, C.byref(t)), self._ctx)
+        return t.as_dict()
+
+    def tuning_key(self, source: "DeviceCloud", target: "DeviceCloud") -> str:
+        buf = C.create_string_buffer(512)
+        _check(lib().cvo_tuning_key(self._ctx, source.handle, target.handle, buf, 512), self._ctx)
+        return buf.value.decode()
 
     def kernel_stats(self) -> dict:
         s = KernelStats()
