@@ -104,7 +104,7 @@ def tune(cvo, ds, dt, args, params=None):
         t = tc.autotune(d1, d2, np.eye(4), cache="default")
     finally:
         tc.close()
-    cvo.set_tuning(t["lanes_per_row"], t["xcd_remap"], t["cell_scale"])
+    cvo.apply_tuning(t)
     return t
 
 

@@ -314,38 +314,52 @@ cvo_status cvo_set_pose_arithmetic(cvo_ctx* ctx, int32_t mode);
  *
  * The reference's association kernel has one fixed shape (one thread per
  * source row over every target, CvoGPU.cu:951-1099); here the grid search has
- * three knobs, none of which changes the pair set, the pair values or the row
- * cap semantics (only the order pairs are summed in, at FP64 rounding level):
- *   lanes_per_row  8 or 16 lanes of a wave share one source row;
- *   xcd_remap      1: each of the eight XCDs takes one contiguous range of
- *                  the spatially sorted row blocks (its own L2 serves the
- *                  overlapping target neighbourhoods), 0: dispatch order;
- *   cell_scale     grid cells are chosen with edge >= cell_scale x the ell-ball
- *                  radius, in [0.4, 1] (smaller: more cells per query box,
- *                  fewer candidates outside the ball).
+ * knobs, none of which changes the pair set, the pair values or the row cap
+ * semantics (only the order a row's pairs are summed in, at FP64 rounding):
+ *   lanes_per_row    8 or 16 lanes of a wave share one source row, at
+ *                    lengthscales >= split_ell (every lengthscale when
+ *                    split_ell <= 0);
+ *   cell_scale       grid cells are chosen with edge >= cell_scale x the
+ *                    ell-ball radius, in [0.4, 1] (smaller: more cells per
+ *                    query box, fewer candidates outside the ball), at
+ *                    lengthscales >= split_ell;
+ *   lanes_small_ell, cell_scale_small_ell
+ *                    the same below split_ell (the device controller applies
+ *                    them per iteration, so they follow the annealing inside
+ *                    the captured iteration graph);
+ *   xcd_remap        1: each of the eight XCDs takes one contiguous range of
+ *                    the spatially sorted row blocks (its own L2 serves the
+ *                    overlapping target neighbourhoods), 0: dispatch order.
  * The tuning of a context applies to every later align / multi-start / BA /
  * inner-product call on it. */
 typedef struct {
   int32_t lanes_per_row;
+  int32_t lanes_small_ell;
+  float split_ell;
   int32_t xcd_remap;
   float cell_scale;
+  float cell_scale_small_ell;
   int32_t source;   /* out: CVO_TUNE_DEFAULT / _SET / _MEASURED / _CACHED */
-  float score_us;   /* out: weighted association time of the chosen setting */
-  int32_t n_tried;  /* out: settings measured (0 when read from the cache) */
+  float score_us;   /* out: modelled association time per iteration of the chosen setting */
+  int32_t n_tried;  /* out: (setting, lengthscale) pairs timed */
 } cvo_tuning;
 enum { CVO_TUNE_DEFAULT = 0, CVO_TUNE_SET = 1, CVO_TUNE_MEASURED = 2, CVO_TUNE_CACHED = 3 };
 
 cvo_status cvo_get_tuning(const cvo_ctx* ctx, cvo_tuning* out);
-/* CVO_ERR_INVALID_ARG for lanes_per_row not in {8, 16}, xcd_remap not in
- * {0, 1} or cell_scale outside [0.4, 1]. */
+/* CVO_ERR_INVALID_ARG for lane counts not in {8, 16}, a negative or
+ * non-finite split_ell, xcd_remap not in {0, 1} or a cell scale outside
+ * [0.4, 1]. */
 cvo_status cvo_set_tuning(cvo_ctx* ctx, const cvo_tuning* t);
-/* Measures the association pass of this source / target pair at T_init for
- * every setting (2 x 2 x 3) at three lengthscales (ell_init, their geometric
- * mean, ell_min; weights 1, 1, 2), sets the fastest on the context and returns
- * it.  With cache_path non-null the result is looked up first under the
- * workload key (cvo_tuning_key) and stored after a measurement (a JSON file,
- * replaced atomically).  The measurement takes well under a second at 60k
- * points. */
+/* Times the association pass of this source / target pair at T_init for
+ * every lane count (8, 16), block order (2) and cell scale (1, 0.7, 0.5) at
+ * four lengthscales log-spaced from ell_init to ell_min, then keeps the
+ * block order and the two-segment (lanes, cell scale) policy with the least
+ * modelled time per registration: the three annealing samples share the
+ * iterations of the decay schedule (decays x 2 x indicator window), ell_min
+ * takes the rest of MAX_ITER.  With cache_path non-null the result is looked up first under
+ * the workload key (cvo_tuning_key) and stored after a measurement (a JSON
+ * file, replaced atomically).  The measurement takes well under a second at
+ * 60k points. */
 cvo_status cvo_autotune(cvo_ctx* ctx, const cvo_cloud_view* src, const cvo_cloud_view* tgt, const float T_init[16],
                         const char* cache_path, cvo_tuning* out);
 cvo_status cvo_autotune_d(cvo_ctx* ctx, const cvo_dcloud* src, const cvo_dcloud* tgt, const float T_init[16],

@@ -138,8 +138,11 @@ class Tuning(C.Structure):
     """cvo_tuning: launch shape of the association kernel."""
     _fields_ = [
         ("lanes_per_row", C.c_int32),
+        ("lanes_small_ell", C.c_int32),
+        ("split_ell", C.c_float),
         ("xcd_remap", C.c_int32),
         ("cell_scale", C.c_float),
+        ("cell_scale_small_ell", C.c_float),
         ("source", C.c_int32),
         ("score_us", C.c_float),
         ("n_tried", C.c_int32),
@@ -148,8 +151,11 @@ class Tuning(C.Structure):
     SOURCES = {0: "default", 1: "set", 2: "measured", 3: "cached"}
 
     def as_dict(self) -> dict:
-        return {"lanes_per_row": self.lanes_per_row, "xcd_remap": self.xcd_remap,
-                "cell_scale": round(float(self.cell_scale), 4), "source": self.SOURCES.get(self.source, self.source),
+        return {"lanes_per_row": self.lanes_per_row, "lanes_small_ell": self.lanes_small_ell,
+                "split_ell": round(float(self.split_ell), 6), "xcd_remap": self.xcd_remap,
+                "cell_scale": round(float(self.cell_scale), 4),
+                "cell_scale_small_ell": round(float(self.cell_scale_small_ell), 4),
+                "source": self.SOURCES.get(self.source, self.source),
                 "score_us": round(float(self.score_us), 3), "n_tried": self.n_tried}
 
 

@@ -301,19 +301,28 @@ std::vector<float> level_sizes(const KConsts& kc, float ell_hi, float ell_lo, fl
   return hs;
 }
 
-// The context's association launch shape.
-void apply_tuning(const cvo_ctx* ctx, KConsts& kc) {
-  kc.group = ctx->tuning.lanes_per_row;
-  kc.xcd_remap = ctx->tuning.xcd_remap;
-  kc.cell_scale = ctx->tuning.cell_scale;
+// An association launch shape (tuning) as kernel constants.
+void apply_tuning(const cvo_tuning& t, KConsts& kc) {
+  const bool split = t.split_ell > 0.0f;
+  kc.pol.split_ell = split ? t.split_ell : 0.0f;
+  kc.pol.group_big = t.lanes_per_row;
+  kc.pol.group_small = split ? t.lanes_small_ell : t.lanes_per_row;
+  kc.pol.scale_big = t.cell_scale;
+  kc.pol.scale_small = split ? t.cell_scale_small_ell : t.cell_scale;
+  kc.max_group = std::max(kc.pol.group_big, kc.pol.group_small);
+  kc.min_scale = std::min(kc.pol.scale_big, kc.pol.scale_small);
+  kc.xcd_remap = t.xcd_remap;
 }
 
+void apply_tuning(const cvo_ctx* ctx, KConsts& kc) { apply_tuning(ctx->tuning, kc); }
+
 // Buffers and descriptors of n_probs problems sharing one source and target;
-// group = lanes per row of the association launch.
+// each problem owns the association blocks of its rows at max_group lanes per
+// row (a level with fewer lanes per row uses a prefix of them).
 cvo_status setup_problems(cvo_ctx* ctx, const cvo_dcloud* s, const cvo_dcloud* t, const GridHost& gh, int n_probs,
-                          int cap_alloc, int trace_cap, int group, Problems& pr) {
+                          int cap_alloc, int trace_cap, int max_group, Problems& pr) {
   const int n = s->n;
-  const int rb = rows_per_block(group);
+  const int rb = rows_per_block(max_group);
   const int nb = (n + rb - 1) / rb;
   pr.n_probs = n_probs;
   pr.total_blocks = nb * n_probs;
@@ -798,11 +807,12 @@ extern "C" cvo_status cvo_align_d(cvo_ctx* ctx, const cvo_dcloud* src, const cvo
   float tq = std::max(std::fabs(T_init[3]), std::max(std::fabs(T_init[7]), std::fabs(T_init[11])));
   GridHost gh;
   float ell_lo = std::min(p.ell_min, p.ell_init);
-  CVO_TRY(build_grid(ctx, tgt, level_sizes(kc, std::max(p.ell_init, p.ell_min), ell_lo, src->max_norm, tq, kc.cell_scale), gh));
+  CVO_TRY(build_grid(ctx, tgt, level_sizes(kc, std::max(p.ell_init, p.ell_min), ell_lo, src->max_norm, tq, kc.min_scale),
+                     gh));
   const bool want_trace = ctx->trace_cap > 0 || ctx->profiling;
   const int trace_cap = want_trace ? std::max(ctx->trace_cap, ctx->profiling ? p.MAX_ITER : 0) : 0;
   Problems pr;
-  CVO_TRY(setup_problems(ctx, src, tgt, gh, 1, p.nearest_neighbors_max, trace_cap, kc.group, pr));
+  CVO_TRY(setup_problems(ctx, src, tgt, gh, 1, p.nearest_neighbors_max, trace_cap, kc.max_group, pr));
   CVO_TRY(ensure(ctx, ctx->b_tmp2, 64 * sizeof(float)));
   CVO_HIP(ctx, hipMemcpyAsync(ctx->b_tmp2.p, T_init, sizeof(float) * 16, hipMemcpyHostToDevice, ctx->stream));
   launch_init_state(pr.dev, 1, as<float>(ctx->b_tmp2), 16, kc, p.ell_init, p.nearest_neighbors_max, ctx->stream);
@@ -870,9 +880,9 @@ static cvo_status single_pass(cvo_ctx* ctx, const cvo_dcloud* src, const cvo_dcl
   CVO_TRY(prepare_source(ctx, src));
   float tq = std::max(std::fabs(T[3]), std::max(std::fabs(T[7]), std::fabs(T[11])));
   GridHost gh;
-  std::vector<float> hs = level_sizes(kc, ell, ell, src->max_norm, tq, kc.cell_scale);
+  std::vector<float> hs = level_sizes(kc, ell, ell, src->max_norm, tq, kc.min_scale);
   CVO_TRY(build_grid(ctx, tgt, hs, gh));
-  CVO_TRY(setup_problems(ctx, src, tgt, gh, 1, p.nearest_neighbors_max, 0, kc.group, pr));
+  CVO_TRY(setup_problems(ctx, src, tgt, gh, 1, p.nearest_neighbors_max, 0, kc.max_group, pr));
   CVO_TRY(ensure(ctx, ctx->b_tmp2, 64 * sizeof(float)));
   CVO_HIP(ctx, hipMemcpyAsync(ctx->b_tmp2.p, T, sizeof(float) * 16, hipMemcpyHostToDevice, ctx->stream));
   launch_init_state(pr.dev, 1, as<float>(ctx->b_tmp2), 16, kc, ell, p.nearest_neighbors_max, ctx->stream);
@@ -999,10 +1009,10 @@ extern "C" cvo_status cvo_align_multistart(cvo_ctx* ctx, const cvo_dcloud* src, 
   const float ell_hi = std::max(std::max(p.ell_init, p.ell_min), score_ell);
   const float ell_lo = std::min(std::min(p.ell_min, p.ell_init), score_ell);
   GridHost gh;
-  CVO_TRY(build_grid(ctx, tgt, level_sizes(kc, ell_hi, ell_lo, src->max_norm, tq, kc.cell_scale), gh));
+  CVO_TRY(build_grid(ctx, tgt, level_sizes(kc, ell_hi, ell_lo, src->max_norm, tq, kc.min_scale), gh));
   const int n_run = (mode == 1) ? n_starts : n_starts;
   Problems pr;
-  CVO_TRY(setup_problems(ctx, src, tgt, gh, n_run, p.nearest_neighbors_max, 0, kc.group, pr));
+  CVO_TRY(setup_problems(ctx, src, tgt, gh, n_run, p.nearest_neighbors_max, 0, kc.max_group, pr));
   CVO_TRY(ensure(ctx, ctx->b_tmp2, sizeof(float) * 16 * (size_t)n_starts + 64));
   CVO_HIP(ctx, hipMemcpyAsync(ctx->b_tmp2.p, T_starts, sizeof(float) * 16 * n_starts, hipMemcpyHostToDevice,
                               ctx->stream));

@@ -69,7 +69,8 @@ struct BaDev {
   int32_t lm_iter, lm_stop, eval_pending, eval_is_candidate;
   int32_t K_init, min_nonzeros, iter_per_ell, counter_nonzero;
   int32_t stopped_no_edges, accepted_total, evals_total, h_cur;
-  float ell_min_f, decay_f, rq_coef, cell_scale;
+  float ell_min_f, decay_f, rq_coef, pad_f;
+  EllPolicy pol;
   double ell_min, ell_max;
   double F, Fn, mu, v, change;
   int32_t free_idx[kBaMaxFrames];
@@ -170,7 +171,9 @@ __global__ __launch_bounds__(256) void ba_prepare_kernel(BaDev* __restrict__ ba,
     const float tq = fmaxf(fabsf(st->Tq[0]), fmaxf(fabsf(st->Tq[1]), fabsf(st->Tq[2])));
     const float r_need =
         ba->rq_coef * st->ell * (1.0f + sn / 500.0f) * 1.0001f + 1e-4f * (1.0f + sn + tq) + 1e-6f;
-    st->level = choose_level(probs[e].grid, r_need * ba->cell_scale);
+    const bool big = policy_big(ba->pol, st->ell);
+    st->group = big ? ba->pol.group_big : ba->pol.group_small;
+    st->level = choose_level(probs[e].grid, r_need * (big ? ba->pol.scale_big : ba->pol.scale_small));
   }
 }
 
@@ -757,7 +760,7 @@ extern "C" cvo_status cvo_align_multiframe(cvo_ctx* ctx, int32_t n_frames, const
   const auto t_start = std::chrono::steady_clock::now();
   KConsts kc = make_kconsts(p, frames[0]->F, frames[0]->K, MODE_BA);
   apply_tuning(ctx, kc);
-  const int rb = rows_per_block(kc.group);
+  const int rb = rows_per_block(kc.max_group);
 
   // frames that are sources / targets of some edge
   std::vector<char> is_src(n_frames, 0), is_tgt(n_frames, 0);
@@ -776,7 +779,7 @@ extern "C" cvo_status cvo_align_multiframe(cvo_ctx* ctx, int32_t n_frames, const
   std::vector<GridHost> grids(n_frames);
   const float ell_hi = std::max(p.multiframe_ell_init * 1.2f, p.multiframe_ell_min);
   const float ell_lo = std::min(p.multiframe_ell_min, p.multiframe_ell_init);
-  const std::vector<float> hs = level_sizes(kc, ell_hi, ell_lo, max_world * 1.05f, max_t * 1.05f + 1.0f, kc.cell_scale);
+  const std::vector<float> hs = level_sizes(kc, ell_hi, ell_lo, max_world * 1.05f, max_t * 1.05f + 1.0f, kc.min_scale);
   for (int f = 0; f < n_frames; ++f) {
     DevBuf* fb = &ctx->b_frames[(size_t)kFrameBufs * f];
     if (is_src[f]) CVO_TRY(prepare_source_into(ctx, frames[f], fb[0], fb[1], fb[2], fb[3]));
@@ -876,7 +879,7 @@ extern "C" cvo_status cvo_align_multiframe(cvo_ctx* ctx, int32_t n_frames, const
   hb->ell_min_f = p.multiframe_ell_min;
   hb->decay_f = p.multiframe_ell_decay_rate;
   hb->rq_coef = kc.rq_coef;
-  hb->cell_scale = kc.cell_scale;
+  hb->pol = kc.pol;
   hb->ell_min = (double)p.multiframe_ell_min;
   hb->ell_max = (double)p.multiframe_ell_init * 1.2;
   int o = 0;

@@ -73,6 +73,7 @@ struct alignas(16) ProblemState {
   float ss, se;
   float flow_norm;
   int32_t level_assoc;  // grid level of the last association pass
+  int32_t group;        // lanes per source row of the next association pass
 };
 
 // Read-only description of one association problem.
@@ -104,6 +105,18 @@ struct AssocProblem {
   cvo_iter_record* trace;  // per iteration (may be null)
 };
 
+// Association launch shape as a function of the lengthscale: lanes per row
+// and grid cell scale above / below split_ell (one setting when split_ell <= 0).
+struct EllPolicy {
+  float split_ell;
+  int32_t group_big, group_small;  // 8 or 16
+  float scale_big, scale_small;    // level choice: cell edge >= scale * radius
+};
+
+__host__ __device__ inline bool policy_big(const EllPolicy& pol, float ell) {
+  return !(pol.split_ell > 0.0f) || ell >= pol.split_ell;
+}
+
 // Scalar constants of the kernel (fill_in_A_mat_gpu, CvoGPU.cu:966-987).
 struct KConsts {
   float sigma2, c_sigma2, s_sigma2;
@@ -128,13 +141,15 @@ struct KConsts {
   int32_t mode;               // 0 align, 1 inner product, 2 BA
   int32_t mirror_pose;        // 1: the reference's FP32 Exp_SEK3 and float pose
   // association launch shape (none of these changes the pair set)
-  int32_t group;              // lanes per source row: 8 or 16
+  EllPolicy pol;
+  int32_t max_group;          // largest lane count of the policy (sizes the grid)
   int32_t xcd_remap;          // 1: contiguous row-block ranges per XCD
-  float cell_scale;           // level choice: cell edge >= cell_scale * radius
+  float min_scale;            // finest cell scale of the policy (sizes the levels)
 };
 
-// Rows per association block for a lane count per row.
+// Rows per association block for a lane count per row (8 or 16).
 __host__ __device__ constexpr int rows_per_block(int group) { return kBlock / group; }
+constexpr int kMaxRowsPerBlock = kBlock / 8;
 
 enum Mode { MODE_ALIGN = 0, MODE_IP = 1, MODE_BA = 2 };
 
@@ -189,7 +204,7 @@ struct cvo_ctx {
   std::vector<DevBuf> b_frames;  // per frame: sorted source (4) and grid (2)
   int32_t ba_last_accepted = 0, ba_last_evals = 0;
   int32_t pose_arith = 0;  // CVO_POSE_FP64 or CVO_POSE_REFERENCE
-  cvo_tuning tuning{16, 0, 1.0f, CVO_TUNE_DEFAULT, 0.0f, 0};
+  cvo_tuning tuning{16, 16, 0.0f, 1, 1.0f, 1.0f, CVO_TUNE_DEFAULT, 0.0f, 0};
   // model scratch
   DevBuf b_mlp[16];
 };
@@ -209,6 +224,7 @@ float radius_for(const KConsts& kc, float ell, float max_norm, float tq);
 std::vector<float> level_sizes(const KConsts& kc, float ell_hi, float ell_lo, float max_norm, float tq,
                                float cell_scale);
 void apply_tuning(const cvo_ctx* ctx, KConsts& kc);
+void apply_tuning(const cvo_tuning& t, KConsts& kc);
 cvo_status ensure_events(cvo_ctx* ctx, size_t n);
 cvo_status pinned(cvo_ctx* ctx, size_t bytes);
 cvo_status prepare_source_into(cvo_ctx* ctx, const cvo_dcloud* s, DevBuf& b_pos, DevBuf& b_feat, DevBuf& b_sem,
@@ -234,8 +250,9 @@ struct Problems {
   cvo_iter_record* trace = nullptr;
   int trace_cap = 0;
 };
+// max_group: the largest lane count any level uses (fewest rows per block).
 cvo_status setup_problems(cvo_ctx* ctx, const cvo_dcloud* s, const cvo_dcloud* t, const GridHost& gh, int n_probs,
-                          int cap_alloc, int trace_cap, int group, Problems& pr);
+                          int cap_alloc, int trace_cap, int max_group, Problems& pr);
 }  // namespace cvo
 
 #define CVO_HIP(ctx, call)                                              \

@@ -12,7 +12,7 @@
 //                                   controller (:464-544, :1289-1433)
 //
 // Rows are processed by groups of G = 8 or 16 lanes (256 / G rows per
-// 256-thread block).  Reductions are block partials summed in fixed order, so
+// 256-thread block), G chosen per iteration from the lengthscale.  Reductions are block partials summed in fixed order, so
 // results are bitwise reproducible run to run.
 
 #include <hip/hip_runtime.h>
@@ -343,7 +343,8 @@ __device__ __forceinline__ void assoc_row(const AssocProblem& P, const KConsts& 
                                    kc.use_gtype ? GL.geo + 2 * (size_t)pidx : nullptr, a);
       }
       const uint64_t m = __ballot(pass);
-      const uint32_t gm = (uint32_t)(m >> (G * ((threadIdx.x & 63) / G))) & ((1u << G) - 1u);
+      constexpr uint32_t kGroupMask = (uint32_t)((1ull << G) - 1ull);
+      const uint32_t gm = (uint32_t)(m >> (G * ((threadIdx.x & 63) / G))) & kGroupMask;
       const uint32_t slot = stored + (uint32_t)__popc(gm & ((1u << g) - 1u));
       if (pass && slot < Ku) {
         if (mode == MODE_BA) P.pairs4[(size_t)r * Ku + slot] = make_float4(y4.x, y4.y, y4.z, a);
@@ -373,16 +374,24 @@ __device__ __forceinline__ void assoc_row(const AssocProblem& P, const KConsts& 
 // Block partials of the association pass: part_a[block] = flow6, ip, -;
 // part_cnt[block] = stored pairs, max row.  Summed by overflow_kernel (first
 // stage) and the step / controller kernels (second stage).
+
+// LDS of one association block, sized for the most rows a block can hold.
+template <int FC, int KC>
+struct AssocLds {
+  static constexpr int FL = FC > 0 ? FC : (FC == 0 ? 1 : 64);
+  static constexpr int KL = KC > 0 ? KC : (KC == 0 ? 1 : 64);
+  double row[kMaxRowsPerBlock][8];
+  uint32_t cnt[kMaxRowsPerBlock];
+  float feat[kMaxRowsPerBlock * FL];  // the block's source rows' features
+  float sem[kMaxRowsPerBlock * KL];   // and class vectors
+};
+
+// Logical block lb of a problem with G lanes per row (256 / G rows).
 template <int FC, int KC, int G>
-__global__ __launch_bounds__(kBlock) void assoc_kernel(const AssocProblem* __restrict__ probs, int n_probs,
-                                                       KConsts kc) {
+__device__ __forceinline__ void assoc_block(const AssocProblem& P, const KConsts& kc, int lb,
+                                            AssocLds<FC, KC>& L) {
   constexpr int RB = rows_per_block(G);
-  const int bid = kc.xcd_remap ? xcd_block((int)blockIdx.x, (int)gridDim.x) : (int)blockIdx.x;
-  const int p = find_problem(probs, n_probs, bid);
-  const AssocProblem& P = probs[p];
   const ProblemState* st = P.st;
-  if (st->done) return;
-  const int lb = bid - P.block_begin;
   const int tid = threadIdx.x;
   const int g = tid & (G - 1);
   const int grp = tid / G;
@@ -395,23 +404,15 @@ __global__ __launch_bounds__(kBlock) void assoc_kernel(const AssocProblem* __res
   load_maps(st, mp);
   const ExactK ek = exact_consts(kc);
   const GridLevel& GL = P.grid->lv[st->level];
-
-  __shared__ double s_row[RB][8];
-  __shared__ uint32_t s_cnt[RB];
-  // the block's source rows' features and class vectors, staged in LDS
-  constexpr int FL = FC > 0 ? FC : (FC == 0 ? 1 : 64);
-  constexpr int KL = KC > 0 ? KC : (KC == 0 ? 1 : 64);
-  __shared__ float s_feat[RB * FL];
-  __shared__ float s_sem[RB * KL];
   const int row0 = lb * RB;
   const int nrows = min(RB, P.n_src - row0);
   if (FC != 0) {
     const int F = Dim<FC>::get(kc.F);
-    for (int t = tid; t < nrows * F; t += kBlock) s_feat[t] = P.src_feat[(size_t)row0 * F + t];
+    for (int t = tid; t < nrows * F; t += kBlock) L.feat[t] = P.src_feat[(size_t)row0 * F + t];
   }
   if (KC != 0) {
     const int KS = Dim<KC>::get(kc.Ks);
-    for (int t = tid; t < nrows * KS; t += kBlock) s_sem[t] = P.src_sem[(size_t)row0 * KS + t];
+    for (int t = tid; t < nrows * KS; t += kBlock) L.sem[t] = P.src_sem[(size_t)row0 * KS + t];
   }
   __syncthreads();
   RowResult res;
@@ -420,8 +421,8 @@ __global__ __launch_bounds__(kBlock) void assoc_kernel(const AssocProblem* __res
   res.stored = 0;
   res.overflow = false;
   if (r < P.n_src)
-    assoc_row<FC, KC, G>(P, kc, ek, GL, mp, r, g, mode, ell, K, s_feat + grp * Dim<FC>::get(kc.F),
-                         s_sem + grp * Dim<KC>::get(kc.Ks), res);
+    assoc_row<FC, KC, G>(P, kc, ek, GL, mp, r, g, mode, ell, K, L.feat + grp * Dim<FC>::get(kc.F),
+                         L.sem + grp * Dim<KC>::get(kc.Ks), res);
 #pragma unroll
   for (int c = 0; c < 7; ++c) {
     double v = res.acc[c];
@@ -441,22 +442,55 @@ __global__ __launch_bounds__(kBlock) void assoc_kernel(const AssocProblem* __res
       }
     }
 #pragma unroll
-    for (int c = 0; c < 7; ++c) s_row[grp][c] = res.overflow ? 0.0 : res.acc[c];
-    s_cnt[grp] = cnt;
+    for (int c = 0; c < 7; ++c) L.row[grp][c] = res.overflow ? 0.0 : res.acc[c];
+    L.cnt[grp] = cnt;
   }
   __syncthreads();
   if (tid < 7) {
     double s = 0.0;
-    for (int q = 0; q < RB; ++q) s += s_row[q][tid];
+    for (int q = 0; q < RB; ++q) s += L.row[q][tid];
     P.part_a[(size_t)lb * 8 + tid] = s;
   } else if (tid == 7) {
     uint32_t nz = 0, mx = 0;
     for (int q = 0; q < RB; ++q) {
-      nz += s_cnt[q];
-      mx = max(mx, s_cnt[q]);
+      nz += L.cnt[q];
+      mx = max(mx, L.cnt[q]);
     }
     P.part_cnt[(size_t)lb * 2] = nz;
     P.part_cnt[(size_t)lb * 2 + 1] = mx;
+  }
+}
+
+// Lanes per row of a problem's next pass (8 or 16).
+__device__ __forceinline__ int state_group(const ProblemState* st) { return st->group == 8 ? 8 : 16; }
+
+// Active association blocks of a problem at a lane count.
+__device__ __forceinline__ int active_blocks(const AssocProblem& P, int group) {
+  const int rb = kBlock / group;
+  return (P.n_src + rb - 1) / rb;
+}
+
+// One launch covers every problem; a problem owns P.n_blocks blocks (its rows
+// at the largest lane count) and uses a prefix of them at its current lane
+// count.  With xcd_remap the active blocks are renumbered so that each
+// XCD holds one contiguous range of rows.
+template <int FC, int KC>
+__global__ __launch_bounds__(kBlock) void assoc_kernel(const AssocProblem* __restrict__ probs, int n_probs,
+                                                       KConsts kc) {
+  const int p = find_problem(probs, n_probs, (int)blockIdx.x);
+  const AssocProblem& P = probs[p];
+  const ProblemState* st = P.st;
+  if (st->done) return;
+  const int group = state_group(st);
+  const int n_act = active_blocks(P, group);
+  int lb = (int)blockIdx.x - P.block_begin;
+  if (lb >= n_act) return;  // uniform
+  if (kc.xcd_remap) lb = xcd_block(lb, n_act);
+  __shared__ AssocLds<FC, KC> L;
+  switch (group) {
+    // two lane counts keep the kernel within 128 VGPRs (four waves per SIMD)
+    case 8: assoc_block<FC, KC, 8>(P, kc, lb, L); break;
+    default: assoc_block<FC, KC, 16>(P, kc, lb, L); break;
   }
 }
 
@@ -479,8 +513,9 @@ __global__ __launch_bounds__(kBlock) void overflow_kernel(const AssocProblem* __
   __shared__ double s_part[10];
   __shared__ double s_tree[10][64];
   {
-    const int per = (P.n_blocks + kOvfBlocks - 1) / kOvfBlocks;
-    const int b0 = blockIdx.x * per, b1 = min(P.n_blocks, b0 + per);
+    const int n_act = active_blocks(P, state_group(st));
+    const int per = (n_act + kOvfBlocks - 1) / kOvfBlocks;
+    const int b0 = blockIdx.x * per, b1 = min(n_act, b0 + per);
     for (int t = tid; t < 10 * 64; t += kBlock) {
       const int c = t / 64, i = t - c * 64;
       double v = 0.0;
@@ -735,6 +770,21 @@ __device__ float needed_radius(const KConsts& kc, const ProblemState* st, float 
   return kc.rq_coef * ell * (1.0f + kc.max_src_norm / 500.0f) * 1.0001f + 1e-4f * (1.0f + kc.max_src_norm + tq) + 1e-6f;
 }
 
+// Lanes per row and grid level of the next association pass at lengthscale
+// ell (the level table given as cell sizes or as the grid).
+__device__ void set_level(ProblemState* st, const KConsts& kc, float ell, const float* level_h, int n_levels) {
+  const bool big = policy_big(kc.pol, ell);
+  st->group = big ? kc.pol.group_big : kc.pol.group_small;
+  st->level = choose_level_tab(level_h, n_levels,
+                               needed_radius(kc, st, ell) * (big ? kc.pol.scale_big : kc.pol.scale_small));
+}
+
+__device__ void set_level(ProblemState* st, const KConsts& kc, float ell, const GridDev* grid) {
+  const bool big = policy_big(kc.pol, ell);
+  st->group = big ? kc.pol.group_big : kc.pol.group_small;
+  st->level = choose_level(grid, needed_radius(kc, st, ell) * (big ? kc.pol.scale_big : kc.pol.scale_small));
+}
+
 // One controller step of align_impl (CvoGPU.cu:1342-1433), FP64 pose state.
 __device__ void controller(ProblemState* st, const KConsts& kc, const AssocProblem& P, const float* level_h,
                            int n_levels) {
@@ -831,7 +881,7 @@ __device__ void controller(ProblemState* st, const KConsts& kc, const AssocProbl
     st->done = 1;
     st->iters_out = kc.max_iter;
   }
-  st->level = choose_level_tab(level_h, n_levels, needed_radius(kc, st, st->ell) * kc.cell_scale);
+  set_level(st, kc, st->ell, level_h, n_levels);
   st->n_overflow = 0;
 }
 
@@ -1079,7 +1129,7 @@ __global__ void init_state_kernel(const AssocProblem* __restrict__ probs, int n_
   st->qs_head = st->qs_size = st->qe_head = st->qe_size = 0;
   st->ss = st->se = 0.0f;
   set_maps(st);
-  st->level = choose_level(P.grid, needed_radius(kc, st, ell0) * kc.cell_scale);
+  set_level(st, kc, ell0, P.grid);
 }
 
 // Switches every problem to a single scoring pass at lengthscale ell with the
@@ -1095,7 +1145,7 @@ __global__ void rescore_state_kernel(const AssocProblem* __restrict__ probs, int
   st->done = 0;
   st->n_overflow = 0;
   set_maps(st);
-  st->level = choose_level(P.grid, needed_radius(kc, st, ell) * kc.cell_scale);
+  set_level(st, kc, ell, P.grid);
 }
 
 // ---------------------------------------------------------------------------
@@ -1180,10 +1230,7 @@ void launch_all_done(const AssocProblem* probs, int n_probs, int32_t* out, hipSt
 template <int FC, int KC>
 static void launch_assoc_t(const AssocProblem* probs, int n_probs, int total_blocks, const KConsts& kc,
                            hipStream_t s) {
-  if (kc.group == 8)
-    hipLaunchKernelGGL((assoc_kernel<FC, KC, 8>), dim3(total_blocks), dim3(kBlock), 0, s, probs, n_probs, kc);
-  else
-    hipLaunchKernelGGL((assoc_kernel<FC, KC, 16>), dim3(total_blocks), dim3(kBlock), 0, s, probs, n_probs, kc);
+  hipLaunchKernelGGL((assoc_kernel<FC, KC>), dim3(total_blocks), dim3(kBlock), 0, s, probs, n_probs, kc);
 }
 
 template <int FC, int KC>

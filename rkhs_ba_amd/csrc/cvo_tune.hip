@@ -1,9 +1,10 @@
 // Association-kernel autotuner and its JSON cache.
 //
-// The knobs (lanes per row, XCD-contiguous block ranges, grid cell scale)
-// change the launch shape and the candidate enumeration of the ell-ball
-// search, never the pair set: every setting is timed on the caller's own
-// clouds at three lengthscales and the fastest is kept on the context.
+// The knobs (lanes per row and grid cell scale per lengthscale range,
+// XCD-contiguous block ranges) change the launch shape and the candidate
+// enumeration of the ell-ball search, never the pair set: every setting is
+// timed on the caller's own clouds at four lengthscales and the policy with
+// the least modelled time per registration is kept on the context.
 
 #include <hip/hip_runtime.h>
 #include <sys/stat.h>
@@ -32,9 +33,13 @@ T* as(DevBuf& b) {
   return reinterpret_cast<T*>(b.p);
 }
 
+bool valid_lanes(int g) { return g == 8 || g == 16; }
+
 bool valid_tuning(const cvo_tuning& t) {
-  return (t.lanes_per_row == 8 || t.lanes_per_row == 16) && (t.xcd_remap == 0 || t.xcd_remap == 1) &&
-         std::isfinite(t.cell_scale) && t.cell_scale >= 0.4f && t.cell_scale <= 1.0f;
+  auto scale_ok = [](float s) { return std::isfinite(s) && s >= 0.4f && s <= 1.0f; };
+  return valid_lanes(t.lanes_per_row) && valid_lanes(t.lanes_small_ell) && std::isfinite(t.split_ell) &&
+         t.split_ell >= 0.0f && (t.xcd_remap == 0 || t.xcd_remap == 1) && scale_ok(t.cell_scale) &&
+         scale_ok(t.cell_scale_small_ell);
 }
 
 // ---- cache file: one entry per line inside "entries" -----------------------
@@ -42,7 +47,9 @@ bool valid_tuning(const cvo_tuning& t) {
 // {
 // "version": 1,
 // "entries": {
-// "<key>": {"lanes_per_row": 8, "xcd_remap": 1, "cell_scale": 0.7, "score_us": 21.5, "n_tried": 12},
+// "<key>": {"lanes_per_row": 16, "lanes_small_ell": 8, "split_ell": 0.11, "xcd_remap": 1,
+//           "cell_scale": 0.7, "cell_scale_small_ell": 1.0, "score_us": 21.5, "n_tried": 48},
+//                                                                     (one line per entry)
 // ...
 // }
 // }
@@ -94,8 +101,10 @@ bool field_num(const std::string& obj, const char* name, double* v) {
 std::string format_entry(const cvo_tuning& t) {
   char buf[256];
   std::snprintf(buf, sizeof(buf),
-                "{\"lanes_per_row\": %d, \"xcd_remap\": %d, \"cell_scale\": %.4f, \"score_us\": %.3f, \"n_tried\": %d}",
-                t.lanes_per_row, t.xcd_remap, (double)t.cell_scale, (double)t.score_us, t.n_tried);
+                "{\"lanes_per_row\": %d, \"lanes_small_ell\": %d, \"split_ell\": %.6g, \"xcd_remap\": %d, "
+                "\"cell_scale\": %.4f, \"cell_scale_small_ell\": %.4f, \"score_us\": %.3f, \"n_tried\": %d}",
+                t.lanes_per_row, t.lanes_small_ell, (double)t.split_ell, t.xcd_remap, (double)t.cell_scale,
+                (double)t.cell_scale_small_ell, (double)t.score_us, t.n_tried);
   return buf;
 }
 
@@ -131,16 +140,22 @@ extern "C" cvo_status cvo_tuning_cache_lookup(const char* path, const char* key,
   auto m = parse_entries(text);
   auto it = m.find(key);
   if (it == m.end()) return CVO_ERR_IO;
-  double lanes = 0, xcd = 0, scale = 0, score = 0, tried = 0;
+  double lanes = 0, small = 0, split = 0, xcd = 0, scale = 0, scale_small = 0, score = 0, tried = 0;
   if (!field_num(it->second, "lanes_per_row", &lanes) || !field_num(it->second, "xcd_remap", &xcd) ||
       !field_num(it->second, "cell_scale", &scale))
     return CVO_ERR_PARSE;
+  if (!field_num(it->second, "lanes_small_ell", &small)) small = lanes;
+  field_num(it->second, "split_ell", &split);
+  if (!field_num(it->second, "cell_scale_small_ell", &scale_small)) scale_small = scale;
   field_num(it->second, "score_us", &score);
   field_num(it->second, "n_tried", &tried);
   cvo_tuning t{};
   t.lanes_per_row = (int32_t)lanes;
+  t.lanes_small_ell = (int32_t)small;
+  t.split_ell = (float)split;
   t.xcd_remap = (int32_t)xcd;
   t.cell_scale = (float)scale;
+  t.cell_scale_small_ell = (float)scale_small;
   t.source = CVO_TUNE_CACHED;
   t.score_us = (float)score;
   t.n_tried = (int32_t)tried;
@@ -184,7 +199,8 @@ extern "C" cvo_status cvo_get_tuning(const cvo_ctx* ctx, cvo_tuning* out) {
 extern "C" cvo_status cvo_set_tuning(cvo_ctx* ctx, const cvo_tuning* t) {
   if (!ctx || !t) return CVO_ERR_INVALID_ARG;
   if (!valid_tuning(*t))
-    return set_err(ctx, CVO_ERR_INVALID_ARG, "tuning: lanes_per_row 8|16, xcd_remap 0|1, cell_scale in [0.4, 1]");
+    return set_err(ctx, CVO_ERR_INVALID_ARG,
+                   "tuning: lanes 8|16, split_ell >= 0, xcd_remap 0|1, cell scales in [0.4, 1]");
   ctx->tuning = *t;
   ctx->tuning.source = CVO_TUNE_SET;
   return CVO_OK;
@@ -235,76 +251,123 @@ extern "C" cvo_status cvo_autotune_d(cvo_ctx* ctx, const cvo_dcloud* src, const 
   CVO_TRY(prepare_source(ctx, src));
   const float tq = std::max(std::fabs(T_init[3]), std::max(std::fabs(T_init[7]), std::fabs(T_init[11])));
   const float e_hi = std::max(p.ell_init, p.ell_min), e_lo = std::min(p.ell_init, p.ell_min);
-  std::vector<float> ells = {e_hi};
-  std::vector<double> w = {1.0};
-  if (e_lo < e_hi) {
-    ells.push_back(std::sqrt(e_hi * e_lo));
+  // lengthscale samples (descending) and the iterations each stands for: the
+  // decay schedule spends about two indicator windows per decay before
+  // ell_min, which keeps the rest of MAX_ITER
+  std::vector<float> ells;
+  std::vector<double> w;
+  const int n_samples = e_lo < e_hi ? 4 : 1;
+  for (int i = 0; i < n_samples; ++i)
+    ells.push_back(n_samples == 1 ? e_hi : (float)(e_hi * std::pow((double)e_lo / e_hi, (double)i / (n_samples - 1))));
+  if (n_samples == 1) {
     w.push_back(1.0);
-    ells.push_back(e_lo);
-    w.push_back(2.0);
+  } else {
+    const double rate = (p.ell_decay_rate > 0.0f && p.ell_decay_rate < 1.0f) ? p.ell_decay_rate : 0.9;
+    const double n_dec = std::ceil(std::log((double)e_lo / e_hi) / std::log(rate));
+    const double anneal = n_dec * 2.0 * std::max(1, p.indicator_window_size) + std::max(0, p.ell_decay_start);
+    const double rest = std::max((double)p.MAX_ITER - anneal, 2.0 * std::max(1, p.indicator_window_size));
+    for (int i = 0; i < n_samples - 1; ++i) w.push_back(anneal / (n_samples - 1));
+    w.push_back(rest);
   }
   CVO_TRY(ensure(ctx, ctx->b_tmp2, 64 * sizeof(float)));
   CVO_HIP(ctx, hipMemcpyAsync(ctx->b_tmp2.p, T_init, sizeof(float) * 16, hipMemcpyHostToDevice, st));
   CVO_TRY(ensure_events(ctx, 2));
   hipEvent_t e0 = ctx->ev_pool[0], e1 = ctx->ev_pool[1];
-  const float scales[3] = {1.0f, 0.7f, 0.5f};
-  const int groups[2] = {16, 8};
-  constexpr int kReps = 5, kRounds = 3;
   const char* vb = std::getenv("CVO_TUNE_VERBOSE");
   const bool verbose = vb && vb[0] && vb[0] != '0';
+  // settings (lanes, cell scale) timed at every sample, per block order
+  constexpr int kNS = 6;
+  const int set_lanes[kNS] = {16, 8, 16, 8, 16, 8};
+  const float set_scale[kNS] = {1.0f, 1.0f, 0.7f, 0.7f, 0.5f, 0.5f};
+  constexpr int kReps = 5, kRounds = 3;
+  // one grid serves every scale: its levels reach the finest scale's radius
+  GridHost gh;
+  CVO_TRY(build_grid(ctx, tgt, level_sizes(kc, e_hi, e_lo, src->max_norm, tq, 0.5f), gh));
+  Problems pr;
+  CVO_TRY(setup_problems(ctx, src, tgt, gh, 1, p.nearest_neighbors_max, 0, 16, pr));
+  double wsum = 0.0;
+  for (double x : w) wsum += x;
   cvo_tuning best{};
   double best_score = 1e300;
   int n_tried = 0;
-  for (float sc : scales) {
-    GridHost gh;
-    CVO_TRY(build_grid(ctx, tgt, level_sizes(kc, e_hi, e_lo, src->max_norm, tq, sc), gh));
-    for (int grp : groups) {
-      Problems pr;
-      CVO_TRY(setup_problems(ctx, src, tgt, gh, 1, p.nearest_neighbors_max, 0, grp, pr));
-      for (int xcd = 0; xcd < 2; ++xcd) {
-        KConsts kv = kc;
-        kv.group = grp;
-        kv.xcd_remap = xcd;
-        kv.cell_scale = sc;
-        double score = 0.0, wsum = 0.0;
-        std::string line;
-        for (size_t i = 0; i < ells.size(); ++i) {
-          launch_init_state(pr.dev, 1, as<float>(ctx->b_tmp2), 16, kv, ells[i], p.nearest_neighbors_max, st);
-          launch_assoc(pr.dev, 1, pr.total_blocks, kv, st);  // warm-up
-          CVO_HIP(ctx, hipGetLastError());
-          float best_ms = 1e30f;
-          for (int rd = 0; rd < kRounds; ++rd) {
-            CVO_HIP(ctx, hipEventRecord(e0, st));
-            for (int r = 0; r < kReps; ++r) launch_assoc(pr.dev, 1, pr.total_blocks, kv, st);
-            CVO_HIP(ctx, hipEventRecord(e1, st));
-            CVO_HIP(ctx, hipEventSynchronize(e1));
-            float ms = 0.0f;
-            CVO_HIP(ctx, hipEventElapsedTime(&ms, e0, e1));
-            best_ms = std::min(best_ms, ms / kReps);
+  for (int xcd = 0; xcd < 2; ++xcd) {
+    // t[i][j]: microseconds per association pass at sample i, setting j
+    double t[4][kNS];
+    for (int j = 0; j < kNS; ++j) {
+      cvo_tuning one{set_lanes[j], set_lanes[j], 0.0f, xcd, set_scale[j], set_scale[j], 0, 0.0f, 0};
+      KConsts kv = kc;
+      apply_tuning(one, kv);
+      kv.max_group = 16;  // the problems' block ranges
+      for (int i = 0; i < n_samples; ++i) {
+        launch_init_state(pr.dev, 1, as<float>(ctx->b_tmp2), 16, kv, ells[i], p.nearest_neighbors_max, st);
+        launch_assoc(pr.dev, 1, pr.total_blocks, kv, st);  // warm-up
+        CVO_HIP(ctx, hipGetLastError());
+        float best_ms = 1e30f;
+        for (int rd = 0; rd < kRounds; ++rd) {
+          CVO_HIP(ctx, hipEventRecord(e0, st));
+          for (int r = 0; r < kReps; ++r) launch_assoc(pr.dev, 1, pr.total_blocks, kv, st);
+          CVO_HIP(ctx, hipEventRecord(e1, st));
+          CVO_HIP(ctx, hipEventSynchronize(e1));
+          float ms = 0.0f;
+          CVO_HIP(ctx, hipEventElapsedTime(&ms, e0, e1));
+          best_ms = std::min(best_ms, ms / kReps);
+        }
+        t[i][j] = 1e3 * best_ms;
+        ++n_tried;
+      }
+    }
+    // two-segment policy: samples [0, k) take one setting, [k, n) another;
+    // k = 0 or n is a single setting
+    for (int k = 0; k <= n_samples; ++k) {
+      double seg[2] = {0.0, 0.0};
+      int sel[2] = {0, 0};
+      for (int part = 0; part < 2; ++part) {
+        const int i0 = part == 0 ? 0 : k, i1 = part == 0 ? k : n_samples;
+        if (i0 == i1) continue;
+        seg[part] = 1e300;
+        for (int j = 0; j < kNS; ++j) {
+          double v = 0.0;
+          for (int i = i0; i < i1; ++i) v += w[i] * t[i][j];
+          if (v < seg[part]) {
+            seg[part] = v;
+            sel[part] = j;
           }
-          score += w[i] * 1e3 * best_ms;
-          wsum += w[i];
+        }
+      }
+      const double score = (seg[0] + seg[1]) / wsum;
+      if (score < best_score) {
+        best_score = score;
+        best.xcd_remap = xcd;
+        const bool single = k == 0 || k == n_samples || sel[0] == sel[1];
+        const int jb = (k == 0) ? sel[1] : sel[0], js = single ? jb : sel[1];
+        best.lanes_per_row = set_lanes[jb];
+        best.cell_scale = set_scale[jb];
+        best.lanes_small_ell = set_lanes[js];
+        best.cell_scale_small_ell = set_scale[js];
+        best.split_ell = single ? 0.0f : std::sqrt(ells[k - 1] * ells[k]);
+      }
+    }
+    if (verbose) {
+      for (int i = 0; i < n_samples; ++i) {
+        std::string line;
+        for (int j = 0; j < kNS; ++j) {
           char b[64];
-          std::snprintf(b, sizeof(b), " ell %.4g: %.2f us", (double)ells[i], 1e3 * best_ms);
+          std::snprintf(b, sizeof(b), " %d/%.1f %.2f", set_lanes[j], (double)set_scale[j], t[i][j]);
           line += b;
         }
-        score /= wsum;
-        if (verbose)
-          std::fprintf(stderr, "[cvo_autotune] lanes %d xcd %d scale %.2f:%s -> %.2f us\n", grp, xcd, (double)sc,
-                       line.c_str(), score);
-        ++n_tried;
-        if (score < best_score) {
-          best_score = score;
-          best.lanes_per_row = grp;
-          best.xcd_remap = xcd;
-          best.cell_scale = sc;
-        }
+        std::fprintf(stderr, "[cvo_autotune] xcd %d ell %.4g (weight %.0f) lanes/scale us:%s\n", xcd,
+                     (double)ells[i], w[i], line.c_str());
       }
     }
   }
   best.source = CVO_TUNE_MEASURED;
   best.score_us = (float)best_score;
   best.n_tried = n_tried;
+  if (verbose)
+    std::fprintf(stderr,
+                 "[cvo_autotune] chosen: lanes %d scale %.2f, below ell %.4g lanes %d scale %.2f; xcd %d; %.2f us\n",
+                 best.lanes_per_row, (double)best.cell_scale, (double)best.split_ell, best.lanes_small_ell,
+                 (double)best.cell_scale_small_ell, best.xcd_remap, best_score);
   ctx->tuning = best;
   if (cache_path) {
     cvo_status s = cvo_tuning_cache_store(cache_path, key, &best);

@@ -98,9 +98,19 @@ def tuning_cache_lookup(path: str, key: str) -> Optional[dict]:
     return t.as_dict()
 
 
+def _tuning(lanes_per_row, xcd_remap, cell_scale, lanes_small_ell, split_ell, cell_scale_small_ell, source=1,
+            score_us=0.0, n_tried=0) -> Tuning:
+    small = lanes_per_row if lanes_small_ell is None else lanes_small_ell
+    scale_small = cell_scale if cell_scale_small_ell is None else cell_scale_small_ell
+    return Tuning(int(lanes_per_row), int(small), float(split_ell), int(xcd_remap), float(cell_scale),
+                  float(scale_small), int(source), float(score_us), int(n_tried))
+
+
 def tuning_cache_store(path: str, key: str, lanes_per_row: int, xcd_remap: int, cell_scale: float,
-                       score_us: float = 0.0, n_tried: int = 0) -> None:
-    t = Tuning(int(lanes_per_row), int(xcd_remap), float(cell_scale), 1, float(score_us), int(n_tried))
+                       lanes_small_ell: Optional[int] = None, split_ell: float = 0.0,
+                       cell_scale_small_ell: Optional[float] = None, score_us: float = 0.0, n_tried: int = 0) -> None:
+    t = _tuning(lanes_per_row, xcd_remap, cell_scale, lanes_small_ell, split_ell, cell_scale_small_ell,
+                score_us=score_us, n_tried=n_tried)
     _check(lib().cvo_tuning_cache_store(path.encode(), key.encode(), C.byref(t)))
 
 
@@ -427,9 +437,19 @@ class CvoGPU:
         _check(lib().cvo_get_tuning(self._ctx, C.byref(t)), self._ctx)
         return t.as_dict()
 
-    def set_tuning(self, lanes_per_row: int = 16, xcd_remap: int = 0, cell_scale: float = 1.0) -> None:
-        t = Tuning(int(lanes_per_row), int(xcd_remap), float(cell_scale), 1, 0.0, 0)
+    def set_tuning(self, lanes_per_row: int = 16, xcd_remap: int = 1, cell_scale: float = 1.0,
+                   lanes_small_ell: Optional[int] = None, split_ell: float = 0.0,
+                   cell_scale_small_ell: Optional[float] = None) -> None:
+        """Association launch shape: lanes_per_row (8 or 16) and cell_scale at
+        lengthscales >= split_ell, lanes_small_ell / cell_scale_small_ell below
+        it (split_ell 0: one setting)."""
+        t = _tuning(lanes_per_row, xcd_remap, cell_scale, lanes_small_ell, split_ell, cell_scale_small_ell)
         _check(lib().cvo_set_tuning(self._ctx, C.byref(t)), self._ctx)
+
+    def apply_tuning(self, t: dict) -> None:
+        """Sets a tuning dict as returned by autotune / get_tuning."""
+        self.set_tuning(t["lanes_per_row"], t["xcd_remap"], t["cell_scale"], lanes_small_ell=t["lanes_small_ell"],
+                        split_ell=t["split_ell"], cell_scale_small_ell=t["cell_scale_small_ell"])
 
     def autotune(self, source, target, init_guess_transform=None, cache: Optional[str] = "default") -> dict:
         """Times every association launch shape on this pair and keeps the

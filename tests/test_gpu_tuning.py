@@ -1,11 +1,12 @@
 """The association-kernel autotuner: every launch shape gives the same pairs.
 
-The knobs (lanes per row, XCD-contiguous row blocks, grid cell scale) change
-only how candidates are enumerated and in which order a row's pairs are summed,
-so the bars are: association CSR (sorted by target id) bit-identical across all
-12 settings; inner products equal to 1e-12 relative (FP64 sum order); align
-poses within 1e-5 rad / m of the default setting and of the oracle's policy
-mode; BA poses within 1e-6 of the oracle after one outer iteration.
+The knobs (lanes per row per lengthscale range, XCD-contiguous row blocks,
+grid cell scale) change only how candidates are enumerated and in which order
+a row's pairs are summed, so the bars are: association CSR (sorted by target
+id) bit-identical across all 12 fixed settings; inner products equal to 1e-12
+relative (FP64 sum order); align poses, with lane counts that switch during
+the annealing, within 1e-5 rad / m of the default setting and of the oracle's
+policy mode; BA poses within 1e-9 of the oracle after one outer iteration.
 """
 import itertools
 import os
@@ -62,13 +63,22 @@ def test_settings_align_to_the_same_pose(ctx, oracle):
     T0 = np.eye(4, dtype=np.float32)
     T_pol = oracle.align(p, src, tgt, T0, precision=oracle.POLICY)[0]
     poses = {}
-    for lanes, xcd, scale in [(16, 0, 1.0), (8, 1, 0.5), (16, 1, 0.7), (8, 0, 1.0)]:
-        say(f"align 5k lanes={lanes} xcd={xcd} scale={scale}")
-        ctx.set_tuning(lanes, xcd, scale)
+    # (lanes, xcd, scale, lanes below split, split, scale below split): 0.15
+    # splits the 0.5 -> 0.05 annealing, so lanes and cell scale change inside
+    # the captured graph
+    for lanes, xcd, scale, small, split, ssmall in [(16, 0, 1.0, 16, 0.0, 1.0), (8, 1, 0.5, 8, 0.0, 0.5),
+                                                    (16, 1, 0.7, 8, 0.15, 1.0), (8, 0, 1.0, 16, 0.15, 0.5)]:
+        say(f"align 5k lanes={lanes}/{small} scale={scale}/{ssmall} split={split} xcd={xcd}")
+        ctx.set_tuning(lanes, xcd, scale, lanes_small_ell=small, split_ell=split, cell_scale_small_ell=ssmall)
+        ctx.set_trace(p.MAX_ITER)
         ret, T, secs, iters, _ = ctx.align(src, tgt, T0)
-        poses[(lanes, xcd, scale)] = T
-        assert rot_dist_rad(T, T_pol) <= 1e-5 and trans_err(T, T_pol) <= 1e-5, (lanes, xcd, scale)
-    base = poses[(16, 0, 1.0)]
+        ells = [r.ell for r in ctx.get_trace()]
+        if split > 0:
+            assert max(ells) > split > min(ells)
+        poses[(lanes, xcd, scale, small, split, ssmall)] = T
+        assert rot_dist_rad(T, T_pol) <= 1e-5 and trans_err(T, T_pol) <= 1e-5, (lanes, scale, small, split)
+    ctx.set_trace(0)
+    base = poses[(16, 0, 1.0, 16, 0.0, 1.0)]
     for k, T in poses.items():
         assert rot_dist_rad(T, base) <= 1e-5 and trans_err(T, base) <= 1e-5, k
 
@@ -81,8 +91,11 @@ def test_autotune_measures_caches_and_applies(ctx, tmp_path):
     cache = str(tmp_path / "sub" / "tune.json")
     say("autotune street20k (measure)")
     t = ctx.autotune(ds, dt, np.eye(4), cache=cache)
-    assert t["source"] == "measured" and t["n_tried"] == 12
-    assert t["lanes_per_row"] in (8, 16) and t["xcd_remap"] in (0, 1) and t["cell_scale"] in (1.0, 0.7, 0.5)
+    # 2 block orders x 6 (lanes, scale) settings x 4 lengthscales
+    assert t["source"] == "measured" and t["n_tried"] == 48
+    assert t["lanes_per_row"] in (8, 16) and t["lanes_small_ell"] in (8, 16) and t["split_ell"] >= 0
+    assert t["xcd_remap"] in (0, 1) and t["cell_scale"] in (1.0, 0.7, 0.5)
+    assert t["cell_scale_small_ell"] in (1.0, 0.7, 0.5)
     assert t["score_us"] > 0
     assert ctx.get_tuning()["lanes_per_row"] == t["lanes_per_row"]
     key = ctx.tuning_key(ds, dt)
@@ -96,9 +109,9 @@ def test_autotune_measures_caches_and_applies(ctx, tmp_path):
         d2s, d2t = c2.upload(src), c2.upload(tgt)
         say("autotune street20k (cached)")
         t2 = c2.autotune(d2s, d2t, np.eye(4), cache=cache)
-        assert t2["source"] == "cached" and t2["n_tried"] == 12
-        assert (t2["lanes_per_row"], t2["xcd_remap"], t2["cell_scale"]) == (
-            t["lanes_per_row"], t["xcd_remap"], t["cell_scale"])
+        assert t2["source"] == "cached" and t2["n_tried"] == 48
+        keys = ("lanes_per_row", "lanes_small_ell", "split_ell", "xcd_remap", "cell_scale", "cell_scale_small_ell")
+        assert all(t2[k] == t[k] for k in keys), (t, t2)
     finally:
         c2.close()
     # the tuned context registers like the default one
@@ -107,12 +120,21 @@ def test_autotune_measures_caches_and_applies(ctx, tmp_path):
 
 
 def test_set_tuning_rejects_bad_values(ctx):
-    for bad in [(4, 0, 1.0), (16, 2, 1.0), (16, 0, 0.3), (8, 1, 1.5), (8, 1, float("nan"))]:
+    for bad in [(4, 0, 1.0), (32, 0, 1.0), (16, 2, 1.0), (16, 0, 0.3), (8, 1, 1.5), (8, 1, float("nan"))]:
         with pytest.raises(CvoError):
             ctx.set_tuning(*bad)
-    ctx.set_tuning(8, 1, 0.5)
+    with pytest.raises(CvoError):
+        ctx.set_tuning(16, 1, 1.0, lanes_small_ell=4, split_ell=0.1)
+    with pytest.raises(CvoError):
+        ctx.set_tuning(16, 1, 1.0, lanes_small_ell=8, split_ell=float("inf"))
+    with pytest.raises(CvoError):
+        ctx.set_tuning(16, 1, 1.0, lanes_small_ell=8, split_ell=0.1, cell_scale_small_ell=0.2)
     t = ctx.get_tuning()
-    assert (t["lanes_per_row"], t["xcd_remap"], t["cell_scale"], t["source"]) == (8, 1, 0.5, "set")
+    assert t["source"] == "default" and (t["lanes_per_row"], t["xcd_remap"], t["cell_scale"]) == (16, 1, 1.0)
+    ctx.set_tuning(8, 1, 0.5, lanes_small_ell=16, split_ell=0.2)
+    t = ctx.get_tuning()
+    assert (t["lanes_per_row"], t["lanes_small_ell"], t["xcd_remap"], t["cell_scale"], t["source"]) == (
+        8, 16, 1, 0.5, "set")
 
 
 def test_ba_with_tuned_association_matches_oracle(oracle):
@@ -123,8 +145,8 @@ def test_ba_with_tuned_association_matches_oracle(oracle):
     c = CvoGPU(os.path.join(CONFIGS, "ba_window.yaml"))
     try:
         c.write_params(p)
-        c.set_tuning(8, 1, 0.5)
-        say("BA window lanes=8 xcd=1 scale=0.5, one outer iteration")
+        c.set_tuning(16, 1, 0.5, lanes_small_ell=8, split_ell=0.3, cell_scale_small_ell=1.0)
+        say("BA window lanes=16/8 scale=0.5/1.0 split=0.3 xcd=1, one outer iteration")
         g = c.align_multiframe(clouds, inits, [1, 0, 0, 0], all_edges(4))
     finally:
         c.close()
