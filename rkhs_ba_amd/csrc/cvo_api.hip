@@ -37,9 +37,7 @@ void launch_morton(const float* xyz, int n, const float* bmin, const float* bmax
 void launch_export(const AssocProblem* P_dev, int n_src, int64_t* counts_orig, const int64_t* row_ptr, int32_t* col,
                    float* val, int phase, hipStream_t s);
 void launch_all_done(const AssocProblem* probs, int n_probs, int32_t* out, hipStream_t s);
-void launch_pass_totals(const AssocProblem* probs, int n_probs, hipStream_t s);
 void launch_rescore(const AssocProblem* probs, int n_probs, const KConsts& kc, float ell, int K, hipStream_t s);
-void launch_controller(const AssocProblem* probs, int n_probs, const KConsts& kc, hipStream_t s);
 
 cvo_status set_err(cvo_ctx* ctx, cvo_status s, const std::string& msg) {
   if (ctx) ctx->last_error = msg;
@@ -336,6 +334,11 @@ cvo_status setup_problems(cvo_ctx* ctx, const cvo_dcloud* s, const cvo_dcloud* t
   CVO_TRY(ensure(ctx, ctx->b_part_ovf, sizeof(double) * 10 * (size_t)kOvfBlocks * n_probs));
   CVO_TRY(ensure(ctx, ctx->b_part_s, sizeof(double) * 4 * (size_t)nb * n_probs));
   CVO_TRY(ensure(ctx, ctx->b_part_cnt, sizeof(uint32_t) * 2 * (size_t)nb * n_probs));
+  const int ng = (nb + kReduceGroup - 1) / kReduceGroup;
+  CVO_TRY(ensure(ctx, ctx->b_part_grp, sizeof(double) * 10 * (size_t)ng * n_probs));
+  CVO_TRY(ensure(ctx, ctx->b_grp_cnt, sizeof(uint32_t) * (size_t)ng * n_probs));
+  // group counters start at zero; every pass leaves them at zero
+  CVO_HIP(ctx, hipMemsetAsync(ctx->b_grp_cnt.p, 0, sizeof(uint32_t) * (size_t)ng * n_probs, ctx->stream));
   pr.trace_cap = trace_cap;
   if (trace_cap > 0) {
     CVO_TRY(ensure(ctx, ctx->b_trace, sizeof(cvo_iter_record) * (size_t)trace_cap * n_probs));
@@ -374,6 +377,8 @@ cvo_status setup_problems(cvo_ctx* ctx, const cvo_dcloud* s, const cvo_dcloud* t
     P.part_ovf = as<double>(ctx->b_part_ovf) + (size_t)q * kOvfBlocks * 10;
     P.part_s = as<double>(ctx->b_part_s) + (size_t)q * nb * 4;
     P.part_cnt = as<uint32_t>(ctx->b_part_cnt) + (size_t)q * nb * 2;
+    P.part_grp = as<double>(ctx->b_part_grp) + (size_t)q * ng * 10;
+    P.grp_cnt = as<uint32_t>(ctx->b_grp_cnt) + (size_t)q * ng;
     P.trace = pr.trace ? pr.trace + (size_t)q * trace_cap : nullptr;
   }
   pr.dev = as<AssocProblem>(ctx->b_probs);
@@ -438,11 +443,9 @@ static cvo_status run_loop(cvo_ctx* ctx, const Problems& pr, const KConsts& kc, 
       launch_assoc(pr.dev, pr.n_probs, pr.total_blocks, kc, st);
       CVO_HIP(ctx, hipEventRecord(e[1], st));
       launch_overflow(pr.dev, pr.n_probs, kc, st);
-      if (!with_step) launch_pass_totals(pr.dev, pr.n_probs, st);
       CVO_HIP(ctx, hipEventRecord(e[2], st));
       if (with_step) {
         launch_step(pr.dev, pr.n_probs, pr.max_rows, kc, st);
-        launch_controller(pr.dev, pr.n_probs, kc, st);
       }
       CVO_HIP(ctx, hipEventRecord(e[3], st));
       bool all = true;
@@ -462,7 +465,6 @@ static cvo_status run_loop(cvo_ctx* ctx, const Problems& pr, const KConsts& kc, 
   } else if (!with_step) {
     launch_assoc(pr.dev, pr.n_probs, pr.total_blocks, kc, st);
     launch_overflow(pr.dev, pr.n_probs, kc, st);
-    launch_pass_totals(pr.dev, pr.n_probs, st);
     CVO_HIP(ctx, hipGetLastError());
   } else {
     // one chunk of iterations, captured in a graph unless CVO_NO_GRAPH is set
@@ -472,7 +474,6 @@ static cvo_status run_loop(cvo_ctx* ctx, const Problems& pr, const KConsts& kc, 
         launch_assoc(pr.dev, pr.n_probs, pr.total_blocks, kc, st);
         launch_overflow(pr.dev, pr.n_probs, kc, st);
         launch_step(pr.dev, pr.n_probs, pr.max_rows, kc, st);
-        launch_controller(pr.dev, pr.n_probs, kc, st);
       }
     };
     const char* ng = std::getenv("CVO_NO_GRAPH");
@@ -662,7 +663,8 @@ extern "C" void cvo_destroy(cvo_ctx* ctx) {
   DevBuf* bufs[] = {&ctx->b_src_pos, &ctx->b_src_feat, &ctx->b_src_sem, &ctx->b_src_geo, &ctx->b_src_key,
                     &ctx->b_src_perm, &ctx->b_grid_store, &ctx->b_pairs, &ctx->b_counts, &ctx->b_ovf,
                     &ctx->b_state, &ctx->b_probs, &ctx->b_gridhdr, &ctx->b_part_a, &ctx->b_part_ovf,
-                    &ctx->b_part_s, &ctx->b_part_cnt, &ctx->b_trace, &ctx->b_tmp0, &ctx->b_tmp1,
+                    &ctx->b_part_s, &ctx->b_part_cnt, &ctx->b_part_grp, &ctx->b_grp_cnt, &ctx->b_trace,
+                    &ctx->b_tmp0, &ctx->b_tmp1,
                     &ctx->b_tmp2, &ctx->b_tmp3, &ctx->b_cub, &ctx->b_export};
   for (DevBuf* b : bufs) free_buf(*b);
   for (auto& b : ctx->b_ba) free_buf(b);

@@ -6,8 +6,8 @@
 // One IRLS outer iteration is one replay of a captured graph:
 //   ba_prepare        per-edge maps (world transforms of both frames), cap K,
 //                     grid level
-//   assoc / overflow  per-edge association between the frames' world points
-//   pass_totals       per-edge stored pairs and largest row
+//   assoc / overflow  per-edge association between the frames' world points,
+//                     with per-edge stored pairs and largest row
 //   ba_edges          edge on/off; stop when no edge has enough pairs
 //   (ba_pair -> ba_edge_reduce -> ba_solve) x (iterations_per_solve + 1)
 //                     Levenberg-Marquardt over the fixed pairs
@@ -660,7 +660,7 @@ static T* as(DevBuf& b) {
 
 enum {
   BA_PAIRS = 0, BA_COUNTS, BA_OVF, BA_STATE, BA_PROBS, BA_PART_A, BA_PART_OVF, BA_PART_CNT,
-  BA_DEV, BA_EDGES, BA_H, BA_HN, BA_PAIR_PART, BA_CONTRIB, BA_FLAG
+  BA_DEV, BA_EDGES, BA_H, BA_HN, BA_PAIR_PART, BA_CONTRIB, BA_FLAG, BA_PART_GRP, BA_GRP_CNT
 };
 
 // Six buffers per frame: sorted source (pos, feat, sem, geo), grid store and
@@ -703,7 +703,6 @@ static cvo_status validate_frames(cvo_ctx* ctx, int n_frames, const cvo_dcloud* 
 
 void launch_assoc(const AssocProblem* probs, int n_probs, int total_blocks, const KConsts& kc, hipStream_t s);
 void launch_overflow(const AssocProblem* probs, int n_probs, const KConsts& kc, hipStream_t s);
-void launch_pass_totals(const AssocProblem* probs, int n_probs, hipStream_t s);
 
 struct BaLaunch {
   BaDev* ba;
@@ -721,7 +720,6 @@ static void ba_assoc_phase(const BaLaunch& L, hipStream_t s) {
   if (L.n_edges > 0) {
     launch_assoc(L.probs, L.n_edges, L.total_blocks, L.kc, s);
     launch_overflow(L.probs, L.n_edges, L.kc, s);
-    launch_pass_totals(L.probs, L.n_edges, s);
   }
   hipLaunchKernelGGL(ba_edges_kernel, dim3(1), dim3(256), 0, s, L.ba, L.edges, L.probs);
 }
@@ -808,6 +806,11 @@ extern "C" cvo_status cvo_align_multiframe(cvo_ctx* ctx, int32_t n_frames, const
   CVO_TRY(ensure(ctx, B[BA_PART_A], sizeof(double) * 8 * (size_t)std::max(total_blocks, 1)));
   CVO_TRY(ensure(ctx, B[BA_PART_OVF], sizeof(double) * 10 * kOvfBlocks * E1));
   CVO_TRY(ensure(ctx, B[BA_PART_CNT], sizeof(uint32_t) * 2 * (size_t)std::max(total_blocks, 1)));
+  // first-stage reduction groups: at most total_blocks / kReduceGroup + one per edge
+  const size_t n_grp = (size_t)total_blocks / kReduceGroup + E1;
+  CVO_TRY(ensure(ctx, B[BA_PART_GRP], sizeof(double) * 10 * n_grp));
+  CVO_TRY(ensure(ctx, B[BA_GRP_CNT], sizeof(uint32_t) * n_grp));
+  CVO_HIP(ctx, hipMemsetAsync(B[BA_GRP_CNT].p, 0, sizeof(uint32_t) * n_grp, st));
   CVO_TRY(ensure(ctx, B[BA_DEV], sizeof(BaDev)));
   CVO_TRY(ensure(ctx, B[BA_EDGES], sizeof(BaEdge) * E1));
   CVO_TRY(ensure(ctx, B[BA_H], sizeof(double) * 2 * kBaMaxDof * kBaMaxDof));
@@ -819,6 +822,7 @@ extern "C" cvo_status cvo_align_multiframe(cvo_ctx* ctx, int32_t n_frames, const
   std::vector<AssocProblem> hp(E1);
   std::vector<BaEdge> he(E1);
   int blk = 0;
+  size_t grp_off = 0;
   for (int e = 0; e < n_edges; ++e) {
     const int f1 = edges[2 * e], f2 = edges[2 * e + 1];
     const cvo_dcloud* s = frames[f1];
@@ -851,6 +855,9 @@ extern "C" cvo_status cvo_align_multiframe(cvo_ctx* ctx, int32_t n_frames, const
     P.part_ovf = as<double>(B[BA_PART_OVF]) + (size_t)e * kOvfBlocks * 10;
     P.part_s = nullptr;
     P.part_cnt = as<uint32_t>(B[BA_PART_CNT]) + (size_t)blk * 2;
+    P.part_grp = as<double>(B[BA_PART_GRP]) + (size_t)grp_off * 10;
+    P.grp_cnt = as<uint32_t>(B[BA_GRP_CNT]) + grp_off;
+    grp_off += (size_t)(nb + kReduceGroup - 1) / kReduceGroup;
     blk += nb;
     BaEdge& ed = he[e];
     std::memset(&ed, 0, sizeof(ed));
@@ -945,7 +952,6 @@ extern "C" cvo_status cvo_align_multiframe(cvo_ctx* ctx, int32_t n_frames, const
       (void)hipEventRecord(ev[2], st);
       if (n_edges > 0) {
         launch_overflow(L.probs, n_edges, kc, st);
-        launch_pass_totals(L.probs, n_edges, st);
       }
       hipLaunchKernelGGL(ba_edges_kernel, dim3(1), dim3(256), 0, st, L.ba, L.edges, L.probs);
       (void)hipEventRecord(ev[3], st);

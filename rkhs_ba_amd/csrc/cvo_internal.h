@@ -15,6 +15,7 @@ constexpr int kMaxLevels = 12;         // grid levels per target
 constexpr int kMaxWindow = 64;         // indicator window capacity
 constexpr int kBlock = 256;            // threads per block of the row kernels
 constexpr int kOvfBlocks = 96;         // capped-row kernel blocks per problem
+constexpr int kReduceGroup = 64;       // association blocks per first-stage reduction group
 constexpr uint32_t kMaxCells = 1u << 25;
 
 // One grid level over a target cloud (uniform dense grid, x-fastest cells).
@@ -42,7 +43,8 @@ struct alignas(16) ProblemState {
   // pose (FP64), R row-major: the reference's (R, T) of align_impl
   double R[9];
   double T[3];
-  // per-iteration accumulators (written by the last block of each pass)
+  // per-iteration totals of the association pass (raw flow sums, before the
+  // 1/c, 1/d scaling), written by the pass's last block
   double flow[6];
   double ip;
   double bcde[4];
@@ -99,9 +101,11 @@ struct AssocProblem {
   int32_t n_blocks;        // assoc/step blocks of this problem
   int32_t trace_cap;
   double* part_a;          // per block: flow6, ip, -> 8 doubles
-  double* part_ovf;        // per ovf block: 8 doubles
+  double* part_ovf;        // per capped-row block: flow6, ip, nnz, max row, - -> 10 doubles
   double* part_s;          // per block: 4 doubles
-  uint32_t* part_cnt;      // per block: nnz (lo,hi) and max row
+  uint32_t* part_cnt;      // per block: nnz and max row
+  double* part_grp;        // per group of kReduceGroup blocks: 10 doubles
+  uint32_t* grp_cnt;       // per group: blocks finished (zero between passes)
   cvo_iter_record* trace;  // per iteration (may be null)
 };
 
@@ -191,7 +195,7 @@ struct cvo_ctx {
   DevBuf b_grid_store;    // all levels' arrays
   DevBuf b_pairs, b_counts, b_ovf;
   DevBuf b_state, b_probs, b_gridhdr;
-  DevBuf b_part_a, b_part_ovf, b_part_s, b_part_cnt;
+  DevBuf b_part_a, b_part_ovf, b_part_s, b_part_cnt, b_part_grp, b_grp_cnt;
   DevBuf b_trace;
   DevBuf b_tmp0, b_tmp1, b_tmp2, b_tmp3, b_cub;
   DevBuf b_export;
@@ -200,7 +204,7 @@ struct cvo_ctx {
   hipEvent_t ev_a = nullptr, ev_b = nullptr;
   std::vector<hipEvent_t> ev_pool;
   // BA scratch
-  DevBuf b_ba[16];
+  DevBuf b_ba[20];
   std::vector<DevBuf> b_frames;  // per frame: sorted source (4) and grid (2)
   int32_t ba_last_accepted = 0, ba_last_evals = 0;
   int32_t pose_arith = 0;  // CVO_POSE_FP64 or CVO_POSE_REFERENCE
@@ -233,7 +237,6 @@ cvo_status build_grid_into(cvo_ctx* ctx, const cvo_dcloud* t, const std::vector<
                            DevBuf& b_store, DevBuf& b_hdr);
 void launch_assoc(const AssocProblem* probs, int n_probs, int total_blocks, const KConsts& kc, hipStream_t s);
 void launch_overflow(const AssocProblem* probs, int n_probs, const KConsts& kc, hipStream_t s);
-void launch_pass_totals(const AssocProblem* probs, int n_probs, hipStream_t s);
 void launch_init_state(const AssocProblem* probs, int n_probs, const float* T16, int stride16, const KConsts& kc,
                        float ell0, int K0, hipStream_t s);
 cvo_status prepare_source(cvo_ctx* ctx, const cvo_dcloud* s);

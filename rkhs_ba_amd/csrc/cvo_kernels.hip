@@ -38,6 +38,31 @@ __device__ __forceinline__ int xcd_block(int bid, int nblk) {
   return (x < r) ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
 }
 
+// Cross-block publication without cache maintenance.  Partials are written
+// with device-scope stores (sc1: written through to the device coherence
+// point) by lanes of wave 0, which then waits for its stores (vmcnt(0))
+// before lane 0 counts the block in with a device-scope atomic; the block
+// that counts last reads the partials with device-scope loads.  An agent
+// fence would instead write back and invalidate the whole XCD L2 in every
+// block, which costs the association kernel its cached target neighbourhoods.
+__device__ __forceinline__ void st_dev(double* p, double v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_dev(uint32_t* p, uint32_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double ld_dev(double* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t ld_dev(uint32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t count_in(uint32_t* counter) {
+  return __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// s_waitcnt vmcnt(0) (expcnt and lgkmcnt left at their maxima).
+__device__ __forceinline__ void wait_stores() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
 // Exact evaluation of a pair's kernel value with FP64 exponentials rounded to
 // float, the reference's rounding points (CvoGPU.cu:1020-1054).  Used only for
 // pairs whose fast FP32 value lies within 2e-5 relative of sp_thres, so the
@@ -372,8 +397,8 @@ __device__ __forceinline__ void assoc_row(const AssocProblem& P, const KConsts& 
 }
 
 // Block partials of the association pass: part_a[block] = flow6, ip, -;
-// part_cnt[block] = stored pairs, max row.  Summed by overflow_kernel (first
-// stage) and the step / controller kernels (second stage).
+// part_cnt[block] = stored pairs, max row; summed into the problem state by
+// the pass's last blocks (assoc_epilogue).
 
 // LDS of one association block, sized for the most rows a block can hold.
 template <int FC, int KC>
@@ -384,6 +409,8 @@ struct AssocLds {
   uint32_t cnt[kMaxRowsPerBlock];
   float feat[kMaxRowsPerBlock * FL];  // the block's source rows' features
   float sem[kMaxRowsPerBlock * KL];   // and class vectors
+  double red[10][kReduceGroup];       // first-stage reduction tree
+  int flag;
 };
 
 // Logical block lb of a problem with G lanes per row (256 / G rows).
@@ -449,15 +476,89 @@ __device__ __forceinline__ void assoc_block(const AssocProblem& P, const KConsts
   if (tid < 7) {
     double s = 0.0;
     for (int q = 0; q < RB; ++q) s += L.row[q][tid];
-    P.part_a[(size_t)lb * 8 + tid] = s;
+    st_dev(&P.part_a[(size_t)lb * 8 + tid], s);
   } else if (tid == 7) {
     uint32_t nz = 0, mx = 0;
     for (int q = 0; q < RB; ++q) {
       nz += L.cnt[q];
       mx = max(mx, L.cnt[q]);
     }
-    P.part_cnt[(size_t)lb * 2] = nz;
-    P.part_cnt[(size_t)lb * 2 + 1] = mx;
+    st_dev(&P.part_cnt[(size_t)lb * 2], nz);
+    st_dev(&P.part_cnt[(size_t)lb * 2 + 1], mx);
+  }
+}
+
+// Fixed-order tree over red[c][0..kReduceGroup): sums, except the max-row
+// column 8.  Every thread of the block takes part; red[c][0] holds the result.
+__device__ __forceinline__ void tree10(double (*red)[kReduceGroup]) {
+  const int tid = threadIdx.x;
+  for (int w = kReduceGroup / 2; w > 0; w >>= 1) {
+    __syncthreads();
+    for (int t = tid; t < 10 * w; t += kBlock) {
+      const int c = t / w, i = t - c * w;
+      red[c][i] = (c == 8) ? fmax(red[c][i], red[c][i + w]) : red[c][i] + red[c][i + w];
+    }
+  }
+  __syncthreads();
+}
+
+// First-stage reduction of an association pass by the blocks that finish
+// last: the last block of each group of kReduceGroup logical blocks sums the
+// group's partials in fixed order, and the block that completes the last
+// group sums the group totals (in fixed order) into the problem state: raw
+// flow sums, ip, stored pairs and the longest row.  Counters return to zero.
+// The block's partial was written by lanes 0-7 of wave 0 with st_dev.
+template <int FC, int KC>
+__device__ void assoc_epilogue(const AssocProblem& P, int n_act, int lb, AssocLds<FC, KC>& L) {
+  const int tid = threadIdx.x;
+  ProblemState* st = P.st;
+  const int grp = lb / kReduceGroup;
+  const int g0 = grp * kReduceGroup;
+  const int gsize = min(kReduceGroup, n_act - g0);
+  const int n_groups = (n_act + kReduceGroup - 1) / kReduceGroup;
+  if (tid < 64) {
+    wait_stores();
+    if (tid == 0) L.flag = (count_in(&P.grp_cnt[grp]) == (uint32_t)(gsize - 1)) ? 1 : 0;
+  }
+  __syncthreads();
+  if (!L.flag) return;  // uniform
+  for (int t = tid; t < 10 * kReduceGroup; t += kBlock) {
+    const int c = t / kReduceGroup, i = t - c * kReduceGroup;
+    double v = 0.0;
+    if (i < gsize) {
+      const int b = g0 + i;
+      if (c < 7) v = ld_dev(&P.part_a[(size_t)b * 8 + c]);
+      else if (c == 7) v = (double)ld_dev(&P.part_cnt[(size_t)b * 2]);
+      else if (c == 8) v = (double)ld_dev(&P.part_cnt[(size_t)b * 2 + 1]);
+    }
+    L.red[c][i] = v;
+  }
+  tree10(L.red);
+  if (tid < 64) {
+    if (tid < 10) st_dev(&P.part_grp[(size_t)grp * 10 + tid], L.red[tid][0]);
+    if (tid == 0) st_dev(&P.grp_cnt[grp], 0u);
+    wait_stores();
+    if (tid == 0) L.flag = (count_in(&st->blocks_done_assoc) == (uint32_t)(n_groups - 1)) ? 1 : 0;
+  }
+  __syncthreads();
+  if (!L.flag) return;  // uniform
+  for (int t = tid; t < 10 * kReduceGroup; t += kBlock) {
+    const int c = t / kReduceGroup, i = t - c * kReduceGroup;
+    double v = 0.0;
+    for (int q = i; q < n_groups; q += kReduceGroup) {
+      const double x = ld_dev(&P.part_grp[(size_t)q * 10 + c]);
+      v = (c == 8) ? fmax(v, x) : v + x;
+    }
+    L.red[c][i] = v;
+  }
+  tree10(L.red);
+  if (tid == 0) {
+    for (int c = 0; c < 6; ++c) st->flow[c] = L.red[c][0];
+    st->ip = L.red[6][0];
+    st->nnz = (uint64_t)L.red[7][0];
+    st->max_row = (uint32_t)L.red[8][0];
+    st->level_assoc = st->level;
+    st_dev(&st->blocks_done_assoc, 0u);
   }
 }
 
@@ -474,9 +575,10 @@ __device__ __forceinline__ int active_blocks(const AssocProblem& P, int group) {
 // at the largest lane count) and uses a prefix of them at its current lane
 // count.  With xcd_remap the active blocks are renumbered so that each
 // XCD holds one contiguous range of rows.
+// Four waves per SIMD: the register budget is 128 VGPRs.
 template <int FC, int KC>
-__global__ __launch_bounds__(kBlock) void assoc_kernel(const AssocProblem* __restrict__ probs, int n_probs,
-                                                       KConsts kc) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void assoc_kernel(
+    const AssocProblem* __restrict__ probs, int n_probs, KConsts kc) {
   const int p = find_problem(probs, n_probs, (int)blockIdx.x);
   const AssocProblem& P = probs[p];
   const ProblemState* st = P.st;
@@ -492,6 +594,7 @@ __global__ __launch_bounds__(kBlock) void assoc_kernel(const AssocProblem* __res
     case 8: assoc_block<FC, KC, 8>(P, kc, lb, L); break;
     default: assoc_block<FC, KC, 16>(P, kc, lb, L); break;
   }
+  assoc_epilogue<FC, KC>(P, n_act, lb, L);
 }
 
 // ---------------------------------------------------------------------------
@@ -504,43 +607,11 @@ template <int FC, int KC>
 __global__ __launch_bounds__(kBlock) void overflow_kernel(const AssocProblem* __restrict__ probs, int n_probs,
                                                           KConsts kc) {
   const AssocProblem& P = probs[blockIdx.y];
-  const ProblemState* st = P.st;
+  ProblemState* st = P.st;
   if (st->done) return;
+  if (st->n_overflow == 0) return;  // uniform: the association totals are final
   const int tid = threadIdx.x;
   const int mode = kc.mode;
-  // (a) this block's slice of the association partials: parallel loads, one
-  // fixed-order LDS tree over all ten columns
-  __shared__ double s_part[10];
-  __shared__ double s_tree[10][64];
-  {
-    const int n_act = active_blocks(P, state_group(st));
-    const int per = (n_act + kOvfBlocks - 1) / kOvfBlocks;
-    const int b0 = blockIdx.x * per, b1 = min(n_act, b0 + per);
-    for (int t = tid; t < 10 * 64; t += kBlock) {
-      const int c = t / 64, i = t - c * 64;
-      double v = 0.0;
-      for (int b = b0 + i; b < b1; b += 64) {
-        if (c < 7) v += P.part_a[(size_t)b * 8 + c];
-        else if (c == 7) v += (double)P.part_cnt[(size_t)b * 2];
-        else if (c == 8) v = fmax(v, (double)P.part_cnt[(size_t)b * 2 + 1]);
-      }
-      s_tree[c][i] = v;
-    }
-    __syncthreads();
-    for (int w = 32; w > 0; w >>= 1) {
-      for (int t = tid; t < 10 * w; t += kBlock) {
-        const int c = t / w, i = t - c * w;
-        s_tree[c][i] = (c == 8) ? fmax(s_tree[c][i], s_tree[c][i + w]) : s_tree[c][i] + s_tree[c][i + w];
-      }
-      __syncthreads();
-    }
-    if (tid < 10) s_part[tid] = s_tree[tid][0];
-  }
-  __syncthreads();
-  if (st->n_overflow == 0) {  // uniform
-    if (tid < 10) P.part_ovf[(size_t)blockIdx.x * 10 + tid] = s_part[tid];
-    return;
-  }
   const float ell = st->ell;
   const uint32_t K = (uint32_t)st->K;
   const uint32_t stamp = (uint32_t)st->iter + 1u;
@@ -635,12 +706,41 @@ __global__ __launch_bounds__(kBlock) void overflow_kernel(const AssocProblem* __
       __syncthreads();
     }
   }
+  // this block's capped rows, then the last block adds all blocks' in fixed
+  // order to the association totals
   double* out = P.part_ovf + (size_t)blockIdx.x * 10;
-  if (tid < 7) out[tid] = s_part[tid] + blk_acc[tid];
+  __shared__ int s_last;
+  if (tid < 64) {
+    if (tid < 7) st_dev(&out[tid], blk_acc[tid]);
+    if (tid == 7) {
+      st_dev(&out[7], (double)blk_nnz);
+      st_dev(&out[8], (double)blk_max);
+      st_dev(&out[9], 0.0);
+    }
+    wait_stores();
+    if (tid == 0) s_last = (count_in(&st->blocks_done_ovf) == (uint32_t)(gridDim.x - 1)) ? 1 : 0;
+  }
+  __syncthreads();
+  if (!s_last) return;  // uniform
+  __shared__ double s_tot[10][128];
+  for (int t = tid; t < 10 * 128; t += kBlock) {
+    const int c = t / 128, i = t - c * 128;
+    s_tot[c][i] = (i < (int)gridDim.x) ? ld_dev(&P.part_ovf[(size_t)i * 10 + c]) : 0.0;
+  }
+  for (int w = 64; w > 0; w >>= 1) {
+    __syncthreads();
+    for (int t = tid; t < 10 * w; t += kBlock) {
+      const int c = t / w, i = t - c * w;
+      s_tot[c][i] = (c == 8) ? fmax(s_tot[c][i], s_tot[c][i + w]) : s_tot[c][i] + s_tot[c][i + w];
+    }
+  }
+  __syncthreads();
   if (tid == 0) {
-    out[7] = s_part[7] + (double)blk_nnz;
-    out[8] = fmax(s_part[8], (double)blk_max);
-    out[9] = 0.0;
+    for (int c = 0; c < 6; ++c) st->flow[c] += s_tot[c][0];
+    st->ip += s_tot[6][0];
+    st->nnz += (uint64_t)s_tot[7][0];
+    st->max_row = max(st->max_row, (uint32_t)s_tot[8][0]);
+    st_dev(&st->blocks_done_ovf, 0u);
   }
 }
 
@@ -898,50 +998,83 @@ __device__ __forceinline__ void cross_f(float a0, float a1, float a2, float b0, 
   o2 = a0 * b1 - a1 * b0;
 }
 
-// Flow of the iteration from the first-stage partials (kOvfBlocks of them),
-// summed in fixed order by every block that needs it.
-__device__ __forceinline__ void reduce_stage1(const AssocProblem& P, double (*s_tree)[128], double out[10]) {
+// The pass's flow, omega / c and v / d (compute_flow_gpu_no_eigen,
+// CvoGPU.cu:2057-2060).
+__device__ __forceinline__ void scaled_flow(const ProblemState* st, const KConsts& kc, double tot[6]) {
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    tot[c] = st->flow[c] / (double)kc.c;
+    tot[3 + c] = st->flow[3 + c] / (double)kc.d;
+  }
+}
+
+// The controller step of a problem, run by the step pass's last block: step
+// coefficients summed over the step blocks in fixed order, then the scalar
+// controller on thread 0 with the state staged in LDS.
+__device__ void controller_tail(const AssocProblem& P, const KConsts& kc, const double tot[6], int nsb) {
+  ProblemState* st = P.st;
   const int tid = threadIdx.x;
-  for (int t = tid; t < 10 * 128; t += kBlock) {
-    const int c = t / 128, i = t - c * 128;
-    s_tree[c][i] = (i < kOvfBlocks) ? P.part_ovf[(size_t)i * 10 + c] : 0.0;
+  __shared__ ProblemState s_st;
+  __shared__ float s_h[kMaxLevels];
+  __shared__ int s_nl;
+  {
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(st);
+    uint32_t* dst = reinterpret_cast<uint32_t*>(&s_st);
+    for (int i = tid; i < (int)(sizeof(ProblemState) / 4); i += kBlock) dst[i] = src[i];
+    if (tid == 0) s_nl = P.grid->n_levels;
+    if (tid < kMaxLevels) s_h[tid] = P.grid->lv[tid].h;
+  }
+  // step partials: each thread sums a strided subset of blocks for all four
+  // coefficients, then one fixed-order tree
+  __shared__ double s_b[4][kBlock];
+  {
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int b = tid; b < nsb; b += kBlock)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) v[c] += ld_dev(&P.part_s[(size_t)b * 4 + c]);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) s_b[c][tid] = v[c];
   }
   __syncthreads();
-  for (int w = 64; w > 0; w >>= 1) {
-    for (int t = tid; t < 10 * w; t += kBlock) {
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    for (int t = tid; t < 4 * w; t += kBlock) {
       const int c = t / w, i = t - c * w;
-      s_tree[c][i] = (c == 8) ? fmax(s_tree[c][i], s_tree[c][i + w]) : s_tree[c][i] + s_tree[c][i + w];
+      s_b[c][i] += s_b[c][i + w];
     }
     __syncthreads();
   }
-#pragma unroll
-  for (int c = 0; c < 10; ++c) out[c] = s_tree[c][0];
+  if (tid == 0) {
+    for (int c = 0; c < 6; ++c) s_st.flow[c] = tot[c];
+    for (int c = 0; c < 4; ++c) s_st.bcde[c] = s_b[c][0];
+    controller(&s_st, kc, P, s_h, s_nl);
+  }
   __syncthreads();
-}
-
-// omega / c and v / d (compute_flow_gpu_no_eigen, CvoGPU.cu:2057-2060).
-__device__ __forceinline__ void scale_flow(double tot[10], const KConsts& kc) {
-#pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    tot[c] /= (double)kc.c;
-    tot[3 + c] /= (double)kc.d;
+  {
+    // the pass counters live at the device coherence point: never written
+    // back from the staged copy; this pass's counter returns to zero
+    constexpr int c0 = (int)(offsetof(ProblemState, blocks_done_assoc) / 4);
+    constexpr int c1 = (int)(offsetof(ProblemState, blocks_done_step) / 4) + 1;
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(&s_st);
+    uint32_t* dst = reinterpret_cast<uint32_t*>(st);
+    for (int i = tid; i < (int)(sizeof(ProblemState) / 4); i += kBlock)
+      if (i < c0 || i >= c1) dst[i] = src[i];
+    if (tid == 0) st_dev(&st->blocks_done_step, 0u);
   }
 }
 
-// One thread per source row; grid (row blocks, problems).
+// One thread per source row; grid (row blocks, problems).  The last block of
+// a problem runs its controller step.
 __global__ __launch_bounds__(kBlock) void step_kernel(const AssocProblem* __restrict__ probs, int n_probs,
                                                       KConsts kc) {
   const AssocProblem& P = probs[blockIdx.y];
-  const ProblemState* st = P.st;
+  ProblemState* st = P.st;
   if (st->done) return;
   const int tid = threadIdx.x;
   const int r = blockIdx.x * kBlock + tid;
   const int nsb = (P.n_src + kBlock - 1) / kBlock;
   if ((int)blockIdx.x >= nsb) return;  // uniform: blocks beyond this problem's rows
-  __shared__ double s_tree[10][128];
-  double tot[10];
-  reduce_stage1(P, s_tree, tot);
-  scale_flow(tot, kc);
+  double tot[6];
+  scaled_flow(st, kc, tot);
   const float ell = st->ell;
   const GridLevel& G = P.grid->lv[st->level];
   Maps mp;
@@ -1009,91 +1142,19 @@ __global__ __launch_bounds__(kBlock) void step_kernel(const AssocProblem* __rest
     if (lane == 0) s_w[c][wave] = v;
   }
   __syncthreads();
-  if (tid < 4) {
-    double v = 0.0;
-    for (int w = 0; w < kBlock / 64; ++w) v += s_w[tid][w];
-    P.part_s[(size_t)blockIdx.x * 4 + tid] = v;
-  }
-}
-
-// One block per problem: totals of the iteration (flow, counters, step
-// coefficients) in fixed order, then the scalar controller on thread 0 with
-// the state staged in LDS.
-__global__ __launch_bounds__(kBlock) void controller_kernel(const AssocProblem* __restrict__ probs, KConsts kc) {
-  const AssocProblem& P = probs[blockIdx.x];
-  ProblemState* st = P.st;
-  if (st->done) return;
-  const int tid = threadIdx.x;
-  __shared__ ProblemState s_st;
-  __shared__ float s_h[kMaxLevels];
-  __shared__ int s_nl;
-  // stage the state
-  {
-    const uint32_t* src = reinterpret_cast<const uint32_t*>(st);
-    uint32_t* dst = reinterpret_cast<uint32_t*>(&s_st);
-    for (int i = tid; i < (int)(sizeof(ProblemState) / 4); i += kBlock) dst[i] = src[i];
-    if (tid == 0) s_nl = P.grid->n_levels;
-    if (tid < kMaxLevels) s_h[tid] = P.grid->lv[tid].h;
-  }
-  __shared__ double s_tree[10][128];
-  double tot[10];
-  reduce_stage1(P, s_tree, tot);
-  scale_flow(tot, kc);
-  // step partials: each thread sums a strided subset of blocks for all four
-  // coefficients, then one fixed-order tree
-  const int nsb = (P.n_src + kBlock - 1) / kBlock;
-  __shared__ double s_b[4][kBlock];
-  {
-    double v[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int b = tid; b < nsb; b += kBlock)
-#pragma unroll
-      for (int c = 0; c < 4; ++c) v[c] += P.part_s[(size_t)b * 4 + c];
-#pragma unroll
-    for (int c = 0; c < 4; ++c) s_b[c][tid] = v[c];
-  }
-  __syncthreads();
-  for (int w = kBlock / 2; w > 0; w >>= 1) {
-    for (int t = tid; t < 4 * w; t += kBlock) {
-      const int c = t / w, i = t - c * w;
-      s_b[c][i] += s_b[c][i + w];
+  __shared__ int s_last;
+  if (tid < 64) {
+    if (tid < 4) {
+      double v = 0.0;
+      for (int w = 0; w < kBlock / 64; ++w) v += s_w[tid][w];
+      st_dev(&P.part_s[(size_t)blockIdx.x * 4 + tid], v);
     }
-    __syncthreads();
-  }
-  double bcde[4];
-#pragma unroll
-  for (int c = 0; c < 4; ++c) bcde[c] = s_b[c][0];
-  if (tid == 0) {
-    for (int c = 0; c < 6; ++c) s_st.flow[c] = tot[c];
-    s_st.ip = tot[6];
-    s_st.nnz = (uint64_t)tot[7];
-    s_st.max_row = (uint32_t)tot[8];
-    s_st.level_assoc = s_st.level;
-    for (int c = 0; c < 4; ++c) s_st.bcde[c] = bcde[c];
-    controller(&s_st, kc, P, s_h, s_nl);
+    wait_stores();
+    if (tid == 0) s_last = (count_in(&st->blocks_done_step) == (uint32_t)(nsb - 1)) ? 1 : 0;
   }
   __syncthreads();
-  {
-    const uint32_t* src = reinterpret_cast<const uint32_t*>(&s_st);
-    uint32_t* dst = reinterpret_cast<uint32_t*>(st);
-    for (int i = tid; i < (int)(sizeof(ProblemState) / 4); i += kBlock) dst[i] = src[i];
-  }
-}
-
-// Totals of a single association pass (inner product / association export):
-// one block per problem.
-__global__ __launch_bounds__(kBlock) void pass_totals_kernel(const AssocProblem* __restrict__ probs) {
-  const AssocProblem& P = probs[blockIdx.x];
-  ProblemState* st = P.st;
-  __shared__ double s_tree[10][128];
-  double tot[10];
-  reduce_stage1(P, s_tree, tot);
-  if (threadIdx.x == 0) {
-    for (int c = 0; c < 6; ++c) st->flow[c] = tot[c];
-    st->ip = tot[6];
-    st->nnz = (uint64_t)tot[7];
-    st->max_row = (uint32_t)tot[8];
-    st->level_assoc = st->level;
-  }
+  if (!s_last) return;  // uniform
+  controller_tail(P, kc, tot, nsb);
 }
 
 // Initial state of a problem: pose from a row-major float 4x4, maps, level.
@@ -1278,13 +1339,6 @@ void launch_rescore(const AssocProblem* probs, int n_probs, const KConsts& kc, f
   hipLaunchKernelGGL(rescore_state_kernel, dim3((n_probs + 63) / 64), dim3(64), 0, s, probs, n_probs, kc, ell, K);
 }
 
-void launch_pass_totals(const AssocProblem* probs, int n_probs, hipStream_t s) {
-  hipLaunchKernelGGL(pass_totals_kernel, dim3(n_probs), dim3(kBlock), 0, s, probs);
-}
-
-void launch_controller(const AssocProblem* probs, int n_probs, const KConsts& kc, hipStream_t s) {
-  hipLaunchKernelGGL(controller_kernel, dim3(n_probs), dim3(kBlock), 0, s, probs, kc);
-}
 
 void launch_init_state(const AssocProblem* probs, int n_probs, const float* T16, int stride16, const KConsts& kc,
                        float ell0, int K0, hipStream_t s) {
