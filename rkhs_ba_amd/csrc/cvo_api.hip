@@ -334,11 +334,6 @@ cvo_status setup_problems(cvo_ctx* ctx, const cvo_dcloud* s, const cvo_dcloud* t
   CVO_TRY(ensure(ctx, ctx->b_part_ovf, sizeof(double) * 10 * (size_t)kOvfBlocks * n_probs));
   CVO_TRY(ensure(ctx, ctx->b_part_s, sizeof(double) * 4 * (size_t)nb * n_probs));
   CVO_TRY(ensure(ctx, ctx->b_part_cnt, sizeof(uint32_t) * 2 * (size_t)nb * n_probs));
-  const int ng = (nb + kReduceGroup - 1) / kReduceGroup;
-  CVO_TRY(ensure(ctx, ctx->b_part_grp, sizeof(double) * 10 * (size_t)ng * n_probs));
-  CVO_TRY(ensure(ctx, ctx->b_grp_cnt, sizeof(uint32_t) * (size_t)ng * n_probs));
-  // group counters start at zero; every pass leaves them at zero
-  CVO_HIP(ctx, hipMemsetAsync(ctx->b_grp_cnt.p, 0, sizeof(uint32_t) * (size_t)ng * n_probs, ctx->stream));
   pr.trace_cap = trace_cap;
   if (trace_cap > 0) {
     CVO_TRY(ensure(ctx, ctx->b_trace, sizeof(cvo_iter_record) * (size_t)trace_cap * n_probs));
@@ -377,8 +372,6 @@ cvo_status setup_problems(cvo_ctx* ctx, const cvo_dcloud* s, const cvo_dcloud* t
     P.part_ovf = as<double>(ctx->b_part_ovf) + (size_t)q * kOvfBlocks * 10;
     P.part_s = as<double>(ctx->b_part_s) + (size_t)q * nb * 4;
     P.part_cnt = as<uint32_t>(ctx->b_part_cnt) + (size_t)q * nb * 2;
-    P.part_grp = as<double>(ctx->b_part_grp) + (size_t)q * ng * 10;
-    P.grp_cnt = as<uint32_t>(ctx->b_grp_cnt) + (size_t)q * ng;
     P.trace = pr.trace ? pr.trace + (size_t)q * trace_cap : nullptr;
   }
   pr.dev = as<AssocProblem>(ctx->b_probs);
@@ -663,8 +656,7 @@ extern "C" void cvo_destroy(cvo_ctx* ctx) {
   DevBuf* bufs[] = {&ctx->b_src_pos, &ctx->b_src_feat, &ctx->b_src_sem, &ctx->b_src_geo, &ctx->b_src_key,
                     &ctx->b_src_perm, &ctx->b_grid_store, &ctx->b_pairs, &ctx->b_counts, &ctx->b_ovf,
                     &ctx->b_state, &ctx->b_probs, &ctx->b_gridhdr, &ctx->b_part_a, &ctx->b_part_ovf,
-                    &ctx->b_part_s, &ctx->b_part_cnt, &ctx->b_part_grp, &ctx->b_grp_cnt, &ctx->b_trace,
-                    &ctx->b_tmp0, &ctx->b_tmp1,
+                    &ctx->b_part_s, &ctx->b_part_cnt, &ctx->b_trace, &ctx->b_tmp0, &ctx->b_tmp1,
                     &ctx->b_tmp2, &ctx->b_tmp3, &ctx->b_cub, &ctx->b_export};
   for (DevBuf* b : bufs) free_buf(*b);
   for (auto& b : ctx->b_ba) free_buf(b);

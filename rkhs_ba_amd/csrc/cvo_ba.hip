@@ -660,7 +660,7 @@ static T* as(DevBuf& b) {
 
 enum {
   BA_PAIRS = 0, BA_COUNTS, BA_OVF, BA_STATE, BA_PROBS, BA_PART_A, BA_PART_OVF, BA_PART_CNT,
-  BA_DEV, BA_EDGES, BA_H, BA_HN, BA_PAIR_PART, BA_CONTRIB, BA_FLAG, BA_PART_GRP, BA_GRP_CNT
+  BA_DEV, BA_EDGES, BA_H, BA_HN, BA_PAIR_PART, BA_CONTRIB, BA_FLAG
 };
 
 // Six buffers per frame: sorted source (pos, feat, sem, geo), grid store and
@@ -806,11 +806,6 @@ extern "C" cvo_status cvo_align_multiframe(cvo_ctx* ctx, int32_t n_frames, const
   CVO_TRY(ensure(ctx, B[BA_PART_A], sizeof(double) * 8 * (size_t)std::max(total_blocks, 1)));
   CVO_TRY(ensure(ctx, B[BA_PART_OVF], sizeof(double) * 10 * kOvfBlocks * E1));
   CVO_TRY(ensure(ctx, B[BA_PART_CNT], sizeof(uint32_t) * 2 * (size_t)std::max(total_blocks, 1)));
-  // first-stage reduction groups: at most total_blocks / kReduceGroup + one per edge
-  const size_t n_grp = (size_t)total_blocks / kReduceGroup + E1;
-  CVO_TRY(ensure(ctx, B[BA_PART_GRP], sizeof(double) * 10 * n_grp));
-  CVO_TRY(ensure(ctx, B[BA_GRP_CNT], sizeof(uint32_t) * n_grp));
-  CVO_HIP(ctx, hipMemsetAsync(B[BA_GRP_CNT].p, 0, sizeof(uint32_t) * n_grp, st));
   CVO_TRY(ensure(ctx, B[BA_DEV], sizeof(BaDev)));
   CVO_TRY(ensure(ctx, B[BA_EDGES], sizeof(BaEdge) * E1));
   CVO_TRY(ensure(ctx, B[BA_H], sizeof(double) * 2 * kBaMaxDof * kBaMaxDof));
@@ -822,7 +817,6 @@ extern "C" cvo_status cvo_align_multiframe(cvo_ctx* ctx, int32_t n_frames, const
   std::vector<AssocProblem> hp(E1);
   std::vector<BaEdge> he(E1);
   int blk = 0;
-  size_t grp_off = 0;
   for (int e = 0; e < n_edges; ++e) {
     const int f1 = edges[2 * e], f2 = edges[2 * e + 1];
     const cvo_dcloud* s = frames[f1];
@@ -855,9 +849,6 @@ extern "C" cvo_status cvo_align_multiframe(cvo_ctx* ctx, int32_t n_frames, const
     P.part_ovf = as<double>(B[BA_PART_OVF]) + (size_t)e * kOvfBlocks * 10;
     P.part_s = nullptr;
     P.part_cnt = as<uint32_t>(B[BA_PART_CNT]) + (size_t)blk * 2;
-    P.part_grp = as<double>(B[BA_PART_GRP]) + (size_t)grp_off * 10;
-    P.grp_cnt = as<uint32_t>(B[BA_GRP_CNT]) + grp_off;
-    grp_off += (size_t)(nb + kReduceGroup - 1) / kReduceGroup;
     blk += nb;
     BaEdge& ed = he[e];
     std::memset(&ed, 0, sizeof(ed));

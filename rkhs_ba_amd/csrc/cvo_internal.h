@@ -15,7 +15,6 @@ constexpr int kMaxLevels = 12;         // grid levels per target
 constexpr int kMaxWindow = 64;         // indicator window capacity
 constexpr int kBlock = 256;            // threads per block of the row kernels
 constexpr int kOvfBlocks = 96;         // capped-row kernel blocks per problem
-constexpr int kReduceGroup = 64;       // association blocks per first-stage reduction group
 constexpr uint32_t kMaxCells = 1u << 25;
 
 // One grid level over a target cloud (uniform dense grid, x-fastest cells).
@@ -44,7 +43,7 @@ struct alignas(16) ProblemState {
   double R[9];
   double T[3];
   // per-iteration totals of the association pass (raw flow sums, before the
-  // 1/c, 1/d scaling), written by the pass's last block
+  // 1/c, 1/d scaling), written by the capped-row kernel's last block
   double flow[6];
   double ip;
   double bcde[4];
@@ -104,8 +103,6 @@ struct AssocProblem {
   double* part_ovf;        // per capped-row block: flow6, ip, nnz, max row, - -> 10 doubles
   double* part_s;          // per block: 4 doubles
   uint32_t* part_cnt;      // per block: nnz and max row
-  double* part_grp;        // per group of kReduceGroup blocks: 10 doubles
-  uint32_t* grp_cnt;       // per group: blocks finished (zero between passes)
   cvo_iter_record* trace;  // per iteration (may be null)
 };
 
@@ -195,7 +192,7 @@ struct cvo_ctx {
   DevBuf b_grid_store;    // all levels' arrays
   DevBuf b_pairs, b_counts, b_ovf;
   DevBuf b_state, b_probs, b_gridhdr;
-  DevBuf b_part_a, b_part_ovf, b_part_s, b_part_cnt, b_part_grp, b_grp_cnt;
+  DevBuf b_part_a, b_part_ovf, b_part_s, b_part_cnt;
   DevBuf b_trace;
   DevBuf b_tmp0, b_tmp1, b_tmp2, b_tmp3, b_cub;
   DevBuf b_export;

@@ -6,14 +6,17 @@
 //                                   (first K by target index) + fused flow
 //                                   (fill_in_A_mat_gpu CvoGPU.cu:951-1099 and
 //                                   compute_flow_gpu_no_eigen :2001-2062)
-//   overflow_kernel               : exact index-ordered scan of rows whose
-//                                   passing pairs exceed the cap K
-//   step_kernel                   : Taylor step terms (:333-462) + device
-//                                   controller (:464-544, :1289-1433)
+//   overflow_kernel               : the pass totals, plus the exact
+//                                   index-ordered scan of rows whose passing
+//                                   pairs exceed the cap K
+//   step_kernel                   : Taylor step terms (:333-462); its last
+//                                   block runs the device controller
+//                                   (:464-544, :1289-1433)
 //
 // Rows are processed by groups of G = 8 or 16 lanes (256 / G rows per
-// 256-thread block), G chosen per iteration from the lengthscale.  Reductions are block partials summed in fixed order, so
-// results are bitwise reproducible run to run.
+// 256-thread block), G chosen per iteration from the lengthscale.  Reductions
+// are block partials summed in fixed order, so results are bitwise
+// reproducible run to run.
 
 #include <hip/hip_runtime.h>
 #include <math.h>
@@ -398,7 +401,7 @@ __device__ __forceinline__ void assoc_row(const AssocProblem& P, const KConsts& 
 
 // Block partials of the association pass: part_a[block] = flow6, ip, -;
 // part_cnt[block] = stored pairs, max row; summed into the problem state by
-// the pass's last blocks (assoc_epilogue).
+// the capped-row kernel that follows.
 
 // LDS of one association block, sized for the most rows a block can hold.
 template <int FC, int KC>
@@ -409,8 +412,6 @@ struct AssocLds {
   uint32_t cnt[kMaxRowsPerBlock];
   float feat[kMaxRowsPerBlock * FL];  // the block's source rows' features
   float sem[kMaxRowsPerBlock * KL];   // and class vectors
-  double red[10][kReduceGroup];       // first-stage reduction tree
-  int flag;
 };
 
 // Logical block lb of a problem with G lanes per row (256 / G rows).
@@ -476,23 +477,24 @@ __device__ __forceinline__ void assoc_block(const AssocProblem& P, const KConsts
   if (tid < 7) {
     double s = 0.0;
     for (int q = 0; q < RB; ++q) s += L.row[q][tid];
-    st_dev(&P.part_a[(size_t)lb * 8 + tid], s);
+    P.part_a[(size_t)lb * 8 + tid] = s;
   } else if (tid == 7) {
     uint32_t nz = 0, mx = 0;
     for (int q = 0; q < RB; ++q) {
       nz += L.cnt[q];
       mx = max(mx, L.cnt[q]);
     }
-    st_dev(&P.part_cnt[(size_t)lb * 2], nz);
-    st_dev(&P.part_cnt[(size_t)lb * 2 + 1], mx);
+    P.part_cnt[(size_t)lb * 2] = nz;
+    P.part_cnt[(size_t)lb * 2 + 1] = mx;
   }
 }
 
-// Fixed-order tree over red[c][0..kReduceGroup): sums, except the max-row
-// column 8.  Every thread of the block takes part; red[c][0] holds the result.
-__device__ __forceinline__ void tree10(double (*red)[kReduceGroup]) {
+// Fixed-order tree over red[c][0..W): sums, except the max-row column 8.
+// Every thread of the block takes part; red[c][0] holds the result.
+template <int W>
+__device__ __forceinline__ void tree10(double (*red)[W]) {
   const int tid = threadIdx.x;
-  for (int w = kReduceGroup / 2; w > 0; w >>= 1) {
+  for (int w = W / 2; w > 0; w >>= 1) {
     __syncthreads();
     for (int t = tid; t < 10 * w; t += kBlock) {
       const int c = t / w, i = t - c * w;
@@ -500,66 +502,6 @@ __device__ __forceinline__ void tree10(double (*red)[kReduceGroup]) {
     }
   }
   __syncthreads();
-}
-
-// First-stage reduction of an association pass by the blocks that finish
-// last: the last block of each group of kReduceGroup logical blocks sums the
-// group's partials in fixed order, and the block that completes the last
-// group sums the group totals (in fixed order) into the problem state: raw
-// flow sums, ip, stored pairs and the longest row.  Counters return to zero.
-// The block's partial was written by lanes 0-7 of wave 0 with st_dev.
-template <int FC, int KC>
-__device__ void assoc_epilogue(const AssocProblem& P, int n_act, int lb, AssocLds<FC, KC>& L) {
-  const int tid = threadIdx.x;
-  ProblemState* st = P.st;
-  const int grp = lb / kReduceGroup;
-  const int g0 = grp * kReduceGroup;
-  const int gsize = min(kReduceGroup, n_act - g0);
-  const int n_groups = (n_act + kReduceGroup - 1) / kReduceGroup;
-  if (tid < 64) {
-    wait_stores();
-    if (tid == 0) L.flag = (count_in(&P.grp_cnt[grp]) == (uint32_t)(gsize - 1)) ? 1 : 0;
-  }
-  __syncthreads();
-  if (!L.flag) return;  // uniform
-  for (int t = tid; t < 10 * kReduceGroup; t += kBlock) {
-    const int c = t / kReduceGroup, i = t - c * kReduceGroup;
-    double v = 0.0;
-    if (i < gsize) {
-      const int b = g0 + i;
-      if (c < 7) v = ld_dev(&P.part_a[(size_t)b * 8 + c]);
-      else if (c == 7) v = (double)ld_dev(&P.part_cnt[(size_t)b * 2]);
-      else if (c == 8) v = (double)ld_dev(&P.part_cnt[(size_t)b * 2 + 1]);
-    }
-    L.red[c][i] = v;
-  }
-  tree10(L.red);
-  if (tid < 64) {
-    if (tid < 10) st_dev(&P.part_grp[(size_t)grp * 10 + tid], L.red[tid][0]);
-    if (tid == 0) st_dev(&P.grp_cnt[grp], 0u);
-    wait_stores();
-    if (tid == 0) L.flag = (count_in(&st->blocks_done_assoc) == (uint32_t)(n_groups - 1)) ? 1 : 0;
-  }
-  __syncthreads();
-  if (!L.flag) return;  // uniform
-  for (int t = tid; t < 10 * kReduceGroup; t += kBlock) {
-    const int c = t / kReduceGroup, i = t - c * kReduceGroup;
-    double v = 0.0;
-    for (int q = i; q < n_groups; q += kReduceGroup) {
-      const double x = ld_dev(&P.part_grp[(size_t)q * 10 + c]);
-      v = (c == 8) ? fmax(v, x) : v + x;
-    }
-    L.red[c][i] = v;
-  }
-  tree10(L.red);
-  if (tid == 0) {
-    for (int c = 0; c < 6; ++c) st->flow[c] = L.red[c][0];
-    st->ip = L.red[6][0];
-    st->nnz = (uint64_t)L.red[7][0];
-    st->max_row = (uint32_t)L.red[8][0];
-    st->level_assoc = st->level;
-    st_dev(&st->blocks_done_assoc, 0u);
-  }
 }
 
 // Lanes per row of a problem's next pass (8 or 16).
@@ -594,13 +536,15 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
     case 8: assoc_block<FC, KC, 8>(P, kc, lb, L); break;
     default: assoc_block<FC, KC, 16>(P, kc, lb, L); break;
   }
-  assoc_epilogue<FC, KC>(P, n_act, lb, L);
 }
 
 // ---------------------------------------------------------------------------
-// Rows whose passing pairs exceed the cap: the reference's exact scan over all
-// targets in ascending index, keeping the first K (CvoGPU.cu:994-996).
-// Block partials part_ovf[block] = flow6, ip, stored, max row.
+// Totals of an association pass, and the rows whose passing pairs exceed the
+// cap: the reference's exact scan over all targets in ascending index,
+// keeping the first K (CvoGPU.cu:994-996).  Each block sums a slice of the
+// association block partials, adds its capped rows and publishes
+// part_ovf[block] = flow6, ip, stored, max row; the last block writes the
+// problem's totals (raw flow sums, ip, stored pairs, longest row).
 // ---------------------------------------------------------------------------
 
 template <int FC, int KC>
@@ -609,9 +553,33 @@ __global__ __launch_bounds__(kBlock) void overflow_kernel(const AssocProblem* __
   const AssocProblem& P = probs[blockIdx.y];
   ProblemState* st = P.st;
   if (st->done) return;
-  if (st->n_overflow == 0) return;  // uniform: the association totals are final
   const int tid = threadIdx.x;
   const int mode = kc.mode;
+  // (a) this block's slice of the association block partials, in fixed order
+  __shared__ double s_tot[10][128];
+  {
+    const int n_act = active_blocks(P, state_group(st));
+    const int per = (n_act + kOvfBlocks - 1) / kOvfBlocks;
+    const int b0 = blockIdx.x * per, b1 = min(n_act, b0 + per);
+    for (int t = tid; t < 10 * 128; t += kBlock) {
+      const int c = t / 128, i = t - c * 128;
+      double v = 0.0;
+      for (int b = b0 + i; b < b1; b += 128) {
+        if (c < 7) v += P.part_a[(size_t)b * 8 + c];
+        else if (c == 7) v += (double)P.part_cnt[(size_t)b * 2];
+        else if (c == 8) v = fmax(v, (double)P.part_cnt[(size_t)b * 2 + 1]);
+      }
+      s_tot[c][i] = v;
+    }
+    tree10<128>(s_tot);
+  }
+  double blk_acc[7];
+#pragma unroll
+  for (int c = 0; c < 7; ++c) blk_acc[c] = s_tot[c][0];
+  uint32_t blk_nnz = (uint32_t)s_tot[7][0], blk_max = (uint32_t)s_tot[8][0];
+  __syncthreads();
+  // (b) rows whose passing pairs exceed the cap (uniform branch)
+  if (st->n_overflow != 0) {
   const float ell = st->ell;
   const uint32_t K = (uint32_t)st->K;
   const uint32_t stamp = (uint32_t)st->iter + 1u;
@@ -625,8 +593,6 @@ __global__ __launch_bounds__(kBlock) void overflow_kernel(const AssocProblem* __
   const int r0 = blockIdx.x * chunk, r1 = min(P.n_src, r0 + chunk);
   __shared__ uint32_t s_wave[kBlock / 64];
   __shared__ double s_red[kBlock];
-  double blk_acc[7] = {0, 0, 0, 0, 0, 0, 0};
-  uint32_t blk_nnz = 0, blk_max = 0;
   for (int r = r0; r < r1; ++r) {
     if (P.ovf_stamp[r] != stamp) continue;  // uniform
     float4 x4 = P.src_pos[r];
@@ -706,8 +672,9 @@ __global__ __launch_bounds__(kBlock) void overflow_kernel(const AssocProblem* __
       __syncthreads();
     }
   }
-  // this block's capped rows, then the last block adds all blocks' in fixed
-  // order to the association totals
+  }
+  // (c) the block's partial (its slice plus its capped rows); the last block
+  // sums all blocks' partials in fixed order into the problem state
   double* out = P.part_ovf + (size_t)blockIdx.x * 10;
   __shared__ int s_last;
   if (tid < 64) {
@@ -722,24 +689,17 @@ __global__ __launch_bounds__(kBlock) void overflow_kernel(const AssocProblem* __
   }
   __syncthreads();
   if (!s_last) return;  // uniform
-  __shared__ double s_tot[10][128];
   for (int t = tid; t < 10 * 128; t += kBlock) {
     const int c = t / 128, i = t - c * 128;
     s_tot[c][i] = (i < (int)gridDim.x) ? ld_dev(&P.part_ovf[(size_t)i * 10 + c]) : 0.0;
   }
-  for (int w = 64; w > 0; w >>= 1) {
-    __syncthreads();
-    for (int t = tid; t < 10 * w; t += kBlock) {
-      const int c = t / w, i = t - c * w;
-      s_tot[c][i] = (c == 8) ? fmax(s_tot[c][i], s_tot[c][i + w]) : s_tot[c][i] + s_tot[c][i + w];
-    }
-  }
-  __syncthreads();
+  tree10<128>(s_tot);
   if (tid == 0) {
-    for (int c = 0; c < 6; ++c) st->flow[c] += s_tot[c][0];
-    st->ip += s_tot[6][0];
-    st->nnz += (uint64_t)s_tot[7][0];
-    st->max_row = max(st->max_row, (uint32_t)s_tot[8][0]);
+    for (int c = 0; c < 6; ++c) st->flow[c] = s_tot[c][0];
+    st->ip = s_tot[6][0];
+    st->nnz = (uint64_t)s_tot[7][0];
+    st->max_row = (uint32_t)s_tot[8][0];
+    st->level_assoc = st->level;
     st_dev(&st->blocks_done_ovf, 0u);
   }
 }
