@@ -258,7 +258,64 @@ def leg_ba(args):
                                          f"{spent:.1f} s of CPU work",
                                "gpu_first_iter_s": float(per[0]),
                                "speedup_first_iter": float(secs[0]) / float(per[0])}
+        log(f"[bench] BA window: the reference's CPU edge states, one outer iteration on {threads} threads")
+        t0 = time.perf_counter()
+        _, it, secs = pyoracle.align_multiframe_cpu_states(p1, clouds, inits, hold, edges, threads=threads)
+        spent = time.perf_counter() - t0
+        out["cpu_states_baseline"] = {
+            "value": float(secs[0]), "unit": "s/iter", "cores": threads, "kind": "port",
+            "sample": ("oracle IRLS outer iteration 1 with BinaryStateCPU edges (IRLS_State_CPU.cpp: fixed-ell "
+                       f"radius search of frame 2 over frame 1, weight ck k, no cap), {spent:.1f} s of CPU work")}
     cvo.close()
+    return out
+
+
+# ---------------------------------------------------------------------------
+# the CPU-runnable configuration: 5k/5k planes and spheres, 10 deg / 0.3 m
+# ---------------------------------------------------------------------------
+
+def leg_c1(args):
+    from rkhs_ba_amd import CvoGPU, read_params_yaml, synth
+    params = read_params_yaml(os.path.join(CONFIGS, "planes_spheres_5k.yaml"))
+    log("[bench] c1: planes and spheres 5k/5k, 10 deg / 0.3 m")
+    src, tgt, T_gt = synth.planes_spheres_pair(args.seed)
+    cvo = CvoGPU(params=params)
+    ds, dt = cvo.upload(src), cvo.upload(tgt)
+    I = np.eye(4, dtype=np.float32)
+    tuning = tune(cvo, ds, dt, args)
+    cvo.align(ds, dt, I)
+    t0 = time.perf_counter()
+    runs = [cvo.align(ds, dt, I) for _ in range(args.steps)]
+    wall = time.perf_counter() - t0
+    ret, T, dev_s, iters, _ = runs[-1]
+    e = pose_error(T, T_gt)
+    cvo.close()
+    log(f"[bench] c1 {wall / args.steps * 1e3:.1f} ms/registration, {iters} iters, err {e[0]:.4f} deg {e[1] * 100:.2f} cm")
+    out = {"registrations_per_s": args.steps / wall, "ms_per_registration": wall / args.steps * 1e3,
+           "iterations": iters, "pose_error": {"deg": e[0], "m": e[1]}, "tuning": tuning}
+    if args.no_cpu or args.profile_only:
+        return out
+    from oracle import pyoracle
+    threads = cpu_threads()
+    log(f"[bench] c1 cpu_ref: oracle FP32-mirror registration on {threads} threads")
+    t0 = time.perf_counter()
+    To, it_o, ret_o, _ = pyoracle.align(params, src, tgt, I)
+    spent = time.perf_counter() - t0
+    eo = pose_error(To, T_gt)
+    out["cpu_baseline"] = {"value": 1.0 / spent, "unit": "registrations/s", "cores": threads, "kind": "port",
+                           "sample": f"one whole oracle FP32-mirror registration ({it_o} iterations)",
+                           "iterations": it_o, "pose_error": {"deg": eo[0], "m": eo[1]}}
+    log(f"[bench] c1 cpu_cvo_legacy: the reference's CPU CVO on {threads} threads")
+    lp = pyoracle.legacy_params(params)
+    Tl, it_l, ret_l, secs_l, _ = pyoracle.cvo_legacy_align(lp, src, tgt, I, threads=threads)
+    el = pose_error(Tl, T_gt)
+    per_it = secs_l / max(it_l, 1)
+    out["cpu_cvo_legacy"] = {
+        "value": 1.0 / secs_l if secs_l > 0 else None, "unit": "registrations/s", "cores": threads, "kind": "port",
+        "sample": ("one whole run of the restated cvo::cvo::align (Cvo.cpp:885-1089) with the same YAML: "
+                   "unnormalised flow, float steps; it stops on its own criteria"),
+        "iterations": it_l, "seconds_per_iteration": per_it, "pose_error": {"deg": el[0], "m": el[1]},
+        "seconds_at_gpu_iterations": per_it * iters}
     return out
 
 
@@ -377,7 +434,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--points", type=int, default=60000)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--legs", default="twoview,ba,multistart,learned")
+    ap.add_argument("--legs", default="twoview,c1,ba,multistart,learned")
     ap.add_argument("--learned-pairs", type=int, default=4, help="pairs for learned vs grid registration")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
     ap.add_argument("--tune", choices=("auto", "default"), default="auto",
@@ -406,6 +463,8 @@ def main():
     tv = leg_twoview(args)
     params = tv.pop("params")
     extra = {}
+    if "c1" in legs:
+        extra["c1_planes5k"] = leg_c1(args)
     if "ba" in legs:
         extra["ba_window"] = leg_ba(args)
     if "multistart" in legs:

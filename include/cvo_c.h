@@ -373,6 +373,20 @@ cvo_status cvo_tuning_key(const cvo_ctx* ctx, const cvo_dcloud* src, const cvo_d
 cvo_status cvo_tuning_cache_lookup(const char* path, const char* key, cvo_tuning* out);
 cvo_status cvo_tuning_cache_store(const char* path, const char* key, const cvo_tuning* t);
 
+/* ---- workspace ------------------------------------------------------------
+ * Replaces CvoState (include/UnifiedCvo/cvo/CvoState.cuh:22-82), which the
+ * reference allocates on every align call and whose N x K pair matrix it
+ * clears every iteration (src/cvo/CvoState.cu:22-157).  A context keeps one
+ * device arena, grown on demand (25% headroom) and reused by later calls; the
+ * row counts make a clear unnecessary. */
+/* Device bytes the context's arena holds now. */
+cvo_status cvo_workspace_bytes(const cvo_ctx* ctx, uint64_t* out);
+/* Caps the arena (0: no cap).  A call that would grow it beyond the cap fails
+ * with CVO_ERR_LIMIT before allocating, and the context stays usable. */
+cvo_status cvo_set_workspace_limit(cvo_ctx* ctx, uint64_t bytes);
+/* Frees the arena (the next call grows it again). */
+cvo_status cvo_release_workspace(cvo_ctx* ctx);
+
 /* ---- profiling ----------------------------------------------------------- */
 
 cvo_status cvo_set_profiling(cvo_ctx* ctx, int32_t on);

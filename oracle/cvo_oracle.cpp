@@ -1190,7 +1190,8 @@ void pair_residual(const double T1[12], const double T2[12], const double p1[3],
 struct Pair {
   int e;
   float p1[3], p2[3];
-  float a;
+  double a;
+  int i1 = -1, i2 = -1;  // point indices in the edge's two frames
 };
 
 // Local (6-dof) Jacobians of one pair: J_k(12) * ComputeJacobian(T_k)(12x6).
@@ -1355,9 +1356,114 @@ extern "C" int orc_ba_normal_equations(int32_t n_frames, const double* poses, in
   return 0;
 }
 
-extern "C" int orc_align_multiframe(const cvo_params* p, int32_t n_frames, const cvo_cloud_view* frames,
-                                    double* poses, const uint8_t* hold_const, int32_t n_edges, const int32_t* edges,
-                                    int32_t threads, int32_t* iters_out, double* per_iter_seconds, int32_t max_records) {
+namespace {
+
+// BinaryStateCPU::update_inner_product + se_kernel (IRLS_State_CPU.cpp:146-309):
+// frame 2 mapped into frame 1 by pose1^-1 pose2 (float), every mapped frame-2
+// point radius-searched over the frame-1 points at the edge's fixed lengthscale
+// (no range-ell, no row cap).  Gates: geometric type dot / (|a| |b|) >= 0.01,
+// distance, colour, semantics; kept when ck k sk geo > sp_thres, with weight
+// ck k (the semantic and geometric-type factors are not in the weight,
+// IRLS_State_CPU.cpp:261).  Pairs (i in frame 1, idx in frame 2) in sensor
+// coordinates.
+void cpu_state_pairs(const cvo_params* p, const cvo_cloud_view& c1, const cvo_cloud_view& c2, const double X1[12],
+                     const double X2[12], float ell, int e, std::vector<Pair>& out, int64_t* nnz) {
+  // f1_to_f2 = pose1^-1 pose2
+  double R1[9], t1[3], R2[9], t2[3];
+  for (int r = 0; r < 3; ++r) {
+    for (int c = 0; c < 3; ++c) {
+      R1[r * 3 + c] = X1[r * 4 + c];
+      R2[r * 3 + c] = X2[r * 4 + c];
+    }
+    t1[r] = X1[r * 4 + 3];
+    t2[r] = X2[r * 4 + 3];
+  }
+  float A[9], b[3];
+  for (int r = 0; r < 3; ++r) {
+    double bb = 0;
+    for (int c = 0; c < 3; ++c) {
+      double s = 0;
+      for (int k = 0; k < 3; ++k) s += R1[k * 3 + r] * R2[k * 3 + c];
+      A[r * 3 + c] = (float)s;
+      bb += R1[c * 3 + r] * (t2[c] - t1[c]);
+    }
+    b[r] = (float)bb;
+  }
+  std::vector<float> y((size_t)c2.n * 3);
+  for (int j = 0; j < c2.n; ++j) {
+    const float* q = c2.xyz + 3 * (size_t)j;
+    for (int r = 0; r < 3; ++r) y[3 * (size_t)j + r] = A[r * 3 + 0] * q[0] + A[r * 3 + 1] * q[1] + A[r * 3 + 2] * q[2] + b[r];
+  }
+  const float sigma2 = p->sigma * p->sigma, c_ell2 = p->c_ell * p->c_ell, s_ell2 = p->s_ell * p->s_ell;
+  const float c_sigma2 = p->c_sigma * p->c_sigma, s_sigma2 = p->s_sigma * p->s_sigma;
+  float d2_thresh = 1, d2_c_thresh = 1, d2_s_thresh = 1;
+  if (p->is_using_geometry) d2_thresh = (float)(-2.0 * ell * ell * std::log((double)(p->sp_thres / sigma2)));
+  if (p->is_using_intensity) d2_c_thresh = (float)(-2.0 * c_ell2 * std::log((double)(p->sp_thres / c_sigma2)));
+  if (p->is_using_semantics) d2_s_thresh = (float)(-2.0 * s_ell2 * std::log((double)(p->sp_thres / s_sigma2)));
+  Grid grid;
+  const double r = std::sqrt((double)d2_thresh);
+  grid.build(c1.xyz, c1.n, std::max(r, 1e-3));
+  std::vector<std::vector<Pair>> per(c2.n);
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int idx = 0; idx < c2.n; ++idx) {
+    const float* q = &y[3 * (size_t)idx];
+    std::vector<std::pair<int, float>> m;
+    grid.query(q, r, [&](int i) {
+      const float* x = c1.xyz + 3 * (size_t)i;
+      float d2 = 0;
+      for (int d = 0; d < 3; ++d) d2 += (x[d] - q[d]) * (x[d] - q[d]);
+      if (d2 < d2_thresh) m.push_back({i, d2});
+    });
+    std::sort(m.begin(), m.end());
+    for (auto& mm : m) {
+      const int i = mm.first;
+      const float d2 = mm.second;
+      float sk = 1, ck = 1, k = 1, geo_sim = 1;
+      if (p->is_using_geometric_type && c1.geo_type && c2.geo_type) {
+        const float* ga = c1.geo_type + 2 * (size_t)i;
+        const float* gb = c2.geo_type + 2 * (size_t)idx;
+        geo_sim = (ga[0] * gb[0] + ga[1] * gb[1]) / std::sqrt(ga[0] * ga[0] + ga[1] * ga[1]) /
+                  std::sqrt(gb[0] * gb[0] + gb[1] * gb[1]);
+        if (geo_sim < 0.01f) continue;
+      }
+      if (p->is_using_geometry) k = (float)(sigma2 * std::exp(-d2 / (2.0 * ell * ell)));
+      if (p->is_using_intensity) {
+        const float d2c = sqdist_n_f(c1.feat + (size_t)i * c1.feat_dim, c2.feat + (size_t)idx * c2.feat_dim, c1.feat_dim);
+        if (!(d2c < d2_c_thresh)) continue;
+        ck = (float)(c_sigma2 * std::exp(-d2c / (2.0 * c_ell2)));
+      }
+      if (p->is_using_semantics) {
+        const float d2s =
+            sqdist_n_f(c1.sem + (size_t)i * c1.num_classes, c2.sem + (size_t)idx * c2.num_classes, c1.num_classes);
+        if (!(d2s < d2_s_thresh)) continue;
+        sk = (float)(s_sigma2 * std::exp(-d2s / (2.0 * s_ell2)));
+      }
+      const float a = ck * k * sk * geo_sim;
+      if (a > p->sp_thres) {
+        Pair pr;
+        pr.e = e;
+        for (int d = 0; d < 3; ++d) {
+          pr.p1[d] = c1.xyz[3 * (size_t)i + d];
+          pr.p2[d] = c2.xyz[3 * (size_t)idx + d];
+        }
+        pr.a = (double)ck * k;
+        pr.i1 = i;
+        pr.i2 = idx;
+        per[idx].push_back(pr);
+      }
+    }
+  }
+  int64_t cnt = 0;
+  for (auto& v : per) {
+    cnt += (int64_t)v.size();
+    out.insert(out.end(), v.begin(), v.end());
+  }
+  *nnz = cnt;
+}
+
+int align_multiframe_impl(const cvo_params* p, int32_t n_frames, const cvo_cloud_view* frames, double* poses,
+                          const uint8_t* hold_const, int32_t n_edges, const int32_t* edges, int32_t threads,
+                          int32_t* iters_out, double* per_iter_seconds, int32_t max_records, bool cpu_states) {
   if (!p || n_frames <= 0 || !frames || !poses || n_edges < 0) return 1;
   int old = omp_get_max_threads();
   if (threads > 0) omp_set_num_threads(threads);
@@ -1392,6 +1498,14 @@ extern "C" int orc_align_multiframe(const cvo_params* p, int32_t n_frames, const
     for (int e = 0; e < n_edges; ++e) {
       const int f1 = ef[e].first, f2 = ef[e].second;
       nnz_last[e] = nnz[e];
+      if (cpu_states) {
+        // BinaryStateCPU: every stored pair becomes a residual
+        // (IRLS_State_CPU.cpp:48-81, no nonzero threshold per edge)
+        cpu_state_pairs(p, frames[f1], frames[f2], X[f1].m, X[f2].m, (float)ell[e], e, pairs, &nnz[e]);
+        if (nnz[e] > p->multiframe_min_nonzeros) counter_nonzero++;
+        on[e] = nnz[e] > 0;
+        continue;
+      }
       if (last_max[e] > 0) K[e] = std::min(p->multiframe_num_neighbors, (int)(last_max[e] * 1.1));
       CloudRef S = ref(&frames[f1]), Tg = ref(&frames[f2]);
       S.xyz = W[f1].data();
@@ -1478,9 +1592,16 @@ extern "C" int orc_align_multiframe(const cvo_params* p, int32_t n_frames, const
         }
       }
     }
-    // per-edge lengthscale (BinaryStateGPU::update_ell, IRLS_State_GPU.cpp:117-146)
+    // per-edge lengthscale (BinaryStateGPU::update_ell, IRLS_State_GPU.cpp:117-146;
+    // BinaryStateCPU::update_ell, IRLS_State_CPU.cpp:287-290: decay while above
+    // the minimum, every outer iteration)
     bool all_min = true;
     for (int e = 0; e < n_edges; ++e) {
+      if (cpu_states) {
+        if (ell[e] > (double)p->multiframe_ell_min) ell[e] = ell[e] * (double)p->multiframe_ell_decay_rate;
+        all_min = all_min && std::fabs((float)ell[e] - p->multiframe_ell_min) < 1e-4;
+        continue;
+      }
       iters_per_ell[e]++;
       if (nnz_last[e] > 0 && nnz[e] <= nnz_last[e] && iters_per_ell[e] / p->multiframe_iter_per_ell > 0) {
         ell[e] = ell[e] * (double)p->multiframe_ell_decay_rate;
@@ -1501,5 +1622,422 @@ extern "C" int orc_align_multiframe(const cvo_params* p, int32_t n_frames, const
     for (int i = 0; i < 12; ++i) poses[12 * f + i] = X[f][i];
   if (iters_out) *iters_out = it;
   omp_set_num_threads(old);
+  return 0;
+}
+
+}  // namespace
+
+extern "C" int orc_align_multiframe(const cvo_params* p, int32_t n_frames, const cvo_cloud_view* frames,
+                                    double* poses, const uint8_t* hold_const, int32_t n_edges, const int32_t* edges,
+                                    int32_t threads, int32_t* iters_out, double* per_iter_seconds, int32_t max_records) {
+  return align_multiframe_impl(p, n_frames, frames, poses, hold_const, n_edges, edges, threads, iters_out,
+                               per_iter_seconds, max_records, false);
+}
+
+extern "C" int orc_align_multiframe_cpu_states(const cvo_params* p, int32_t n_frames, const cvo_cloud_view* frames,
+                                               double* poses, const uint8_t* hold_const, int32_t n_edges,
+                                               const int32_t* edges, int32_t threads, int32_t* iters_out,
+                                               double* per_iter_seconds, int32_t max_records) {
+  return align_multiframe_impl(p, n_frames, frames, poses, hold_const, n_edges, edges, threads, iters_out,
+                               per_iter_seconds, max_records, true);
+}
+
+extern "C" int orc_cpu_state_pairs(const cvo_params* p, const cvo_cloud_view* c1, const cvo_cloud_view* c2,
+                                   const double X1[12], const double X2[12], float ell, int32_t cap, int32_t* rows,
+                                   int32_t* cols, double* weights, int64_t* nnz) {
+  if (!p || !c1 || !c2 || !X1 || !X2 || !nnz || cap < 0) return 1;
+  std::vector<Pair> pairs;
+  cpu_state_pairs(p, *c1, *c2, X1, X2, ell, 0, pairs, nnz);
+  const int n = (int)std::min<int64_t>(*nnz, cap);
+  for (int q = 0; q < n; ++q) {
+    rows[q] = pairs[q].i1;
+    cols[q] = pairs[q].i2;
+    weights[q] = pairs[q].a;
+  }
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// The reference's CPU two-view CVO, cvo::cvo::align (src/cvo/Cvo.cpp:885-1089),
+// restated as the CPU baseline of the small configuration.  Differences from
+// the GPU path it restates: a neighbour search per iteration over the moved
+// target (the reference rebuilds a kd-tree, Cvo.cpp:368-369; here a grid gives
+// the same radius sets), fixed lengthscale (no range-ell) and no row cap; the
+// flow is not normalised (:692-693); float step coefficients with the exact
+// imag() == 0 root test, min_step when no root and a 0.8 cap (:701-830); the
+// pose update with the dynamic float Exp_SEK3 (LieGroup.cpp:214-241); the stop
+// distance is the Frobenius norm of the matrix logarithm (:262-272); and the
+// indicator is A.sum() with decrease and increase branches (:1287-1381).
+// ---------------------------------------------------------------------------
+
+namespace {
+
+struct LegacyRow {
+  std::vector<int> j;
+  std::vector<float> a;
+};
+
+// se_kernel (Cvo.cpp:349-456) at lengthscale l over the moved target yp.
+void legacy_se_kernel(const orc_legacy_params* lp, const cvo_cloud_view* src, const cvo_cloud_view* tgt,
+                      const std::vector<float>& yp, float l, std::vector<LegacyRow>& rows) {
+  const int N = src->n, M = tgt->n, F = src->feat_dim, KS = src->num_classes;
+  const float s2 = lp->sigma * lp->sigma;
+  const float d2_thres = (float)(-2.0 * l * l * std::log((double)(lp->sp_thres / s2)));
+  const float d2_c_thres =
+      (float)(-2.0 * lp->c_ell * lp->c_ell * std::log((double)(lp->sp_thres / lp->c_sigma / lp->c_sigma)));
+  const double r = std::sqrt((double)std::max(d2_thres, 0.0f));
+  Grid g;
+  g.build(yp.data(), M, std::max(r, 1e-3));
+  rows.assign(N, LegacyRow());
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int i = 0; i < N; ++i) {
+    const float* x = src->xyz + 3 * (size_t)i;
+    std::vector<std::pair<int, float>> m;
+    g.query(x, r, [&](int j) {
+      const float* y = &yp[3 * (size_t)j];
+      float d2 = 0;
+      for (int d = 0; d < 3; ++d) d2 += (x[d] - y[d]) * (x[d] - y[d]);
+      if (d2 < d2_thres) m.push_back({j, d2});
+    });
+    std::sort(m.begin(), m.end());  // the sparse row's column order
+    LegacyRow& R = rows[i];
+    for (auto& mm : m) {
+      const int j = mm.first;
+      const float d2 = mm.second;
+      const float d2_color = F > 0 ? sqdist_n_f(src->feat + (size_t)i * F, tgt->feat + (size_t)j * F, F) : 0.0f;
+      if (!(d2_color < d2_c_thres)) continue;
+      const float k = (float)(s2 * std::exp(-(double)d2 / (2.0 * l * l)));
+      const float ck = (float)(lp->c_sigma * lp->c_sigma * std::exp(-(double)d2_color / (2.0 * lp->c_ell * lp->c_ell)));
+      float sk = 1.0f;
+      if (lp->use_semantics && KS > 0) {
+        const float d2s = sqdist_n_f(src->sem + (size_t)i * KS, tgt->sem + (size_t)j * KS, KS);
+        sk = (float)(lp->s_sigma * lp->s_sigma * std::exp(-(double)d2s / (2.0 * lp->s_ell * lp->s_ell)));
+      }
+      const float a = ck * k * sk * 1.0f;
+      if (a > lp->sp_thres) {
+        R.j.push_back(j);
+        R.a.push_back(a);
+      }
+    }
+  }
+}
+
+inline void cross_f3(const float* a, const float* b, float* o) {
+  o[0] = a[1] * b[2] - a[2] * b[1];
+  o[1] = a[2] * b[0] - a[0] * b[2];
+  o[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+inline void matvec3(const float* A, const float* x, float* o) {
+  for (int r = 0; r < 3; ++r) o[r] = A[r * 3 + 0] * x[0] + A[r * 3 + 1] * x[1] + A[r * 3 + 2] * x[2];
+}
+
+inline void matmul3(const float* A, const float* B, float* o) {
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) o[r * 3 + c] = A[r * 3 + 0] * B[0 * 3 + c] + A[r * 3 + 1] * B[1 * 3 + c] + A[r * 3 + 2] * B[2 * 3 + c];
+}
+
+// compute_flow (Cvo.cpp:557-698): per row (1/c) A_i [x_i x y_j] and
+// (1/d) A_i [y_j - x_i] in float, summed over rows in double; not normalised.
+void legacy_flow(const orc_legacy_params* lp, const cvo_cloud_view* src, const std::vector<float>& yp,
+                 const std::vector<LegacyRow>& rows, float omega[3], float v[3]) {
+  const int N = src->n;
+  const float ic = 1.0f / lp->c, id = 1.0f / lp->d;
+  std::vector<double> po((size_t)N * 3), pv((size_t)N * 3);
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < N; ++i) {
+    const float* x = src->xyz + 3 * (size_t)i;
+    float o[3] = {0, 0, 0}, w[3] = {0, 0, 0};
+    const LegacyRow& R = rows[i];
+    for (size_t q = 0; q < R.j.size(); ++q) {
+      const float* y = &yp[3 * (size_t)R.j[q]];
+      const float s = ic * R.a[q], t = id * R.a[q];
+      float cr[3];
+      cross_f3(x, y, cr);
+      for (int d = 0; d < 3; ++d) {
+        o[d] += s * cr[d];
+        w[d] += t * (y[d] - x[d]);
+      }
+    }
+    for (int d = 0; d < 3; ++d) {
+      po[3 * (size_t)i + d] = (double)o[d];
+      pv[3 * (size_t)i + d] = (double)w[d];
+    }
+  }
+  double so[3] = {0, 0, 0}, sv[3] = {0, 0, 0};
+  for (int i = 0; i < N; ++i)
+    for (int d = 0; d < 3; ++d) {
+      so[d] += po[3 * (size_t)i + d];
+      sv[d] += pv[3 * (size_t)i + d];
+    }
+  for (int d = 0; d < 3; ++d) {
+    omega[d] = (float)so[d];
+    v[d] = (float)sv[d];
+  }
+}
+
+// compute_step_size (Cvo.cpp:701-830).
+float legacy_step(const orc_legacy_params* lp, const cvo_cloud_view* src, const std::vector<float>& yp, int M,
+                  const std::vector<LegacyRow>& rows, const float omega[3], const float v[3], float ell,
+                  double bcde_out[4]) {
+  const int N = src->n;
+  const float W[9] = {0, -omega[2], omega[1], omega[2], 0, -omega[0], -omega[1], omega[0], 0};
+  float W2[9], W3[9], W4[9];
+  matmul3(W, W, W2);
+  matmul3(W2, W, W3);
+  matmul3(W3, W, W4);
+  float Wv[3], W2v[3], W3v[3];
+  matvec3(W, v, Wv);
+  matvec3(W2, v, W2v);
+  matvec3(W3, v, W3v);
+  std::vector<float> xiz((size_t)M * 3), xi2z((size_t)M * 3), xi3z((size_t)M * 3), xi4z((size_t)M * 3), nx2(M), xdx(M),
+      ec(M);
+#pragma omp parallel for schedule(static)
+  for (int j = 0; j < M; ++j) {
+    const float* y = &yp[3 * (size_t)j];
+    float a[3], b2[3], b3[3], b4[3];
+    cross_f3(omega, y, a);
+    float t2[3], t3[3], t4[3];
+    matvec3(W2, y, t2);
+    matvec3(W3, y, t3);
+    matvec3(W4, y, t4);
+    for (int d = 0; d < 3; ++d) {
+      a[d] += v[d];
+      b2[d] = t2[d] + Wv[d];
+      b3[d] = t3[d] + W2v[d];
+      b4[d] = t4[d] + W3v[d];
+      xiz[3 * (size_t)j + d] = a[d];
+      xi2z[3 * (size_t)j + d] = b2[d];
+      xi3z[3 * (size_t)j + d] = b3[d];
+      xi4z[3 * (size_t)j + d] = b4[d];
+    }
+    nx2[j] = a[0] * a[0] + a[1] * a[1] + a[2] * a[2];
+    xdx[j] = -(a[0] * b2[0] + a[1] * b2[1] + a[2] * b2[2]);
+    ec[j] = (b2[0] * b2[0] + b2[1] * b2[1] + b2[2] * b2[2]) + 2.0f * (a[0] * b3[0] + a[1] * b3[1] + a[2] * b3[2]);
+  }
+  const float tc = (float)(1.0 / (2.0 * ell * ell));
+  std::vector<double> pb((size_t)N * 4);
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < N; ++i) {
+    const float* x = src->xyz + 3 * (size_t)i;
+    double Bi = 0, Ci = 0, Di = 0, Ei = 0;
+    const LegacyRow& R = rows[i];
+    for (size_t q = 0; q < R.j.size(); ++q) {
+      const int j = R.j[q];
+      const float* y = &yp[3 * (size_t)j];
+      const float df[3] = {x[0] - y[0], x[1] - y[1], x[2] - y[2]};
+      auto dot = [&](const std::vector<float>& m) {
+        return m[3 * (size_t)j] * df[0] + m[3 * (size_t)j + 1] * df[1] + m[3 * (size_t)j + 2] * df[2];
+      };
+      const float beta = -2.0f * tc * dot(xiz);
+      const float gamma = -tc * (nx2[j] + 2.0f * dot(xi2z));
+      const float delta = 2.0f * tc * (xdx[j] + (-dot(xi3z)));
+      const float epsil = -tc * (ec[j] + 2.0f * dot(xi4z));
+      const float A = R.a[q];
+      Bi += (double)(A * beta);
+      Ci += (double)A * ((double)gamma + (double)(beta * beta) / 2.0);
+      Di += (double)A * ((double)(delta + beta * gamma) + (double)(beta * beta * beta) / 6.0);
+      Ei += (double)A * ((double)(epsil + beta * delta) + 0.5 * (double)beta * (double)beta * (double)gamma +
+                         0.5 * (double)gamma * (double)gamma +
+                         (1.0 / 24.0) * (double)beta * (double)beta * (double)beta * (double)beta);
+    }
+    pb[4 * (size_t)i] = Bi;
+    pb[4 * (size_t)i + 1] = Ci;
+    pb[4 * (size_t)i + 2] = Di;
+    pb[4 * (size_t)i + 3] = Ei;
+  }
+  double B = 0, C = 0, D = 0, E = 0;
+  for (int i = 0; i < N; ++i) {
+    B += pb[4 * (size_t)i];
+    C += pb[4 * (size_t)i + 1];
+    D += pb[4 * (size_t)i + 2];
+    E += pb[4 * (size_t)i + 3];
+  }
+  bcde_out[0] = B;
+  bcde_out[1] = C;
+  bcde_out[2] = D;
+  bcde_out[3] = E;
+  // roots of the float cubic 4E s^3 + 3D s^2 + 2C s + B; a root counts when
+  // it is real (the reference's exact imag() == 0 on its companion eigenvalues)
+  const double coef[4] = {(double)(4.0f * (float)E), (double)(3.0f * (float)D), (double)(2.0f * (float)C),
+                          (double)(float)B};
+  double re[3], im[3];
+  cubic_roots(coef, re, im);
+  float temp = std::numeric_limits<float>::max();
+  for (int q = 0; q < 3; ++q) {
+    const float rr = (float)re[q];
+    if (std::isfinite(re[q]) && rr > 0 && rr < temp && im[q] == 0.0) temp = rr;
+  }
+  float step = temp == std::numeric_limits<float>::max() ? lp->min_step : temp;
+  return step > 0.8f ? 0.8f : step;
+}
+
+// compute_indicator (Cvo.cpp:1287-1381) with its flag semantics: a decrease
+// request waits while ell <= ell_min, an increase request while ell >= ell_max.
+struct LegacyIndicator {
+  std::deque<float> qs, qe;
+  float ss = 0, se = 0;
+  bool decrease = false, increase = false, last_decrease = false;
+  void update(float indicator, const orc_legacy_params* lp, float& ell) {
+    const float W = lp->ell_reduced_2;
+    if ((float)qs.size() < W) {
+      qs.push_back(indicator);
+      ss += indicator;
+    } else if ((float)qe.size() < W) {
+      qe.push_back(indicator);
+      se += indicator;
+    } else if (std::fabs(1 - se / ss) < lp->ell_reduced_3) {
+      decrease = true;
+      qs.clear();
+      qe.clear();
+      ss = se = 0;
+    } else if (se / ss < 0.7f) {
+      increase = true;
+      qs.clear();
+      qe.clear();
+      ss = se = 0;
+    } else {
+      se -= qe.front();
+      ss += qe.front();
+      qs.push_back(qe.front());
+      qe.pop_front();
+      ss -= qs.front();
+      qs.pop_front();
+      qe.push_back(indicator);
+      se += indicator;
+    }
+    if (decrease && ell > lp->ell_min) {
+      ell = ell * lp->ell_reduced_1;
+      last_decrease = true;
+      decrease = false;
+    } else {
+      // `~decrease && last_decrease` (Cvo.cpp:1367): ~ of a bool is never 0
+      if (last_decrease) last_decrease = false;
+    }
+    if (increase && ell < lp->ell_max) {
+      ell = ell * (1 / lp->ell_reduced_1);
+      increase = false;
+    }
+  }
+};
+
+}  // namespace
+
+extern "C" int orc_cvo_legacy_align(const orc_legacy_params* lp, const cvo_cloud_view* src,
+                                    const cvo_cloud_view* tgt, const float T_init[16], int32_t threads, float T_out[16],
+                                    int32_t* iters_out, int32_t* ret_out, double* seconds, orc_legacy_record* trace,
+                                    int32_t trace_cap) {
+  if (!lp || !src || !tgt || !T_init || !T_out) return 1;
+  for (int i = 0; i < 16; ++i) T_out[i] = (i % 5 == 0) ? 1.0f : 0.0f;
+  if (iters_out) *iters_out = 0;
+  if (ret_out) *ret_out = 0;
+  if (src->n == 0 || tgt->n == 0) return 0;  // set_pcd returns without a problem
+  if (src->feat_dim != tgt->feat_dim) return 1;
+  const int old = omp_get_max_threads();
+  if (threads > 0) omp_set_num_threads(threads);
+  const auto t0 = std::chrono::steady_clock::now();
+  const int M = tgt->n;
+  // set_pcd (Cvo.cpp:1182-1242): R, T from the init guess, ell = ell_init
+  float R[9], T[3];
+  for (int r = 0; r < 3; ++r) {
+    for (int c = 0; c < 3; ++c) R[r * 3 + c] = T_init[r * 4 + c];
+    T[r] = T_init[r * 4 + 3];
+  }
+  float ell = lp->ell_init;
+  LegacyIndicator ind;
+  int iter = lp->MAX_ITER, ret = 0;
+  std::vector<float> yp((size_t)M * 3);
+  std::vector<LegacyRow> rows;
+  auto applied = [&](float A[9], float b[3]) {  // update_tf: [R^T, -R^T T]
+    for (int r = 0; r < 3; ++r) {
+      for (int c = 0; c < 3; ++c) A[r * 3 + c] = R[c * 3 + r];
+      b[r] = -(R[0 * 3 + r] * T[0] + R[1 * 3 + r] * T[1] + R[2 * 3 + r] * T[2]);
+    }
+  };
+  for (int k = 0; k < lp->MAX_ITER; ++k) {
+    float A[9], b[3];
+    applied(A, b);
+    for (int j = 0; j < M; ++j) {  // transform_pcd
+      const float* y = tgt->xyz + 3 * (size_t)j;
+      for (int r = 0; r < 3; ++r) yp[3 * (size_t)j + r] = A[r * 3 + 0] * y[0] + A[r * 3 + 1] * y[1] + A[r * 3 + 2] * y[2] + b[r];
+    }
+    legacy_se_kernel(lp, src, tgt, yp, ell, rows);
+    float omega[3], v[3];
+    legacy_flow(lp, src, yp, rows, omega, v);
+    double bcde[4];
+    const float step = legacy_step(lp, src, yp, M, rows, omega, v, ell, bcde);
+    int64_t nnz = 0;
+    float asum = 0.0f;
+    for (const auto& row : rows) {
+      nnz += (int64_t)row.j.size();
+      for (float a : row.a) asum += a;
+    }
+    const double on = std::sqrt((double)omega[0] * omega[0] + (double)omega[1] * omega[1] + (double)omega[2] * omega[2]);
+    const double vn = std::sqrt((double)v[0] * v[0] + (double)v[1] * v[1] + (double)v[2] * v[2]);
+    orc_legacy_record rec{};
+    rec.ell = ell;
+    rec.step = step;
+    rec.nnz = (int32_t)nnz;
+    rec.indicator = asum;
+    for (int d = 0; d < 3; ++d) {
+      rec.omega[d] = omega[d];
+      rec.v[d] = v[d];
+    }
+    for (int c = 0; c < 4; ++c) rec.bcde[c] = (float)bcde[c];
+    rec.dist = 0.0f;
+    if (on < lp->eps && vn < lp->eps) {
+      iter = k;
+      if (on < 1e-8 && vn < 1e-8) ret = -1;
+      if (trace && k < trace_cap) trace[k] = rec;
+      break;
+    }
+    const float xi[6] = {omega[0], omega[1], omega[2], v[0], v[1], v[2]};
+    float dt[12];
+    exp_sek3_f32(xi, step, dt);
+    float dR[9], dT[3], Rn[9];
+    for (int r = 0; r < 3; ++r) {
+      for (int c = 0; c < 3; ++c) dR[r * 3 + c] = dt[r * 4 + c];
+      dT[r] = dt[r * 4 + 3];
+    }
+    float Tn[3];
+    matvec3(R, dT, Tn);
+    for (int r = 0; r < 3; ++r) T[r] = Tn[r] + T[r];
+    matmul3(R, dR, Rn);
+    std::memcpy(R, Rn, sizeof(R));
+    // dist_se3: Frobenius norm of the 4x4 matrix logarithm, sqrt(2|w|^2 + |rho|^2)
+    double d12[12], lg[6];
+    for (int i = 0; i < 12; ++i) d12[i] = dt[i];
+    log_se3(d12, lg);
+    const float dist = (float)std::sqrt(2.0 * (lg[3] * lg[3] + lg[4] * lg[4] + lg[5] * lg[5]) +
+                                        lg[0] * lg[0] + lg[1] * lg[1] + lg[2] * lg[2]);
+    rec.dist = dist;
+    if (trace && k < trace_cap) trace[k] = rec;
+    if (dist < lp->eps_2) {
+      iter = k;
+      break;
+    }
+    ind.update(asum, lp, ell);
+  }
+  float A[9], b[3];
+  applied(A, b);
+  for (int r = 0; r < 3; ++r) {
+    for (int c = 0; c < 3; ++c) T_out[r * 4 + c] = A[r * 3 + c];
+    T_out[r * 4 + 3] = b[r];
+  }
+  if (iters_out) *iters_out = iter;
+  if (ret_out) *ret_out = ret;
+  if (seconds) *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  omp_set_num_threads(old);
+  return 0;
+}
+
+extern "C" int orc_legacy_indicator_sequence(const orc_legacy_params* lp, int32_t n, const float* ind, float* ell_out) {
+  if (!lp || n < 0 || !ind || !ell_out) return 1;
+  LegacyIndicator I;
+  float ell = lp->ell_init;
+  for (int k = 0; k < n; ++k) {
+    I.update(ind[k], lp, ell);
+    ell_out[k] = ell;
+  }
   return 0;
 }

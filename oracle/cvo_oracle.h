@@ -140,6 +140,53 @@ int orc_ba_normal_equations(int32_t n_frames, const double* poses, int32_t n_pai
                             const float* p1, const float* p2, const float* a,
                             double* H, double* g, double* cost);
 
+/* BinaryStateCPU semantics (src/cvo/IRLS_State_CPU.cpp:48-328): the same IRLS
+ * window with the reference's CPU edge states -- frame 2 mapped into frame 1
+ * and radius-searched over frame 1 at a fixed lengthscale (no range-ell, no
+ * row cap), weight ck k (no semantic or geometric-type factor), every pair a
+ * residual, lengthscale decayed every outer iteration while above the
+ * minimum. */
+int orc_align_multiframe_cpu_states(const cvo_params* p, int32_t n_frames, const cvo_cloud_view* frames,
+                                    double* poses, const uint8_t* hold_const, int32_t n_edges,
+                                    const int32_t* edges, int32_t threads, int32_t* iters,
+                                    double* per_iter_seconds, int32_t max_records);
+/* One BinaryStateCPU association: pairs (frame-1 row, frame-2 column, weight),
+ * in frame-2 order then frame-1 order; the first cap are written. */
+int orc_cpu_state_pairs(const cvo_params* p, const cvo_cloud_view* c1, const cvo_cloud_view* c2,
+                        const double X1[12], const double X2[12], float ell, int32_t cap, int32_t* rows,
+                        int32_t* cols, double* weights, int64_t* nnz);
+
+/* ---- the reference's CPU two-view CVO (src/cvo/Cvo.cpp:349-1381) --------- */
+
+/* The 27-number parameter file of cvo::cvo (Cvo.cpp:89-122) reduced to the
+ * fields align() reads. */
+typedef struct {
+  float ell_init, ell_min, ell_max;
+  float sigma, sp_thres, c, d, c_ell, c_sigma, s_ell, s_sigma;
+  int32_t MAX_ITER;
+  float min_step, eps, eps_2;
+  float ell_reduced_1;   /* lengthscale factor of a decrease (1/factor on an increase) */
+  float ell_reduced_2;   /* indicator window length */
+  float ell_reduced_3;   /* indicator stability threshold */
+  int32_t use_semantics; /* IS_USING_SEMANTICS (not defined in the reference's CPU library) */
+} orc_legacy_params;
+
+typedef struct {
+  float ell, step, dist, indicator;  /* indicator = A.sum() of the iteration */
+  int32_t nnz;
+  float omega[3], v[3];              /* the (unnormalised) flow */
+  float bcde[4];
+} orc_legacy_record;
+
+/* cvo::cvo::set_pcd + align: T_init is the init guess (R, T), T_out the final
+ * `transform` [R^T, -R^T T]; iters as the reference counts them. */
+int orc_cvo_legacy_align(const orc_legacy_params* lp, const cvo_cloud_view* src, const cvo_cloud_view* tgt,
+                         const float T_init[16], int32_t threads, float T_out[16], int32_t* iters, int32_t* ret,
+                         double* seconds, orc_legacy_record* trace, int32_t trace_cap);
+/* compute_indicator over a scripted sequence of A.sum() values: the
+ * lengthscale after each call. */
+int orc_legacy_indicator_sequence(const orc_legacy_params* lp, int32_t n, const float* ind, float* ell_out);
+
 #ifdef __cplusplus
 }
 #endif

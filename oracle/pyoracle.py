@@ -242,3 +242,85 @@ def align_multiframe(params, frames, poses, hold_const, edges, threads=8, max_re
     out = np.tile(np.eye(4), (len(frames), 1, 1))
     out[:, :3, :4] = P.reshape(-1, 3, 4)
     return out, it.value, secs[:it.value]
+
+
+def align_multiframe_cpu_states(params, frames, poses, hold_const, edges, threads=8, max_records=1000):
+    """The IRLS window with the reference's BinaryStateCPU edges
+    (src/cvo/IRLS_State_CPU.cpp:48-328)."""
+    held = [HeldView(f) for f in frames]
+    views = (held[0].view.__class__ * len(held))(*[h.view for h in held])
+    P = np.ascontiguousarray(np.asarray(poses, np.float64)[:, :3, :4].reshape(-1))
+    hc = np.ascontiguousarray(hold_const, np.uint8)
+    E = np.ascontiguousarray(edges, np.int32).reshape(-1)
+    it = C.c_int32()
+    secs = np.zeros(max_records)
+    lib().orc_align_multiframe_cpu_states(C.byref(params), len(frames), views, _p(P), _p(hc), len(E) // 2, _p(E),
+                                          int(threads), C.byref(it), _p(secs), max_records)
+    out = np.tile(np.eye(4), (len(frames), 1, 1))
+    out[:, :3, :4] = P.reshape(-1, 3, 4)
+    return out, it.value, secs[:it.value]
+
+
+def cpu_state_pairs(params, c1, c2, X1, X2, ell, cap=2_000_000):
+    """One BinaryStateCPU association: (rows in frame 1, cols in frame 2,
+    weights ck k)."""
+    h1, h2 = HeldView(c1), HeldView(c2)
+    A = np.ascontiguousarray(np.asarray(X1, np.float64)[:3, :4].reshape(-1))
+    B = np.ascontiguousarray(np.asarray(X2, np.float64)[:3, :4].reshape(-1))
+    rows = np.zeros(cap, np.int32)
+    cols = np.zeros(cap, np.int32)
+    w = np.zeros(cap, np.float64)
+    nnz = C.c_int64()
+    lib().orc_cpu_state_pairs(C.byref(params), h1.ref, h2.ref, _p(A), _p(B), C.c_float(ell), cap, _p(rows), _p(cols),
+                              _p(w), C.byref(nnz))
+    n = min(nnz.value, cap)
+    return rows[:n], cols[:n], w[:n]
+
+
+class LegacyParams(C.Structure):
+    """orc_legacy_params: the fields cvo::cvo::align reads (Cvo.cpp:89-122)."""
+    _fields_ = [(n, C.c_float) for n in ("ell_init", "ell_min", "ell_max", "sigma", "sp_thres", "c", "d", "c_ell",
+                                         "c_sigma", "s_ell", "s_sigma")] + [
+        ("MAX_ITER", C.c_int32)] + [(n, C.c_float) for n in ("min_step", "eps", "eps_2", "ell_reduced_1",
+                                                           "ell_reduced_2", "ell_reduced_3")] + [
+        ("use_semantics", C.c_int32)]
+
+
+class LegacyRecord(C.Structure):
+    _fields_ = [("ell", C.c_float), ("step", C.c_float), ("dist", C.c_float), ("indicator", C.c_float),
+                ("nnz", C.c_int32), ("omega", C.c_float * 3), ("v", C.c_float * 3), ("bcde", C.c_float * 4)]
+
+
+def legacy_params(params: CvoParams) -> LegacyParams:
+    """The CPU CVO's parameters from a CvoParams: the lengthscale decay factor,
+    indicator window and threshold play ell_reduced_1..3."""
+    lp = LegacyParams()
+    for n in ("ell_init", "ell_min", "ell_max", "sigma", "sp_thres", "c", "d", "c_ell", "c_sigma", "s_ell", "s_sigma",
+              "MAX_ITER", "min_step", "eps", "eps_2"):
+        setattr(lp, n, getattr(params, n))
+    lp.ell_reduced_1 = params.ell_decay_rate
+    lp.ell_reduced_2 = float(params.indicator_window_size)
+    lp.ell_reduced_3 = params.indicator_stable_threshold
+    lp.use_semantics = 0
+    return lp
+
+
+def cvo_legacy_align(lp: LegacyParams, src, tgt, T_init, threads=8, trace_cap=0):
+    """The reference's CPU two-view CVO (src/cvo/Cvo.cpp:885-1089): returns
+    (T_out, iters, ret, seconds, trace)."""
+    hs, ht = HeldView(src), HeldView(tgt)
+    T0 = mat16(T_init)
+    T = np.zeros(16, np.float32)
+    it, ret, secs = C.c_int32(), C.c_int32(), C.c_double()
+    tr = (LegacyRecord * max(1, trace_cap))()
+    lib().orc_cvo_legacy_align(C.byref(lp), hs.ref, ht.ref, _p(T0), int(threads), _p(T), C.byref(it), C.byref(ret),
+                               C.byref(secs), tr, int(trace_cap))
+    n = min(trace_cap, it.value + 1) if trace_cap else 0
+    return T.reshape(4, 4), it.value, ret.value, secs.value, [tr[i] for i in range(n)]
+
+
+def legacy_indicator_sequence(lp: LegacyParams, seq):
+    s = np.ascontiguousarray(seq, np.float32)
+    out = np.zeros(len(s), np.float32)
+    lib().orc_legacy_indicator_sequence(C.byref(lp), len(s), _p(s), _p(out))
+    return out

@@ -35,7 +35,7 @@ void launch_gather(const uint32_t* order, int n, const float* xyz, const float* 
                    hipStream_t s);
 void launch_morton(const float* xyz, int n, const float* bmin, const float* bmax, uint32_t* key, hipStream_t s);
 void launch_export(const AssocProblem* P_dev, int n_src, int64_t* counts_orig, const int64_t* row_ptr, int32_t* col,
-                   float* val, int phase, hipStream_t s);
+                   float* val, int phase, int sort_threshold, hipStream_t s);
 void launch_all_done(const AssocProblem* probs, int n_probs, int32_t* out, hipStream_t s);
 void launch_rescore(const AssocProblem* probs, int n_probs, const KConsts& kc, float ell, int K, hipStream_t s);
 
@@ -50,18 +50,26 @@ cvo_status hip_check(cvo_ctx* ctx, hipError_t e, const char* what) {
   return set_err(ctx, CVO_ERR_HIP, m);
 }
 
+// Grows one arena buffer (25% headroom); the context's total stays within its
+// workspace limit, if one is set.
 cvo_status ensure(cvo_ctx* ctx, DevBuf& b, size_t bytes) {
   if (bytes == 0) bytes = 16;
   if (b.cap >= bytes) return CVO_OK;
+  const size_t cap = bytes + bytes / 4;
+  if (ctx->ws_limit > 0 && ctx->ws_bytes - b.cap + cap > ctx->ws_limit)
+    return set_err(ctx, CVO_ERR_LIMIT,
+                   "workspace limit: " + std::to_string(ctx->ws_bytes - b.cap + cap) + " bytes needed, limit " +
+                       std::to_string(ctx->ws_limit));
   if (b.p) {
     CVO_HIP(ctx, hipStreamSynchronize(ctx->stream));
     CVO_HIP(ctx, hipFree(b.p));
+    ctx->ws_bytes -= b.cap;
     b.p = nullptr;
     b.cap = 0;
   }
-  size_t cap = bytes + bytes / 4;
   CVO_HIP(ctx, hipMalloc(&b.p, cap));
   b.cap = cap;
+  ctx->ws_bytes += cap;
   return CVO_OK;
 }
 
@@ -545,7 +553,7 @@ static cvo_status export_assoc(cvo_ctx* ctx, const cvo_dcloud* s, const cvo_dclo
   int64_t* cnt = as<int64_t>(ctx->b_export);
   int64_t* rp = cnt + (n + 1);
   CVO_HIP(ctx, hipMemsetAsync(cnt, 0, sizeof(int64_t) * (n + 1), st));
-  launch_export(pr.dev, n, cnt, nullptr, nullptr, nullptr, 0, st);
+  launch_export(pr.dev, n, cnt, nullptr, nullptr, nullptr, 0, 0, st);
   size_t tmp = 0;
   CVO_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, cnt, rp, n + 1, st));
   CVO_TRY(ensure(ctx, ctx->b_cub, tmp));
@@ -556,7 +564,7 @@ static cvo_status export_assoc(cvo_ctx* ctx, const cvo_dcloud* s, const cvo_dclo
   CVO_TRY(ensure(ctx, ctx->b_tmp2, (sizeof(int32_t) + sizeof(float)) * (size_t)std::max<int64_t>(nnz, 1)));
   int32_t* dcol = as<int32_t>(ctx->b_tmp2);
   float* dval = reinterpret_cast<float*>(dcol + std::max<int64_t>(nnz, 1));
-  launch_export(pr.dev, n, nullptr, rp, dcol, dval, 1, st);
+  launch_export(pr.dev, n, nullptr, rp, dcol, dval, 1, ctx->params.multiframe_is_sorting_inner_product, st);
   CVO_HIP(ctx, hipGetLastError());
   out->rows = n;
   out->cols = t->n;
@@ -644,24 +652,30 @@ extern "C" cvo_status cvo_set_params(cvo_ctx* ctx, const cvo_params* p) {
   return CVO_OK;
 }
 
-static void free_buf(DevBuf& b) {
+static void free_buf(cvo_ctx* ctx, DevBuf& b) {
   if (b.p) (void)hipFree(b.p);
+  ctx->ws_bytes -= b.cap;
   b.p = nullptr;
   b.cap = 0;
 }
 
-extern "C" void cvo_destroy(cvo_ctx* ctx) {
-  if (!ctx) return;
-  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+// Every arena buffer of a context.
+static void for_each_buf(cvo_ctx* ctx, void (*fn)(cvo_ctx*, DevBuf&)) {
   DevBuf* bufs[] = {&ctx->b_src_pos, &ctx->b_src_feat, &ctx->b_src_sem, &ctx->b_src_geo, &ctx->b_src_key,
                     &ctx->b_src_perm, &ctx->b_grid_store, &ctx->b_pairs, &ctx->b_counts, &ctx->b_ovf,
                     &ctx->b_state, &ctx->b_probs, &ctx->b_gridhdr, &ctx->b_part_a, &ctx->b_part_ovf,
                     &ctx->b_part_s, &ctx->b_part_cnt, &ctx->b_trace, &ctx->b_tmp0, &ctx->b_tmp1,
                     &ctx->b_tmp2, &ctx->b_tmp3, &ctx->b_cub, &ctx->b_export};
-  for (DevBuf* b : bufs) free_buf(*b);
-  for (auto& b : ctx->b_ba) free_buf(b);
-  for (auto& b : ctx->b_frames) free_buf(b);
-  for (auto& b : ctx->b_mlp) free_buf(b);
+  for (DevBuf* b : bufs) fn(ctx, *b);
+  for (auto& b : ctx->b_ba) fn(ctx, b);
+  for (auto& b : ctx->b_frames) fn(ctx, b);
+  for (auto& b : ctx->b_mlp) fn(ctx, b);
+}
+
+extern "C" void cvo_destroy(cvo_ctx* ctx) {
+  if (!ctx) return;
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  for_each_buf(ctx, free_buf);
   for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
   if (ctx->pinned) (void)hipHostFree(ctx->pinned);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -1076,5 +1090,24 @@ extern "C" cvo_status cvo_set_pose_arithmetic(cvo_ctx* ctx, int32_t mode) {
   if (mode != CVO_POSE_FP64 && mode != CVO_POSE_REFERENCE)
     return set_err(ctx, CVO_ERR_INVALID_ARG, "pose arithmetic must be CVO_POSE_FP64 or CVO_POSE_REFERENCE");
   ctx->pose_arith = mode;
+  return CVO_OK;
+}
+
+extern "C" cvo_status cvo_workspace_bytes(const cvo_ctx* ctx, uint64_t* out) {
+  if (!ctx || !out) return CVO_ERR_INVALID_ARG;
+  *out = (uint64_t)ctx->ws_bytes;
+  return CVO_OK;
+}
+
+extern "C" cvo_status cvo_set_workspace_limit(cvo_ctx* ctx, uint64_t bytes) {
+  if (!ctx) return CVO_ERR_INVALID_ARG;
+  ctx->ws_limit = (size_t)bytes;
+  return CVO_OK;
+}
+
+extern "C" cvo_status cvo_release_workspace(cvo_ctx* ctx) {
+  if (!ctx) return CVO_ERR_INVALID_ARG;
+  CVO_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  for_each_buf(ctx, free_buf);
   return CVO_OK;
 }

@@ -187,7 +187,8 @@ struct cvo_ctx {
   int32_t trace_n = 0;
   bool profiling = false;
   cvo_kernel_stats stats{};
-  // workspace arena, grown on demand
+  // workspace arena, grown on demand; ws_bytes its total, ws_limit a cap (0 none)
+  size_t ws_bytes = 0, ws_limit = 0;
   DevBuf b_src_pos, b_src_feat, b_src_sem, b_src_geo, b_src_key, b_src_perm;
   DevBuf b_grid_store;    // all levels' arrays
   DevBuf b_pairs, b_counts, b_ovf;

@@ -1181,26 +1181,32 @@ __global__ void export_counts(const AssocProblem* __restrict__ P, int64_t* __res
 }
 
 // One 64-lane wave per row: gathers (target id, a), bitonic sort by id in LDS.
+// One row of the association export, sorted in LDS (bitonic, row <= 1024
+// pairs) by target id, or -- for rows with more than sort_threshold pairs when
+// sort_threshold > 0 -- by value descending with ties by id, the order the
+// reference's multiframe_is_sorting_inner_product gives a row
+// (CvoGPU.cu:1080-1095).  Keys: (id, value bits) or (~value bits, id); the
+// values are positive, so their bits order like the values.
 __global__ __launch_bounds__(64) void export_rows(const AssocProblem* __restrict__ P, const int64_t* __restrict__ row_ptr,
-                                                  int32_t* __restrict__ col, float* __restrict__ val) {
-  __shared__ uint32_t s_id[1024];
-  __shared__ float s_v[1024];
+                                                  int32_t* __restrict__ col, float* __restrict__ val,
+                                                  int sort_threshold) {
+  __shared__ uint64_t s_key[1024];
   const int r = blockIdx.x;
   if (r >= P->n_src) return;
   const GridLevel& G = P->grid->lv[P->st->level_assoc];
   const uint32_t cnt = P->counts[r];
   const uint32_t orig = __float_as_uint(P->src_pos[r].w);
+  const bool by_value = sort_threshold > 0 && cnt > (uint32_t)sort_threshold;
   uint32_t n2 = 1;
   while (n2 < cnt) n2 <<= 1;
   for (uint32_t s = threadIdx.x; s < n2; s += 64) {
+    uint64_t key = ~0ull;
     if (s < cnt) {
-      uint2 e = P->pairs[(size_t)r * P->cap_alloc + s];
-      s_id[s] = __float_as_uint(G.pos[e.x].w);
-      s_v[s] = __uint_as_float(e.y);
-    } else {
-      s_id[s] = 0xffffffffu;
-      s_v[s] = 0.0f;
+      const uint2 e = P->pairs[(size_t)r * P->cap_alloc + s];
+      const uint32_t id = __float_as_uint(G.pos[e.x].w);
+      key = by_value ? (((uint64_t)(~e.y)) << 32) | id : (((uint64_t)id) << 32) | e.y;
     }
+    s_key[s] = key;
   }
   __syncthreads();
   for (uint32_t k = 2; k <= n2; k <<= 1) {
@@ -1209,13 +1215,10 @@ __global__ __launch_bounds__(64) void export_rows(const AssocProblem* __restrict
         uint32_t ixj = i ^ j;
         if (ixj > i) {
           bool up = (i & k) == 0;
-          uint32_t a = s_id[i], b = s_id[ixj];
+          const uint64_t a = s_key[i], b = s_key[ixj];
           if ((a > b) == up) {
-            s_id[i] = b;
-            s_id[ixj] = a;
-            float t = s_v[i];
-            s_v[i] = s_v[ixj];
-            s_v[ixj] = t;
+            s_key[i] = b;
+            s_key[ixj] = a;
           }
         }
       }
@@ -1224,8 +1227,10 @@ __global__ __launch_bounds__(64) void export_rows(const AssocProblem* __restrict
   }
   const int64_t base = row_ptr[orig];
   for (uint32_t s = threadIdx.x; s < cnt; s += 64) {
-    col[base + s] = (int32_t)s_id[s];
-    val[base + s] = s_v[s];
+    const uint64_t key = s_key[s];
+    const uint32_t hi = (uint32_t)(key >> 32), lo = (uint32_t)key;
+    col[base + s] = (int32_t)(by_value ? lo : hi);
+    val[base + s] = __uint_as_float(by_value ? ~hi : lo);
   }
 }
 
@@ -1333,11 +1338,11 @@ void launch_morton(const float* xyz, int n, const float* bmin, const float* bmax
 }
 
 void launch_export(const AssocProblem* P_dev, int n_src, int64_t* counts_orig, const int64_t* row_ptr, int32_t* col,
-                   float* val, int phase, hipStream_t s) {
+                   float* val, int phase, int sort_threshold, hipStream_t s) {
   if (phase == 0)
     hipLaunchKernelGGL(export_counts, dim3((n_src + 255) / 256), dim3(256), 0, s, P_dev, counts_orig);
   else
-    hipLaunchKernelGGL(export_rows, dim3(n_src), dim3(64), 0, s, P_dev, row_ptr, col, val);
+    hipLaunchKernelGGL(export_rows, dim3(n_src), dim3(64), 0, s, P_dev, row_ptr, col, val, sort_threshold);
 }
 
 }  // namespace cvo

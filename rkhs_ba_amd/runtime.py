@@ -60,6 +60,9 @@ def lib() -> C.CDLL:
         L.cvo_align_multistart.argtypes = [P, P, P, i32, P, f, i32, P, P, P, P, P, P]
         L.cvo_align_multiframe.argtypes = [P, i32, P, P, P, i32, P, P, P, P, i32]
         L.cvo_set_pose_arithmetic.argtypes = [P, i32]
+        L.cvo_workspace_bytes.argtypes = [P, P]
+        L.cvo_set_workspace_limit.argtypes = [P, C.c_uint64]
+        L.cvo_release_workspace.argtypes = [P]
         L.cvo_get_tuning.argtypes = [P, P]
         L.cvo_set_tuning.argtypes = [P, P]
         L.cvo_autotune.argtypes = [P, P, P, P, C.c_char_p, P]
@@ -430,6 +433,20 @@ class CvoGPU:
 
     def set_profiling(self, on: bool) -> None:
         _check(lib().cvo_set_profiling(self._ctx, int(bool(on))), self._ctx)
+
+    # -- workspace arena ------------------------------------------------------
+    def workspace_bytes(self) -> int:
+        n = C.c_uint64()
+        _check(lib().cvo_workspace_bytes(self._ctx, C.byref(n)), self._ctx)
+        return n.value
+
+    def set_workspace_limit(self, nbytes: int) -> None:
+        """Caps the context's device arena (0: no cap); a call that would
+        exceed it raises CvoError (size beyond a documented limit)."""
+        _check(lib().cvo_set_workspace_limit(self._ctx, C.c_uint64(int(nbytes))), self._ctx)
+
+    def release_workspace(self) -> None:
+        _check(lib().cvo_release_workspace(self._ctx), self._ctx)
 
     # -- association-kernel tuning ------------------------------------------
     def get_tuning(self) -> dict:
