@@ -39,7 +39,15 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per association launch on the 60k street workload, from rocprofv3
 # FETCH_SIZE / WRITE_SIZE passes scaled by a 256 MiB copy probe
 # (profiles/assoc_street60k_pmc_traffic.md)
-ASSOC_TRAFFIC_BYTES = 42.94e6
+ASSOC_TRAFFIC_BYTES = 24.57e6
+# per-launch work of the association kernel on the 60k street registration
+# with its autotuned launch shape, from rocprofv3 counters (SQ_INSTS_VALU_TRANS_F32
+# x 64 lanes; SQ_INSTS_VALU_FLOPS_FP32 x 64): the transcendental and FP32
+# roofs are checked against the live launch time
+ASSOC_EXP_PER_LAUNCH = 4.45e6
+ASSOC_FP32_FLOP_PER_LAUNCH = 2.81e8
+EXP_PEAK_PER_S = 256 * 16 * 2.4e9   # quarter-rate v_exp_f32: 16 lanes per CU per clock
+FP32_PEAK_TFLOPS = 157.3
 CONFIGS = os.path.join(ROOT, "rkhs_ba_amd", "configs")
 
 
@@ -91,6 +99,12 @@ def cpu_twoview(params, src, tgt, T_gt, trace, threads, budget_s=12.0):
 def tune(cvo, ds, dt, args, params=None):
     """Association launch shape for this workload: the autotuner's choice
     (cached per workload key), or the library default with --tune default."""
+    if args.tuning:
+        v = [float(x) for x in args.tuning.split(",")]
+        small = (int(v[3]), v[4], v[5]) if len(v) >= 6 else (int(v[0]), 0.0, v[2])
+        cvo.set_tuning(int(v[0]), int(v[1]), v[2], lanes_small_ell=small[0], split_ell=small[1],
+                       cell_scale_small_ell=small[2])
+        return cvo.get_tuning()
     if args.tune == "default":
         return cvo.get_tuning()
     if params is None:
@@ -145,10 +159,17 @@ def leg_twoview(args):
     assoc_ms = st["ms"]["assoc"] / n_assoc
     assoc_bytes = st["bytes"]["assoc"] / n_assoc
     achieved = assoc_bytes / (assoc_ms * 1e-3) / 1e9
+    exp_rate = ASSOC_EXP_PER_LAUNCH / (assoc_ms * 1e-3)
+    flop_rate = ASSOC_FP32_FLOP_PER_LAUNCH / (assoc_ms * 1e-3) / 1e12
     out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                        "frac": achieved / HBM_PEAK_GBS, "traffic": args.traffic,
                        "kernel": "assoc_kernel", "avg_launch_ms": assoc_ms,
-                       "algorithmic_bytes_per_launch": assoc_bytes, "launches": n_assoc}
+                       "algorithmic_bytes_per_launch": assoc_bytes, "launches": n_assoc,
+                       "transcendental": {"achieved_per_s": exp_rate, "peak_per_s": EXP_PEAK_PER_S,
+                                          "frac": exp_rate / EXP_PEAK_PER_S, "per_launch": ASSOC_EXP_PER_LAUNCH},
+                       "fp32": {"achieved_tflops": flop_rate, "peak_tflops": FP32_PEAK_TFLOPS,
+                                "frac": flop_rate / FP32_PEAK_TFLOPS, "per_launch": ASSOC_FP32_FLOP_PER_LAUNCH},
+                       "note": "latency-bound (dependent gathers); per-launch counts from rocprofv3 counters"}
     out["kernels_ms_per_launch"] = {
         "assoc": assoc_ms,
         "overflow": st["ms"]["overflow"] / max(1, st["launches"]["overflow"]),
@@ -427,6 +448,25 @@ def leg_learned(args):
     return out
 
 
+def assoc_probe(args):
+    """Association passes of the 60k street pair at two lengthscales for each
+    (lanes, XCD order) at cell scale 1, six passes each, for per-dispatch
+    counters; prints one line per setting."""
+    from rkhs_ba_amd import CvoGPU, read_params_yaml, synth
+    params = read_params_yaml(os.path.join(CONFIGS, "street_60k.yaml"))
+    src, tgt, T = synth.street_pair(0, n=args.points)
+    cvo = CvoGPU(params=params)
+    ds, dt = cvo.upload(src), cvo.upload(tgt)
+    Ti = np.linalg.inv(T)
+    for ell in (0.5, 0.05):
+        for lanes, xcd in ((16, 0), (16, 1), (8, 0), (8, 1)):
+            cvo.set_tuning(lanes, xcd, 1.0)
+            for _ in range(6):
+                cvo.inner_product_gpu(ds, dt, Ti, ell)
+            log(f"[probe] ell {ell} lanes {lanes} xcd_remap {xcd}")
+    cvo.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -439,12 +479,20 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
     ap.add_argument("--tune", choices=("auto", "default"), default="auto",
                     help="association launch shape: autotuned (cached per workload) or the library default")
+    ap.add_argument("--tuning", default="",
+                    help="fixed launch shape LANES,XCD,SCALE[,LANES_SMALL,SPLIT_ELL,SCALE_SMALL] (profiling runs, "
+                         "where a profiler would distort the autotuner's timings)")
     ap.add_argument("--profile-only", action="store_true", help="timed loops only (for rocprofv3)")
     ap.add_argument("--traffic", type=float, default=ASSOC_TRAFFIC_BYTES,
                     help="HBM bytes per association launch from rocprofv3 counters")
+    ap.add_argument("--assoc-probe", action="store_true",
+                    help="only association passes of the 60k street pair at fixed launch shapes (for counters)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy 256 MiB device-to-device 5 times (calibrates HBM byte counters)")
     args = ap.parse_args()
+    if args.assoc_probe:
+        assoc_probe(args)
+        return
     if args.copy_probe:
         import torch
         src = torch.empty(64 << 20, dtype=torch.float32, device="cuda")
