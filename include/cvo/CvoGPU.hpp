@@ -248,6 +248,16 @@ class CvoGPU {
     take(a, association);
   }
 
+  // void compute_association_gpu(source, target, T, const Eigen::Matrix3f&
+  // non_isotropic_kernel, association) (CvoGPU.cu:1873-1952); kernel row-major.
+  void compute_association_gpu(const CvoPointCloud& source, const CvoPointCloud& target, const float T[16],
+                               const float non_isotropic_kernel[9], Association& association) const {
+    const cvo_cloud_view s = source.view(), t = target.view();
+    cvo_assoc a{};
+    check(cvo_compute_association_nonisotropic(ctx_, &s, &t, T, non_isotropic_kernel, &a), ctx_);
+    take(a, association);
+  }
+
 #ifdef CVO_FACADE_HAS_EIGEN
   int align(const CvoPointCloud& source, const CvoPointCloud& target, const Eigen::Matrix4f& T_init,
             Eigen::Ref<Eigen::Matrix4f> T_out, Association* association = nullptr,

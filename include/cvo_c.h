@@ -231,6 +231,17 @@ cvo_status cvo_function_angle(cvo_ctx* ctx, const cvo_cloud_view* src, const cvo
 /* CvoGPU::compute_association_gpu (src/cvo/CvoGPU.cu:1836-1871). */
 cvo_status cvo_compute_association(cvo_ctx* ctx, const cvo_cloud_view* src, const cvo_cloud_view* tgt,
                                    const float T[16], float ell, cvo_assoc* out);
+/* CvoGPU::compute_association_gpu with a non-isotropic kernel
+ * (CvoGPU.hpp:241-246, CvoGPU.cu:1873-1952, fill_in_A_mat_gpu_dense_mat_kernel
+ * :719-830): kernel is the row-major 3x3 K; per source point x,
+ * M = U K^-1 U^T with U = [u, z x u, z], z = x/|x|, u = (1, 1, -(z0+z1)/z2),
+ * k = sigma^2 exp(-(x-y)^T M (x-y) / 2) (no lengthscale, no distance gate),
+ * colour and semantic gates as the isotropic kernel, geometric type off; a row
+ * keeps the first nearest_neighbors_max passing targets by index.  Rows in
+ * source order, target ids ascending.  CVO_ERR_INVALID_ARG for a non-finite or
+ * singular kernel. */
+cvo_status cvo_compute_association_nonisotropic(cvo_ctx* ctx, const cvo_cloud_view* src, const cvo_cloud_view* tgt,
+                                                const float T[16], const float kernel[9], cvo_assoc* out);
 void cvo_assoc_free(cvo_assoc* a);
 
 /* ---- batched multi-start global registration ----------------------------- */

@@ -2041,3 +2041,87 @@ extern "C" int orc_legacy_indicator_sequence(const orc_legacy_params* lp, int32_
   }
   return 0;
 }
+
+// ---------------------------------------------------------------------------
+// fill_in_A_mat_gpu_dense_mat_kernel (src/cvo/CvoGPU.cu:719-830) through
+// inner_product_non_isotropic_impl (:1873-1952): per source point the metric
+// U K^-1 U^T (rotate_kernel_and_inverse :134-161), k = sigma^2 exp(-d2 / 2), no
+// distance gate, geometric type off, first K passing targets by index.
+// ---------------------------------------------------------------------------
+
+extern "C" int orc_assoc_nonisotropic(const cvo_params* p, const cvo_cloud_view* src, const cvo_cloud_view* tgt,
+                                      const float T[16], const float kernel[9], int32_t K, int32_t* counts,
+                                      int32_t* cols, float* vals) {
+  if (!p || !src || !tgt || !T || !kernel || !counts || K <= 0) return 1;
+  // float inverse of K by cofactors (computed in double, rounded)
+  double k[9];
+  for (int i = 0; i < 9; ++i) k[i] = kernel[i];
+  const double det = k[0] * (k[4] * k[8] - k[5] * k[7]) + k[1] * (k[5] * k[6] - k[3] * k[8]) + k[2] * (k[3] * k[7] - k[4] * k[6]);
+  if (!(det != 0.0) || !std::isfinite(det)) return 2;
+  const float Ki[9] = {(float)((k[4] * k[8] - k[5] * k[7]) / det), (float)((k[2] * k[7] - k[1] * k[8]) / det),
+                       (float)((k[1] * k[5] - k[2] * k[4]) / det), (float)((k[5] * k[6] - k[3] * k[8]) / det),
+                       (float)((k[0] * k[8] - k[2] * k[6]) / det), (float)((k[2] * k[3] - k[0] * k[5]) / det),
+                       (float)((k[3] * k[7] - k[4] * k[6]) / det), (float)((k[1] * k[6] - k[0] * k[7]) / det),
+                       (float)((k[0] * k[4] - k[1] * k[3]) / det)};
+  float A[9], b[3];
+  applied_transform(pose_from16(T), A, b);
+  const int N = src->n, M = tgt->n, F = src->feat_dim, KS = src->num_classes;
+  const bool geo = p->is_using_geometry, col = p->is_using_intensity && F > 0, sem = p->is_using_semantics && KS > 0;
+  const float sigma2 = p->sigma * p->sigma, c_sigma2 = p->c_sigma * p->c_sigma, s_sigma2 = p->s_sigma * p->s_sigma;
+  const float c_ell2 = p->c_ell * p->c_ell, s_ell2 = p->s_ell * p->s_ell;
+  const float d2_c_thres = col ? (float)(-2.0 * c_ell2 * std::log((double)(p->sp_thres / c_sigma2))) : 1.0f;
+  const float d2_s_thres = sem ? (float)(-2.0 * s_ell2 * std::log((double)(p->sp_thres / s_sigma2))) : 1.0f;
+  std::vector<float> yp((size_t)M * 3);
+  for (int j = 0; j < M; ++j) xform_f(A, b, tgt->xyz + 3 * (size_t)j, &yp[3 * (size_t)j]);
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int i = 0; i < N; ++i) {
+    const float* x = src->xyz + 3 * (size_t)i;
+    float z[3] = {x[0], x[1], x[2]};
+    const float n2 = std::fmaf(x[2], x[2], std::fmaf(x[1], x[1], x[0] * x[0]));
+    if (n2 > 0.0f) {
+      const float nr = std::sqrt(n2);
+      for (int d = 0; d < 3; ++d) z[d] = x[d] / nr;
+    }
+    const float u[3] = {1.0f, 1.0f, (-1.0f) / z[2] * (z[0] + z[1])};
+    const float w[3] = {z[1] * u[2] - z[2] * u[1], z[2] * u[0] - z[0] * u[2], z[0] * u[1] - z[1] * u[0]};
+    const float U[9] = {u[0], w[0], z[0], u[1], w[1], z[1], u[2], w[2], z[2]};
+    float T1[9], Mt[9];
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c)
+        T1[r * 3 + c] = std::fmaf(U[r * 3 + 2], Ki[6 + c], std::fmaf(U[r * 3 + 1], Ki[3 + c], U[r * 3] * Ki[c]));
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c)
+        Mt[r * 3 + c] = std::fmaf(T1[r * 3 + 2], U[c * 3 + 2], std::fmaf(T1[r * 3 + 1], U[c * 3 + 1], T1[r * 3] * U[c * 3]));
+    int n = 0;
+    for (int j = 0; j < M && n < K; ++j) {
+      const float* y = &yp[3 * (size_t)j];
+      float kk = 1.0f, ck = 1.0f, sk = 1.0f;
+      if (geo) {
+        const float d[3] = {x[0] - y[0], x[1] - y[1], x[2] - y[2]};
+        const float t0 = std::fmaf(d[2], Mt[6], std::fmaf(d[1], Mt[3], d[0] * Mt[0]));
+        const float t1 = std::fmaf(d[2], Mt[7], std::fmaf(d[1], Mt[4], d[0] * Mt[1]));
+        const float t2 = std::fmaf(d[2], Mt[8], std::fmaf(d[1], Mt[5], d[0] * Mt[2]));
+        const float d2 = std::fmaf(t2, d[2], std::fmaf(t1, d[1], t0 * d[0]));
+        kk = (float)(sigma2 * std::exp(-(double)d2 / 2.0));
+      }
+      if (col) {
+        const float d2c = sqdist_n_f(src->feat + (size_t)i * F, tgt->feat + (size_t)j * F, F);
+        if (!(d2c < d2_c_thres)) continue;
+        ck = (float)(c_sigma2 * std::exp(-(double)d2c / (2.0 * c_ell2)));
+      }
+      if (sem) {
+        const float d2s = sqdist_n_f(src->sem + (size_t)i * KS, tgt->sem + (size_t)j * KS, KS);
+        if (!(d2s < d2_s_thres)) continue;
+        sk = (float)(s_sigma2 * std::exp(-(double)d2s / (2.0 * s_ell2)));
+      }
+      const float a = ck * kk * sk * 1.0f;
+      if (a > p->sp_thres) {
+        cols[(size_t)i * K + n] = j;
+        vals[(size_t)i * K + n] = a;
+        ++n;
+      }
+    }
+    counts[i] = n;
+  }
+  return 0;
+}

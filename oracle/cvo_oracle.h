@@ -187,6 +187,13 @@ int orc_cvo_legacy_align(const orc_legacy_params* lp, const cvo_cloud_view* src,
  * lengthscale after each call. */
 int orc_legacy_indicator_sequence(const orc_legacy_params* lp, int32_t n, const float* ind, float* ell_out);
 
+/* compute_association_gpu with a non-isotropic kernel (CvoGPU.cu:719-830,
+ * :1873-1952): counts n, cols / vals n*K (first K passing targets by index).
+ * Returns 2 for a singular kernel. */
+int orc_assoc_nonisotropic(const cvo_params* p, const cvo_cloud_view* src, const cvo_cloud_view* tgt,
+                           const float T[16], const float kernel[9], int32_t K, int32_t* counts, int32_t* cols,
+                           float* vals);
+
 #ifdef __cplusplus
 }
 #endif

@@ -324,3 +324,20 @@ def legacy_indicator_sequence(lp: LegacyParams, seq):
     out = np.zeros(len(s), np.float32)
     lib().orc_legacy_indicator_sequence(C.byref(lp), len(s), _p(s), _p(out))
     return out
+
+
+def assoc_nonisotropic(params, src, tgt, T, kernel, K=None):
+    """compute_association_gpu with a 3x3 kernel (CvoGPU.cu:719-830):
+    (counts, cols, vals) with K = nearest_neighbors_max slots per row."""
+    K = K or params.nearest_neighbors_max
+    hs, ht = HeldView(src), HeldView(tgt)
+    T16 = mat16(T)
+    Km = np.ascontiguousarray(np.asarray(kernel, np.float32).reshape(9))
+    n = hs.view.n
+    counts = np.zeros(n, np.int32)
+    cols = np.full((n, K), -1, np.int32)
+    vals = np.zeros((n, K), np.float32)
+    rc = lib().orc_assoc_nonisotropic(C.byref(params), hs.ref, ht.ref, _p(T16), _p(Km), int(K), _p(counts), _p(cols),
+                                      _p(vals))
+    assert rc == 0, rc
+    return counts, cols, vals

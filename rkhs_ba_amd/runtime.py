@@ -55,6 +55,7 @@ def lib() -> C.CDLL:
         L.cvo_inner_product_d.argtypes = [P, P, P, P, f, P]
         L.cvo_function_angle.argtypes = [P, P, P, P, f, i32, P]
         L.cvo_compute_association.argtypes = [P, P, P, P, f, P]
+        L.cvo_compute_association_nonisotropic.argtypes = [P, P, P, P, P, P]
         L.cvo_align.argtypes = [P, P, P, P, P, P, P, P, P]
         L.cvo_align_d.argtypes = [P, P, P, P, P, P, P, P, P]
         L.cvo_align_multistart.argtypes = [P, P, P, i32, P, f, i32, P, P, P, P, P, P]
@@ -403,13 +404,19 @@ class CvoGPU:
                                         int(bool(is_approximate)), C.byref(out)), self._ctx)
         return out.value
 
-    def compute_association_gpu(self, source, target, T_target_frame_to_source_frame, lengthscale: float) -> Association:
-        """compute_association_gpu (CvoGPU.cu:1836-1871)."""
+    def compute_association_gpu(self, source, target, T_target_frame_to_source_frame, lengthscale) -> Association:
+        """compute_association_gpu (CvoGPU.cu:1836-1871); with a 3x3 matrix in
+        place of the lengthscale, the non-isotropic overload (CvoGPU.cu:1873-1952)."""
         T0 = mat16(T_target_frame_to_source_frame)
         hs, ht = HeldView(source), HeldView(target)
         a = Assoc()
-        _check(lib().cvo_compute_association(self._ctx, hs.ref, ht.ref, T0.ctypes.data, C.c_float(lengthscale),
-                                             C.byref(a)), self._ctx)
+        if np.ndim(lengthscale) == 2:
+            Km = np.ascontiguousarray(np.asarray(lengthscale, np.float32).reshape(9))
+            _check(lib().cvo_compute_association_nonisotropic(self._ctx, hs.ref, ht.ref, T0.ctypes.data,
+                                                              Km.ctypes.data, C.byref(a)), self._ctx)
+        else:
+            _check(lib().cvo_compute_association(self._ctx, hs.ref, ht.ref, T0.ctypes.data, C.c_float(lengthscale),
+                                                 C.byref(a)), self._ctx)
         if not a.row_ptr:
             n = hs.view.n
             return Association(n, ht.view.n, np.zeros(n + 1, np.int64), np.zeros(0, np.int32), np.zeros(0, np.float32))
