@@ -364,10 +364,22 @@ __device__ __forceinline__ void assoc_row(const AssocProblem& P, const KConsts& 
       if (idx < total) {
         pidx = run_start + (idx - run_excl);
         y4 = GL.pos[pidx];
+        // with compile-time dimensions the candidate's features and class
+        // vector are loaded together with its position, before the gates
+        // that may not need them: one memory round trip instead of three
+        float yfr[FC > 0 ? FC : 1], ysr[KC > 0 ? KC : 1];
+        if (FC > 0) {
+#pragma unroll
+          for (int f = 0; f < (FC > 0 ? FC : 1); ++f) yfr[f] = GL.feat[(size_t)pidx * F + f];
+        }
+        if (KC > 0) {
+#pragma unroll
+          for (int k = 0; k < (KC > 0 ? KC : 1); ++k) ysr[k] = GL.sem[(size_t)pidx * KS + k];
+        }
         affine3(mp.A, mp.b, y4.x, y4.y, y4.z, y0, y1, y2);
         pass = pair_kernel<FC, KC>(kc, ek, x0, x1, x2, y0, y1, y2, d2t, qg, l, xf,
-                                   FC != 0 ? GL.feat + (size_t)pidx * F : nullptr, xs,
-                                   KC != 0 ? GL.sem + (size_t)pidx * KS : nullptr, xg,
+                                   FC > 0 ? yfr : (FC != 0 ? GL.feat + (size_t)pidx * F : nullptr), xs,
+                                   KC > 0 ? ysr : (KC != 0 ? GL.sem + (size_t)pidx * KS : nullptr), xg,
                                    kc.use_gtype ? GL.geo + 2 * (size_t)pidx : nullptr, a);
       }
       const uint64_t m = __ballot(pass);
