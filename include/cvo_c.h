@@ -242,6 +242,17 @@ cvo_status cvo_compute_association(cvo_ctx* ctx, const cvo_cloud_view* src, cons
  * singular kernel. */
 cvo_status cvo_compute_association_nonisotropic(cvo_ctx* ctx, const cvo_cloud_view* src, const cvo_cloud_view* tgt,
                                                 const float T[16], const float kernel[9], cvo_assoc* out);
+/* One K-nearest association pass, the one CvoGPU::align runs each iteration
+ * with is_using_kdtree (se_kernel_kdtree, CvoGPU.cu:1104-1154;
+ * fill_in_A_mat_cukdtree :832-949), at pose T, lengthscale ell and row cap K
+ * (1..512): per source point the K nearest targets (ties by ascending target
+ * index) that pass the gates at the fixed lengthscale (no range scaling),
+ * geometric-type floor 0.001.  The reference keeps this pass inside align
+ * (no entry point of its own); it is exposed here for inspection.  Rows in
+ * source order, target ids ascending.  CVO_ERR_UNSUPPORTED without
+ * is_using_geometry. */
+cvo_status cvo_compute_association_knn(cvo_ctx* ctx, const cvo_cloud_view* src, const cvo_cloud_view* tgt,
+                                       const float T[16], float ell, int32_t K, cvo_assoc* out);
 void cvo_assoc_free(cvo_assoc* a);
 
 /* ---- batched multi-start global registration ----------------------------- */

@@ -37,6 +37,8 @@
 
 namespace {
 
+constexpr int kKnnK = 512;  // KDTREE_K_SIZE (thirdparty/cugicp/cukdtree/cukdtree.h:12)
+
 // ---------------------------------------------------------------------------
 // Canonical float helpers shared (by definition, not by code) with the HIP path.
 // ---------------------------------------------------------------------------
@@ -151,8 +153,10 @@ struct PairEval {
   double margin;
 };
 
+// gmin: geometric-type similarity floor, 0.01 (fill_in_A_mat_gpu, CvoGPU.cu:1015)
+// or 0.001 (fill_in_A_mat_cukdtree, :898).
 inline PairEval eval_pair_mirror(const Consts& k, const CloudRef& S, const CloudRef& Tg, int i, int j,
-                                 const float* xi, const float* yp, float l, float d2_thres) {
+                                 const float* xi, const float* yp, float l, float d2_thres, float gmin = 0.01f) {
   PairEval r{false, 0, 0.0f, 1e30};
   float geo_sim = 1.0f, kk = 1.0f, ck = 1.0f, sk = 1.0f;
   if (k.gtype) {
@@ -160,8 +164,8 @@ inline PairEval eval_pair_mirror(const Consts& k, const CloudRef& S, const Cloud
     const float* gb = Tg.geo + 2 * (size_t)j;
     float n2a = dot_n_f(ga, ga, 2), n2b = dot_n_f(gb, gb, 2), dab = dot_n_f(ga, gb, 2);
     geo_sim = dab * dab / (n2a * n2b);
-    r.margin = std::min(r.margin, std::fabs((double)geo_sim - 0.01) / 0.01);
-    if (geo_sim < 0.01f) { r.gate_stage = 0; return r; }
+    r.margin = std::min(r.margin, std::fabs((double)geo_sim - (double)gmin) / (double)gmin);
+    if (geo_sim < gmin) { r.gate_stage = 0; return r; }
   }
   if (k.geo) {
     float d2 = sqdist3_f(yp, xi);
@@ -189,7 +193,7 @@ inline PairEval eval_pair_mirror(const Consts& k, const CloudRef& S, const Cloud
 
 // FP64 evaluation of one pair (same algorithm in double).
 inline PairEval eval_pair_fp64(const Consts& k, const cvo_params* p, const CloudRef& S, const CloudRef& Tg,
-                               int i, int j, const double* xi, const double* yp, double l) {
+                               int i, int j, const double* xi, const double* yp, double l, double gmin = 0.01) {
   PairEval r{false, 0, 0.0f, 1e30};
   double geo_sim = 1, kk = 1, ck = 1, sk = 1;
   if (k.gtype) {
@@ -199,7 +203,7 @@ inline PairEval eval_pair_fp64(const Consts& k, const cvo_params* p, const Cloud
     double n2b = (double)gb[0] * gb[0] + (double)gb[1] * gb[1];
     double dab = (double)ga[0] * gb[0] + (double)ga[1] * gb[1];
     geo_sim = dab * dab / (n2a * n2b);
-    if (geo_sim < 0.01) return r;
+    if (geo_sim < gmin) return r;
   }
   double sigma2 = (double)p->sigma * p->sigma;
   if (k.geo) {
@@ -367,6 +371,106 @@ void assoc_core(const cvo_params* p, const CloudRef& S, const CloudRef& Tg, cons
   out.cnt.passed = c_pass;
   out.cnt.stored = c_store;
   out.cnt.max_row = mrow;
+}
+
+// K-nearest association of CvoGPU::align with is_using_kdtree (se_kernel_kdtree
+// CvoGPU.cu:1104-1154, fill_in_A_mat_cukdtree :832-949, NearestKSearch
+// cukdtree.cuh:365-376): a row takes its K nearest targets, nearest first
+// (ties by ascending target index), and stores those that pass the gates at
+// the fixed lengthscale ell (no range scaling, :863) with the geometric-type
+// floor 0.001 (:898).  Nearness is the squared distance the gate uses (float
+// for the mirror and policy precisions, double for FP64).  The reference's
+// tree is approximate (its backtracking never revisits the root's far side,
+// cukdtree.cuh:326) and queries it with T x instead of T^-1 x (CvoGPU_impl.cu:
+// 459 with the tree over the untransformed target, CvoState.cu:98); this
+// restatement is the exact K nearest in the frame the gates use.
+// brute: scan every target per row instead of the grid.
+void knn_assoc_core(const cvo_params* p, const CloudRef& S, const CloudRef& Tg, const Pose64& P, float ell, int K,
+                    int precision, int brute, AssocOut& out) {
+  const Consts k = make_consts(p);
+  const int N = S.n, M = Tg.n;
+  const int Kc = std::max(K, 0);
+  out.counts.assign(N, 0);
+  out.cols.assign((size_t)N * std::max(Kc, 1), -1);
+  out.vals.assign((size_t)N * std::max(Kc, 1), 0.0f);
+  std::vector<float> yp((size_t)M * 3);
+  std::vector<double> yp64;
+  float A[9], b[3];
+  applied_transform(P, A, b);
+  for (int j = 0; j < M; ++j) xform_f(A, b, Tg.xyz + 3 * (size_t)j, &yp[3 * (size_t)j]);
+  const bool fp64 = precision == ORC_FP64;
+  if (fp64) {
+    yp64.resize((size_t)M * 3);
+    for (int j = 0; j < M; ++j) {
+      const float* y = Tg.xyz + 3 * (size_t)j;
+      for (int i = 0; i < 3; ++i) {
+        double s = -(P.R[0 * 3 + i] * P.T[0] + P.R[1 * 3 + i] * P.T[1] + P.R[2 * 3 + i] * P.T[2]);
+        for (int c = 0; c < 3; ++c) s += P.R[c * 3 + i] * (double)y[c];
+        yp64[3 * (size_t)j + i] = s;
+      }
+    }
+  }
+  const float l = ell;
+  const float d2t = d2_geo_thres(k, l);
+  const double l64 = (double)ell;
+  const double thr64 = -2.0 * l64 * l64 * std::log((double)p->sp_thres / ((double)p->sigma * p->sigma));
+  const double rball = std::sqrt(std::max(0.0, std::max((double)d2t, thr64))) * 1.001 + 1e-6;
+  Grid grid;
+  if (!brute) grid.build(yp.data(), M, std::max(rball, 1e-6));
+  int64_t c_pass = 0, c_store = 0;
+  int32_t mrow = 0;
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : c_pass, c_store) reduction(max : mrow)
+  for (int i = 0; i < N; ++i) {
+    const float* xi = S.xyz + 3 * (size_t)i;
+    const double xi64[3] = {xi[0], xi[1], xi[2]};
+    // (distance, index) of the ball targets
+    std::vector<std::pair<double, int>> ball;
+    auto consider = [&](int j) {
+      if (fp64) {
+        const double* y = &yp64[3 * (size_t)j];
+        const double dx = y[0] - xi64[0], dy = y[1] - xi64[1], dz = y[2] - xi64[2];
+        const double d2 = dx * dx + dy * dy + dz * dz;
+        if (d2 < thr64) ball.push_back({d2, j});
+      } else {
+        const float d2 = sqdist3_f(&yp[3 * (size_t)j], xi);
+        if (d2 < d2t) ball.push_back({(double)d2, j});
+      }
+    };
+    if (brute) {
+      for (int j = 0; j < M; ++j) consider(j);
+    } else {
+      const double r = rball + 1e-5 * (1.0 + std::fabs(xi[0]) + std::fabs(xi[1]) + std::fabs(xi[2]));
+      grid.query(xi, r, consider);
+    }
+    const size_t take = std::min(ball.size(), (size_t)Kc);
+    std::partial_sort(ball.begin(), ball.begin() + take, ball.end());
+    int stored = 0;
+    for (size_t q = 0; q < take; ++q) {
+      const int j = ball[q].second;
+      PairEval e;
+      if (fp64) e = eval_pair_fp64(k, p, S, Tg, i, j, xi64, &yp64[3 * (size_t)j], l64, 0.001);
+      else e = eval_pair_mirror(k, S, Tg, i, j, xi, &yp[3 * (size_t)j], l, d2t, 0.001f);
+      const bool pass = e.ok && (fp64 ? (double)e.margin > (double)p->sp_thres : e.a > p->sp_thres);
+      if (!pass) continue;
+      c_pass++;
+      out.cols[(size_t)i * Kc + stored] = j;
+      out.vals[(size_t)i * Kc + stored] = e.a;
+      stored++;
+    }
+    out.counts[i] = stored;
+    c_store += stored;
+    mrow = std::max(mrow, stored);
+  }
+  out.cnt.passed = c_pass;
+  out.cnt.stored = c_store;
+  out.cnt.max_row = mrow;
+}
+
+// The row cap of the next K-nearest pass: the mean stored row rounded up
+// times is_using_kdtree, at most KDTREE_K_SIZE (CvoGPU.cu:1416-1417).
+int knn_next_k(const cvo_params* p, int64_t nnz, int rows) {
+  const int mean = (int)std::ceil((double)nnz / (double)rows);
+  return std::min(kKnnK, mean * p->is_using_kdtree);
 }
 
 // ---------------------------------------------------------------------------
@@ -879,6 +983,19 @@ extern "C" int orc_assoc(const cvo_params* p, const cvo_cloud_view* src, const c
   return 0;
 }
 
+extern "C" int orc_assoc_knn(const cvo_params* p, const cvo_cloud_view* src, const cvo_cloud_view* tgt,
+                             const float T[16], float ell, int32_t K, int32_t precision, int32_t brute,
+                             int32_t* counts, int32_t* cols, float* vals, orc_counters* cnt) {
+  if (!p || !src || !tgt || !T || K <= 0) return 1;
+  AssocOut o;
+  knn_assoc_core(p, ref(src), ref(tgt), pose_from16(T), ell, K, precision, brute, o);
+  if (counts) std::memcpy(counts, o.counts.data(), sizeof(int32_t) * o.counts.size());
+  if (cols) std::memcpy(cols, o.cols.data(), sizeof(int32_t) * o.cols.size());
+  if (vals) std::memcpy(vals, o.vals.data(), sizeof(float) * o.vals.size());
+  if (cnt) *cnt = o.cnt;
+  return 0;
+}
+
 extern "C" int orc_pair_value(const cvo_params* p, const cvo_cloud_view* src, const cvo_cloud_view* tgt,
                               const float T[16], float ell, int32_t i, int32_t j, double* a_out,
                               double* gate_margin) {
@@ -970,7 +1087,8 @@ extern "C" int orc_align(const cvo_params* p, const cvo_cloud_view* src, const c
   const int fkern = precision;                                         // flow and step terms
   const bool float_pose = precision == ORC_MIRROR;
   const bool float_ell = precision != ORC_FP64;
-  int K = p->nearest_neighbors_max;
+  const bool knn = p->is_using_kdtree != 0;
+  int K = knn ? kKnnK : p->nearest_neighbors_max;  // CvoGPU.cu:1287
   Indicator<float> ind32;
   Indicator<double> ind64;
   int ret = 0, k = 0;
@@ -984,11 +1102,18 @@ extern "C" int orc_align(const cvo_params* p, const cvo_cloud_view* src, const c
     for (int j = 0; j < M; ++j) xform_f(A, b, Tg.xyz + 3 * (size_t)j, &yp[3 * (size_t)j]);
     AssocOut o;
     float ell_used = float_ell ? ell : (float)ell64;
-    assoc_core(p, S, Tg, P, ell_used, K, kern, brute, o);
+    int stride = K;
+    if (knn) {
+      knn_assoc_core(p, S, Tg, P, ell_used, K, kern, brute, o);
+      stride = std::max(K, 1);
+    } else {
+      assoc_core(p, S, Tg, P, ell_used, K, kern, brute, o);
+    }
     double sum6[6], unit6[6];
-    flow_core(p, N, S.xyz, yp.data(), K, o.counts.data(), o.cols.data(), o.vals.data(), fkern, sum6, unit6);
+    flow_core(p, N, S.xyz, yp.data(), stride, o.counts.data(), o.cols.data(), o.vals.data(), fkern, sum6, unit6);
     double bcde[4];
-    step_core(p, ell_used, N, S.xyz, yp.data(), K, o.counts.data(), o.cols.data(), o.vals.data(), unit6, fkern, bcde);
+    step_core(p, ell_used, N, S.xyz, yp.data(), stride, o.counts.data(), o.cols.data(), o.vals.data(), unit6, fkern,
+              bcde);
     double step = select_step(p, bcde);
     if (precision == ORC_MIRROR) step = (double)(float)step;  // cvo_state->step is float
     // stop test on the normalised flow (CvoGPU.cu:1354-1360)
@@ -1061,7 +1186,8 @@ extern "C" int orc_align(const cvo_params* p, const cvo_cloud_view* src, const c
       ell64 = ell64 * (double)p->ell_decay_rate;
       if (ell64 < (double)p->ell_min) ell64 = p->ell_min;
     }
-    K = std::min(p->nearest_neighbors_max, (int)(o.cnt.max_row * 1.2));
+    if (knn) K = knn_next_k(p, o.cnt.stored, N);
+    else K = std::min(p->nearest_neighbors_max, (int)(o.cnt.max_row * 1.2));
   }
   if (iters) *iters = k;
   if (ret_code) *ret_code = ret;
@@ -1511,7 +1637,10 @@ int align_multiframe_impl(const cvo_params* p, int32_t n_frames, const cvo_cloud
       S.xyz = W[f1].data();
       Tg.xyz = W[f2].data();
       AssocOut o;
-      assoc_core(p, S, Tg, I64, (float)ell[e], K[e], ORC_MIRROR, 0, o);
+      // BinaryStateGPU::update_inner_product: K nearest with is_using_kdtree
+      // (IRLS_State_GPU.cu:171-215), the ball association otherwise
+      if (p->is_using_kdtree) knn_assoc_core(p, S, Tg, I64, (float)ell[e], K[e], ORC_MIRROR, 0, o);
+      else assoc_core(p, S, Tg, I64, (float)ell[e], K[e], ORC_MIRROR, 0, o);
       nnz[e] = o.cnt.stored;
       last_max[e] = o.cnt.max_row;
       if (std::getenv("ORC_BA_DEBUG"))

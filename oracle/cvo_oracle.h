@@ -51,6 +51,15 @@ int orc_assoc(const cvo_params* p, const cvo_cloud_view* src, const cvo_cloud_vi
               const float T[16], float ell, int32_t K, int32_t precision, int32_t brute,
               int32_t* counts, int32_t* cols, float* vals, orc_counters* cnt);
 
+/* The K-nearest association of CvoGPU::align with is_using_kdtree
+ * (se_kernel_kdtree, src/cvo/CvoGPU.cu:1104-1154; fill_in_A_mat_cukdtree
+ * :832-949): per row the K nearest targets (ties by index), nearest first,
+ * that pass the gates at the fixed lengthscale ell.  counts[N]; cols/vals
+ * [N*K] in ascending distance.  brute != 0 scans every target. */
+int orc_assoc_knn(const cvo_params* p, const cvo_cloud_view* src, const cvo_cloud_view* tgt,
+                  const float T[16], float ell, int32_t K, int32_t precision, int32_t brute,
+                  int32_t* counts, int32_t* cols, float* vals, orc_counters* cnt);
+
 /* For the near-cutoff exemption of the parity tests: the reference's product
  * kernel value a (before the sp_thres test) of one pair, or -1 when a gate
  * rejects it, plus the relative margin of the closest gate. */

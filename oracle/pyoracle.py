@@ -42,6 +42,7 @@ def lib():
         _lib.orc_inner_product.restype = d
         _lib.orc_time_iterations.restype = d
         _lib.orc_assoc.argtypes = [P, P, P, P, f, i32, i32, i32, P, P, P, P]
+        _lib.orc_assoc_knn.argtypes = [P, P, P, P, f, i32, i32, i32, P, P, P, P]
         _lib.orc_pair_value.argtypes = [P, P, P, P, f, i32, i32, P, P]
         _lib.orc_flow.argtypes = [P, i32, P, P, i32, P, P, P, i32, P, P]
         _lib.orc_step_coeffs.argtypes = [P, f, i32, P, P, i32, P, P, P, P, i32, P]
@@ -68,6 +69,22 @@ def assoc(params: CvoParams, src, tgt, T, ell, K, precision=MIRROR, brute=False)
     T16 = mat16(T)
     rc = lib().orc_assoc(C.byref(params), hs.ref, ht.ref, _p(T16), C.c_float(ell), K, precision, int(brute),
                          _p(counts), _p(cols), _p(vals), C.byref(cnt))
+    assert rc == 0
+    return counts, cols.reshape(N, K), vals.reshape(N, K), cnt
+
+
+def assoc_knn(params: CvoParams, src, tgt, T, ell, K, precision=MIRROR, brute=False):
+    """K-nearest association of align with is_using_kdtree (CvoGPU.cu:832-949,
+    1104-1154): rows in ascending distance, ties by target index."""
+    hs, ht = HeldView(src), HeldView(tgt)
+    N = hs.view.n
+    counts = np.zeros(N, np.int32)
+    cols = np.full(N * K, -1, np.int32)
+    vals = np.zeros(N * K, np.float32)
+    cnt = Counters()
+    T16 = mat16(T)
+    rc = lib().orc_assoc_knn(C.byref(params), hs.ref, ht.ref, _p(T16), C.c_float(ell), K, precision, int(brute),
+                             _p(counts), _p(cols), _p(vals), C.byref(cnt))
     assert rc == 0
     return counts, cols.reshape(N, K), vals.reshape(N, K), cnt
 

@@ -127,6 +127,10 @@ KConsts make_kconsts(const cvo_params& p, int F, int Ks, int mode) {
   k.range_ell_in_step = p.is_using_range_ell;
   k.rq_coef = k.use_geo ? sqrtf(-2.0f * k.lg) * 1.001f : 0.0f;
   k.mode = mode;
+  // the K-nearest association serves align and the multiframe states; the
+  // inner product, function angle and association export use the ball
+  // association in either setting (CvoGPU.cu:1310 is align's only branch)
+  k.knn = mode == MODE_IP ? 0 : p.is_using_kdtree;
   return k;
 }
 
@@ -139,7 +143,10 @@ cvo_status validate_pair(cvo_ctx* ctx, const cvo_dcloud* s, const cvo_dcloud* t)
   const cvo_params& p = ctx->params;
   if (!s || !t) return set_err(ctx, CVO_ERR_INVALID_ARG, "null cloud");
   if (s->n == 0 || t->n == 0) return set_err(ctx, CVO_ERR_EMPTY, "empty cloud");
-  if (p.is_using_kdtree) return set_err(ctx, CVO_ERR_UNSUPPORTED, "is_using_kdtree is not supported");
+  if (p.is_using_kdtree < 0 || p.is_using_kdtree > kKnnMaxK)
+    return set_err(ctx, CVO_ERR_LIMIT, "is_using_kdtree (the K-nearest multiplier) must be in [0, 512]");
+  if (p.is_using_kdtree && !p.is_using_geometry)
+    return set_err(ctx, CVO_ERR_UNSUPPORTED, "is_using_kdtree needs is_using_geometry");
   if (p.is_using_intensity && (!s->feat || !t->feat || s->F != t->F || s->F <= 0))
     return set_err(ctx, CVO_ERR_INVALID_ARG, "is_using_intensity needs features of equal dimension on both clouds");
   if (p.is_using_semantics && (!s->sem || !t->sem || s->K != t->K || s->K <= 0))
@@ -820,10 +827,12 @@ extern "C" cvo_status cvo_align_d(cvo_ctx* ctx, const cvo_dcloud* src, const cvo
   const bool want_trace = ctx->trace_cap > 0 || ctx->profiling;
   const int trace_cap = want_trace ? std::max(ctx->trace_cap, ctx->profiling ? p.MAX_ITER : 0) : 0;
   Problems pr;
-  CVO_TRY(setup_problems(ctx, src, tgt, gh, 1, p.nearest_neighbors_max, trace_cap, kc.max_group, pr));
+  // the first pass takes KDTREE_K_SIZE nearest with is_using_kdtree (CvoGPU.cu:1287)
+  const int K0 = kc.knn > 0 ? kKnnMaxK : p.nearest_neighbors_max;
+  CVO_TRY(setup_problems(ctx, src, tgt, gh, 1, std::max(K0, p.nearest_neighbors_max), trace_cap, kc.max_group, pr));
   CVO_TRY(ensure(ctx, ctx->b_tmp2, 64 * sizeof(float)));
   CVO_HIP(ctx, hipMemcpyAsync(ctx->b_tmp2.p, T_init, sizeof(float) * 16, hipMemcpyHostToDevice, ctx->stream));
-  launch_init_state(pr.dev, 1, as<float>(ctx->b_tmp2), 16, kc, p.ell_init, p.nearest_neighbors_max, ctx->stream);
+  launch_init_state(pr.dev, 1, as<float>(ctx->b_tmp2), 16, kc, p.ell_init, K0, ctx->stream);
   CVO_HIP(ctx, hipGetLastError());
   double dev_s = 0.0;
   if (p.MAX_ITER > 0) CVO_TRY(run_loop(ctx, pr, kc, p.MAX_ITER, true, &dev_s));
@@ -875,11 +884,14 @@ extern "C" cvo_status cvo_align(cvo_ctx* ctx, const cvo_cloud_view* src, const c
 
 // Single association pass at pose T and lengthscale ell; the problem stays in
 // the context's buffers for an export.
+// knn_K > 0: the K-nearest pass with row cap knn_K instead of the ball pass.
 static cvo_status single_pass(cvo_ctx* ctx, const cvo_dcloud* src, const cvo_dcloud* tgt, const float T[16], float ell,
-                              Problems& pr, KConsts& kc, double* ip) {
+                              Problems& pr, KConsts& kc, double* ip, int knn_K = 0) {
   CVO_TRY(validate_pair(ctx, src, tgt));
   const cvo_params& p = ctx->params;
   kc = make_kconsts(p, src->F, src->K, MODE_IP);
+  kc.knn = knn_K > 0 ? 1 : 0;
+  const int K0 = knn_K > 0 ? knn_K : p.nearest_neighbors_max;
   apply_tuning(ctx, kc);
   kc.n_src_f = (float)src->n;
   kc.n_tgt_f = (float)tgt->n;
@@ -890,10 +902,10 @@ static cvo_status single_pass(cvo_ctx* ctx, const cvo_dcloud* src, const cvo_dcl
   GridHost gh;
   std::vector<float> hs = level_sizes(kc, ell, ell, src->max_norm, tq, kc.min_scale);
   CVO_TRY(build_grid(ctx, tgt, hs, gh));
-  CVO_TRY(setup_problems(ctx, src, tgt, gh, 1, p.nearest_neighbors_max, 0, kc.max_group, pr));
+  CVO_TRY(setup_problems(ctx, src, tgt, gh, 1, K0, 0, kc.max_group, pr));
   CVO_TRY(ensure(ctx, ctx->b_tmp2, 64 * sizeof(float)));
   CVO_HIP(ctx, hipMemcpyAsync(ctx->b_tmp2.p, T, sizeof(float) * 16, hipMemcpyHostToDevice, ctx->stream));
-  launch_init_state(pr.dev, 1, as<float>(ctx->b_tmp2), 16, kc, ell, p.nearest_neighbors_max, ctx->stream);
+  launch_init_state(pr.dev, 1, as<float>(ctx->b_tmp2), 16, kc, ell, K0, ctx->stream);
   CVO_TRY(run_loop(ctx, pr, kc, 1, false, nullptr));
   ProblemState hs2;
   CVO_HIP(ctx, hipMemcpyAsync(&hs2, pr.states, sizeof(ProblemState), hipMemcpyDeviceToHost, ctx->stream));
@@ -965,6 +977,25 @@ extern "C" cvo_status cvo_compute_association(cvo_ctx* ctx, const cvo_cloud_view
   return export_assoc(ctx, a.c, b.c, pr, out);
 }
 
+extern "C" cvo_status cvo_compute_association_knn(cvo_ctx* ctx, const cvo_cloud_view* src, const cvo_cloud_view* tgt,
+                                                  const float T[16], float ell, int32_t K, cvo_assoc* out) {
+  if (!ctx || !src || !tgt || !T || !out) return CVO_ERR_INVALID_ARG;
+  std::memset(out, 0, sizeof(*out));
+  if (K < 1 || K > kKnnMaxK) return set_err(ctx, CVO_ERR_LIMIT, "K must be in [1, 512]");
+  if (!(ell > 0.0f)) return set_err(ctx, CVO_ERR_INVALID_ARG, "ell must be positive");
+  if (!ctx->params.is_using_geometry)
+    return set_err(ctx, CVO_ERR_UNSUPPORTED, "the K-nearest association needs is_using_geometry");
+  if (src->n == 0 || tgt->n == 0) return CVO_OK;
+  Uploaded a{ctx}, b{ctx};
+  CVO_TRY(cvo_cloud_upload(ctx, src, &a.c));
+  CVO_TRY(cvo_cloud_upload(ctx, tgt, &b.c));
+  Problems pr;
+  KConsts kc;
+  double ip = 0.0;
+  CVO_TRY(single_pass(ctx, a.c, b.c, T, ell, pr, kc, &ip, K));
+  return export_assoc(ctx, a.c, b.c, pr, out);
+}
+
 extern "C" void cvo_assoc_free(cvo_assoc* a) {
   if (!a) return;
   std::free(a->row_ptr);
@@ -1009,6 +1040,7 @@ extern "C" cvo_status cvo_align_multistart(cvo_ctx* ctx, const cvo_dcloud* src, 
   kc.max_src_norm = src->max_norm;
   KConsts kci = kc;
   kci.mode = MODE_IP;
+  kci.knn = 0;  // scoring is an inner product: the ball association
   CVO_TRY(prepare_source(ctx, src));
   float tq = 0.0f;
   for (int s = 0; s < n_starts; ++s)
@@ -1020,12 +1052,12 @@ extern "C" cvo_status cvo_align_multistart(cvo_ctx* ctx, const cvo_dcloud* src, 
   CVO_TRY(build_grid(ctx, tgt, level_sizes(kc, ell_hi, ell_lo, src->max_norm, tq, kc.min_scale), gh));
   const int n_run = (mode == 1) ? n_starts : n_starts;
   Problems pr;
-  CVO_TRY(setup_problems(ctx, src, tgt, gh, n_run, p.nearest_neighbors_max, 0, kc.max_group, pr));
+  const int K0 = kc.knn > 0 ? kKnnMaxK : p.nearest_neighbors_max;
+  CVO_TRY(setup_problems(ctx, src, tgt, gh, n_run, std::max(K0, p.nearest_neighbors_max), 0, kc.max_group, pr));
   CVO_TRY(ensure(ctx, ctx->b_tmp2, sizeof(float) * 16 * (size_t)n_starts + 64));
   CVO_HIP(ctx, hipMemcpyAsync(ctx->b_tmp2.p, T_starts, sizeof(float) * 16 * n_starts, hipMemcpyHostToDevice,
                               ctx->stream));
-  launch_init_state(pr.dev, n_starts, as<float>(ctx->b_tmp2), 16, kc, p.ell_init, p.nearest_neighbors_max,
-                    ctx->stream);
+  launch_init_state(pr.dev, n_starts, as<float>(ctx->b_tmp2), 16, kc, p.ell_init, K0, ctx->stream);
   CVO_HIP(ctx, hipGetLastError());
   std::vector<ProblemState> hs(n_starts);
   std::vector<double> cosv(n_starts, 0.0);

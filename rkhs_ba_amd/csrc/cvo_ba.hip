@@ -694,6 +694,9 @@ static cvo_status validate_frames(cvo_ctx* ctx, int n_frames, const cvo_dcloud* 
   }
   if (p.multiframe_num_neighbors < 1 || p.multiframe_num_neighbors > 1024)
     return set_err(ctx, CVO_ERR_LIMIT, "multiframe_num_neighbors must be in [1, 1024]");
+  // the K-nearest search keeps at most KDTREE_K_SIZE per row (cukdtree.h:12,40)
+  if (p.is_using_kdtree && p.multiframe_num_neighbors > kKnnMaxK)
+    return set_err(ctx, CVO_ERR_LIMIT, "is_using_kdtree needs multiframe_num_neighbors <= 512");
   if (p.multiframe_iter_per_ell <= 0) return set_err(ctx, CVO_ERR_INVALID_ARG, "multiframe_iter_per_ell must be > 0");
   if (!(p.multiframe_ell_init > 0.0f) || !(p.multiframe_ell_min > 0.0f))
     return set_err(ctx, CVO_ERR_INVALID_ARG, "multiframe lengthscales must be positive");

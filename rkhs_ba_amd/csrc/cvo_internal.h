@@ -16,6 +16,7 @@ constexpr int kMaxWindow = 64;         // indicator window capacity
 constexpr int kBlock = 256;            // threads per block of the row kernels
 constexpr int kOvfBlocks = 96;         // capped-row kernel blocks per problem
 constexpr uint32_t kMaxCells = 1u << 25;
+constexpr int kKnnMaxK = 512;          // K-nearest row cap (KDTREE_K_SIZE, cukdtree.h:12)
 
 // One grid level over a target cloud (uniform dense grid, x-fastest cells).
 struct GridLevel {
@@ -146,6 +147,7 @@ struct KConsts {
   int32_t max_group;          // largest lane count of the policy (sizes the grid)
   int32_t xcd_remap;          // 1: contiguous row-block ranges per XCD
   float min_scale;            // finest cell scale of the policy (sizes the levels)
+  int32_t knn;                // is_using_kdtree: K-nearest association, the K multiplier (0 off)
 };
 
 // Rows per association block for a lane count per row (8 or 16).

@@ -227,19 +227,21 @@ __device__ __forceinline__ ExactK exact_consts(const KConsts& kc) {
 }
 
 // The kernel value of a candidate pair: gates, then the product kernel.
-// Returns false when a gate rejects or a <= sp_thres.
+// Returns false when a gate rejects or a <= sp_thres.  gmin: the geometric
+// type similarity below which a pair is rejected (0.01 in fill_in_A_mat_gpu,
+// CvoGPU.cu:1015; 0.001 in fill_in_A_mat_cukdtree, :898).
 template <int FC, int KC>
 __device__ __forceinline__ bool pair_kernel(const KConsts& kc, const ExactK& ek, float x0, float x1, float x2,
                                             float y0, float y1, float y2, float d2t, float qg, float l,
                                             const float* xf, const float* yf, const float* xs, const float* ys,
-                                            const float* xg, const float* yg, float& a) {
+                                            const float* xg, const float* yg, float& a, float gmin = 0.01f) {
   const float dx = y0 - x0, dy = y1 - x1, dz = y2 - x2;
   const float d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
   if (kc.use_geo && !(d2 < d2t)) return false;
   float geo_sim = 1.0f;
   if (kc.use_gtype) {
     geo_sim = geo_similarity(xg, yg);
-    if (geo_sim < 0.01f) return false;
+    if (geo_sim < gmin) return false;
   }
   float d2c = 0.0f, d2s = 0.0f;
   if (FC != 0) {
@@ -411,6 +413,261 @@ __device__ __forceinline__ void assoc_row(const AssocProblem& P, const KConsts& 
   out.overflow = stored > Ku;
 }
 
+// ---------------------------------------------------------------------------
+// K-nearest association (is_using_kdtree: se_kernel_kdtree, CvoGPU.cu:1104-1154,
+// fill_in_A_mat_cukdtree :832-949).  A row's candidates are its K nearest
+// targets, nearest first; each must then pass the gates, the distance gate at
+// the fixed lengthscale (no range scaling, :863).  A target outside the
+// distance gate's ball can never be stored, and a target inside it is among
+// the K nearest exactly when fewer than K ball targets are nearer, so the
+// selection runs over the ball alone.  Nearness is the key (float d2 bits,
+// target id): ties by ascending id.  A row counts its ball targets; with more
+// than K, a most-significant-digit radix select over the keys (8 bits a pass,
+// one 256-bin LDS histogram per row) finds the K-th smallest key.  The last
+// pass applies the gates to the selected targets and stores the passing ones.
+// ---------------------------------------------------------------------------
+
+// Orders this wave's LDS accesses before and after it (no other wave shares a
+// row's histogram).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Query box of a row in a grid level: the cell runs (x-rows of cells) it spans.
+struct CellBox {
+  int lo0, lo1, lo2, nx_run, span1, nruns;
+};
+
+__device__ __forceinline__ CellBox cell_box(const GridLevel& GL, float q0, float q1, float q2, float rq) {
+  CellBox B{0, 0, 0, 0, 0, 0};
+  int lo0 = cell_floor((q0 - rq - GL.ox) * GL.inv_h, GL.nx), hi0 = cell_floor((q0 + rq - GL.ox) * GL.inv_h, GL.nx);
+  int lo1 = cell_floor((q1 - rq - GL.oy) * GL.inv_h, GL.ny), hi1 = cell_floor((q1 + rq - GL.oy) * GL.inv_h, GL.ny);
+  int lo2 = cell_floor((q2 - rq - GL.oz) * GL.inv_h, GL.nz), hi2 = cell_floor((q2 + rq - GL.oz) * GL.inv_h, GL.nz);
+  if (hi0 < 0 || lo0 >= GL.nx || hi1 < 0 || lo1 >= GL.ny || hi2 < 0 || lo2 >= GL.nz) return B;
+  lo0 = max(lo0, 0); hi0 = min(hi0, GL.nx - 1);
+  lo1 = max(lo1, 0); hi1 = min(hi1, GL.ny - 1);
+  lo2 = max(lo2, 0); hi2 = min(hi2, GL.nz - 1);
+  B.lo0 = lo0;
+  B.lo1 = lo1;
+  B.lo2 = lo2;
+  B.nx_run = hi0 - lo0 + 1;
+  B.span1 = hi1 - lo1 + 1;
+  B.nruns = B.span1 * (hi2 - lo2 + 1);
+  return B;
+}
+
+// Calls fn(valid, sorted position) for every target of the box, G at a time:
+// all G lanes of the group call together (valid false past the end), so fn
+// may use group-wide ballots and shuffles.
+template <int G, class Fn>
+__device__ __forceinline__ void for_each_in_box(const GridLevel& GL, const CellBox& B, int g, Fn&& fn) {
+  for (int rbase = 0; rbase < B.nruns; rbase += G) {
+    const int myrun = rbase + g;
+    uint32_t rs = 0, rl = 0;
+    if (myrun < B.nruns) {
+      const int iz = myrun / B.span1, iy = myrun - iz * B.span1;
+      const uint32_t c0 =
+          ((uint32_t)(B.lo2 + iz) * (uint32_t)GL.ny + (uint32_t)(B.lo1 + iy)) * (uint32_t)GL.nx + (uint32_t)B.lo0;
+      rs = GL.cell_start[c0];
+      rl = GL.cell_start[c0 + (uint32_t)B.nx_run] - rs;
+    }
+    uint32_t incl = rl;
+#pragma unroll
+    for (int o = 1; o < G; o <<= 1) {
+      uint32_t t = __shfl_up(incl, o, G);
+      if (g >= o) incl += t;
+    }
+    const uint32_t total = __shfl(incl, G - 1, G);
+    const uint32_t excl = incl - rl;
+    for (uint32_t base = 0; base < total; base += G) {
+      const uint32_t idx = base + (uint32_t)g;
+      int kk = 0;
+#pragma unroll
+      for (int s = G / 2; s > 0; s >>= 1) {
+        uint32_t e = __shfl(excl, kk + s, G);
+        if (e <= idx) kk += s;
+      }
+      const uint32_t run_start = __shfl(rs, kk, G);
+      const uint32_t run_excl = __shfl(excl, kk, G);
+      const bool valid = idx < total;
+      fn(valid, valid ? run_start + (idx - run_excl) : 0u);
+    }
+  }
+}
+
+// The group's bits of a wave ballot.
+template <int G>
+__device__ __forceinline__ uint32_t group_bits(uint64_t m) {
+  constexpr uint32_t kGroupMask = (uint32_t)((1ull << G) - 1ull);
+  return (uint32_t)(m >> (G * ((threadIdx.x & 63) / G))) & kGroupMask;
+}
+
+template <int FC, int KC, int G>
+__device__ __forceinline__ void knn_row(const AssocProblem& P, const KConsts& kc, const ExactK& ek,
+                                        const GridLevel& GL, const Maps& mp, int r, int g, int mode, float ell, int K,
+                                        const float* xf, const float* xs, uint32_t* hist, RowResult& out) {
+#pragma unroll
+  for (int c = 0; c < 7; ++c) out.acc[c] = 0.0;
+  out.stored = 0;
+  out.overflow = false;
+  const uint32_t Ku = (uint32_t)max(K, 0);
+  if (Ku == 0) return;
+  const int F = Dim<FC>::get(kc.F);
+  const int KS = Dim<KC>::get(kc.Ks);
+  float4 x4 = P.src_pos[r];
+  float x0 = x4.x, x1 = x4.y, x2 = x4.z;
+  if (mp.src_xf) {
+    float t0, t1, t2;
+    affine3(mp.As, mp.bs, x0, x1, x2, t0, t1, t2);
+    x0 = t0; x1 = t1; x2 = t2;
+  }
+  const float l = ell;
+  const float d2t = (float)(-2.0 * (double)l * (double)l * (double)kc.lg);
+  const float qg = (float)(1.0 / (2.0 * (double)l * (double)l));
+  float xg[2] = {0.0f, 0.0f};
+  if (kc.use_gtype) {
+    xg[0] = P.src_geo[2 * (size_t)r];
+    xg[1] = P.src_geo[2 * (size_t)r + 1];
+  }
+  float q0, q1, q2;
+  affine3(mp.Rq, mp.Tq, x0, x1, x2, q0, q1, q2);
+  const float rq = sqrtf(d2t) * 1.001f + 1e-4f * (1.0f + fmaxf(fabsf(q0), fmaxf(fabsf(q1), fabsf(q2))));
+  const CellBox B = cell_box(GL, q0, q1, q2, rq);
+  // key of a target, ~0 outside the ball (no ball key reaches it: d2 bits of
+  // a finite d2 < d2t stay below 0x7f800000)
+  auto key_of = [&](uint32_t pidx, float4& y4, float& y0, float& y1, float& y2) -> uint64_t {
+    y4 = GL.pos[pidx];
+    affine3(mp.A, mp.b, y4.x, y4.y, y4.z, y0, y1, y2);
+    const float dx = y0 - x0, dy = y1 - x1, dz = y2 - x2;
+    const float d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+    if (!(d2 < d2t)) return ~0ull;
+    return ((uint64_t)__float_as_uint(d2) << 32) | (uint64_t)__float_as_uint(y4.w);
+  };
+  // (1) ball size and key range
+  uint32_t cnt = 0;
+  uint64_t kmin = ~0ull, kmax = 0;
+  for_each_in_box<G>(GL, B, g, [&](bool valid, uint32_t pidx) {
+    if (!valid) return;
+    float4 y4;
+    float y0, y1, y2;
+    const uint64_t key = key_of(pidx, y4, y0, y1, y2);
+    if (key == ~0ull) return;
+    cnt++;
+    kmin = min(kmin, key);
+    kmax = max(kmax, key);
+  });
+#pragma unroll
+  for (int o = G / 2; o > 0; o >>= 1) {
+    cnt += __shfl_xor(cnt, o, G);
+    kmin = min(kmin, (uint64_t)__shfl_xor((unsigned long long)kmin, o, G));
+    kmax = max(kmax, (uint64_t)__shfl_xor((unsigned long long)kmax, o, G));
+  }
+  // (2) selection: a target is taken when (key >> tsh) <= (thr >> tsh)
+  uint64_t thr = ~0ull;
+  int tsh = 0;
+  if (cnt > Ku) {
+    // keys are distinct (target ids), so kmin != kmax and plen < 64
+    uint32_t need = Ku;
+    int plen = __clzll((long long)(kmin ^ kmax));
+    uint64_t prefix = plen > 0 ? (kmin & ~(~0ull >> plen)) : 0ull;
+    constexpr int CH = 256 / G;
+    for (int pass = 0; pass < 8 && plen < 64; ++pass) {
+      const int db = min(8, 64 - plen);
+      const int sh = 64 - plen - db;
+#pragma unroll
+      for (int i = 0; i < CH; ++i) hist[g * CH + i] = 0u;
+      wave_lds_sync();
+      for_each_in_box<G>(GL, B, g, [&](bool valid, uint32_t pidx) {
+        if (!valid) return;
+        float4 y4;
+        float y0, y1, y2;
+        const uint64_t key = key_of(pidx, y4, y0, y1, y2);
+        if (key == ~0ull) return;
+        if (plen > 0 && (key >> (64 - plen)) != (prefix >> (64 - plen))) return;
+        atomicAdd(&hist[(uint32_t)(key >> sh) & 0xffu], 1u);
+      });
+      wave_lds_sync();
+      // the bucket holding the need-th key: chunk sums, group prefix, then
+      // a sequential scan inside the one chunk that crosses need
+      uint32_t csum = 0;
+#pragma unroll
+      for (int i = 0; i < CH; ++i) csum += hist[g * CH + i];
+      uint32_t incl = csum;
+#pragma unroll
+      for (int o = 1; o < G; o <<= 1) {
+        uint32_t t = __shfl_up(incl, o, G);
+        if (g >= o) incl += t;
+      }
+      const uint32_t excl = incl - csum;
+      const bool mine = excl < need && incl >= need;
+      uint32_t b = 0, less = 0, cb = 0;
+      if (mine) {
+        uint32_t run = excl;
+        for (int i = 0; i < CH; ++i) {
+          const uint32_t h = hist[g * CH + i];
+          if (run + h >= need) {
+            b = (uint32_t)(g * CH + i);
+            less = run;
+            cb = h;
+            break;
+          }
+          run += h;
+        }
+      }
+      const uint32_t gm = group_bits<G>(__ballot(mine));
+      wave_lds_sync();  // histogram reads done before the next pass clears it
+      if (gm == 0u) break;  // cannot happen while the counts are consistent
+      const int owner = __ffs(gm) - 1;
+      b = __shfl(b, owner, G);
+      less = __shfl(less, owner, G);
+      cb = __shfl(cb, owner, G);
+      prefix |= (uint64_t)b << sh;
+      thr = prefix;
+      tsh = sh;
+      if (less + cb == need) break;
+      // continue inside bucket b; thr stays its upper bound meanwhile
+      need -= less;
+      plen += db;
+    }
+  }
+  // (3) gates on the selected targets, stored in slots by ballot
+  uint32_t stored = 0;
+  const uint64_t thr_s = thr >> tsh;
+  for_each_in_box<G>(GL, B, g, [&](bool valid, uint32_t pidx) {
+    bool pass = false;
+    float a = 0.0f, y0 = 0.0f, y1 = 0.0f, y2 = 0.0f;
+    float4 y4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (valid) {
+      const uint64_t key = key_of(pidx, y4, y0, y1, y2);
+      if (key != ~0ull && (key >> tsh) <= thr_s)
+        pass = pair_kernel<FC, KC>(kc, ek, x0, x1, x2, y0, y1, y2, d2t, qg, l, xf,
+                                   FC != 0 ? GL.feat + (size_t)pidx * F : nullptr, xs,
+                                   KC != 0 ? GL.sem + (size_t)pidx * KS : nullptr, xg,
+                                   kc.use_gtype ? GL.geo + 2 * (size_t)pidx : nullptr, a, 0.001f);
+    }
+    const uint32_t gm = group_bits<G>(__ballot(pass));
+    const uint32_t slot = stored + (uint32_t)__popc(gm & ((1u << g) - 1u));
+    if (pass && slot < Ku) {
+      if (mode == MODE_BA) P.pairs4[(size_t)r * Ku + slot] = make_float4(y4.x, y4.y, y4.z, a);
+      else P.pairs[(size_t)r * P.cap_alloc + slot] = make_uint2(pidx, __float_as_uint(a));
+      if (mode == MODE_ALIGN) {
+        const float c0 = x1 * y2 - x2 * y1, c1 = x2 * y0 - x0 * y2, c2 = x0 * y1 - x1 * y0;
+        out.acc[0] += (double)(c0 * a);
+        out.acc[1] += (double)(c1 * a);
+        out.acc[2] += (double)(c2 * a);
+        out.acc[3] += (double)((y0 - x0) * a);
+        out.acc[4] += (double)((y1 - x1) * a);
+        out.acc[5] += (double)((y2 - x2) * a);
+      }
+      out.acc[6] += (double)a;
+    }
+    stored += (uint32_t)__popc(gm);
+  });
+  out.stored = min(stored, Ku);
+}
+
 // Block partials of the association pass: part_a[block] = flow6, ip, -;
 // part_cnt[block] = stored pairs, max row; summed into the problem state by
 // the capped-row kernel that follows.
@@ -426,10 +683,11 @@ struct AssocLds {
   float sem[kMaxRowsPerBlock * KL];   // and class vectors
 };
 
-// Logical block lb of a problem with G lanes per row (256 / G rows).
-template <int FC, int KC, int G>
+// Logical block lb of a problem with G lanes per row (256 / G rows); KNN:
+// the K-nearest rows (hist: 256 bins per row).
+template <int FC, int KC, int G, bool KNN>
 __device__ __forceinline__ void assoc_block(const AssocProblem& P, const KConsts& kc, int lb,
-                                            AssocLds<FC, KC>& L) {
+                                            AssocLds<FC, KC>& L, uint32_t* hist) {
   constexpr int RB = rows_per_block(G);
   const ProblemState* st = P.st;
   const int tid = threadIdx.x;
@@ -460,9 +718,14 @@ __device__ __forceinline__ void assoc_block(const AssocProblem& P, const KConsts
   for (int c = 0; c < 7; ++c) res.acc[c] = 0.0;
   res.stored = 0;
   res.overflow = false;
-  if (r < P.n_src)
-    assoc_row<FC, KC, G>(P, kc, ek, GL, mp, r, g, mode, ell, K, L.feat + grp * Dim<FC>::get(kc.F),
-                         L.sem + grp * Dim<KC>::get(kc.Ks), res);
+  if (r < P.n_src) {
+    if constexpr (KNN)
+      knn_row<FC, KC, G>(P, kc, ek, GL, mp, r, g, mode, ell, K, L.feat + grp * Dim<FC>::get(kc.F),
+                         L.sem + grp * Dim<KC>::get(kc.Ks), hist + grp * 256, res);
+    else
+      assoc_row<FC, KC, G>(P, kc, ek, GL, mp, r, g, mode, ell, K, L.feat + grp * Dim<FC>::get(kc.F),
+                           L.sem + grp * Dim<KC>::get(kc.Ks), res);
+  }
 #pragma unroll
   for (int c = 0; c < 7; ++c) {
     double v = res.acc[c];
@@ -545,8 +808,30 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
   __shared__ AssocLds<FC, KC> L;
   switch (group) {
     // two lane counts keep the kernel within 128 VGPRs (four waves per SIMD)
-    case 8: assoc_block<FC, KC, 8>(P, kc, lb, L); break;
-    default: assoc_block<FC, KC, 16>(P, kc, lb, L); break;
+    case 8: assoc_block<FC, KC, 8, false>(P, kc, lb, L, nullptr); break;
+    default: assoc_block<FC, KC, 16, false>(P, kc, lb, L, nullptr); break;
+  }
+}
+
+// The K-nearest association pass (is_using_kdtree), same launch shape and
+// outputs; rows never exceed their cap, so no capped-row work follows.
+template <int FC, int KC>
+__global__ __launch_bounds__(kBlock) void knn_assoc_kernel(const AssocProblem* __restrict__ probs, int n_probs,
+                                                           KConsts kc) {
+  const int p = find_problem(probs, n_probs, (int)blockIdx.x);
+  const AssocProblem& P = probs[p];
+  const ProblemState* st = P.st;
+  if (st->done) return;
+  const int group = state_group(st);
+  const int n_act = active_blocks(P, group);
+  int lb = (int)blockIdx.x - P.block_begin;
+  if (lb >= n_act) return;  // uniform
+  if (kc.xcd_remap) lb = xcd_block(lb, n_act);
+  __shared__ AssocLds<FC, KC> L;
+  __shared__ uint32_t hist[kMaxRowsPerBlock * 256];
+  switch (group) {
+    case 8: assoc_block<FC, KC, 8, true>(P, kc, lb, L, hist); break;
+    default: assoc_block<FC, KC, 16, true>(P, kc, lb, L, hist); break;
   }
 }
 
@@ -946,8 +1231,15 @@ __device__ void controller(ProblemState* st, const KConsts& kc, const AssocProbl
     if (e < kc.ell_min) e = kc.ell_min;
     st->ell = e;
   }
-  st->K = min(kc.nn_max, (int)((double)st->max_row * 1.2));
-  if (st->K < 1) st->K = 1;
+  if (kc.knn > 0) {
+    // K nearest: the mean stored row rounded up times is_using_kdtree, at
+    // most KDTREE_K_SIZE (CvoGPU.cu:1416-1417); zero stays zero
+    const int mean = (int)ceil((double)st->nnz / (double)P.n_src);
+    st->K = min(kKnnMaxK, mean * kc.knn);
+  } else {
+    st->K = min(kc.nn_max, (int)((double)st->max_row * 1.2));
+    if (st->K < 1) st->K = 1;
+  }
   st->iter = k + 1;
   if (st->iter >= kc.max_iter) {
     st->done = 1;
@@ -1268,7 +1560,10 @@ void launch_all_done(const AssocProblem* probs, int n_probs, int32_t* out, hipSt
 template <int FC, int KC>
 static void launch_assoc_t(const AssocProblem* probs, int n_probs, int total_blocks, const KConsts& kc,
                            hipStream_t s) {
-  hipLaunchKernelGGL((assoc_kernel<FC, KC>), dim3(total_blocks), dim3(kBlock), 0, s, probs, n_probs, kc);
+  if (kc.knn > 0)
+    hipLaunchKernelGGL((knn_assoc_kernel<FC, KC>), dim3(total_blocks), dim3(kBlock), 0, s, probs, n_probs, kc);
+  else
+    hipLaunchKernelGGL((assoc_kernel<FC, KC>), dim3(total_blocks), dim3(kBlock), 0, s, probs, n_probs, kc);
 }
 
 template <int FC, int KC>

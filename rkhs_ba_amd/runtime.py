@@ -56,6 +56,7 @@ def lib() -> C.CDLL:
         L.cvo_function_angle.argtypes = [P, P, P, P, f, i32, P]
         L.cvo_compute_association.argtypes = [P, P, P, P, f, P]
         L.cvo_compute_association_nonisotropic.argtypes = [P, P, P, P, P, P]
+        L.cvo_compute_association_knn.argtypes = [P, P, P, P, f, i32, P]
         L.cvo_align.argtypes = [P, P, P, P, P, P, P, P, P]
         L.cvo_align_d.argtypes = [P, P, P, P, P, P, P, P, P]
         L.cvo_align_multistart.argtypes = [P, P, P, i32, P, f, i32, P, P, P, P, P, P]
@@ -417,6 +418,21 @@ class CvoGPU:
         else:
             _check(lib().cvo_compute_association(self._ctx, hs.ref, ht.ref, T0.ctypes.data, C.c_float(lengthscale),
                                                  C.byref(a)), self._ctx)
+        if not a.row_ptr:
+            n = hs.view.n
+            return Association(n, ht.view.n, np.zeros(n + 1, np.int64), np.zeros(0, np.int32), np.zeros(0, np.float32))
+        return _take_assoc(a)
+
+    def compute_association_knn(self, source, target, T_target_frame_to_source_frame, lengthscale,
+                                K: int) -> Association:
+        """The K-nearest association pass of align with is_using_kdtree
+        (se_kernel_kdtree, CvoGPU.cu:1104-1154) at one pose, lengthscale and
+        row cap K in [1, 512]."""
+        T0 = mat16(T_target_frame_to_source_frame)
+        hs, ht = HeldView(source), HeldView(target)
+        a = Assoc()
+        _check(lib().cvo_compute_association_knn(self._ctx, hs.ref, ht.ref, T0.ctypes.data, C.c_float(lengthscale),
+                                                 int(K), C.byref(a)), self._ctx)
         if not a.row_ptr:
             n = hs.view.n
             return Association(n, ht.view.n, np.zeros(n + 1, np.int64), np.zeros(0, np.int32), np.zeros(0, np.float32))

@@ -1,0 +1,215 @@
+"""GPU K-nearest association (is_using_kdtree) against the oracle.
+
+Bars (written in each test):
+  * one pass (cvo_compute_association_knn): the same (row, target) sets as the
+    oracle, except pairs whose value lies within 1e-5 relative of sp_thres;
+    stored values within 4e-6 relative (one FP32 exp of the summed exponent
+    against three FP64 exps rounded to float).  The distances are the same
+    float operations on both sides, so the K-th place is decided identically;
+  * align: iterations, return code and the row-cap schedule (512, then
+    min(512, ceil(nnz / N) * is_using_kdtree)) equal to the oracle's policy
+    mode, the first iterations' caps and pair counts equal, the converged
+    pose within 5e-5 (the spread the oracle itself shows from a 1e-7
+    change of the start);
+  * multi-start: every start bitwise equal to a single-start align;
+  * bundle adjustment: poses within 1e-9 of the oracle after one outer
+    iteration.
+"""
+import math
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import CONFIGS, params_from_yaml, rot_dist_rad, trans_err
+from knn_cases import knn_pair, knn_params
+from rkhs_ba_amd import CvoError, CvoGPU, synth
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def compare_knn(oracle, p, src, tgt, T, ell, K, g, val_rtol=4e-6):
+    counts, cols, vals, _ = oracle.assoc_knn(p, src, tgt, T, ell, K, precision=oracle.MIRROR)
+    want = {(i, int(cols[i, s])): float(vals[i, s]) for i in range(src.n) for s in range(counts[i])}
+    rows, gcol, gval = g.pairs()
+    got = {(int(r), int(c)): float(v) for r, c, v in zip(rows, gcol, gval)}
+    diff = set(got) ^ set(want)
+    bad = [k for k in diff if abs(got.get(k, want.get(k)) - p.sp_thres) > 1e-5 * p.sp_thres]
+    assert not bad, bad[:10]
+    common = set(got) & set(want)
+    rel = max((abs(got[k] - want[k]) / want[k] for k in common), default=0.0)
+    assert rel <= val_rtol, rel
+    assert np.diff(g.row_ptr).max(initial=0) <= K
+    for i in range(src.n):
+        seg = gcol[g.row_ptr[i]:g.row_ptr[i + 1]]
+        assert (np.diff(seg) > 0).all()
+    return len(common)
+
+
+@pytest.fixture(scope="module")
+def knn_ctx():
+    c = CvoGPU(os.path.join(CONFIGS, "planes_spheres_5k.yaml"))
+    yield c
+    c.close()
+
+
+def test_knn_pass_matches_oracle(knn_ctx, oracle):
+    p = knn_params(params_from_yaml("planes_spheres_5k.yaml"))
+    knn_ctx.write_params(p)
+    src, tgt, T = knn_pair(0)
+    for K, ell in [(512, 0.5), (40, 0.5), (5, 0.5), (1, 0.3), (512, 0.1), (17, 0.9)]:
+        say(f"knn pass K={K} ell={ell}")
+        g = knn_ctx.compute_association_knn(src, tgt, T, ell, K)
+        n = compare_knn(oracle, p, src, tgt, T, ell, K, g)
+        assert n > 300
+
+
+def test_knn_pass_street_semantics_tuned_shapes(knn_ctx, oracle):
+    """19 classes, 20k points, both lane counts and cell scales: the same pairs."""
+    p = params_from_yaml("street_60k.yaml")
+    knn_ctx.write_params(p)
+    src, tgt, Tgt = synth.street_pair(4, n=20000)
+    cases = [(24, 0.5), (512, 0.2)]
+    ref = {}
+    knn_ctx.set_tuning(16, 1, 1.0)
+    for K, ell in cases:
+        say(f"knn pass street20k lanes=16 scale=1.0 K={K} ell={ell}")
+        g = knn_ctx.compute_association_knn(src, tgt, Tgt, ell, K)
+        assert compare_knn(oracle, p, src, tgt, Tgt, ell, K, g) > 10000
+        ref[(K, ell)] = g
+    knn_ctx.set_tuning(8, 0, 0.5)
+    for K, ell in cases:
+        say(f"knn pass street20k lanes=8 scale=0.5 K={K} ell={ell}")
+        g = knn_ctx.compute_association_knn(src, tgt, Tgt, ell, K)
+        np.testing.assert_array_equal(g.row_ptr, ref[(K, ell)].row_ptr)
+        np.testing.assert_array_equal(g.col, ref[(K, ell)].col)
+    knn_ctx.set_tuning(16, 1, 1.0)
+
+
+def test_knn_ties_and_errors(knn_ctx):
+    from rkhs_ba_amd.pointcloud import CvoPointCloud
+    p = params_from_yaml("planes_spheres_5k.yaml")
+    p.is_using_intensity = 0
+    knn_ctx.write_params(p)
+    x = np.array([[10.0, 0.0, 0.0]], np.float32)
+    off = np.array([[0.01, 0, 0], [0, 0.02, 0], [0, 0, 0.03], [0.04, 0, 0]], np.float32)
+    y = x + np.concatenate([off, off, off])
+    s, t = CvoPointCloud.from_arrays(x), CvoPointCloud.from_arrays(y)
+    say("knn ties")
+    g = knn_ctx.compute_association_knn(s, t, np.eye(4), 0.5, 5)
+    assert sorted(g.col.tolist()) == [0, 1, 4, 5, 8]
+    for bad_k in (0, 513):
+        with pytest.raises(CvoError):
+            knn_ctx.compute_association_knn(s, t, np.eye(4), 0.5, bad_k)
+    p.is_using_geometry = 0
+    knn_ctx.write_params(p)
+    with pytest.raises(CvoError):
+        knn_ctx.compute_association_knn(s, t, np.eye(4), 0.5, 5)
+    p.is_using_geometry = 1
+    p.is_using_kdtree = 600
+    knn_ctx.write_params(p)
+    with pytest.raises(CvoError):
+        knn_ctx.align(s, t, np.eye(4))
+
+
+@pytest.mark.parametrize("mult", [1, 3])
+def test_knn_align_matches_oracle(knn_ctx, oracle, mult):
+    # the full 2000-iteration workload: the trajectory is sensitive in its
+    # middle (a 1e-7 change of the start moves the oracle's own pose by
+    # ~1e-3 around iteration 60, in the ball association as well) and
+    # converges by the end (the oracle from that perturbed start ends within
+    # 1.5e-8 for m = 1 and 2e-5 rad / 8e-6 m for m = 3): poses within 5e-5
+    p = params_from_yaml("planes_spheres_5k.yaml")
+    p.is_using_kdtree = mult
+    knn_ctx.write_params(p)
+    src, tgt, Tgt = synth.planes_spheres_pair(0)
+    knn_ctx.set_trace(p.MAX_ITER)
+    say(f"knn align 5k m={mult}")
+    ret, T, secs, iters, _ = knn_ctx.align(src, tgt, np.eye(4))
+    tr = knn_ctx.get_trace()
+    knn_ctx.set_trace(0)
+    To, it_o, ret_o, tr_o = oracle.align(p, src, tgt, np.eye(4), precision=oracle.POLICY, trace_cap=p.MAX_ITER)
+    assert (ret, iters) == (ret_o, it_o)
+    # the row-cap schedule, and the first 25 iterations step by step
+    assert tr[0].num_neighbors == 512
+    for k in range(len(tr) - 1):
+        assert tr[k + 1].num_neighbors == min(512, math.ceil(tr[k].nnz / src.n) * mult), k
+    assert [(r.num_neighbors, r.nnz) for r in tr[:25]] == [(r.num_neighbors, r.nnz) for r in tr_o[:25]]
+    assert rot_dist_rad(T, To) <= 5e-5 and trans_err(T, To) <= 5e-5, (rot_dist_rad(T, To), trans_err(T, To))
+    Tinv = np.linalg.inv(Tgt)
+    assert rot_dist_rad(T, Tinv) < 0.01 * rot_dist_rad(np.eye(4), Tinv)
+
+
+def test_knn_align_all_gates_far_scene(knn_ctx, oracle):
+    """Colour, semantics and geometric type, 60 m from the origin: the first
+    twelve iterations step by step (this scene has no single attractor, so
+    later poses drift apart under rounding-level differences)."""
+    p = knn_params(params_from_yaml("planes_spheres_5k.yaml"))
+    p.is_using_kdtree = 2
+    p.MAX_ITER = 12
+    knn_ctx.write_params(p)
+    src, tgt, T = knn_pair(0)
+    knn_ctx.set_trace(p.MAX_ITER)
+    say("knn align far scene, colour + semantics + geometric type, 12 iterations")
+    ret, Tg, secs, iters, _ = knn_ctx.align(src, tgt, np.eye(4))
+    tr = knn_ctx.get_trace()
+    knn_ctx.set_trace(0)
+    To, it_o, ret_o, tr_o = oracle.align(p, src, tgt, np.eye(4), precision=oracle.POLICY, trace_cap=p.MAX_ITER)
+    assert (ret, iters) == (ret_o, it_o) and iters == 12
+    assert [(r.num_neighbors, r.nnz) for r in tr] == [(r.num_neighbors, r.nnz) for r in tr_o]
+    # the stored values differ by up to 4e-6 relative (one FP32 exp against
+    # three FP64 exps), so do the flows summed from them
+    for a, b in zip(tr, tr_o):
+        assert abs(a.flow_norm - b.flow_norm) <= 1e-5 * b.flow_norm
+    assert rot_dist_rad(Tg, To) <= 1e-5 and trans_err(Tg, To) <= 1e-5, (rot_dist_rad(Tg, To), trans_err(Tg, To))
+
+
+def test_knn_multistart_equals_single_starts(knn_ctx):
+    p = params_from_yaml("multistart_20k.yaml")
+    p.is_using_kdtree = 2
+    p.MAX_ITER = 150
+    knn_ctx.write_params(p)
+    src, tgt, Tgt = synth.planes_spheres_pair(11, 3000, 3000)
+    rng = np.random.default_rng(1)
+    Ts = [np.eye(4)]
+    for _ in range(3):
+        R = synth.rot_axis_angle(rng.normal(size=3), np.deg2rad(rng.uniform(0, 20)))
+        Ts.append(synth.rigid(R, rng.normal(size=3) * 0.05))
+    Ts = np.array(Ts, np.float32)
+    say("knn multistart 4 starts")
+    out = knn_ctx.align_multistart(src, tgt, Ts, 0.3, mode=0)
+    for s in range(len(Ts)):
+        say(f"knn single start {s}")
+        ret, T, secs, iters, _ = knn_ctx.align(src, tgt, Ts[s])
+        assert np.array_equal(T, out["per_start_T"][s]), s
+        assert iters == out["per_start_iters"][s]
+
+
+def test_knn_ba_one_outer_iteration_matches_oracle(oracle):
+    from test_gpu_ba import all_edges, window
+    p = params_from_yaml("ba_window.yaml")
+    p.is_using_kdtree = 1
+    p.multiframe_max_iters = 1
+    clouds, gts, inits = window()
+    c = CvoGPU(os.path.join(CONFIGS, "ba_window.yaml"))
+    try:
+        c.write_params(p)
+        say("knn BA window, one outer iteration")
+        g = c.align_multiframe(clouds, inits, [1, 0, 0, 0], all_edges(4))
+        p.multiframe_num_neighbors = 600
+        c.write_params(p)
+        with pytest.raises(CvoError):
+            c.align_multiframe(clouds, inits, [1, 0, 0, 0], all_edges(4))
+    finally:
+        c.close()
+    p.multiframe_num_neighbors = 256
+    P, it, _ = oracle.align_multiframe(p, clouds, inits, [1, 0, 0, 0], all_edges(4), threads=8)
+    assert g["iters"] == it == 1
+    assert np.abs(g["poses"][:, :3, :4] - inits[:, :3, :4]).max() > 1e-6
+    np.testing.assert_allclose(g["poses"][:, :3, :4], P[:, :3, :4], rtol=0, atol=1e-9)
