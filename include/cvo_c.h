@@ -212,6 +212,28 @@ cvo_status cvo_align_d(cvo_ctx* ctx, const cvo_dcloud* src, const cvo_dcloud* tg
                        const float T_init[16], float T_out[16], cvo_assoc* assoc_out,
                        double* seconds, int32_t* iters, int32_t* ret_code);
 
+/* AdaptiveCvoGPU::align (AdaptiveCvoGPU.hpp:171-175, AdaptiveCvoGPU.cu:1077-1248):
+ * the lengthscale follows its gradient instead of the indicator schedule.
+ * Each iteration associates source-target at the fixed lengthscale plus the
+ * two self associations (source-source, moved target-target), steps along the
+ * unnormalised flow (omega / c, v / d; step = smallest positive real root of
+ * the cubic, min_step if none, at most 0.8), then
+ * ell <- ell - dl_step * dl with
+ *   dl = (2/l^3 sum_A -a|y-x|^2 + 1/l^3 sum_Axx a|x'-x|^2 + 1/l^3 sum_Ayy a|y'-y|^2)
+ *        / (nnz(Axx) + nnz(Ayy) - 2 nnz(A)),
+ * shrinking ell and ell_max to 0.7 ell_max past ell_max, at least ell_min.
+ * Association: colour gate always on (features required on both clouds),
+ * class factor without a gate when is_using_semantics, no geometric type, the
+ * first min(512, nearest_neighbors_max) passing targets by index.  Pose
+ * conventions and stop tests as cvo_align; the trace records each iteration's
+ * lengthscale. */
+cvo_status cvo_align_adaptive(cvo_ctx* ctx, const cvo_cloud_view* src, const cvo_cloud_view* tgt,
+                              const float T_init[16], float T_out[16], double* seconds, int32_t* iters,
+                              int32_t* ret_code);
+cvo_status cvo_align_adaptive_d(cvo_ctx* ctx, const cvo_dcloud* src, const cvo_dcloud* tgt,
+                                const float T_init[16], float T_out[16], double* seconds, int32_t* iters,
+                                int32_t* ret_code);
+
 /* Optional per-iteration trace of the next align call(s): up to cap records. */
 cvo_status cvo_set_trace(cvo_ctx* ctx, int32_t cap);
 cvo_status cvo_get_trace(cvo_ctx* ctx, cvo_iter_record* out, int32_t cap, int32_t* n_out);

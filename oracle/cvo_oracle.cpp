@@ -794,6 +794,9 @@ void cubic_roots(const double c[4], double re[3], double im[3]) {
     }
     re[k] = (double)z[k].real();
     im[k] = (double)z[k].imag();
+    // a real root reached from a complex start keeps an imaginary residue
+    // far below rounding (1e-98 and the like): it is real
+    if (std::fabs(im[k]) <= 1e-14 * std::max(1.0, std::fabs(re[k]))) im[k] = 0.0;
   }
 }
 
@@ -1204,6 +1207,246 @@ extern "C" int orc_align(const cvo_params* p, const cvo_cloud_view* src, const c
       for (int c = 0; c < 3; ++c) T_out[r * 4 + c] = (float)Rd[c * 3 + r];
       T_out[r * 4 + 3] = (float)(-(Rd[0 * 3 + r] * Td[0] + Rd[1 * 3 + r] * Td[1] + Rd[2 * 3 + r] * Td[2]));
     }
+  }
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Adaptive-lengthscale CVO (AdaptiveCvoGPU::align, AdaptiveCvoGPU.cu:1077-1248)
+// ---------------------------------------------------------------------------
+
+namespace {
+
+// fill_in_A_mat_gpu of AdaptiveCvoGPU.cu:263-408: rows of X against Y in
+// ascending target index, the first K that pass: distance gate at the fixed
+// lengthscale, colour gate always, class factor without a gate (when
+// is_using_semantics), no geometric type; each factor an FP64 exp rounded to
+// float.  Also the per-row sums of a |y - x|^2 (float row sums for the
+// mirror, FP64 of the float products otherwise).
+struct AcvoRows {
+  std::vector<int32_t> counts, cols;
+  std::vector<float> vals;
+  std::vector<double> ad2;  // per row
+  int64_t nnz = 0;
+};
+
+void acvo_assoc(const cvo_params* p, int N, const float* X, const float* XF, const float* XS, int M, const float* Y,
+                const float* YF, const float* YS, int F, int C, float ell, int K, int precision, AcvoRows& o) {
+  const float s2 = p->sigma * p->sigma;
+  const float l = ell;
+  const float d2_thres = (float)(-2.0 * (double)l * (double)l * (double)logf(p->sp_thres / s2));
+  const float d2_c_thres = (float)(-2.0 * (double)p->c_ell * (double)p->c_ell *
+                                   (double)logf(p->sp_thres / p->c_sigma / p->c_sigma));
+  const bool sem = p->is_using_semantics && C > 0 && XS && YS;
+  o.counts.assign(N, 0);
+  o.cols.assign((size_t)N * K, -1);
+  o.vals.assign((size_t)N * K, 0.0f);
+  o.ad2.assign(N, 0.0);
+  int64_t nnz = 0;
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : nnz)
+  for (int i = 0; i < N; ++i) {
+    const float* x = X + 3 * (size_t)i;
+    int n = 0;
+    float rowf = 0.0f;
+    double rowd = 0.0;
+    for (int j = 0; j < M && n < K; ++j) {
+      const float* y = Y + 3 * (size_t)j;
+      const float d2 = sqdist3_f(y, x);
+      if (!(d2 < d2_thres)) continue;
+      const float d2c = sqdist_n_f(XF + (size_t)i * F, YF + (size_t)j * F, F);
+      if (!(d2c < d2_c_thres)) continue;
+      const float k = (float)((double)s2 * std::exp(-(double)d2 / (2.0 * (double)l * (double)l)));
+      const float ck = (float)((double)(p->c_sigma * p->c_sigma) *
+                               std::exp(-(double)d2c / (2.0 * (double)p->c_ell * (double)p->c_ell)));
+      float sk = 1.0f;
+      if (sem) {
+        const float d2s = sqdist_n_f(XS + (size_t)i * C, YS + (size_t)j * C, C);
+        sk = (float)((double)(p->s_sigma * p->s_sigma) *
+                     std::exp(-(double)d2s / (2.0 * (double)p->s_ell * (double)p->s_ell)));
+      }
+      const float a = ck * k * sk;
+      if (!(a > p->sp_thres)) continue;
+      o.cols[(size_t)i * K + n] = j;
+      o.vals[(size_t)i * K + n] = a;
+      n++;
+      rowf += d2 * a;
+      rowd += (double)(d2 * a);
+    }
+    o.counts[i] = n;
+    o.ad2[i] = precision == ORC_MIRROR ? (double)rowf : rowd;
+    nnz += n;
+  }
+  o.nnz = nnz;
+}
+
+// Step of the adaptive loop (AdaptiveCvoGPU.cu:1039-1074): the cubic
+// 4E s^3 + 3D s^2 + 2C s + B with float coefficients, its smallest positive
+// real root, min_step when there is none, at most 0.8.
+float adaptive_select_step(const cvo_params* p, const double bcde[4]) {
+  const double cf[4] = {(double)(4.0f * (float)bcde[3]), (double)(3.0f * (float)bcde[2]),
+                        (double)(2.0f * (float)bcde[1]), (double)(float)bcde[0]};
+  double re[3], im[3];
+  cubic_roots(cf, re, im);
+  double temp = std::numeric_limits<double>::max();
+  for (int i = 0; i < 3; ++i)
+    if (std::isfinite(re[i]) && re[i] > 0 && re[i] < temp && im[i] == 0.0) temp = re[i];
+  float step = temp == std::numeric_limits<double>::max() ? p->min_step : (float)temp;
+  return step > 0.8f ? 0.8f : step;
+}
+
+}  // namespace
+
+extern "C" float orc_select_step_adaptive(const cvo_params* p, const double bcde[4]) {
+  return adaptive_select_step(p, bcde);
+}
+
+extern "C" int orc_assoc_adaptive(const cvo_params* p, const cvo_cloud_view* src, const cvo_cloud_view* tgt,
+                                  const float T[16], float ell, int32_t K, int32_t precision, int32_t* counts,
+                                  int32_t* cols, float* vals, double* ad2) {
+  if (!p || !src || !tgt || !T || K <= 0 || !src->feat || !tgt->feat) return 1;
+  const int M = tgt->n;
+  std::vector<float> yp((size_t)M * 3);
+  float Am[9], b[3];
+  applied_transform(pose_from16(T), Am, b);
+  for (int j = 0; j < M; ++j) xform_f(Am, b, tgt->xyz + 3 * (size_t)j, &yp[3 * (size_t)j]);
+  AcvoRows o;
+  const int C = p->is_using_semantics ? src->num_classes : 0;
+  acvo_assoc(p, src->n, src->xyz, src->feat, src->sem, M, yp.data(), tgt->feat, tgt->sem, src->feat_dim, C, ell, K,
+             precision, o);
+  if (counts) std::memcpy(counts, o.counts.data(), sizeof(int32_t) * o.counts.size());
+  if (cols) std::memcpy(cols, o.cols.data(), sizeof(int32_t) * o.cols.size());
+  if (vals) std::memcpy(vals, o.vals.data(), sizeof(float) * o.vals.size());
+  if (ad2) std::memcpy(ad2, o.ad2.data(), sizeof(double) * o.ad2.size());
+  return 0;
+}
+
+extern "C" int orc_align_adaptive(const cvo_params* p, const cvo_cloud_view* src, const cvo_cloud_view* tgt,
+                                  const float T_init[16], int32_t precision, float T_out[16], int32_t* iters,
+                                  int32_t* ret_code, cvo_iter_record* trace, int32_t trace_cap) {
+  if (!p || !src || !tgt || !T_init || !T_out || !src->feat || !tgt->feat) return 1;
+  if (precision != ORC_MIRROR && precision != ORC_POLICY) return 1;
+  const int N = src->n, M = tgt->n, F = src->feat_dim;
+  const int C = p->is_using_semantics ? src->num_classes : 0;
+  const int K = std::min(kKnnK, p->nearest_neighbors_max);
+  const bool float_pose = precision == ORC_MIRROR;
+  cvo_params ps = *p;
+  ps.is_using_range_ell = 0;  // the step's 1/(2 l^2) uses the lengthscale as is
+  float Rf[9], Tf[3];
+  double Rd[9], Td[3];
+  for (int r = 0; r < 3; ++r) {
+    for (int c = 0; c < 3; ++c) Rf[r * 3 + c] = Rd[r * 3 + c] = T_init[r * 4 + c];
+    Tf[r] = Td[r] = T_init[r * 4 + 3];
+  }
+  float ell = p->ell_init, ell_max = p->ell_max;
+  int ret = 0, k = 0;
+  std::vector<float> yp((size_t)M * 3);
+  AcvoRows A, Axx, Ayy;
+  for (; k < p->MAX_ITER; ++k) {
+    Pose64 P;
+    for (int i = 0; i < 9; ++i) P.R[i] = float_pose ? (double)Rf[i] : Rd[i];
+    for (int i = 0; i < 3; ++i) P.T[i] = float_pose ? (double)Tf[i] : Td[i];
+    float Am[9], b[3];
+    applied_transform(P, Am, b);
+    for (int j = 0; j < M; ++j) xform_f(Am, b, tgt->xyz + 3 * (size_t)j, &yp[3 * (size_t)j]);
+    acvo_assoc(p, N, src->xyz, src->feat, src->sem, M, yp.data(), tgt->feat, tgt->sem, F, C, ell, K, precision, A);
+    acvo_assoc(p, N, src->xyz, src->feat, src->sem, N, src->xyz, src->feat, src->sem, F, C, ell, K, precision, Axx);
+    acvo_assoc(p, M, yp.data(), tgt->feat, tgt->sem, M, yp.data(), tgt->feat, tgt->sem, F, C, ell, K, precision, Ayy);
+    double flow6[6], unit6[6];
+    flow_core(p, N, src->xyz, yp.data(), K, A.counts.data(), A.cols.data(), A.vals.data(), precision, flow6, unit6);
+    float xf[6];
+    for (int d = 0; d < 6; ++d) xf[d] = (float)flow6[d];
+    double bcde[4];
+    step_core(&ps, ell, N, src->xyz, yp.data(), K, A.counts.data(), A.cols.data(), A.vals.data(), flow6, precision,
+              bcde);
+    const double nw = std::sqrt((double)xf[0] * xf[0] + (double)xf[1] * xf[1] + (double)xf[2] * xf[2]);
+    const double nv = std::sqrt((double)xf[3] * xf[3] + (double)xf[4] * xf[4] + (double)xf[5] * xf[5]);
+    cvo_iter_record rec{};
+    rec.ell = ell;
+    rec.nnz = (int32_t)A.nnz;
+    rec.num_neighbors = K;
+    rec.flow_norm = (float)std::sqrt(nw * nw + nv * nv);
+    if (nw < p->eps && nv < p->eps) {
+      if (nw < 1e-8 && nv < 1e-8) ret = -1;
+      if (trace && k < trace_cap) trace[k] = rec;
+      break;
+    }
+    // step: coefficients in float, smallest positive real root, min_step if
+    // none, at most 0.8 (AdaptiveCvoGPU.cu:1039-1074)
+    const float step = adaptive_select_step(p, bcde);
+    rec.step = step;
+    double dRT[12];
+    if (float_pose) {
+      float dt[12];
+      exp_sek3_f32(xf, step, dt);
+      float Tn[3], Rn[9];
+      for (int r = 0; r < 3; ++r)
+        Tn[r] = (float)((double)Rf[r * 3 + 0] * dt[3] + (double)Rf[r * 3 + 1] * dt[7] + (double)Rf[r * 3 + 2] * dt[11] +
+                        (double)Tf[r]);
+      for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c)
+          Rn[r * 3 + c] = (float)((double)Rf[r * 3 + 0] * dt[c] + (double)Rf[r * 3 + 1] * dt[4 + c] +
+                                  (double)Rf[r * 3 + 2] * dt[8 + c]);
+      std::memcpy(Rf, Rn, sizeof(Rf));
+      std::memcpy(Tf, Tn, sizeof(Tf));
+      for (int i = 0; i < 12; ++i) dRT[i] = dt[i];
+    } else {
+      double xi[6];
+      for (int d = 0; d < 6; ++d) xi[d] = (double)xf[d];
+      exp_se3_f64(xi, (double)step, dRT);
+      double Rn[9], Tn[3];
+      for (int r = 0; r < 3; ++r) {
+        Tn[r] = Td[r];
+        for (int c = 0; c < 3; ++c) Tn[r] += Rd[r * 3 + c] * dRT[c * 4 + 3];
+        for (int c = 0; c < 3; ++c) {
+          double sacc = 0;
+          for (int m = 0; m < 3; ++m) sacc += Rd[r * 3 + m] * dRT[m * 4 + c];
+          Rn[r * 3 + c] = sacc;
+        }
+      }
+      std::memcpy(Rd, Rn, sizeof(Rd));
+      std::memcpy(Td, Tn, sizeof(Td));
+    }
+    const double dist = log_se3_norm(dRT);
+    rec.dist = (float)dist;
+    rec.ip = (float)((double)A.nnz / std::sqrt((double)N * (double)M));
+    if (trace && k < trace_cap) trace[k] = rec;
+    if (dist < p->eps_2) break;
+    // the lengthscale gradient (AdaptiveCvoGPU.cu:548-720, 839-890)
+    const float l3 = ell * ell * ell;
+    double sa = 0, sxx = 0, syy = 0;
+    for (int i = 0; i < N; ++i) sa += A.ad2[i];
+    for (int i = 0; i < N; ++i) sxx += Axx.ad2[i];
+    for (int i = 0; i < M; ++i) syy += Ayy.ad2[i];
+    float dl_a, dl_yy;
+    if (precision == ORC_MIRROR) {
+      // per row: 2/l^3 * (-sum_A) then + 1/l^3 * sum_Axx, in float; rows summed in FP64
+      double ta = 0, ty = 0;
+      for (int i = 0; i < N; ++i)
+        ta += (double)(2.0f / l3 * (float)(-A.ad2[i])) + (double)(1.0f / l3 * (float)Axx.ad2[i]);
+      for (int i = 0; i < M; ++i) ty += (double)(1.0f / l3 * (float)Ayy.ad2[i]);
+      dl_a = (float)ta;
+      dl_yy = (float)ty;
+    } else {
+      dl_a = (float)((-2.0 * sa + sxx) / (double)l3);
+      dl_yy = (float)(syy / (double)l3);
+    }
+    const uint32_t denom = (uint32_t)Axx.nnz + (uint32_t)Ayy.nnz - 2u * (uint32_t)A.nnz;
+    const double dl = (double)(dl_a + dl_yy) / (double)denom;
+    float e = (float)((double)ell - p->dl_step * dl);
+    if (e >= ell_max) {
+      e = (float)((double)ell_max * 0.7);
+      ell_max = (float)((double)ell_max * 0.7);
+    }
+    ell = e < p->ell_min ? p->ell_min : e;
+  }
+  if (iters) *iters = k;
+  if (ret_code) *ret_code = ret;
+  std::memset(T_out, 0, sizeof(float) * 16);
+  T_out[15] = 1.0f;
+  for (int r = 0; r < 3; ++r) {
+    for (int c = 0; c < 3; ++c) T_out[r * 4 + c] = float_pose ? Rf[c * 3 + r] : (float)Rd[c * 3 + r];
+    T_out[r * 4 + 3] = float_pose ? -(Rf[0 * 3 + r] * Tf[0] + Rf[1 * 3 + r] * Tf[1] + Rf[2 * 3 + r] * Tf[2])
+                                  : (float)(-(Rd[0 * 3 + r] * Td[0] + Rd[1 * 3 + r] * Td[1] + Rd[2 * 3 + r] * Td[2]));
   }
   return 0;
 }

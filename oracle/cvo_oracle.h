@@ -60,6 +60,21 @@ int orc_assoc_knn(const cvo_params* p, const cvo_cloud_view* src, const cvo_clou
                   const float T[16], float ell, int32_t K, int32_t precision, int32_t brute,
                   int32_t* counts, int32_t* cols, float* vals, orc_counters* cnt);
 
+/* AdaptiveCvoGPU::align (src/cvo/AdaptiveCvoGPU.cu:1077-1248): the
+ * lengthscale follows its gradient from the A, Axx, Ayy associations.
+ * precision MIRROR (float pose and row sums) or POLICY (the library's FP64
+ * pose and sums). */
+int orc_align_adaptive(const cvo_params* p, const cvo_cloud_view* src, const cvo_cloud_view* tgt,
+                       const float T_init[16], int32_t precision, float T_out[16], int32_t* iters,
+                       int32_t* ret_code, cvo_iter_record* trace, int32_t trace_cap);
+/* Its association (AdaptiveCvoGPU.cu:263-408) at pose T: counts[N], cols /
+ * vals [N*K] in ascending target index, ad2[N] = per-row sum of a |y - x|^2. */
+int orc_assoc_adaptive(const cvo_params* p, const cvo_cloud_view* src, const cvo_cloud_view* tgt,
+                       const float T[16], float ell, int32_t K, int32_t precision, int32_t* counts,
+                       int32_t* cols, float* vals, double* ad2);
+/* Its step rule (AdaptiveCvoGPU.cu:1039-1074) on B, C, D, E. */
+float orc_select_step_adaptive(const cvo_params* p, const double bcde[4]);
+
 /* For the near-cutoff exemption of the parity tests: the reference's product
  * kernel value a (before the sp_thres test) of one pair, or -1 when a gate
  * rejects it, plus the relative margin of the closest gate. */

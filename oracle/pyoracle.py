@@ -43,6 +43,10 @@ def lib():
         _lib.orc_time_iterations.restype = d
         _lib.orc_assoc.argtypes = [P, P, P, P, f, i32, i32, i32, P, P, P, P]
         _lib.orc_assoc_knn.argtypes = [P, P, P, P, f, i32, i32, i32, P, P, P, P]
+        _lib.orc_align_adaptive.argtypes = [P, P, P, P, i32, P, P, P, P, i32]
+        _lib.orc_assoc_adaptive.argtypes = [P, P, P, P, f, i32, i32, P, P, P, P]
+        _lib.orc_select_step_adaptive.argtypes = [P, P]
+        _lib.orc_select_step_adaptive.restype = f
         _lib.orc_pair_value.argtypes = [P, P, P, P, f, i32, i32, P, P]
         _lib.orc_flow.argtypes = [P, i32, P, P, i32, P, P, P, i32, P, P]
         _lib.orc_step_coeffs.argtypes = [P, f, i32, P, P, i32, P, P, P, P, i32, P]
@@ -87,6 +91,40 @@ def assoc_knn(params: CvoParams, src, tgt, T, ell, K, precision=MIRROR, brute=Fa
                              _p(counts), _p(cols), _p(vals), C.byref(cnt))
     assert rc == 0
     return counts, cols.reshape(N, K), vals.reshape(N, K), cnt
+
+
+def align_adaptive(params, src, tgt, T_init, precision=MIRROR, trace_cap=0):
+    """AdaptiveCvoGPU::align (AdaptiveCvoGPU.cu:1077-1248): (T, iters, ret, trace)."""
+    hs, ht = HeldView(src), HeldView(tgt)
+    T16 = mat16(T_init)
+    Tout = np.zeros(16, np.float32)
+    iters = C.c_int32()
+    ret = C.c_int32()
+    tr = (IterRecord * max(1, trace_cap))()
+    rc = lib().orc_align_adaptive(C.byref(params), hs.ref, ht.ref, _p(T16), precision, _p(Tout), C.byref(iters),
+                                  C.byref(ret), tr if trace_cap else None, trace_cap)
+    assert rc == 0, rc
+    trace = [tr[i] for i in range(min(trace_cap, iters.value + 1))] if trace_cap else []
+    return Tout.reshape(4, 4), iters.value, ret.value, trace
+
+
+def assoc_adaptive(params, src, tgt, T, ell, K, precision=MIRROR):
+    """The adaptive loop's association: (counts, cols, vals, per-row sum a|y-x|^2)."""
+    hs, ht = HeldView(src), HeldView(tgt)
+    N = hs.view.n
+    counts = np.zeros(N, np.int32)
+    cols = np.full(N * K, -1, np.int32)
+    vals = np.zeros(N * K, np.float32)
+    ad2 = np.zeros(N, np.float64)
+    rc = lib().orc_assoc_adaptive(C.byref(params), hs.ref, ht.ref, _p(mat16(T)), C.c_float(ell), K, precision,
+                                  _p(counts), _p(cols), _p(vals), _p(ad2))
+    assert rc == 0, rc
+    return counts, cols.reshape(N, K), vals.reshape(N, K), ad2
+
+
+def select_step_adaptive(params, bcde):
+    b = np.ascontiguousarray(bcde, np.float64)
+    return float(lib().orc_select_step_adaptive(C.byref(params), _p(b)))
 
 
 def pair_value(params, src, tgt, T, ell, i, j):

@@ -38,6 +38,7 @@ void launch_export(const AssocProblem* P_dev, int n_src, int64_t* counts_orig, c
                    float* val, int phase, int sort_threshold, hipStream_t s);
 void launch_all_done(const AssocProblem* probs, int n_probs, int32_t* out, hipStream_t s);
 void launch_rescore(const AssocProblem* probs, int n_probs, const KConsts& kc, float ell, int K, hipStream_t s);
+void launch_adaptive_sync(const AssocProblem* probs, const KConsts& kc, hipStream_t s);
 
 cvo_status set_err(cvo_ctx* ctx, cvo_status s, const std::string& msg) {
   if (ctx) ctx->last_error = msg;
@@ -436,6 +437,12 @@ static cvo_status run_loop(cvo_ctx* ctx, const Problems& pr, const KConsts& kc, 
     for (int q = 0; q < pr.n_probs; ++q) *all = *all && fl[q];
     return CVO_OK;
   };
+  // the step pass and controller; an adaptive-lengthscale run steps its
+  // registration problem only, then its self-association problems follow it
+  auto step_phase = [&]() {
+    launch_step(pr.dev, kc.adaptive ? 1 : pr.n_probs, pr.max_rows, kc, st);
+    if (kc.adaptive) launch_adaptive_sync(pr.dev, kc, st);
+  };
   CVO_TRY(ensure_events(ctx, 8));
   hipEvent_t t0 = ctx->ev_pool[0], t1 = ctx->ev_pool[1];
   hipEvent_t done_ev[2] = {ctx->ev_pool[2], ctx->ev_pool[3]};
@@ -452,9 +459,7 @@ static cvo_status run_loop(cvo_ctx* ctx, const Problems& pr, const KConsts& kc, 
       CVO_HIP(ctx, hipEventRecord(e[1], st));
       launch_overflow(pr.dev, pr.n_probs, kc, st);
       CVO_HIP(ctx, hipEventRecord(e[2], st));
-      if (with_step) {
-        launch_step(pr.dev, pr.n_probs, pr.max_rows, kc, st);
-      }
+      if (with_step) step_phase();
       CVO_HIP(ctx, hipEventRecord(e[3], st));
       bool all = true;
       CVO_TRY(read_all_done(&all));
@@ -481,7 +486,7 @@ static cvo_status run_loop(cvo_ctx* ctx, const Problems& pr, const KConsts& kc, 
       for (int it = 0; it < kChunk; ++it) {
         launch_assoc(pr.dev, pr.n_probs, pr.total_blocks, kc, st);
         launch_overflow(pr.dev, pr.n_probs, kc, st);
-        launch_step(pr.dev, pr.n_probs, pr.max_rows, kc, st);
+        step_phase();
       }
     };
     const char* ng = std::getenv("CVO_NO_GRAPH");
@@ -677,6 +682,7 @@ static void for_each_buf(cvo_ctx* ctx, void (*fn)(cvo_ctx*, DevBuf&)) {
   for (auto& b : ctx->b_ba) fn(ctx, b);
   for (auto& b : ctx->b_frames) fn(ctx, b);
   for (auto& b : ctx->b_mlp) fn(ctx, b);
+  for (auto& b : ctx->b_acvo) fn(ctx, b);
 }
 
 extern "C" void cvo_destroy(cvo_ctx* ctx) {
@@ -1115,6 +1121,179 @@ extern "C" cvo_status cvo_align_multistart(cvo_ctx* ctx, const cvo_dcloud* src, 
   if (per_start_iters)
     for (int s = 0; s < n_starts; ++s) per_start_iters[s] = its[s];
   return CVO_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Adaptive-lengthscale registration (AdaptiveCvoGPU::align)
+// ---------------------------------------------------------------------------
+
+// Kernel constants of AdaptiveCvoGPU's association (AdaptiveCvoGPU.cu:263-408):
+// the fixed-lengthscale distance gate, the colour gate always on (its
+// threshold as written there), the class factor without a gate, no geometric
+// type; row cap min(KDTREE_K_SIZE, nearest_neighbors_max).
+static KConsts adaptive_kconsts(const cvo_params& p, int F, int Ks) {
+  KConsts k = make_kconsts(p, F, Ks, MODE_ALIGN);
+  k.knn = 0;
+  k.adaptive = 1;
+  k.dl_step = p.dl_step;
+  k.ell_max0 = p.ell_max;
+  k.use_geo = 1;
+  k.use_col = 1;
+  k.use_gtype = 0;
+  k.d2_c_thres = (float)(-2.0 * (double)p.c_ell * (double)p.c_ell *
+                         (double)logf(p.sp_thres / p.c_sigma / p.c_sigma));
+  k.d2_s_thres = INFINITY;
+  k.lg = logf(p.sp_thres / k.sigma2);
+  k.rq_coef = sqrtf(-2.0f * k.lg) * 1.001f;
+  k.amp = k.sigma2 * k.c_sigma2 * (k.use_sem ? k.s_sigma2 : 1.0f);
+  k.range_ell_in_step = 0;
+  return k;
+}
+
+extern "C" cvo_status cvo_align_adaptive_d(cvo_ctx* ctx, const cvo_dcloud* src, const cvo_dcloud* tgt,
+                                           const float T_init[16], float T_out[16], double* seconds, int32_t* iters,
+                                           int32_t* ret_code) {
+  if (!ctx || !T_init || !T_out) return CVO_ERR_INVALID_ARG;
+  if (ret_code) *ret_code = 0;
+  if (iters) *iters = 0;
+  if (seconds) *seconds = 0.0;
+  for (int i = 0; i < 16; ++i) T_out[i] = (i % 5 == 0) ? 1.0f : 0.0f;
+  if (!src || !tgt || src->n == 0 || tgt->n == 0) return set_err(ctx, CVO_ERR_EMPTY, "empty point cloud");
+  CVO_TRY(validate_pair(ctx, src, tgt));
+  if (!src->feat || !tgt->feat || src->F != tgt->F || src->F <= 0)
+    return set_err(ctx, CVO_ERR_INVALID_ARG, "the adaptive-lengthscale association needs features on both clouds");
+  const cvo_params& p = ctx->params;
+  if (!(p.ell_init > 0.0f) || !(p.ell_min > 0.0f) || !(p.ell_max > 0.0f))
+    return set_err(ctx, CVO_ERR_INVALID_ARG, "lengthscales must be positive");
+  if (p.is_using_semantics && (!src->sem || !tgt->sem || src->K != tgt->K))
+    return set_err(ctx, CVO_ERR_INVALID_ARG, "is_using_semantics needs class vectors on both clouds");
+  KConsts kc = adaptive_kconsts(p, src->F, p.is_using_semantics ? src->K : 0);
+  if (!p.is_using_semantics) kc.use_sem = 0;
+  kc.amp = kc.sigma2 * kc.c_sigma2 * (kc.use_sem ? kc.s_sigma2 : 1.0f);
+  apply_tuning(ctx, kc);
+  kc.mirror_pose = ctx->pose_arith == CVO_POSE_REFERENCE ? 1 : 0;
+  kc.n_src_f = (float)src->n;
+  kc.n_tgt_f = (float)tgt->n;
+  kc.max_src_norm = std::max(src->max_norm, tgt->max_norm);
+  const int K = std::min(kKnnMaxK, p.nearest_neighbors_max);
+  hipStream_t st = ctx->stream;
+  // sorted rows: source into the context's buffers, target into its own
+  CVO_TRY(prepare_source(ctx, src));
+  DevBuf* B = ctx->b_acvo;
+  CVO_TRY(prepare_source_into(ctx, tgt, B[0], B[1], B[2], B[3]));
+  const float tq = std::max(std::fabs(T_init[3]), std::max(std::fabs(T_init[7]), std::fabs(T_init[11])));
+  const float ell_hi = std::max(std::max(p.ell_init, p.ell_max), p.ell_min);
+  const float ell_lo = std::min(p.ell_min, p.ell_init);
+  GridHost gy, gx;
+  CVO_TRY(build_grid(ctx, tgt, level_sizes(kc, ell_hi, ell_lo, kc.max_src_norm, tq, kc.min_scale), gy));
+  CVO_TRY(build_grid_into(ctx, src, level_sizes(kc, ell_hi, ell_lo, kc.max_src_norm, 0.0f, kc.min_scale), gx, B[4],
+                          B[5]));
+  // three problems: x against T^-1 y (the registration), x against x, and
+  // T^-1 y against itself
+  const int rb = rows_per_block(kc.max_group);
+  const int rows[3] = {src->n, src->n, tgt->n};
+  int nb[3], blk0[3], total = 0;
+  size_t row0[3], nrows = 0;
+  for (int q = 0; q < 3; ++q) {
+    nb[q] = (rows[q] + rb - 1) / rb;
+    blk0[q] = total;
+    total += nb[q];
+    row0[q] = nrows;
+    nrows += (size_t)rows[q];
+  }
+  const bool want_trace = ctx->trace_cap > 0 || ctx->profiling;
+  const int trace_cap = want_trace ? std::max(ctx->trace_cap, ctx->profiling ? p.MAX_ITER : 0) : 0;
+  CVO_TRY(ensure(ctx, ctx->b_pairs, sizeof(uint2) * nrows * K));
+  CVO_TRY(ensure(ctx, ctx->b_counts, sizeof(uint32_t) * nrows));
+  CVO_TRY(ensure(ctx, ctx->b_ovf, sizeof(uint32_t) * nrows));
+  CVO_TRY(ensure(ctx, ctx->b_state, sizeof(ProblemState) * 3));
+  CVO_TRY(ensure(ctx, ctx->b_probs, sizeof(AssocProblem) * 3));
+  CVO_TRY(ensure(ctx, ctx->b_part_a, sizeof(double) * 8 * (size_t)total));
+  CVO_TRY(ensure(ctx, ctx->b_part_ovf, sizeof(double) * 10 * (size_t)kOvfBlocks * 3));
+  CVO_TRY(ensure(ctx, ctx->b_part_s, sizeof(double) * 4 * (size_t)total));
+  CVO_TRY(ensure(ctx, ctx->b_part_cnt, sizeof(uint32_t) * 2 * (size_t)total));
+  if (trace_cap > 0) CVO_TRY(ensure(ctx, ctx->b_trace, sizeof(cvo_iter_record) * (size_t)trace_cap));
+  CVO_HIP(ctx, hipMemsetAsync(ctx->b_ovf.p, 0, sizeof(uint32_t) * nrows, st));
+  CVO_HIP(ctx, hipMemsetAsync(ctx->b_state.p, 0, sizeof(ProblemState) * 3, st));
+  Problems pr;
+  pr.n_probs = 3;
+  pr.total_blocks = total;
+  pr.max_rows = std::max(src->n, tgt->n);
+  pr.states = as<ProblemState>(ctx->b_state);
+  pr.trace = trace_cap > 0 ? as<cvo_iter_record>(ctx->b_trace) : nullptr;
+  pr.trace_cap = trace_cap;
+  pr.host.resize(3);
+  const cvo_dcloud* tgts[3] = {tgt, src, tgt};
+  const GridHost* grids[3] = {&gy, &gx, &gy};
+  for (int q = 0; q < 3; ++q) {
+    AssocProblem& P = pr.host[q];
+    std::memset(&P, 0, sizeof(P));
+    const bool from_tgt = q == 2;
+    const cvo_dcloud* s = from_tgt ? tgt : src;
+    P.src_pos = from_tgt ? as<float4>(B[0]) : as<float4>(ctx->b_src_pos);
+    P.src_feat = from_tgt ? as<float>(B[1]) : as<float>(ctx->b_src_feat);
+    P.src_sem = s->sem ? (from_tgt ? as<float>(B[2]) : as<float>(ctx->b_src_sem)) : nullptr;
+    P.src_geo = s->geo ? (from_tgt ? as<float>(B[3]) : as<float>(ctx->b_src_geo)) : nullptr;
+    P.n_src = rows[q];
+    P.n_tgt = tgts[q]->n;
+    P.grid = grids[q]->dev_ptr;
+    P.tgt_xyz = tgts[q]->xyz;
+    P.tgt_feat = tgts[q]->feat;
+    P.tgt_sem = tgts[q]->sem;
+    P.tgt_geo = tgts[q]->geo;
+    P.st = pr.states + q;
+    P.pairs = as<uint2>(ctx->b_pairs) + row0[q] * K;
+    P.counts = as<uint32_t>(ctx->b_counts) + row0[q];
+    P.ovf_stamp = as<uint32_t>(ctx->b_ovf) + row0[q];
+    P.cap_alloc = K;
+    P.block_begin = blk0[q];
+    P.n_blocks = nb[q];
+    P.trace_cap = q == 0 ? trace_cap : 0;
+    P.part_a = as<double>(ctx->b_part_a) + (size_t)blk0[q] * 8;
+    P.part_ovf = as<double>(ctx->b_part_ovf) + (size_t)q * kOvfBlocks * 10;
+    P.part_s = as<double>(ctx->b_part_s) + (size_t)blk0[q] * 4;
+    P.part_cnt = as<uint32_t>(ctx->b_part_cnt) + (size_t)blk0[q] * 2;
+    P.trace = q == 0 ? pr.trace : nullptr;
+    P.xx_st = q == 0 ? pr.states + 1 : nullptr;
+    P.yy_st = q == 0 ? pr.states + 2 : nullptr;
+  }
+  pr.dev = as<AssocProblem>(ctx->b_probs);
+  CVO_HIP(ctx, hipMemcpyAsync(pr.dev, pr.host.data(), sizeof(AssocProblem) * 3, hipMemcpyHostToDevice, st));
+  CVO_TRY(ensure(ctx, ctx->b_tmp2, 64 * sizeof(float)));
+  CVO_HIP(ctx, hipMemcpyAsync(ctx->b_tmp2.p, T_init, sizeof(float) * 16, hipMemcpyHostToDevice, st));
+  launch_init_state(pr.dev, 1, as<float>(ctx->b_tmp2), 16, kc, p.ell_init, K, st);
+  launch_adaptive_sync(pr.dev, kc, st);
+  CVO_HIP(ctx, hipGetLastError());
+  double dev_s = 0.0;
+  if (p.MAX_ITER > 0) CVO_TRY(run_loop(ctx, pr, kc, p.MAX_ITER, true, &dev_s));
+  ProblemState hs;
+  CVO_HIP(ctx, hipMemcpyAsync(&hs, pr.states, sizeof(ProblemState), hipMemcpyDeviceToHost, st));
+  CVO_HIP(ctx, hipStreamSynchronize(st));
+  pose_to_output(hs, T_out);
+  const int n_it = p.MAX_ITER <= 0 ? 0 : (hs.done ? hs.iters_out : p.MAX_ITER);
+  if (iters) *iters = n_it;
+  if (ret_code) *ret_code = hs.ret;
+  if (seconds) *seconds = dev_s;
+  if (want_trace) {
+    const int n_rec = std::min(trace_cap, n_it + 1);
+    ctx->trace.resize(n_rec);
+    if (n_rec > 0)
+      CVO_HIP(ctx, hipMemcpy(ctx->trace.data(), pr.trace, sizeof(cvo_iter_record) * n_rec, hipMemcpyDeviceToHost));
+    ctx->trace_n = n_rec;
+  }
+  return CVO_OK;
+}
+
+extern "C" cvo_status cvo_align_adaptive(cvo_ctx* ctx, const cvo_cloud_view* src, const cvo_cloud_view* tgt,
+                                         const float T_init[16], float T_out[16], double* seconds, int32_t* iters,
+                                         int32_t* ret_code) {
+  if (!ctx || !src || !tgt || !T_init || !T_out) return CVO_ERR_INVALID_ARG;
+  if (src->n == 0 || tgt->n == 0) return cvo_align_adaptive_d(ctx, nullptr, nullptr, T_init, T_out, seconds, iters,
+                                                              ret_code);
+  Uploaded a{ctx}, b{ctx};
+  CVO_TRY(cvo_cloud_upload(ctx, src, &a.c));
+  CVO_TRY(cvo_cloud_upload(ctx, tgt, &b.c));
+  return cvo_align_adaptive_d(ctx, a.c, b.c, T_init, T_out, seconds, iters, ret_code);
 }
 
 extern "C" cvo_status cvo_set_pose_arithmetic(cvo_ctx* ctx, int32_t mode) {

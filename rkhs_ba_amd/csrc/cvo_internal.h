@@ -76,6 +76,10 @@ struct alignas(16) ProblemState {
   float flow_norm;
   int32_t level_assoc;  // grid level of the last association pass
   int32_t group;        // lanes per source row of the next association pass
+  // adaptive lengthscale (AdaptiveCvoGPU): its current bound and the last
+  // gradient step dl
+  float ell_max;
+  double dl;
 };
 
 // Read-only description of one association problem.
@@ -105,6 +109,10 @@ struct AssocProblem {
   double* part_s;          // per block: 4 doubles
   uint32_t* part_cnt;      // per block: nnz and max row
   cvo_iter_record* trace;  // per iteration (may be null)
+  // adaptive lengthscale: the source and target self-association problems
+  // whose totals the controller reads (null otherwise)
+  ProblemState* xx_st;
+  ProblemState* yy_st;
 };
 
 // Association launch shape as a function of the lengthscale: lanes per row
@@ -148,6 +156,11 @@ struct KConsts {
   int32_t xcd_remap;          // 1: contiguous row-block ranges per XCD
   float min_scale;            // finest cell scale of the policy (sizes the levels)
   int32_t knn;                // is_using_kdtree: K-nearest association, the K multiplier (0 off)
+  // adaptive lengthscale (AdaptiveCvoGPU): fixed-lengthscale ball, the ip
+  // column carries sum(a d2), unnormalised flow, the dl-gradient controller
+  int32_t adaptive;
+  double dl_step;
+  float ell_max0;             // initial lengthscale bound (ell_max)
 };
 
 // Rows per association block for a lane count per row (8 or 16).
@@ -211,6 +224,8 @@ struct cvo_ctx {
   cvo_tuning tuning{16, 16, 0.0f, 1, 1.0f, 1.0f, CVO_TUNE_DEFAULT, 0.0f, 0};
   // model scratch
   DevBuf b_mlp[16];
+  // adaptive-lengthscale scratch: sorted target rows (4), source grid (2)
+  DevBuf b_acvo[8];
 };
 
 // Internal helpers shared by the translation units.

@@ -87,6 +87,12 @@ __device__ __forceinline__ float geo_similarity(const float* ga, const float* gb
   return dab * dab / (n2a * n2b);
 }
 
+// |y - x|^2 with the association's FMA order.
+__device__ __forceinline__ float sq_dist(float x0, float x1, float x2, float y0, float y1, float y2) {
+  const float dx = y0 - x0, dy = y1 - x1, dz = y2 - x2;
+  return fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+}
+
 // floor(v) clamped to [-1, n] in float before the conversion, so extreme or
 // non-finite query coordinates cannot overflow the int cell index (NaN -> -1).
 __device__ __forceinline__ int cell_floor(float v, int n) {
@@ -306,7 +312,8 @@ __device__ __forceinline__ void assoc_row(const AssocProblem& P, const KConsts& 
     affine3(mp.As, mp.bs, x0, x1, x2, t0, t1, t2);
     x0 = t0; x1 = t1; x2 = t2;
   }
-  const float l = range_ell(ell, x0, x1, x2);
+  // the adaptive-lengthscale association uses ell as is (AdaptiveCvoGPU.cu:281)
+  const float l = kc.adaptive ? ell : range_ell(ell, x0, x1, x2);
   const float d2t = (float)(-2.0 * (double)l * (double)l * (double)kc.lg);
   const float qg = (float)(1.0 / (2.0 * (double)l * (double)l));
   float xg[2] = {0.0f, 0.0f};
@@ -400,7 +407,7 @@ __device__ __forceinline__ void assoc_row(const AssocProblem& P, const KConsts& 
           out.acc[4] += (double)((y1 - x1) * a);
           out.acc[5] += (double)((y2 - x2) * a);
         }
-        out.acc[6] += (double)a;
+        out.acc[6] += (double)(kc.adaptive ? sq_dist(x0, x1, x2, y0, y1, y2) * a : a);
       }
       stored += (uint32_t)__popc(gm);
       if (stored > Ku) {
@@ -899,7 +906,7 @@ __global__ __launch_bounds__(kBlock) void overflow_kernel(const AssocProblem* __
       affine3(mp.As, mp.bs, x0, x1, x2, t0, t1, t2);
       x0 = t0; x1 = t1; x2 = t2;
     }
-    const float l = range_ell(ell, x0, x1, x2);
+    const float l = kc.adaptive ? ell : range_ell(ell, x0, x1, x2);
     const float d2t = kc.use_geo ? (float)(-2.0 * (double)l * (double)l * (double)kc.lg) : 1.0f;
     const float qg = (float)(1.0 / (2.0 * (double)l * (double)l));
     const float* xf = P.src_feat ? P.src_feat + (size_t)r * kc.F : nullptr;
@@ -949,7 +956,7 @@ __global__ __launch_bounds__(kBlock) void overflow_kernel(const AssocProblem* __
           acc[4] += (double)((y1 - x1) * a);
           acc[5] += (double)((y2 - x2) * a);
         }
-        acc[6] += (double)a;
+        acc[6] += (double)(kc.adaptive ? sq_dist(x0, x1, x2, y0, y1, y2) * a : a);
       }
       stored += tot;
       __syncthreads();
@@ -1142,6 +1149,9 @@ __device__ void set_level(ProblemState* st, const KConsts& kc, float ell, const 
   st->level = choose_level(grid, needed_radius(kc, st, ell) * (big ? kc.pol.scale_big : kc.pol.scale_small));
 }
 
+__device__ double apply_step(ProblemState* st, const KConsts& kc, const double xi[6], const float xf[6],
+                             double& step);
+
 // One controller step of align_impl (CvoGPU.cu:1342-1433), FP64 pose state.
 __device__ void controller(ProblemState* st, const KConsts& kc, const AssocProblem& P, const float* level_h,
                            int n_levels) {
@@ -1176,6 +1186,48 @@ __device__ void controller(ProblemState* st, const KConsts& kc, const AssocProbl
     return;
   }
   double step = select_step_dev(kc, st->bcde);
+  const double dist = apply_step(st, kc, xi, xf, step);
+  st->step = step;
+  st->dist = dist;
+  rec.step = (float)step;
+  rec.dist = (float)dist;
+  const bool dec = indicator_update(st, rec.ip, kc.window, kc.ind_thr);
+  if (P.trace && k < P.trace_cap) P.trace[k] = rec;
+  set_maps(st);
+  if (dist < (double)kc.eps_2) {
+    st->done = 1;
+    st->iters_out = k;
+    return;
+  }
+  if (k > kc.ell_decay_start && dec) {
+    float e = st->ell * kc.ell_decay_rate;
+    if (e < kc.ell_min) e = kc.ell_min;
+    st->ell = e;
+  }
+  if (kc.knn > 0) {
+    // K nearest: the mean stored row rounded up times is_using_kdtree, at
+    // most KDTREE_K_SIZE (CvoGPU.cu:1416-1417); zero stays zero
+    const int mean = (int)ceil((double)st->nnz / (double)P.n_src);
+    st->K = min(kKnnMaxK, mean * kc.knn);
+  } else {
+    st->K = min(kc.nn_max, (int)((double)st->max_row * 1.2));
+    if (st->K < 1) st->K = 1;
+  }
+  st->iter = k + 1;
+  if (st->iter >= kc.max_iter) {
+    st->done = 1;
+    st->iters_out = kc.max_iter;
+  }
+  set_level(st, kc, st->ell, level_h, n_levels);
+  st->n_overflow = 0;
+}
+
+// Pose update by exp(step xi) (CvoGPU.cu:1362-1379): FP64 by default, or the
+// reference's float arithmetic (float step, FP32 Exp_SEK3, pose products in
+// double rounded to float).  Returns the step's SE(3) distance; step is
+// rounded to float in the reference mode.
+__device__ double apply_step(ProblemState* st, const KConsts& kc, const double xi[6], const float xf[6],
+                             double& step) {
   double dR[9], dT[3];
   double dist;
   if (kc.mirror_pose) {
@@ -1214,11 +1266,65 @@ __device__ void controller(ProblemState* st, const KConsts& kc, const AssocProbl
     const double xn = sqrt(xi[0] * xi[0] + xi[1] * xi[1] + xi[2] * xi[2] + xi[3] * xi[3] + xi[4] * xi[4] + xi[5] * xi[5]);
     dist = (step * wn < 3.0) ? step * xn : se3_log_norm(dR, dT);
   }
+  return dist;
+}
+
+// Step length of the adaptive-lengthscale loop (AdaptiveCvoGPU.cu:1006-1075):
+// the cubic's coefficients rounded to float, its smallest positive real root,
+// min_step when there is none, at most 0.8.
+__device__ double select_step_adaptive(const KConsts& kc, const double bcde[4]) {
+  double re[3], im[3];
+  cubic_solve((double)(4.0f * (float)bcde[3]), (double)(3.0f * (float)bcde[2]), (double)(2.0f * (float)bcde[1]),
+              (double)(float)bcde[0], re, im);
+  double temp = 1.7976931348623157e308;
+  for (int i = 0; i < 3; ++i)
+    if (re[i] > 0 && re[i] < temp && im[i] == 0.0) temp = re[i];
+  double step = temp == 1.7976931348623157e308 ? (double)kc.min_step : (double)(float)temp;
+  return step > 0.8 ? 0.8 : step;
+}
+
+// One controller step of AdaptiveCvoGPU::align (AdaptiveCvoGPU.cu:1117-1205):
+// unnormalised flow (omega / c, v / d), stop when both parts are below eps,
+// exp(step xi), stop when the step's distance is below eps_2, then the
+// lengthscale moves against its gradient
+//   dl = (2/l^3 sum_A -a |y-x|^2 + 1/l^3 sum_Axx a |x'-x|^2 + 1/l^3 sum_Ayy a |y'-y|^2)
+//        / (nnz(Axx) + nnz(Ayy) - 2 nnz(A))            (:445-890, unsigned count)
+//   l <- l - dl_step dl; past ell_max both shrink to 0.7 ell_max; at least ell_min.
+// The source and target self-association totals come from P.xx_st / P.yy_st
+// (their passes ran with this one; st->ip holds sum(a |y-x|^2)).
+__device__ void controller_adaptive(ProblemState* st, const KConsts& kc, const AssocProblem& P, const float* level_h,
+                                    int n_levels) {
+  cvo_iter_record rec;
+  const int k = st->iter;
+  float xf[6];
+  double xi[6];
+  for (int c = 0; c < 6; ++c) {
+    xf[c] = (float)st->flow[c];
+    xi[c] = (double)xf[c];
+  }
+  const double nw = sqrt(xi[0] * xi[0] + xi[1] * xi[1] + xi[2] * xi[2]);
+  const double nv = sqrt(xi[3] * xi[3] + xi[4] * xi[4] + xi[5] * xi[5]);
+  rec.ell = st->ell;
+  rec.nnz = (int32_t)st->nnz;
+  rec.max_row = (int32_t)st->max_row;
+  rec.num_neighbors = st->K;
+  rec.flow_norm = (float)sqrt(nw * nw + nv * nv);
+  rec.step = 0.0f;
+  rec.dist = 0.0f;
+  rec.ip = (float)((double)st->nnz / sqrt((double)kc.n_src_f * (double)kc.n_tgt_f));
+  if (nw < (double)kc.eps && nv < (double)kc.eps) {
+    st->ret = (nw < 1e-8 && nv < 1e-8) ? -1 : 0;
+    st->done = 1;
+    st->iters_out = k;
+    if (P.trace && k < P.trace_cap) P.trace[k] = rec;
+    return;
+  }
+  double step = select_step_adaptive(kc, st->bcde);
+  const double dist = apply_step(st, kc, xi, xf, step);
   st->step = step;
   st->dist = dist;
   rec.step = (float)step;
   rec.dist = (float)dist;
-  const bool dec = indicator_update(st, rec.ip, kc.window, kc.ind_thr);
   if (P.trace && k < P.trace_cap) P.trace[k] = rec;
   set_maps(st);
   if (dist < (double)kc.eps_2) {
@@ -1226,20 +1332,20 @@ __device__ void controller(ProblemState* st, const KConsts& kc, const AssocProbl
     st->iters_out = k;
     return;
   }
-  if (k > kc.ell_decay_start && dec) {
-    float e = st->ell * kc.ell_decay_rate;
-    if (e < kc.ell_min) e = kc.ell_min;
-    st->ell = e;
+  const ProblemState* sx = P.xx_st;
+  const ProblemState* sy = P.yy_st;
+  const float l3 = st->ell * st->ell * st->ell;
+  const float dl_a = (float)((-2.0 * st->ip + sx->ip) / (double)l3);
+  const float dl_yy = (float)(sy->ip / (double)l3);
+  const uint32_t denom = (uint32_t)sx->nnz + (uint32_t)sy->nnz - 2u * (uint32_t)st->nnz;
+  const double dl = (double)(dl_a + dl_yy) / (double)denom;
+  st->dl = dl;
+  float ell = (float)((double)st->ell - kc.dl_step * dl);
+  if (ell >= st->ell_max) {
+    ell = (float)((double)st->ell_max * 0.7);
+    st->ell_max = (float)((double)st->ell_max * 0.7);
   }
-  if (kc.knn > 0) {
-    // K nearest: the mean stored row rounded up times is_using_kdtree, at
-    // most KDTREE_K_SIZE (CvoGPU.cu:1416-1417); zero stays zero
-    const int mean = (int)ceil((double)st->nnz / (double)P.n_src);
-    st->K = min(kKnnMaxK, mean * kc.knn);
-  } else {
-    st->K = min(kc.nn_max, (int)((double)st->max_row * 1.2));
-    if (st->K < 1) st->K = 1;
-  }
+  st->ell = ell < kc.ell_min ? kc.ell_min : ell;
   st->iter = k + 1;
   if (st->iter >= kc.max_iter) {
     st->done = 1;
@@ -1247,6 +1353,36 @@ __device__ void controller(ProblemState* st, const KConsts& kc, const AssocProbl
   }
   set_level(st, kc, st->ell, level_h, n_levels);
   st->n_overflow = 0;
+}
+
+// The self-association problems follow the registration problem of an
+// adaptive-lengthscale run: lengthscale, iteration, done flag; the target
+// problem's points move with the target map.  One thread.
+__global__ void adaptive_sync_kernel(const AssocProblem* __restrict__ probs, KConsts kc) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const ProblemState* s0 = probs[0].st;
+  for (int q = 1; q < 3; ++q) {
+    const AssocProblem& P = probs[q];
+    ProblemState* st = P.st;
+    st->ell = s0->ell;
+    st->K = s0->K;
+    st->iter = s0->iter;
+    st->done = s0->done;
+    st->n_overflow = 0;
+    for (int i = 0; i < 9; ++i) {
+      const float e = (i % 4 == 0) ? 1.0f : 0.0f;
+      st->A[i] = q == 2 ? s0->A[i] : e;
+      st->As[i] = q == 2 ? s0->A[i] : e;
+      st->Rq[i] = q == 2 ? s0->Rq[i] : e;
+    }
+    for (int i = 0; i < 3; ++i) {
+      st->b[i] = q == 2 ? s0->b[i] : 0.0f;
+      st->bs[i] = q == 2 ? s0->b[i] : 0.0f;
+      st->Tq[i] = q == 2 ? s0->Tq[i] : 0.0f;
+    }
+    st->has_src_xf = q == 2 ? 1 : 0;
+    set_level(st, kc, st->ell, P.grid);
+  }
 }
 
 
@@ -1310,7 +1446,8 @@ __device__ void controller_tail(const AssocProblem& P, const KConsts& kc, const 
   if (tid == 0) {
     for (int c = 0; c < 6; ++c) s_st.flow[c] = tot[c];
     for (int c = 0; c < 4; ++c) s_st.bcde[c] = s_b[c][0];
-    controller(&s_st, kc, P, s_h, s_nl);
+    if (kc.adaptive) controller_adaptive(&s_st, kc, P, s_h, s_nl);
+    else controller(&s_st, kc, P, s_h, s_nl);
   }
   __syncthreads();
   {
@@ -1343,11 +1480,13 @@ __global__ __launch_bounds__(kBlock) void step_kernel(const AssocProblem* __rest
   const GridLevel& G = P.grid->lv[st->level];
   Maps mp;
   load_maps(st, mp);
-  // normalised flow (compute_flow, CvoGPU.cu:2099-2104), used in float
+  // normalised flow (compute_flow, CvoGPU.cu:2099-2104), used in float; the
+  // adaptive-lengthscale loop steps along the flow as summed
+  // (AdaptiveCvoGPU.cu:839-840)
   double n2 = 0.0;
 #pragma unroll
   for (int c = 0; c < 6; ++c) n2 += tot[c] * tot[c];
-  const double nrm = n2 > 0.0 ? sqrt(n2) : 1.0;
+  const double nrm = (n2 > 0.0 && !kc.adaptive) ? sqrt(n2) : 1.0;
   const float w0 = (float)(tot[0] / nrm), w1 = (float)(tot[1] / nrm), w2 = (float)(tot[2] / nrm);
   const float v0 = (float)(tot[3] / nrm), v1 = (float)(tot[4] / nrm), v2 = (float)(tot[5] / nrm);
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
@@ -1453,6 +1592,8 @@ __global__ void init_state_kernel(const AssocProblem* __restrict__ probs, int n_
   st->blocks_done_assoc = st->blocks_done_ovf = st->blocks_done_step = 0;
   st->qs_head = st->qs_size = st->qe_head = st->qe_size = 0;
   st->ss = st->se = 0.0f;
+  st->ell_max = kc.ell_max0;
+  st->dl = 0.0;
   set_maps(st);
   set_level(st, kc, ell0, P.grid);
 }
@@ -1605,6 +1746,10 @@ void launch_overflow(const AssocProblem* probs, int n_probs, const KConsts& kc, 
 void launch_step(const AssocProblem* probs, int n_probs, int max_src_rows, const KConsts& kc, hipStream_t s) {
   hipLaunchKernelGGL(step_kernel, dim3((max_src_rows + kBlock - 1) / kBlock, n_probs), dim3(kBlock), 0, s, probs,
                      n_probs, kc);
+}
+
+void launch_adaptive_sync(const AssocProblem* probs, const KConsts& kc, hipStream_t s) {
+  hipLaunchKernelGGL(adaptive_sync_kernel, dim3(1), dim3(64), 0, s, probs, kc);
 }
 
 void launch_rescore(const AssocProblem* probs, int n_probs, const KConsts& kc, float ell, int K, hipStream_t s) {

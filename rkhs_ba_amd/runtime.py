@@ -57,6 +57,8 @@ def lib() -> C.CDLL:
         L.cvo_compute_association.argtypes = [P, P, P, P, f, P]
         L.cvo_compute_association_nonisotropic.argtypes = [P, P, P, P, P, P]
         L.cvo_compute_association_knn.argtypes = [P, P, P, P, f, i32, P]
+        L.cvo_align_adaptive.argtypes = [P, P, P, P, P, P, P, P]
+        L.cvo_align_adaptive_d.argtypes = [P, P, P, P, P, P, P, P]
         L.cvo_align.argtypes = [P, P, P, P, P, P, P, P, P]
         L.cvo_align_d.argtypes = [P, P, P, P, P, P, P, P, P]
         L.cvo_align_multistart.argtypes = [P, P, P, i32, P, f, i32, P, P, P, P, P, P]
@@ -284,6 +286,25 @@ class CvoGPU:
         _check(st, self._ctx)
         assoc = _take_assoc(a) if (association and a.row_ptr) else None
         return ret.value, Tout.reshape(4, 4), secs.value, iters.value, assoc
+
+    def align_adaptive(self, source, target, init_guess_transform):
+        """AdaptiveCvoGPU::align (AdaptiveCvoGPU.cu:1077-1248): the lengthscale
+        follows its gradient from the self inner products.  Returns (ret,
+        transform 4x4, seconds, iters)."""
+        T0 = mat16(init_guess_transform)
+        Tout = np.zeros(16, np.float32)
+        secs = C.c_double()
+        iters = C.c_int32()
+        ret = C.c_int32()
+        if isinstance(source, DeviceCloud):
+            st = lib().cvo_align_adaptive_d(self._ctx, source.handle, target.handle, T0.ctypes.data,
+                                            Tout.ctypes.data, C.byref(secs), C.byref(iters), C.byref(ret))
+        else:
+            hs, ht = HeldView(source), HeldView(target)
+            st = lib().cvo_align_adaptive(self._ctx, hs.ref, ht.ref, T0.ctypes.data, Tout.ctypes.data,
+                                          C.byref(secs), C.byref(iters), C.byref(ret))
+        _check(st, self._ctx)
+        return ret.value, Tout.reshape(4, 4), secs.value, iters.value
 
     def align_result(self, source, target, init_guess_T_s2t, is_returning_association: bool = False) -> CvoResultInfo:
         """CvoResultInfo align(source, target, T_init_s2t, is_returning_association)
