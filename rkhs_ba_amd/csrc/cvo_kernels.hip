@@ -851,6 +851,50 @@ __global__ __launch_bounds__(kBlock) void knn_assoc_kernel(const AssocProblem* _
 // problem's totals (raw flow sums, ip, stored pairs, longest row).
 // ---------------------------------------------------------------------------
 
+// The pass totals of a problem by one block: thread t takes association
+// blocks t, t + 256, ... (four at a time, for loads in flight), then a fixed
+// tree: wave shuffles, then the four waves in order.
+__device__ __noinline__ void pass_totals_one_block(const AssocProblem& P, ProblemState* st) {
+  const int tid = threadIdx.x;
+  const int n_act = active_blocks(P, state_group(st));
+  double acc[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  double nz = 0.0, mx = 0.0;
+#pragma unroll 4
+  for (int b = tid; b < n_act; b += kBlock) {
+    const double2* pa = reinterpret_cast<const double2*>(P.part_a + (size_t)b * 8);
+    const double2 q0 = pa[0], q1 = pa[1], q2 = pa[2], q3 = pa[3];
+    const uint2 cn = reinterpret_cast<const uint2*>(P.part_cnt)[b];
+    acc[0] += q0.x; acc[1] += q0.y; acc[2] += q1.x; acc[3] += q1.y;
+    acc[4] += q2.x; acc[5] += q2.y; acc[6] += q3.x;
+    nz += (double)cn.x;
+    mx = fmax(mx, (double)cn.y);
+  }
+  __shared__ double s_w[9][kBlock / 64];
+  const int lane = tid & 63, wave = tid >> 6;
+#pragma unroll
+  for (int c = 0; c < 9; ++c) {
+    double v = c < 7 ? acc[c] : (c == 7 ? nz : mx);
+    for (int o = 32; o > 0; o >>= 1) {
+      const double u = __shfl_xor(v, o, 64);
+      v = c == 8 ? fmax(v, u) : v + u;
+    }
+    if (lane == 0) s_w[c][wave] = v;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double t[9];
+    for (int c = 0; c < 9; ++c) {
+      t[c] = 0.0;
+      for (int w = 0; w < kBlock / 64; ++w) t[c] = c == 8 ? fmax(t[c], s_w[c][w]) : t[c] + s_w[c][w];
+    }
+    for (int c = 0; c < 6; ++c) st->flow[c] = t[c];
+    st->ip = t[6];
+    st->nnz = (uint64_t)t[7];
+    st->max_row = (uint32_t)t[8];
+    st->level_assoc = st->level;
+  }
+}
+
 template <int FC, int KC>
 __global__ __launch_bounds__(kBlock) void overflow_kernel(const AssocProblem* __restrict__ probs, int n_probs,
                                                           KConsts kc) {
@@ -859,6 +903,12 @@ __global__ __launch_bounds__(kBlock) void overflow_kernel(const AssocProblem* __
   if (st->done) return;
   const int tid = threadIdx.x;
   const int mode = kc.mode;
+  if (st->n_overflow == 0) {
+    // no capped row: block 0 alone sums every association partial (no
+    // cross-block publication), the other blocks have nothing to do
+    if (blockIdx.x == 0) pass_totals_one_block(P, st);
+    return;  // uniform per block
+  }
   // (a) this block's slice of the association block partials, in fixed order
   __shared__ double s_tot[10][128];
   {
