@@ -23,9 +23,8 @@
 
 namespace cvo {
 
-void launch_assoc(const AssocProblem* probs, int n_probs, int total_blocks, const KConsts& kc, hipStream_t s);
-void launch_overflow(const AssocProblem* probs, int n_probs, const KConsts& kc, hipStream_t s);
-void launch_step(const AssocProblem* probs, int n_probs, int max_src_rows, const KConsts& kc, hipStream_t s);
+void launch_step(const AssocProblem* probs, int n_probs, int max_src_rows, const KConsts& kc, hipStream_t s,
+                 const AssocProblem* host_one);
 void launch_init_state(const AssocProblem* probs, int n_probs, const float* T16, int stride16, const KConsts& kc,
                        float ell0, int K0, hipStream_t s);
 void launch_grid_count(const float* xyz, int n, const GridLevel& L, uint32_t* key, uint32_t* count, hipStream_t s);
@@ -440,7 +439,7 @@ static cvo_status run_loop(cvo_ctx* ctx, const Problems& pr, const KConsts& kc, 
   // the step pass and controller; an adaptive-lengthscale run steps its
   // registration problem only, then its self-association problems follow it
   auto step_phase = [&]() {
-    launch_step(pr.dev, kc.adaptive ? 1 : pr.n_probs, pr.max_rows, kc, st);
+    launch_step(pr.dev, kc.adaptive ? 1 : pr.n_probs, pr.max_rows, kc, st, pr.host.data());
     if (kc.adaptive) launch_adaptive_sync(pr.dev, kc, st);
   };
   CVO_TRY(ensure_events(ctx, 8));
@@ -455,9 +454,9 @@ static cvo_status run_loop(cvo_ctx* ctx, const Problems& pr, const KConsts& kc, 
     for (int i = 0; i < 6; ++i) e[i] = ctx->ev_pool[8 + i];
     for (int it = 0; it < max_iter; ++it) {
       CVO_HIP(ctx, hipEventRecord(e[0], st));
-      launch_assoc(pr.dev, pr.n_probs, pr.total_blocks, kc, st);
+      launch_assoc(pr.dev, pr.n_probs, pr.total_blocks, kc, st, pr.host.data());
       CVO_HIP(ctx, hipEventRecord(e[1], st));
-      launch_overflow(pr.dev, pr.n_probs, kc, st);
+      launch_overflow(pr.dev, pr.n_probs, kc, st, pr.host.data());
       CVO_HIP(ctx, hipEventRecord(e[2], st));
       if (with_step) step_phase();
       CVO_HIP(ctx, hipEventRecord(e[3], st));
@@ -476,16 +475,16 @@ static cvo_status run_loop(cvo_ctx* ctx, const Problems& pr, const KConsts& kc, 
       if (!with_step || all) break;
     }
   } else if (!with_step) {
-    launch_assoc(pr.dev, pr.n_probs, pr.total_blocks, kc, st);
-    launch_overflow(pr.dev, pr.n_probs, kc, st);
+    launch_assoc(pr.dev, pr.n_probs, pr.total_blocks, kc, st, pr.host.data());
+    launch_overflow(pr.dev, pr.n_probs, kc, st, pr.host.data());
     CVO_HIP(ctx, hipGetLastError());
   } else {
     // one chunk of iterations, captured in a graph unless CVO_NO_GRAPH is set
     // (direct launches, e.g. under a kernel tracer)
     auto launch_chunk = [&]() {
       for (int it = 0; it < kChunk; ++it) {
-        launch_assoc(pr.dev, pr.n_probs, pr.total_blocks, kc, st);
-        launch_overflow(pr.dev, pr.n_probs, kc, st);
+        launch_assoc(pr.dev, pr.n_probs, pr.total_blocks, kc, st, pr.host.data());
+        launch_overflow(pr.dev, pr.n_probs, kc, st, pr.host.data());
         step_phase();
       }
     };

@@ -704,8 +704,6 @@ static cvo_status validate_frames(cvo_ctx* ctx, int n_frames, const cvo_dcloud* 
   return CVO_OK;
 }
 
-void launch_assoc(const AssocProblem* probs, int n_probs, int total_blocks, const KConsts& kc, hipStream_t s);
-void launch_overflow(const AssocProblem* probs, int n_probs, const KConsts& kc, hipStream_t s);
 
 struct BaLaunch {
   BaDev* ba;

@@ -800,11 +800,8 @@ __device__ __forceinline__ int active_blocks(const AssocProblem& P, int group) {
 // count.  With xcd_remap the active blocks are renumbered so that each
 // XCD holds one contiguous range of rows.
 // Four waves per SIMD: the register budget is 128 VGPRs.
-template <int FC, int KC>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void assoc_kernel(
-    const AssocProblem* __restrict__ probs, int n_probs, KConsts kc) {
-  const int p = find_problem(probs, n_probs, (int)blockIdx.x);
-  const AssocProblem& P = probs[p];
+template <int FC, int KC, bool KNN>
+__device__ __forceinline__ void assoc_pass(const AssocProblem& P, const KConsts& kc) {
   const ProblemState* st = P.st;
   if (st->done) return;
   const int group = state_group(st);
@@ -813,32 +810,42 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
   if (lb >= n_act) return;  // uniform
   if (kc.xcd_remap) lb = xcd_block(lb, n_act);
   __shared__ AssocLds<FC, KC> L;
-  switch (group) {
-    // two lane counts keep the kernel within 128 VGPRs (four waves per SIMD)
-    case 8: assoc_block<FC, KC, 8, false>(P, kc, lb, L, nullptr); break;
-    default: assoc_block<FC, KC, 16, false>(P, kc, lb, L, nullptr); break;
+  if constexpr (KNN) {
+    __shared__ uint32_t hist[kMaxRowsPerBlock * 256];
+    switch (group) {
+      case 8: assoc_block<FC, KC, 8, true>(P, kc, lb, L, hist); break;
+      default: assoc_block<FC, KC, 16, true>(P, kc, lb, L, hist); break;
+    }
+  } else {
+    switch (group) {
+      // two lane counts keep the kernel within 128 VGPRs (four waves per SIMD)
+      case 8: assoc_block<FC, KC, 8, false>(P, kc, lb, L, nullptr); break;
+      default: assoc_block<FC, KC, 16, false>(P, kc, lb, L, nullptr); break;
+    }
+  }
+}
+
+// ONE: a single problem, its descriptor by value in the kernel arguments (read
+// with scalar loads, no dependent load before the first state read).
+template <int FC, int KC, bool ONE>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void assoc_kernel(
+    const AssocProblem* __restrict__ probs, int n_probs, KConsts kc, AssocProblem one) {
+  if constexpr (ONE) {
+    assoc_pass<FC, KC, false>(one, kc);
+  } else {
+    assoc_pass<FC, KC, false>(probs[find_problem(probs, n_probs, (int)blockIdx.x)], kc);
   }
 }
 
 // The K-nearest association pass (is_using_kdtree), same launch shape and
 // outputs; rows never exceed their cap, so no capped-row work follows.
-template <int FC, int KC>
+template <int FC, int KC, bool ONE>
 __global__ __launch_bounds__(kBlock) void knn_assoc_kernel(const AssocProblem* __restrict__ probs, int n_probs,
-                                                           KConsts kc) {
-  const int p = find_problem(probs, n_probs, (int)blockIdx.x);
-  const AssocProblem& P = probs[p];
-  const ProblemState* st = P.st;
-  if (st->done) return;
-  const int group = state_group(st);
-  const int n_act = active_blocks(P, group);
-  int lb = (int)blockIdx.x - P.block_begin;
-  if (lb >= n_act) return;  // uniform
-  if (kc.xcd_remap) lb = xcd_block(lb, n_act);
-  __shared__ AssocLds<FC, KC> L;
-  __shared__ uint32_t hist[kMaxRowsPerBlock * 256];
-  switch (group) {
-    case 8: assoc_block<FC, KC, 8, true>(P, kc, lb, L, hist); break;
-    default: assoc_block<FC, KC, 16, true>(P, kc, lb, L, hist); break;
+                                                           KConsts kc, AssocProblem one) {
+  if constexpr (ONE) {
+    assoc_pass<FC, KC, true>(one, kc);
+  } else {
+    assoc_pass<FC, KC, true>(probs[find_problem(probs, n_probs, (int)blockIdx.x)], kc);
   }
 }
 
@@ -854,7 +861,7 @@ __global__ __launch_bounds__(kBlock) void knn_assoc_kernel(const AssocProblem* _
 // The pass totals of a problem by one block: thread t takes association
 // blocks t, t + 256, ... (four at a time, for loads in flight), then a fixed
 // tree: wave shuffles, then the four waves in order.
-__device__ __noinline__ void pass_totals_one_block(const AssocProblem& P, ProblemState* st) {
+__device__ __forceinline__ void pass_totals_one_block(const AssocProblem& P, ProblemState* st) {
   const int tid = threadIdx.x;
   const int n_act = active_blocks(P, state_group(st));
   double acc[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
@@ -896,9 +903,7 @@ __device__ __noinline__ void pass_totals_one_block(const AssocProblem& P, Proble
 }
 
 template <int FC, int KC>
-__global__ __launch_bounds__(kBlock) void overflow_kernel(const AssocProblem* __restrict__ probs, int n_probs,
-                                                          KConsts kc) {
-  const AssocProblem& P = probs[blockIdx.y];
+__device__ __forceinline__ void overflow_body(const AssocProblem& P, const KConsts& kc) {
   ProblemState* st = P.st;
   if (st->done) return;
   const int tid = threadIdx.x;
@@ -1058,6 +1063,15 @@ __global__ __launch_bounds__(kBlock) void overflow_kernel(const AssocProblem* __
   }
 }
 
+// A single problem's descriptor comes by value in the kernel arguments (no
+// dependent load before the first state read); several come from memory.
+template <int FC, int KC, bool ONE>
+__global__ __launch_bounds__(kBlock) void overflow_kernel(const AssocProblem* __restrict__ probs, KConsts kc,
+                                                          AssocProblem one) {
+  if constexpr (ONE) overflow_body<FC, KC>(one, kc);
+  else overflow_body<FC, KC>(probs[blockIdx.y], kc);
+}
+
 // ---------------------------------------------------------------------------
 // Device controller pieces (FP64)
 // ---------------------------------------------------------------------------
@@ -1203,7 +1217,7 @@ __device__ double apply_step(ProblemState* st, const KConsts& kc, const double x
                              double& step);
 
 // One controller step of align_impl (CvoGPU.cu:1342-1433), FP64 pose state.
-__device__ void controller(ProblemState* st, const KConsts& kc, const AssocProblem& P, const float* level_h,
+__device__ __forceinline__ void controller(ProblemState* st, const KConsts& kc, const AssocProblem& P, const float* level_h,
                            int n_levels) {
   cvo_iter_record rec;
   const int k = st->iter;
@@ -1342,7 +1356,8 @@ __device__ double select_step_adaptive(const KConsts& kc, const double bcde[4]) 
 //   l <- l - dl_step dl; past ell_max both shrink to 0.7 ell_max; at least ell_min.
 // The source and target self-association totals come from P.xx_st / P.yy_st
 // (their passes ran with this one; st->ip holds sum(a |y-x|^2)).
-__device__ void controller_adaptive(ProblemState* st, const KConsts& kc, const AssocProblem& P, const float* level_h,
+__device__ __forceinline__ void controller_adaptive(ProblemState* st, const KConsts& kc, const AssocProblem& P,
+                                                    const float* level_h,
                                     int n_levels) {
   cvo_iter_record rec;
   const int k = st->iter;
@@ -1461,7 +1476,7 @@ __device__ __forceinline__ void scaled_flow(const ProblemState* st, const KConst
 // The controller step of a problem, run by the step pass's last block: step
 // coefficients summed over the step blocks in fixed order, then the scalar
 // controller on thread 0 with the state staged in LDS.
-__device__ void controller_tail(const AssocProblem& P, const KConsts& kc, const double tot[6], int nsb) {
+__device__ __forceinline__ void controller_tail(const AssocProblem& P, const KConsts& kc, const double tot[6], int nsb) {
   ProblemState* st = P.st;
   const int tid = threadIdx.x;
   __shared__ ProblemState s_st;
@@ -1515,9 +1530,7 @@ __device__ void controller_tail(const AssocProblem& P, const KConsts& kc, const 
 
 // One thread per source row; grid (row blocks, problems).  The last block of
 // a problem runs its controller step.
-__global__ __launch_bounds__(kBlock) void step_kernel(const AssocProblem* __restrict__ probs, int n_probs,
-                                                      KConsts kc) {
-  const AssocProblem& P = probs[blockIdx.y];
+__device__ __forceinline__ void step_pass(const AssocProblem& P, const KConsts& kc) {
   ProblemState* st = P.st;
   if (st->done) return;
   const int tid = threadIdx.x;
@@ -1608,6 +1621,13 @@ __global__ __launch_bounds__(kBlock) void step_kernel(const AssocProblem* __rest
   __syncthreads();
   if (!s_last) return;  // uniform
   controller_tail(P, kc, tot, nsb);
+}
+
+template <bool ONE>
+__global__ __launch_bounds__(kBlock) void step_kernel(const AssocProblem* __restrict__ probs, KConsts kc,
+                                                      AssocProblem one) {
+  if constexpr (ONE) step_pass(one, kc);
+  else step_pass(probs[blockIdx.y], kc);
 }
 
 // Initial state of a problem: pose from a row-major float 4x4, maps, level.
@@ -1750,16 +1770,23 @@ void launch_all_done(const AssocProblem* probs, int n_probs, int32_t* out, hipSt
 
 template <int FC, int KC>
 static void launch_assoc_t(const AssocProblem* probs, int n_probs, int total_blocks, const KConsts& kc,
-                           hipStream_t s) {
-  if (kc.knn > 0)
-    hipLaunchKernelGGL((knn_assoc_kernel<FC, KC>), dim3(total_blocks), dim3(kBlock), 0, s, probs, n_probs, kc);
-  else
-    hipLaunchKernelGGL((assoc_kernel<FC, KC>), dim3(total_blocks), dim3(kBlock), 0, s, probs, n_probs, kc);
+                           const AssocProblem& one, bool use_one, hipStream_t s) {
+  const dim3 g(total_blocks), b(kBlock);
+  if (kc.knn > 0) {
+    if (use_one) hipLaunchKernelGGL((knn_assoc_kernel<FC, KC, true>), g, b, 0, s, probs, n_probs, kc, one);
+    else hipLaunchKernelGGL((knn_assoc_kernel<FC, KC, false>), g, b, 0, s, probs, n_probs, kc, one);
+  } else {
+    if (use_one) hipLaunchKernelGGL((assoc_kernel<FC, KC, true>), g, b, 0, s, probs, n_probs, kc, one);
+    else hipLaunchKernelGGL((assoc_kernel<FC, KC, false>), g, b, 0, s, probs, n_probs, kc, one);
+  }
 }
 
 template <int FC, int KC>
-static void launch_ovf_t(const AssocProblem* probs, int n_probs, const KConsts& kc, hipStream_t s) {
-  hipLaunchKernelGGL((overflow_kernel<FC, KC>), dim3(kOvfBlocks, n_probs), dim3(kBlock), 0, s, probs, n_probs, kc);
+static void launch_ovf_t(const AssocProblem* probs, int n_probs, const KConsts& kc, const AssocProblem& one,
+                         bool use_one, hipStream_t s) {
+  const dim3 g(kOvfBlocks, n_probs), b(kBlock);
+  if (use_one) hipLaunchKernelGGL((overflow_kernel<FC, KC, true>), g, b, 0, s, probs, kc, one);
+  else hipLaunchKernelGGL((overflow_kernel<FC, KC, false>), g, b, 0, s, probs, kc, one);
 }
 
 static int fsel(const KConsts& kc) {
@@ -1785,17 +1812,32 @@ static int ksel(const KConsts& kc) {
     else FN<-1, -1>(__VA_ARGS__);                             \
   } while (0)
 
-void launch_assoc(const AssocProblem* probs, int n_probs, int total_blocks, const KConsts& kc, hipStream_t s) {
-  CVO_DISPATCH_FK(launch_assoc_t, probs, n_probs, total_blocks, kc, s);
+void launch_assoc(const AssocProblem* probs, int n_probs, int total_blocks, const KConsts& kc, hipStream_t s,
+                  const AssocProblem* host_one) {
+  AssocProblem one{};
+  const bool use_one = n_probs == 1 && host_one;
+  if (use_one) one = *host_one;
+  CVO_DISPATCH_FK(launch_assoc_t, probs, n_probs, total_blocks, kc, one, use_one, s);
 }
 
-void launch_overflow(const AssocProblem* probs, int n_probs, const KConsts& kc, hipStream_t s) {
-  CVO_DISPATCH_FK(launch_ovf_t, probs, n_probs, kc, s);
+void launch_overflow(const AssocProblem* probs, int n_probs, const KConsts& kc, hipStream_t s,
+                     const AssocProblem* host_one) {
+  AssocProblem one{};
+  const bool use_one = n_probs == 1 && host_one;
+  if (use_one) one = *host_one;
+  CVO_DISPATCH_FK(launch_ovf_t, probs, n_probs, kc, one, use_one, s);
 }
 
-void launch_step(const AssocProblem* probs, int n_probs, int max_src_rows, const KConsts& kc, hipStream_t s) {
-  hipLaunchKernelGGL(step_kernel, dim3((max_src_rows + kBlock - 1) / kBlock, n_probs), dim3(kBlock), 0, s, probs,
-                     n_probs, kc);
+void launch_step(const AssocProblem* probs, int n_probs, int max_src_rows, const KConsts& kc, hipStream_t s,
+                 const AssocProblem* host_one) {
+  const dim3 g((max_src_rows + kBlock - 1) / kBlock, n_probs), b(kBlock);
+  AssocProblem one{};
+  if (n_probs == 1 && host_one) {
+    one = *host_one;
+    hipLaunchKernelGGL(step_kernel<true>, g, b, 0, s, probs, kc, one);
+  } else {
+    hipLaunchKernelGGL(step_kernel<false>, g, b, 0, s, probs, kc, one);
+  }
 }
 
 void launch_adaptive_sync(const AssocProblem* probs, const KConsts& kc, hipStream_t s) {

@@ -300,12 +300,12 @@ extern "C" cvo_status cvo_autotune_d(cvo_ctx* ctx, const cvo_dcloud* src, const 
       kv.max_group = 16;  // the problems' block ranges
       for (int i = 0; i < n_samples; ++i) {
         launch_init_state(pr.dev, 1, as<float>(ctx->b_tmp2), 16, kv, ells[i], p.nearest_neighbors_max, st);
-        launch_assoc(pr.dev, 1, pr.total_blocks, kv, st);  // warm-up
+        launch_assoc(pr.dev, 1, pr.total_blocks, kv, st, pr.host.data());  // warm-up
         CVO_HIP(ctx, hipGetLastError());
         float best_ms = 1e30f;
         for (int rd = 0; rd < kRounds; ++rd) {
           CVO_HIP(ctx, hipEventRecord(e0, st));
-          for (int r = 0; r < kReps; ++r) launch_assoc(pr.dev, 1, pr.total_blocks, kv, st);
+          for (int r = 0; r < kReps; ++r) launch_assoc(pr.dev, 1, pr.total_blocks, kv, st, pr.host.data());
           CVO_HIP(ctx, hipEventRecord(e1, st));
           CVO_HIP(ctx, hipEventSynchronize(e1));
           float ms = 0.0f;
