@@ -233,6 +233,14 @@ cvo_status cvo_align_adaptive(cvo_ctx* ctx, const cvo_cloud_view* src, const cvo
 cvo_status cvo_align_adaptive_d(cvo_ctx* ctx, const cvo_dcloud* src, const cvo_dcloud* tgt,
                                 const float T_init[16], float T_out[16], double* seconds, int32_t* iters,
                                 int32_t* ret_code);
+/* AdaptiveCvoGPU::inner_product (AdaptiveCvoGPU.hpp:178-180,
+ * AdaptiveCvoGPU.cu:1338-1371, se_kernel_init_ell_cpu :1251-1336): the target
+ * moved by T_s2t itself (rot y + trans), every pair that passes the adaptive
+ * association's gates at ell_init (no row cap), sum(a) / N * 1e6 / M.  The
+ * reference computes it on the CPU; here it is one uncapped association
+ * pass.  0 on empty input. */
+cvo_status cvo_inner_product_adaptive(cvo_ctx* ctx, const cvo_cloud_view* src, const cvo_cloud_view* tgt,
+                                      const float T_s2t[16], float* out);
 
 /* Optional per-iteration trace of the next align call(s): up to cap records. */
 cvo_status cvo_set_trace(cvo_ctx* ctx, int32_t cap);

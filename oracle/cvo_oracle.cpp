@@ -1320,6 +1320,35 @@ extern "C" int orc_assoc_adaptive(const cvo_params* p, const cvo_cloud_view* src
   return 0;
 }
 
+extern "C" double orc_inner_product_adaptive(const cvo_params* p, const cvo_cloud_view* src,
+                                             const cvo_cloud_view* tgt, const float T_s2t[16]) {
+  if (!p || !src || !tgt || !T_s2t || !src->feat || !tgt->feat) return -1.0;
+  if (src->n == 0 || tgt->n == 0) return 0.0;
+  // the target moved by T itself: the applied map of its inverse (float, as
+  // the library forms it)
+  float Ti[16];
+  for (int r = 0; r < 3; ++r) {
+    for (int c = 0; c < 3; ++c) Ti[r * 4 + c] = T_s2t[c * 4 + r];
+    Ti[r * 4 + 3] = (float)(-((double)T_s2t[0 * 4 + r] * T_s2t[3] + (double)T_s2t[1 * 4 + r] * T_s2t[7] +
+                              (double)T_s2t[2 * 4 + r] * T_s2t[11]));
+  }
+  Ti[12] = Ti[13] = Ti[14] = 0.0f;
+  Ti[15] = 1.0f;
+  const int M = tgt->n;
+  std::vector<float> yp((size_t)M * 3);
+  float Am[9], b[3];
+  applied_transform(pose_from16(Ti), Am, b);
+  for (int j = 0; j < M; ++j) xform_f(Am, b, tgt->xyz + 3 * (size_t)j, &yp[3 * (size_t)j]);
+  AcvoRows o;
+  const int C = p->is_using_semantics ? src->num_classes : 0;
+  acvo_assoc(p, src->n, src->xyz, src->feat, src->sem, M, yp.data(), tgt->feat, tgt->sem, src->feat_dim, C,
+             p->ell_init, M, ORC_MIRROR, o);
+  double sum = 0.0;
+  for (int i = 0; i < src->n; ++i)
+    for (int q = 0; q < o.counts[i]; ++q) sum += (double)o.vals[(size_t)i * M + q];
+  return (double)(float)(sum / (double)src->n * 1e6 / (double)M);
+}
+
 extern "C" int orc_align_adaptive(const cvo_params* p, const cvo_cloud_view* src, const cvo_cloud_view* tgt,
                                   const float T_init[16], int32_t precision, float T_out[16], int32_t* iters,
                                   int32_t* ret_code, cvo_iter_record* trace, int32_t trace_cap) {

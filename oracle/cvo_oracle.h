@@ -72,6 +72,11 @@ int orc_align_adaptive(const cvo_params* p, const cvo_cloud_view* src, const cvo
 int orc_assoc_adaptive(const cvo_params* p, const cvo_cloud_view* src, const cvo_cloud_view* tgt,
                        const float T[16], float ell, int32_t K, int32_t precision, int32_t* counts,
                        int32_t* cols, float* vals, double* ad2);
+/* AdaptiveCvoGPU::inner_product (AdaptiveCvoGPU.cu:1251-1371): the target
+ * moved by T_s2t, every pair passing the adaptive gates at ell_init (no row
+ * cap), sum(a) / N * 1e6 / M. */
+double orc_inner_product_adaptive(const cvo_params* p, const cvo_cloud_view* src, const cvo_cloud_view* tgt,
+                                  const float T_s2t[16]);
 /* Its step rule (AdaptiveCvoGPU.cu:1039-1074) on B, C, D, E. */
 float orc_select_step_adaptive(const cvo_params* p, const double bcde[4]);
 

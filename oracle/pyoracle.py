@@ -47,6 +47,8 @@ def lib():
         _lib.orc_assoc_adaptive.argtypes = [P, P, P, P, f, i32, i32, P, P, P, P]
         _lib.orc_select_step_adaptive.argtypes = [P, P]
         _lib.orc_select_step_adaptive.restype = f
+        _lib.orc_inner_product_adaptive.argtypes = [P, P, P, P]
+        _lib.orc_inner_product_adaptive.restype = d
         _lib.orc_pair_value.argtypes = [P, P, P, P, f, i32, i32, P, P]
         _lib.orc_flow.argtypes = [P, i32, P, P, i32, P, P, P, i32, P, P]
         _lib.orc_step_coeffs.argtypes = [P, f, i32, P, P, i32, P, P, P, P, i32, P]
@@ -120,6 +122,11 @@ def assoc_adaptive(params, src, tgt, T, ell, K, precision=MIRROR):
                                   _p(counts), _p(cols), _p(vals), _p(ad2))
     assert rc == 0, rc
     return counts, cols.reshape(N, K), vals.reshape(N, K), ad2
+
+
+def inner_product_adaptive(params, src, tgt, T_s2t):
+    hs, ht = HeldView(src), HeldView(tgt)
+    return lib().orc_inner_product_adaptive(C.byref(params), hs.ref, ht.ref, _p(mat16(T_s2t)))
 
 
 def select_step_adaptive(params, bcde):

@@ -890,11 +890,15 @@ extern "C" cvo_status cvo_align(cvo_ctx* ctx, const cvo_cloud_view* src, const c
 // Single association pass at pose T and lengthscale ell; the problem stays in
 // the context's buffers for an export.
 // knn_K > 0: the K-nearest pass with row cap knn_K instead of the ball pass.
+static KConsts adaptive_kconsts(const cvo_params& p, int F, int Ks);
+
+// kc_in: the pass's kernel constants (mode MODE_IP) instead of the
+// reference's association.
 static cvo_status single_pass(cvo_ctx* ctx, const cvo_dcloud* src, const cvo_dcloud* tgt, const float T[16], float ell,
-                              Problems& pr, KConsts& kc, double* ip, int knn_K = 0) {
+                              Problems& pr, KConsts& kc, double* ip, int knn_K = 0, const KConsts* kc_in = nullptr) {
   CVO_TRY(validate_pair(ctx, src, tgt));
   const cvo_params& p = ctx->params;
-  kc = make_kconsts(p, src->F, src->K, MODE_IP);
+  kc = kc_in ? *kc_in : make_kconsts(p, src->F, src->K, MODE_IP);
   kc.knn = knn_K > 0 ? 1 : 0;
   const int K0 = knn_K > 0 ? knn_K : p.nearest_neighbors_max;
   apply_tuning(ctx, kc);
@@ -1134,6 +1138,8 @@ static KConsts adaptive_kconsts(const cvo_params& p, int F, int Ks) {
   KConsts k = make_kconsts(p, F, Ks, MODE_ALIGN);
   k.knn = 0;
   k.adaptive = 1;
+  k.fixed_ell = 1;
+  k.ip_ad2 = 1;
   k.dl_step = p.dl_step;
   k.ell_max0 = p.ell_max;
   k.use_geo = 1;
@@ -1293,6 +1299,44 @@ extern "C" cvo_status cvo_align_adaptive(cvo_ctx* ctx, const cvo_cloud_view* src
   CVO_TRY(cvo_cloud_upload(ctx, src, &a.c));
   CVO_TRY(cvo_cloud_upload(ctx, tgt, &b.c));
   return cvo_align_adaptive_d(ctx, a.c, b.c, T_init, T_out, seconds, iters, ret_code);
+}
+
+extern "C" cvo_status cvo_inner_product_adaptive(cvo_ctx* ctx, const cvo_cloud_view* src, const cvo_cloud_view* tgt,
+                                                 const float T_s2t[16], float* out) {
+  if (!ctx || !src || !tgt || !T_s2t || !out) return CVO_ERR_INVALID_ARG;
+  *out = 0.0f;
+  if (src->n == 0 || tgt->n == 0) return CVO_OK;  // the reference returns 0
+  Uploaded a{ctx}, b{ctx};
+  CVO_TRY(cvo_cloud_upload(ctx, src, &a.c));
+  CVO_TRY(cvo_cloud_upload(ctx, tgt, &b.c));
+  if (!a.c->feat || !b.c->feat || a.c->F != b.c->F)
+    return set_err(ctx, CVO_ERR_INVALID_ARG, "the adaptive-lengthscale association needs features on both clouds");
+  const cvo_params& p = ctx->params;
+  if (p.is_using_semantics && (!a.c->sem || !b.c->sem || a.c->K != b.c->K))
+    return set_err(ctx, CVO_ERR_INVALID_ARG, "is_using_semantics needs class vectors on both clouds");
+  KConsts kc = adaptive_kconsts(p, a.c->F, p.is_using_semantics ? a.c->K : 0);
+  if (!p.is_using_semantics) kc.use_sem = 0;
+  kc.amp = kc.sigma2 * kc.c_sigma2 * (kc.use_sem ? kc.s_sigma2 : 1.0f);
+  kc.mode = MODE_IP;
+  kc.adaptive = 0;
+  kc.ip_ad2 = 0;
+  kc.uncapped = 1;
+  // the target moves by T itself (rot y + trans); a pass moves it by the
+  // inverse of its pose, so the pass gets T^-1
+  float Ti[16];
+  for (int r = 0; r < 3; ++r) {
+    for (int c = 0; c < 3; ++c) Ti[r * 4 + c] = T_s2t[c * 4 + r];
+    Ti[r * 4 + 3] = (float)(-((double)T_s2t[0 * 4 + r] * T_s2t[3] + (double)T_s2t[1 * 4 + r] * T_s2t[7] +
+                              (double)T_s2t[2 * 4 + r] * T_s2t[11]));
+  }
+  Ti[12] = Ti[13] = Ti[14] = 0.0f;
+  Ti[15] = 1.0f;
+  Problems pr;
+  KConsts kcu;
+  double ip = 0.0;
+  CVO_TRY(single_pass(ctx, a.c, b.c, Ti, p.ell_init, pr, kcu, &ip, 0, &kc));
+  *out = (float)(ip / (double)src->n * 1e6 / (double)tgt->n);
+  return CVO_OK;
 }
 
 extern "C" cvo_status cvo_set_pose_arithmetic(cvo_ctx* ctx, int32_t mode) {

@@ -161,6 +161,9 @@ struct KConsts {
   int32_t adaptive;
   double dl_step;
   float ell_max0;             // initial lengthscale bound (ell_max)
+  int32_t fixed_ell;          // the association uses ell as is (no range scaling)
+  int32_t ip_ad2;             // the ip column sums a |y - x|^2 instead of a
+  int32_t uncapped;           // sum every passing pair (no row cap, no capped rows)
 };
 
 // Rows per association block for a lane count per row (8 or 16).

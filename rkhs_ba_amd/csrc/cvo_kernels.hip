@@ -313,7 +313,7 @@ __device__ __forceinline__ void assoc_row(const AssocProblem& P, const KConsts& 
     x0 = t0; x1 = t1; x2 = t2;
   }
   // the adaptive-lengthscale association uses ell as is (AdaptiveCvoGPU.cu:281)
-  const float l = kc.adaptive ? ell : range_ell(ell, x0, x1, x2);
+  const float l = kc.fixed_ell ? ell : range_ell(ell, x0, x1, x2);
   const float d2t = (float)(-2.0 * (double)l * (double)l * (double)kc.lg);
   const float qg = (float)(1.0 / (2.0 * (double)l * (double)l));
   float xg[2] = {0.0f, 0.0f};
@@ -395,9 +395,11 @@ __device__ __forceinline__ void assoc_row(const AssocProblem& P, const KConsts& 
       constexpr uint32_t kGroupMask = (uint32_t)((1ull << G) - 1ull);
       const uint32_t gm = (uint32_t)(m >> (G * ((threadIdx.x & 63) / G))) & kGroupMask;
       const uint32_t slot = stored + (uint32_t)__popc(gm & ((1u << g) - 1u));
-      if (pass && slot < Ku) {
-        if (mode == MODE_BA) P.pairs4[(size_t)r * Ku + slot] = make_float4(y4.x, y4.y, y4.z, a);
-        else P.pairs[(size_t)r * P.cap_alloc + slot] = make_uint2(pidx, __float_as_uint(a));
+      if (pass && (slot < Ku || kc.uncapped)) {
+        if (slot < Ku) {
+          if (mode == MODE_BA) P.pairs4[(size_t)r * Ku + slot] = make_float4(y4.x, y4.y, y4.z, a);
+          else P.pairs[(size_t)r * P.cap_alloc + slot] = make_uint2(pidx, __float_as_uint(a));
+        }
         if (mode == MODE_ALIGN) {
           const float c0 = x1 * y2 - x2 * y1, c1 = x2 * y0 - x0 * y2, c2 = x0 * y1 - x1 * y0;
           out.acc[0] += (double)(c0 * a);
@@ -407,17 +409,18 @@ __device__ __forceinline__ void assoc_row(const AssocProblem& P, const KConsts& 
           out.acc[4] += (double)((y1 - x1) * a);
           out.acc[5] += (double)((y2 - x2) * a);
         }
-        out.acc[6] += (double)(kc.adaptive ? sq_dist(x0, x1, x2, y0, y1, y2) * a : a);
+        out.acc[6] += (double)(kc.ip_ad2 ? sq_dist(x0, x1, x2, y0, y1, y2) * a : a);
       }
       stored += (uint32_t)__popc(gm);
-      if (stored > Ku) {
+      if (stored > Ku && !kc.uncapped) {
         stop = true;
         break;
       }
     }
   }
-  out.stored = stored;
-  out.overflow = stored > Ku;
+  // uncapped: every passing pair is summed, the first K stored
+  out.stored = kc.uncapped ? min(stored, Ku) : stored;
+  out.overflow = stored > Ku && !kc.uncapped;
 }
 
 // ---------------------------------------------------------------------------
@@ -961,7 +964,7 @@ __device__ __forceinline__ void overflow_body(const AssocProblem& P, const KCons
       affine3(mp.As, mp.bs, x0, x1, x2, t0, t1, t2);
       x0 = t0; x1 = t1; x2 = t2;
     }
-    const float l = kc.adaptive ? ell : range_ell(ell, x0, x1, x2);
+    const float l = kc.fixed_ell ? ell : range_ell(ell, x0, x1, x2);
     const float d2t = kc.use_geo ? (float)(-2.0 * (double)l * (double)l * (double)kc.lg) : 1.0f;
     const float qg = (float)(1.0 / (2.0 * (double)l * (double)l));
     const float* xf = P.src_feat ? P.src_feat + (size_t)r * kc.F : nullptr;
@@ -1011,7 +1014,7 @@ __device__ __forceinline__ void overflow_body(const AssocProblem& P, const KCons
           acc[4] += (double)((y1 - x1) * a);
           acc[5] += (double)((y2 - x2) * a);
         }
-        acc[6] += (double)(kc.adaptive ? sq_dist(x0, x1, x2, y0, y1, y2) * a : a);
+        acc[6] += (double)(kc.ip_ad2 ? sq_dist(x0, x1, x2, y0, y1, y2) * a : a);
       }
       stored += tot;
       __syncthreads();

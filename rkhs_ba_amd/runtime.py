@@ -59,6 +59,7 @@ def lib() -> C.CDLL:
         L.cvo_compute_association_knn.argtypes = [P, P, P, P, f, i32, P]
         L.cvo_align_adaptive.argtypes = [P, P, P, P, P, P, P, P]
         L.cvo_align_adaptive_d.argtypes = [P, P, P, P, P, P, P, P]
+        L.cvo_inner_product_adaptive.argtypes = [P, P, P, P, P]
         L.cvo_align.argtypes = [P, P, P, P, P, P, P, P, P]
         L.cvo_align_d.argtypes = [P, P, P, P, P, P, P, P, P]
         L.cvo_align_multistart.argtypes = [P, P, P, i32, P, f, i32, P, P, P, P, P, P]
@@ -305,6 +306,16 @@ class CvoGPU:
                                           C.byref(secs), C.byref(iters), C.byref(ret))
         _check(st, self._ctx)
         return ret.value, Tout.reshape(4, 4), secs.value, iters.value
+
+    def inner_product_adaptive(self, source, target, s2t_frame_transform) -> float:
+        """AdaptiveCvoGPU::inner_product (AdaptiveCvoGPU.cu:1338-1371): the
+        target moved by the transform, the adaptive association at ell_init
+        without a row cap, sum(a) / N * 1e6 / M."""
+        T0 = mat16(s2t_frame_transform)
+        hs, ht = HeldView(source), HeldView(target)
+        out = C.c_float()
+        _check(lib().cvo_inner_product_adaptive(self._ctx, hs.ref, ht.ref, T0.ctypes.data, C.byref(out)), self._ctx)
+        return out.value
 
     def align_result(self, source, target, init_guess_T_s2t, is_returning_association: bool = False) -> CvoResultInfo:
         """CvoResultInfo align(source, target, T_init_s2t, is_returning_association)

@@ -109,3 +109,17 @@ def test_adaptive_errors(actx):
     empty = synth.Cloud(np.zeros((0, 3), np.float32), np.zeros((0, 5), np.float32))
     with pytest.raises(CvoError):
         actx.align_adaptive(src, empty, np.eye(4))
+
+
+def test_adaptive_inner_product_matches_oracle(actx, oracle):
+    p = knn_params(params_from_yaml("planes_spheres_5k.yaml"))
+    actx.write_params(p)
+    src, tgt, T = knn_pair(0)
+    for Tq in (T, np.eye(4, dtype=np.float32)):
+        say("adaptive inner product")
+        got = actx.inner_product_adaptive(src, tgt, Tq)
+        want = oracle.inner_product_adaptive(p, src, tgt, Tq)
+        assert want > 0 and abs(got - want) <= 1e-5 * want, (got, want)
+    empty = synth.Cloud(np.zeros((0, 3), np.float32), np.zeros((0, 5), np.float32))
+    assert actx.inner_product_adaptive(src, empty, T) == 0.0
+

@@ -175,3 +175,20 @@ def test_adaptive_registers(oracle, precision):
     assert trans_err(To, Ti) < 0.05 * trans_err(np.eye(4), Ti)
     assert it < p.MAX_ITER and ret == 0
     assert len({round(r.ell, 6) for r in tr}) > 5  # the lengthscale moves
+
+
+def test_adaptive_inner_product_matches_brute_force(oracle, far):
+    """Every passing pair at ell_init, no row cap, the target moved by T."""
+    p, src, tgt, T = far
+    got = oracle.inner_product_adaptive(p, src, tgt, T)
+    R, t = T[:3, :3].astype(np.float64), T[:3, 3].astype(np.float64)
+    y = tgt.xyz.astype(np.float64) @ R.T + t
+    ell = float(np.float32(p.ell_init))
+    tot = 0.0
+    for i in range(src.n):
+        a, _, ok, _ = brute_row(p, src.xyz[i].astype(np.float64), src.feat[i], src.sem[i], y, tgt.feat, tgt.sem,
+                                ell, True)
+        tot += a[ok].sum()
+    want = tot / src.n * 1e6 / tgt.n
+    assert want > 0 and abs(got - want) <= 2e-4 * want, (got, want)
+
