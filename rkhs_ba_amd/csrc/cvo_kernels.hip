@@ -905,6 +905,165 @@ __device__ __forceinline__ void pass_totals_one_block(const AssocProblem& P, Pro
   }
 }
 
+// A row whose passing pairs exceed the cap keeps the first K passing targets
+// by ascending target index (CvoGPU.cu:994-996).  Every passing target lies
+// in the distance gate's ball, so one wave finds them there: it counts the
+// ball's passing targets and their index range, selects the K smallest
+// indices by a most-significant-digit radix select (8 bits a pass, a 256-bin
+// LDS histogram), then stores exactly those.  acc: the lane's share of the
+// row's flow and ip; returns the stored count.
+template <int FC, int KC>
+__device__ uint32_t capped_row_ball(const AssocProblem& P, const KConsts& kc, const ExactK& ek,
+                                    const GridLevel& GL, const Maps& mp, int r, float ell, uint32_t K,
+                                    uint32_t* hist, double acc[7]) {
+  const int g = threadIdx.x & 63;
+  const int mode = kc.mode;
+  const int F = Dim<FC>::get(kc.F);
+  const int KS = Dim<KC>::get(kc.Ks);
+  float4 x4 = P.src_pos[r];
+  float x0 = x4.x, x1 = x4.y, x2 = x4.z;
+  if (mp.src_xf) {
+    float t0, t1, t2;
+    affine3(mp.As, mp.bs, x0, x1, x2, t0, t1, t2);
+    x0 = t0; x1 = t1; x2 = t2;
+  }
+  const float l = kc.fixed_ell ? ell : range_ell(ell, x0, x1, x2);
+  const float d2t = (float)(-2.0 * (double)l * (double)l * (double)kc.lg);
+  const float qg = (float)(1.0 / (2.0 * (double)l * (double)l));
+  const float* xf = P.src_feat ? P.src_feat + (size_t)r * kc.F : nullptr;
+  const float* xs = P.src_sem ? P.src_sem + (size_t)r * kc.Ks : nullptr;
+  float xg[2] = {0.0f, 0.0f};
+  if (kc.use_gtype) {
+    xg[0] = P.src_geo[2 * (size_t)r];
+    xg[1] = P.src_geo[2 * (size_t)r + 1];
+  }
+  float q0, q1, q2;
+  affine3(mp.Rq, mp.Tq, x0, x1, x2, q0, q1, q2);
+  const float rq = sqrtf(d2t) * 1.001f + 1e-4f * (1.0f + fmaxf(fabsf(q0), fmaxf(fabsf(q1), fabsf(q2))));
+  const CellBox B = cell_box(GL, q0, q1, q2, rq);
+  // a passing target's key is its index; ~0 for the others
+  auto eval = [&](uint32_t pidx, float4& y4, float& y0, float& y1, float& y2, float& a) -> uint64_t {
+    y4 = GL.pos[pidx];
+    affine3(mp.A, mp.b, y4.x, y4.y, y4.z, y0, y1, y2);
+    const bool pass = pair_kernel<FC, KC>(kc, ek, x0, x1, x2, y0, y1, y2, d2t, qg, l, xf,
+                                          FC != 0 ? GL.feat + (size_t)pidx * F : nullptr, xs,
+                                          KC != 0 ? GL.sem + (size_t)pidx * KS : nullptr, xg,
+                                          kc.use_gtype ? GL.geo + 2 * (size_t)pidx : nullptr, a);
+    return pass ? (uint64_t)__float_as_uint(y4.w) : ~0ull;
+  };
+  uint32_t cnt = 0;
+  uint64_t kmin = ~0ull, kmax = 0;
+  for_each_in_box<64>(GL, B, g, [&](bool valid, uint32_t pidx) {
+    if (!valid) return;
+    float4 y4;
+    float y0, y1, y2, a;
+    const uint64_t key = eval(pidx, y4, y0, y1, y2, a);
+    if (key == ~0ull) return;
+    cnt++;
+    kmin = min(kmin, key);
+    kmax = max(kmax, key);
+  });
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    cnt += __shfl_xor(cnt, o, 64);
+    kmin = min(kmin, (uint64_t)__shfl_xor((unsigned long long)kmin, o, 64));
+    kmax = max(kmax, (uint64_t)__shfl_xor((unsigned long long)kmax, o, 64));
+  }
+  uint64_t thr = ~0ull;
+  int tsh = 0;
+  if (cnt > K && K > 0) {
+    // indices are distinct, so kmin != kmax and plen < 64
+    uint32_t need = K;
+    int plen = __clzll((long long)(kmin ^ kmax));
+    uint64_t prefix = plen > 0 ? (kmin & ~(~0ull >> plen)) : 0ull;
+    constexpr int CH = 256 / 64;
+    for (int pass = 0; pass < 8 && plen < 64; ++pass) {
+      const int db = min(8, 64 - plen);
+      const int sh = 64 - plen - db;
+#pragma unroll
+      for (int i = 0; i < CH; ++i) hist[g * CH + i] = 0u;
+      wave_lds_sync();
+      for_each_in_box<64>(GL, B, g, [&](bool valid, uint32_t pidx) {
+        if (!valid) return;
+        float4 y4;
+        float y0, y1, y2, a;
+        const uint64_t key = eval(pidx, y4, y0, y1, y2, a);
+        if (key == ~0ull) return;
+        if (plen > 0 && (key >> (64 - plen)) != (prefix >> (64 - plen))) return;
+        atomicAdd(&hist[(uint32_t)(key >> sh) & 0xffu], 1u);
+      });
+      wave_lds_sync();
+      uint32_t csum = 0;
+#pragma unroll
+      for (int i = 0; i < CH; ++i) csum += hist[g * CH + i];
+      uint32_t incl = csum;
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        uint32_t t = __shfl_up(incl, o, 64);
+        if (g >= o) incl += t;
+      }
+      const uint32_t excl = incl - csum;
+      const bool mine = excl < need && incl >= need;
+      uint32_t b = 0, less = 0, cb = 0;
+      if (mine) {
+        uint32_t run = excl;
+        for (int i = 0; i < CH; ++i) {
+          const uint32_t h = hist[g * CH + i];
+          if (run + h >= need) {
+            b = (uint32_t)(g * CH + i);
+            less = run;
+            cb = h;
+            break;
+          }
+          run += h;
+        }
+      }
+      const uint64_t gm = __ballot(mine);
+      wave_lds_sync();
+      if (gm == 0ull) break;  // cannot happen while the counts are consistent
+      const int owner = __ffsll((unsigned long long)gm) - 1;
+      b = __shfl(b, owner, 64);
+      less = __shfl(less, owner, 64);
+      cb = __shfl(cb, owner, 64);
+      prefix |= (uint64_t)b << sh;
+      thr = prefix;
+      tsh = sh;
+      if (less + cb == need) break;
+      need -= less;
+      plen += db;
+    }
+  }
+  uint32_t stored = 0;
+  const uint64_t thr_s = thr >> tsh;
+  for_each_in_box<64>(GL, B, g, [&](bool valid, uint32_t pidx) {
+    bool take = false;
+    float a = 0.0f, y0 = 0.0f, y1 = 0.0f, y2 = 0.0f;
+    float4 y4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (valid) {
+      const uint64_t key = eval(pidx, y4, y0, y1, y2, a);
+      take = key != ~0ull && (key >> tsh) <= thr_s;
+    }
+    const uint64_t m = __ballot(take);
+    const uint32_t slot = stored + (uint32_t)__popcll(m & ((g == 0) ? 0ull : (~0ull >> (64 - g))));
+    if (take && slot < K) {
+      if (mode == MODE_BA) P.pairs4[(size_t)r * K + slot] = make_float4(y4.x, y4.y, y4.z, a);
+      else P.pairs[(size_t)r * P.cap_alloc + slot] = make_uint2(pidx, __float_as_uint(a));
+      if (mode == MODE_ALIGN) {
+        const float c0 = x1 * y2 - x2 * y1, c1 = x2 * y0 - x0 * y2, c2 = x0 * y1 - x1 * y0;
+        acc[0] += (double)(c0 * a);
+        acc[1] += (double)(c1 * a);
+        acc[2] += (double)(c2 * a);
+        acc[3] += (double)((y0 - x0) * a);
+        acc[4] += (double)((y1 - x1) * a);
+        acc[5] += (double)((y2 - x2) * a);
+      }
+      acc[6] += (double)(kc.ip_ad2 ? sq_dist(x0, x1, x2, y0, y1, y2) * a : a);
+    }
+    stored += (uint32_t)__popcll(m);
+  });
+  return min(stored, K);
+}
+
 template <int FC, int KC>
 __device__ __forceinline__ void overflow_body(const AssocProblem& P, const KConsts& kc) {
   ProblemState* st = P.st;
@@ -955,6 +1114,52 @@ __device__ __forceinline__ void overflow_body(const AssocProblem& P, const KCons
   const int r0 = blockIdx.x * chunk, r1 = min(P.n_src, r0 + chunk);
   __shared__ uint32_t s_wave[kBlock / 64];
   __shared__ double s_red[kBlock];
+  if (kc.use_geo) {
+    // the chunk's capped rows, one wave per row in turn (wave w takes the
+    // w-th, (w+4)-th, ... capped row of the chunk)
+    __shared__ uint32_t s_hist[kBlock / 64][256];
+    __shared__ double s_wacc[kBlock / 64][9];
+    const int lane = tid & 63, wave = tid >> 6;
+    double wacc[7] = {0, 0, 0, 0, 0, 0, 0};
+    uint32_t wnnz = 0, wmax = 0, seen = 0;
+    for (int base = r0; base < r1; base += 64) {
+      const int rr = base + lane;
+      uint64_t m = __ballot(rr < r1 && P.ovf_stamp[rr] == stamp);
+      while (m) {
+        const int b = __ffsll((unsigned long long)m) - 1;
+        m &= m - 1;
+        if ((int)(seen % (kBlock / 64)) == wave) {
+          const int r = base + b;
+          double racc[7] = {0, 0, 0, 0, 0, 0, 0};
+          const uint32_t kept = capped_row_ball<FC, KC>(P, kc, ek, G, mp, r, ell, K, s_hist[wave], racc);
+          if (lane == 0) P.counts[r] = kept;
+#pragma unroll
+          for (int c = 0; c < 7; ++c) wacc[c] += racc[c];
+          wnnz += kept;
+          wmax = max(wmax, kept);
+        }
+        seen++;
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < 7; ++c) {
+      double v = wacc[c];
+      for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+      if (lane == 0) s_wacc[wave][c] = v;
+    }
+    if (lane == 0) {
+      s_wacc[wave][7] = (double)wnnz;
+      s_wacc[wave][8] = (double)wmax;
+    }
+    __syncthreads();
+    // every thread folds the waves in the same order (threads 0-7 publish)
+    for (int w = 0; w < kBlock / 64; ++w) {
+#pragma unroll
+      for (int c = 0; c < 7; ++c) blk_acc[c] += s_wacc[w][c];
+      blk_nnz += (uint32_t)s_wacc[w][7];
+      blk_max = max(blk_max, (uint32_t)s_wacc[w][8]);
+    }
+  } else
   for (int r = r0; r < r1; ++r) {
     if (P.ovf_stamp[r] != stamp) continue;  // uniform
     float4 x4 = P.src_pos[r];
