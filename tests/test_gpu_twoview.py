@@ -97,6 +97,26 @@ def test_association_row_cap_exact_path(cvo_gpu, oracle):
     compare_assoc(oracle, p, src, tgt, Tgt, 0.9, a, K=6)
 
 
+def test_association_row_cap_large_target_semantics(cvo_gpu, oracle):
+    """Capped rows against a 70k-point target with the 19-class gate: target
+    indices above 2^16 (the selection of the K smallest passing indices takes
+    three 8-bit digits), caps of 1, 7 and 100 with thousands of capped rows
+    each; the kept pairs are the first K passing targets by index, as the
+    oracle keeps them."""
+    p = params_from_yaml("street_60k.yaml")
+    src, tgt, Tgt = synth.street_pair(4, n=70000)
+    src = src.subset(np.arange(0, src.n, 25))
+    for K, ell in ((1, 0.5), (7, 0.5), (100, 1.0)):
+        p.nearest_neighbors_max = K
+        cvo_gpu.write_params(p)
+        say(f"assoc cap={K} ell={ell} street 2.8k x 70k")
+        a = cvo_gpu.compute_association_gpu(src, tgt, Tgt, ell)
+        counts = np.diff(a.row_ptr)
+        assert counts.max() == K and (counts == K).sum() > 1000
+        assert a.col.max() >= 65536
+        compare_assoc(oracle, p, src, tgt, Tgt, ell, a, K=K)
+
+
 def test_association_geometric_type_and_no_color(cvo_gpu, oracle):
     p = params_from_yaml("planes_spheres_5k.yaml")
     p.is_using_intensity = 0
