@@ -16,6 +16,10 @@ Further legs, reported as nested objects of the same line:
   learned     the learned proposer: inference of 256 pairs x 2048 points
               (MFMA roofline of its forward), and global registration from its
               8 proposals (grid fallback) against the plain 96-start grid
+  knn         the 60k street registration with is_using_kdtree = 1 (K nearest)
+  adaptive    the adaptive-lengthscale registration of the 5k planes pair
+  aniso       the non-isotropic association (3x3 kernel) of the 60k street pair
+              in camera axes
 
 Each leg carries the roofline of its dominant kernel (HIP events on the
 library's stream) and the CPU oracle timed on the host cores on a bounded
@@ -448,6 +452,204 @@ def leg_learned(args):
     return out
 
 
+# ---------------------------------------------------------------------------
+# the paths beside the plain association: K nearest, adaptive lengthscale,
+# non-isotropic kernel
+# ---------------------------------------------------------------------------
+
+def cpu_by_lengthscale(trace, time_at, budget_s=12.0):
+    """Estimate of a whole registration on the host: the GPU run's iterations
+    grouped by (lengthscale, row cap); time_at(ell, K) times one oracle pass
+    for a spread of the groups within the budget, every other group takes the
+    time of the sampled group nearest in log(ell)."""
+    groups = {}
+    for r in trace:
+        key = (round(float(r.ell), 7), max(int(r.num_neighbors), 1))
+        groups[key] = groups.get(key, 0) + 1
+    keys = sorted(groups, reverse=True)
+    t0 = time.time()
+    sampled = {}
+    first = time_at(*keys[0])
+    sampled[keys[0]] = first
+    n_fit = int(max(1, min(len(keys), budget_s / max(first, 1e-3))))
+    for i in np.linspace(0, len(keys) - 1, n_fit).round().astype(int)[1:]:
+        k = keys[i]
+        if k not in sampled:
+            sampled[k] = time_at(*k)
+    total = 0.0
+    for k, n in groups.items():
+        near = min(sampled, key=lambda s: abs(np.log(s[0]) - np.log(k[0])))
+        total += n * sampled[near]
+    return total, len(sampled), len(keys), time.time() - t0
+
+
+def leg_knn(args):
+    """The 60k street registration with is_using_kdtree = 1: each row's
+    K nearest targets inside the fixed-lengthscale gate (K = 512 on the first
+    iteration, then ceil(stored pairs / N))."""
+    from rkhs_ba_amd import CvoGPU, read_params_yaml, synth
+    params = read_params_yaml(os.path.join(CONFIGS, "street_60k.yaml"))
+    params.is_using_kdtree = 1
+    log(f"[bench] knn: street scene n={args.points}, is_using_kdtree 1")
+    src, tgt, T_gt = synth.street_pair(args.seed, n=args.points)
+    cvo = CvoGPU(params=params)
+    ds, dt = cvo.upload(src), cvo.upload(tgt)
+    I = np.eye(4, dtype=np.float32)
+    tuning = tune(cvo, ds, dt, args)
+    cvo.align(ds, dt, I)
+    steps = max(1, min(args.steps, 3))
+    t0 = time.perf_counter()
+    runs = [cvo.align(ds, dt, I) for _ in range(steps)]
+    wall = (time.perf_counter() - t0) / steps
+    ret, T, dev_s, iters, _ = runs[-1]
+    e = pose_error(T, T_gt)
+    log(f"[bench] knn {wall * 1e3:.1f} ms/registration, {iters} iters, err {e[0]:.4f} deg {e[1] * 100:.2f} cm")
+    out = {"registrations_per_s": 1.0 / wall, "ms_per_registration": wall * 1e3, "iterations": iters,
+           "pose_error": {"deg": e[0], "m": e[1]}, "tuning": tuning}
+    if args.profile_only:
+        cvo.close()
+        return out
+    cvo.reset_kernel_stats()
+    cvo.set_profiling(True)
+    cvo.set_trace(params.MAX_ITER)
+    cvo.align(ds, dt, I)
+    cvo.set_profiling(False)
+    trace = cvo.get_trace()
+    cvo.set_trace(0)
+    st = cvo.kernel_stats()
+    n = max(1, st["launches"]["assoc"])
+    ms, by = st["ms"]["assoc"] / n, st["bytes"]["assoc"] / n
+    out["roofline"] = {"bound": "hbm", "achieved": by / (ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                       "frac": by / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "kernel": "knn_assoc_kernel",
+                       "avg_launch_ms": ms, "algorithmic_bytes_per_launch": by, "launches": n, "traffic": None,
+                       "note": "latency-bound: the ball is scanned 2 + select passes times per row"}
+    out["kernels_ms_per_launch"] = {k: st["ms"][k] / max(1, st["launches"][k]) for k in ("assoc", "overflow", "step")}
+    if not args.no_cpu:
+        from oracle import pyoracle
+        threads = cpu_threads()
+        log(f"[bench] knn cpu baseline ({threads} threads)")
+
+        def time_at(ell, K):
+            t = time.perf_counter()
+            pyoracle.assoc_knn(params, src, tgt, T_gt, ell, K, precision=pyoracle.MIRROR)
+            return time.perf_counter() - t
+
+        est, n_s, n_g, spent = cpu_by_lengthscale(trace, time_at)
+        out["cpu_baseline"] = {"value": 1.0 / est, "unit": "registrations/s", "cores": threads, "kind": "port",
+                               "sample": (f"oracle K-nearest association (FP32 mirror, OpenMP) timed at {n_s} of the "
+                                          f"{n_g} (lengthscale, K) settings of the GPU run, weighted by its "
+                                          f"iterations; step terms not counted; {spent:.1f} s of CPU work"),
+                               "seconds_per_registration": est}
+    cvo.close()
+    return out
+
+
+def leg_adaptive(args):
+    """AdaptiveCvoGPU::align on the 5k/5k planes and spheres pair: three
+    association problems per iteration (x-y, x-x, y-y) and the lengthscale
+    gradient."""
+    from rkhs_ba_amd import CvoGPU, read_params_yaml, synth
+    params = read_params_yaml(os.path.join(CONFIGS, "planes_spheres_5k.yaml"))
+    log("[bench] adaptive: planes and spheres 5k/5k, 10 deg / 0.3 m")
+    src, tgt, T_gt = synth.planes_spheres_pair(args.seed)
+    cvo = CvoGPU(params=params)
+    ds, dt = cvo.upload(src), cvo.upload(tgt)
+    I = np.eye(4, dtype=np.float32)
+    cvo.align_adaptive(ds, dt, I)
+    steps = max(1, min(args.steps, 5))
+    t0 = time.perf_counter()
+    runs = [cvo.align_adaptive(ds, dt, I) for _ in range(steps)]
+    wall = (time.perf_counter() - t0) / steps
+    ret, T, dev_s, iters = runs[-1]
+    e = pose_error(T, T_gt)
+    log(f"[bench] adaptive {wall * 1e3:.1f} ms/registration, {iters} iters, err {e[0]:.4f} deg {e[1] * 100:.2f} cm")
+    out = {"registrations_per_s": 1.0 / wall, "ms_per_registration": wall * 1e3, "iterations": iters,
+           "pose_error": {"deg": e[0], "m": e[1]}}
+    if args.profile_only:
+        cvo.close()
+        return out
+    cvo.reset_kernel_stats()
+    cvo.set_profiling(True)
+    cvo.align_adaptive(ds, dt, I)
+    cvo.set_profiling(False)
+    st = cvo.kernel_stats()
+    n = max(1, st["launches"]["assoc"])
+    ms, by = st["ms"]["assoc"] / n, st["bytes"]["assoc"] / n
+    out["roofline"] = {"bound": "hbm", "achieved": by / (ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                       "frac": by / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "kernel": "assoc_kernel (three problems)",
+                       "avg_launch_ms": ms, "algorithmic_bytes_per_launch": by, "launches": n, "traffic": None,
+                       "note": "bytes count every point three times and the registration problem's pairs only"}
+    out["kernels_ms_per_launch"] = {k: st["ms"][k] / max(1, st["launches"][k]) for k in ("assoc", "overflow", "step")}
+    if not args.no_cpu:
+        from oracle import pyoracle
+        log(f"[bench] adaptive cpu baseline: one oracle registration")
+        t0 = time.perf_counter()
+        To, it_o, ret_o, _ = pyoracle.align_adaptive(params, src, tgt, I, precision=pyoracle.MIRROR)
+        spent = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": 1.0 / spent, "unit": "registrations/s", "cores": cpu_threads(),
+                               "kind": "port", "sample": f"one whole oracle registration (FP32 mirror, OpenMP "
+                                                         f"associations), {it_o} iterations",
+                               "iterations": it_o}
+    cvo.close()
+    return out
+
+
+ANISO_KERNEL = np.diag([1.0, 0.2, 3.0]).astype(np.float32)
+# world (x forward, y left, z up) -> camera (x right, y down, z forward)
+CAMERA_FROM_WORLD = np.array([[0, -1, 0, 0], [0, 0, -1, 0], [1, 0, 0, 0], [0, 0, 0, 1]], np.float64)
+
+
+def leg_aniso(args):
+    """compute_association_gpu with a 3x3 kernel on the 60k street pair (19
+    classes, colour), kernel diag(1, 0.2, 3) (the third axis along the
+    viewing ray), the clouds in camera axes: the reference builds each
+    point's frame from its ray around the camera's z axis (degenerate for
+    rays in the z = 0 plane), so the workload is the scene as a forward
+    camera frames it.  Each row's ellipsoid is searched on the target grid."""
+    from rkhs_ba_amd import CvoGPU, read_params_yaml, synth
+    params = read_params_yaml(os.path.join(CONFIGS, "street_60k.yaml"))
+    log(f"[bench] non-isotropic association: street scene n={args.points}, camera axes")
+    src, tgt, T_gt = synth.street_pair(args.seed, n=args.points)
+    C4 = CAMERA_FROM_WORLD
+    src, tgt = src.transformed(C4), tgt.transformed(C4)
+    T_gt = (C4 @ np.asarray(T_gt, np.float64) @ C4.T).astype(np.float32)
+    cvo = CvoGPU(params=params)
+    a = cvo.compute_association_gpu(src, tgt, T_gt, ANISO_KERNEL)
+    steps = max(1, min(args.steps, 5))
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        a = cvo.compute_association_gpu(src, tgt, T_gt, ANISO_KERNEL)
+    wall = (time.perf_counter() - t0) / steps
+    cvo.reset_kernel_stats()
+    cvo.set_profiling(True)
+    for _ in range(steps):
+        cvo.compute_association_gpu(src, tgt, T_gt, ANISO_KERNEL)
+    cvo.set_profiling(False)
+    st = cvo.kernel_stats()
+    n = max(1, st["launches"]["aniso"])
+    ms, by = st["ms"]["aniso"] / n, st["bytes"]["aniso"] / n
+    log(f"[bench] non-isotropic {wall * 1e3:.2f} ms per call, kernel {ms:.3f} ms, {a.nnz} pairs")
+    out = {"calls_per_s": 1.0 / wall, "ms_per_call": wall * 1e3, "kernel_ms": ms, "pairs": int(a.nnz),
+           "note": "a call uploads both clouds, builds the target grid, associates, compacts and copies the "
+                   "CSR back; kernel_ms is the association kernel alone (HIP events)",
+           "roofline": {"bound": "hbm", "achieved": by / (ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": by / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "kernel": "aniso_assoc_kernel",
+                        "avg_launch_ms": ms, "algorithmic_bytes_per_launch": by, "launches": n,
+                        "traffic": None}}
+    if not args.no_cpu and not args.profile_only:
+        from oracle import pyoracle
+        log("[bench] non-isotropic cpu baseline: one whole oracle association")
+        t0 = time.perf_counter()
+        pyoracle.assoc_nonisotropic(params, src, tgt, T_gt, ANISO_KERNEL)
+        spent = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": 1.0 / spent, "unit": "associations/s", "cores": cpu_threads(),
+                               "kind": "port", "sample": ("one whole oracle association (every target per row, "
+                                                          f"OpenMP), {spent:.1f} s of CPU work"),
+                               "seconds_per_association": spent}
+    cvo.close()
+    return out
+
+
 def assoc_probe(args):
     """Association passes of the 60k street pair at two lengthscales for each
     (lanes, XCD order) at cell scale 1, six passes each, for per-dispatch
@@ -474,7 +676,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--points", type=int, default=60000)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--legs", default="twoview,c1,ba,multistart,learned")
+    ap.add_argument("--legs", default="twoview,c1,ba,multistart,learned,knn,adaptive,aniso")
     ap.add_argument("--learned-pairs", type=int, default=4, help="pairs for learned vs grid registration")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
     ap.add_argument("--tune", choices=("auto", "default"), default="auto",
@@ -519,6 +721,12 @@ def main():
         extra["multistart"] = leg_multistart(args)
     if "learned" in legs:
         extra["learned"] = leg_learned(args)
+    if "knn" in legs:
+        extra["knn_street60k"] = leg_knn(args)
+    if "adaptive" in legs:
+        extra["adaptive_planes5k"] = leg_adaptive(args)
+    if "aniso" in legs:
+        extra["nonisotropic_street60k"] = leg_aniso(args)
     cpu = tv.get("cpu_baseline")
     out = {
         "metric": METRIC,

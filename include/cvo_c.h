@@ -156,6 +156,7 @@ enum {
   CVO_KSTAT_BA_PAIR = 4,   /* BA pair weights / normal equations            */
   CVO_KSTAT_BA_SOLVE = 5,  /* BA FP64 solve                                 */
   CVO_KSTAT_MLP = 6,       /* learned proposer forward (bytes[] holds flops) */
+  CVO_KSTAT_ANISO = 7,     /* non-isotropic association                     */
   CVO_KSTAT_COUNT = 8
 };
 typedef struct {

@@ -131,7 +131,7 @@ class KernelStats(C.Structure):
     ]
 
 
-KSTAT_NAMES = ["assoc", "overflow", "step", "grid", "ba_pair", "ba_solve", "mlp", "spare"]
+KSTAT_NAMES = ["assoc", "overflow", "step", "grid", "ba_pair", "ba_solve", "mlp", "aniso"]
 
 
 class Tuning(C.Structure):

@@ -803,6 +803,21 @@ static void account_bytes(cvo_ctx* ctx, const cvo_dcloud* s, const cvo_dcloud* t
   }
 }
 
+// The adaptive-lengthscale association covers three problems (x against
+// the moved y, x against x, y against y): every point's data three times per
+// launch; the stored pairs counted are the registration problem's (the self
+// problems' counts are not traced), so the figure is a lower bound.
+static void account_bytes_adaptive(cvo_ctx* ctx, const cvo_dcloud* x, const cvo_dcloud* y, const KConsts& kc,
+                                   const std::vector<cvo_iter_record>& tr, int n_it) {
+  const double per_pt = 12.0 + (kc.use_col ? 4.0 * kc.F : 0.0) + (kc.use_sem ? 4.0 * kc.Ks : 0.0);
+  for (int i = 0; i < n_it && i < (int)tr.size(); ++i) {
+    const double S = tr[i].nnz;
+    ctx->stats.bytes[CVO_KSTAT_ASSOC] += per_pt * 3.0 * (x->n + y->n) + 8.0 * S + 4.0 * (2.0 * x->n + y->n);
+    ctx->stats.bytes[CVO_KSTAT_STEP] += 8.0 * S + 4.0 * x->n + 12.0 * (x->n + y->n);
+    ctx->stats.pairs_stored += S;
+  }
+}
+
 extern "C" cvo_status cvo_align_d(cvo_ctx* ctx, const cvo_dcloud* src, const cvo_dcloud* tgt, const float T_init[16],
                                   float T_out[16], cvo_assoc* assoc_out, double* seconds, int32_t* iters,
                                   int32_t* ret_code) {
@@ -1285,6 +1300,7 @@ extern "C" cvo_status cvo_align_adaptive_d(cvo_ctx* ctx, const cvo_dcloud* src, 
     if (n_rec > 0)
       CVO_HIP(ctx, hipMemcpy(ctx->trace.data(), pr.trace, sizeof(cvo_iter_record) * n_rec, hipMemcpyDeviceToHost));
     ctx->trace_n = n_rec;
+    if (ctx->profiling) account_bytes_adaptive(ctx, src, tgt, kc, ctx->trace, n_rec);
   }
   return CVO_OK;
 }

@@ -131,4 +131,25 @@ __device__ inline int choose_level(const GridDev* grid, float r_need) {
   return L;
 }
 
+// Ascending bitonic sort of key[0, n2) in LDS by the threads of one block
+// (n2 a power of two; pad with ~0).  Every thread of the block calls it.
+__device__ inline void block_bitonic_sort(uint64_t* key, uint32_t n2) {
+  for (uint32_t k = 2; k <= n2; k <<= 1) {
+    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+      for (uint32_t i = threadIdx.x; i < n2; i += blockDim.x) {
+        const uint32_t ixj = i ^ j;
+        if (ixj > i) {
+          const bool up = (i & k) == 0;
+          const uint64_t a = key[i], b = key[ixj];
+          if ((a > b) == up) {
+            key[i] = b;
+            key[ixj] = a;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
 }  // namespace cvo

@@ -6,9 +6,9 @@
 //                                   (first K by target index) + fused flow
 //                                   (fill_in_A_mat_gpu CvoGPU.cu:951-1099 and
 //                                   compute_flow_gpu_no_eigen :2001-2062)
-//   overflow_kernel               : the pass totals, plus the exact
-//                                   index-ordered scan of rows whose passing
-//                                   pairs exceed the cap K
+//   overflow_kernel               : the pass totals, plus the rows whose
+//                                   passing pairs exceed the cap K (their K
+//                                   smallest passing target indices)
 //   step_kernel                   : Taylor step terms (:333-462); its last
 //                                   block runs the device controller
 //                                   (:464-544, :1289-1433)
@@ -24,6 +24,7 @@
 
 #include "cvo_internal.h"
 #include "cvo_device.h"
+#include "cvo_ball.h"
 
 namespace cvo {
 
@@ -91,13 +92,6 @@ __device__ __forceinline__ float geo_similarity(const float* ga, const float* gb
 __device__ __forceinline__ float sq_dist(float x0, float x1, float x2, float y0, float y1, float y2) {
   const float dx = y0 - x0, dy = y1 - x1, dz = y2 - x2;
   return fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-}
-
-// floor(v) clamped to [-1, n] in float before the conversion, so extreme or
-// non-finite query coordinates cannot overflow the int cell index (NaN -> -1).
-__device__ __forceinline__ int cell_floor(float v, int n) {
-  v = fminf(fmaxf(v, -1.0f), (float)n);
-  return (int)floorf(v);
 }
 
 // Locate the problem of a flat block index.
@@ -437,83 +431,6 @@ __device__ __forceinline__ void assoc_row(const AssocProblem& P, const KConsts& 
 // pass applies the gates to the selected targets and stores the passing ones.
 // ---------------------------------------------------------------------------
 
-// Orders this wave's LDS accesses before and after it (no other wave shares a
-// row's histogram).
-__device__ __forceinline__ void wave_lds_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-// Query box of a row in a grid level: the cell runs (x-rows of cells) it spans.
-struct CellBox {
-  int lo0, lo1, lo2, nx_run, span1, nruns;
-};
-
-__device__ __forceinline__ CellBox cell_box(const GridLevel& GL, float q0, float q1, float q2, float rq) {
-  CellBox B{0, 0, 0, 0, 0, 0};
-  int lo0 = cell_floor((q0 - rq - GL.ox) * GL.inv_h, GL.nx), hi0 = cell_floor((q0 + rq - GL.ox) * GL.inv_h, GL.nx);
-  int lo1 = cell_floor((q1 - rq - GL.oy) * GL.inv_h, GL.ny), hi1 = cell_floor((q1 + rq - GL.oy) * GL.inv_h, GL.ny);
-  int lo2 = cell_floor((q2 - rq - GL.oz) * GL.inv_h, GL.nz), hi2 = cell_floor((q2 + rq - GL.oz) * GL.inv_h, GL.nz);
-  if (hi0 < 0 || lo0 >= GL.nx || hi1 < 0 || lo1 >= GL.ny || hi2 < 0 || lo2 >= GL.nz) return B;
-  lo0 = max(lo0, 0); hi0 = min(hi0, GL.nx - 1);
-  lo1 = max(lo1, 0); hi1 = min(hi1, GL.ny - 1);
-  lo2 = max(lo2, 0); hi2 = min(hi2, GL.nz - 1);
-  B.lo0 = lo0;
-  B.lo1 = lo1;
-  B.lo2 = lo2;
-  B.nx_run = hi0 - lo0 + 1;
-  B.span1 = hi1 - lo1 + 1;
-  B.nruns = B.span1 * (hi2 - lo2 + 1);
-  return B;
-}
-
-// Calls fn(valid, sorted position) for every target of the box, G at a time:
-// all G lanes of the group call together (valid false past the end), so fn
-// may use group-wide ballots and shuffles.
-template <int G, class Fn>
-__device__ __forceinline__ void for_each_in_box(const GridLevel& GL, const CellBox& B, int g, Fn&& fn) {
-  for (int rbase = 0; rbase < B.nruns; rbase += G) {
-    const int myrun = rbase + g;
-    uint32_t rs = 0, rl = 0;
-    if (myrun < B.nruns) {
-      const int iz = myrun / B.span1, iy = myrun - iz * B.span1;
-      const uint32_t c0 =
-          ((uint32_t)(B.lo2 + iz) * (uint32_t)GL.ny + (uint32_t)(B.lo1 + iy)) * (uint32_t)GL.nx + (uint32_t)B.lo0;
-      rs = GL.cell_start[c0];
-      rl = GL.cell_start[c0 + (uint32_t)B.nx_run] - rs;
-    }
-    uint32_t incl = rl;
-#pragma unroll
-    for (int o = 1; o < G; o <<= 1) {
-      uint32_t t = __shfl_up(incl, o, G);
-      if (g >= o) incl += t;
-    }
-    const uint32_t total = __shfl(incl, G - 1, G);
-    const uint32_t excl = incl - rl;
-    for (uint32_t base = 0; base < total; base += G) {
-      const uint32_t idx = base + (uint32_t)g;
-      int kk = 0;
-#pragma unroll
-      for (int s = G / 2; s > 0; s >>= 1) {
-        uint32_t e = __shfl(excl, kk + s, G);
-        if (e <= idx) kk += s;
-      }
-      const uint32_t run_start = __shfl(rs, kk, G);
-      const uint32_t run_excl = __shfl(excl, kk, G);
-      const bool valid = idx < total;
-      fn(valid, valid ? run_start + (idx - run_excl) : 0u);
-    }
-  }
-}
-
-// The group's bits of a wave ballot.
-template <int G>
-__device__ __forceinline__ uint32_t group_bits(uint64_t m) {
-  constexpr uint32_t kGroupMask = (uint32_t)((1ull << G) - 1ull);
-  return (uint32_t)(m >> (G * ((threadIdx.x & 63) / G))) & kGroupMask;
-}
-
 template <int FC, int KC, int G>
 __device__ __forceinline__ void knn_row(const AssocProblem& P, const KConsts& kc, const ExactK& ek,
                                         const GridLevel& GL, const Maps& mp, int r, int g, int mode, float ell, int K,
@@ -555,103 +472,26 @@ __device__ __forceinline__ void knn_row(const AssocProblem& P, const KConsts& kc
     if (!(d2 < d2t)) return ~0ull;
     return ((uint64_t)__float_as_uint(d2) << 32) | (uint64_t)__float_as_uint(y4.w);
   };
-  // (1) ball size and key range
-  uint32_t cnt = 0;
-  uint64_t kmin = ~0ull, kmax = 0;
-  for_each_in_box<G>(GL, B, g, [&](bool valid, uint32_t pidx) {
-    if (!valid) return;
-    float4 y4;
-    float y0, y1, y2;
-    const uint64_t key = key_of(pidx, y4, y0, y1, y2);
-    if (key == ~0ull) return;
-    cnt++;
-    kmin = min(kmin, key);
-    kmax = max(kmax, key);
-  });
-#pragma unroll
-  for (int o = G / 2; o > 0; o >>= 1) {
-    cnt += __shfl_xor(cnt, o, G);
-    kmin = min(kmin, (uint64_t)__shfl_xor((unsigned long long)kmin, o, G));
-    kmax = max(kmax, (uint64_t)__shfl_xor((unsigned long long)kmax, o, G));
-  }
-  // (2) selection: a target is taken when (key >> tsh) <= (thr >> tsh)
-  uint64_t thr = ~0ull;
-  int tsh = 0;
-  if (cnt > Ku) {
-    // keys are distinct (target ids), so kmin != kmax and plen < 64
-    uint32_t need = Ku;
-    int plen = __clzll((long long)(kmin ^ kmax));
-    uint64_t prefix = plen > 0 ? (kmin & ~(~0ull >> plen)) : 0ull;
-    constexpr int CH = 256 / G;
-    for (int pass = 0; pass < 8 && plen < 64; ++pass) {
-      const int db = min(8, 64 - plen);
-      const int sh = 64 - plen - db;
-#pragma unroll
-      for (int i = 0; i < CH; ++i) hist[g * CH + i] = 0u;
-      wave_lds_sync();
-      for_each_in_box<G>(GL, B, g, [&](bool valid, uint32_t pidx) {
-        if (!valid) return;
+  // (1)-(2) the K nearest of the ball: thr / tsh bound their keys
+  uint64_t thr;
+  int tsh;
+  select_smallest<G>(
+      [&](auto&& fn) { for_each_in_box<G>(GL, B, g, fn); }, g, Ku, hist,
+      [&](uint32_t pidx) {
         float4 y4;
         float y0, y1, y2;
-        const uint64_t key = key_of(pidx, y4, y0, y1, y2);
-        if (key == ~0ull) return;
-        if (plen > 0 && (key >> (64 - plen)) != (prefix >> (64 - plen))) return;
-        atomicAdd(&hist[(uint32_t)(key >> sh) & 0xffu], 1u);
-      });
-      wave_lds_sync();
-      // the bucket holding the need-th key: chunk sums, group prefix, then
-      // a sequential scan inside the one chunk that crosses need
-      uint32_t csum = 0;
-#pragma unroll
-      for (int i = 0; i < CH; ++i) csum += hist[g * CH + i];
-      uint32_t incl = csum;
-#pragma unroll
-      for (int o = 1; o < G; o <<= 1) {
-        uint32_t t = __shfl_up(incl, o, G);
-        if (g >= o) incl += t;
-      }
-      const uint32_t excl = incl - csum;
-      const bool mine = excl < need && incl >= need;
-      uint32_t b = 0, less = 0, cb = 0;
-      if (mine) {
-        uint32_t run = excl;
-        for (int i = 0; i < CH; ++i) {
-          const uint32_t h = hist[g * CH + i];
-          if (run + h >= need) {
-            b = (uint32_t)(g * CH + i);
-            less = run;
-            cb = h;
-            break;
-          }
-          run += h;
-        }
-      }
-      const uint32_t gm = group_bits<G>(__ballot(mine));
-      wave_lds_sync();  // histogram reads done before the next pass clears it
-      if (gm == 0u) break;  // cannot happen while the counts are consistent
-      const int owner = __ffs(gm) - 1;
-      b = __shfl(b, owner, G);
-      less = __shfl(less, owner, G);
-      cb = __shfl(cb, owner, G);
-      prefix |= (uint64_t)b << sh;
-      thr = prefix;
-      tsh = sh;
-      if (less + cb == need) break;
-      // continue inside bucket b; thr stays its upper bound meanwhile
-      need -= less;
-      plen += db;
-    }
-  }
+        return key_of(pidx, y4, y0, y1, y2);
+      },
+      thr, tsh);
   // (3) gates on the selected targets, stored in slots by ballot
   uint32_t stored = 0;
-  const uint64_t thr_s = thr >> tsh;
   for_each_in_box<G>(GL, B, g, [&](bool valid, uint32_t pidx) {
     bool pass = false;
     float a = 0.0f, y0 = 0.0f, y1 = 0.0f, y2 = 0.0f;
     float4 y4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (valid) {
       const uint64_t key = key_of(pidx, y4, y0, y1, y2);
-      if (key != ~0ull && (key >> tsh) <= thr_s)
+      if (selected(key, thr, tsh))
         pass = pair_kernel<FC, KC>(kc, ek, x0, x1, x2, y0, y1, y2, d2t, qg, l, xf,
                                    FC != 0 ? GL.feat + (size_t)pidx * F : nullptr, xs,
                                    KC != 0 ? GL.sem + (size_t)pidx * KS : nullptr, xg,
@@ -854,8 +694,8 @@ __global__ __launch_bounds__(kBlock) void knn_assoc_kernel(const AssocProblem* _
 
 // ---------------------------------------------------------------------------
 // Totals of an association pass, and the rows whose passing pairs exceed the
-// cap: the reference's exact scan over all targets in ascending index,
-// keeping the first K (CvoGPU.cu:994-996).  Each block sums a slice of the
+// cap, which keep their first K passing targets by index (the reference's
+// scan over all targets in ascending index, CvoGPU.cu:994-996).  Each block sums a slice of the
 // association block partials, adds its capped rows and publishes
 // part_ovf[block] = flow6, ip, stored, max row; the last block writes the
 // problem's totals (raw flow sums, ip, stored pairs, longest row).
@@ -951,97 +791,24 @@ __device__ uint32_t capped_row_ball(const AssocProblem& P, const KConsts& kc, co
                                           kc.use_gtype ? GL.geo + 2 * (size_t)pidx : nullptr, a);
     return pass ? (uint64_t)__float_as_uint(y4.w) : ~0ull;
   };
-  uint32_t cnt = 0;
-  uint64_t kmin = ~0ull, kmax = 0;
-  for_each_in_box<64>(GL, B, g, [&](bool valid, uint32_t pidx) {
-    if (!valid) return;
-    float4 y4;
-    float y0, y1, y2, a;
-    const uint64_t key = eval(pidx, y4, y0, y1, y2, a);
-    if (key == ~0ull) return;
-    cnt++;
-    kmin = min(kmin, key);
-    kmax = max(kmax, key);
-  });
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    cnt += __shfl_xor(cnt, o, 64);
-    kmin = min(kmin, (uint64_t)__shfl_xor((unsigned long long)kmin, o, 64));
-    kmax = max(kmax, (uint64_t)__shfl_xor((unsigned long long)kmax, o, 64));
-  }
-  uint64_t thr = ~0ull;
-  int tsh = 0;
-  if (cnt > K && K > 0) {
-    // indices are distinct, so kmin != kmax and plen < 64
-    uint32_t need = K;
-    int plen = __clzll((long long)(kmin ^ kmax));
-    uint64_t prefix = plen > 0 ? (kmin & ~(~0ull >> plen)) : 0ull;
-    constexpr int CH = 256 / 64;
-    for (int pass = 0; pass < 8 && plen < 64; ++pass) {
-      const int db = min(8, 64 - plen);
-      const int sh = 64 - plen - db;
-#pragma unroll
-      for (int i = 0; i < CH; ++i) hist[g * CH + i] = 0u;
-      wave_lds_sync();
-      for_each_in_box<64>(GL, B, g, [&](bool valid, uint32_t pidx) {
-        if (!valid) return;
+  uint64_t thr;
+  int tsh;
+  select_smallest<64>(
+      [&](auto&& fn) { for_each_in_box<64>(GL, B, g, fn); }, g, K, hist,
+      [&](uint32_t pidx) {
         float4 y4;
         float y0, y1, y2, a;
-        const uint64_t key = eval(pidx, y4, y0, y1, y2, a);
-        if (key == ~0ull) return;
-        if (plen > 0 && (key >> (64 - plen)) != (prefix >> (64 - plen))) return;
-        atomicAdd(&hist[(uint32_t)(key >> sh) & 0xffu], 1u);
-      });
-      wave_lds_sync();
-      uint32_t csum = 0;
-#pragma unroll
-      for (int i = 0; i < CH; ++i) csum += hist[g * CH + i];
-      uint32_t incl = csum;
-#pragma unroll
-      for (int o = 1; o < 64; o <<= 1) {
-        uint32_t t = __shfl_up(incl, o, 64);
-        if (g >= o) incl += t;
-      }
-      const uint32_t excl = incl - csum;
-      const bool mine = excl < need && incl >= need;
-      uint32_t b = 0, less = 0, cb = 0;
-      if (mine) {
-        uint32_t run = excl;
-        for (int i = 0; i < CH; ++i) {
-          const uint32_t h = hist[g * CH + i];
-          if (run + h >= need) {
-            b = (uint32_t)(g * CH + i);
-            less = run;
-            cb = h;
-            break;
-          }
-          run += h;
-        }
-      }
-      const uint64_t gm = __ballot(mine);
-      wave_lds_sync();
-      if (gm == 0ull) break;  // cannot happen while the counts are consistent
-      const int owner = __ffsll((unsigned long long)gm) - 1;
-      b = __shfl(b, owner, 64);
-      less = __shfl(less, owner, 64);
-      cb = __shfl(cb, owner, 64);
-      prefix |= (uint64_t)b << sh;
-      thr = prefix;
-      tsh = sh;
-      if (less + cb == need) break;
-      need -= less;
-      plen += db;
-    }
-  }
+        return eval(pidx, y4, y0, y1, y2, a);
+      },
+      thr, tsh);
   uint32_t stored = 0;
-  const uint64_t thr_s = thr >> tsh;
   for_each_in_box<64>(GL, B, g, [&](bool valid, uint32_t pidx) {
     bool take = false;
     float a = 0.0f, y0 = 0.0f, y1 = 0.0f, y2 = 0.0f;
     float4 y4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (valid) {
       const uint64_t key = eval(pidx, y4, y0, y1, y2, a);
-      take = key != ~0ull && (key >> tsh) <= thr_s;
+      take = selected(key, thr, tsh);
     }
     const uint64_t m = __ballot(take);
     const uint32_t slot = stored + (uint32_t)__popcll(m & ((g == 0) ? 0ull : (~0ull >> (64 - g))));
@@ -1932,22 +1699,7 @@ __global__ __launch_bounds__(64) void export_rows(const AssocProblem* __restrict
     s_key[s] = key;
   }
   __syncthreads();
-  for (uint32_t k = 2; k <= n2; k <<= 1) {
-    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-      for (uint32_t i = threadIdx.x; i < n2; i += 64) {
-        uint32_t ixj = i ^ j;
-        if (ixj > i) {
-          bool up = (i & k) == 0;
-          const uint64_t a = s_key[i], b = s_key[ixj];
-          if ((a > b) == up) {
-            s_key[i] = b;
-            s_key[ixj] = a;
-          }
-        }
-      }
-      __syncthreads();
-    }
-  }
+  block_bitonic_sort(s_key, n2);
   const int64_t base = row_ptr[orig];
   for (uint32_t s = threadIdx.x; s < cnt; s += 64) {
     const uint64_t key = s_key[s];
