@@ -642,7 +642,7 @@ __device__ __forceinline__ int active_blocks(const AssocProblem& P, int group) {
 // at the largest lane count) and uses a prefix of them at its current lane
 // count.  With xcd_remap the active blocks are renumbered so that each
 // XCD holds one contiguous range of rows.
-// Four waves per SIMD: the register budget is 128 VGPRs.
+// Register budget: see assoc_kernel.
 template <int FC, int KC, bool KNN>
 __device__ __forceinline__ void assoc_pass(const AssocProblem& P, const KConsts& kc) {
   const ProblemState* st = P.st;
@@ -661,7 +661,7 @@ __device__ __forceinline__ void assoc_pass(const AssocProblem& P, const KConsts&
     }
   } else {
     switch (group) {
-      // two lane counts keep the kernel within 128 VGPRs (four waves per SIMD)
+      // two lane counts keep the kernel within its register budget
       case 8: assoc_block<FC, KC, 8, false>(P, kc, lb, L, nullptr); break;
       default: assoc_block<FC, KC, 16, false>(P, kc, lb, L, nullptr); break;
     }
@@ -670,8 +670,12 @@ __device__ __forceinline__ void assoc_pass(const AssocProblem& P, const KConsts&
 
 // ONE: a single problem, its descriptor by value in the kernel arguments (read
 // with scalar loads, no dependent load before the first state read).
+// Occupancy: a single problem runs at five waves per SIMD (96 VGPRs, a
+// 16-byte spill; the 60k street registration 128.3 -> 125.0 ms, six waves
+// spill 96 bytes and take 157 ms); several problems at four (128 VGPRs: at
+// five the batched variant spills 64 bytes and the 96-start run slows 8%).
 template <int FC, int KC, bool ONE>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void assoc_kernel(
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ONE ? 5 : 4))) void assoc_kernel(
     const AssocProblem* __restrict__ probs, int n_probs, KConsts kc, AssocProblem one) {
   if constexpr (ONE) {
     assoc_pass<FC, KC, false>(one, kc);
