@@ -369,9 +369,11 @@ cvo_status cvo_set_pose_arithmetic(cvo_ctx* ctx, int32_t mode);
  * source row over every target, CvoGPU.cu:951-1099); here the grid search has
  * knobs, none of which changes the pair set, the pair values or the row cap
  * semantics (only the order a row's pairs are summed in, at FP64 rounding):
- *   lanes_per_row    8 or 16 lanes of a wave share one source row, at
+ *   lanes_per_row    4, 8 or 16 lanes of a wave share one source row, at
  *                    lengthscales >= split_ell (every lengthscale when
- *                    split_ell <= 0);
+ *                    split_ell <= 0); launches over several problems
+ *                    (multi-start, adaptive, BA) and the K-nearest pass use
+ *                    at least 8;
  *   cell_scale       grid cells are chosen with edge >= cell_scale x the
  *                    ell-ball radius, in [0.4, 1] (smaller: more cells per
  *                    query box, fewer candidates outside the ball), at

@@ -314,20 +314,27 @@ std::vector<float> level_sizes(const KConsts& kc, float ell_hi, float ell_lo, fl
   return hs;
 }
 
-// An association launch shape (tuning) as kernel constants.
-void apply_tuning(const cvo_tuning& t, KConsts& kc) {
+// An association launch shape (tuning) as kernel constants.  batched: the
+// launches carry several problems (their kernel has no four-lane path).
+void apply_tuning(const cvo_tuning& t, KConsts& kc, bool batched) {
   const bool split = t.split_ell > 0.0f;
   kc.pol.split_ell = split ? t.split_ell : 0.0f;
   kc.pol.group_big = t.lanes_per_row;
   kc.pol.group_small = split ? t.lanes_small_ell : t.lanes_per_row;
   kc.pol.scale_big = t.cell_scale;
   kc.pol.scale_small = split ? t.cell_scale_small_ell : t.cell_scale;
+  if (kc.knn > 0 || batched) {
+    // the K-nearest pass keeps one 256-bin histogram per row, a batched
+    // launch has no four-lane path: 8 or 16 lanes
+    kc.pol.group_big = std::max(kc.pol.group_big, 8);
+    kc.pol.group_small = std::max(kc.pol.group_small, 8);
+  }
   kc.max_group = std::max(kc.pol.group_big, kc.pol.group_small);
   kc.min_scale = std::min(kc.pol.scale_big, kc.pol.scale_small);
   kc.xcd_remap = t.xcd_remap;
 }
 
-void apply_tuning(const cvo_ctx* ctx, KConsts& kc) { apply_tuning(ctx->tuning, kc); }
+void apply_tuning(const cvo_ctx* ctx, KConsts& kc, bool batched) { apply_tuning(ctx->tuning, kc, batched); }
 
 // Buffers and descriptors of n_probs problems sharing one source and target;
 // each problem owns the association blocks of its rows at max_group lanes per
@@ -1057,7 +1064,7 @@ extern "C" cvo_status cvo_align_multistart(cvo_ctx* ctx, const cvo_dcloud* src, 
   hipEvent_t e0 = ctx->ev_pool[4], e1 = ctx->ev_pool[5];
   CVO_HIP(ctx, hipEventRecord(e0, ctx->stream));
   KConsts kc = make_kconsts(p, src->F, src->K, MODE_ALIGN);
-  apply_tuning(ctx, kc);
+  apply_tuning(ctx, kc, true);  // batched launches
   kc.mirror_pose = ctx->pose_arith == CVO_POSE_REFERENCE ? 1 : 0;
   kc.n_src_f = (float)src->n;
   kc.n_tgt_f = (float)tgt->n;
@@ -1190,7 +1197,7 @@ extern "C" cvo_status cvo_align_adaptive_d(cvo_ctx* ctx, const cvo_dcloud* src, 
   KConsts kc = adaptive_kconsts(p, src->F, p.is_using_semantics ? src->K : 0);
   if (!p.is_using_semantics) kc.use_sem = 0;
   kc.amp = kc.sigma2 * kc.c_sigma2 * (kc.use_sem ? kc.s_sigma2 : 1.0f);
-  apply_tuning(ctx, kc);
+  apply_tuning(ctx, kc, true);  // batched launches
   kc.mirror_pose = ctx->pose_arith == CVO_POSE_REFERENCE ? 1 : 0;
   kc.n_src_f = (float)src->n;
   kc.n_tgt_f = (float)tgt->n;

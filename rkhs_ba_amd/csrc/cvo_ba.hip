@@ -758,7 +758,7 @@ extern "C" cvo_status cvo_align_multiframe(cvo_ctx* ctx, int32_t n_frames, const
   hipStream_t st = ctx->stream;
   const auto t_start = std::chrono::steady_clock::now();
   KConsts kc = make_kconsts(p, frames[0]->F, frames[0]->K, MODE_BA);
-  apply_tuning(ctx, kc);
+  apply_tuning(ctx, kc, true);  // one batched launch over the edges
   const int rb = rows_per_block(kc.max_group);
 
   // frames that are sources / targets of some edge

@@ -119,7 +119,7 @@ struct AssocProblem {
 // and grid cell scale above / below split_ell (one setting when split_ell <= 0).
 struct EllPolicy {
   float split_ell;
-  int32_t group_big, group_small;  // 8 or 16
+  int32_t group_big, group_small;  // 4, 8 or 16 (8 or 16 for the K-nearest pass)
   float scale_big, scale_small;    // level choice: cell edge >= scale * radius
 };
 
@@ -168,7 +168,8 @@ struct KConsts {
 
 // Rows per association block for a lane count per row (8 or 16).
 __host__ __device__ constexpr int rows_per_block(int group) { return kBlock / group; }
-constexpr int kMaxRowsPerBlock = kBlock / 8;
+constexpr int kMaxRowsPerBlock = kBlock / 4;  // 4, 8 or 16 lanes per row
+constexpr int kKnnRowsPerBlock = kBlock / 8;  // the K-nearest pass: 8 or 16 lanes per row
 
 enum Mode { MODE_ALIGN = 0, MODE_IP = 1, MODE_BA = 2 };
 
@@ -245,8 +246,8 @@ cvo_status validate_pair(cvo_ctx* ctx, const cvo_dcloud* s, const cvo_dcloud* t)
 float radius_for(const KConsts& kc, float ell, float max_norm, float tq);
 std::vector<float> level_sizes(const KConsts& kc, float ell_hi, float ell_lo, float max_norm, float tq,
                                float cell_scale);
-void apply_tuning(const cvo_ctx* ctx, KConsts& kc);
-void apply_tuning(const cvo_tuning& t, KConsts& kc);
+void apply_tuning(const cvo_ctx* ctx, KConsts& kc, bool batched = false);
+void apply_tuning(const cvo_tuning& t, KConsts& kc, bool batched = false);
 cvo_status ensure_events(cvo_ctx* ctx, size_t n);
 cvo_status pinned(cvo_ctx* ctx, size_t bytes);
 cvo_status prepare_source_into(cvo_ctx* ctx, const cvo_dcloud* s, DevBuf& b_pos, DevBuf& b_feat, DevBuf& b_sem,

@@ -630,7 +630,9 @@ __device__ __forceinline__ void tree10(double (*red)[W]) {
 }
 
 // Lanes per row of a problem's next pass (8 or 16).
-__device__ __forceinline__ int state_group(const ProblemState* st) { return st->group == 8 ? 8 : 16; }
+__device__ __forceinline__ int state_group(const ProblemState* st) {
+  return st->group == 4 ? 4 : (st->group == 8 ? 8 : 16);
+}
 
 // Active association blocks of a problem at a lane count.
 __device__ __forceinline__ int active_blocks(const AssocProblem& P, int group) {
@@ -643,7 +645,10 @@ __device__ __forceinline__ int active_blocks(const AssocProblem& P, int group) {
 // count.  With xcd_remap the active blocks are renumbered so that each
 // XCD holds one contiguous range of rows.
 // Register budget: see assoc_kernel.
-template <int FC, int KC, bool KNN>
+// FOUR: the four-lanes-per-row path is compiled in (single-problem kernels
+// only: it raises the batched variant's register count past its budget; the
+// host never selects four lanes for a batched launch).
+template <int FC, int KC, bool KNN, bool FOUR>
 __device__ __forceinline__ void assoc_pass(const AssocProblem& P, const KConsts& kc) {
   const ProblemState* st = P.st;
   if (st->done) return;
@@ -654,7 +659,7 @@ __device__ __forceinline__ void assoc_pass(const AssocProblem& P, const KConsts&
   if (kc.xcd_remap) lb = xcd_block(lb, n_act);
   __shared__ AssocLds<FC, KC> L;
   if constexpr (KNN) {
-    __shared__ uint32_t hist[kMaxRowsPerBlock * 256];
+    __shared__ uint32_t hist[kKnnRowsPerBlock * 256];
     switch (group) {
       case 8: assoc_block<FC, KC, 8, true>(P, kc, lb, L, hist); break;
       default: assoc_block<FC, KC, 16, true>(P, kc, lb, L, hist); break;
@@ -662,6 +667,9 @@ __device__ __forceinline__ void assoc_pass(const AssocProblem& P, const KConsts&
   } else {
     switch (group) {
       // two lane counts keep the kernel within its register budget
+      case 4:
+        if constexpr (FOUR) assoc_block<FC, KC, 4, false>(P, kc, lb, L, nullptr);
+        break;
       case 8: assoc_block<FC, KC, 8, false>(P, kc, lb, L, nullptr); break;
       default: assoc_block<FC, KC, 16, false>(P, kc, lb, L, nullptr); break;
     }
@@ -678,9 +686,9 @@ template <int FC, int KC, bool ONE>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ONE ? 5 : 4))) void assoc_kernel(
     const AssocProblem* __restrict__ probs, int n_probs, KConsts kc, AssocProblem one) {
   if constexpr (ONE) {
-    assoc_pass<FC, KC, false>(one, kc);
+    assoc_pass<FC, KC, false, true>(one, kc);
   } else {
-    assoc_pass<FC, KC, false>(probs[find_problem(probs, n_probs, (int)blockIdx.x)], kc);
+    assoc_pass<FC, KC, false, false>(probs[find_problem(probs, n_probs, (int)blockIdx.x)], kc);
   }
 }
 
@@ -690,9 +698,9 @@ template <int FC, int KC, bool ONE>
 __global__ __launch_bounds__(kBlock) void knn_assoc_kernel(const AssocProblem* __restrict__ probs, int n_probs,
                                                            KConsts kc, AssocProblem one) {
   if constexpr (ONE) {
-    assoc_pass<FC, KC, true>(one, kc);
+    assoc_pass<FC, KC, true, false>(one, kc);
   } else {
-    assoc_pass<FC, KC, true>(probs[find_problem(probs, n_probs, (int)blockIdx.x)], kc);
+    assoc_pass<FC, KC, true, false>(probs[find_problem(probs, n_probs, (int)blockIdx.x)], kc);
   }
 }
 

@@ -512,7 +512,7 @@ class CvoGPU:
     def set_tuning(self, lanes_per_row: int = 16, xcd_remap: int = 1, cell_scale: float = 1.0,
                    lanes_small_ell: Optional[int] = None, split_ell: float = 0.0,
                    cell_scale_small_ell: Optional[float] = None) -> None:
-        """Association launch shape: lanes_per_row (8 or 16) and cell_scale at
+        """Association launch shape: lanes_per_row (4, 8 or 16) and cell_scale at
         lengthscales >= split_ell, lanes_small_ell / cell_scale_small_ell below
         it (split_ell 0: one setting)."""
         t = _tuning(lanes_per_row, xcd_remap, cell_scale, lanes_small_ell, split_ell, cell_scale_small_ell)

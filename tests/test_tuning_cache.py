@@ -32,7 +32,7 @@ def test_round_trip_and_replace(tmp_path):
 
 def test_rejects_bad_values_and_keys(tmp_path):
     path = str(tmp_path / "tune.json")
-    for bad in [(4, 0, 1.0), (32, 0, 1.0), (16, 2, 1.0), (16, 0, 0.2), (8, 0, 1.2)]:
+    for bad in [(2, 0, 1.0), (32, 0, 1.0), (16, 2, 1.0), (16, 0, 0.2), (8, 0, 1.2)]:
         with pytest.raises(CvoError):
             tuning_cache_store(path, KEY_A, *bad)
     with pytest.raises(CvoError):
