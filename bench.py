@@ -43,13 +43,13 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per association launch on the 60k street workload, from rocprofv3
 # FETCH_SIZE / WRITE_SIZE passes scaled by a 256 MiB copy probe
 # (profiles/assoc_street60k_pmc_traffic.md)
-ASSOC_TRAFFIC_BYTES = 27.35e6
+ASSOC_TRAFFIC_BYTES = 26.71e6
 # per-launch work of the association kernel on the 60k street registration
 # with its autotuned launch shape, from rocprofv3 counters (SQ_INSTS_VALU_TRANS_F32
 # x 64 lanes; SQ_INSTS_VALU_FLOPS_FP32 x 64): the transcendental and FP32
 # roofs are checked against the live launch time
-ASSOC_EXP_PER_LAUNCH = 4.45e6
-ASSOC_FP32_FLOP_PER_LAUNCH = 2.89e8
+ASSOC_EXP_PER_LAUNCH = 3.96e6
+ASSOC_FP32_FLOP_PER_LAUNCH = 2.92e8
 EXP_PEAK_PER_S = 256 * 16 * 2.4e9   # quarter-rate v_exp_f32: 16 lanes per CU per clock
 FP32_PEAK_TFLOPS = 157.3
 CONFIGS = os.path.join(ROOT, "rkhs_ba_amd", "configs")
