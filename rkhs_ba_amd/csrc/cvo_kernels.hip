@@ -522,23 +522,27 @@ __device__ __forceinline__ void knn_row(const AssocProblem& P, const KConsts& kc
 // part_cnt[block] = stored pairs, max row; summed into the problem state by
 // the capped-row kernel that follows.
 
-// LDS of one association block, sized for the most rows a block can hold.
-template <int FC, int KC>
+// LDS of one association block, sized for the most rows a block of the
+// variant can hold (ROWS: 64 with the four-lane path, else 32; the K-nearest
+// variant's histograms leave no room for more).
+template <int FC, int KC, int ROWS>
 struct AssocLds {
   static constexpr int FL = FC > 0 ? FC : (FC == 0 ? 1 : 64);
   static constexpr int KL = KC > 0 ? KC : (KC == 0 ? 1 : 64);
-  double row[kMaxRowsPerBlock][8];
-  uint32_t cnt[kMaxRowsPerBlock];
-  float feat[kMaxRowsPerBlock * FL];  // the block's source rows' features
-  float sem[kMaxRowsPerBlock * KL];   // and class vectors
+  static constexpr int kRows = ROWS;
+  double row[ROWS][8];
+  uint32_t cnt[ROWS];
+  float feat[ROWS * FL];  // the block's source rows' features
+  float sem[ROWS * KL];   // and class vectors
 };
 
 // Logical block lb of a problem with G lanes per row (256 / G rows); KNN:
 // the K-nearest rows (hist: 256 bins per row).
-template <int FC, int KC, int G, bool KNN>
+template <int FC, int KC, int G, bool KNN, int ROWS>
 __device__ __forceinline__ void assoc_block(const AssocProblem& P, const KConsts& kc, int lb,
-                                            AssocLds<FC, KC>& L, uint32_t* hist) {
+                                            AssocLds<FC, KC, ROWS>& L, uint32_t* hist) {
   constexpr int RB = rows_per_block(G);
+  static_assert(RB <= ROWS, "the block's rows exceed its LDS");
   const ProblemState* st = P.st;
   const int tid = threadIdx.x;
   const int g = tid & (G - 1);
@@ -657,7 +661,7 @@ __device__ __forceinline__ void assoc_pass(const AssocProblem& P, const KConsts&
   int lb = (int)blockIdx.x - P.block_begin;
   if (lb >= n_act) return;  // uniform
   if (kc.xcd_remap) lb = xcd_block(lb, n_act);
-  __shared__ AssocLds<FC, KC> L;
+  __shared__ AssocLds<FC, KC, FOUR ? kMaxRowsPerBlock : kKnnRowsPerBlock> L;
   if constexpr (KNN) {
     __shared__ uint32_t hist[kKnnRowsPerBlock * 256];
     switch (group) {

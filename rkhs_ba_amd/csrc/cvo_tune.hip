@@ -217,11 +217,11 @@ extern "C" cvo_status cvo_tuning_key(const cvo_ctx* ctx, const cvo_dcloud* src, 
   const cvo_params& p = ctx->params;
   char tmp[512];
   const int n = std::snprintf(
-      tmp, sizeof(tmp), "%s/%dcu|src%.1f|tgt%.1f|F%d|K%d|geo%d|col%d|sem%d|gt%d|ell%.4g-%.4g|sp%.3g|nn%d",
+      tmp, sizeof(tmp), "%s/%dcu|src%.1f|tgt%.1f|F%d|K%d|geo%d|col%d|sem%d|gt%d|ell%.4g-%.4g|sp%.3g|nn%d|knn%d",
       arch.c_str(), prop.multiProcessorCount, size_class(src->n), size_class(tgt->n), src->F, src->K,
       p.is_using_geometry, p.is_using_intensity, p.is_using_semantics, src->has_geo && tgt->has_geo ? 1 : 0,
       (double)std::max(p.ell_init, p.ell_min), (double)std::min(p.ell_init, p.ell_min), (double)p.sp_thres,
-      p.nearest_neighbors_max);
+      p.nearest_neighbors_max, p.is_using_kdtree);
   if (n < 0 || n >= len) return CVO_ERR_LIMIT;
   std::memcpy(buf, tmp, (size_t)n + 1);
   return CVO_OK;

@@ -100,7 +100,13 @@ def test_autotune_measures_caches_and_applies(ctx, tmp_path):
     assert t["score_us"] > 0
     assert ctx.get_tuning()["lanes_per_row"] == t["lanes_per_row"]
     key = ctx.tuning_key(ds, dt)
-    assert key.startswith("gfx950") and "F5" in key and "K19" in key
+    assert key.startswith("gfx950") and "F5" in key and "K19" in key and key.endswith("|knn0")
+    # the K-nearest pass is another kernel: its own entry
+    p.is_using_kdtree = 1
+    ctx.write_params(p)
+    assert ctx.tuning_key(ds, dt) == key[:-1] + "1"
+    p.is_using_kdtree = 0
+    ctx.write_params(p)
     from rkhs_ba_amd.runtime import tuning_cache_lookup
     e = tuning_cache_lookup(cache, key)
     assert e is not None and e["cell_scale"] == t["cell_scale"]

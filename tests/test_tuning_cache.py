@@ -7,8 +7,8 @@ import pytest
 
 from rkhs_ba_amd.runtime import CvoError, tuning_cache_lookup, tuning_cache_store
 
-KEY_A = "gfx950/256cu|src16.0|tgt16.0|F5|K19|geo1|col1|sem1|gt0|ell0.5-0.05|sp0.001|nn512"
-KEY_B = "gfx950/256cu|src14.5|tgt14.5|F5|K0|geo1|col1|sem0|gt0|ell0.5-0.05|sp0.001|nn512"
+KEY_A = "gfx950/256cu|src16.0|tgt16.0|F5|K19|geo1|col1|sem1|gt0|ell0.5-0.05|sp0.001|nn512|knn0"
+KEY_B = "gfx950/256cu|src14.5|tgt14.5|F5|K0|geo1|col1|sem0|gt0|ell0.5-0.05|sp0.001|nn512|knn0"
 
 
 def test_round_trip_and_replace(tmp_path):
