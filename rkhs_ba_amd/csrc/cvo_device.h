@@ -9,6 +9,20 @@
 
 namespace cvo {
 
+// Load through a pointer known to address global memory.  Pointers read from
+// device structures are generic to the compiler, which then emits flat loads;
+// those also count against the LDS wait counter, so a wait for the row's LDS
+// data would wait for every gather in flight as well.
+template <class T>
+__device__ __forceinline__ T ld_global(const T* p) {
+  return *(const __attribute__((address_space(1))) T*)p;
+}
+__device__ __forceinline__ float4 ld_global(const float4* p) {
+  typedef float v4f __attribute__((ext_vector_type(4)));
+  const v4f v = *(const __attribute__((address_space(1))) v4f*)p;
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+
 __device__ __forceinline__ void affine3(const float* A, const float* b, float y0, float y1, float y2, float& o0,
                                         float& o1, float& o2) {
   o0 = fmaf(A[2], y2, fmaf(A[1], y1, fmaf(A[0], y0, b[0])));

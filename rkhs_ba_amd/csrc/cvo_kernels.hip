@@ -299,7 +299,7 @@ __device__ __forceinline__ void assoc_row(const AssocProblem& P, const KConsts& 
   }
   const int F = Dim<FC>::get(kc.F);
   const int KS = Dim<KC>::get(kc.Ks);
-  float4 x4 = P.src_pos[r];
+  float4 x4 = ld_global(P.src_pos + r);
   float x0 = x4.x, x1 = x4.y, x2 = x4.z;
   if (mp.src_xf) {
     float t0, t1, t2;
@@ -338,8 +338,8 @@ __device__ __forceinline__ void assoc_row(const AssocProblem& P, const KConsts& 
     if (myrun < nruns) {
       const int iz = myrun / span1, iy = myrun - iz * span1;
       const uint32_t c0 = ((uint32_t)(lo2 + iz) * (uint32_t)GL.ny + (uint32_t)(lo1 + iy)) * (uint32_t)GL.nx + (uint32_t)lo0;
-      rs = GL.cell_start[c0];
-      rl = GL.cell_start[c0 + (uint32_t)(hi0 - lo0) + 1u] - rs;
+      rs = ld_global(GL.cell_start + c0);
+      rl = ld_global(GL.cell_start + c0 + (uint32_t)(hi0 - lo0) + 1u) - rs;
     }
     uint32_t incl = rl;
 #pragma unroll
@@ -366,18 +366,18 @@ __device__ __forceinline__ void assoc_row(const AssocProblem& P, const KConsts& 
       float4 y4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
       if (idx < total) {
         pidx = run_start + (idx - run_excl);
-        y4 = GL.pos[pidx];
+        y4 = ld_global(GL.pos + pidx);
         // with compile-time dimensions the candidate's features and class
         // vector are loaded together with its position, before the gates
         // that may not need them: one memory round trip instead of three
         float yfr[FC > 0 ? FC : 1], ysr[KC > 0 ? KC : 1];
         if (FC > 0) {
 #pragma unroll
-          for (int f = 0; f < (FC > 0 ? FC : 1); ++f) yfr[f] = GL.feat[(size_t)pidx * F + f];
+          for (int f = 0; f < (FC > 0 ? FC : 1); ++f) yfr[f] = ld_global(GL.feat + (size_t)pidx * F + f);
         }
         if (KC > 0) {
 #pragma unroll
-          for (int k = 0; k < (KC > 0 ? KC : 1); ++k) ysr[k] = GL.sem[(size_t)pidx * KS + k];
+          for (int k = 0; k < (KC > 0 ? KC : 1); ++k) ysr[k] = ld_global(GL.sem + (size_t)pidx * KS + k);
         }
         affine3(mp.A, mp.b, y4.x, y4.y, y4.z, y0, y1, y2);
         pass = pair_kernel<FC, KC>(kc, ek, x0, x1, x2, y0, y1, y2, d2t, qg, l, xf,
