@@ -7,6 +7,7 @@
 #pragma once
 
 #include <cstdint>
+#include <algorithm>
 #include <cstring>
 #include <list>
 #include <memory>
@@ -160,30 +161,51 @@ class CvoGPU {
     check(st, ctx_);
     if (registration_seconds) *registration_seconds = secs;
     if (num_iters) *num_iters = it;
-    if (association) take(a, *association);
+    if (association && a.row_ptr) take(a, *association);
     return ret;
   }
 
   // CvoResultInfo align(source, target, T_init_s2t, is_returning_association)
-  // (CvoGPU.cu:1566-1611): inverts its init; -1 on empty input.
+  // (CvoGPU.cu:1566-1611): starts from the inverse of its init, or with
+  // is_global_angle_registration from get_nearest_init_pose's result itself;
+  // return_code -1 on empty input.
   CvoResultInfo align(const CvoPointCloud& source, const CvoPointCloud& target, const float T_init_s2t[16],
                       bool is_returning_association = false) const {
     CvoResultInfo r;
-    if (source.size() == 0 || target.size() == 0) {
+    const cvo_cloud_view s = source.view(), t = target.view();
+    cvo_assoc a{};
+    int32_t it = 0, ret = 0;
+    cvo_status st = cvo_align_s2t(ctx_, &s, &t, T_init_s2t, r.T_s2t, is_returning_association ? &a : nullptr,
+                                  &r.registration_seconds, &it, &ret);
+    if (st == CVO_ERR_EMPTY) {
       r.return_code = -1;
       return r;
     }
-    float inv[16];
-    rigid_inverse(T_init_s2t, inv);
-    r.return_code = align(source, target, inv, r.T_s2t, is_returning_association ? &r.association : nullptr,
-                          &r.registration_seconds, &r.num_iters);
+    check(st, ctx_);
+    r.num_iters = it;
+    r.return_code = ret;
+    if (is_returning_association && a.row_ptr) take(a, r.association);
     return r;
   }
 
-  // int align(frames, hold_const, edges, seconds): the IRLS bundle adjustment
-  // (CvoGPU.cu:1614-1671).  Poses of the frames are updated in place.
+  // Matrix4f get_nearest_init_pose(source, target, ell, num_guess, times)
+  // (CvoGPU.hpp:249-253, CvoGPU.cpp:367-398): the rotation of the 60-rotation
+  // table with the largest approximate cos angle, as a row-major 4x4
+  // source-to-target transform; *times is incremented by the search time.
+  void get_nearest_init_pose(const CvoPointCloud& source, const CvoPointCloud& target, float init_guess_ell,
+                             int num_guess, float T_out[16], double* times = nullptr) const {
+    const cvo_cloud_view s = source.view(), t = target.view();
+    check(cvo_get_nearest_init_pose(ctx_, &s, &t, init_guess_ell, num_guess, T_out, times), ctx_);
+  }
+
+  // int align(frames, hold_const, edges, seconds[, inliers, num_iters]): the
+  // IRLS bundle adjustment (CvoGPU.cu:1614-1671, :1955-1999); the edges are
+  // frame pairs (the BinaryState list of the reference).  Poses of the
+  // frames are updated in place; inliers (optional) receives each edge's
+  // association of the last association pass, in edge order.
   int align(std::vector<CvoFrame::Ptr>& frames, const std::vector<bool>& hold_const,
-            const std::list<std::pair<CvoFrame::Ptr, CvoFrame::Ptr>>& edges, double* seconds) const {
+            const std::list<std::pair<CvoFrame::Ptr, CvoFrame::Ptr>>& edges, double* seconds,
+            std::list<std::shared_ptr<Association>>* inliers = nullptr, int* num_iters = nullptr) const {
     const int n = (int)frames.size();
     std::vector<cvo_dcloud*> dev(n, nullptr);
     std::vector<double> poses((size_t)n * 12);
@@ -206,25 +228,44 @@ class CvoGPU {
       e.push_back(b);
     }
     int32_t iters = 0;
+    const int32_t n_edges = (int32_t)(e.size() / 2);
+    std::vector<cvo_assoc> per_edge(inliers ? std::max(n_edges, 1) : 0);
     if (st == CVO_OK)
-      st = cvo_align_multiframe(ctx_, n, dev.data(), poses.data(), hc.data(), (int32_t)(e.size() / 2), e.data(),
-                                seconds, &iters, nullptr, 0);
+      st = cvo_align_multiframe(ctx_, n, dev.data(), poses.data(), hc.data(), n_edges, e.data(),
+                                inliers ? per_edge.data() : nullptr, seconds, &iters, nullptr, 0);
     for (cvo_dcloud* d : dev)
       if (d) cvo_cloud_free(ctx_, d);
     check(st, ctx_);
     for (int f = 0; f < n; ++f) std::memcpy(frames[f]->pose_vec, &poses[12 * (size_t)f], 12 * sizeof(double));
+    if (num_iters) *num_iters = iters;
+    if (inliers)
+      for (int32_t k = 0; k < n_edges; ++k) {
+        auto as = std::make_shared<Association>();
+        if (per_edge[k].row_ptr) take(per_edge[k], *as);
+        inliers->push_back(as);
+      }
     return 0;
   }
 
   // float function_angle(source, target, T, ell, is_approximate, is_gpu)
-  // (CvoGPU.cu:1768-1806); always computed on the GPU.
+  // (CvoGPU.cu:1768-1806); is_gpu false takes the inner_product_cpu
+  // semantics (computed on the GPU as well).
   float function_angle(const CvoPointCloud& source, const CvoPointCloud& target, const float t2s[16], float ell,
                        bool is_approximate = true, bool is_gpu = true) const {
-    (void)is_gpu;
     const cvo_cloud_view s = source.view(), t = target.view();
     double c = 0.0;
-    check(cvo_function_angle(ctx_, &s, &t, t2s, ell, is_approximate ? 1 : 0, &c), ctx_);
+    check(cvo_function_angle_ex(ctx_, &s, &t, t2s, ell, is_approximate ? 1 : 0, is_gpu ? 1 : 0, &c), ctx_);
     return (float)c;
+  }
+
+  // float inner_product_cpu(source, target, T, ell) (CvoGPU.cpp:204-231):
+  // fixed lengthscale, no row cap, no gates but the radius.
+  float inner_product_cpu(const CvoPointCloud& source, const CvoPointCloud& target, const float T[16],
+                          float ell) const {
+    const cvo_cloud_view s = source.view(), t = target.view();
+    double ip = 0.0;
+    check(cvo_inner_product_cpu(ctx_, &s, &t, T, ell, &ip), ctx_);
+    return (float)ip;
   }
 
   // float inner_product_gpu(source, target, T, ell) (CvoGPU.cu:1734-1748).
@@ -267,19 +308,17 @@ class CvoGPU {
     T_out = b;
     return r;
   }
+  Eigen::Matrix4f get_nearest_init_pose(const CvoPointCloud& source, const CvoPointCloud& target,
+                                        float init_guess_ell, int num_guess, double* times) const {
+    Eigen::Matrix<float, 4, 4, Eigen::RowMajor> m;
+    get_nearest_init_pose(source, target, init_guess_ell, num_guess, m.data(), times);
+    return m;
+  }
 #endif
 
   cvo_ctx* handle() const { return ctx_; }
 
  private:
-  static void rigid_inverse(const float T[16], float out[16]) {
-    for (int r = 0; r < 3; ++r) {
-      for (int c = 0; c < 3; ++c) out[r * 4 + c] = T[c * 4 + r];
-      out[r * 4 + 3] = -(T[0 * 4 + r] * T[3] + T[1 * 4 + r] * T[7] + T[2 * 4 + r] * T[11]);
-    }
-    out[12] = out[13] = out[14] = 0.0f;
-    out[15] = 1.0f;
-  }
   static void take(cvo_assoc& a, Association& out) {
     out.row_ptr.assign(a.row_ptr, a.row_ptr + a.rows + 1);
     out.col.assign(a.col, a.col + a.nnz);

@@ -205,13 +205,32 @@ int32_t cvo_cloud_size(const cvo_dcloud* c);
  * Empty input: returns CVO_ERR_EMPTY with ret_code 0 and T_out = T_init^-1
  * semantics of the reference's early return (it leaves transform untouched;
  * here T_out is set to the identity).  seconds = device time of the
- * iteration loop, like the reference's cudaEvent timer. */
+ * iteration loop, like the reference's cudaEvent timer.  With
+ * is_global_angle_registration > 0 the registration starts from the inverse
+ * of cvo_get_nearest_init_pose(source, target, ell_init_first_frame) instead
+ * of T_init, and seconds includes the search (CvoGPU.cu:1531-1543,
+ * :1555-1556). */
 cvo_status cvo_align(cvo_ctx* ctx, const cvo_cloud_view* src, const cvo_cloud_view* tgt,
                      const float T_init[16], float T_out[16], cvo_assoc* assoc_out,
                      double* seconds, int32_t* iters, int32_t* ret_code);
 cvo_status cvo_align_d(cvo_ctx* ctx, const cvo_dcloud* src, const cvo_dcloud* tgt,
                        const float T_init[16], float T_out[16], cvo_assoc* assoc_out,
                        double* seconds, int32_t* iters, int32_t* ret_code);
+
+/* CvoResultInfo CvoGPU::align(source, target, init_guess_T_s2t,
+ * is_returning_association) (include/UnifiedCvo/cvo/CvoGPU.hpp:75-80,
+ * src/cvo/CvoGPU.cu:1566-1611): the registration starts from the inverse of
+ * T_init_s2t (a general 4x4 inverse) or, with is_global_angle_registration > 0,
+ * from cvo_get_nearest_init_pose's result itself -- not its inverse: the
+ * reference's two overloads differ there (:1539 against :1587).  T_out,
+ * assoc_out, seconds and iters as cvo_align.  Empty input returns
+ * CVO_ERR_EMPTY with ret_code -1 (the reference's return_code). */
+cvo_status cvo_align_s2t(cvo_ctx* ctx, const cvo_cloud_view* src, const cvo_cloud_view* tgt,
+                         const float T_init_s2t[16], float T_out[16], cvo_assoc* assoc_out, double* seconds,
+                         int32_t* iters, int32_t* ret_code);
+cvo_status cvo_align_s2t_d(cvo_ctx* ctx, const cvo_dcloud* src, const cvo_dcloud* tgt, const float T_init_s2t[16],
+                           float T_out[16], cvo_assoc* assoc_out, double* seconds, int32_t* iters,
+                           int32_t* ret_code);
 
 /* AdaptiveCvoGPU::align (AdaptiveCvoGPU.hpp:171-175, AdaptiveCvoGPU.cu:1077-1248):
  * the lengthscale follows its gradient instead of the indicator schedule.
@@ -259,6 +278,21 @@ cvo_status cvo_inner_product_d(cvo_ctx* ctx, const cvo_dcloud* src, const cvo_dc
  * ip / (sqrt(N) sqrt(M)), otherwise ip / sqrt(ip_xx ip_yy). */
 cvo_status cvo_function_angle(cvo_ctx* ctx, const cvo_cloud_view* src, const cvo_cloud_view* tgt,
                               const float T[16], float ell, int32_t approx, double* cos_out);
+/* CvoGPU::inner_product_cpu (include/UnifiedCvo/cvo/CvoGPU.hpp:230-233,
+ * src/cvo/CvoGPU.cpp:204-231, se_kernel_init_ell_cpu :114-186), computed on
+ * the GPU with that function's semantics: the target moved into the source
+ * frame by T^-1, every pair within the squared radius
+ * -2 ell^2 log(sp_thres / sigma^2) at the fixed lengthscale (no range
+ * scaling), no row cap and no colour, semantic or geometric-type gate; a pair
+ * adds a = ck k sk (k only with is_using_geometry, ck with is_using_intensity,
+ * sk with is_using_semantics) when a > sp_thres.  0 on empty input. */
+cvo_status cvo_inner_product_cpu(cvo_ctx* ctx, const cvo_cloud_view* src, const cvo_cloud_view* tgt,
+                                 const float T[16], float ell, double* ip);
+/* CvoGPU::function_angle with its is_gpu argument (CvoGPU.cu:1768-1806):
+ * is_gpu != 0 uses cvo_inner_product's association, is_gpu == 0 the
+ * inner_product_cpu semantics of cvo_inner_product_cpu (:1780-1783, :1797-1800). */
+cvo_status cvo_function_angle_ex(cvo_ctx* ctx, const cvo_cloud_view* src, const cvo_cloud_view* tgt,
+                                 const float T[16], float ell, int32_t approx, int32_t is_gpu, double* cos_out);
 /* CvoGPU::compute_association_gpu (src/cvo/CvoGPU.cu:1836-1871). */
 cvo_status cvo_compute_association(cvo_ctx* ctx, const cvo_cloud_view* src, const cvo_cloud_view* tgt,
                                    const float T[16], float ell, cvo_assoc* out);
@@ -286,6 +320,28 @@ cvo_status cvo_compute_association_knn(cvo_ctx* ctx, const cvo_cloud_view* src, 
                                        const float T[16], float ell, int32_t K, cvo_assoc* out);
 void cvo_assoc_free(cvo_assoc* a);
 
+/* ---- global initialisation ------------------------------------------------ */
+
+/* The 60 rotations of gen_icosahedron_init_rots (src/cvo/CvoGPU.cpp:279-342),
+ * row-major 3x3 each, in that table's order.  Generated (the icosahedral group
+ * in the table's orientation, FP64, rounded to float once); they agree with
+ * the reference's float literals to 1e-6.  Needs no device. */
+cvo_status cvo_icosahedral_rotations(float out[540]);
+
+/* CvoGPU::get_nearest_init_pose (include/UnifiedCvo/cvo/CvoGPU.hpp:249-253,
+ * src/cvo/CvoGPU.cpp:367-398): each rotation R of the table is scored by the
+ * approximate cos angle function_angle(source, target, [R^T 0; 0 1], ell) --
+ * all 60 in one batched association pass -- and, scanning in table order, a
+ * score replaces the best only when strictly larger, starting from 0; T_out =
+ * [R 0; 0 1] of the winner (source to target frame), the identity when no
+ * score is positive.  num_guess is accepted and unused, as in the reference.
+ * seconds (optional) is incremented by the search's wall time (the reference
+ * adds its whole seconds). */
+cvo_status cvo_get_nearest_init_pose(cvo_ctx* ctx, const cvo_cloud_view* src, const cvo_cloud_view* tgt, float ell,
+                                     int32_t num_guess, float T_out[16], double* seconds);
+cvo_status cvo_get_nearest_init_pose_d(cvo_ctx* ctx, const cvo_dcloud* src, const cvo_dcloud* tgt, float ell,
+                                       int32_t num_guess, float T_out[16], double* seconds);
+
 /* ---- batched multi-start global registration ----------------------------- */
 
 /* Replaces CvoGPU::get_nearest_init_pose + align (src/cvo/CvoGPU.cpp:367-398,
@@ -305,20 +361,27 @@ cvo_status cvo_align_multistart(cvo_ctx* ctx, const cvo_dcloud* src, const cvo_d
 
 /* ---- multi-frame IRLS bundle adjustment ---------------------------------- */
 
-/* Replaces CvoGPU::align(frames, hold_const, edges, seconds)
- * (src/cvo/CvoGPU.cu:1614-1671) together with CvoBatchIRLS::solve
- * (src/cvo/IRLS.cpp:148-367) and BinaryStateGPU (src/cvo/IRLS_State_GPU.cu:
- * 144-336, src/cvo/IRLS_State_GPU.cpp:10-146).  frames: n_frames device
- * clouds (sensor frame).  poses: n_frames * 12 doubles, row-major 3x4
- * world <- sensor (CvoFrame::pose_vec), updated in place.  hold_const
- * (nullable): nonzero keeps a frame fixed.  edges: 2 * n_edges frame indices
- * (f1, f2); each edge associates f1's points (rows) with f2's.  Limits: 64
- * frames, 20 free frames, 4096 edges.  per_iter_seconds (optional, capacity
- * max_iter_records) receives the wall time of each IRLS outer iteration;
- * seconds the whole call; iters the outer iterations run. */
+/* Replaces CvoGPU::align(frames, hold_const, edges, seconds) and
+ * CvoGPU::align(frames, hold_const, edge_states, seconds, inliers, num_iters)
+ * (include/UnifiedCvo/cvo/CvoGPU.hpp:146-164, src/cvo/CvoGPU.cu:1614-1671,
+ * :1955-1999) together with CvoBatchIRLS::solve (src/cvo/IRLS.cpp:148-367)
+ * and BinaryStateGPU (src/cvo/IRLS_State_GPU.cu:144-351,
+ * src/cvo/IRLS_State_GPU.cpp:10-146).  frames: n_frames device clouds (sensor
+ * frame).  poses: n_frames * 12 doubles, row-major 3x4 world <- sensor
+ * (CvoFrame::pose_vec), updated in place.  hold_const (nullable): nonzero
+ * keeps a frame fixed.  edges: 2 * n_edges frame indices (f1, f2); each edge
+ * associates f1's points (rows) with f2's.  per_edge (nullable, n_edges
+ * entries): each edge's association of the last outer iteration that ran
+ * one, in CSR form (rows: f1's points, cols: f2's ids ascending) -- the
+ * `inliers` list of BinaryState::export_association (IRLS_State_GPU.cu:
+ * 345-351); free each with cvo_assoc_free.  Limits: 64 frames, 20 free frames,
+ * 4096 edges.  per_iter_seconds (optional, capacity max_iter_records)
+ * receives the wall time of each IRLS outer iteration; seconds the whole
+ * call; iters the outer iterations run. */
 cvo_status cvo_align_multiframe(cvo_ctx* ctx, int32_t n_frames, const cvo_dcloud* const* frames, double* poses,
-                                const uint8_t* hold_const, int32_t n_edges, const int32_t* edges, double* seconds,
-                                int32_t* iters, double* per_iter_seconds, int32_t max_iter_records);
+                                const uint8_t* hold_const, int32_t n_edges, const int32_t* edges,
+                                cvo_assoc* per_edge, double* seconds, int32_t* iters, double* per_iter_seconds,
+                                int32_t max_iter_records);
 
 /* ---- learned pose proposer (no counterpart in the reference) -------------- */
 

@@ -1018,6 +1018,93 @@ extern "C" int orc_pair_value(const cvo_params* p, const cvo_cloud_view* src, co
   return e.ok ? 0 : 2;
 }
 
+// orc_pair_value over n (i, j) pairs: a (-1 when a gate rejects) and the
+// smallest relative gate margin of each.
+extern "C" int orc_pair_values(const cvo_params* p, const cvo_cloud_view* src, const cvo_cloud_view* tgt,
+                               const float T[16], float ell, int64_t n, const int32_t* ii, const int32_t* jj,
+                               double* a_out, double* margin_out) {
+  const Consts k = make_consts(p);
+  CloudRef S = ref(src), Tg = ref(tgt);
+  float A[9], b[3];
+  applied_transform(pose_from16(T), A, b);
+#pragma omp parallel for schedule(static)
+  for (int64_t q = 0; q < n; ++q) {
+    float yp[3];
+    xform_f(A, b, Tg.xyz + 3 * (size_t)jj[q], yp);
+    const float* xi = S.xyz + 3 * (size_t)ii[q];
+    const float l = range_ell(ell, xi);
+    const PairEval e = eval_pair_mirror(k, S, Tg, ii[q], jj[q], xi, yp, l, d2_geo_thres(k, l));
+    a_out[q] = e.ok ? (double)e.a : -1.0;
+    margin_out[q] = e.margin;
+  }
+  return 0;
+}
+
+// CvoGPU::inner_product_cpu (src/cvo/CvoGPU.cpp:204-231) with
+// se_kernel_init_ell_cpu (:114-186): the target moved into the source frame by
+// T^-1 (float, canonical FMA order), a radius search of squared radius
+// d2_thres = -2 l^2 log(sp / sigma^2) at the fixed lengthscale l = ell (no
+// range scaling, no row cap), and for every target inside it
+// a = ck k sk -- k = float(sigma^2 exp(-d2 / 2l^2)) with geometry,
+// ck = float(c_sigma^2 exp(-|f_x - f_y|^2 / 2 c_ell^2)) with intensity,
+// sk = float(s_sigma^2 exp(-|s_x - s_y|^2 / 2 s_ell^2)) with semantics, no
+// colour / semantic / geometric-type gate -- counted when a > sp_thres.  The
+// reference sums the float sparse matrix; here the sum is FP64.  Searching
+// the grid or scanning every target (brute) gives the same set.  pairs_out
+// (optional): the number of pairs counted.
+extern "C" double orc_inner_product_cpu(const cvo_params* p, const cvo_cloud_view* src, const cvo_cloud_view* tgt,
+                                        const float T[16], float ell, int32_t brute, int64_t* pairs_out) {
+  const Consts k = make_consts(p);
+  CloudRef S = ref(src), Tg = ref(tgt);
+  const int N = S.n, M = Tg.n;
+  if (N == 0 || M == 0) {
+    if (pairs_out) *pairs_out = 0;
+    return 0.0;
+  }
+  float A[9], b[3];
+  applied_transform(pose_from16(T), A, b);
+  std::vector<float> yp((size_t)M * 3);
+  for (int j = 0; j < M; ++j) xform_f(A, b, Tg.xyz + 3 * (size_t)j, &yp[3 * (size_t)j]);
+  const float l = ell;
+  const float d2t = static_cast<float>(-2.0 * (double)l * (double)l * (double)k.lg);
+  Grid grid;
+  const double r = std::sqrt(std::max(0.0, (double)d2t)) * 1.001 + 1e-6;
+  if (!brute) grid.build(yp.data(), M, std::max(r, 1e-6));
+  double total = 0.0;
+  int64_t cnt = 0;
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : total, cnt)
+  for (int i = 0; i < N; ++i) {
+    const float* xi = S.xyz + 3 * (size_t)i;
+    auto visit = [&](int j) {
+      const float d2 = sqdist3_f(&yp[3 * (size_t)j], xi);
+      if (!(d2 < d2t)) return;
+      float kk = 1.0f, ck = 1.0f, sk = 1.0f;
+      if (k.geo) kk = static_cast<float>((double)k.sigma2 * std::exp((double)(-d2) / (2.0 * (double)l * (double)l)));
+      if (k.col) {
+        const float d2c = sqdist_n_f(S.feat + (size_t)i * S.F, Tg.feat + (size_t)j * Tg.F, S.F);
+        ck = static_cast<float>((double)k.c_sigma2 * std::exp((double)(-d2c) / (2.0 * (double)k.c2)));
+      }
+      if (k.sem) {
+        const float d2s = sqdist_n_f(S.sem + (size_t)i * S.K, Tg.sem + (size_t)j * Tg.K, S.K);
+        sk = static_cast<float>((double)k.s_sigma2 *
+                                std::exp((double)(-d2s) / (2.0 * (double)k.s_ell * (double)k.s_ell)));
+      }
+      const float a = ck * kk * sk;
+      if (a > k.sp) {
+        total += (double)a;
+        cnt++;
+      }
+    };
+    if (brute) {
+      for (int j = 0; j < M; ++j) visit(j);
+    } else {
+      grid.query(xi, r, visit);
+    }
+  }
+  if (pairs_out) *pairs_out = cnt;
+  return total;
+}
+
 extern "C" int orc_transform_target(const float T[16], int32_t m, const float* xyz, float* out) {
   float A[9], b[3];
   applied_transform(pose_from16(T), A, b);

@@ -223,6 +223,14 @@ int orc_assoc_nonisotropic(const cvo_params* p, const cvo_cloud_view* src, const
                            const float T[16], const float kernel[9], int32_t K, int32_t* counts, int32_t* cols,
                            float* vals);
 
+/* orc_pair_value over n (i, j) pairs (a = -1 for a rejected pair). */
+int orc_pair_values(const cvo_params* p, const cvo_cloud_view* src, const cvo_cloud_view* tgt, const float T[16],
+                    float ell, int64_t n, const int32_t* i, const int32_t* j, double* a_out, double* margin_out);
+/* CvoGPU::inner_product_cpu (CvoGPU.cpp:114-231): fixed lengthscale, radius
+ * gate only, no row cap; FP64 sum of the float pair values. */
+double orc_inner_product_cpu(const cvo_params* p, const cvo_cloud_view* src, const cvo_cloud_view* tgt,
+                             const float T[16], float ell, int32_t brute, int64_t* pairs_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -50,6 +50,9 @@ def lib():
         _lib.orc_inner_product_adaptive.argtypes = [P, P, P, P]
         _lib.orc_inner_product_adaptive.restype = d
         _lib.orc_pair_value.argtypes = [P, P, P, P, f, i32, i32, P, P]
+        _lib.orc_pair_values.argtypes = [P, P, P, P, f, C.c_int64, P, P, P, P]
+        _lib.orc_inner_product_cpu.argtypes = [P, P, P, P, f, i32, P]
+        _lib.orc_inner_product_cpu.restype = d
         _lib.orc_flow.argtypes = [P, i32, P, P, i32, P, P, P, i32, P, P]
         _lib.orc_step_coeffs.argtypes = [P, f, i32, P, P, i32, P, P, P, P, i32, P]
         _lib.orc_exp_sek3_f32.argtypes = [P, f, P]
@@ -141,6 +144,28 @@ def pair_value(params, src, tgt, T, ell, i, j):
     T16 = mat16(T)
     lib().orc_pair_value(C.byref(params), hs.ref, ht.ref, _p(T16), C.c_float(ell), int(i), int(j), C.byref(a), C.byref(m))
     return a.value, m.value
+
+
+def pair_values(params, src, tgt, T, ell, i, j):
+    """pair_value over arrays of (i, j): (a, margin) arrays; a = -1 where a
+    gate rejects the pair."""
+    hs, ht = HeldView(src), HeldView(tgt)
+    ii = np.ascontiguousarray(i, np.int32)
+    jj = np.ascontiguousarray(j, np.int32)
+    a = np.zeros(ii.size, np.float64)
+    m = np.zeros(ii.size, np.float64)
+    lib().orc_pair_values(C.byref(params), hs.ref, ht.ref, _p(mat16(T)), C.c_float(ell), ii.size, _p(ii), _p(jj),
+                          _p(a), _p(m))
+    return a, m
+
+
+def inner_product_cpu(params, src, tgt, T, ell, brute=False):
+    """CvoGPU::inner_product_cpu (CvoGPU.cpp:114-231): (sum, pair count)."""
+    hs, ht = HeldView(src), HeldView(tgt)
+    n = C.c_int64()
+    v = lib().orc_inner_product_cpu(C.byref(params), hs.ref, ht.ref, _p(mat16(T)), C.c_float(ell), int(brute),
+                                    C.byref(n))
+    return v, n.value
 
 
 def transform_target(T, xyz):

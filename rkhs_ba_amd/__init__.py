@@ -6,8 +6,8 @@ reference ``cvo::CvoGPU`` interface plus seeded synthetic scenes.
 """
 from ._abi import CvoParams, PARAM_NAMES  # noqa: F401
 from .runtime import (Association, CvoError, CvoFrame, CvoGPU, CvoResultInfo, DeviceCloud,  # noqa: F401
-                      default_params, lib, read_params_yaml, version)
+                      default_params, icosahedral_rotations, lib, read_params_yaml, version)
 from . import synth  # noqa: F401
 
 __all__ = ["CvoGPU", "CvoFrame", "CvoParams", "CvoError", "CvoResultInfo", "Association", "DeviceCloud",
-           "default_params", "read_params_yaml", "synth", "lib", "version"]
+           "default_params", "read_params_yaml", "synth", "lib", "version", "icosahedral_rotations"]

@@ -347,7 +347,8 @@ cvo_status setup_problems(cvo_ctx* ctx, const cvo_dcloud* s, const cvo_dcloud* t
   pr.n_probs = n_probs;
   pr.total_blocks = nb * n_probs;
   pr.max_rows = n;
-  CVO_TRY(ensure(ctx, ctx->b_pairs, sizeof(uint2) * (size_t)n * cap_alloc * n_probs));
+  // cap_alloc 0: the passes only sum (no pair array)
+  if (cap_alloc > 0) CVO_TRY(ensure(ctx, ctx->b_pairs, sizeof(uint2) * (size_t)n * cap_alloc * n_probs));
   CVO_TRY(ensure(ctx, ctx->b_counts, sizeof(uint32_t) * (size_t)n * n_probs));
   CVO_TRY(ensure(ctx, ctx->b_ovf, sizeof(uint32_t) * (size_t)n * n_probs));
   CVO_TRY(ensure(ctx, ctx->b_state, sizeof(ProblemState) * n_probs));
@@ -383,7 +384,7 @@ cvo_status setup_problems(cvo_ctx* ctx, const cvo_dcloud* s, const cvo_dcloud* t
     P.tgt_sem = t->sem;
     P.tgt_geo = t->geo;
     P.st = pr.states + q;
-    P.pairs = as<uint2>(ctx->b_pairs) + (size_t)q * n * cap_alloc;
+    P.pairs = cap_alloc > 0 ? as<uint2>(ctx->b_pairs) + (size_t)q * n * cap_alloc : nullptr;
     P.counts = as<uint32_t>(ctx->b_counts) + (size_t)q * n;
     P.ovf_stamp = as<uint32_t>(ctx->b_ovf) + (size_t)q * n;
     P.cap_alloc = cap_alloc;
@@ -565,13 +566,20 @@ static void mat_inverse_rigid(const float T[16], float out[16]) {
 
 static cvo_status export_assoc(cvo_ctx* ctx, const cvo_dcloud* s, const cvo_dcloud* t, const Problems& pr,
                                cvo_assoc* out) {
+  return export_problem(ctx, pr.dev, s->n, t->n, out);
+}
+
+// The stored pairs of one problem (device descriptor P_dev, n_src rows) as a
+// host CSR: rows in input order, target ids ascending (or by value, see
+// export_rows).
+cvo_status export_problem(cvo_ctx* ctx, const AssocProblem* P_dev, int n_src, int n_tgt, cvo_assoc* out) {
   hipStream_t st = ctx->stream;
-  const int n = s->n;
+  const int n = n_src;
   CVO_TRY(ensure(ctx, ctx->b_export, sizeof(int64_t) * (size_t)(n + 1) * 2));
   int64_t* cnt = as<int64_t>(ctx->b_export);
   int64_t* rp = cnt + (n + 1);
   CVO_HIP(ctx, hipMemsetAsync(cnt, 0, sizeof(int64_t) * (n + 1), st));
-  launch_export(pr.dev, n, cnt, nullptr, nullptr, nullptr, 0, 0, st);
+  launch_export(P_dev, n, cnt, nullptr, nullptr, nullptr, 0, 0, st);
   size_t tmp = 0;
   CVO_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, cnt, rp, n + 1, st));
   CVO_TRY(ensure(ctx, ctx->b_cub, tmp));
@@ -582,10 +590,10 @@ static cvo_status export_assoc(cvo_ctx* ctx, const cvo_dcloud* s, const cvo_dclo
   CVO_TRY(ensure(ctx, ctx->b_tmp2, (sizeof(int32_t) + sizeof(float)) * (size_t)std::max<int64_t>(nnz, 1)));
   int32_t* dcol = as<int32_t>(ctx->b_tmp2);
   float* dval = reinterpret_cast<float*>(dcol + std::max<int64_t>(nnz, 1));
-  launch_export(pr.dev, n, nullptr, rp, dcol, dval, 1, ctx->params.multiframe_is_sorting_inner_product, st);
+  launch_export(P_dev, n, nullptr, rp, dcol, dval, 1, ctx->params.multiframe_is_sorting_inner_product, st);
   CVO_HIP(ctx, hipGetLastError());
   out->rows = n;
-  out->cols = t->n;
+  out->cols = n_tgt;
   out->nnz = nnz;
   out->row_ptr = static_cast<int64_t*>(std::malloc(sizeof(int64_t) * (n + 1)));
   out->col = static_cast<int32_t*>(std::malloc(sizeof(int32_t) * std::max<int64_t>(nnz, 1)));
@@ -825,9 +833,11 @@ static void account_bytes_adaptive(cvo_ctx* ctx, const cvo_dcloud* x, const cvo_
   }
 }
 
-extern "C" cvo_status cvo_align_d(cvo_ctx* ctx, const cvo_dcloud* src, const cvo_dcloud* tgt, const float T_init[16],
-                                  float T_out[16], cvo_assoc* assoc_out, double* seconds, int32_t* iters,
-                                  int32_t* ret_code) {
+// The registration loop of both align overloads (align_impl, CvoGPU.cu:
+// 1239-1477) from the init transform as given.
+static cvo_status align_impl_d(cvo_ctx* ctx, const cvo_dcloud* src, const cvo_dcloud* tgt, const float T_init[16],
+                               float T_out[16], cvo_assoc* assoc_out, double* seconds, int32_t* iters,
+                               int32_t* ret_code) {
   if (!ctx || !T_init || !T_out) return CVO_ERR_INVALID_ARG;
   if (ret_code) *ret_code = 0;
   if (iters) *iters = 0;
@@ -889,6 +899,172 @@ extern "C" cvo_status cvo_align_d(cvo_ctx* ctx, const cvo_dcloud* src, const cvo
   return CVO_OK;
 }
 
+// General 4x4 inverse (cofactors, FP64) rounded to float: the Eigen
+// Matrix4f::inverse() of the reference's align overloads.
+static void inverse4(const float m[16], float out[16]) {
+  double a[16], inv[16];
+  for (int i = 0; i < 16; ++i) a[i] = (double)m[i];
+  inv[0] = a[5] * a[10] * a[15] - a[5] * a[11] * a[14] - a[9] * a[6] * a[15] + a[9] * a[7] * a[14] +
+           a[13] * a[6] * a[11] - a[13] * a[7] * a[10];
+  inv[4] = -a[4] * a[10] * a[15] + a[4] * a[11] * a[14] + a[8] * a[6] * a[15] - a[8] * a[7] * a[14] -
+           a[12] * a[6] * a[11] + a[12] * a[7] * a[10];
+  inv[8] = a[4] * a[9] * a[15] - a[4] * a[11] * a[13] - a[8] * a[5] * a[15] + a[8] * a[7] * a[13] +
+           a[12] * a[5] * a[11] - a[12] * a[7] * a[9];
+  inv[12] = -a[4] * a[9] * a[14] + a[4] * a[10] * a[13] + a[8] * a[5] * a[14] - a[8] * a[6] * a[13] -
+            a[12] * a[5] * a[10] + a[12] * a[6] * a[9];
+  inv[1] = -a[1] * a[10] * a[15] + a[1] * a[11] * a[14] + a[9] * a[2] * a[15] - a[9] * a[3] * a[14] -
+           a[13] * a[2] * a[11] + a[13] * a[3] * a[10];
+  inv[5] = a[0] * a[10] * a[15] - a[0] * a[11] * a[14] - a[8] * a[2] * a[15] + a[8] * a[3] * a[14] +
+           a[12] * a[2] * a[11] - a[12] * a[3] * a[10];
+  inv[9] = -a[0] * a[9] * a[15] + a[0] * a[11] * a[13] + a[8] * a[1] * a[15] - a[8] * a[3] * a[13] -
+           a[12] * a[1] * a[11] + a[12] * a[3] * a[9];
+  inv[13] = a[0] * a[9] * a[14] - a[0] * a[10] * a[13] - a[8] * a[1] * a[14] + a[8] * a[2] * a[13] +
+            a[12] * a[1] * a[10] - a[12] * a[2] * a[9];
+  inv[2] = a[1] * a[6] * a[15] - a[1] * a[7] * a[14] - a[5] * a[2] * a[15] + a[5] * a[3] * a[14] +
+           a[13] * a[2] * a[7] - a[13] * a[3] * a[6];
+  inv[6] = -a[0] * a[6] * a[15] + a[0] * a[7] * a[14] + a[4] * a[2] * a[15] - a[4] * a[3] * a[14] -
+           a[12] * a[2] * a[7] + a[12] * a[3] * a[6];
+  inv[10] = a[0] * a[5] * a[15] - a[0] * a[7] * a[13] - a[4] * a[1] * a[15] + a[4] * a[3] * a[13] +
+            a[12] * a[1] * a[7] - a[12] * a[3] * a[5];
+  inv[14] = -a[0] * a[5] * a[14] + a[0] * a[6] * a[13] + a[4] * a[1] * a[14] - a[4] * a[2] * a[13] -
+            a[12] * a[1] * a[6] + a[12] * a[2] * a[5];
+  inv[3] = -a[1] * a[6] * a[11] + a[1] * a[7] * a[10] + a[5] * a[2] * a[11] - a[5] * a[3] * a[10] -
+           a[9] * a[2] * a[7] + a[9] * a[3] * a[6];
+  inv[7] = a[0] * a[6] * a[11] - a[0] * a[7] * a[10] - a[4] * a[2] * a[11] + a[4] * a[3] * a[10] +
+           a[8] * a[2] * a[7] - a[8] * a[3] * a[6];
+  inv[11] = -a[0] * a[5] * a[11] + a[0] * a[7] * a[9] + a[4] * a[1] * a[11] - a[4] * a[3] * a[9] -
+            a[8] * a[1] * a[7] + a[8] * a[3] * a[5];
+  inv[15] = a[0] * a[5] * a[10] - a[0] * a[6] * a[9] - a[4] * a[1] * a[10] + a[4] * a[2] * a[9] +
+            a[8] * a[1] * a[6] - a[8] * a[2] * a[5];
+  const double det = a[0] * inv[0] + a[1] * inv[4] + a[2] * inv[8] + a[3] * inv[12];
+  for (int i = 0; i < 16; ++i) out[i] = (float)(inv[i] / det);
+}
+
+// Approximate cos angles of one source / target pair at n poses (row-major
+// 4x4 each, the inner-product convention), all in one batched association
+// pass at lengthscale ell with the row cap nearest_neighbors_max: the
+// function_angle scores of get_nearest_init_pose (CvoGPU.cpp:381-389).
+static cvo_status score_poses(cvo_ctx* ctx, const cvo_dcloud* src, const cvo_dcloud* tgt, int n, const float* T16,
+                              float ell, std::vector<double>& ip) {
+  CVO_TRY(validate_pair(ctx, src, tgt));
+  const cvo_params& p = ctx->params;
+  KConsts kc = make_kconsts(p, src->F, src->K, MODE_IP);
+  kc.knn = 0;
+  apply_tuning(ctx, kc, n > 1);
+  kc.n_src_f = (float)src->n;
+  kc.n_tgt_f = (float)tgt->n;
+  kc.max_src_norm = src->max_norm;
+  kc.max_iter = 1;
+  CVO_TRY(prepare_source(ctx, src));
+  float tq = 0.0f;
+  for (int s = 0; s < n; ++s)
+    tq = std::max(tq, std::max(std::fabs(T16[16 * s + 3]), std::max(std::fabs(T16[16 * s + 7]), std::fabs(T16[16 * s + 11]))));
+  GridHost gh;
+  CVO_TRY(build_grid(ctx, tgt, level_sizes(kc, ell, ell, src->max_norm, tq, kc.min_scale), gh));
+  Problems pr;
+  CVO_TRY(setup_problems(ctx, src, tgt, gh, n, 0, 0, kc.max_group, pr));
+  CVO_TRY(ensure(ctx, ctx->b_tmp2, sizeof(float) * 16 * (size_t)n + 64));
+  CVO_HIP(ctx, hipMemcpyAsync(ctx->b_tmp2.p, T16, sizeof(float) * 16 * n, hipMemcpyHostToDevice, ctx->stream));
+  launch_init_state(pr.dev, n, as<float>(ctx->b_tmp2), 16, kc, ell, p.nearest_neighbors_max, ctx->stream);
+  CVO_TRY(run_loop(ctx, pr, kc, 1, false, nullptr));
+  std::vector<ProblemState> hs(n);
+  CVO_HIP(ctx, hipMemcpyAsync(hs.data(), pr.states, sizeof(ProblemState) * n, hipMemcpyDeviceToHost, ctx->stream));
+  CVO_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  ip.resize(n);
+  for (int s = 0; s < n; ++s) ip[s] = hs[s].ip;
+  return CVO_OK;
+}
+
+// get_nearest_init_pose (CvoGPU.cpp:367-398): the table rotation whose
+// approximate cos angle is the first strict maximum above 0, as [R 0; 0 1];
+// the identity when none is positive.  secs += the search's wall time.
+static cvo_status nearest_init_pose(cvo_ctx* ctx, const cvo_dcloud* src, const cvo_dcloud* tgt, float ell,
+                                    float T_out[16], double* secs) {
+  const auto t0 = std::chrono::steady_clock::now();
+  float rots[540];
+  CVO_TRY(cvo_icosahedral_rotations(rots));
+  // each candidate is scored at the inverse of [R 0; 0 1], i.e. [R^T 0; 0 1]
+  std::vector<float> T(60 * 16, 0.0f);
+  for (int i = 0; i < 60; ++i) {
+    float* t = &T[16 * i];
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) t[r * 4 + c] = rots[9 * i + c * 3 + r];
+    t[15] = 1.0f;
+  }
+  std::vector<double> ip;
+  CVO_TRY(score_poses(ctx, src, tgt, 60, T.data(), ell, ip));
+  // function_angle in float: fxfz / (sqrt(N) sqrt(M)) (CvoGPU.cu:1785-1802)
+  const float fx = (float)std::sqrt((double)src->n), fz = (float)std::sqrt((double)tgt->n);
+  float best = 0.0f;
+  int best_i = -1;
+  for (int i = 0; i < 60; ++i) {
+    const float c = (float)ip[i] / (fx * fz);
+    if (c > best) {
+      best = c;
+      best_i = i;
+    }
+  }
+  for (int i = 0; i < 16; ++i) T_out[i] = (i % 5 == 0) ? 1.0f : 0.0f;
+  if (best_i >= 0)
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) T_out[r * 4 + c] = rots[9 * best_i + r * 3 + c];
+  if (secs) *secs += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return CVO_OK;
+}
+
+extern "C" cvo_status cvo_get_nearest_init_pose_d(cvo_ctx* ctx, const cvo_dcloud* src, const cvo_dcloud* tgt,
+                                                  float ell, int32_t num_guess, float T_out[16], double* seconds) {
+  (void)num_guess;
+  if (!ctx || !T_out) return CVO_ERR_INVALID_ARG;
+  for (int i = 0; i < 16; ++i) T_out[i] = (i % 5 == 0) ? 1.0f : 0.0f;
+  if (!(ell > 0.0f)) return set_err(ctx, CVO_ERR_INVALID_ARG, "ell must be positive");
+  if (!src || !tgt || src->n == 0 || tgt->n == 0) return set_err(ctx, CVO_ERR_EMPTY, "empty point cloud");
+  return nearest_init_pose(ctx, src, tgt, ell, T_out, seconds);
+}
+
+extern "C" cvo_status cvo_align_d(cvo_ctx* ctx, const cvo_dcloud* src, const cvo_dcloud* tgt, const float T_init[16],
+                                  float T_out[16], cvo_assoc* assoc_out, double* seconds, int32_t* iters,
+                                  int32_t* ret_code) {
+  if (!ctx || !T_init || !T_out) return CVO_ERR_INVALID_ARG;
+  const cvo_params& p = ctx->params;
+  if (p.is_global_angle_registration > 0 && src && tgt && src->n > 0 && tgt->n > 0) {
+    // init = get_nearest_init_pose(...).inverse() (CvoGPU.cu:1531-1543)
+    double search = 0.0;
+    float Tg[16], Ti[16];
+    CVO_TRY(nearest_init_pose(ctx, src, tgt, p.ell_init_first_frame, Tg, &search));
+    inverse4(Tg, Ti);
+    CVO_TRY(align_impl_d(ctx, src, tgt, Ti, T_out, assoc_out, seconds, iters, ret_code));
+    if (seconds) *seconds += search;
+    return CVO_OK;
+  }
+  return align_impl_d(ctx, src, tgt, T_init, T_out, assoc_out, seconds, iters, ret_code);
+}
+
+extern "C" cvo_status cvo_align_s2t_d(cvo_ctx* ctx, const cvo_dcloud* src, const cvo_dcloud* tgt,
+                                      const float T_init_s2t[16], float T_out[16], cvo_assoc* assoc_out,
+                                      double* seconds, int32_t* iters, int32_t* ret_code) {
+  if (!ctx || !T_init_s2t || !T_out) return CVO_ERR_INVALID_ARG;
+  if (!src || !tgt || src->n == 0 || tgt->n == 0) {
+    if (ret_code) *ret_code = -1;
+    if (iters) *iters = 0;
+    if (seconds) *seconds = 0.0;
+    for (int i = 0; i < 16; ++i) T_out[i] = (i % 5 == 0) ? 1.0f : 0.0f;
+    return set_err(ctx, CVO_ERR_EMPTY, "empty point cloud");
+  }
+  const cvo_params& p = ctx->params;
+  float T0[16];
+  double search = 0.0;
+  if (p.is_global_angle_registration > 0) {
+    // init = get_nearest_init_pose(...) itself, not its inverse (CvoGPU.cu:1580-1591)
+    CVO_TRY(nearest_init_pose(ctx, src, tgt, p.ell_init_first_frame, T0, &search));
+  } else {
+    inverse4(T_init_s2t, T0);  // init_guess_T_s2t.inverse() (:1578)
+  }
+  CVO_TRY(align_impl_d(ctx, src, tgt, T0, T_out, assoc_out, seconds, iters, ret_code));
+  if (seconds) *seconds += search;
+  return CVO_OK;
+}
+
 // RAII holder for uploaded clouds of a host-view call.
 struct Uploaded {
   cvo_ctx* ctx;
@@ -909,6 +1085,29 @@ extern "C" cvo_status cvo_align(cvo_ctx* ctx, const cvo_cloud_view* src, const c
   return cvo_align_d(ctx, a.c, b.c, T_init, T_out, assoc_out, seconds, iters, ret_code);
 }
 
+extern "C" cvo_status cvo_align_s2t(cvo_ctx* ctx, const cvo_cloud_view* src, const cvo_cloud_view* tgt,
+                                    const float T_init_s2t[16], float T_out[16], cvo_assoc* assoc_out,
+                                    double* seconds, int32_t* iters, int32_t* ret_code) {
+  if (!ctx || !src || !tgt || !T_init_s2t || !T_out) return CVO_ERR_INVALID_ARG;
+  if (src->n == 0 || tgt->n == 0)
+    return cvo_align_s2t_d(ctx, nullptr, nullptr, T_init_s2t, T_out, assoc_out, seconds, iters, ret_code);
+  Uploaded a{ctx}, b{ctx};
+  CVO_TRY(cvo_cloud_upload(ctx, src, &a.c));
+  CVO_TRY(cvo_cloud_upload(ctx, tgt, &b.c));
+  return cvo_align_s2t_d(ctx, a.c, b.c, T_init_s2t, T_out, assoc_out, seconds, iters, ret_code);
+}
+
+extern "C" cvo_status cvo_get_nearest_init_pose(cvo_ctx* ctx, const cvo_cloud_view* src, const cvo_cloud_view* tgt,
+                                                float ell, int32_t num_guess, float T_out[16], double* seconds) {
+  if (!ctx || !src || !tgt || !T_out) return CVO_ERR_INVALID_ARG;
+  if (src->n == 0 || tgt->n == 0)
+    return cvo_get_nearest_init_pose_d(ctx, nullptr, nullptr, ell, num_guess, T_out, seconds);
+  Uploaded a{ctx}, b{ctx};
+  CVO_TRY(cvo_cloud_upload(ctx, src, &a.c));
+  CVO_TRY(cvo_cloud_upload(ctx, tgt, &b.c));
+  return cvo_get_nearest_init_pose_d(ctx, a.c, b.c, ell, num_guess, T_out, seconds);
+}
+
 // Single association pass at pose T and lengthscale ell; the problem stays in
 // the context's buffers for an export.
 // knn_K > 0: the K-nearest pass with row cap knn_K instead of the ball pass.
@@ -916,8 +1115,10 @@ static KConsts adaptive_kconsts(const cvo_params& p, int F, int Ks);
 
 // kc_in: the pass's kernel constants (mode MODE_IP) instead of the
 // reference's association.
+// keep_pairs false: the pass only sums (nothing to export afterwards).
 static cvo_status single_pass(cvo_ctx* ctx, const cvo_dcloud* src, const cvo_dcloud* tgt, const float T[16], float ell,
-                              Problems& pr, KConsts& kc, double* ip, int knn_K = 0, const KConsts* kc_in = nullptr) {
+                              Problems& pr, KConsts& kc, double* ip, int knn_K = 0, const KConsts* kc_in = nullptr,
+                              bool keep_pairs = true) {
   CVO_TRY(validate_pair(ctx, src, tgt));
   const cvo_params& p = ctx->params;
   kc = kc_in ? *kc_in : make_kconsts(p, src->F, src->K, MODE_IP);
@@ -933,7 +1134,7 @@ static cvo_status single_pass(cvo_ctx* ctx, const cvo_dcloud* src, const cvo_dcl
   GridHost gh;
   std::vector<float> hs = level_sizes(kc, ell, ell, src->max_norm, tq, kc.min_scale);
   CVO_TRY(build_grid(ctx, tgt, hs, gh));
-  CVO_TRY(setup_problems(ctx, src, tgt, gh, 1, K0, 0, kc.max_group, pr));
+  CVO_TRY(setup_problems(ctx, src, tgt, gh, 1, keep_pairs ? K0 : 0, 0, kc.max_group, pr));
   CVO_TRY(ensure(ctx, ctx->b_tmp2, 64 * sizeof(float)));
   CVO_HIP(ctx, hipMemcpyAsync(ctx->b_tmp2.p, T, sizeof(float) * 16, hipMemcpyHostToDevice, ctx->stream));
   launch_init_state(pr.dev, 1, as<float>(ctx->b_tmp2), 16, kc, ell, K0, ctx->stream);
@@ -952,7 +1153,48 @@ extern "C" cvo_status cvo_inner_product_d(cvo_ctx* ctx, const cvo_dcloud* src, c
   if (!src || !tgt || src->n == 0 || tgt->n == 0) return set_err(ctx, CVO_ERR_EMPTY, "empty point cloud");
   Problems pr;
   KConsts kc;
-  return single_pass(ctx, src, tgt, T, ell, pr, kc, ip);
+  return single_pass(ctx, src, tgt, T, ell, pr, kc, ip, 0, nullptr, false);
+}
+
+// Kernel constants of inner_product_cpu (se_kernel_init_ell_cpu,
+// CvoGPU.cpp:114-186): the radius gate always (nanoflann radius search with
+// squared radius -2 l^2 log(sp / sigma^2)), the fixed lengthscale, every pair
+// inside it summed (no row cap), the factors of the enabled terms without
+// their gates, no geometric type.
+static KConsts cpu_ip_kconsts(const cvo_params& p, int F, int Ks) {
+  KConsts k = make_kconsts(p, F, Ks, MODE_IP);
+  k.knn = 0;
+  k.fixed_ell = 1;
+  k.uncapped = 1;
+  k.use_gtype = 0;
+  k.d2_c_thres = INFINITY;
+  k.d2_s_thres = INFINITY;
+  if (!k.use_geo) {
+    // the radius search still gates; k stays 1
+    k.use_geo = 1;
+    k.no_geo_factor = 1;
+    k.rq_coef = sqrtf(-2.0f * k.lg) * 1.001f;
+  }
+  return k;
+}
+
+static cvo_status inner_product_cpu_d(cvo_ctx* ctx, const cvo_dcloud* src, const cvo_dcloud* tgt, const float T[16],
+                                      float ell, double* ip) {
+  const KConsts kc_in = cpu_ip_kconsts(ctx->params, src->F, src->K);
+  Problems pr;
+  KConsts kc;
+  return single_pass(ctx, src, tgt, T, ell, pr, kc, ip, 0, &kc_in, false);
+}
+
+extern "C" cvo_status cvo_inner_product_cpu(cvo_ctx* ctx, const cvo_cloud_view* src, const cvo_cloud_view* tgt,
+                                            const float T[16], float ell, double* ip) {
+  if (!ctx || !src || !tgt || !T || !ip) return CVO_ERR_INVALID_ARG;
+  *ip = 0.0;
+  if (src->n == 0 || tgt->n == 0) return CVO_OK;  // inner_product_cpu returns 0 (CvoGPU.cpp:209-211)
+  Uploaded a{ctx}, b{ctx};
+  CVO_TRY(cvo_cloud_upload(ctx, src, &a.c));
+  CVO_TRY(cvo_cloud_upload(ctx, tgt, &b.c));
+  return inner_product_cpu_d(ctx, a.c, b.c, T, ell, ip);
 }
 
 extern "C" cvo_status cvo_inner_product(cvo_ctx* ctx, const cvo_cloud_view* src, const cvo_cloud_view* tgt,
@@ -968,6 +1210,12 @@ extern "C" cvo_status cvo_inner_product(cvo_ctx* ctx, const cvo_cloud_view* src,
 
 extern "C" cvo_status cvo_function_angle(cvo_ctx* ctx, const cvo_cloud_view* src, const cvo_cloud_view* tgt,
                                          const float T[16], float ell, int32_t approx, double* cos_out) {
+  return cvo_function_angle_ex(ctx, src, tgt, T, ell, approx, 1, cos_out);
+}
+
+extern "C" cvo_status cvo_function_angle_ex(cvo_ctx* ctx, const cvo_cloud_view* src, const cvo_cloud_view* tgt,
+                                            const float T[16], float ell, int32_t approx, int32_t is_gpu,
+                                            double* cos_out) {
   if (!ctx || !src || !tgt || !T || !cos_out) return CVO_ERR_INVALID_ARG;
   *cos_out = 0.0;
   // function_angle returns 0 on empty input (CvoGPU.cu:1775-1777)
@@ -975,8 +1223,12 @@ extern "C" cvo_status cvo_function_angle(cvo_ctx* ctx, const cvo_cloud_view* src
   Uploaded a{ctx}, b{ctx};
   CVO_TRY(cvo_cloud_upload(ctx, src, &a.c));
   CVO_TRY(cvo_cloud_upload(ctx, tgt, &b.c));
+  // is_gpu selects inner_product_gpu or inner_product_cpu (CvoGPU.cu:1780-1783)
+  auto inner = [&](const cvo_dcloud* x, const cvo_dcloud* y, const float* Tm, double* out) -> cvo_status {
+    return is_gpu ? cvo_inner_product_d(ctx, x, y, Tm, ell, out) : inner_product_cpu_d(ctx, x, y, Tm, ell, out);
+  };
   double fxfz = 0.0;
-  CVO_TRY(cvo_inner_product_d(ctx, a.c, b.c, T, ell, &fxfz));
+  CVO_TRY(inner(a.c, b.c, T, &fxfz));
   double fx, fz;
   if (approx) {
     fx = std::sqrt((double)src->n);
@@ -984,8 +1236,8 @@ extern "C" cvo_status cvo_function_angle(cvo_ctx* ctx, const cvo_cloud_view* src
   } else {
     const float I[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
     double xx = 0, zz = 0;
-    CVO_TRY(cvo_inner_product_d(ctx, a.c, a.c, I, ell, &xx));
-    CVO_TRY(cvo_inner_product_d(ctx, b.c, b.c, I, ell, &zz));
+    CVO_TRY(inner(a.c, a.c, I, &xx));
+    CVO_TRY(inner(b.c, b.c, I, &zz));
     fx = std::sqrt(xx);
     fz = std::sqrt(zz);
   }
@@ -1132,7 +1384,7 @@ extern "C" cvo_status cvo_align_multistart(cvo_ctx* ctx, const cvo_dcloud* src, 
     }
     int32_t it = 0, ret = 0;
     double secs = 0;
-    CVO_TRY(cvo_align_d(ctx, src, tgt, T_starts + 16 * (size_t)best, T_best, nullptr, &secs, &it, &ret));
+    CVO_TRY(align_impl_d(ctx, src, tgt, T_starts + 16 * (size_t)best, T_best, nullptr, &secs, &it, &ret));
     its[best] = it;
   }
   CVO_HIP(ctx, hipEventRecord(e1, ctx->stream));

@@ -660,7 +660,7 @@ static T* as(DevBuf& b) {
 
 enum {
   BA_PAIRS = 0, BA_COUNTS, BA_OVF, BA_STATE, BA_PROBS, BA_PART_A, BA_PART_OVF, BA_PART_CNT,
-  BA_DEV, BA_EDGES, BA_H, BA_HN, BA_PAIR_PART, BA_CONTRIB, BA_FLAG
+  BA_DEV, BA_EDGES, BA_H, BA_HN, BA_PAIR_PART, BA_CONTRIB, BA_FLAG, BA_PAIR_IDS
 };
 
 // Six buffers per frame: sorted source (pos, feat, sem, geo), grid store and
@@ -747,9 +747,11 @@ using namespace cvo;
 
 extern "C" cvo_status cvo_align_multiframe(cvo_ctx* ctx, int32_t n_frames, const cvo_dcloud* const* frames,
                                            double* poses, const uint8_t* hold_const, int32_t n_edges,
-                                           const int32_t* edges, double* seconds, int32_t* iters,
-                                           double* per_iter_seconds, int32_t max_iter_records) {
+                                           const int32_t* edges, cvo_assoc* per_edge, double* seconds,
+                                           int32_t* iters, double* per_iter_seconds, int32_t max_iter_records) {
   if (!ctx) return CVO_ERR_INVALID_ARG;
+  if (per_edge)
+    for (int e = 0; e < n_edges; ++e) std::memset(&per_edge[e], 0, sizeof(cvo_assoc));
   if (seconds) *seconds = 0.0;
   if (iters) *iters = 0;
   int nfree = 0;
@@ -812,6 +814,8 @@ extern "C" cvo_status cvo_align_multiframe(cvo_ctx* ctx, int32_t n_frames, const
   CVO_TRY(ensure(ctx, B[BA_H], sizeof(double) * 2 * kBaMaxDof * kBaMaxDof));
   CVO_TRY(ensure(ctx, B[BA_PAIR_PART], sizeof(double) * kBaTerms * (size_t)max_pair_blocks * E1));
   CVO_TRY(ensure(ctx, B[BA_CONTRIB], sizeof(double) * kBaContrib * E1));
+  // the export needs each stored pair's target (its sorted position)
+  if (per_edge) CVO_TRY(ensure(ctx, B[BA_PAIR_IDS], sizeof(uint32_t) * std::max<size_t>(tot_rows, 1) * cap));
   CVO_HIP(ctx, hipMemsetAsync(B[BA_OVF].p, 0, sizeof(uint32_t) * std::max<size_t>(tot_rows, 1), st));
   CVO_HIP(ctx, hipMemsetAsync(B[BA_STATE].p, 0, sizeof(ProblemState) * E1, st));
 
@@ -841,6 +845,7 @@ extern "C" cvo_status cvo_align_multiframe(cvo_ctx* ctx, int32_t n_frames, const
     P.st = as<ProblemState>(B[BA_STATE]) + e;
     P.pairs = nullptr;
     P.pairs4 = as<float4>(B[BA_PAIRS]) + row_off[e] * cap;
+    P.pair_ids = per_edge ? as<uint32_t>(B[BA_PAIR_IDS]) + row_off[e] * cap : nullptr;
     P.counts = as<uint32_t>(B[BA_COUNTS]) + row_off[e];
     P.ovf_stamp = as<uint32_t>(B[BA_OVF]) + row_off[e];
     P.cap_alloc = cap;
@@ -1040,6 +1045,13 @@ extern "C" cvo_status cvo_align_multiframe(cvo_ctx* ctx, int32_t n_frames, const
   ctx->ba_last_accepted = out->accepted_total;
   ctx->ba_last_evals = out->evals_total;
   delete out;
+  // with multiframe_max_iters > 0 the first outer iteration associates
+  const bool any_pass = p.multiframe_max_iters > 0;
+  // each edge's pairs of the last association pass (BinaryStateGPU::
+  // export_association, IRLS_State_GPU.cu:345-351; CvoGPU.cu:1990-1996)
+  if (per_edge && any_pass)
+    for (int e = 0; e < n_edges; ++e)
+      CVO_TRY(export_problem(ctx, L.probs + e, frames[edges[2 * e]]->n, frames[edges[2 * e + 1]]->n, &per_edge[e]));
   if (seconds) *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
   return CVO_OK;
 }

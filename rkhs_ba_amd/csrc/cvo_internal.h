@@ -98,6 +98,7 @@ struct AssocProblem {
   ProblemState* st;
   uint2* pairs;            // n_src * cap_alloc: (sorted target position, a bits)
   float4* pairs4;          // bundle adjustment: n_src * cap_alloc (target xyz in its frame, a)
+  uint32_t* pair_ids;      // bundle adjustment, optional: the sorted target position of each pairs4 slot
   uint32_t* counts;        // n_src
   uint32_t* ovf_stamp;     // n_src
   int32_t cap_alloc;
@@ -164,6 +165,7 @@ struct KConsts {
   int32_t fixed_ell;          // the association uses ell as is (no range scaling)
   int32_t ip_ad2;             // the ip column sums a |y - x|^2 instead of a
   int32_t uncapped;           // sum every passing pair (no row cap, no capped rows)
+  int32_t no_geo_factor;      // the distance gate applies, the geometric factor k does not
 };
 
 // Rows per association block for a lane count per row (8 or 16).
@@ -182,6 +184,14 @@ enum Mode { MODE_ALIGN = 0, MODE_IP = 1, MODE_BA = 2 };
 struct DevBuf {
   void* p = nullptr;
   size_t cap = 0;
+};
+
+// A bump region of device memory that ensure() carves from instead of
+// allocating (the no-allocation entry points): base / size of the region,
+// used bytes so far.
+struct BumpArena {
+  char* base = nullptr;
+  size_t size = 0, used = 0;
 };
 
 struct cvo_dcloud {
@@ -230,6 +240,10 @@ struct cvo_ctx {
   DevBuf b_mlp[16];
   // adaptive-lengthscale scratch: sorted target rows (4), source grid (2)
   DevBuf b_acvo[8];
+  // while set, ensure() carves from this region and never allocates
+  BumpArena* bump = nullptr;
+  // grid cell count limit of build_grid_into
+  uint32_t grid_max_cells = 1u << 25;
 };
 
 // Internal helpers shared by the translation units.
@@ -262,6 +276,7 @@ void launch_overflow(const AssocProblem* probs, int n_probs, const KConsts& kc, 
 void launch_init_state(const AssocProblem* probs, int n_probs, const float* T16, int stride16, const KConsts& kc,
                        float ell0, int K0, hipStream_t s);
 cvo_status prepare_source(cvo_ctx* ctx, const cvo_dcloud* s);
+cvo_status export_problem(cvo_ctx* ctx, const AssocProblem* P_dev, int n_src, int n_tgt, cvo_assoc* out);
 cvo_status build_grid(cvo_ctx* ctx, const cvo_dcloud* t, const std::vector<float>& hs, GridHost& gh);
 
 // Descriptors of n_probs association problems sharing one source and target.

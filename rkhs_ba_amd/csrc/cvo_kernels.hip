@@ -64,8 +64,15 @@ __device__ __forceinline__ uint32_t ld_dev(uint32_t* p) {
 __device__ __forceinline__ uint32_t count_in(uint32_t* counter) {
   return __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-// s_waitcnt vmcnt(0) (expcnt and lgkmcnt left at their maxima).
-__device__ __forceinline__ void wait_stores() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+// s_waitcnt vmcnt(0) (expcnt and lgkmcnt left at their maxima), between two
+// compiler-only fences: the builtin orders the hardware, the fences keep the
+// compiler from moving the partial stores past it or the completion atomic
+// before it (neither fence emits an instruction or a cache operation).
+__device__ __forceinline__ void wait_stores() {
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_s_waitcnt(0x0F70);
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
 
 // Exact evaluation of a pair's kernel value with FP64 exponentials rounded to
 // float, the reference's rounding points (CvoGPU.cu:1020-1054).  Used only for
@@ -220,7 +227,7 @@ __device__ __forceinline__ ExactK exact_consts(const KConsts& kc) {
   e.s_sigma2 = kc.s_sigma2;
   e.c2 = kc.c2;
   e.s_ell = kc.s_ell;
-  e.use_geo = kc.use_geo;
+  e.use_geo = kc.use_geo && !kc.no_geo_factor;
   e.use_col = kc.use_col;
   e.use_sem = kc.use_sem;
   return e;
@@ -262,7 +269,7 @@ __device__ __forceinline__ bool pair_kernel(const KConsts& kc, const ExactK& ek,
     }
     if (!(d2s < kc.d2_s_thres)) return false;
   }
-  float ex = kc.use_geo ? d2 * qg : 0.0f;
+  float ex = (kc.use_geo && !kc.no_geo_factor) ? d2 * qg : 0.0f;
   if (FC != 0) ex = fmaf(d2c, kc.q_c, ex);
   if (KC != 0) ex = fmaf(d2s, kc.q_s, ex);
   a = kc.amp * expf(-ex) * geo_sim;
@@ -270,6 +277,24 @@ __device__ __forceinline__ bool pair_kernel(const KConsts& kc, const ExactK& ek,
     a = exact_value(ek.sigma2, ek.c_sigma2, ek.s_sigma2, ek.c2, ek.s_ell, ek.use_geo, ek.use_col, ek.use_sem, d2, l,
                     d2c, d2s, geo_sim);
   return a > kc.sp;
+}
+
+// Stores one kept pair of row r in slot `slot`: the bundle adjustment keeps
+// the target's own coordinates and a at the iteration's row stride K (and,
+// when asked, the target's sorted position for an association export);
+// the other modes keep (sorted target position, a) at the allocated stride,
+// unless the pass only sums (no pair array).
+// NOBA: the launch never carries a bundle-adjustment problem (single-problem
+// kernels), so that store is compiled out.
+template <bool NOBA = false>
+__device__ __forceinline__ void store_pair(const AssocProblem& P, int mode, int r, uint32_t K, uint32_t slot,
+                                           uint32_t pidx, const float4& y4, float a) {
+  if (!NOBA && mode == MODE_BA) {
+    P.pairs4[(size_t)r * K + slot] = make_float4(y4.x, y4.y, y4.z, a);
+    if (P.pair_ids) P.pair_ids[(size_t)r * K + slot] = pidx;
+  } else if (P.pairs) {
+    P.pairs[(size_t)r * P.cap_alloc + slot] = make_uint2(pidx, __float_as_uint(a));
+  }
 }
 
 struct RowResult {
@@ -281,7 +306,7 @@ struct RowResult {
 // One row handled by a group of G lanes.  FC/KC: color/semantic dimension when the
 // gate is on (0 off, -1 runtime).  MODE_ALIGN accumulates the flow, MODE_IP and
 // MODE_BA only the sum of a.
-template <int FC, int KC, int G>
+template <int FC, int KC, int G, bool NOBA>
 __device__ __forceinline__ void assoc_row(const AssocProblem& P, const KConsts& kc, const ExactK& ek,
                                           const GridLevel& GL, const Maps& mp, int r, int g, int mode, float ell,
                                           int K, const float* xf, const float* xs, RowResult& out) {
@@ -390,10 +415,7 @@ __device__ __forceinline__ void assoc_row(const AssocProblem& P, const KConsts& 
       const uint32_t gm = (uint32_t)(m >> (G * ((threadIdx.x & 63) / G))) & kGroupMask;
       const uint32_t slot = stored + (uint32_t)__popc(gm & ((1u << g) - 1u));
       if (pass && (slot < Ku || kc.uncapped)) {
-        if (slot < Ku) {
-          if (mode == MODE_BA) P.pairs4[(size_t)r * Ku + slot] = make_float4(y4.x, y4.y, y4.z, a);
-          else P.pairs[(size_t)r * P.cap_alloc + slot] = make_uint2(pidx, __float_as_uint(a));
-        }
+        if (slot < Ku) store_pair<NOBA>(P, mode, r, Ku, slot, pidx, y4, a);
         if (mode == MODE_ALIGN) {
           const float c0 = x1 * y2 - x2 * y1, c1 = x2 * y0 - x0 * y2, c2 = x0 * y1 - x1 * y0;
           out.acc[0] += (double)(c0 * a);
@@ -500,8 +522,7 @@ __device__ __forceinline__ void knn_row(const AssocProblem& P, const KConsts& kc
     const uint32_t gm = group_bits<G>(__ballot(pass));
     const uint32_t slot = stored + (uint32_t)__popc(gm & ((1u << g) - 1u));
     if (pass && slot < Ku) {
-      if (mode == MODE_BA) P.pairs4[(size_t)r * Ku + slot] = make_float4(y4.x, y4.y, y4.z, a);
-      else P.pairs[(size_t)r * P.cap_alloc + slot] = make_uint2(pidx, __float_as_uint(a));
+      store_pair(P, mode, r, Ku, slot, pidx, y4, a);
       if (mode == MODE_ALIGN) {
         const float c0 = x1 * y2 - x2 * y1, c1 = x2 * y0 - x0 * y2, c2 = x0 * y1 - x1 * y0;
         out.acc[0] += (double)(c0 * a);
@@ -577,8 +598,9 @@ __device__ __forceinline__ void assoc_block(const AssocProblem& P, const KConsts
       knn_row<FC, KC, G>(P, kc, ek, GL, mp, r, g, mode, ell, K, L.feat + grp * Dim<FC>::get(kc.F),
                          L.sem + grp * Dim<KC>::get(kc.Ks), hist + grp * 256, res);
     else
-      assoc_row<FC, KC, G>(P, kc, ek, GL, mp, r, g, mode, ell, K, L.feat + grp * Dim<FC>::get(kc.F),
-                           L.sem + grp * Dim<KC>::get(kc.Ks), res);
+      assoc_row<FC, KC, G, (ROWS == kMaxRowsPerBlock)>(P, kc, ek, GL, mp, r, g, mode, ell, K,
+                                                       L.feat + grp * Dim<FC>::get(kc.F),
+                                                       L.sem + grp * Dim<KC>::get(kc.Ks), res);
   }
 #pragma unroll
   for (int c = 0; c < 7; ++c) {
@@ -829,8 +851,7 @@ __device__ uint32_t capped_row_ball(const AssocProblem& P, const KConsts& kc, co
     const uint64_t m = __ballot(take);
     const uint32_t slot = stored + (uint32_t)__popcll(m & ((g == 0) ? 0ull : (~0ull >> (64 - g))));
     if (take && slot < K) {
-      if (mode == MODE_BA) P.pairs4[(size_t)r * K + slot] = make_float4(y4.x, y4.y, y4.z, a);
-      else P.pairs[(size_t)r * P.cap_alloc + slot] = make_uint2(pidx, __float_as_uint(a));
+      store_pair(P, mode, r, K, slot, pidx, y4, a);
       if (mode == MODE_ALIGN) {
         const float c0 = x1 * y2 - x2 * y1, c1 = x2 * y0 - x0 * y2, c2 = x0 * y1 - x1 * y0;
         acc[0] += (double)(c0 * a);
@@ -989,10 +1010,9 @@ __device__ __forceinline__ void overflow_body(const AssocProblem& P, const KCons
       before += (uint32_t)__popcll(m & ((lane == 0) ? 0ull : (~0ull >> (64 - lane))));
       const uint32_t slot = stored + before;
       if (pass && slot < K) {
-        if (mode == MODE_BA)
-          P.pairs4[(size_t)r * K + slot] =
-              make_float4(P.tgt_xyz[3 * (size_t)j], P.tgt_xyz[3 * (size_t)j + 1], P.tgt_xyz[3 * (size_t)j + 2], a);
-        else P.pairs[(size_t)r * P.cap_alloc + slot] = make_uint2(G.inv_perm[j], __float_as_uint(a));
+        const float4 y4 =
+            make_float4(P.tgt_xyz[3 * (size_t)j], P.tgt_xyz[3 * (size_t)j + 1], P.tgt_xyz[3 * (size_t)j + 2], 0.0f);
+        store_pair(P, mode, r, K, slot, G.inv_perm[j], y4, a);
         if (mode == MODE_ALIGN) {
           const float c0 = x1 * y2 - x2 * y1, c1 = x2 * y0 - x0 * y2, c2 = x0 * y1 - x1 * y0;
           acc[0] += (double)(c0 * a);
@@ -1705,10 +1725,15 @@ __global__ __launch_bounds__(64) void export_rows(const AssocProblem* __restrict
   const bool by_value = sort_threshold > 0 && cnt > (uint32_t)sort_threshold;
   uint32_t n2 = 1;
   while (n2 < cnt) n2 <<= 1;
+  // bundle adjustment rows: (target coordinates, a) at the pass's row stride
+  // K, the sorted target positions beside them
+  const bool ba_rows = P->pairs4 != nullptr;
+  const size_t row0 = ba_rows ? (size_t)r * (size_t)P->st->K : (size_t)r * P->cap_alloc;
   for (uint32_t s = threadIdx.x; s < n2; s += 64) {
     uint64_t key = ~0ull;
     if (s < cnt) {
-      const uint2 e = P->pairs[(size_t)r * P->cap_alloc + s];
+      const uint2 e = ba_rows ? make_uint2(P->pair_ids[row0 + s], __float_as_uint(P->pairs4[row0 + s].w))
+                              : P->pairs[row0 + s];
       const uint32_t id = __float_as_uint(G.pos[e.x].w);
       key = by_value ? (((uint64_t)(~e.y)) << 32) | id : (((uint64_t)id) << 32) | e.y;
     }

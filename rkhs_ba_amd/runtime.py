@@ -63,7 +63,14 @@ def lib() -> C.CDLL:
         L.cvo_align.argtypes = [P, P, P, P, P, P, P, P, P]
         L.cvo_align_d.argtypes = [P, P, P, P, P, P, P, P, P]
         L.cvo_align_multistart.argtypes = [P, P, P, i32, P, f, i32, P, P, P, P, P, P]
-        L.cvo_align_multiframe.argtypes = [P, i32, P, P, P, i32, P, P, P, P, i32]
+        L.cvo_align_multiframe.argtypes = [P, i32, P, P, P, i32, P, P, P, P, P, i32]
+        L.cvo_align_s2t.argtypes = [P, P, P, P, P, P, P, P, P]
+        L.cvo_align_s2t_d.argtypes = [P, P, P, P, P, P, P, P, P]
+        L.cvo_get_nearest_init_pose.argtypes = [P, P, P, f, i32, P, P]
+        L.cvo_get_nearest_init_pose_d.argtypes = [P, P, P, f, i32, P, P]
+        L.cvo_icosahedral_rotations.argtypes = [P]
+        L.cvo_inner_product_cpu.argtypes = [P, P, P, P, f, P]
+        L.cvo_function_angle_ex.argtypes = [P, P, P, P, f, i32, i32, P]
         L.cvo_set_pose_arithmetic.argtypes = [P, i32]
         L.cvo_workspace_bytes.argtypes = [P, P]
         L.cvo_set_workspace_limit.argtypes = [P, C.c_uint64]
@@ -319,16 +326,50 @@ class CvoGPU:
 
     def align_result(self, source, target, init_guess_T_s2t, is_returning_association: bool = False) -> CvoResultInfo:
         """CvoResultInfo align(source, target, T_init_s2t, is_returning_association)
-        (CvoGPU.cu:1566-1611): inverts its init; -1 on empty input."""
+        (CvoGPU.cu:1566-1611): starts from the inverse of its init, or with
+        is_global_angle_registration from get_nearest_init_pose's result
+        itself; return_code -1 on empty input."""
         res = CvoResultInfo()
-        if getattr(source, "n", 0) == 0 or getattr(target, "n", 0) == 0:
+        T0 = mat16(init_guess_T_s2t)
+        Tout = np.zeros(16, np.float32)
+        secs = C.c_double()
+        iters = C.c_int32()
+        ret = C.c_int32()
+        a = Assoc()
+        ap = C.byref(a) if is_returning_association else None
+        if isinstance(source, DeviceCloud):
+            st = lib().cvo_align_s2t_d(self._ctx, source.handle, target.handle, T0.ctypes.data, Tout.ctypes.data, ap,
+                                       C.byref(secs), C.byref(iters), C.byref(ret))
+        else:
+            hs, ht = HeldView(source), HeldView(target)
+            st = lib().cvo_align_s2t(self._ctx, hs.ref, ht.ref, T0.ctypes.data, Tout.ctypes.data, ap,
+                                     C.byref(secs), C.byref(iters), C.byref(ret))
+        if st == 2:
             res.return_code = -1
             return res
-        T0 = np.linalg.inv(np.asarray(init_guess_T_s2t, np.float64)).astype(np.float32)
-        ret, T, secs, iters, assoc = self.align(source, target, T0, association=is_returning_association)
-        res.T_s2t, res.num_iters, res.registration_seconds, res.return_code = T, iters, secs, ret
-        res.association = assoc
+        _check(st, self._ctx)
+        res.T_s2t, res.num_iters, res.registration_seconds, res.return_code = (Tout.reshape(4, 4), iters.value,
+                                                                                secs.value, ret.value)
+        if is_returning_association and a.row_ptr:
+            res.association = _take_assoc(a)
         return res
+
+    def get_nearest_init_pose(self, source, target, init_guess_ell: float, num_guess: int = 0):
+        """get_nearest_init_pose (CvoGPU.cpp:367-398): the rotation of the
+        60-rotation table with the largest approximate cos angle (first strict
+        maximum above 0, else the identity) as a 4x4 source-to-target transform.
+        Returns (T, seconds)."""
+        T = np.zeros(16, np.float32)
+        secs = C.c_double(0.0)
+        if isinstance(source, DeviceCloud):
+            st = lib().cvo_get_nearest_init_pose_d(self._ctx, source.handle, target.handle, C.c_float(init_guess_ell),
+                                                   int(num_guess), T.ctypes.data, C.byref(secs))
+        else:
+            hs, ht = HeldView(source), HeldView(target)
+            st = lib().cvo_get_nearest_init_pose(self._ctx, hs.ref, ht.ref, C.c_float(init_guess_ell), int(num_guess),
+                                                 T.ctypes.data, C.byref(secs))
+        _check(st, self._ctx)
+        return T.reshape(4, 4), secs.value
 
     def align_multistart(self, source, target, T_starts, score_ell: float, mode: int = 0) -> dict:
         """Batched global registration: every start in T_starts (n,4,4, align
@@ -360,12 +401,15 @@ class CvoGPU:
         return {"T_best": T_best.reshape(4, 4), "best_index": best.value, "per_start_T": per_T.reshape(n, 4, 4),
                 "per_start_cos": per_cos, "per_start_iters": per_it, "seconds": secs.value}
 
-    def align_multiframe(self, frames, poses, hold_const=None, edges=(), max_records: int = 1024) -> dict:
+    def align_multiframe(self, frames, poses, hold_const=None, edges=(), max_records: int = 1024,
+                         inliers: bool = False) -> dict:
         """IRLS bundle adjustment of a window of frames (CvoGPU::align over
-        frames, CvoGPU.cu:1614-1671; CvoBatchIRLS::solve, IRLS.cpp:148-367).
-        frames: clouds (sensor frame) or DeviceClouds; poses: (n,3,4) or (n,4,4)
-        world <- sensor; edges: (E,2) frame index pairs.  Returns the updated
-        poses (n,4,4), the outer iteration count and the timings."""
+        frames, CvoGPU.cu:1614-1671, :1955-1999; CvoBatchIRLS::solve,
+        IRLS.cpp:148-367).  frames: clouds (sensor frame) or DeviceClouds;
+        poses: (n,3,4) or (n,4,4) world <- sensor; edges: (E,2) frame index
+        pairs.  Returns the updated poses (n,4,4), the outer iteration count,
+        the timings and, with inliers, each edge's Association of the last
+        association pass (BinaryState::export_association)."""
         own = []
         dev = []
         for fr in frames:
@@ -385,18 +429,31 @@ class CvoGPU:
         secs = C.c_double()
         it = C.c_int32()
         per = np.zeros(max(1, max_records), np.float64)
+        n_e = E.shape[0]
+        per_edge = (Assoc * max(1, n_e))() if inliers else None
         try:
             _check(lib().cvo_align_multiframe(self._ctx, n, handles, P.ctypes.data,
-                                              hc.ctypes.data if hc is not None else None, E.shape[0],
-                                              E.ctypes.data if E.size else None, C.byref(secs), C.byref(it),
-                                              per.ctypes.data, max_records), self._ctx)
+                                              hc.ctypes.data if hc is not None else None, n_e,
+                                              E.ctypes.data if E.size else None, per_edge, C.byref(secs),
+                                              C.byref(it), per.ctypes.data, max_records), self._ctx)
         finally:
             for c in own:
                 c.free()
         out = np.tile(np.eye(4), (n, 1, 1))
         out[:, :3, :4] = P.reshape(n, 3, 4)
-        return {"poses": out, "iters": it.value, "seconds": secs.value,
-                "per_iter_seconds": per[:min(it.value + 1, max_records)]}
+        r = {"poses": out, "iters": it.value, "seconds": secs.value,
+             "per_iter_seconds": per[:min(it.value + 1, max_records)]}
+        if inliers:
+            r["inliers"] = []
+            for e in range(n_e):
+                a = per_edge[e]
+                if a.row_ptr:
+                    r["inliers"].append(_take_assoc(a))
+                else:
+                    rows = dev[int(E[e, 0])].n
+                    r["inliers"].append(Association(rows, dev[int(E[e, 1])].n, np.zeros(rows + 1, np.int64),
+                                                    np.zeros(0, np.int32), np.zeros(0, np.float32)))
+        return r
 
     def align_frames(self, frames, hold_const, edges) -> float:
         """CvoGPU::align(std::vector<CvoFrame::Ptr>&, hold_const, edges, seconds)
@@ -428,14 +485,27 @@ class CvoGPU:
 
     def function_angle(self, source, target, t2s_frame_transform, ell: float, is_approximate: bool = True,
                        is_gpu: bool = True) -> float:
-        """function_angle (CvoGPU.cu:1768-1806).  is_gpu is accepted for
-        signature parity; the computation always runs on the GPU."""
+        """function_angle (CvoGPU.cu:1768-1806).  is_gpu False gives the
+        inner_product_cpu semantics (fixed lengthscale, no row cap, no gates
+        but the radius); both run on the GPU."""
         T0 = mat16(t2s_frame_transform)
         hs, ht = HeldView(source), HeldView(target)
         out = C.c_double()
-        _check(lib().cvo_function_angle(self._ctx, hs.ref, ht.ref, T0.ctypes.data, C.c_float(ell),
-                                        int(bool(is_approximate)), C.byref(out)), self._ctx)
+        _check(lib().cvo_function_angle_ex(self._ctx, hs.ref, ht.ref, T0.ctypes.data, C.c_float(ell),
+                                           int(bool(is_approximate)), int(bool(is_gpu)), C.byref(out)), self._ctx)
         return out.value
+
+    def inner_product_cpu(self, source, target, t2s_frame_transform, ell: float) -> float:
+        """inner_product_cpu (CvoGPU.cpp:204-231), computed on the GPU with
+        that function's semantics: the target moved by the inverse transform,
+        every pair inside the radius at the fixed lengthscale, no row cap and
+        no colour / semantic / geometric-type gate."""
+        T0 = mat16(t2s_frame_transform)
+        hs, ht = HeldView(source), HeldView(target)
+        ip = C.c_double()
+        _check(lib().cvo_inner_product_cpu(self._ctx, hs.ref, ht.ref, T0.ctypes.data, C.c_float(ell), C.byref(ip)),
+               self._ctx)
+        return ip.value
 
     def compute_association_gpu(self, source, target, T_target_frame_to_source_frame, lengthscale) -> Association:
         """compute_association_gpu (CvoGPU.cu:1836-1871); with a 3x3 matrix in
@@ -557,3 +627,12 @@ class CvoGPU:
 
 def version() -> str:
     return lib().cvo_version().decode()
+
+
+def icosahedral_rotations() -> np.ndarray:
+    """The 60 rotations of get_nearest_init_pose's table
+    (gen_icosahedron_init_rots, CvoGPU.cpp:279-342) in the table's order,
+    (60, 3, 3) float32, as the library holds them (no device needed)."""
+    out = np.zeros(540, np.float32)
+    _check(lib().cvo_icosahedral_rotations(out.ctypes.data))
+    return out.reshape(60, 3, 3)

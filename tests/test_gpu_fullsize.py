@@ -1,0 +1,91 @@
+"""Parity at the full street size: 60,000 / 60,000 points, 5 features, 19
+classes, the generating pose of seed 0 (the registration workload the
+benchmark times).
+
+  * association at lengthscales 0.5, 0.15 and 0.05 with the workload's row cap
+    (512), and with caps 1, 7 and 100 (thousands of capped rows): the rule of
+    tests/assoc_compare.py (exact sets except near-cutoff pairs, a capped row
+    holds exactly the first K passing targets by index);
+  * the registration in the reference's float pose arithmetic against the
+    FP32-mirror oracle for 1, 5 and 20 iterations: per-iteration stored pairs
+    within 1e-5 relative and longest row within 1 (pairs at the cutoff flip
+    with last-bit pose differences), flow norm and step within 1e-5
+    relative, poses within 1e-6 rad / 1e-6 m.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from assoc_compare import compare_with_oracle
+from conftest import params_from_yaml, rot_dist_rad, trans_err
+from rkhs_ba_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def street60k():
+    return synth.street_pair(0, n=60000)
+
+
+@pytest.mark.parametrize("ell", [0.5, 0.15, 0.05])
+def test_street60k_association(cvo_gpu, oracle, street60k, ell):
+    p = params_from_yaml("street_60k.yaml")
+    cvo_gpu.write_params(p)
+    src, tgt, Tgt = street60k
+    say(f"street 60k association ell={ell}")
+    a = cvo_gpu.compute_association_gpu(src, tgt, Tgt, ell)
+    out = compare_with_oracle(oracle, p, src, tgt, Tgt, ell, a)
+    say(f"  {out}")
+    assert out["common"] > {0.5: 1_000_000, 0.15: 200_000, 0.05: 30_000}[ell]
+
+
+@pytest.mark.parametrize("K,ell", [(1, 0.5), (7, 0.5), (100, 1.0)])
+def test_street60k_capped_rows(cvo_gpu, oracle, street60k, K, ell):
+    p = params_from_yaml("street_60k.yaml")
+    p.nearest_neighbors_max = K
+    cvo_gpu.write_params(p)
+    src, tgt, Tgt = street60k
+    say(f"street 60k association cap={K} ell={ell}")
+    a = cvo_gpu.compute_association_gpu(src, tgt, Tgt, ell)
+    counts = np.diff(a.row_ptr)
+    assert counts.max() == K and (counts == K).sum() > 1000
+    out = compare_with_oracle(oracle, p, src, tgt, Tgt, ell, a, K=K)
+    say(f"  {out}")
+    assert out["capped_rows"] > 1000
+
+
+def test_street60k_registration_follows_the_mirror(cvo_gpu, oracle, street60k):
+    src, tgt, _ = street60k
+    p = params_from_yaml("street_60k.yaml")
+    cvo_gpu.set_pose_arithmetic("reference")
+    try:
+        for n in (1, 5, 20):
+            p.MAX_ITER = n
+            cvo_gpu.write_params(p)
+            cvo_gpu.set_trace(n + 1)
+            say(f"street 60k registration, reference pose arithmetic, {n} iterations")
+            ret, T, _, it, _ = cvo_gpu.align(src, tgt, np.eye(4))
+            tr = cvo_gpu.get_trace()
+            To, it_o, ret_o, tro = oracle.align(p, src, tgt, np.eye(4), trace_cap=n + 1)
+            assert (ret, it) == (ret_o, it_o)
+            for k, (g, o) in enumerate(zip(tr[:n], tro[:n])):
+                # the mirror sums each row's flow in FP32, the library every
+                # pair in FP64, so the float poses part in their last bits
+                # and a pair at the cutoff may flip: stored pairs within
+                # 1e-5 relative, longest row within 1
+                assert abs(g.nnz - o.nnz) <= max(2, 1e-5 * o.nnz), (k, g.nnz, o.nnz)
+                assert abs(g.max_row - o.max_row) <= 1 and abs(g.num_neighbors - o.num_neighbors) <= 1, k
+                assert g.flow_norm == pytest.approx(o.flow_norm, rel=1e-5), k
+                assert g.step == pytest.approx(o.step, rel=1e-5), k
+            say(f"  pose diff {rot_dist_rad(T, To):.2e} rad {trans_err(T, To):.2e} m")
+            assert rot_dist_rad(T, To) <= 1e-6 and trans_err(T, To) <= 1e-6, n
+    finally:
+        cvo_gpu.set_trace(0)
+        cvo_gpu.set_pose_arithmetic("fp64")

@@ -3,8 +3,10 @@
 Bars (written in each test):
   * association: identical (row, target) sets except pairs whose oracle value is
     within 1e-6 relative of sp_thres or whose distance is within 1e-6 relative
-    of a gate; stored values within 4e-6 relative (one FP32 exp of the summed
-    exponent vs the reference's three FP64 exps rounded to float);
+    of a gate, and a capped row holds exactly the first K passing targets by
+    index (tests/assoc_compare.py); stored values within 4e-6 relative (one
+    FP32 exp of the summed exponent vs the reference's three FP64 exps
+    rounded to float);
   * inner product: 1e-5 relative of the FP64 oracle;
   * align: final pose against the FP32-mirror oracle and the generating pose.
 """
@@ -14,6 +16,7 @@ import sys
 import numpy as np
 import pytest
 
+from assoc_compare import compare_with_oracle
 from conftest import rot_dist_rad, CONFIGS, params_from_yaml, rot_err_deg, trans_err
 from rkhs_ba_amd import synth
 
@@ -26,33 +29,11 @@ def say(msg):
 
 
 def compare_assoc(oracle, params, src, tgt, T, ell, gpu_assoc, K=None, val_rtol=4e-6):
-    K = K or params.nearest_neighbors_max
-    counts, cols, vals, _ = oracle.assoc(params, src, tgt, T, ell, K)
-    want = {}
-    for i in range(src.n):
-        for s in range(counts[i]):
-            want[(i, int(cols[i, s]))] = float(vals[i, s])
-    rows, gcol, gval = gpu_assoc.pairs()
-    got = {(int(r), int(c)): float(v) for r, c, v in zip(rows, gcol, gval)}
-    diff = set(got) ^ set(want)
-    bad = []
-    for (i, j) in diff:
-        a, margin = oracle.pair_value(params, src, tgt, T, ell, i, j)
-        near_sp = a > 0 and abs(a - params.sp_thres) <= 1e-6 * params.sp_thres
-        if not (near_sp or margin <= 1e-6):
-            bad.append((i, j, a, margin, (i, j) in got))
-    # rows that hit the cap keep the first K by target index: a near-cutoff
-    # pair may shift which K are kept; only exempt rows at the cap
-    bad = [b for b in bad if counts[b[0]] < K]
-    assert not bad, bad[:10]
-    common = set(got) & set(want)
-    rel = max((abs(got[k] - want[k]) / want[k] for k in common), default=0.0)
-    assert rel <= val_rtol, rel
-    # the GPU CSR is sorted by target id within each row
-    for i in range(src.n):
-        seg = gcol[gpu_assoc.row_ptr[i]:gpu_assoc.row_ptr[i + 1]]
-        assert (np.diff(seg) > 0).all()
-    return len(common), len(diff), rel
+    """The association rule of tests/assoc_compare.py: exact sets except
+    near-cutoff pairs; a capped row keeps exactly the first K passing targets
+    by index (checked target by target where the row differs)."""
+    out = compare_with_oracle(oracle, params, src, tgt, T, ell, gpu_assoc, K=K, val_rtol=val_rtol)
+    return out["common"], out["diff"], out["rel"]
 
 
 def test_association_planes_spheres_fixed_poses(cvo_gpu, oracle):
@@ -190,6 +171,35 @@ def test_reference_pose_arithmetic_follows_the_mirror_trajectory(cvo_gpu, oracle
             assert (ret, it) == (ret_o, it_o)
             assert close_pose(T, To, 1e-6), (n, rot_dist_rad(T, To), trans_err(T, To))
     finally:
+        cvo_gpu.set_pose_arithmetic("fp64")
+
+
+def test_capped_rows_inside_align_follow_the_mirror(cvo_gpu, oracle):
+    # a small nearest_neighbors_max keeps rows over the cap in every
+    # iteration (K = min(8, int(1.2 max_row)), CvoGPU.cu:1431), so the capped
+    # path (first K passing targets by index, :994-996) feeds the flow and
+    # step terms: per-iteration stored pairs and longest row identical to the
+    # FP32 mirror, poses within 1e-6 after 1, 5 and 20 iterations
+    p = params_from_yaml("planes_spheres_5k.yaml")
+    p.nearest_neighbors_max = 8
+    src, tgt, _ = synth.planes_spheres_pair(2, 3000, 3000)
+    cvo_gpu.set_pose_arithmetic("reference")
+    try:
+        for n in (1, 5, 20):
+            p.MAX_ITER = n
+            cvo_gpu.write_params(p)
+            cvo_gpu.set_trace(n + 1)
+            say(f"capped rows in align, {n} iterations")
+            ret, T, _, it, _ = cvo_gpu.align(src, tgt, np.eye(4))
+            tr = cvo_gpu.get_trace()
+            To, it_o, ret_o, tro = oracle.align(p, src, tgt, np.eye(4), trace_cap=n + 1)
+            assert (ret, it) == (ret_o, it_o)
+            for k, (g, o) in enumerate(zip(tr[:n], tro[:n])):
+                assert (g.nnz, g.max_row, g.num_neighbors) == (o.nnz, o.max_row, o.num_neighbors), k
+                assert g.max_row == g.num_neighbors  # rows at the cap in every iteration
+            assert close_pose(T, To, 1e-6), (n, rot_dist_rad(T, To), trans_err(T, To))
+    finally:
+        cvo_gpu.set_trace(0)
         cvo_gpu.set_pose_arithmetic("fp64")
 
 
