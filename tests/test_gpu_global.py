@@ -89,44 +89,57 @@ def test_nearest_init_pose_identity_when_nothing_overlaps(cvo_gpu):
 
 
 def test_align_with_global_angle_registration(cvo_gpu, oracle):
-    p = params_from_yaml("planes_spheres_5k.yaml")
-    p.is_global_angle_registration = 1
-    p.ell_init_first_frame = 0.5
-    p.MAX_ITER = 600
-    cvo_gpu.write_params(p)
+    # 20 iterations from the searched start follow the oracle's registration
+    # from the same start (the int overload inverts the searched pose -- a
+    # general 4x4 inverse rounded to float, like Eigen's -- the CvoResultInfo
+    # overload does not); 600 iterations undo the rotation.  (This scene
+    # amplifies a 3e-8 change of the start to 2e-4 m by iteration 50, so the
+    # comparison stops at 20.)
     k = 7
     src, tgt, R = rotated_pair(k)
     table = icosahedral_rotations()
     G = np.eye(4, dtype=np.float32)
     G[:3, :3] = table[k]
+    Gi = np.linalg.inv(G.astype(np.float64)).astype(np.float32)
+    p0 = params_from_yaml("planes_spheres_5k.yaml")
+    p0.MAX_ITER = 20
+    p = params_from_yaml("planes_spheres_5k.yaml")
+    p.is_global_angle_registration = 1
+    p.ell_init_first_frame = 0.5
+    p.MAX_ITER = 20
+    cvo_gpu.write_params(p)
     say("align, int overload, global init")
     ret, T, secs, iters, _ = cvo_gpu.align(src, tgt, np.eye(4))
-    p0 = params_from_yaml("planes_spheres_5k.yaml")
-    p0.MAX_ITER = 600
-    Ti = np.linalg.inv(G.astype(np.float64)).astype(np.float32)
-    To, it_o, ret_o, _ = oracle.align(p0, src, tgt, Ti, precision=oracle.POLICY)
+    To, it_o, ret_o, _ = oracle.align(p0, src, tgt, Gi, precision=oracle.POLICY)
     assert (ret, iters) == (ret_o, it_o)
-    assert rot_dist_rad(T, To) <= 1e-5 and trans_err(T, To) <= 1e-5, (rot_dist_rad(T, To), trans_err(T, To))
-    # the registration undoes the table rotation
-    assert rot_dist_rad(T[:3, :3].T, R) < 0.02
+    assert rot_dist_rad(T, To) <= 1e-6 and trans_err(T, To) <= 1e-6, (rot_dist_rad(T, To), trans_err(T, To))
+    assert secs > 0.0
     say("align, CvoResultInfo overload, global init (not inverted)")
     res = cvo_gpu.align_result(src, tgt, np.eye(4))
     To2, it2, ret2, _ = oracle.align(p0, src, tgt, G, precision=oracle.POLICY)
     assert (res.return_code, res.num_iters) == (ret2, it2)
-    assert rot_dist_rad(res.T_s2t, To2) <= 1e-5 and trans_err(res.T_s2t, To2) <= 1e-5
+    assert rot_dist_rad(res.T_s2t, To2) <= 1e-6 and trans_err(res.T_s2t, To2) <= 1e-6
     assert res.registration_seconds > 0.0
+    p.MAX_ITER = 600
+    cvo_gpu.write_params(p)
+    ret, T, _, _, _ = cvo_gpu.align(src, tgt, np.eye(4))
+    assert rot_dist_rad(T[:3, :3].T, R) < 0.3  # the table rotation plus the pair's own 10 degrees
 
 
-def test_align_result_inverts_its_init(cvo_gpu, oracle):
+def test_align_result_inverts_its_init(cvo_gpu):
+    # an init whose inverse is exact in float: both overloads start from the
+    # same pose and give the same registration bit for bit
     p = params_from_yaml("planes_spheres_5k.yaml")
     p.MAX_ITER = 200
     cvo_gpu.write_params(p)
     src, tgt, Tgt = synth.planes_spheres_pair(4, 2000, 2000)
-    Ts2t = synth.rigid(synth.rot_axis_angle(np.array([0.3, 1.0, 0.2]), 0.05), np.array([0.05, -0.02, 0.01]))
+    Ts2t = np.array([[0, -1, 0, 0.25], [1, 0, 0, -0.5], [0, 0, 1, 0.125], [0, 0, 0, 1]], np.float32)
+    Tinv = np.array([[0, 1, 0, 0.5], [-1, 0, 0, 0.25], [0, 0, 1, -0.125], [0, 0, 0, 1]], np.float32)
+    tgt = tgt.transformed(np.linalg.inv(Ts2t.astype(np.float64)))
     res = cvo_gpu.align_result(src, tgt, Ts2t)
-    ret, T, _, iters, _ = cvo_gpu.align(src, tgt, np.linalg.inv(Ts2t))
+    ret, T, _, iters, _ = cvo_gpu.align(src, tgt, Tinv)
     assert (res.return_code, res.num_iters) == (ret, iters)
-    assert rot_dist_rad(res.T_s2t, T) <= 1e-6 and trans_err(res.T_s2t, T) <= 1e-6
+    assert np.array_equal(res.T_s2t, T)
     empty = synth.Cloud(np.zeros((0, 3), np.float32), np.zeros((0, 5), np.float32))
     assert cvo_gpu.align_result(src, empty, Ts2t).return_code == -1
 
