@@ -9,10 +9,10 @@ benchmark times).
   * the registration in the reference's float pose arithmetic against the
     FP32-mirror oracle for 1, 5 and 20 iterations: per-iteration stored pairs
     within 1e-5 relative and longest row within 1 (pairs at the cutoff flip
-    with last-bit pose differences), flow norm within 1e-5 relative and
-    step within 1e-4 relative (the step terms are FP64 sums of
-    float per-pair terms against the reference's float matrix powers), poses
-    within 1e-6 rad / 1e-6 m.
+    with last-bit pose differences), flow norm within 2e-4 relative (one
+    flipped pair moves it by ~4e-5) and step within 1e-4 relative (the step
+    terms are FP64 sums of float per-pair terms against the reference's float
+    matrix powers), poses within 1e-6 rad / 1e-6 m.
 """
 import sys
 
@@ -84,7 +84,8 @@ def test_street60k_registration_follows_the_mirror(cvo_gpu, oracle, street60k):
                 # 1e-5 relative, longest row within 1
                 assert abs(g.nnz - o.nnz) <= max(2, 1e-5 * o.nnz), (k, g.nnz, o.nnz)
                 assert abs(g.max_row - o.max_row) <= 1 and abs(g.num_neighbors - o.num_neighbors) <= 1, k
-                assert g.flow_norm == pytest.approx(o.flow_norm, rel=1e-5), k
+                # one pair at the cutoff moves the flow norm by ~4e-5 here
+                assert g.flow_norm == pytest.approx(o.flow_norm, rel=2e-4), k
                 # B..E: FP64 sums of float per-pair terms (library) against
                 # the reference's float matrix powers (mirror); the cubic's
                 # root amplifies their difference to a few 1e-5
