@@ -274,6 +274,19 @@ cvo_status cvo_inner_product(cvo_ctx* ctx, const cvo_cloud_view* src, const cvo_
                              const float T[16], float ell, double* ip);
 cvo_status cvo_inner_product_d(cvo_ctx* ctx, const cvo_dcloud* src, const cvo_dcloud* tgt,
                                const float T[16], float ell, double* ip);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): cvo_inner_product_d computed with the same kernels in the
+ * library's fixed per-GPU scratch array (64 MiB, a file-scope device array
+ * that each GPU the library runs on holds until the process ends) instead of
+ * the context's arena.  Calls no hipMalloc, hipFree or allocating library
+ * routine; the arena is left as it was.  At most 131072 points per cloud, and
+ * the pass must fit the scratch array (its grid is capped at 2^21 cells):
+ * CVO_ERR_LIMIT otherwise.  Every call on one GPU shares that GPU's array and
+ * keeps nothing from earlier calls: calls that overlap on one GPU (several
+ * host threads or contexts) race. */
+cvo_status cvo_inner_product_d_noalloc(cvo_ctx* ctx, const cvo_dcloud* src, const cvo_dcloud* tgt,
+                                       const float T[16], float ell, double* ip);
 /* CvoGPU::function_angle (src/cvo/CvoGPU.cu:1768-1806): approx != 0 gives
  * ip / (sqrt(N) sqrt(M)), otherwise ip / sqrt(ip_xx ip_yy). */
 cvo_status cvo_function_angle(cvo_ctx* ctx, const cvo_cloud_view* src, const cvo_cloud_view* tgt,

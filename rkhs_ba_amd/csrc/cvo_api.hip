@@ -16,6 +16,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -55,6 +56,19 @@ cvo_status hip_check(cvo_ctx* ctx, hipError_t e, const char* what) {
 cvo_status ensure(cvo_ctx* ctx, DevBuf& b, size_t bytes) {
   if (bytes == 0) bytes = 16;
   if (b.cap >= bytes) return CVO_OK;
+  if (ctx->bump) {
+    // carve from the call's fixed region; never allocate
+    BumpArena& a = *ctx->bump;
+    const size_t need = (bytes + 255) / 256 * 256;
+    if (a.used + need > a.size)
+      return set_err(ctx, CVO_ERR_LIMIT,
+                     "scratch: " + std::to_string(a.used + need) + " bytes needed, " + std::to_string(a.size) +
+                         " available");
+    b.p = a.base + a.used;
+    b.cap = need;
+    a.used += need;
+    return CVO_OK;
+  }
   const size_t cap = bytes + bytes / 4;
   if (ctx->ws_limit > 0 && ctx->ws_bytes - b.cap + cap > ctx->ws_limit)
     return set_err(ctx, CVO_ERR_LIMIT,
@@ -230,7 +244,8 @@ cvo_status build_grid_into(cvo_ctx* ctx, const cvo_dcloud* t, const std::vector<
     for (int d = 0; d < 3; ++d) ext[d] = (double)t->bmax[d] - (double)t->bmin[d] + 2.0 * pad;
     double cells = 1.0;
     for (int d = 0; d < 3; ++d) cells *= std::floor(ext[d] / h) + 1.0;
-    if (cells > (double)kMaxCells) h = (float)(h * std::cbrt(cells / (double)kMaxCells) * 1.01);
+    const double cell_cap = (double)ctx->grid_max_cells;
+    if (cells > cell_cap) h = (float)(h * std::cbrt(cells / cell_cap) * 1.01);
     lv.h = h;
     lv.inv_h = 1.0f / h;
     lv.ox = t->bmin[0] - pad;
@@ -636,6 +651,34 @@ extern "C" const char* cvo_status_string(cvo_status s) {
 
 extern "C" const char* cvo_last_error(const cvo_ctx* ctx) { return ctx ? ctx->last_error.c_str() : ""; }
 
+// ---------------------------------------------------------------------------
+// Scratch memory of the no-allocation entry point: a fixed device array per
+// GPU.  In host code its name stands for a host-side shadow, not a device
+// address; the device address comes from hipGetSymbolAddress, once per device
+// with that device current, and only that address is handed to the runtime
+// and the kernels.
+// ---------------------------------------------------------------------------
+constexpr size_t kScratchBytes = 64ull << 20;
+constexpr int kNoAllocMaxPoints = 1 << 17;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) char scratch[kScratchBytes];
+static std::mutex g_scratch_mu;
+static char* g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of this device's scratch array (ctx->device current).
+static cvo_status scratch_address(cvo_ctx* ctx) {
+  if (ctx->device < 0 || ctx->device >= kMaxDevices) return set_err(ctx, CVO_ERR_LIMIT, "device ordinal above 63");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[ctx->device]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) return hip_check(ctx, e != hipSuccess ? e : hipErrorInvalidSymbol, "hipGetSymbolAddress(scratch)");
+    g_scratch_ptr[ctx->device] = static_cast<char*>(ptr);
+  }
+  ctx->scratch = g_scratch_ptr[ctx->device];
+  return CVO_OK;
+}
+
 extern "C" cvo_status cvo_create_from_params(const cvo_params* p, cvo_ctx** out) {
   if (!p || !out) return CVO_ERR_INVALID_ARG;
   *out = nullptr;
@@ -648,6 +691,11 @@ extern "C" cvo_status cvo_create_from_params(const cvo_params* p, cvo_ctx** out)
     return CVO_ERR_HIP;
   }
   if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete ctx;
+    return CVO_ERR_HIP;
+  }
+  if (scratch_address(ctx) != CVO_OK) {
+    (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return CVO_ERR_HIP;
   }
@@ -686,17 +734,21 @@ static void free_buf(cvo_ctx* ctx, DevBuf& b) {
 }
 
 // Every arena buffer of a context.
+static std::vector<DevBuf*> all_bufs(cvo_ctx* ctx) {
+  std::vector<DevBuf*> v = {&ctx->b_src_pos, &ctx->b_src_feat, &ctx->b_src_sem, &ctx->b_src_geo, &ctx->b_src_key,
+                            &ctx->b_src_perm, &ctx->b_grid_store, &ctx->b_pairs, &ctx->b_counts, &ctx->b_ovf,
+                            &ctx->b_state, &ctx->b_probs, &ctx->b_gridhdr, &ctx->b_part_a, &ctx->b_part_ovf,
+                            &ctx->b_part_s, &ctx->b_part_cnt, &ctx->b_trace, &ctx->b_tmp0, &ctx->b_tmp1,
+                            &ctx->b_tmp2, &ctx->b_tmp3, &ctx->b_cub, &ctx->b_export};
+  for (auto& b : ctx->b_ba) v.push_back(&b);
+  for (auto& b : ctx->b_frames) v.push_back(&b);
+  for (auto& b : ctx->b_mlp) v.push_back(&b);
+  for (auto& b : ctx->b_acvo) v.push_back(&b);
+  return v;
+}
+
 static void for_each_buf(cvo_ctx* ctx, void (*fn)(cvo_ctx*, DevBuf&)) {
-  DevBuf* bufs[] = {&ctx->b_src_pos, &ctx->b_src_feat, &ctx->b_src_sem, &ctx->b_src_geo, &ctx->b_src_key,
-                    &ctx->b_src_perm, &ctx->b_grid_store, &ctx->b_pairs, &ctx->b_counts, &ctx->b_ovf,
-                    &ctx->b_state, &ctx->b_probs, &ctx->b_gridhdr, &ctx->b_part_a, &ctx->b_part_ovf,
-                    &ctx->b_part_s, &ctx->b_part_cnt, &ctx->b_trace, &ctx->b_tmp0, &ctx->b_tmp1,
-                    &ctx->b_tmp2, &ctx->b_tmp3, &ctx->b_cub, &ctx->b_export};
-  for (DevBuf* b : bufs) fn(ctx, *b);
-  for (auto& b : ctx->b_ba) fn(ctx, b);
-  for (auto& b : ctx->b_frames) fn(ctx, b);
-  for (auto& b : ctx->b_mlp) fn(ctx, b);
-  for (auto& b : ctx->b_acvo) fn(ctx, b);
+  for (DevBuf* b : all_bufs(ctx)) fn(ctx, *b);
 }
 
 extern "C" void cvo_destroy(cvo_ctx* ctx) {
@@ -1638,5 +1690,80 @@ extern "C" cvo_status cvo_release_workspace(cvo_ctx* ctx) {
   if (!ctx) return CVO_ERR_INVALID_ARG;
   CVO_HIP(ctx, hipStreamSynchronize(ctx->stream));
   for_each_buf(ctx, free_buf);
+  return CVO_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Inner product without device allocation
+// ---------------------------------------------------------------------------
+
+// For the duration of one call every arena buffer of the context is swapped
+// for an empty one and ensure() carves from the device's scratch array; the
+// arena buffers come back untouched afterwards.  The grid of the call is
+// capped at 2^21 cells (larger cells, the same pairs).
+struct ScratchScope {
+  cvo_ctx* ctx;
+  BumpArena arena;
+  std::vector<DevBuf*> bufs;
+  std::vector<DevBuf> saved;
+  uint32_t cells;
+  explicit ScratchScope(cvo_ctx* c) : ctx(c) {
+    arena.base = ctx->scratch;
+    arena.size = kScratchBytes;
+    arena.used = 0;
+    bufs = all_bufs(ctx);
+    saved.reserve(bufs.size());
+    for (DevBuf* b : bufs) {
+      saved.push_back(*b);
+      *b = DevBuf{};
+    }
+    cells = ctx->grid_max_cells;
+    ctx->grid_max_cells = 1u << 21;
+    ctx->bump = &arena;
+  }
+  ~ScratchScope() {
+    for (size_t i = 0; i < bufs.size(); ++i) *bufs[i] = saved[i];
+    ctx->grid_max_cells = cells;
+    ctx->bump = nullptr;
+  }
+};
+
+extern "C" cvo_status cvo_inner_product_d_noalloc(cvo_ctx* ctx, const cvo_dcloud* src, const cvo_dcloud* tgt,
+                                                  const float T[16], float ell, double* ip) {
+  if (!ctx || !T || !ip) return CVO_ERR_INVALID_ARG;
+  *ip = 0.0;
+  if (!src || !tgt || src->n == 0 || tgt->n == 0) return set_err(ctx, CVO_ERR_EMPTY, "empty point cloud");
+  if (!(ell > 0.0f)) return set_err(ctx, CVO_ERR_INVALID_ARG, "ell must be positive");
+  if (src->n > kNoAllocMaxPoints || tgt->n > kNoAllocMaxPoints)
+    return set_err(ctx, CVO_ERR_LIMIT, "cvo_inner_product_d_noalloc takes at most 131072 points per cloud");
+  if (!ctx->scratch) return set_err(ctx, CVO_ERR_HIP, "no scratch address on this device");
+  CVO_TRY(validate_pair(ctx, src, tgt));
+  ScratchScope scope(ctx);
+  const cvo_params& p = ctx->params;
+  KConsts kc = make_kconsts(p, src->F, src->K, MODE_IP);
+  apply_tuning(ctx, kc);
+  kc.n_src_f = (float)src->n;
+  kc.n_tgt_f = (float)tgt->n;
+  kc.max_src_norm = src->max_norm;
+  kc.max_iter = 1;
+  const int K0 = p.nearest_neighbors_max;
+  CVO_TRY(prepare_source(ctx, src));
+  const float tq = std::max(std::fabs(T[3]), std::max(std::fabs(T[7]), std::fabs(T[11])));
+  GridHost gh;
+  // one level: cells at least the ball radius
+  CVO_TRY(build_grid(ctx, tgt, level_sizes(kc, ell, ell, src->max_norm, tq, 1.0f), gh));
+  Problems pr;
+  CVO_TRY(setup_problems(ctx, src, tgt, gh, 1, 0, 0, kc.max_group, pr));
+  CVO_TRY(ensure(ctx, ctx->b_tmp2, 64 * sizeof(float)));
+  hipStream_t st = ctx->stream;
+  CVO_HIP(ctx, hipMemcpyAsync(ctx->b_tmp2.p, T, sizeof(float) * 16, hipMemcpyHostToDevice, st));
+  launch_init_state(pr.dev, 1, as<float>(ctx->b_tmp2), 16, kc, ell, K0, st);
+  launch_assoc(pr.dev, 1, pr.total_blocks, kc, st, pr.host.data());
+  launch_overflow(pr.dev, 1, kc, st, pr.host.data());
+  CVO_HIP(ctx, hipGetLastError());
+  ProblemState hs;
+  CVO_HIP(ctx, hipMemcpyAsync(&hs, pr.states, sizeof(ProblemState), hipMemcpyDeviceToHost, st));
+  CVO_HIP(ctx, hipStreamSynchronize(st));
+  *ip = hs.ip;
   return CVO_OK;
 }

@@ -244,6 +244,8 @@ struct cvo_ctx {
   BumpArena* bump = nullptr;
   // grid cell count limit of build_grid_into
   uint32_t grid_max_cells = 1u << 25;
+  // device address of this device's scratch array (no-allocation calls)
+  char* scratch = nullptr;
 };
 
 // Internal helpers shared by the translation units.

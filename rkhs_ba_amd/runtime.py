@@ -71,6 +71,7 @@ def lib() -> C.CDLL:
         L.cvo_icosahedral_rotations.argtypes = [P]
         L.cvo_inner_product_cpu.argtypes = [P, P, P, P, f, P]
         L.cvo_function_angle_ex.argtypes = [P, P, P, P, f, i32, i32, P]
+        L.cvo_inner_product_d_noalloc.argtypes = [P, P, P, P, f, P]
         L.cvo_set_pose_arithmetic.argtypes = [P, i32]
         L.cvo_workspace_bytes.argtypes = [P, P]
         L.cvo_set_workspace_limit.argtypes = [P, C.c_uint64]
@@ -494,6 +495,18 @@ class CvoGPU:
         _check(lib().cvo_function_angle_ex(self._ctx, hs.ref, ht.ref, T0.ctypes.data, C.c_float(ell),
                                            int(bool(is_approximate)), int(bool(is_gpu)), C.byref(out)), self._ctx)
         return out.value
+
+    def inner_product_noalloc(self, source: "DeviceCloud", target: "DeviceCloud", init_guess_transform,
+                              ell: float) -> float:
+        """inner_product_gpu on device-resident clouds without any device
+        allocation: the library's fixed per-GPU scratch array holds every
+        intermediate (at most 131072 points per cloud; calls that overlap on
+        one GPU race for it)."""
+        T0 = mat16(init_guess_transform)
+        ip = C.c_double()
+        _check(lib().cvo_inner_product_d_noalloc(self._ctx, source.handle, target.handle, T0.ctypes.data,
+                                                 C.c_float(ell), C.byref(ip)), self._ctx)
+        return ip.value
 
     def inner_product_cpu(self, source, target, t2s_frame_transform, ell: float) -> float:
         """inner_product_cpu (CvoGPU.cpp:204-231), computed on the GPU with
