@@ -43,7 +43,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per association launch on the 60k street workload, from rocprofv3
 # FETCH_SIZE / WRITE_SIZE passes scaled by a 256 MiB copy probe
 # (profiles/assoc_street60k_pmc_traffic.md)
-ASSOC_TRAFFIC_BYTES = 26.71e6
+ASSOC_TRAFFIC_BYTES = 25.79e6
 # per-launch work of the association kernel on the 60k street registration
 # with its autotuned launch shape, from rocprofv3 counters (SQ_INSTS_VALU_TRANS_F32
 # x 64 lanes; SQ_INSTS_VALU_FLOPS_FP32 x 64): the transcendental and FP32
@@ -67,6 +67,33 @@ def pose_error(T, T_gt):
 
 def cpu_threads():
     return min(16, os.cpu_count() or 1)
+
+
+def cpu_host(threads):
+    """The host the CPU baselines ran on: threads used, CPU model, logical
+    CPUs, the oracle's compiler and flags."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    from oracle import pyoracle
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        affinity = None
+    return {"threads": threads, "cpu_model": model, "nproc": os.cpu_count(), "affinity": affinity,
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS"), "compiler": pyoracle.build_info()}
+
+
+def rot_trans_diff(A, B):
+    """Rotation angle (rad) and translation distance (m) between two poses."""
+    E = np.linalg.inv(np.asarray(A, np.float64)) @ np.asarray(B, np.float64)
+    return (float(np.arccos(np.clip((np.trace(E[:3, :3]) - 1) / 2, -1, 1))), float(np.linalg.norm(E[:3, 3])))
 
 
 # ---------------------------------------------------------------------------
@@ -151,6 +178,19 @@ def leg_twoview(args):
            "tuning": tuning}
     if args.profile_only:
         return out
+    # host to host: host clouds in (upload, source sort, grid build, the loop)
+    # and the pose out, the median over at least five registrations
+    n_h2h = max(5, args.steps)
+    log(f"[bench] two-view host-to-host: {n_h2h} registrations from host memory")
+    h2h = []
+    for _ in range(n_h2h):
+        t0 = time.perf_counter()
+        cvo.align(src, tgt, I)
+        h2h.append(time.perf_counter() - t0)
+    med = float(np.median(h2h))
+    out["host_to_host"] = {"ms_median": med * 1e3, "registrations_per_s": 1.0 / med, "runs": n_h2h,
+                           "ms_min": float(np.min(h2h)) * 1e3, "ms_max": float(np.max(h2h)) * 1e3,
+                           "note": "host clouds in, pose out: upload, sort, grid build, the loop, readback"}
     log("[bench] two-view profiled registration")
     cvo.reset_kernel_stats()
     cvo.set_profiling(True)
@@ -173,6 +213,8 @@ def leg_twoview(args):
                                           "frac": exp_rate / EXP_PEAK_PER_S, "per_launch": ASSOC_EXP_PER_LAUNCH},
                        "fp32": {"achieved_tflops": flop_rate, "peak_tflops": FP32_PEAK_TFLOPS,
                                 "frac": flop_rate / FP32_PEAK_TFLOPS, "per_launch": ASSOC_FP32_FLOP_PER_LAUNCH},
+                       "timing": "HIP events at the kernel's own start and end (hipExtLaunchKernelGGL), "
+                                 "the library's stream, every launch of one registration",
                        "note": "latency-bound (dependent gathers); per-launch counts from rocprofv3 counters"}
     out["kernels_ms_per_launch"] = {
         "assoc": assoc_ms,
@@ -187,7 +229,22 @@ def leg_twoview(args):
             "sample": (f"oracle FP32-mirror iteration (grid radius search, OpenMP) timed {sampled} times over the "
                        f"{len(per_iter)} lengthscales of the GPU run, weighted by the GPU's iterations per "
                        f"lengthscale ({len(trace)} iterations); {spent:.1f} s of CPU work"),
-            "seconds_per_registration": est_s}
+            "seconds_per_registration": est_s, "host": cpu_host(threads)}
+    if args.cpu_full:
+        from oracle import pyoracle
+        log(f"[bench] two-view: one whole oracle registration in FP64 (the library's default pose arithmetic), "
+            f"OMP_NUM_THREADS={os.environ.get('OMP_NUM_THREADS')}")
+        t0 = time.perf_counter()
+        To, it_o, ret_o, _ = pyoracle.align(params, src, tgt, I, precision=pyoracle.POLICY)
+        spent = time.perf_counter() - t0
+        dr, dt_ = rot_trans_diff(T, To)
+        eo = pose_error(To, T_gt)
+        log(f"[bench] two-view oracle: {spent:.1f} s, {it_o} iterations; GPU vs oracle {dr:.2e} rad {dt_:.2e} m")
+        out["cpu_full_registration"] = {
+            "seconds": spent, "registrations_per_s": 1.0 / spent, "iterations": it_o, "ret": ret_o,
+            "precision": "FP64 pose arithmetic and sums (the library's default)",
+            "gpu_vs_oracle": {"rad": dr, "m": dt_}, "oracle_pose_error": {"deg": eo[0], "m": eo[1]},
+            "host": cpu_host(int(os.environ.get("OMP_NUM_THREADS") or (os.cpu_count() or 1)))}
     cvo.close()
     return out
 
@@ -278,6 +335,7 @@ def leg_ba(args):
         _, it, secs = pyoracle.align_multiframe(p1, clouds, inits, hold, edges, threads=threads)
         spent = time.perf_counter() - t0
         out["cpu_baseline"] = {"value": float(secs[0]), "unit": "s/iter", "cores": threads, "kind": "port",
+                               "host": cpu_host(threads),
                                "sample": f"oracle IRLS outer iteration 1 (association of all {len(edges)} edges "
                                          f"+ up to {p1.multiframe_iterations_per_solve} LM candidates), "
                                          f"{spent:.1f} s of CPU work",
@@ -288,7 +346,7 @@ def leg_ba(args):
         _, it, secs = pyoracle.align_multiframe_cpu_states(p1, clouds, inits, hold, edges, threads=threads)
         spent = time.perf_counter() - t0
         out["cpu_states_baseline"] = {
-            "value": float(secs[0]), "unit": "s/iter", "cores": threads, "kind": "port",
+            "value": float(secs[0]), "unit": "s/iter", "cores": threads, "kind": "port", "host": cpu_host(threads),
             "sample": ("oracle IRLS outer iteration 1 with BinaryStateCPU edges (IRLS_State_CPU.cpp: fixed-ell "
                        f"radius search of frame 2 over frame 1, weight ck k, no cap), {spent:.1f} s of CPU work")}
     cvo.close()
@@ -329,7 +387,11 @@ def leg_c1(args):
     eo = pose_error(To, T_gt)
     out["cpu_baseline"] = {"value": 1.0 / spent, "unit": "registrations/s", "cores": threads, "kind": "port",
                            "sample": f"one whole oracle FP32-mirror registration ({it_o} iterations)",
-                           "iterations": it_o, "pose_error": {"deg": eo[0], "m": eo[1]}}
+                           "iterations": it_o, "pose_error": {"deg": eo[0], "m": eo[1]}, "host": cpu_host(threads)}
+    # the GPU pose against the FP64 oracle (the library's default arithmetic)
+    Tp, it_p, _, _ = pyoracle.align(params, src, tgt, I, precision=pyoracle.POLICY)
+    dr, dt_ = rot_trans_diff(T, Tp)
+    out["gpu_vs_fp64_oracle"] = {"rad": dr, "m": dt_, "oracle_iterations": it_p, "gpu_iterations": iters}
     log(f"[bench] c1 cpu_cvo_legacy: the reference's CPU CVO on {threads} threads")
     lp = pyoracle.legacy_params(params)
     Tl, it_l, ret_l, secs_l, _ = pyoracle.cvo_legacy_align(lp, src, tgt, I, threads=threads)
@@ -337,6 +399,7 @@ def leg_c1(args):
     per_it = secs_l / max(it_l, 1)
     out["cpu_cvo_legacy"] = {
         "value": 1.0 / secs_l if secs_l > 0 else None, "unit": "registrations/s", "cores": threads, "kind": "port",
+        "host": cpu_host(threads),
         "sample": ("one whole run of the restated cvo::cvo::align (Cvo.cpp:885-1089) with the same YAML: "
                    "unnormalised flow, float steps; it stops on its own criteria"),
         "iterations": it_l, "seconds_per_iteration": per_it, "pose_error": {"deg": el[0], "m": el[1]},
@@ -380,6 +443,7 @@ def leg_multistart(args):
                                       params.nearest_neighbors_max, 3, threads)
         est = t1 * out["iterations_total"]
         out["cpu_baseline"] = {"value": 96 / est, "unit": "starts/s", "cores": threads, "kind": "port",
+                               "host": cpu_host(threads),
                                "sample": "oracle iteration at ell_init timed 3 times x the GPU run's total "
                                          "iterations over all 96 starts",
                                "seconds_estimate": est}
@@ -446,7 +510,8 @@ def leg_learned(args):
         cpu_s = (time.perf_counter() - t0) * 256 / 32
         out["cpu_baseline"] = {"value": 256 / cpu_s, "unit": "pairs/s (inference)", "cores": cpu_threads(),
                                "kind": "port", "sample": "numpy FP64 forward of the same network on 32 of the "
-                                                         "256 pairs, scaled to the batch"}
+                                                         "256 pairs, scaled to the batch",
+                               "host": cpu_host(cpu_threads())}
     prop.close()
     cvo.close()
     return out
@@ -536,6 +601,7 @@ def leg_knn(args):
 
         est, n_s, n_g, spent = cpu_by_lengthscale(trace, time_at)
         out["cpu_baseline"] = {"value": 1.0 / est, "unit": "registrations/s", "cores": threads, "kind": "port",
+                               "host": cpu_host(threads),
                                "sample": (f"oracle K-nearest association (FP32 mirror, OpenMP) timed at {n_s} of the "
                                           f"{n_g} (lengthscale, K) settings of the GPU run, weighted by its "
                                           f"iterations; step terms not counted; {spent:.1f} s of CPU work"),
@@ -589,7 +655,11 @@ def leg_adaptive(args):
         out["cpu_baseline"] = {"value": 1.0 / spent, "unit": "registrations/s", "cores": cpu_threads(),
                                "kind": "port", "sample": f"one whole oracle registration (FP32 mirror, OpenMP "
                                                          f"associations), {it_o} iterations",
-                               "iterations": it_o}
+                               "iterations": it_o, "host": cpu_host(cpu_threads())}
+        # the GPU pose against the FP64 oracle (the library's default arithmetic)
+        Tp, it_p, _, _ = pyoracle.align_adaptive(params, src, tgt, I, precision=pyoracle.POLICY)
+        dr, dt_ = rot_trans_diff(T, Tp)
+        out["gpu_vs_fp64_oracle"] = {"rad": dr, "m": dt_, "oracle_iterations": it_p, "gpu_iterations": iters}
     cvo.close()
     return out
 
@@ -643,6 +713,7 @@ def leg_aniso(args):
         pyoracle.assoc_nonisotropic(params, src, tgt, T_gt, ANISO_KERNEL)
         spent = time.perf_counter() - t0
         out["cpu_baseline"] = {"value": 1.0 / spent, "unit": "associations/s", "cores": cpu_threads(),
+                               "host": cpu_host(cpu_threads()),
                                "kind": "port", "sample": ("one whole oracle association (every target per row, "
                                                           f"OpenMP), {spent:.1f} s of CPU work"),
                                "seconds_per_association": spent}
@@ -679,6 +750,9 @@ def main():
     ap.add_argument("--legs", default="twoview,c1,ba,multistart,learned,knn,adaptive,aniso")
     ap.add_argument("--learned-pairs", type=int, default=4, help="pairs for learned vs grid registration")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
+    ap.add_argument("--cpu-full", action="store_true",
+                    help="also time one whole oracle registration of the 60k street pair in FP64 (minutes) and "
+                         "compare the GPU pose with it")
     ap.add_argument("--tune", choices=("auto", "default"), default="auto",
                     help="association launch shape: autotuned (cached per workload) or the library default")
     ap.add_argument("--tuning", default="",
@@ -750,7 +824,9 @@ def main():
         "roofline": tv.get("roofline"),
         "tuning": tv.get("tuning"),
         "kernels_ms_per_launch": tv.get("kernels_ms_per_launch"),
+        "host_to_host": tv.get("host_to_host"),
         "cpu_baseline": cpu,
+        "cpu_full_registration": tv.get("cpu_full_registration"),
         "speedup_vs_cpu": tv["value"] / cpu["value"] if cpu and cpu["value"] else None,
     }
     out.update(extra)

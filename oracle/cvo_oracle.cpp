@@ -2613,3 +2613,8 @@ extern "C" int orc_assoc_nonisotropic(const cvo_params* p, const cvo_cloud_view*
   }
   return 0;
 }
+
+#ifndef ORC_CXXFLAGS
+#define ORC_CXXFLAGS "(flags not recorded)"
+#endif
+extern "C" const char* orc_build_info(void) { return "g++ " __VERSION__ ", " ORC_CXXFLAGS; }

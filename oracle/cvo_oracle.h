@@ -231,6 +231,9 @@ int orc_pair_values(const cvo_params* p, const cvo_cloud_view* src, const cvo_cl
 double orc_inner_product_cpu(const cvo_params* p, const cvo_cloud_view* src, const cvo_cloud_view* tgt,
                              const float T[16], float ell, int32_t brute, int64_t* pairs_out);
 
+// Compiler and flags this library was built with (for benchmark reports).
+const char* orc_build_info(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -37,6 +37,7 @@ def lib():
         _lib = C.CDLL(_LIB_PATH)
         d, f, i32 = C.c_double, C.c_float, C.c_int32
         P = C.c_void_p
+        _lib.orc_build_info.restype = C.c_char_p
         _lib.orc_log_se3_norm.restype = d
         _lib.orc_select_step.restype = d
         _lib.orc_inner_product.restype = d
@@ -428,3 +429,8 @@ def assoc_nonisotropic(params, src, tgt, T, kernel, K=None):
                                       _p(vals))
     assert rc == 0, rc
     return counts, cols, vals
+
+
+def build_info():
+    """Compiler and flags of the oracle library."""
+    return lib().orc_build_info().decode()

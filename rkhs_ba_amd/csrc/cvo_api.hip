@@ -25,11 +25,14 @@
 namespace cvo {
 
 void launch_step(const AssocProblem* probs, int n_probs, int max_src_rows, const KConsts& kc, hipStream_t s,
-                 const AssocProblem* host_one);
+                 const AssocProblem* host_one, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 void launch_init_state(const AssocProblem* probs, int n_probs, const float* T16, int stride16, const KConsts& kc,
                        float ell0, int K0, hipStream_t s);
 void launch_grid_count(const float* xyz, int n, const GridLevel& L, uint32_t* key, uint32_t* count, hipStream_t s);
 void launch_iota(uint32_t* v, int n, hipStream_t s);
+// zero fill of bytes (a multiple of 4) by a kernel: no memset node in a
+// caller's graph capture
+void launch_zero(void* p, size_t bytes, hipStream_t s);
 void launch_gather(const uint32_t* order, int n, const float* xyz, const float* feat, int F, const float* sem, int K,
                    const float* geo, float4* pos, float* feat_s, float* sem_s, float* geo_s, uint32_t* inv_perm,
                    hipStream_t s);
@@ -277,7 +280,7 @@ cvo_status build_grid_into(cvo_ctx* ctx, const cvo_dcloud* t, const std::vector<
     GridLevel& lv = G.lv[l];
     uint32_t nc = (uint32_t)lv.nx * (uint32_t)lv.ny * (uint32_t)lv.nz;
     uint32_t* cell_start = reinterpret_cast<uint32_t*>(base + off_cells[l]);
-    CVO_HIP(ctx, hipMemsetAsync(counts, 0, sizeof(uint32_t) * (nc + 1), st));
+    launch_zero(counts, sizeof(uint32_t) * (nc + 1), st);
     launch_grid_count(t->xyz, n, lv, keys, counts, st);
     size_t tmp = 0;
     CVO_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, counts, cell_start, (int)nc + 1, st));
@@ -380,8 +383,8 @@ cvo_status setup_problems(cvo_ctx* ctx, const cvo_dcloud* s, const cvo_dcloud* t
     pr.trace = nullptr;
   }
   // ovf stamps start at 0 (iteration stamps are iter+1 >= 1)
-  CVO_HIP(ctx, hipMemsetAsync(ctx->b_ovf.p, 0, sizeof(uint32_t) * (size_t)n * n_probs, ctx->stream));
-  CVO_HIP(ctx, hipMemsetAsync(ctx->b_state.p, 0, sizeof(ProblemState) * n_probs, ctx->stream));
+  launch_zero(ctx->b_ovf.p, sizeof(uint32_t) * (size_t)n * n_probs, ctx->stream);
+  launch_zero(ctx->b_state.p, sizeof(ProblemState) * n_probs, ctx->stream);
   pr.states = as<ProblemState>(ctx->b_state);
   pr.host.resize(n_probs);
   for (int q = 0; q < n_probs; ++q) {
@@ -461,8 +464,8 @@ static cvo_status run_loop(cvo_ctx* ctx, const Problems& pr, const KConsts& kc, 
   };
   // the step pass and controller; an adaptive-lengthscale run steps its
   // registration problem only, then its self-association problems follow it
-  auto step_phase = [&]() {
-    launch_step(pr.dev, kc.adaptive ? 1 : pr.n_probs, pr.max_rows, kc, st, pr.host.data());
+  auto step_phase = [&](hipEvent_t e0, hipEvent_t e1) {
+    launch_step(pr.dev, kc.adaptive ? 1 : pr.n_probs, pr.max_rows, kc, st, pr.host.data(), e0, e1);
     if (kc.adaptive) launch_adaptive_sync(pr.dev, kc, st);
   };
   CVO_TRY(ensure_events(ctx, 8));
@@ -471,24 +474,21 @@ static cvo_status run_loop(cvo_ctx* ctx, const Problems& pr, const KConsts& kc, 
   CVO_HIP(ctx, hipEventRecord(t0, st));
 
   if (ctx->profiling) {
-    // kernel by kernel, events around each launch
+    // kernel by kernel, each timed by events at its own start and end (the
+    // kernel alone, as a kernel tracer times it)
     CVO_TRY(ensure_events(ctx, 8 + 6));
     hipEvent_t e[6];
     for (int i = 0; i < 6; ++i) e[i] = ctx->ev_pool[8 + i];
     for (int it = 0; it < max_iter; ++it) {
-      CVO_HIP(ctx, hipEventRecord(e[0], st));
-      launch_assoc(pr.dev, pr.n_probs, pr.total_blocks, kc, st, pr.host.data());
-      CVO_HIP(ctx, hipEventRecord(e[1], st));
-      launch_overflow(pr.dev, pr.n_probs, kc, st, pr.host.data());
-      CVO_HIP(ctx, hipEventRecord(e[2], st));
-      if (with_step) step_phase();
-      CVO_HIP(ctx, hipEventRecord(e[3], st));
+      launch_assoc(pr.dev, pr.n_probs, pr.total_blocks, kc, st, pr.host.data(), e[0], e[1]);
+      launch_overflow(pr.dev, pr.n_probs, kc, st, pr.host.data(), e[2], e[3]);
+      if (with_step) step_phase(e[4], e[5]);
       bool all = true;
       CVO_TRY(read_all_done(&all));
-      float ms[3];
+      float ms[3] = {0.0f, 0.0f, 0.0f};
       CVO_HIP(ctx, hipEventElapsedTime(&ms[0], e[0], e[1]));
-      CVO_HIP(ctx, hipEventElapsedTime(&ms[1], e[1], e[2]));
-      CVO_HIP(ctx, hipEventElapsedTime(&ms[2], e[2], e[3]));
+      CVO_HIP(ctx, hipEventElapsedTime(&ms[1], e[2], e[3]));
+      if (with_step) CVO_HIP(ctx, hipEventElapsedTime(&ms[2], e[4], e[5]));
       ctx->stats.ms[CVO_KSTAT_ASSOC] += ms[0];
       ctx->stats.ms[CVO_KSTAT_OVERFLOW] += ms[1];
       ctx->stats.ms[CVO_KSTAT_STEP] += with_step ? ms[2] : 0.0;
@@ -508,7 +508,7 @@ static cvo_status run_loop(cvo_ctx* ctx, const Problems& pr, const KConsts& kc, 
       for (int it = 0; it < kChunk; ++it) {
         launch_assoc(pr.dev, pr.n_probs, pr.total_blocks, kc, st, pr.host.data());
         launch_overflow(pr.dev, pr.n_probs, kc, st, pr.host.data());
-        step_phase();
+        step_phase(nullptr, nullptr);
       }
     };
     const char* ng = std::getenv("CVO_NO_GRAPH");
@@ -1544,8 +1544,8 @@ extern "C" cvo_status cvo_align_adaptive_d(cvo_ctx* ctx, const cvo_dcloud* src, 
   CVO_TRY(ensure(ctx, ctx->b_part_s, sizeof(double) * 4 * (size_t)total));
   CVO_TRY(ensure(ctx, ctx->b_part_cnt, sizeof(uint32_t) * 2 * (size_t)total));
   if (trace_cap > 0) CVO_TRY(ensure(ctx, ctx->b_trace, sizeof(cvo_iter_record) * (size_t)trace_cap));
-  CVO_HIP(ctx, hipMemsetAsync(ctx->b_ovf.p, 0, sizeof(uint32_t) * nrows, st));
-  CVO_HIP(ctx, hipMemsetAsync(ctx->b_state.p, 0, sizeof(ProblemState) * 3, st));
+  launch_zero(ctx->b_ovf.p, sizeof(uint32_t) * nrows, st);
+  launch_zero(ctx->b_state.p, sizeof(ProblemState) * 3, st);
   Problems pr;
   pr.n_probs = 3;
   pr.total_blocks = total;

@@ -29,6 +29,7 @@
 //   iterations_per_solve candidates, |dF|/F < 1e-5 after an accepted step,
 //   ||g||inf < 1e-5 or ||d|| < 1e-5 (||x|| + 1e-5).
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdint.h>
@@ -944,9 +945,9 @@ extern "C" cvo_status cvo_align_multiframe(cvo_ctx* ctx, int32_t n_frames, const
     } else if (ctx->profiling) {
       (void)hipEventRecord(ev[0], st);
       hipLaunchKernelGGL(ba_prepare_kernel, dim3(1), dim3(256), 0, st, L.ba, L.edges, L.probs);
-      (void)hipEventRecord(ev[1], st);
-      if (n_edges > 0) launch_assoc(L.probs, n_edges, total_blocks, kc, st);
-      (void)hipEventRecord(ev[2], st);
+      // association kernel timed alone (events at its own start and end)
+      if (n_edges > 0) launch_assoc(L.probs, n_edges, total_blocks, kc, st, nullptr, ev[1], ev[2]);
+      else (void)hipEventRecord(ev[1], st), (void)hipEventRecord(ev[2], st);
       if (n_edges > 0) {
         launch_overflow(L.probs, n_edges, kc, st);
       }
@@ -978,10 +979,9 @@ extern "C" cvo_status cvo_align_multiframe(cvo_ctx* ctx, int32_t n_frames, const
         (void)hipMemcpyAsync(pend, &L.ba->eval_pending, sizeof(int32_t) * 2, hipMemcpyDeviceToHost, st);
         (void)hipStreamSynchronize(st);
         const bool work = pend[0] != 0;
-        (void)hipEventRecord(ev[4], st);
-        hipLaunchKernelGGL(ba_pair_kernel, dim3(L.max_pair_blocks, n_edges), dim3(kBaRowsPerBlock), 0, st, L.ba,
-                           L.edges, L.probs, L.part, L.max_pair_blocks);
-        (void)hipEventRecord(ev[5], st);
+        // the pair kernel timed alone
+        hipExtLaunchKernelGGL(ba_pair_kernel, dim3(L.max_pair_blocks, n_edges), dim3(kBaRowsPerBlock), 0, st, ev[4],
+                              ev[5], 0, L.ba, L.edges, L.probs, L.part, L.max_pair_blocks);
         hipLaunchKernelGGL(ba_edge_reduce_kernel, dim3(n_edges), dim3(256), 0, st, L.ba, L.edges, L.part,
                            L.max_pair_blocks, L.contrib);
         hipLaunchKernelGGL(ba_assemble_kernel, dim3(L.n_asm_blocks), dim3(64), 0, st, L.ba, L.edges, L.contrib,

@@ -270,11 +270,12 @@ cvo_status prepare_source_into(cvo_ctx* ctx, const cvo_dcloud* s, DevBuf& b_pos,
                                DevBuf& b_geo);
 cvo_status build_grid_into(cvo_ctx* ctx, const cvo_dcloud* t, const std::vector<float>& hs, GridHost& gh,
                            DevBuf& b_store, DevBuf& b_hdr);
+// host_one: the host copy of a single problem's descriptor, passed by value;
+// e0 / e1 (optional): events recorded at the kernel's own start and end
 void launch_assoc(const AssocProblem* probs, int n_probs, int total_blocks, const KConsts& kc, hipStream_t s,
-                  const AssocProblem* host_one = nullptr);
-// host_one: the host copy of a single problem's descriptor, passed by value
+                  const AssocProblem* host_one = nullptr, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 void launch_overflow(const AssocProblem* probs, int n_probs, const KConsts& kc, hipStream_t s,
-                     const AssocProblem* host_one = nullptr);
+                     const AssocProblem* host_one = nullptr, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 void launch_init_state(const AssocProblem* probs, int n_probs, const float* T16, int stride16, const KConsts& kc,
                        float ell0, int K0, hipStream_t s);
 cvo_status prepare_source(cvo_ctx* ctx, const cvo_dcloud* s);

@@ -18,6 +18,7 @@
 // are block partials summed in fixed order, so results are bitwise
 // reproducible run to run.
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdint.h>
@@ -129,6 +130,12 @@ __global__ void grid_cell_count(const float* __restrict__ xyz, int n, float ox, 
   uint32_t c = ((uint32_t)cz * (uint32_t)ny + (uint32_t)cy) * (uint32_t)nx + (uint32_t)cx;
   key[j] = c;
   atomicAdd(&count[c], 1u);
+}
+
+// Zero fill, in place of a memset node (which a caller's graph capture
+// should not record).
+__global__ void zero_u32(uint32_t* __restrict__ v, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) v[i] = 0u;
 }
 
 __global__ void iota_u32(uint32_t* __restrict__ v, int n) {
@@ -1769,25 +1776,34 @@ void launch_all_done(const AssocProblem* probs, int n_probs, int32_t* out, hipSt
   hipLaunchKernelGGL(all_done_kernel, dim3(1), dim3(256), 0, s, probs, n_probs, out);
 }
 
+// A launch, bracketed by the events e0 / e1 at the kernel's own start and end
+// when e0 is given (profiling: the events time the kernel alone, as a kernel
+// tracer does, not the dispatch around it).
+#define CVO_LAUNCH(KERNEL, GRID, BLOCK, STREAM, E0, E1, ...)                                  \
+  do {                                                                                      \
+    if (E0) hipExtLaunchKernelGGL(KERNEL, GRID, BLOCK, 0, STREAM, E0, E1, 0, __VA_ARGS__); \
+    else hipLaunchKernelGGL(KERNEL, GRID, BLOCK, 0, STREAM, __VA_ARGS__);                  \
+  } while (0)
+
 template <int FC, int KC>
 static void launch_assoc_t(const AssocProblem* probs, int n_probs, int total_blocks, const KConsts& kc,
-                           const AssocProblem& one, bool use_one, hipStream_t s) {
+                           const AssocProblem& one, bool use_one, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
   const dim3 g(total_blocks), b(kBlock);
   if (kc.knn > 0) {
-    if (use_one) hipLaunchKernelGGL((knn_assoc_kernel<FC, KC, true>), g, b, 0, s, probs, n_probs, kc, one);
-    else hipLaunchKernelGGL((knn_assoc_kernel<FC, KC, false>), g, b, 0, s, probs, n_probs, kc, one);
+    if (use_one) CVO_LAUNCH((knn_assoc_kernel<FC, KC, true>), g, b, s, e0, e1, probs, n_probs, kc, one);
+    else CVO_LAUNCH((knn_assoc_kernel<FC, KC, false>), g, b, s, e0, e1, probs, n_probs, kc, one);
   } else {
-    if (use_one) hipLaunchKernelGGL((assoc_kernel<FC, KC, true>), g, b, 0, s, probs, n_probs, kc, one);
-    else hipLaunchKernelGGL((assoc_kernel<FC, KC, false>), g, b, 0, s, probs, n_probs, kc, one);
+    if (use_one) CVO_LAUNCH((assoc_kernel<FC, KC, true>), g, b, s, e0, e1, probs, n_probs, kc, one);
+    else CVO_LAUNCH((assoc_kernel<FC, KC, false>), g, b, s, e0, e1, probs, n_probs, kc, one);
   }
 }
 
 template <int FC, int KC>
 static void launch_ovf_t(const AssocProblem* probs, int n_probs, const KConsts& kc, const AssocProblem& one,
-                         bool use_one, hipStream_t s) {
+                         bool use_one, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
   const dim3 g(kOvfBlocks, n_probs), b(kBlock);
-  if (use_one) hipLaunchKernelGGL((overflow_kernel<FC, KC, true>), g, b, 0, s, probs, kc, one);
-  else hipLaunchKernelGGL((overflow_kernel<FC, KC, false>), g, b, 0, s, probs, kc, one);
+  if (use_one) CVO_LAUNCH((overflow_kernel<FC, KC, true>), g, b, s, e0, e1, probs, kc, one);
+  else CVO_LAUNCH((overflow_kernel<FC, KC, false>), g, b, s, e0, e1, probs, kc, one);
 }
 
 static int fsel(const KConsts& kc) {
@@ -1814,30 +1830,30 @@ static int ksel(const KConsts& kc) {
   } while (0)
 
 void launch_assoc(const AssocProblem* probs, int n_probs, int total_blocks, const KConsts& kc, hipStream_t s,
-                  const AssocProblem* host_one) {
+                  const AssocProblem* host_one, hipEvent_t e0, hipEvent_t e1) {
   AssocProblem one{};
   const bool use_one = n_probs == 1 && host_one;
   if (use_one) one = *host_one;
-  CVO_DISPATCH_FK(launch_assoc_t, probs, n_probs, total_blocks, kc, one, use_one, s);
+  CVO_DISPATCH_FK(launch_assoc_t, probs, n_probs, total_blocks, kc, one, use_one, s, e0, e1);
 }
 
 void launch_overflow(const AssocProblem* probs, int n_probs, const KConsts& kc, hipStream_t s,
-                     const AssocProblem* host_one) {
+                     const AssocProblem* host_one, hipEvent_t e0, hipEvent_t e1) {
   AssocProblem one{};
   const bool use_one = n_probs == 1 && host_one;
   if (use_one) one = *host_one;
-  CVO_DISPATCH_FK(launch_ovf_t, probs, n_probs, kc, one, use_one, s);
+  CVO_DISPATCH_FK(launch_ovf_t, probs, n_probs, kc, one, use_one, s, e0, e1);
 }
 
 void launch_step(const AssocProblem* probs, int n_probs, int max_src_rows, const KConsts& kc, hipStream_t s,
-                 const AssocProblem* host_one) {
+                 const AssocProblem* host_one, hipEvent_t e0, hipEvent_t e1) {
   const dim3 g((max_src_rows + kBlock - 1) / kBlock, n_probs), b(kBlock);
   AssocProblem one{};
   if (n_probs == 1 && host_one) {
     one = *host_one;
-    hipLaunchKernelGGL(step_kernel<true>, g, b, 0, s, probs, kc, one);
+    CVO_LAUNCH(step_kernel<true>, g, b, s, e0, e1, probs, kc, one);
   } else {
-    hipLaunchKernelGGL(step_kernel<false>, g, b, 0, s, probs, kc, one);
+    CVO_LAUNCH(step_kernel<false>, g, b, s, e0, e1, probs, kc, one);
   }
 }
 
@@ -1859,6 +1875,13 @@ void launch_init_state(const AssocProblem* probs, int n_probs, const float* T16,
 void launch_grid_count(const float* xyz, int n, const GridLevel& L, uint32_t* key, uint32_t* count, hipStream_t s) {
   hipLaunchKernelGGL(grid_cell_count, dim3((n + 255) / 256), dim3(256), 0, s, xyz, n, L.ox, L.oy, L.oz, L.inv_h, L.nx,
                      L.ny, L.nz, key, count);
+}
+
+void launch_zero(void* p, size_t bytes, hipStream_t s) {
+  const size_t n = bytes / 4;
+  if (n == 0) return;
+  const size_t blocks = std::min<size_t>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(zero_u32, dim3((unsigned)blocks), dim3(256), 0, s, static_cast<uint32_t*>(p), n);
 }
 
 void launch_iota(uint32_t* v, int n, hipStream_t s) {
