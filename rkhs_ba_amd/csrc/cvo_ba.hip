@@ -53,7 +53,8 @@ constexpr int kBaMaxEdges = 4096;
 constexpr int kBaTerms = 28;                   // S (21, upper triangle), h (6), c (1)
 constexpr int kBaContrib = 121;                // B11, B12, B22 (36 each), g1, g2 (6 each), c
 constexpr int kBaRowsPerBlock = 256;           // pair pass: one row per thread
-constexpr int kSolveThreads = 64;              // one wave: the factorisation's barriers stay cheap
+constexpr int kSolveThreads = 256;             // four waves: the Cholesky's trailing update spread over rows
+static_assert(kSolveThreads == 256 && 6 * kBaMaxFree <= kSolveThreads, "the solve's reductions assume four waves");
 
 struct BaEdge {
   double ell;
@@ -392,6 +393,36 @@ __device__ inline double wave_max(double v) {
   return v;
 }
 
+// Flat element e of the trailing lower triangle of an n x n factorisation,
+// ordered by column descending: column k = n - 1 - m holds entries
+// m (m + 1) / 2 ... m (m + 1) / 2 + m, rows k ... n - 1.  So the elements of
+// the columns right of j are exactly e < (n - j - 1)(n - j) / 2.
+__device__ inline void trailing_index(int e, int n, int& i, int& k) {
+  int m = (int)((sqrtf(8.0f * (float)e + 1.0f) - 1.0f) * 0.5f);
+  while ((m + 1) * (m + 2) / 2 <= e) ++m;
+  while (m * (m + 1) / 2 > e) --m;
+  k = n - 1 - m;
+  i = k + (e - m * (m + 1) / 2);
+}
+
+// Sum and maximum over the four waves of a solve block (waves in order);
+// every thread calls them and gets the result.  The leading barrier keeps a
+// previous call's readers ahead of this call's writers.
+__device__ inline double block_sum4(double v, double* s_red) {
+  v = wave_sum(v);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  return ((s_red[0] + s_red[1]) + s_red[2]) + s_red[3];
+}
+__device__ inline double block_max4(double v, double* s_red) {
+  v = wave_max(v);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  return fmax(fmax(s_red[0], s_red[1]), fmax(s_red[2], s_red[3]));
+}
+
 // H, g and F at the poses Xe from the edge blocks: one workgroup per 6x6 block
 // (fi, fj) of H, then one per 6-block of g (the first also sums F); every sum
 // runs over the edges in index order.  H goes to the buffer that is not the
@@ -446,18 +477,23 @@ __global__ __launch_bounds__(64) void ba_assemble_kernel(BaDev* __restrict__ ba,
   }
 }
 
-// One wave: the Levenberg-Marquardt decision on an evaluated candidate, then
-// the next candidate (or the end of the solve).  Scalars of the schedule are
-// kept uniform in registers and written back by lane 0; the damped system is
-// factored in LDS (right-looking Cholesky), the triangular solves keep the
+// One block of four waves: the Levenberg-Marquardt decision on an evaluated
+// candidate, then the next candidate (or the end of the solve).  Scalars of
+// the schedule are kept uniform in every thread (block-wide reductions) and
+// written back by thread 0; thread i < n holds entry i of the gradient; the
+// damped system is factored in LDS (right-looking Cholesky, the trailing
+// update over all four waves), the triangular solves run in wave 0 with the
 // right-hand side in registers (rows lane and lane + 64).
 __global__ __launch_bounds__(kSolveThreads) void ba_solve_kernel(BaDev* __restrict__ ba, double* __restrict__ Hbuf) {
   if (ba->done || !ba->eval_pending) return;
-  const int lane = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int n = ba->n;
   const int ld = n | 1;  // odd row stride in LDS
   __shared__ double s_A[kBaMaxDof * (kBaMaxDof + 1)];
   __shared__ double s_d[kBaMaxDof];
+  __shared__ double s_g[kBaMaxDof];
+  __shared__ double s_inv[kBaMaxDof];  // 1 / L(j, j)
+  __shared__ double s_red[kSolveThreads / 64];
   const bool candidate = ba->eval_is_candidate != 0;
   int cur = ba->h_cur;
   double mu = ba->mu, v = ba->v, F = ba->F;
@@ -465,18 +501,23 @@ __global__ __launch_bounds__(kSolveThreads) void ba_solve_kernel(BaDev* __restri
   int lm_iter = ba->lm_iter, lm_stop = ba->lm_stop;
   int accepted_total = ba->accepted_total, evals_total = ba->evals_total;
   const int max_lm = ba->iters_per_solve;
-  const int r0 = lane, r1 = lane + 64;
   bool take = !candidate;
   if (candidate) {
-    // predicted decrease -(g.d + d.H.d / 2) at the current linearisation
+    // predicted decrease -(g.d + d.H.d / 2) at the current linearisation,
+    // H read once, coalesced
     const double* H = Hbuf + (size_t)cur * kBaMaxDof * kBaMaxDof;
+    if (tid < n) s_d[tid] = ba->d[tid];
+    __syncthreads();
     double part = 0.0;
-    for (int i = lane; i < n; i += 64) {
-      double hd = 0.0;
-      for (int j = 0; j < n; ++j) hd += H[(size_t)i * n + j] * ba->d[j];
-      part += ba->d[i] * (ba->g[i] + 0.5 * hd);
+    // eight loads in flight per thread
+#pragma unroll 8
+    for (int idx = tid; idx < n * n; idx += kSolveThreads) {
+      const int i = idx / n, j = idx - i * n;
+      part += s_d[i] * H[idx] * s_d[j];
     }
-    const double pred = -wave_sum(part);
+    part *= 0.5;
+    if (tid < n) part += s_d[tid] * ba->g[tid];
+    const double pred = -block_sum4(part, s_red);
     const double rho = pred > 0.0 ? (F - Fn) / pred : -1.0;
     evals_total++;
     if (rho > 1e-3) {
@@ -492,43 +533,50 @@ __global__ __launch_bounds__(kSolveThreads) void ba_solve_kernel(BaDev* __restri
     }
     lm_iter++;
   }
-  // current linearisation: g in registers, poses X
+  // current linearisation: thread i's gradient entry, poses X
   const double* Xs = (take && candidate) ? ba->Xe : ba->X;
-  double g0 = 0.0, g1 = 0.0;
+  double g = 0.0;
   if (take) {
-    if (r0 < n) g0 = ba->gn[r0];
-    if (r1 < n) g1 = ba->gn[r1];
-    if (r0 < n) ba->g[r0] = g0;
-    if (r1 < n) ba->g[r1] = g1;
+    if (tid < n) {
+      g = ba->gn[tid];
+      ba->g[tid] = g;
+    }
     F = Fn;
     cur = 1 - cur;
   } else {
-    if (r0 < n) g0 = ba->g[r0];
-    if (r1 < n) g1 = ba->g[r1];
+    if (tid < n) g = ba->g[tid];
   }
+  if (tid < n) s_g[tid] = g;
   double xs2 = 0.0;
-  for (int i = lane; i < ba->n_frames * 12; i += 64) {
+  for (int i = tid; i < ba->n_frames * 12; i += kSolveThreads) {
     const double x = Xs[i];
     xs2 += x * x;
     if (take && candidate) ba->X[i] = x;
   }
-  const double xn = sqrt(wave_sum(xs2));
+  const double xn = sqrt(block_sum4(xs2, s_red));
   const double* H = Hbuf + (size_t)cur * kBaMaxDof * kBaMaxDof;
   int pending = 0;
+  // every thread holds the same control values (block-wide reductions), so
+  // the loop and its barriers are uniform
   while (true) {
     if (lm_stop || lm_iter >= max_lm) break;
-    const double gmax = wave_max(fmax(fabs(g0), fabs(g1)));
+    const double gmax = block_max4(fabs(g), s_red);
     if (gmax < 1e-5) {
       lm_stop = 1;
       break;
     }
-    for (int idx = lane; idx < n * n; idx += 64) {
+#pragma unroll 8
+    for (int idx = tid; idx < n * n; idx += kSolveThreads) {
       const int i = idx / n, j = idx - i * n;
       double x = H[idx];
       if (i == j) x += fmin(fmax(x, 1e-6), 1e32) / mu;
       s_A[i * ld + j] = x;
     }
     __syncthreads();
+    // right-looking Cholesky: column j scaled, then the trailing update with
+    // the waves over rows and the lanes over columns; the diagonal's
+    // reciprocal goes to s_inv (the pivot itself is not rewritten while
+    // other waves read it)
     bool fail = false;
     for (int j = 0; j < n; ++j) {
       const double ajj = s_A[j * ld + j];
@@ -536,13 +584,28 @@ __global__ __launch_bounds__(kSolveThreads) void ba_solve_kernel(BaDev* __restri
         fail = true;
         break;
       }
-      const double djj = sqrt(ajj);
-      for (int i = j + 1 + lane; i < n; i += 64) s_A[i * ld + j] /= djj;
-      if (lane == 0) s_A[j * ld + j] = djj;
+      const double inv = 1.0 / sqrt(ajj);
+      for (int i = j + 1 + tid; i < n; i += kSolveThreads) s_A[i * ld + j] *= inv;
+      if (tid == 0) s_inv[j] = inv;
       __syncthreads();
-      for (int i = j + 1 + lane; i < n; i += 64) {
-        const double lij = s_A[i * ld + j];
-        for (int k = j + 1; k <= i; ++k) s_A[i * ld + k] -= lij * s_A[k * ld + j];
+      // the trailing triangle {(i, k): j < k <= i < n} as M flat elements,
+      // four per thread a trip with all reads issued before the writes (each
+      // element has one writer; column j is only read here)
+      const int M = (n - j - 1) * (n - j) / 2;
+      for (int e0 = tid; e0 < M; e0 += 4 * kSolveThreads) {
+        int ii[4], kk[4];
+        double lij[4], lkj[4], aik[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int e = min(e0 + u * kSolveThreads, M - 1);
+          trailing_index(e, n, ii[u], kk[u]);
+          lij[u] = s_A[ii[u] * ld + j];
+          lkj[u] = s_A[kk[u] * ld + j];
+          aik[u] = s_A[ii[u] * ld + kk[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+          if (e0 + u * kSolveThreads < M) s_A[ii[u] * ld + kk[u]] = aik[u] - lij[u] * lkj[u];
       }
       __syncthreads();
     }
@@ -553,42 +616,38 @@ __global__ __launch_bounds__(kSolveThreads) void ba_solve_kernel(BaDev* __restri
       __syncthreads();
       continue;
     }
-    // L y = -g, then L^T d = y
-    double b0 = -g0, b1 = -g1;
-    for (int i = 0; i < n; ++i) {
-      const double bi = (i < 64) ? __shfl(b0, i, 64) : __shfl(b1, i - 64, 64);
-      const double yi = bi / s_A[i * ld + i];
-      if (r0 == i) b0 = yi;
-      if (r1 == i) b1 = yi;
-      if (r0 > i && r0 < n) b0 -= s_A[r0 * ld + i] * yi;
-      if (r1 > i && r1 < n) b1 -= s_A[r1 * ld + i] * yi;
+    // L y = -g, then L^T d = y: one wave, rows r and r + 64 per lane
+    if (wave == 0) {
+      const int r0 = lane, r1 = lane + 64;
+      double b0 = r0 < n ? -s_g[r0] : 0.0, b1 = r1 < n ? -s_g[r1] : 0.0;
+      for (int i = 0; i < n; ++i) {
+        const double bi = (i < 64) ? __shfl(b0, i, 64) : __shfl(b1, i - 64, 64);
+        const double yi = bi * s_inv[i];
+        if (r0 == i) b0 = yi;
+        if (r1 == i) b1 = yi;
+        if (r0 > i && r0 < n) b0 -= s_A[r0 * ld + i] * yi;
+        if (r1 > i && r1 < n) b1 -= s_A[r1 * ld + i] * yi;
+      }
+      for (int i = n - 1; i >= 0; --i) {
+        const double bi = (i < 64) ? __shfl(b0, i, 64) : __shfl(b1, i - 64, 64);
+        const double xi = bi * s_inv[i];
+        if (r0 == i) b0 = xi;
+        if (r1 == i) b1 = xi;
+        if (r0 < i) b0 -= s_A[i * ld + r0] * xi;
+        if (r1 < i) b1 -= s_A[i * ld + r1] * xi;
+      }
+      if (r0 < n) s_d[r0] = b0;
+      if (r1 < n) s_d[r1] = b1;
     }
-    for (int i = n - 1; i >= 0; --i) {
-      const double bi = (i < 64) ? __shfl(b0, i, 64) : __shfl(b1, i - 64, 64);
-      const double xi = bi / s_A[i * ld + i];
-      if (r0 == i) b0 = xi;
-      if (r1 == i) b1 = xi;
-      if (r0 < i) b0 -= s_A[i * ld + r0] * xi;
-      if (r1 < i) b1 -= s_A[i * ld + r1] * xi;
-    }
-    double dd = 0.0;
-    if (r0 < n) dd += b0 * b0;
-    if (r1 < n) dd += b1 * b1;
-    const double dn = sqrt(wave_sum(dd));
+    __syncthreads();
+    const double dd = tid < n ? s_d[tid] * s_d[tid] : 0.0;
+    const double dn = sqrt(block_sum4(dd, s_red));
     if (dn < 1e-5 * (xn + 1e-5)) {
       lm_stop = 1;
       break;
     }
-    if (r0 < n) {
-      s_d[r0] = b0;
-      ba->d[r0] = b0;
-    }
-    if (r1 < n) {
-      s_d[r1] = b1;
-      ba->d[r1] = b1;
-    }
-    __syncthreads();
-    for (int f = lane; f < ba->n_frames; f += 64) {
+    if (tid < n) ba->d[tid] = s_d[tid];
+    for (int f = tid; f < ba->n_frames; f += kSolveThreads) {
       const int o = ba->free_idx[f];
       if (o >= 0) pose_plus(Xs + 12 * f, s_d + 6 * o, ba->Xe + 12 * f);
       else
@@ -597,7 +656,7 @@ __global__ __launch_bounds__(kSolveThreads) void ba_solve_kernel(BaDev* __restri
     pending = 1;
     break;
   }
-  if (lane == 0) {
+  if (tid == 0) {
     ba->h_cur = cur;
     ba->mu = mu;
     ba->v = v;

@@ -662,15 +662,16 @@ struct cvo_model {
   uint16_t* W3b = nullptr;
   uint16_t* W4b = nullptr;
   uint16_t* W5b = nullptr;
-  DevBuf ws[24];           // batch workspace
+  DevBuf ws[32];           // batch workspace (WS_COUNT used)
 };
 
 namespace {
 
 enum {
   WS_PTS = 0, WS_X, WS_CEN, WS_H1, WS_H2, WS_POOL, WS_Z, WS_GVAL, WS_GARG, WS_Z1, WS_O, WS_TGT, WS_DO, WS_DOB,
-  WS_DZ1, WS_DZ, WS_ROWV, WS_XG, WS_H1G, WS_DH2, WS_DH1, WS_LOSS, WS_QT, WS_CONF
+  WS_DZ1, WS_DZ, WS_ROWV, WS_XG, WS_H1G, WS_DH2, WS_DH1, WS_LOSS, WS_QT, WS_CONF, WS_STAGE, WS_COUNT
 };
+static_assert(WS_COUNT <= 32, "the model's workspace array holds 32 buffers");
 
 template <class T>
 T* wsp(cvo_model* m, int i) {
@@ -747,15 +748,29 @@ cvo_status check_batch(cvo_ctx* ctx, int n_pairs, int npts) {
   return CVO_OK;
 }
 
-cvo_status upload_points(cvo_model* m, int n_pairs, int npts, const float* a, const float* b) {
+// The two batches into [pair][cloud][point][8] (each pair's two clouds
+// interleaved).  Host input: one contiguous copy per batch into a staging
+// buffer (a strided copy from pageable memory moves row by row), then a
+// device-to-device interleave; device input: the interleave alone.
+cvo_status upload_points(cvo_model* m, int n_pairs, int npts, const float* a, const float* b, bool on_device) {
   cvo_ctx* ctx = m->ctx;
   const size_t per = sizeof(float) * (size_t)npts * kMlpIn;
   CVO_TRY(ensure(ctx, m->ws[WS_PTS], per * 2 * n_pairs));
   float* d = wsp<float>(m, WS_PTS);
-  // [pair][cloud][point][8]: interleave the two clouds of each pair
-  CVO_HIP(ctx, hipMemcpy2DAsync(d, 2 * per, a, per, per, n_pairs, hipMemcpyHostToDevice, ctx->stream));
-  CVO_HIP(ctx, hipMemcpy2DAsync(reinterpret_cast<char*>(d) + per, 2 * per, b, per, per, n_pairs,
+  const float* sa = a;
+  const float* sb = b;
+  if (!on_device) {
+    CVO_TRY(ensure(ctx, m->ws[WS_STAGE], per * 2 * n_pairs));
+    float* st = wsp<float>(m, WS_STAGE);
+    CVO_HIP(ctx, hipMemcpyAsync(st, a, per * n_pairs, hipMemcpyHostToDevice, ctx->stream));
+    CVO_HIP(ctx, hipMemcpyAsync(reinterpret_cast<char*>(st) + per * n_pairs, b, per * n_pairs,
                                 hipMemcpyHostToDevice, ctx->stream));
+    sa = st;
+    sb = reinterpret_cast<const float*>(reinterpret_cast<const char*>(st) + per * n_pairs);
+  }
+  CVO_HIP(ctx, hipMemcpy2DAsync(d, 2 * per, sa, per, per, n_pairs, hipMemcpyDeviceToDevice, ctx->stream));
+  CVO_HIP(ctx, hipMemcpy2DAsync(reinterpret_cast<char*>(d) + per, 2 * per, sb, per, per, n_pairs,
+                                hipMemcpyDeviceToDevice, ctx->stream));
   return CVO_OK;
 }
 
@@ -857,7 +872,7 @@ extern "C" cvo_status cvo_model_gradient(cvo_ctx* ctx, cvo_model* m, int32_t n_p
   if (!ctx || !m || !a || !b || !targets) return CVO_ERR_INVALID_ARG;
   CVO_TRY(check_batch(ctx, n_pairs, npts));
   hipStream_t s = ctx->stream;
-  CVO_TRY(upload_points(m, n_pairs, npts, a, b));
+  CVO_TRY(upload_points(m, n_pairs, npts, a, b, false));
   CVO_TRY(forward(m, n_pairs, npts));
   const int clouds = 2 * n_pairs;
   const int R = clouds * kH3;
@@ -960,14 +975,13 @@ extern "C" cvo_status cvo_model_train_step(cvo_ctx* ctx, cvo_model* m, int32_t n
   return CVO_OK;
 }
 
-extern "C" cvo_status cvo_propose_poses(cvo_ctx* ctx, const cvo_model* mc, int32_t n_pairs, int32_t npts,
-                                        const float* a, const float* b, float* quat_t, float* conf,
-                                        float* raw_out) {
+static cvo_status propose_impl(cvo_ctx* ctx, const cvo_model* mc, int32_t n_pairs, int32_t npts, const float* a,
+                               const float* b, bool on_device, float* quat_t, float* conf, float* raw_out) {
   if (!ctx || !mc || !a || !b || !quat_t || !conf) return CVO_ERR_INVALID_ARG;
   CVO_TRY(check_batch(ctx, n_pairs, npts));
   cvo_model* m = const_cast<cvo_model*>(mc);
   hipStream_t s = ctx->stream;
-  CVO_TRY(upload_points(m, n_pairs, npts, a, b));
+  CVO_TRY(upload_points(m, n_pairs, npts, a, b, on_device));
   if (ctx->profiling) {
     CVO_TRY(ensure_events(ctx, 2));
     CVO_HIP(ctx, hipEventRecord(ctx->ev_pool[0], s));
@@ -996,6 +1010,18 @@ extern "C" cvo_status cvo_propose_poses(cvo_ctx* ctx, const cvo_model* mc, int32
     CVO_HIP(ctx, hipMemcpyAsync(raw_out, m->ws[WS_O].p, sizeof(float) * n_pairs * kMlpOut, hipMemcpyDeviceToHost, s));
   CVO_HIP(ctx, hipStreamSynchronize(s));
   return CVO_OK;
+}
+
+extern "C" cvo_status cvo_propose_poses(cvo_ctx* ctx, const cvo_model* m, int32_t n_pairs, int32_t npts,
+                                        const float* a, const float* b, float* quat_t, float* conf,
+                                        float* raw_out) {
+  return propose_impl(ctx, m, n_pairs, npts, a, b, false, quat_t, conf, raw_out);
+}
+
+extern "C" cvo_status cvo_propose_poses_d(cvo_ctx* ctx, const cvo_model* m, int32_t n_pairs, int32_t npts,
+                                          const float* a_dev, const float* b_dev, float* quat_t, float* conf,
+                                          float* raw_out) {
+  return propose_impl(ctx, m, n_pairs, npts, a_dev, b_dev, true, quat_t, conf, raw_out);
 }
 
 // BF16 MFMA GEMM on caller data (host arrays): C = A B with A (M x K) and

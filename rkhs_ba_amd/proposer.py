@@ -40,6 +40,7 @@ def _setup():
     L.cvo_model_gradient.argtypes = [P, P, i32, i32, P, P, P, P, P]
     L.cvo_model_train_step.argtypes = [P, P, i32, i32, P, P, P, f, P]
     L.cvo_propose_poses.argtypes = [P, P, i32, i32, P, P, P, P, P]
+    L.cvo_propose_poses_d.argtypes = [P, P, i32, i32, P, P, P, P, P]
     L.cvo_gemm_bf16.argtypes = [P, i32, i32, i32, P, P, P]
     L._mlp_ready = True
     return L
@@ -122,14 +123,26 @@ class PoseProposer:
         return loss.value
 
     def propose(self, a, b):
-        """Returns quat_t (n, 8, 7), conf (n, 8), raw head outputs (n, 64)."""
-        a, b = _f32(a), _f32(b)
+        """Returns quat_t (n, 8, 7), conf (n, 8), raw head outputs (n, 64).
+        a, b: (n, npts, 8) float32 host arrays, or DeviceArray batches
+        already in device memory (cvo_propose_poses_d)."""
+        from .runtime import DeviceArray
+        on_device = isinstance(a, DeviceArray) and isinstance(b, DeviceArray)
+        if on_device:
+            if len(a.shape) != 3 or a.shape[2] != 8 or a.shape != b.shape:
+                raise ValueError("device batches must be two (n, npts, 8) float32 arrays of one shape")
+        else:
+            a, b = _f32(a), _f32(b)
         n, npts = a.shape[0], a.shape[1]
         qt = np.zeros((n, K, 7), np.float32)
         conf = np.zeros((n, K), np.float32)
         raw = np.zeros((n, 64), np.float32)
-        _check(self._L.cvo_propose_poses(self.cvo._ctx, self._m, n, npts, a.ctypes.data, b.ctypes.data,
-                                         qt.ctypes.data, conf.ctypes.data, raw.ctypes.data), self.cvo._ctx)
+        if on_device:
+            _check(self._L.cvo_propose_poses_d(self.cvo._ctx, self._m, n, npts, a.ptr, b.ptr,
+                                               qt.ctypes.data, conf.ctypes.data, raw.ctypes.data), self.cvo._ctx)
+        else:
+            _check(self._L.cvo_propose_poses(self.cvo._ctx, self._m, n, npts, a.ctypes.data, b.ctypes.data,
+                                             qt.ctypes.data, conf.ctypes.data, raw.ctypes.data), self.cvo._ctx)
         return qt, conf, raw
 
 

@@ -198,6 +198,53 @@ def _take_assoc(a: Assoc) -> Association:
     return Association(n, int(a.cols), row_ptr, col, val)
 
 
+_hip_rt = None
+
+
+def _hip():
+    """The HIP runtime the library is linked against (already loaded with it;
+    a second HIP runtime in the process, such as the one a framework wheel
+    bundles, cannot share the device with it)."""
+    global _hip_rt
+    if _hip_rt is None:
+        lib()
+        h = C.CDLL("libamdhip64.so.7")
+        h.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        h.hipFree.argtypes = [C.c_void_p]
+        h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        _hip_rt = h
+    return _hip_rt
+
+
+class DeviceArray:
+    """A float32 array copied into device memory through the library's own
+    HIP runtime: the input of the entry points that take device pointers
+    (cvo_propose_poses_d).  .ptr is the device address."""
+
+    def __init__(self, arr):
+        a = np.ascontiguousarray(arr, np.float32)
+        self.shape, self.dtype, self.nbytes = a.shape, a.dtype, a.nbytes
+        h = _hip()
+        p = C.c_void_p()
+        if h.hipMalloc(C.byref(p), max(1, a.nbytes)) != 0:
+            raise CvoError(7, "hipMalloc failed")
+        self.ptr = p.value
+        if h.hipMemcpy(C.c_void_p(self.ptr), a.ctypes.data_as(C.c_void_p), a.nbytes, 1) != 0:  # host to device
+            self.free()
+            raise CvoError(5, "hipMemcpy failed")
+
+    def free(self):
+        if getattr(self, "ptr", None):
+            _hip().hipFree(C.c_void_p(self.ptr))
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
 class DeviceCloud:
     """A cloud resident in device memory (cvo_cloud_upload)."""
 

@@ -2,7 +2,7 @@
 
 Bars (written in each test):
   * one pass (cvo_compute_association_knn): the same (row, target) sets as the
-    oracle, except pairs whose value lies within 1e-5 relative of sp_thres;
+    oracle, except pairs whose value lies within 1e-6 relative of sp_thres;
     stored values within 4e-6 relative (one FP32 exp of the summed exponent
     against three FP64 exps rounded to float).  The distances are the same
     float operations on both sides, so the K-th place is decided identically;
@@ -40,8 +40,10 @@ def compare_knn(oracle, p, src, tgt, T, ell, K, g, val_rtol=4e-6):
     rows, gcol, gval = g.pairs()
     got = {(int(r), int(c)): float(v) for r, c, v in zip(rows, gcol, gval)}
     diff = set(got) ^ set(want)
-    bad = [k for k in diff if abs(got.get(k, want.get(k)) - p.sp_thres) > 1e-5 * p.sp_thres]
-    assert not bad, bad[:10]
+    band = max((abs(got.get(k, want.get(k)) - p.sp_thres) / p.sp_thres for k in diff), default=0.0)
+    say(f"  knn pass: {len(diff)} differing pairs, widest {band:.2e} of sp_thres")
+    bad = [k for k in diff if abs(got.get(k, want.get(k)) - p.sp_thres) > 1e-6 * p.sp_thres]
+    assert not bad, (band, bad[:10])
     common = set(got) & set(want)
     rel = max((abs(got[k] - want[k]) / want[k] for k in common), default=0.0)
     assert rel <= val_rtol, rel

@@ -61,6 +61,20 @@ def test_forward_matches_oracle(prop):
     assert rel(qt, qo) < 1e-2 and rel(conf, co) < 1e-2
 
 
+def test_device_batches_give_the_host_result(prop):
+    """cvo_propose_poses_d on batches already in device memory: bitwise the
+    host-array call."""
+    from rkhs_ba_amd import DeviceArray
+    a, b, t = batch(2, 5, 512)
+    say("proposer host vs device batches")
+    qt, conf, raw = prop.propose(a, b)
+    da, db = DeviceArray(a), DeviceArray(b)
+    qd, cd, rd = prop.propose(da, db)
+    assert np.array_equal(qt, qd) and np.array_equal(conf, cd) and np.array_equal(raw, rd)
+    with pytest.raises(ValueError):
+        prop.propose(da, DeviceArray(b[:4]))
+
+
 def test_gradient_matches_oracle(prop):
     a, b, t = batch(2, 4, 128)
     w = prop.get_weights().astype(np.float64)
