@@ -491,7 +491,7 @@ def leg_learned(args):
     out = {"inference_pairs_per_s": 256 / wall, "inference_batch_ms": wall * 1e3, "forward_ms": fwd_ms,
            "inference_batch_ms_host_inputs": wall_host * 1e3,
            "roofline": {"bound": "mfma", "achieved": achieved, "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
-                        "frac": achieved / MFMA_BF16_PEAK_TFLOPS, "kernel": "gemm_bf16_kernel (encoder + head)",
+                        "frac": achieved / MFMA_BF16_PEAK_TFLOPS, "kernel": "encoder_fused_kernel + head gemm_bf16_kernel",
                         "flops_per_launch": flops, "traffic": None}}
     n = args.learned_pairs
     ok_g = ok_l = 0

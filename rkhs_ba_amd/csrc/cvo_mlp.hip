@@ -207,6 +207,176 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(GemmArgs g) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Fused point encoder for inference: the three per-point layers (16 -> 64 ->
+// 128 -> 256, bias and ReLU, BF16 between layers as in the layer-by-layer
+// path) and the max-pool over the cloud in one kernel.  One workgroup per
+// cloud: the weights are staged once, transposed (n rows of k), so both MFMA
+// operands are 16-byte LDS loads; the cloud's points go through in tiles of
+// 128, each of the four waves owning 32 rows of a tile from the input to the
+// pool, so the activations never leave LDS.  Layer 1 (k = 16) runs on
+// v_mfma_f32_16x16x16_bf16, layers 2 and 3 on v_mfma_f32_16x16x32_bf16.  The
+// pool keeps, per column, the largest (value bits, ~row) key of the cloud
+// (ties to the lower row), folded in registers and reduced across lanes and
+// waves in fixed order: one plain store per cloud and channel.
+// ---------------------------------------------------------------------------
+
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+
+constexpr int kFT = 128;  // points per tile
+struct alignas(16) EncLds {
+  uint16_t w1t[kH1][kMlpInPad + 8];   // [n][k], rows padded to keep 16-byte alignment
+  uint16_t w2t[kH2][kH1 + 8];
+  uint16_t w3t[kH3][kH2 + 8];
+  uint16_t x[kFT][kMlpInPad + 8];
+  uint16_t h1[kFT][kH1 + 8];
+  uint16_t h2[kFT][kH2 + 8];
+  unsigned long long pool[4][kH3];
+};
+
+__device__ __forceinline__ bf16x8_t lds8(const uint16_t* p) {
+  return __builtin_bit_cast(bf16x8_t, *reinterpret_cast<const uint4*>(p));
+}
+__device__ __forceinline__ short4_t lds4(const uint16_t* p) {
+  return __builtin_bit_cast(short4_t, *reinterpret_cast<const uint2*>(p));
+}
+
+__global__ __launch_bounds__(256) void encoder_fused_kernel(const uint16_t* __restrict__ X, int npts,
+                                                            const uint16_t* __restrict__ W1b,
+                                                            const uint16_t* __restrict__ W2b,
+                                                            const uint16_t* __restrict__ W3b,
+                                                            const float* __restrict__ w,
+                                                            unsigned long long* __restrict__ pool) {
+  __shared__ EncLds L;  // 155 KB of the CU's 160 KB
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int cloud = blockIdx.x;
+  const float* b1 = w + oB1;
+  const float* b2 = w + oB2;
+  const float* b3 = w + oB3;
+  for (int i = tid; i < kMlpInPad * kH1; i += 256) L.w1t[i % kH1][i / kH1] = W1b[i];
+  for (int i = tid; i < kH1 * kH2; i += 256) L.w2t[i % kH2][i / kH2] = W2b[i];
+  for (int i = tid; i < kH2 * kH3; i += 256) L.w3t[i % kH3][i / kH3] = W3b[i];
+  const int col = lane & 15, kq = lane >> 4;  // MFMA operand column / k group
+  // running pool keys of this lane: column t * 16 + col, the wave's rows
+  unsigned long long best[kH3 / 16];
+#pragma unroll
+  for (int t = 0; t < kH3 / 16; ++t) best[t] = 0ull;
+  const uint16_t* Xc = X + (size_t)cloud * npts * kMlpInPad;
+  for (int t0 = 0; t0 < npts; t0 += kFT) {
+    __syncthreads();  // the weights are staged / the previous tile is done with x
+    {
+      const int r = tid >> 1, h = (tid & 1) * 8;
+      if (t0 + r < npts)
+        *reinterpret_cast<uint4*>(&L.x[r][h]) =
+            *reinterpret_cast<const uint4*>(Xc + (size_t)(t0 + r) * kMlpInPad + h);
+    }
+    __syncthreads();
+    const int rw = wave * 32;  // this wave's rows of the tile: rw .. rw + 31
+    // npts is a multiple of 64: a wave's 32 rows are all in the cloud or
+    // none; a wave past the end waits for the others at the next barrier
+    if (t0 + rw >= npts) continue;
+    // layer 1: 16 -> 64
+#pragma unroll
+    for (int rt = 0; rt < 2; ++rt) {
+      const short4_t a = lds4(&L.x[rw + rt * 16 + col][kq * 4]);
+#pragma unroll
+      for (int ct = 0; ct < kH1 / 16; ++ct) {
+        const short4_t b = lds4(&L.w1t[ct * 16 + col][kq * 4]);
+        float4_t acc = (float4_t){0.0f, 0.0f, 0.0f, 0.0f};
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+        const int c = ct * 16 + col;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          float x = acc[v] + b1[c];
+          x = x > 0.0f ? x : 0.0f;
+          L.h1[rw + rt * 16 + kq * 4 + v][c] = f2bf(x);
+        }
+      }
+    }
+    __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0): the wave's own h1 rows are written
+    __builtin_amdgcn_wave_barrier();
+    // layer 2: 64 -> 128
+#pragma unroll 2
+    for (int ct = 0; ct < kH2 / 16; ++ct) {
+      float4_t acc[2] = {(float4_t){0.0f, 0.0f, 0.0f, 0.0f}, (float4_t){0.0f, 0.0f, 0.0f, 0.0f}};
+#pragma unroll
+      for (int ks = 0; ks < kH1 / 32; ++ks) {
+        const bf16x8_t b = lds8(&L.w2t[ct * 16 + col][ks * 32 + kq * 8]);
+#pragma unroll
+        for (int rt = 0; rt < 2; ++rt) {
+          const bf16x8_t a = lds8(&L.h1[rw + rt * 16 + col][ks * 32 + kq * 8]);
+          acc[rt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[rt], 0, 0, 0);
+        }
+      }
+      const int c = ct * 16 + col;
+#pragma unroll
+      for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          float x = acc[rt][v] + b2[c];
+          x = x > 0.0f ? x : 0.0f;
+          L.h2[rw + rt * 16 + kq * 4 + v][c] = f2bf(x);
+        }
+    }
+    __builtin_amdgcn_s_waitcnt(0xC07F);
+    __builtin_amdgcn_wave_barrier();
+    // layer 3: 128 -> 256, four column tiles at a time, folded into the pool
+    for (int cc = 0; cc < kH3 / 16; cc += 4) {
+      float4_t acc[2][4];
+#pragma unroll
+      for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[rt][j] = (float4_t){0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+      for (int ks = 0; ks < kH2 / 32; ++ks) {
+        const bf16x8_t a0 = lds8(&L.h2[rw + col][ks * 32 + kq * 8]);
+        const bf16x8_t a1 = lds8(&L.h2[rw + 16 + col][ks * 32 + kq * 8]);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const bf16x8_t b = lds8(&L.w3t[(cc + j) * 16 + col][ks * 32 + kq * 8]);
+          acc[0][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b, acc[0][j], 0, 0, 0);
+          acc[1][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b, acc[1][j], 0, 0, 0);
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int c = (cc + j) * 16 + col;
+        const float bias = b3[c];
+        unsigned long long m = best[cc + j];
+#pragma unroll
+        for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            float x = acc[rt][j][v] + bias;
+            x = x > 0.0f ? x : 0.0f;  // never -0: the key orders by the bits
+            const uint32_t row = (uint32_t)(t0 + rw + rt * 16 + kq * 4 + v);
+            const unsigned long long key =
+                ((unsigned long long)__float_as_uint(x) << 32) | (unsigned long long)(0xFFFFFFFFu - row);
+            m = m > key ? m : key;
+          }
+        best[cc + j] = m;
+      }
+    }
+  }
+  // lanes sharing a column (col, +16, +32, +48), then the four waves in order
+#pragma unroll
+  for (int t = 0; t < kH3 / 16; ++t) {
+    unsigned long long k = best[t];
+    unsigned long long o = __shfl_xor(k, 16, 64);
+    k = k > o ? k : o;
+    o = __shfl_xor(k, 32, 64);
+    k = k > o ? k : o;
+    if (lane < 16) L.pool[wave][t * 16 + lane] = k;
+  }
+  __syncthreads();
+  {
+    const int c = tid;  // 256 channels, one a thread
+    unsigned long long k = L.pool[0][c];
+    for (int wv = 1; wv < 4; ++wv) k = k > L.pool[wv][c] ? k : L.pool[wv][c];
+    pool[(size_t)cloud * kH3 + c] = k;
+  }
+}
+
 static void gemm(const GemmArgs& a, hipStream_t s) {
   dim3 grid((a.N + 63) / 64, (a.M + 63) / 64);
   hipLaunchKernelGGL(gemm_bf16_kernel, grid, dim3(256), 0, s, a);
@@ -679,16 +849,20 @@ T* wsp(cvo_model* m, int i) {
 }
 
 // Forward over n_pairs pairs of npts points; inputs already in WS_PTS
-// ([pair][cloud][point][8] FP32).  Leaves z1 (BF16) and o (FP32).
-cvo_status forward(cvo_model* m, int n_pairs, int npts) {
+// ([pair][cloud][point][8] FP32).  Leaves z1 (BF16) and o (FP32).  fused:
+// the point encoder in one kernel (inference); otherwise layer by layer with
+// the activations h1, h2 kept for the backward pass (training).
+cvo_status forward(cvo_model* m, int n_pairs, int npts, bool fused = false) {
   cvo_ctx* ctx = m->ctx;
   hipStream_t s = ctx->stream;
   const int clouds = 2 * n_pairs;
   const int64_t P = (int64_t)clouds * npts;
   CVO_TRY(ensure(ctx, m->ws[WS_X], sizeof(uint16_t) * P * kMlpInPad));
   CVO_TRY(ensure(ctx, m->ws[WS_CEN], sizeof(float) * 12 * clouds));
-  CVO_TRY(ensure(ctx, m->ws[WS_H1], sizeof(uint16_t) * P * kH1));
-  CVO_TRY(ensure(ctx, m->ws[WS_H2], sizeof(uint16_t) * P * kH2));
+  if (!fused) {
+    CVO_TRY(ensure(ctx, m->ws[WS_H1], sizeof(uint16_t) * P * kH1));
+    CVO_TRY(ensure(ctx, m->ws[WS_H2], sizeof(uint16_t) * P * kH2));
+  }
   CVO_TRY(ensure(ctx, m->ws[WS_POOL], sizeof(unsigned long long) * clouds * kH3));
   CVO_TRY(ensure(ctx, m->ws[WS_Z], sizeof(uint16_t) * n_pairs * kZ));
   CVO_TRY(ensure(ctx, m->ws[WS_GVAL], sizeof(float) * clouds * kH3));
@@ -698,8 +872,14 @@ cvo_status forward(cvo_model* m, int n_pairs, int npts) {
   hipLaunchKernelGGL(mlp_prep_kernel, dim3(clouds), dim3(256), 0, s, wsp<float>(m, WS_PTS), npts,
                      wsp<uint16_t>(m, WS_X), wsp<float>(m, WS_CEN));
   const float* w = m->w;
+  const int64_t npool = (int64_t)clouds * kH3;
+  GemmArgs g;
+  if (fused) {
+    hipLaunchKernelGGL(encoder_fused_kernel, dim3(clouds), dim3(256), 0, s, wsp<uint16_t>(m, WS_X), npts, m->W1b,
+                       m->W2b, m->W3b, w, wsp<unsigned long long>(m, WS_POOL));
+  } else {
   // layer 1: 16 (padded) -> 64
-  GemmArgs g = gemm_args(wsp<uint16_t>(m, WS_X), kMlpInPad, 1, m->W1b, kH1, 1, (int)P, kH1, kMlpInPad);
+  g = gemm_args(wsp<uint16_t>(m, WS_X), kMlpInPad, 1, m->W1b, kH1, 1, (int)P, kH1, kMlpInPad);
   g.bias = w + oB1;
   g.relu = 1;
   g.Cb = wsp<uint16_t>(m, WS_H1);
@@ -713,7 +893,6 @@ cvo_status forward(cvo_model* m, int n_pairs, int npts) {
   g.ldcb = kH2;
   gemm(g, s);
   // layer 3: 128 -> 256 with the max-pool epilogue
-  const int64_t npool = (int64_t)clouds * kH3;
   hipLaunchKernelGGL(fill_u64_kernel, dim3((unsigned)((npool + 255) / 256)), dim3(256), 0, s,
                      wsp<unsigned long long>(m, WS_POOL), npool, 0ull);
   g = gemm_args(wsp<uint16_t>(m, WS_H2), kH2, 1, m->W3b, kH3, 1, (int)P, kH3, kH2);
@@ -722,6 +901,7 @@ cvo_status forward(cvo_model* m, int n_pairs, int npts) {
   g.pool = wsp<unsigned long long>(m, WS_POOL);
   g.group = npts;
   gemm(g, s);
+  }
   hipLaunchKernelGGL(mlp_unpack_pool_kernel, dim3((unsigned)((npool + 255) / 256)), dim3(256), 0, s,
                      wsp<unsigned long long>(m, WS_POOL), n_pairs, wsp<uint16_t>(m, WS_Z), wsp<float>(m, WS_GVAL),
                      wsp<int32_t>(m, WS_GARG));
@@ -986,7 +1166,7 @@ static cvo_status propose_impl(cvo_ctx* ctx, const cvo_model* mc, int32_t n_pair
     CVO_TRY(ensure_events(ctx, 2));
     CVO_HIP(ctx, hipEventRecord(ctx->ev_pool[0], s));
   }
-  CVO_TRY(forward(m, n_pairs, npts));
+  CVO_TRY(forward(m, n_pairs, npts, true));
   if (ctx->profiling) {
     CVO_HIP(ctx, hipEventRecord(ctx->ev_pool[1], s));
     CVO_HIP(ctx, hipEventSynchronize(ctx->ev_pool[1]));
