@@ -332,8 +332,13 @@ def leg_ba(args):
         p1.multiframe_max_iters = 1
         log(f"[bench] BA window cpu baseline: one outer iteration on {threads} threads")
         t0 = time.perf_counter()
-        _, it, secs = pyoracle.align_multiframe(p1, clouds, inits, hold, edges, threads=threads)
+        P1, it, secs = pyoracle.align_multiframe(p1, clouds, inits, hold, edges, threads=threads)
         spent = time.perf_counter() - t0
+        # the GPU's poses after the same single outer iteration
+        cvo.write_params(p1)
+        g1 = cvo.align_multiframe(dev, inits, hold, edges)
+        out["gpu_vs_fp64_oracle_1iter"] = {
+            "max_abs_pose_entry": float(np.abs(g1["poses"][:, :3, :4] - P1[:, :3, :4]).max())}
         out["cpu_baseline"] = {"value": float(secs[0]), "unit": "s/iter", "cores": threads, "kind": "port",
                                "host": cpu_host(threads),
                                "sample": f"oracle IRLS outer iteration 1 (association of all {len(edges)} edges "

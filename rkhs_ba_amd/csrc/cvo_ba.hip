@@ -53,8 +53,11 @@ constexpr int kBaMaxEdges = 4096;
 constexpr int kBaTerms = 28;                   // S (21, upper triangle), h (6), c (1)
 constexpr int kBaContrib = 121;                // B11, B12, B22 (36 each), g1, g2 (6 each), c
 constexpr int kBaRowsPerBlock = 256;           // pair pass: one row per thread
+constexpr int kPairLoads = 4;                  // pair pass: loads in flight per lane
 constexpr int kSolveThreads = 256;             // four waves: the Cholesky's trailing update spread over rows
 static_assert(kSolveThreads == 256 && 6 * kBaMaxFree <= kSolveThreads, "the solve's reductions assume four waves");
+constexpr int kChol = 8;                       // Cholesky block: columns a panel
+constexpr int kTri = 8;                        // triangular solves: steps a load batch
 
 struct BaEdge {
   double ell;
@@ -248,8 +251,8 @@ __global__ __launch_bounds__(kBaRowsPerBlock) void ba_pair_kernel(const BaDev* _
   uint32_t cnt = 0;
   double X0 = 0.0, X1 = 0.0, X2 = 0.0;
   if (r_own < P.n_src) {
-    cnt = P.counts[r_own];
-    const float4 x4 = P.src_pos[r_own];
+    cnt = ld_global(P.counts + r_own);
+    const float4 x4 = ld_global(P.src_pos + r_own);
     const double p0 = x4.x, p1 = x4.y, p2 = x4.z;
     X0 = fma(T1[2], p2, fma(T1[1], p1, fma(T1[0], p0, T1[3])));
     X1 = fma(T1[6], p2, fma(T1[5], p1, fma(T1[4], p0, T1[7])));
@@ -266,39 +269,54 @@ __global__ __launch_bounds__(kBaRowsPerBlock) void ba_pair_kernel(const BaDev* _
   double acc[kBaTerms];
 #pragma unroll
   for (int k = 0; k < kBaTerms; ++k) acc[k] = 0.0;
-  for (uint32_t base = 0; base < total; base += 64) {
-    const uint32_t idx = base + (uint32_t)lane;
-    // row of flattened pair idx: the last lane whose exclusive offset <= idx
-    int k = 0;
+  // kPairLoads pairs a lane a trip: the loads go out unconditionally (past
+  // the end a lane reloads the last pair, which it then skips) so that all
+  // are in flight before the first is used, then the pairs are summed in
+  // flattened order, lane by lane as one pair a trip would
+  for (uint32_t base = 0; base < total; base += 64 * kPairLoads) {
+    int kr[kPairLoads];
+    float4 yv[kPairLoads];
 #pragma unroll
-    for (int s = 32; s > 0; s >>= 1) {
-      const uint32_t ex = __shfl(excl, k + s, 64);
-      if (ex <= idx) k += s;
+    for (int u = 0; u < kPairLoads; ++u) {
+      const uint32_t idx = min(base + (uint32_t)(u * 64 + lane), total - 1);
+      // row of flattened pair idx: the last lane whose exclusive offset <= idx
+      int k = 0;
+#pragma unroll
+      for (int s = 32; s > 0; s >>= 1) {
+        const uint32_t ex = __shfl(excl, k + s, 64);
+        if (ex <= idx) k += s;
+      }
+      const uint32_t slot = idx - __shfl(excl, k, 64);
+      kr[u] = k;
+      yv[u] = ld_global(P.pairs4 + (size_t)(wr0 + k) * row_stride + slot);
     }
-    const uint32_t slot = idx - __shfl(excl, k, 64);
-    const double Y0 = __shfl(X0, k, 64), Y1 = __shfl(X1, k, 64), Y2 = __shfl(X2, k, 64);
-    if (idx < total) {
-      // FP64 with explicit fused multiply-adds (the library is built with
-      // contraction off for the FP32 gates)
-      const float4 y4 = P.pairs4[(size_t)(wr0 + k) * row_stride + slot];
-      const double a = (double)y4.w;
-      const double q0 = y4.x, q1 = y4.y, q2 = y4.z;
-      const double w0 = Y0 - fma(T2[2], q2, fma(T2[1], q1, fma(T2[0], q0, T2[3])));
-      const double w1 = Y1 - fma(T2[6], q2, fma(T2[5], q1, fma(T2[4], q0, T2[7])));
-      const double w2 = Y2 - fma(T2[10], q2, fma(T2[9], q1, fma(T2[8], q0, T2[11])));
-      const double res = a * fma(w2, w2, fma(w1, w1, w0 * w0));
-      const double u[6] = {w0, w1, w2, fma(Y1, w2, -Y2 * w1), fma(Y2, w0, -Y0 * w2), fma(Y0, w1, -Y1 * w0)};
-      int q = 0;
 #pragma unroll
-      for (int i = 0; i < 6; ++i)
+    for (int t = 0; t < kPairLoads; ++t) {
+      const int k = kr[t];
+      const double Y0 = __shfl(X0, k, 64), Y1 = __shfl(X1, k, 64), Y2 = __shfl(X2, k, 64);
+      if (base + (uint32_t)(t * 64 + lane) < total) {
+        // FP64 with explicit fused multiply-adds (the library is built with
+        // contraction off for the FP32 gates)
+        const float4 y4 = yv[t];
+        const double a = (double)y4.w;
+        const double q0 = y4.x, q1 = y4.y, q2 = y4.z;
+        const double w0 = Y0 - fma(T2[2], q2, fma(T2[1], q1, fma(T2[0], q0, T2[3])));
+        const double w1 = Y1 - fma(T2[6], q2, fma(T2[5], q1, fma(T2[4], q0, T2[7])));
+        const double w2 = Y2 - fma(T2[10], q2, fma(T2[9], q1, fma(T2[8], q0, T2[11])));
+        const double res = a * fma(w2, w2, fma(w1, w1, w0 * w0));
+        const double u[6] = {w0, w1, w2, fma(Y1, w2, -Y2 * w1), fma(Y2, w0, -Y0 * w2), fma(Y0, w1, -Y1 * w0)};
+        int q = 0;
 #pragma unroll
-        for (int j = i; j < 6; ++j) {
-          acc[q] = fma(u[i], u[j], acc[q]);
-          ++q;
-        }
+        for (int i = 0; i < 6; ++i)
 #pragma unroll
-      for (int i = 0; i < 6; ++i) acc[21 + i] = fma(res, u[i], acc[21 + i]);
-      acc[27] = fma(res, res, acc[27]);
+          for (int j = i; j < 6; ++j) {
+            acc[q] = fma(u[i], u[j], acc[q]);
+            ++q;
+          }
+#pragma unroll
+        for (int i = 0; i < 6; ++i) acc[21 + i] = fma(res, u[i], acc[21 + i]);
+        acc[27] = fma(res, res, acc[27]);
+      }
     }
   }
   // fixed-order block partial: wave shuffles, then the four waves in LDS
@@ -405,6 +423,22 @@ __device__ inline void trailing_index(int e, int n, int& i, int& k) {
   i = k + (e - m * (m + 1) / 2);
 }
 
+// Orders one wave's LDS accesses across the point (other lanes' writes before
+// it are seen by reads after it); the compiler moves no LDS access over it.
+__device__ __forceinline__ void panel_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Lane l's value of v (l uniform), moved through scalar registers.
+__device__ __forceinline__ double readlane_f64(double v, int l) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, l);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 // Sum and maximum over the four waves of a solve block (waves in order);
 // every thread calls them and gets the result.  The leading barrier keeps a
 // previous call's readers ahead of this call's writers.
@@ -494,6 +528,7 @@ __global__ __launch_bounds__(kSolveThreads) void ba_solve_kernel(BaDev* __restri
   __shared__ double s_g[kBaMaxDof];
   __shared__ double s_inv[kBaMaxDof];  // 1 / L(j, j)
   __shared__ double s_red[kSolveThreads / 64];
+  __shared__ int s_fail;
   const bool candidate = ba->eval_is_candidate != 0;
   int cur = ba->h_cur;
   double mu = ba->mu, v = ba->v, F = ba->F;
@@ -573,39 +608,61 @@ __global__ __launch_bounds__(kSolveThreads) void ba_solve_kernel(BaDev* __restri
       s_A[i * ld + j] = x;
     }
     __syncthreads();
-    // right-looking Cholesky: column j scaled, then the trailing update with
-    // the waves over rows and the lanes over columns; the diagonal's
-    // reciprocal goes to s_inv (the pivot itself is not rewritten while
-    // other waves read it)
+    // blocked right-looking Cholesky, kChol columns a block: wave 0 factors
+    // the block's panel (rows below the block, its columns) alone -- a wave's
+    // LDS operations complete in order, so wave-scope fences order it --
+    // then all four waves subtract the block's columns from the trailing
+    // triangle, per element in the same order as column by column (the
+    // result is the unblocked one bit for bit); the diagonal's reciprocal
+    // goes to s_inv
     bool fail = false;
-    for (int j = 0; j < n; ++j) {
-      const double ajj = s_A[j * ld + j];
-      if (!(ajj > 0.0)) {
+    for (int j0 = 0; j0 < n; j0 += kChol) {
+      const int jb = min(kChol, n - j0);
+      if (wave == 0) {
+        int f = 0;
+        for (int c = 0; c < jb; ++c) {
+          const int j = j0 + c;
+          const double ajj = s_A[j * ld + j];
+          if (!(ajj > 0.0)) {
+            f = 1;
+            break;
+          }
+          const double inv = 1.0 / sqrt(ajj);
+          for (int i = j + 1 + lane; i < n; i += 64) s_A[i * ld + j] *= inv;
+          if (lane == 0) s_inv[j] = inv;
+          panel_lds_sync();  // the scaled column, read across lanes below
+          for (int i = j + 1 + lane; i < n; i += 64) {
+            const double lij = s_A[i * ld + j];
+            for (int k = j + 1; k < j0 + jb && k <= i; ++k) s_A[i * ld + k] -= lij * s_A[k * ld + j];
+          }
+          panel_lds_sync();  // the next column's diagonal and entries
+        }
+        if (lane == 0) s_fail = f;
+      }
+      __syncthreads();
+      if (s_fail) {  // uniform
         fail = true;
         break;
       }
-      const double inv = 1.0 / sqrt(ajj);
-      for (int i = j + 1 + tid; i < n; i += kSolveThreads) s_A[i * ld + j] *= inv;
-      if (tid == 0) s_inv[j] = inv;
-      __syncthreads();
-      // the trailing triangle {(i, k): j < k <= i < n} as M flat elements,
-      // four per thread a trip with all reads issued before the writes (each
-      // element has one writer; column j is only read here)
-      const int M = (n - j - 1) * (n - j) / 2;
+      // the trailing triangle {(i, k): j0 + jb <= k <= i < n} as M flat
+      // elements (the columns right of column j0 + jb - 1), four per thread
+      // a trip with all reads issued before the writes
+      const int rem = n - j0 - jb;
+      const int M = rem * (rem + 1) / 2;
       for (int e0 = tid; e0 < M; e0 += 4 * kSolveThreads) {
         int ii[4], kk[4];
-        double lij[4], lkj[4], aik[4];
+        double x[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
           const int e = min(e0 + u * kSolveThreads, M - 1);
           trailing_index(e, n, ii[u], kk[u]);
-          lij[u] = s_A[ii[u] * ld + j];
-          lkj[u] = s_A[kk[u] * ld + j];
-          aik[u] = s_A[ii[u] * ld + kk[u]];
+          double a = s_A[ii[u] * ld + kk[u]];
+          for (int c = 0; c < jb; ++c) a -= s_A[ii[u] * ld + j0 + c] * s_A[kk[u] * ld + j0 + c];
+          x[u] = a;
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u)
-          if (e0 + u * kSolveThreads < M) s_A[ii[u] * ld + kk[u]] = aik[u] - lij[u] * lkj[u];
+          if (e0 + u * kSolveThreads < M) s_A[ii[u] * ld + kk[u]] = x[u];
       }
       __syncthreads();
     }
@@ -616,25 +673,55 @@ __global__ __launch_bounds__(kSolveThreads) void ba_solve_kernel(BaDev* __restri
       __syncthreads();
       continue;
     }
-    // L y = -g, then L^T d = y: one wave, rows r and r + 64 per lane
+    // L y = -g, then L^T d = y: one wave, rows r and r + 64 per lane; the
+    // solved entry moves by readlane, and the L entries and reciprocals of
+    // kTri steps are read from LDS ahead of the steps that use them (rows
+    // clamped to n - 1 for the loads; those values are not used)
     if (wave == 0) {
       const int r0 = lane, r1 = lane + 64;
+      const int c0 = min(r0, n - 1), c1 = min(r1, n - 1);
       double b0 = r0 < n ? -s_g[r0] : 0.0, b1 = r1 < n ? -s_g[r1] : 0.0;
-      for (int i = 0; i < n; ++i) {
-        const double bi = (i < 64) ? __shfl(b0, i, 64) : __shfl(b1, i - 64, 64);
-        const double yi = bi * s_inv[i];
-        if (r0 == i) b0 = yi;
-        if (r1 == i) b1 = yi;
-        if (r0 > i && r0 < n) b0 -= s_A[r0 * ld + i] * yi;
-        if (r1 > i && r1 < n) b1 -= s_A[r1 * ld + i] * yi;
+      for (int i0 = 0; i0 < n; i0 += kTri) {
+        double l0[kTri], l1[kTri], sv[kTri];
+#pragma unroll
+        for (int u = 0; u < kTri; ++u) {
+          const int i = min(i0 + u, n - 1);
+          l0[u] = s_A[c0 * ld + i];
+          l1[u] = s_A[c1 * ld + i];
+          sv[u] = s_inv[i];
+        }
+#pragma unroll
+        for (int u = 0; u < kTri; ++u) {
+          const int i = i0 + u;
+          if (i >= n) break;  // uniform
+          const double bi = i < 64 ? readlane_f64(b0, i) : readlane_f64(b1, i - 64);
+          const double yi = bi * sv[u];
+          if (r0 == i) b0 = yi;
+          if (r1 == i) b1 = yi;
+          if (r0 > i && r0 < n) b0 -= l0[u] * yi;
+          if (r1 > i && r1 < n) b1 -= l1[u] * yi;
+        }
       }
-      for (int i = n - 1; i >= 0; --i) {
-        const double bi = (i < 64) ? __shfl(b0, i, 64) : __shfl(b1, i - 64, 64);
-        const double xi = bi * s_inv[i];
-        if (r0 == i) b0 = xi;
-        if (r1 == i) b1 = xi;
-        if (r0 < i) b0 -= s_A[i * ld + r0] * xi;
-        if (r1 < i) b1 -= s_A[i * ld + r1] * xi;
+      for (int i0 = n - 1; i0 >= 0; i0 -= kTri) {
+        double l0[kTri], l1[kTri], sv[kTri];
+#pragma unroll
+        for (int u = 0; u < kTri; ++u) {
+          const int i = max(i0 - u, 0);
+          l0[u] = s_A[i * ld + c0];
+          l1[u] = s_A[i * ld + c1];
+          sv[u] = s_inv[i];
+        }
+#pragma unroll
+        for (int u = 0; u < kTri; ++u) {
+          const int i = i0 - u;
+          if (i < 0) break;  // uniform
+          const double bi = i < 64 ? readlane_f64(b0, i) : readlane_f64(b1, i - 64);
+          const double xi = bi * sv[u];
+          if (r0 == i) b0 = xi;
+          if (r1 == i) b1 = xi;
+          if (r0 < i) b0 -= l0[u] * xi;
+          if (r1 < i) b1 -= l1[u] * xi;
+        }
       }
       if (r0 < n) s_d[r0] = b0;
       if (r1 < n) s_d[r1] = b1;

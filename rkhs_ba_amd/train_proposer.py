@@ -3,7 +3,13 @@ and backward kernels (cvo_model_train_step) and writes its weights.
 
 Data: seeded pairs of the planes/spheres generator (two independent samples
 of a random scene, the second moved by a uniform random rotation and 2 m), 2048
-points per cloud; the label is the generating source-to-target transform.
+points per cloud.  Labels (--label cvo, the default): the converged poses of
+a batched multi-start CVO registration of each training pair (super-Fibonacci
+start rotations with centroid-matching translations, the plain grid's
+construction, at the global-registration parameters); the best-cos converged
+pose is the label.  A pair whose converged pose lies farther than 0.05 rad /
+0.1 m from the generating transform (CVO settled in another basin) is dropped.
+--label generator uses the generating transform itself.
 
     python -m rkhs_ba_amd.train_proposer --pairs 50000 --out rkhs_ba_amd/weights/proposer_k8.npy
 """
@@ -37,6 +43,39 @@ def dataset(first_seed, n, npts, workers):
             np.concatenate([p[2] for p in parts]))
 
 
+def cvo_labels(a, b, t, starts, max_iter, score_ell=0.3, accept_rad=0.05, accept_m=0.1):
+    """Labels from converged multi-start CVO: (kept index, labels (n, 7),
+    seconds).  The label is the source-to-target transform (align-init
+    convention) of the best-cos start after max_iter iterations."""
+    from rkhs_ba_amd import CvoGPU, read_params_yaml, synth
+    from rkhs_ba_amd.proposer import grid_starts, rot_to_quat
+    params = read_params_yaml(os.path.join(ROOT, "rkhs_ba_amd", "configs", "multistart_20k.yaml"))
+    params.MAX_ITER = max_iter
+    cvo = CvoGPU(params=params)
+    keep, labels = [], []
+    t0 = time.time()
+    try:
+        for i in range(a.shape[0]):
+            src = synth.Cloud(np.ascontiguousarray(a[i, :, :3]), np.ascontiguousarray(a[i, :, 3:8]))
+            tgt = synth.Cloud(np.ascontiguousarray(b[i, :, :3]), np.ascontiguousarray(b[i, :, 3:8]))
+            r = cvo.align_multistart(src, tgt, grid_starts(src, tgt, starts), score_ell, mode=0)
+            T = np.linalg.inv(np.asarray(r["T_best"], np.float64))  # align output -> init convention
+            Tg = np.eye(4)
+            Tg[:3, :3] = synth.quat_to_rot(t[i, :4].astype(np.float64))
+            Tg[:3, 3] = t[i, 4:7]
+            E = np.linalg.inv(Tg) @ T
+            ang = float(np.arccos(np.clip((np.trace(E[:3, :3]) - 1) / 2, -1, 1)))
+            if ang <= accept_rad and np.linalg.norm(E[:3, 3]) <= accept_m:
+                keep.append(i)
+                labels.append(np.concatenate([rot_to_quat(T[:3, :3]), T[:3, 3]]).astype(np.float32))
+            if (i + 1) % 2000 == 0:
+                print(f"[train] labelled {i + 1}/{a.shape[0]}, kept {len(keep)}, "
+                      f"{(time.time() - t0) / (i + 1) * 1e3:.1f} ms per pair", flush=True)
+    finally:
+        cvo.close()
+    return np.array(keep, np.int64), np.array(labels, np.float32).reshape(-1, 7), time.time() - t0
+
+
 def quat_angle_deg(q, qs):
     d = np.abs(np.sum(q * qs, axis=-1))
     return np.degrees(2 * np.arccos(np.clip(d, -1, 1)))
@@ -67,6 +106,9 @@ def main():
     ap.add_argument("--workers", type=int, default=16)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--out", default=os.path.join(ROOT, "rkhs_ba_amd", "weights", "proposer_k8.npy"))
+    ap.add_argument("--label", choices=("cvo", "generator"), default="cvo")
+    ap.add_argument("--label-starts", type=int, default=24, help="multi-start CVO starts per training pair")
+    ap.add_argument("--label-iters", type=int, default=300, help="CVO iterations per start")
     args = ap.parse_args()
     from rkhs_ba_amd import CvoGPU
     from rkhs_ba_amd.proposer import PoseProposer
@@ -75,6 +117,14 @@ def main():
     a, b, t = dataset(1_000_000 * (args.seed + 1), args.pairs, args.npts, args.workers)
     va, vb, vt = dataset(7_000_000 + args.seed, args.val, args.npts, args.workers)
     print(f"[train] data ready in {time.time() - t0:.1f} s", flush=True)
+    label_info = {"label": args.label}
+    if args.label == "cvo":
+        keep, lab, secs = cvo_labels(a, b, t, args.label_starts, args.label_iters)
+        label_info.update(starts=args.label_starts, iterations=args.label_iters, kept=int(keep.size),
+                          of=int(a.shape[0]), seconds=secs)
+        print(f"[train] CVO labels: kept {keep.size} of {a.shape[0]} pairs in {secs:.1f} s", flush=True)
+        a, b, t = a[keep], b[keep], lab
+        args.pairs = int(keep.size)
     cvo = CvoGPU()
     prop = PoseProposer(cvo, seed=args.seed)
     rng = np.random.default_rng(args.seed)
@@ -106,6 +156,7 @@ def main():
             break
     prop.save(args.out)
     summary = {"pairs": args.pairs, "npts": args.npts, "batch": args.batch, "lr": args.lr, "steps": step,
+               "labels": label_info,
                "train_seconds": time.time() - t_train, "ms_per_step": (time.time() - t_train) / max(1, step) * 1e3,
                "history": history}
     with open(os.path.splitext(args.out)[0] + ".json", "w") as f:

@@ -126,6 +126,37 @@ def test_align_with_global_angle_registration(cvo_gpu, oracle):
     assert rot_dist_rad(T[:3, :3].T, R) < 0.3  # the table rotation plus the pair's own 10 degrees
 
 
+def test_reference_global_yaml_follows_the_oracle(cvo_gpu, oracle):
+    """The reference's own cvo_global_params.yaml (is_global_angle_registration
+    1, the search at ell_init_first_frame = 10, colour and semantics on): the
+    library's search picks the oracle's table entry on a street pair turned by
+    a table rotation, and 20 iterations from it follow the oracle's
+    registration from the same start within 1e-5, the two-view policy bound
+    (FP64 sums in another order; measured 1.1e-6 rad / 3.6e-6 m)."""
+    from rkhs_ba_amd import read_params_yaml
+    p = read_params_yaml(os.path.join(GOLDEN, "yaml", "cvo_global_params.yaml"))
+    assert p.is_global_angle_registration == 1
+    p.MAX_ITER = 20
+    src, tgt, _ = synth.street_pair(3, n=3000)
+    tgt = tgt.transformed(synth.rigid(reference_table()[11].astype(np.float64), np.zeros(3)))
+    cvo_gpu.write_params(p)
+    say("reference global YAML: search")
+    T_s, _ = cvo_gpu.get_nearest_init_pose(src, tgt, p.ell_init_first_frame)
+    want = oracle_nearest(oracle, p, src, tgt, p.ell_init_first_frame)
+    assert table_index(T_s) == want
+    G = np.eye(4, dtype=np.float32)
+    if want >= 0:
+        G[:3, :3] = reference_table()[want]
+    Gi = np.linalg.inv(G.astype(np.float64)).astype(np.float32)
+    say(f"reference global YAML: align from table entry {want}")
+    ret, T, secs, iters, _ = cvo_gpu.align(src, tgt, np.eye(4))
+    p0 = p.copy()
+    p0.is_global_angle_registration = 0
+    To, it_o, ret_o, _ = oracle.align(p0, src, tgt, Gi, precision=oracle.POLICY)
+    assert (ret, iters) == (ret_o, it_o)
+    assert rot_dist_rad(T, To) <= 1e-5 and trans_err(T, To) <= 1e-5, (rot_dist_rad(T, To), trans_err(T, To))
+
+
 def test_align_result_inverts_its_init(cvo_gpu):
     # an init whose inverse is exact in float: both overloads start from the
     # same pose and give the same registration bit for bit

@@ -1,0 +1,66 @@
+"""Learned global registration on 50 held-out pairs against the FP64 oracle.
+
+Pairs: 20k/20k planes-and-spheres scenes moved by a uniform random rotation
+and 2 m (seeds 7100-7149, never used in training or in the other tests).
+Each pair is registered by the plain 96-start grid and by the learned path
+(8 proposals refined by batched CVO, judged by the cos angle, grid fallback
+below the confidence floor).  Success is the north star's: the final pose
+within 0.01 rad and 1 cm of the FP64 oracle's converged pose, the oracle
+started from the generating transform (its basin) with the same parameters.
+Bars: the learned path succeeds at least as often as the grid, at >= 5x its
+throughput.
+
+About ten minutes (50 single-thread FP64 oracle registrations on the host):
+skipped unless RKHS_BA_AMD_SLOW=1.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import CONFIGS, params_from_yaml, rot_dist_rad, trans_err
+
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skipif(os.environ.get("RKHS_BA_AMD_SLOW") != "1",
+                                 reason="50-pair evaluation of several minutes; run with RKHS_BA_AMD_SLOW=1")]
+
+N_PAIRS = 50
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.mark.timeout(1200)
+def test_learned_success_on_50_heldout_pairs(cvo_gpu, oracle):
+    from rkhs_ba_amd import synth
+    from rkhs_ba_amd.proposer import DEFAULT_MIN_COS, PoseProposer, register_global
+    p = params_from_yaml("multistart_20k.yaml")
+    p.MAX_ITER = 1000
+    cvo_gpu.write_params(p)
+    prop = PoseProposer.load(cvo_gpu)
+    ok_g = ok_l = fallbacks = 0
+    t_g = t_l = 0.0
+    try:
+        for s in range(N_PAIRS):
+            src, tgt, Tgt = synth.planes_spheres_pair(7100 + s, 20000, 20000, uniform_rot=True, trans=2.0)
+            g = register_global(cvo_gpu, src, tgt, None)
+            l = register_global(cvo_gpu, src, tgt, prop, min_cos=DEFAULT_MIN_COS)
+            To, _, _, _ = oracle.align(p, src, tgt, Tgt, precision=oracle.FP64)
+            sg = rot_dist_rad(g["T_best"], To) <= 0.01 and trans_err(g["T_best"], To) <= 0.01
+            sl = rot_dist_rad(l["T_best"], To) <= 0.01 and trans_err(l["T_best"], To) <= 0.01
+            ok_g += sg
+            ok_l += sl
+            fallbacks += l["fallback"]
+            t_g += g["seconds"]
+            t_l += l["seconds"]
+            say(f"pair {s}: grid {'ok' if sg else 'FAIL'} {g['seconds']:.2f} s, learned {'ok' if sl else 'FAIL'} "
+                f"{l['seconds']:.2f} s{' (fallback)' if l['fallback'] else ''}")
+    finally:
+        prop.close()
+    say(f"grid {ok_g}/{N_PAIRS} in {t_g:.1f} s, learned {ok_l}/{N_PAIRS} in {t_l:.1f} s "
+        f"({t_g / t_l:.1f}x), {fallbacks} fallbacks")
+    assert ok_l >= ok_g
+    assert t_l * 5 <= t_g
