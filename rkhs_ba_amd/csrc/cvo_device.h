@@ -22,6 +22,11 @@ __device__ __forceinline__ float4 ld_global(const float4* p) {
   const v4f v = *(const __attribute__((address_space(1))) v4f*)p;
   return make_float4(v.x, v.y, v.z, v.w);
 }
+__device__ __forceinline__ uint2 ld_global(const uint2* p) {
+  typedef unsigned v2u __attribute__((ext_vector_type(2)));
+  const v2u v = *(const __attribute__((address_space(1))) v2u*)p;
+  return make_uint2(v.x, v.y);
+}
 
 __device__ __forceinline__ void affine3(const float* A, const float* b, float y0, float y1, float y2, float& o0,
                                         float& o1, float& o2) {

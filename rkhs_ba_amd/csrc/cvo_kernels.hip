@@ -1548,6 +1548,7 @@ __device__ __forceinline__ void controller_tail(const AssocProblem& P, const KCo
 
 // One thread per source row; grid (row blocks, problems).  The last block of
 // a problem runs its controller step.
+constexpr int kStepLoads = 4;  // pairs whose loads are in flight together per thread
 __device__ __forceinline__ void step_pass(const AssocProblem& P, const KConsts& kc) {
   ProblemState* st = P.st;
   if (st->done) return;
@@ -1585,12 +1586,10 @@ __device__ __forceinline__ void step_pass(const AssocProblem& P, const KConsts& 
       temp_ell = (float)(((double)px / 500.0 + 1.0) * (double)ell);
     }
     const float tc = (float)(1.0 / (2.0 * (double)temp_ell * (double)temp_ell));
-    const uint32_t cnt = P.counts[r];
+    const uint32_t cnt = ld_global(P.counts + r);
     const uint2* row = P.pairs + (size_t)r * P.cap_alloc;
-    for (uint32_t s = 0; s < cnt; ++s) {
-      const uint2 e = row[s];
-      const float a = __uint_as_float(e.y);
-      const float4 y4 = G.pos[e.x];
+    // one pair's Taylor terms, summed into acc in pair order
+    auto step_pair = [&](const float a, const float4 y4) {
       float z0, z1, z2;
       affine3(mp.A, mp.b, y4.x, y4.y, y4.z, z0, z1, z2);
       // xi^k z: xiz = w x z + v, xi^{k+1} z = w x xi^k z
@@ -1614,6 +1613,23 @@ __device__ __forceinline__ void step_pass(const AssocProblem& P, const KConsts& 
       acc[2] += ad * ((double)(delta + beta * gamma) + (double)(beta * beta * beta) / 6.0);
       acc[3] += ad * ((double)(epsil + beta * delta) + 0.5 * bd * bd * gd + 0.5 * gd * gd +
                       (1.0 / 24.0) * bd * bd * bd * bd);
+    };
+    // kStepLoads pairs a trip: their slots, then their target gathers, go out
+    // before the first pair is summed
+    uint32_t s = 0;
+    for (; s + kStepLoads <= cnt; s += kStepLoads) {
+      uint2 e[kStepLoads];
+      float4 y[kStepLoads];
+#pragma unroll
+      for (int u = 0; u < kStepLoads; ++u) e[u] = ld_global(row + s + u);
+#pragma unroll
+      for (int u = 0; u < kStepLoads; ++u) y[u] = ld_global(G.pos + e[u].x);
+#pragma unroll
+      for (int u = 0; u < kStepLoads; ++u) step_pair(__uint_as_float(e[u].y), y[u]);
+    }
+    for (; s < cnt; ++s) {
+      const uint2 e = ld_global(row + s);
+      step_pair(__uint_as_float(e.y), ld_global(G.pos + e.x));
     }
   }
   // block partial in fixed order: wave shuffles then LDS over the 4 waves
