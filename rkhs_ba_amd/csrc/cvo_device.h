@@ -27,6 +27,21 @@ __device__ __forceinline__ uint2 ld_global(const uint2* p) {
   const v2u v = *(const __attribute__((address_space(1))) v2u*)p;
   return make_uint2(v.x, v.y);
 }
+// Stores through a pointer known to address global memory (a flat store would
+// also hold the LDS wait counter until the store is acknowledged).
+__device__ __forceinline__ void st_global(uint32_t* p, uint32_t v) {
+  *(__attribute__((address_space(1))) uint32_t*)p = v;
+}
+__device__ __forceinline__ void st_global(uint2* p, uint2 v) {
+  typedef unsigned v2u __attribute__((ext_vector_type(2)));
+  const v2u t = {v.x, v.y};
+  *(__attribute__((address_space(1))) v2u*)p = t;
+}
+__device__ __forceinline__ void st_global(float4* p, float4 v) {
+  typedef float v4f __attribute__((ext_vector_type(4)));
+  const v4f t = {v.x, v.y, v.z, v.w};
+  *(__attribute__((address_space(1))) v4f*)p = t;
+}
 
 __device__ __forceinline__ void affine3(const float* A, const float* b, float y0, float y1, float y2, float& o0,
                                         float& o1, float& o2) {

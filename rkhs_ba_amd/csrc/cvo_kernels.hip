@@ -254,7 +254,9 @@ __device__ __forceinline__ bool pair_kernel(const KConsts& kc, const ExactK& ek,
   if (kc.use_geo && !(d2 < d2t)) return false;
   float geo_sim = 1.0f;
   if (kc.use_gtype) {
-    geo_sim = geo_similarity(xg, yg);
+    // yg always addresses the target cloud's type vectors in global memory
+    const float ygl[2] = {ld_global(yg), ld_global(yg + 1)};
+    geo_sim = geo_similarity(xg, ygl);
     if (geo_sim < gmin) return false;
   }
   float d2c = 0.0f, d2s = 0.0f;
@@ -297,10 +299,10 @@ template <bool NOBA = false>
 __device__ __forceinline__ void store_pair(const AssocProblem& P, int mode, int r, uint32_t K, uint32_t slot,
                                            uint32_t pidx, const float4& y4, float a) {
   if (!NOBA && mode == MODE_BA) {
-    P.pairs4[(size_t)r * K + slot] = make_float4(y4.x, y4.y, y4.z, a);
-    if (P.pair_ids) P.pair_ids[(size_t)r * K + slot] = pidx;
+    st_global(P.pairs4 + (size_t)r * K + slot, make_float4(y4.x, y4.y, y4.z, a));
+    if (P.pair_ids) st_global(P.pair_ids + (size_t)r * K + slot, pidx);
   } else if (P.pairs) {
-    P.pairs[(size_t)r * P.cap_alloc + slot] = make_uint2(pidx, __float_as_uint(a));
+    st_global(P.pairs + (size_t)r * P.cap_alloc + slot, make_uint2(pidx, __float_as_uint(a)));
   }
 }
 
@@ -583,7 +585,9 @@ __device__ __forceinline__ void assoc_block(const AssocProblem& P, const KConsts
   Maps mp;
   load_maps(st, mp);
   const ExactK ek = exact_consts(kc);
-  const GridLevel& GL = P.grid->lv[st->level];
+  // a copy: the pair stores below cannot alias it, so its fields stay in
+  // registers instead of being reloaded after every store
+  const GridLevel GL = P.grid->lv[st->level];
   const int row0 = lb * RB;
   const int nrows = min(RB, P.n_src - row0);
   if (FC != 0) {
@@ -721,7 +725,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ONE ? 5 
   if constexpr (ONE) {
     assoc_pass<FC, KC, false, true>(one, kc);
   } else {
-    assoc_pass<FC, KC, false, false>(probs[find_problem(probs, n_probs, (int)blockIdx.x)], kc);
+    // a register copy of the problem: the pair stores cannot alias it
+    const AssocProblem P = probs[find_problem(probs, n_probs, (int)blockIdx.x)];
+    assoc_pass<FC, KC, false, false>(P, kc);
   }
 }
 
@@ -733,7 +739,8 @@ __global__ __launch_bounds__(kBlock) void knn_assoc_kernel(const AssocProblem* _
   if constexpr (ONE) {
     assoc_pass<FC, KC, true, false>(one, kc);
   } else {
-    assoc_pass<FC, KC, true, false>(probs[find_problem(probs, n_probs, (int)blockIdx.x)], kc);
+    const AssocProblem P = probs[find_problem(probs, n_probs, (int)blockIdx.x)];
+    assoc_pass<FC, KC, true, false>(P, kc);
   }
 }
 
