@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "cvo_device.h"
 #include "cvo_internal.h"
 
 namespace cvo {
@@ -67,8 +68,8 @@ __device__ __forceinline__ void for_each_in_box(const GridLevel& GL, const CellB
       const int iz = myrun / B.span1, iy = myrun - iz * B.span1;
       const uint32_t c0 =
           ((uint32_t)(B.lo2 + iz) * (uint32_t)GL.ny + (uint32_t)(B.lo1 + iy)) * (uint32_t)GL.nx + (uint32_t)B.lo0;
-      rs = GL.cell_start[c0];
-      rl = GL.cell_start[c0 + (uint32_t)B.nx_run] - rs;
+      rs = ld_global(GL.cell_start + c0);
+      rl = ld_global(GL.cell_start + c0 + (uint32_t)B.nx_run) - rs;
     }
     uint32_t incl = rl;
 #pragma unroll

@@ -29,8 +29,9 @@ __device__ __forceinline__ uint2 ld_global(const uint2* p) {
 }
 // Stores through a pointer known to address global memory (a flat store would
 // also hold the LDS wait counter until the store is acknowledged).
-__device__ __forceinline__ void st_global(uint32_t* p, uint32_t v) {
-  *(__attribute__((address_space(1))) uint32_t*)p = v;
+template <class T>
+__device__ __forceinline__ void st_global(T* p, T v) {
+  *(__attribute__((address_space(1))) T*)p = v;
 }
 __device__ __forceinline__ void st_global(uint2* p, uint2 v) {
   typedef unsigned v2u __attribute__((ext_vector_type(2)));

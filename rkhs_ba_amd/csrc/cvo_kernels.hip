@@ -346,8 +346,8 @@ __device__ __forceinline__ void assoc_row(const AssocProblem& P, const KConsts& 
   const float qg = (float)(1.0 / (2.0 * (double)l * (double)l));
   float xg[2] = {0.0f, 0.0f};
   if (kc.use_gtype) {
-    xg[0] = P.src_geo[2 * (size_t)r];
-    xg[1] = P.src_geo[2 * (size_t)r + 1];
+    xg[0] = ld_global(P.src_geo + 2 * (size_t)r);
+    xg[1] = ld_global(P.src_geo + 2 * (size_t)r + 1);
   }
   // query box in the target frame
   float q0, q1, q2;
@@ -474,7 +474,7 @@ __device__ __forceinline__ void knn_row(const AssocProblem& P, const KConsts& kc
   if (Ku == 0) return;
   const int F = Dim<FC>::get(kc.F);
   const int KS = Dim<KC>::get(kc.Ks);
-  float4 x4 = P.src_pos[r];
+  float4 x4 = ld_global(P.src_pos + r);
   float x0 = x4.x, x1 = x4.y, x2 = x4.z;
   if (mp.src_xf) {
     float t0, t1, t2;
@@ -486,8 +486,8 @@ __device__ __forceinline__ void knn_row(const AssocProblem& P, const KConsts& kc
   const float qg = (float)(1.0 / (2.0 * (double)l * (double)l));
   float xg[2] = {0.0f, 0.0f};
   if (kc.use_gtype) {
-    xg[0] = P.src_geo[2 * (size_t)r];
-    xg[1] = P.src_geo[2 * (size_t)r + 1];
+    xg[0] = ld_global(P.src_geo + 2 * (size_t)r);
+    xg[1] = ld_global(P.src_geo + 2 * (size_t)r + 1);
   }
   float q0, q1, q2;
   affine3(mp.Rq, mp.Tq, x0, x1, x2, q0, q1, q2);
@@ -496,7 +496,7 @@ __device__ __forceinline__ void knn_row(const AssocProblem& P, const KConsts& kc
   // key of a target, ~0 outside the ball (no ball key reaches it: d2 bits of
   // a finite d2 < d2t stay below 0x7f800000)
   auto key_of = [&](uint32_t pidx, float4& y4, float& y0, float& y1, float& y2) -> uint64_t {
-    y4 = GL.pos[pidx];
+    y4 = ld_global(GL.pos + pidx);
     affine3(mp.A, mp.b, y4.x, y4.y, y4.z, y0, y1, y2);
     const float dx = y0 - x0, dy = y1 - x1, dz = y2 - x2;
     const float d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
@@ -522,11 +522,23 @@ __device__ __forceinline__ void knn_row(const AssocProblem& P, const KConsts& kc
     float4 y4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (valid) {
       const uint64_t key = key_of(pidx, y4, y0, y1, y2);
-      if (selected(key, thr, tsh))
+      if (selected(key, thr, tsh)) {
+        // compile-time dimensions: the target's features and class vector
+        // loaded into registers through global-address loads
+        float yfr[FC > 0 ? FC : 1], ysr[KC > 0 ? KC : 1];
+        if (FC > 0) {
+#pragma unroll
+          for (int f = 0; f < (FC > 0 ? FC : 1); ++f) yfr[f] = ld_global(GL.feat + (size_t)pidx * F + f);
+        }
+        if (KC > 0) {
+#pragma unroll
+          for (int k = 0; k < (KC > 0 ? KC : 1); ++k) ysr[k] = ld_global(GL.sem + (size_t)pidx * KS + k);
+        }
         pass = pair_kernel<FC, KC>(kc, ek, x0, x1, x2, y0, y1, y2, d2t, qg, l, xf,
-                                   FC != 0 ? GL.feat + (size_t)pidx * F : nullptr, xs,
-                                   KC != 0 ? GL.sem + (size_t)pidx * KS : nullptr, xg,
+                                   FC > 0 ? yfr : (FC != 0 ? GL.feat + (size_t)pidx * F : nullptr), xs,
+                                   KC > 0 ? ysr : (KC != 0 ? GL.sem + (size_t)pidx * KS : nullptr), xg,
                                    kc.use_gtype ? GL.geo + 2 * (size_t)pidx : nullptr, a, 0.001f);
+      }
     }
     const uint32_t gm = group_bits<G>(__ballot(pass));
     const uint32_t slot = stored + (uint32_t)__popc(gm & ((1u << g) - 1u));
@@ -592,11 +604,11 @@ __device__ __forceinline__ void assoc_block(const AssocProblem& P, const KConsts
   const int nrows = min(RB, P.n_src - row0);
   if (FC != 0) {
     const int F = Dim<FC>::get(kc.F);
-    for (int t = tid; t < nrows * F; t += kBlock) L.feat[t] = P.src_feat[(size_t)row0 * F + t];
+    for (int t = tid; t < nrows * F; t += kBlock) L.feat[t] = ld_global(P.src_feat + (size_t)row0 * F + t);
   }
   if (KC != 0) {
     const int KS = Dim<KC>::get(kc.Ks);
-    for (int t = tid; t < nrows * KS; t += kBlock) L.sem[t] = P.src_sem[(size_t)row0 * KS + t];
+    for (int t = tid; t < nrows * KS; t += kBlock) L.sem[t] = ld_global(P.src_sem + (size_t)row0 * KS + t);
   }
   __syncthreads();
   RowResult res;
@@ -625,9 +637,9 @@ __device__ __forceinline__ void assoc_block(const AssocProblem& P, const KConsts
     if (r < P.n_src) {
       // a capped row is counted by the capped-row kernel, which rewrites it
       cnt = res.overflow ? 0u : res.stored;
-      P.counts[r] = cnt;
+      st_global(P.counts + r, cnt);
       if (res.overflow) {
-        P.ovf_stamp[r] = (uint32_t)iter + 1u;
+        st_global(P.ovf_stamp + r, (uint32_t)iter + 1u);
         atomicAdd(&P.st->n_overflow, 1u);
       }
     }
@@ -639,15 +651,15 @@ __device__ __forceinline__ void assoc_block(const AssocProblem& P, const KConsts
   if (tid < 7) {
     double s = 0.0;
     for (int q = 0; q < RB; ++q) s += L.row[q][tid];
-    P.part_a[(size_t)lb * 8 + tid] = s;
+    st_global(P.part_a + (size_t)lb * 8 + tid, s);
   } else if (tid == 7) {
     uint32_t nz = 0, mx = 0;
     for (int q = 0; q < RB; ++q) {
       nz += L.cnt[q];
       mx = max(mx, L.cnt[q]);
     }
-    P.part_cnt[(size_t)lb * 2] = nz;
-    P.part_cnt[(size_t)lb * 2 + 1] = mx;
+    st_global(P.part_cnt + (size_t)lb * 2, nz);
+    st_global(P.part_cnt + (size_t)lb * 2 + 1, mx);
   }
 }
 
@@ -812,7 +824,7 @@ __device__ uint32_t capped_row_ball(const AssocProblem& P, const KConsts& kc, co
   const int mode = kc.mode;
   const int F = Dim<FC>::get(kc.F);
   const int KS = Dim<KC>::get(kc.Ks);
-  float4 x4 = P.src_pos[r];
+  float4 x4 = ld_global(P.src_pos + r);
   float x0 = x4.x, x1 = x4.y, x2 = x4.z;
   if (mp.src_xf) {
     float t0, t1, t2;
@@ -822,12 +834,23 @@ __device__ uint32_t capped_row_ball(const AssocProblem& P, const KConsts& kc, co
   const float l = kc.fixed_ell ? ell : range_ell(ell, x0, x1, x2);
   const float d2t = (float)(-2.0 * (double)l * (double)l * (double)kc.lg);
   const float qg = (float)(1.0 / (2.0 * (double)l * (double)l));
-  const float* xf = P.src_feat ? P.src_feat + (size_t)r * kc.F : nullptr;
-  const float* xs = P.src_sem ? P.src_sem + (size_t)r * kc.Ks : nullptr;
+  // with compile-time dimensions the row's and each candidate's features and
+  // class vectors are read into registers through global-address loads
+  float xfr[FC > 0 ? FC : 1], xsr[KC > 0 ? KC : 1];
+  if (FC > 0 && P.src_feat) {
+#pragma unroll
+    for (int f = 0; f < (FC > 0 ? FC : 1); ++f) xfr[f] = ld_global(P.src_feat + (size_t)r * F + f);
+  }
+  if (KC > 0 && P.src_sem) {
+#pragma unroll
+    for (int k = 0; k < (KC > 0 ? KC : 1); ++k) xsr[k] = ld_global(P.src_sem + (size_t)r * KS + k);
+  }
+  const float* xf = P.src_feat ? (FC > 0 ? xfr : P.src_feat + (size_t)r * kc.F) : nullptr;
+  const float* xs = P.src_sem ? (KC > 0 ? xsr : P.src_sem + (size_t)r * kc.Ks) : nullptr;
   float xg[2] = {0.0f, 0.0f};
   if (kc.use_gtype) {
-    xg[0] = P.src_geo[2 * (size_t)r];
-    xg[1] = P.src_geo[2 * (size_t)r + 1];
+    xg[0] = ld_global(P.src_geo + 2 * (size_t)r);
+    xg[1] = ld_global(P.src_geo + 2 * (size_t)r + 1);
   }
   float q0, q1, q2;
   affine3(mp.Rq, mp.Tq, x0, x1, x2, q0, q1, q2);
@@ -835,11 +858,20 @@ __device__ uint32_t capped_row_ball(const AssocProblem& P, const KConsts& kc, co
   const CellBox B = cell_box(GL, q0, q1, q2, rq);
   // a passing target's key is its index; ~0 for the others
   auto eval = [&](uint32_t pidx, float4& y4, float& y0, float& y1, float& y2, float& a) -> uint64_t {
-    y4 = GL.pos[pidx];
+    y4 = ld_global(GL.pos + pidx);
+    float yfr[FC > 0 ? FC : 1], ysr[KC > 0 ? KC : 1];
+    if (FC > 0) {
+#pragma unroll
+      for (int f = 0; f < (FC > 0 ? FC : 1); ++f) yfr[f] = ld_global(GL.feat + (size_t)pidx * F + f);
+    }
+    if (KC > 0) {
+#pragma unroll
+      for (int k = 0; k < (KC > 0 ? KC : 1); ++k) ysr[k] = ld_global(GL.sem + (size_t)pidx * KS + k);
+    }
     affine3(mp.A, mp.b, y4.x, y4.y, y4.z, y0, y1, y2);
     const bool pass = pair_kernel<FC, KC>(kc, ek, x0, x1, x2, y0, y1, y2, d2t, qg, l, xf,
-                                          FC != 0 ? GL.feat + (size_t)pidx * F : nullptr, xs,
-                                          KC != 0 ? GL.sem + (size_t)pidx * KS : nullptr, xg,
+                                          FC > 0 ? yfr : (FC != 0 ? GL.feat + (size_t)pidx * F : nullptr), xs,
+                                          KC > 0 ? ysr : (KC != 0 ? GL.sem + (size_t)pidx * KS : nullptr), xg,
                                           kc.use_gtype ? GL.geo + 2 * (size_t)pidx : nullptr, a);
     return pass ? (uint64_t)__float_as_uint(y4.w) : ~0ull;
   };
@@ -904,9 +936,9 @@ __device__ __forceinline__ void overflow_body(const AssocProblem& P, const KCons
       const int c = t / 128, i = t - c * 128;
       double v = 0.0;
       for (int b = b0 + i; b < b1; b += 128) {
-        if (c < 7) v += P.part_a[(size_t)b * 8 + c];
-        else if (c == 7) v += (double)P.part_cnt[(size_t)b * 2];
-        else if (c == 8) v = fmax(v, (double)P.part_cnt[(size_t)b * 2 + 1]);
+        if (c < 7) v += ld_global(P.part_a + (size_t)b * 8 + c);
+        else if (c == 7) v += (double)ld_global(P.part_cnt + (size_t)b * 2);
+        else if (c == 8) v = fmax(v, (double)ld_global(P.part_cnt + (size_t)b * 2 + 1));
       }
       s_tot[c][i] = v;
     }
@@ -925,7 +957,7 @@ __device__ __forceinline__ void overflow_body(const AssocProblem& P, const KCons
   Maps mp;
   load_maps(st, mp);
   const ExactK ek = exact_consts(kc);
-  const GridLevel& G = P.grid->lv[st->level];
+  const GridLevel G = P.grid->lv[st->level];  // a register copy, as in the association
   const int F = Dim<FC>::get(kc.F);
   const int KS = Dim<KC>::get(kc.Ks);
   const int chunk = (P.n_src + kOvfBlocks - 1) / kOvfBlocks;
@@ -942,7 +974,7 @@ __device__ __forceinline__ void overflow_body(const AssocProblem& P, const KCons
     uint32_t wnnz = 0, wmax = 0, seen = 0;
     for (int base = r0; base < r1; base += 64) {
       const int rr = base + lane;
-      uint64_t m = __ballot(rr < r1 && P.ovf_stamp[rr] == stamp);
+      uint64_t m = __ballot(rr < r1 && ld_global(P.ovf_stamp + rr) == stamp);
       while (m) {
         const int b = __ffsll((unsigned long long)m) - 1;
         m &= m - 1;
@@ -950,7 +982,7 @@ __device__ __forceinline__ void overflow_body(const AssocProblem& P, const KCons
           const int r = base + b;
           double racc[7] = {0, 0, 0, 0, 0, 0, 0};
           const uint32_t kept = capped_row_ball<FC, KC>(P, kc, ek, G, mp, r, ell, K, s_hist[wave], racc);
-          if (lane == 0) P.counts[r] = kept;
+          if (lane == 0) st_global(P.counts + r, kept);
 #pragma unroll
           for (int c = 0; c < 7; ++c) wacc[c] += racc[c];
           wnnz += kept;
@@ -979,8 +1011,8 @@ __device__ __forceinline__ void overflow_body(const AssocProblem& P, const KCons
     }
   } else
   for (int r = r0; r < r1; ++r) {
-    if (P.ovf_stamp[r] != stamp) continue;  // uniform
-    float4 x4 = P.src_pos[r];
+    if (ld_global(P.ovf_stamp + r) != stamp) continue;  // uniform
+    float4 x4 = ld_global(P.src_pos + r);
     float x0 = x4.x, x1 = x4.y, x2 = x4.z;
     if (mp.src_xf) {
       float t0, t1, t2;
@@ -994,8 +1026,8 @@ __device__ __forceinline__ void overflow_body(const AssocProblem& P, const KCons
     const float* xs = P.src_sem ? P.src_sem + (size_t)r * kc.Ks : nullptr;
     float xg[2] = {0.0f, 0.0f};
     if (kc.use_gtype) {
-      xg[0] = P.src_geo[2 * (size_t)r];
-      xg[1] = P.src_geo[2 * (size_t)r + 1];
+      xg[0] = ld_global(P.src_geo + 2 * (size_t)r);
+      xg[1] = ld_global(P.src_geo + 2 * (size_t)r + 1);
     }
     double acc[7] = {0, 0, 0, 0, 0, 0, 0};
     uint32_t stored = 0;
@@ -1004,7 +1036,7 @@ __device__ __forceinline__ void overflow_body(const AssocProblem& P, const KCons
       bool pass = false;
       float a = 0.0f, y0 = 0.0f, y1 = 0.0f, y2 = 0.0f;
       if (j < P.n_tgt) {
-        affine3(mp.A, mp.b, P.tgt_xyz[3 * (size_t)j], P.tgt_xyz[3 * (size_t)j + 1], P.tgt_xyz[3 * (size_t)j + 2],
+        affine3(mp.A, mp.b, ld_global(P.tgt_xyz + 3 * (size_t)j), ld_global(P.tgt_xyz + 3 * (size_t)j + 1), ld_global(P.tgt_xyz + 3 * (size_t)j + 2),
                 y0, y1, y2);
         pass = pair_kernel<FC, KC>(kc, ek, x0, x1, x2, y0, y1, y2, d2t, qg, l, xf,
                                    FC != 0 ? P.tgt_feat + (size_t)j * F : nullptr, xs,
@@ -1025,7 +1057,7 @@ __device__ __forceinline__ void overflow_body(const AssocProblem& P, const KCons
       const uint32_t slot = stored + before;
       if (pass && slot < K) {
         const float4 y4 =
-            make_float4(P.tgt_xyz[3 * (size_t)j], P.tgt_xyz[3 * (size_t)j + 1], P.tgt_xyz[3 * (size_t)j + 2], 0.0f);
+            make_float4(ld_global(P.tgt_xyz + 3 * (size_t)j), ld_global(P.tgt_xyz + 3 * (size_t)j + 1), ld_global(P.tgt_xyz + 3 * (size_t)j + 2), 0.0f);
         store_pair(P, mode, r, K, slot, G.inv_perm[j], y4, a);
         if (mode == MODE_ALIGN) {
           const float c0 = x1 * y2 - x2 * y1, c1 = x2 * y0 - x0 * y2, c2 = x0 * y1 - x1 * y0;
@@ -1042,7 +1074,7 @@ __device__ __forceinline__ void overflow_body(const AssocProblem& P, const KCons
       __syncthreads();
     }
     const uint32_t kept = min(stored, K);
-    if (tid == 0) P.counts[r] = kept;
+    if (tid == 0) st_global(P.counts + r, kept);
     blk_nnz += kept;
     blk_max = max(blk_max, kept);
     for (int c = 0; c < 7; ++c) {
@@ -1094,7 +1126,10 @@ template <int FC, int KC, bool ONE>
 __global__ __launch_bounds__(kBlock) void overflow_kernel(const AssocProblem* __restrict__ probs, KConsts kc,
                                                           AssocProblem one) {
   if constexpr (ONE) overflow_body<FC, KC>(one, kc);
-  else overflow_body<FC, KC>(probs[blockIdx.y], kc);
+  else {
+    const AssocProblem P = probs[blockIdx.y];  // a register copy: stores cannot alias it
+    overflow_body<FC, KC>(P, kc);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1566,7 +1601,7 @@ __device__ __forceinline__ void step_pass(const AssocProblem& P, const KConsts& 
   double tot[6];
   scaled_flow(st, kc, tot);
   const float ell = st->ell;
-  const GridLevel& G = P.grid->lv[st->level];
+  const GridLevel G = P.grid->lv[st->level];  // a register copy, as in the association
   Maps mp;
   load_maps(st, mp);
   // normalised flow (compute_flow, CvoGPU.cu:2099-2104), used in float; the
@@ -1580,7 +1615,7 @@ __device__ __forceinline__ void step_pass(const AssocProblem& P, const KConsts& 
   const float v0 = (float)(tot[3] / nrm), v1 = (float)(tot[4] / nrm), v2 = (float)(tot[5] / nrm);
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
   if (r < P.n_src) {
-    float4 x4 = P.src_pos[r];
+    float4 x4 = ld_global(P.src_pos + r);
     float x0 = x4.x, x1 = x4.y, x2 = x4.z;
     if (mp.src_xf) {
       float t0, t1, t2;
@@ -1668,7 +1703,10 @@ template <bool ONE>
 __global__ __launch_bounds__(kBlock) void step_kernel(const AssocProblem* __restrict__ probs, KConsts kc,
                                                       AssocProblem one) {
   if constexpr (ONE) step_pass(one, kc);
-  else step_pass(probs[blockIdx.y], kc);
+  else {
+    const AssocProblem P = probs[blockIdx.y];  // a register copy
+    step_pass(P, kc);
+  }
 }
 
 // Initial state of a problem: pose from a row-major float 4x4, maps, level.
