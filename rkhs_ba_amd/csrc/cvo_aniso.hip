@@ -101,7 +101,7 @@ __device__ __forceinline__ bool aniso_pair(const AnisoArgs& g, const float x[3],
   if (g.use_col) {
     float d2c = 0.0f;
     for (int f = 0; f < g.F; ++f) {
-      const float t = xf[f] - yf[f];
+      const float t = ld_global(xf + f) - ld_global(yf + f);  // both clouds' features in global memory
       d2c = fmaf(t, t, d2c);
     }
     if (!(d2c < g.d2_c_thres)) return false;
@@ -110,7 +110,7 @@ __device__ __forceinline__ bool aniso_pair(const AnisoArgs& g, const float x[3],
   if (g.use_sem) {
     float d2s = 0.0f;
     for (int c = 0; c < g.Ks; ++c) {
-      const float t = xs[c] - ys[c];
+      const float t = ld_global(xs + c) - ld_global(ys + c);
       d2s = fmaf(t, t, d2s);
     }
     if (!(d2s < g.d2_s_thres)) return false;
@@ -232,8 +232,8 @@ __device__ inline void for_each_in_ellipsoid(const GridLevel& GL, const CellBox&
       const int hi = min(cell_floor((E.q0 + xmax + E.w - GL.ox) * GL.inv_h, GL.nx), B.lo0 + B.nx_run - 1);
       if (lo <= hi) {
         const uint32_t c0 = ((uint32_t)cz * (uint32_t)GL.ny + (uint32_t)cy) * (uint32_t)GL.nx + (uint32_t)lo;
-        rs = GL.cell_start[c0];
-        rl = GL.cell_start[c0 + (uint32_t)(hi - lo + 1)] - rs;
+        rs = ld_global(GL.cell_start + c0);
+        rl = ld_global(GL.cell_start + c0 + (uint32_t)(hi - lo + 1)) - rs;
       }
     }
     uint32_t incl = rl;
@@ -266,7 +266,7 @@ __device__ inline void for_each_in_ellipsoid(const GridLevel& GL, const CellBox&
 // those instead.
 __device__ inline void aniso_box_row(const AnisoArgs& g, int r, int lane, const float x[3], const float M[9],
                                      const RowEllipsoid& E, uint32_t* hist) {
-  const GridLevel& GL = g.grid->lv[0];
+  const GridLevel GL = g.grid->lv[0];  // a register copy: the pair stores cannot alias it
   const float* xf = g.sf + (size_t)r * g.F;
   const float* xs = g.ss + (size_t)r * g.Ks;
   const uint32_t K = (uint32_t)g.K;
@@ -274,7 +274,7 @@ __device__ inline void aniso_box_row(const AnisoArgs& g, int r, int lane, const 
   auto each = [&](auto&& fn) { for_each_in_ellipsoid(GL, B, E, lane, fn); };
   // a passing target's key is its index; ~0 for the others
   auto eval = [&](uint32_t pidx, float& a) -> uint64_t {
-    const float4 y4 = GL.pos[pidx];
+    const float4 y4 = ld_global(GL.pos + pidx);
     const bool pass = aniso_pair(g, x, M, xf, xs, y4.x, y4.y, y4.z, GL.feat ? GL.feat + (size_t)pidx * g.F : nullptr,
                                  GL.sem ? GL.sem + (size_t)pidx * g.Ks : nullptr, a);
     return pass ? (uint64_t)__float_as_uint(y4.w) : ~0ull;
