@@ -11,7 +11,15 @@ pose is the label.  A pair whose converged pose lies farther than 0.05 rad /
 0.1 m from the generating transform (CVO settled in another basin) is dropped.
 --label generator uses the generating transform itself.
 
+Labelling takes ~17 ms per pair, so a large set can be labelled in parts:
+--label-only FILE labels the --pairs pairs of --seed and writes (first seed,
+count, kept indices, labels) to FILE without training; --labels FILE ...
+regenerates those files' pairs from their seeds and adds their kept pairs to
+the training set.
+
     python -m rkhs_ba_amd.train_proposer --pairs 50000 --out rkhs_ba_amd/weights/proposer_k8.npy
+    python -m rkhs_ba_amd.train_proposer --seed 1 --pairs 42000 --label-only labels_1.npz
+    python -m rkhs_ba_amd.train_proposer --seed 2 --pairs 42000 --labels labels_1.npz
 """
 from __future__ import annotations
 
@@ -109,12 +117,17 @@ def main():
     ap.add_argument("--label", choices=("cvo", "generator"), default="cvo")
     ap.add_argument("--label-starts", type=int, default=24, help="multi-start CVO starts per training pair")
     ap.add_argument("--label-iters", type=int, default=300, help="CVO iterations per start")
+    ap.add_argument("--label-only", default=None, help="label the pairs, write them to this .npz and stop")
+    ap.add_argument("--labels", nargs="*", default=[], help=".npz files of --label-only runs to train on as well")
     args = ap.parse_args()
+    if (args.label_only or args.labels) and args.label != "cvo":
+        ap.error("--label-only and --labels need --label cvo")
     from rkhs_ba_amd import CvoGPU
     from rkhs_ba_amd.proposer import PoseProposer
     t0 = time.time()
     print(f"[train] generating {args.pairs} + {args.val} pairs on {args.workers} workers", flush=True)
-    a, b, t = dataset(1_000_000 * (args.seed + 1), args.pairs, args.npts, args.workers)
+    first_seed = 1_000_000 * (args.seed + 1)
+    a, b, t = dataset(first_seed, args.pairs, args.npts, args.workers)
     va, vb, vt = dataset(7_000_000 + args.seed, args.val, args.npts, args.workers)
     print(f"[train] data ready in {time.time() - t0:.1f} s", flush=True)
     label_info = {"label": args.label}
@@ -123,8 +136,29 @@ def main():
         label_info.update(starts=args.label_starts, iterations=args.label_iters, kept=int(keep.size),
                           of=int(a.shape[0]), seconds=secs)
         print(f"[train] CVO labels: kept {keep.size} of {a.shape[0]} pairs in {secs:.1f} s", flush=True)
+        if args.label_only:
+            np.savez(args.label_only, first_seed=np.int64(first_seed), n=np.int64(a.shape[0]), keep=keep,
+                     labels=lab, starts=np.int64(args.label_starts), iterations=np.int64(args.label_iters),
+                     seconds=np.float64(secs))
+            print(f"[train] wrote {args.label_only}", flush=True)
+            return
         a, b, t = a[keep], b[keep], lab
-        args.pairs = int(keep.size)
+        parts = [dict(first_seed=first_seed, kept=int(keep.size), of=int(args.pairs))]
+        for path in args.labels:
+            z = np.load(path, allow_pickle=False)
+            if int(z["starts"]) != args.label_starts or int(z["iterations"]) != args.label_iters:
+                raise SystemExit(f"{path}: labelled with other CVO settings")
+            za, zb, _ = dataset(int(z["first_seed"]), int(z["n"]), args.npts, args.workers)
+            k = z["keep"]
+            a, b, t = np.concatenate([a, za[k]]), np.concatenate([b, zb[k]]), np.concatenate([t, z["labels"]])
+            parts.append(dict(first_seed=int(z["first_seed"]), kept=int(k.size), of=int(z["n"])))
+            label_info["kept"] += int(k.size)
+            label_info["of"] += int(z["n"])
+            label_info["seconds"] += float(z["seconds"])
+            del za, zb
+            print(f"[train] {path}: {k.size} of {int(z['n'])} pairs", flush=True)
+        label_info["parts"] = parts
+        args.pairs = int(a.shape[0])
     cvo = CvoGPU()
     prop = PoseProposer(cvo, seed=args.seed)
     rng = np.random.default_rng(args.seed)
