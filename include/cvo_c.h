@@ -533,6 +533,18 @@ cvo_status cvo_reset_kernel_stats(cvo_ctx* ctx);
 /* Library version string. */
 const char* cvo_version(void);
 
+/* No counterpart in the reference: runs the device's controller math on the
+ * given inputs so that tests can hold it against published values (numpy
+ * roots, scipy expm).  Cubic i: roots of coef[4i] s^3 + coef[4i+1] s^2 +
+ * coef[4i+2] s + coef[4i+3] into re[3i..3i+2] / im[3i..3i+2] (the closed form
+ * the step selection uses, CvoGPU.cu:495-536).  Twist i: T64[12i..] = the FP64
+ * SE(3) exponential of dt[i] * xi[6i..6i+5] ([omega; v], row-major 3x4),
+ * log_norm[i] = the norm of its logarithm (dist_se3, CvoGPU.cpp:28-33), T32 =
+ * the reference's float Exp_SEK3 (LieGroup.cpp:244-275) of the same twist.
+ * Uses the current device, allocates and synchronises. */
+cvo_status cvo_device_math_check(int32_t n_cubic, const double* coef, double* re, double* im, int32_t n_exp,
+                                 const double* xi, const double* dt, double* T64, double* log_norm, float* T32);
+
 #ifdef __cplusplus
 }
 #endif

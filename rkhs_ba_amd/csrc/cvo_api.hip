@@ -33,6 +33,8 @@ void launch_iota(uint32_t* v, int n, hipStream_t s);
 // zero fill of bytes (a multiple of 4) by a kernel: no memset node in a
 // caller's graph capture
 void launch_zero(void* p, size_t bytes, hipStream_t s);
+void launch_math_check(int n_cubic, const double* coef, double* re, double* im, int n_exp, const double* xi,
+                       const double* dt, double* T64, double* log_norm, float* T32, hipStream_t s);
 void launch_gather(const uint32_t* order, int n, const float* xyz, const float* feat, int F, const float* sem, int K,
                    const float* geo, float4* pos, float* feat_s, float* sem_s, float* geo_s, uint32_t* inv_perm,
                    hipStream_t s);
@@ -632,6 +634,42 @@ using namespace cvo;
 // ---------------------------------------------------------------------------
 
 extern "C" const char* cvo_version(void) { return "rkhs_ba_amd 0.1 (gfx950)"; }
+
+extern "C" cvo_status cvo_device_math_check(int32_t n_cubic, const double* coef, double* re, double* im,
+                                            int32_t n_exp, const double* xi, const double* dt, double* T64,
+                                            double* log_norm, float* T32) {
+  if (n_cubic < 0 || n_exp < 0 || (n_cubic > 0 && (!coef || !re || !im)) ||
+      (n_exp > 0 && (!xi || !dt || !T64 || !log_norm || !T32)))
+    return CVO_ERR_INVALID_ARG;
+  if (n_cubic == 0 && n_exp == 0) return CVO_OK;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return CVO_ERR_NO_DEVICE;
+  const size_t nc = (size_t)n_cubic, ne = (size_t)n_exp;
+  // one buffer: coef 4, re 3, im 3 per cubic; xi 6, dt 1, T64 12, log 1 (doubles), T32 12 (floats) per twist
+  const size_t nd = 10 * nc + 20 * ne;
+  const size_t bytes = nd * sizeof(double) + 12 * ne * sizeof(float);
+  void* buf = nullptr;
+  if (hipMalloc(&buf, bytes) != hipSuccess) return CVO_ERR_OOM;
+  double* d = static_cast<double*>(buf);
+  double *d_coef = d, *d_re = d + 4 * nc, *d_im = d + 7 * nc, *d_xi = d + 10 * nc, *d_dt = d_xi + 6 * ne;
+  double *d_T64 = d_dt + ne, *d_log = d_T64 + 12 * ne;
+  float* d_T32 = reinterpret_cast<float*>(d + nd);
+  bool ok = true;
+  ok = ok && (nc == 0 || hipMemcpy(d_coef, coef, 4 * nc * sizeof(double), hipMemcpyHostToDevice) == hipSuccess);
+  ok = ok && (ne == 0 || hipMemcpy(d_xi, xi, 6 * ne * sizeof(double), hipMemcpyHostToDevice) == hipSuccess);
+  ok = ok && (ne == 0 || hipMemcpy(d_dt, dt, ne * sizeof(double), hipMemcpyHostToDevice) == hipSuccess);
+  if (ok) {
+    cvo::launch_math_check(n_cubic, d_coef, d_re, d_im, n_exp, d_xi, d_dt, d_T64, d_log, d_T32, nullptr);
+    ok = hipGetLastError() == hipSuccess && hipDeviceSynchronize() == hipSuccess;
+  }
+  ok = ok && (nc == 0 || (hipMemcpy(re, d_re, 3 * nc * sizeof(double), hipMemcpyDeviceToHost) == hipSuccess &&
+                          hipMemcpy(im, d_im, 3 * nc * sizeof(double), hipMemcpyDeviceToHost) == hipSuccess));
+  ok = ok && (ne == 0 || (hipMemcpy(T64, d_T64, 12 * ne * sizeof(double), hipMemcpyDeviceToHost) == hipSuccess &&
+                          hipMemcpy(log_norm, d_log, ne * sizeof(double), hipMemcpyDeviceToHost) == hipSuccess &&
+                          hipMemcpy(T32, d_T32, 12 * ne * sizeof(float), hipMemcpyDeviceToHost) == hipSuccess));
+  (void)hipFree(buf);
+  return ok ? CVO_OK : CVO_ERR_HIP;
+}
 
 extern "C" const char* cvo_status_string(cvo_status s) {
   switch (s) {

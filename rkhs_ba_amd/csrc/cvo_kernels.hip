@@ -1938,6 +1938,45 @@ void launch_grid_count(const float* xyz, int n, const GridLevel& L, uint32_t* ke
                      L.ny, L.nz, key, count);
 }
 
+// The controller's device math on given inputs, for checks against
+// published values: the cubic (roots of coef[4i..4i+3], highest power first;
+// re / im 3 each), the FP64 SE(3) exponential of dt[i] * xi[6i..] (row-major
+// 3x4) with the log norm of its result, and the reference's float exponential
+// of the same twist.
+__global__ void math_check_kernel(int n_cubic, const double* __restrict__ coef, double* __restrict__ re,
+                                  double* __restrict__ im, int n_exp, const double* __restrict__ xi,
+                                  const double* __restrict__ dt, double* __restrict__ T64,
+                                  double* __restrict__ log_norm, float* __restrict__ T32) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n_cubic) {
+    double r[3], m[3];
+    cubic_solve(coef[4 * i], coef[4 * i + 1], coef[4 * i + 2], coef[4 * i + 3], r, m);
+    for (int k = 0; k < 3; ++k) {
+      re[3 * i + k] = r[k];
+      im[3 * i + k] = m[k];
+    }
+  }
+  if (i < n_exp) {
+    double R[9], t[3];
+    se3_exp(xi + 6 * i, dt[i], R, t);
+    for (int r = 0; r < 3; ++r) {
+      for (int c = 0; c < 3; ++c) T64[12 * i + 4 * r + c] = R[3 * r + c];
+      T64[12 * i + 4 * r + 3] = t[r];
+    }
+    log_norm[i] = se3_log_norm(R, t);
+    float xf[6];
+    for (int k = 0; k < 6; ++k) xf[k] = (float)xi[6 * i + k];
+    exp_sek3_f32(xf, (float)dt[i], T32 + 12 * i);
+  }
+}
+
+void launch_math_check(int n_cubic, const double* coef, double* re, double* im, int n_exp, const double* xi,
+                       const double* dt, double* T64, double* log_norm, float* T32, hipStream_t s) {
+  const int n = n_cubic > n_exp ? n_cubic : n_exp;
+  hipLaunchKernelGGL(math_check_kernel, dim3((n + 63) / 64), dim3(64), 0, s, n_cubic, coef, re, im, n_exp, xi, dt,
+                     T64, log_norm, T32);
+}
+
 void launch_zero(void* p, size_t bytes, hipStream_t s) {
   const size_t n = bytes / 4;
   if (n == 0) return;
