@@ -44,6 +44,9 @@ HBM_PEAK_GBS = 8000.0
 # FETCH_SIZE / WRITE_SIZE passes scaled by a 256 MiB copy probe
 # (profiles/assoc_street60k_pmc_traffic.md)
 ASSOC_TRAFFIC_BYTES = 25.79e6
+# HBM bytes per BA pair-pass launch on the 12-frame window, the same way
+# (profiles/ba_window12_pmc_traffic.md)
+BA_PAIR_TRAFFIC_BYTES = 912.5e6
 # per-launch work of the association kernel on the 60k street registration
 # with its autotuned launch shape, from rocprofv3 counters (SQ_INSTS_VALU_TRANS_F32
 # x 64 lanes; SQ_INSTS_VALU_FLOPS_FP32 x 64): the transcendental and FP32
@@ -320,7 +323,8 @@ def leg_ba(args):
     out["roofline"] = {"bound": "hbm", "achieved": pair_bytes / (pair_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
                        "unit": "GB/s", "frac": pair_bytes / (pair_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
                        "kernel": "ba_pair_kernel", "avg_launch_ms": pair_ms,
-                       "algorithmic_bytes_per_launch": pair_bytes, "launches": n_pair, "traffic": None}
+                       "algorithmic_bytes_per_launch": pair_bytes, "launches": n_pair,
+                       "traffic": BA_PAIR_TRAFFIC_BYTES}
     out["assoc_roofline"] = {"achieved": assoc_bytes / (assoc_ms * 1e-3) / 1e9, "avg_launch_ms": assoc_ms,
                              "algorithmic_bytes_per_launch": assoc_bytes}
     out["kernels_ms_per_iteration"] = {
