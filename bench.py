@@ -43,7 +43,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per association launch on the 60k street workload, from rocprofv3
 # FETCH_SIZE / WRITE_SIZE passes scaled by a 256 MiB copy probe
 # (profiles/assoc_street60k_pmc_traffic.md)
-ASSOC_TRAFFIC_BYTES = 25.79e6
+ASSOC_TRAFFIC_BYTES = 25.78e6
 # HBM bytes per BA pair-pass launch on the 12-frame window, the same way
 # (profiles/ba_window12_pmc_traffic.md)
 BA_PAIR_TRAFFIC_BYTES = 912.5e6
