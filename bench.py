@@ -456,6 +456,15 @@ def leg_multistart(args):
                                "sample": "oracle iteration at ell_init timed 3 times x the GPU run's total "
                                          "iterations over all 96 starts",
                                "seconds_estimate": est}
+        # the winning start registered by the FP64 oracle from the same start
+        best = int(np.argmax(r["per_start_cos"]))
+        log(f"[bench] multi-start: FP64 oracle from start {best}")
+        t1 = time.perf_counter()
+        To, it_o, _, _ = pyoracle.align(params, src, tgt, Ts[best], precision=pyoracle.FP64)
+        dr, dt_ = rot_trans_diff(r["per_start_T"][best], To)
+        out["gpu_vs_fp64_oracle_best_start"] = {"rad": dr, "m": dt_, "oracle_iterations": int(it_o),
+                                                "gpu_iterations": int(r["per_start_iters"][best]),
+                                                "oracle_seconds": time.perf_counter() - t1}
     cvo.close()
     return out
 
