@@ -4,9 +4,9 @@ Pairs: 20k/20k planes-and-spheres scenes moved by a uniform random rotation
 and 2 m (seeds 7100-7149, never used in training or in the other tests).
 Each pair is registered by the plain 96-start grid and by the learned path
 (8 proposals refined by batched CVO, judged by the cos angle, grid fallback
-below the confidence floor).  Success is the north star's: the final pose
-within 0.01 rad and 1 cm of the FP64 oracle's converged pose, the oracle
-started from the generating transform (its basin) with the same parameters.
+below the confidence floor).  Success: the final pose within 0.01 rad and
+1 cm of the FP64 oracle's converged pose, the oracle started from the
+generating transform (its basin) with the same parameters.
 Bars: the learned path succeeds at least as often as the grid, at >= 5x its
 throughput.
 
