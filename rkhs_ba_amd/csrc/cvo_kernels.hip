@@ -366,14 +366,45 @@ __device__ __forceinline__ void assoc_row(const AssocProblem& P, const KConsts& 
   }
   uint32_t stored = 0;
   bool stop = false;
+  // batched launches (bundle adjustment, multi-start): a run's cells beyond
+  // the query radius hold no passing target, so the run is clipped to the
+  // ball's chord over its (y, z) cell square, or skipped; the slack covers
+  // the float rounding of the points' cell assignment, and rq itself exceeds
+  // the distance gate's radius, so no passing target is lost and the passing
+  // targets keep their enumeration order.  (Single-problem launches do not
+  // clip: their fine cells leave little to cut, and the extra arithmetic on
+  // the row's dependent chain costs more.)
+  constexpr bool CLIP = !NOBA;
+  const float rq2 = rq * rq, slack = 1e-3f * GL.h;
+  (void)rq2;
+  (void)slack;
   for (int rbase = 0; rbase < nruns && !stop; rbase += G) {
     const int myrun = rbase + g;
     uint32_t rs = 0, rl = 0;
     if (myrun < nruns) {
       const int iz = myrun / span1, iy = myrun - iz * span1;
-      const uint32_t c0 = ((uint32_t)(lo2 + iz) * (uint32_t)GL.ny + (uint32_t)(lo1 + iy)) * (uint32_t)GL.nx + (uint32_t)lo0;
-      rs = ld_global(GL.cell_start + c0);
-      rl = ld_global(GL.cell_start + c0 + (uint32_t)(hi0 - lo0) + 1u) - rs;
+      if constexpr (CLIP) {
+        const int cy = lo1 + iy, cz = lo2 + iz;
+        const float ys = GL.oy + (float)cy * GL.h, zs = GL.oz + (float)cz * GL.h;
+        const float dy = fmaxf(0.0f, fmaxf(ys - q1, q1 - (ys + GL.h)) - slack);
+        const float dz = fmaxf(0.0f, fmaxf(zs - q2, q2 - (zs + GL.h)) - slack);
+        const float rem = rq2 - (dy * dy + dz * dz);
+        if (rem >= 0.0f) {  // else the run lies beyond the radius
+          const float ch = sqrtf(rem) + slack;
+          const int lo = max(lo0, cell_floor((q0 - ch - GL.ox) * GL.inv_h, GL.nx));
+          const int hi = min(hi0, cell_floor((q0 + ch - GL.ox) * GL.inv_h, GL.nx));
+          if (lo <= hi) {
+            const uint32_t c0 = ((uint32_t)cz * (uint32_t)GL.ny + (uint32_t)cy) * (uint32_t)GL.nx + (uint32_t)lo;
+            rs = ld_global(GL.cell_start + c0);
+            rl = ld_global(GL.cell_start + c0 + (uint32_t)(hi - lo) + 1u) - rs;
+          }
+        }
+      } else {
+        const uint32_t c0 =
+            ((uint32_t)(lo2 + iz) * (uint32_t)GL.ny + (uint32_t)(lo1 + iy)) * (uint32_t)GL.nx + (uint32_t)lo0;
+        rs = ld_global(GL.cell_start + c0);
+        rl = ld_global(GL.cell_start + c0 + (uint32_t)(hi0 - lo0) + 1u) - rs;
+      }
     }
     uint32_t incl = rl;
 #pragma unroll
