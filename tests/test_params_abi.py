@@ -116,6 +116,20 @@ def test_status_strings_and_version():
     assert b"gfx950" in L.cvo_version()
 
 
+def test_device_math_check_validates_before_the_device():
+    """Bad sizes and missing buffers fail with CVO_ERR_INVALID_ARG, and an empty
+    request succeeds, without a device call (so on a host without a GPU)."""
+    from rkhs_ba_amd import lib
+    L = lib()
+    L.cvo_device_math_check.argtypes = [C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32] + [C.c_void_p] * 5
+    none5 = [None] * 5
+    assert L.cvo_device_math_check(-1, None, None, None, 0, *none5) == 1
+    assert L.cvo_device_math_check(0, None, None, None, -3, *none5) == 1
+    assert L.cvo_device_math_check(3, None, None, None, 0, *none5) == 1
+    assert L.cvo_device_math_check(0, None, None, None, 2, *none5) == 1
+    assert L.cvo_device_math_check(0, None, None, None, 0, *none5) == 0
+
+
 def test_abi_struct_sizes():
     """The ctypes mirror has the C layout (checked against field offsets)."""
     from rkhs_ba_amd._abi import CvoParams, CloudView, Assoc, IterRecord
